@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark: FSI steps/s of the FP64 fluid step at 1024^2 on one MI355X.
+
+A step is one pass of the reference's time loop body (time_step.f90:178-341)
+over the 1024^2 workload: create_mask -> EvolveFluidAB2 (cal_nlk, AB2 update,
+pressure, cal_drag, cal_energy, time_avg_vort) -> GetForces (stresses, Delta
+interpolation, surface integral) -> leading-edge kinematics -> swap, with all
+fields resident in HBM.  The beam is driven by prescribed heave
+0.1 sin(2 pi t) (the Newton solve of the flexible beam stays on the host and
+is not part of this path).  Inputs: band-limited Gaussian vorticity noise
+(seed 0), synthetic.
+
+Prints one JSON line.  roofline: the kernel with the largest total time,
+its algorithmic bytes per launch (compulsory reads + writes of its operands)
+divided by its mean duration measured with HIP events on the simulation's
+stream.  cpu_baseline: the CPU oracle (C/OpenMP restatement of the same step)
+on a bounded sample of the same workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["create_mask", "startup_velocity", "cols_vortk_n0", "rows_nonlinear", "step_scalars",
+           "cols_ab2_update", "cols_vortk_n1", "rows_velocity_pressure", "reductions",
+           "cols_pressure", "rows_pressure_box", "delta_interp", "beam_forces"]
+
+
+def alg_bytes(name: str, nx: int, ny: int, visc: bool, box_rows: int, box_cols: int) -> float:
+    """Compulsory HBM bytes of one launch: every operand read once and every
+    result written once (S = one nx*ny FP64 field = 8 nx ny bytes)."""
+    S = 8.0 * nx * ny
+    rowS = 8.0 * nx
+    return {
+        "cols_vortk_n0": 3 * S,                     # W0 -> A, B
+        "rows_nonlinear": 7 * S,                    # A, B, u1, u2, chi_s -> Rc, Rpx
+        "cols_ab2_update": 6 * S,                   # Rc, Rpx, W0, nlk(n1) -> nlk(n0), W1
+        "cols_vortk_n1": (5 if visc else 4) * S,    # W1 -> omega, psi, i ky psi (, -ky^2 psi)
+        "rows_velocity_pressure": 9 * S,            # 3 inputs + vor_mean r/w -> u1, u2, Rq, Rqy
+        "cols_pressure": 2 * S + box_rows * rowS,   # Rq, Rqy -> q on the beam rows
+        "rows_pressure_box": box_rows * rowS * (6 if visc else 3) + 3 * 8.0 * box_rows * box_cols,
+    }.get(name, 0.0)
+
+
+def hip_sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+            return
+    except Exception:
+        pass
+    try:
+        C.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def read_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/pmc_<kernel>_<n>.txt), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{n}.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(p, seconds: float, threads: int):
+    from oracle import oracle
+
+    oracle.set_threads(threads)
+    o = oracle.Oracle(oracle.make_params(**{k: getattr(p, k) for k in (
+        "nx", "ny", "ns", "nu", "dt_fixed", "iMotion", "heave_amp", "heave_freq")}))
+    vk = o.init_noise(seed=0, amp=1e-3, kmax2=64)
+    o.set_vortk(vk)
+    o.run(2)  # startup + one AB2 step (warm-up, not timed)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.run(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 50:
+            break
+    o.close()
+    return steps / el, steps, el
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1024, help="grid size (1024 or 2048)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)
+        return 2
+
+    from fsibe_2d_amd import fsi_large
+    from fsibe_2d_amd._lib import check, lib
+    from fsibe_2d_amd.inputs import band_limited_noise
+    from fsibe_2d_amd.solver import FluidSim
+    from fsibe_2d_amd.spectral import Spectral
+
+    n = args.n
+    p = fsi_large(n)
+    sp = Spectral(n, n, p.xl, p.yl, p.nu)
+    vk = band_limited_noise(sp, sp.dealiase_mask(), seed=0, amp=1e-3, kmax2=64)
+    sp.close()
+    sim = FluidSim(p)
+    sim.set_vortk(vk)
+
+    sim.run(args.warmup)
+    hip_sync()
+    t0 = time.perf_counter()
+    out = sim.run(args.steps)
+    hip_sync()
+    el = time.perf_counter() - t0
+    steps_per_s = args.steps / el
+
+    roofline = None
+    prof = {}
+    if not args.no_profile:
+        L = lib()
+        check(L.fsibe_profile_enable(sim.h, 1))
+        hip_sync()
+        t1 = time.perf_counter()
+        sim.run(args.steps)
+        hip_sync()
+        el_prof = time.perf_counter() - t1
+        nk = len(KERNELS)
+        ms = (C.c_double * nk)()
+        cnt = (C.c_longlong * nk)()
+        L.fsibe_profile_read(sim.h, ms, cnt, nk)
+        check(L.fsibe_profile_enable(sim.h, 0))
+        fb = sim.box("press")[1]
+        box_rows, box_cols = fb[3] - fb[2] + 1, fb[1] - fb[0] + 1
+        for i, name in enumerate(KERNELS):
+            if cnt[i]:
+                avg_us = 1e3 * ms[i] / cnt[i]
+                b = alg_bytes(name, n, n, True, box_rows, box_cols)
+                prof[name] = {"avg_us": round(avg_us, 3), "launches": int(cnt[i]),
+                              "alg_bytes": b, "GBps": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
+        dom = max(prof, key=lambda k: prof[k]["avg_us"] * prof[k]["launches"])
+        d = prof[dom]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4) if d["GBps"] else None,
+                    "traffic": read_traffic(dom, n), "avg_us": d["avg_us"],
+                    "alg_bytes_per_launch": d["alg_bytes"]}
+        prof["_profiled_ms_per_step"] = round(1e3 * el_prof / args.steps, 4)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+        rate, k, sec = cpu_baseline(p, args.cpu_seconds, threads)
+        cpu = {"value": round(rate, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": f"{k} AB2 steps of the same {n}^2 FSI workload after startup + 1 step, "
+                         f"{sec:.1f} s, C/OpenMP oracle with {threads} threads"}
+
+    step_bytes = 848.0 * n * n  # reference's per-step algorithmic bytes (19 transforms + 30 passes)
+    rec = {
+        "metric": f"FSI steps/s at {n}^2 FP64",
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"fsi_{n}: {n}^2 periodic grid, domain 4x4, Re=1000, beam 128 nodes "
+                               f"heaving 0.1 sin(2 pi t), dt=5e-5, iSponge=3, iViscous=1, forces every step",
+                   "grid": n, "ns": 128, "dt": 5e-5},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "step_alg_GBps": round(step_bytes * steps_per_s / 1e9, 1),
+        "step_alg_frac": round(step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
+        "last_step": {"dt1": out["dt1"], "Force": out["Force"], "Force_press": out["Force_press"]},
+        "kernels": prof,
+    }
+    print(json.dumps(rec))
+    sim.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,1518 @@
+// fsibe_hip.hip -- host side of libfsibe_hip.so (C ABI in include/fsibe_hip.h)
+//
+// One HIP stream per handle.  Tier A (fsibe_spectral) is the reference's FFT
+// and operator plugin on the packed layout; Tier B (fsibe_sim) keeps the
+// fluid state in HBM as half spectra and runs the fused step pipeline of
+// kernels_step.hpp.  The host computes, with the same libm calls the
+// reference makes, everything that depends only on the beam (mask primitive
+// geometry, interpolation targets, bounding boxes), uploads it in one copy per
+// step, and reads back one small result record per step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fsibe_hip.h"
+#include "kernels.hpp"
+#include "kernels_mask.hpp"
+#include "kernels_step.hpp"
+
+using namespace fsibe;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int set_err(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+extern "C" const char *fsibe_last_error(void) { return g_err.c_str(); }
+
+#define HIPCHK(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return set_err(FSIBE_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),    \
+                     __FILE__, __LINE__);                                                   \
+  } while (0)
+
+#define CHK(call)                  \
+  do {                             \
+    int r_ = (call);               \
+    if (r_ != FSIBE_OK) return r_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// planning
+// ---------------------------------------------------------------------------
+// radices for length L (8 first, then 4s, a 2, 3s); 0 if unsupported
+static bool make_plan(int L, FftPlan &p) {
+  p = FftPlan{};
+  p.L = L;
+  if (L < 1) return false;
+  int m = L, n3 = 0, e2 = 0;
+  while (m % 3 == 0) { m /= 3; ++n3; }
+  while (m % 2 == 0) { m /= 2; ++e2; }
+  if (m != 1) return false;
+  std::vector<int> r;
+  for (int i = 0; i < n3; ++i) r.push_back(3);
+  int a = e2 / 3, b = e2 % 3;
+  if (b == 1 && a >= 1) { a -= 1; r.push_back(4); r.push_back(4); }
+  else if (b == 1) r.push_back(2);
+  else if (b == 2) r.push_back(4);
+  for (int i = 0; i < a; ++i) r.push_back(8);
+  if ((int)r.size() > 16) return false;
+  p.np = (int)r.size();
+  for (int i = 0; i < p.np; ++i) p.r[i] = r[i];
+  return true;
+}
+
+static std::vector<double2> twiddles(int n) {
+  std::vector<double2> t(n);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int k = 0; k < n; ++k) {
+    long double a = twopi * (long double)k / (long double)n;
+    t[k] = make_double2((double)cosl(a), (double)(-sinl(a)));
+  }
+  return t;
+}
+
+static int emax_of(long long elems) {
+  long long e = (elems + kNT - 1) / kNT;
+  if (e <= 4) return 4;
+  if (e <= 8) return 8;
+  if (e <= 16) return 16;
+  if (e <= 32) return 32;
+  return -1;
+}
+
+static int lds_limit() {
+  static int lim = -1;
+  if (lim < 0) {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 65536;
+    lim = v;
+  }
+  return lim;
+}
+
+template <typename K>
+static int prep_lds(K kernel, size_t lds) {
+  if (lds > (size_t)lds_limit())
+    return set_err(FSIBE_ESIZE, "tile needs %zu bytes of LDS (limit %d)", lds, lds_limit());
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return set_err(FSIBE_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+  }
+  return FSIBE_OK;
+}
+
+#define LAUNCH_E(E, KT, grid, lds, st, ...)                                                    \
+  do {                                                                                        \
+    const int e__ = (E);                                                                      \
+    if (e__ == 4) { CHK(prep_lds(KT<4>, lds)); KT<4><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
+    else if (e__ == 8) { CHK(prep_lds(KT<8>, lds)); KT<8><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
+    else if (e__ == 16) { CHK(prep_lds(KT<16>, lds)); KT<16><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
+    else if (e__ == 32) { CHK(prep_lds(KT<32>, lds)); KT<32><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
+    else return set_err(FSIBE_ESIZE, "transform tile too large for one workgroup");         \
+    HIPCHK(hipGetLastError());                                                                \
+  } while (0)
+
+static int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
+
+// columns per tile for a column kernel holding `nfld` fields of length ny
+static int pick_tx(int H, int ny, int nfld, int split) {
+  int forced = env_int("FSIBE_TX", 0);
+  if (forced > 0 && H % forced == 0 && (long long)forced * nfld * ny <= 32LL * kNT) return forced;
+  int best = 1;
+  for (int tx = 1; tx <= 16; tx *= 2) {
+    if (H % tx) break;
+    if ((long long)tx * nfld * ny > 32LL * kNT) break;
+    if ((size_t)tx * nfld * ny * sizeof(double2) > (size_t)lds_limit()) break;
+    if ((long long)(H / tx) * split < 256 && tx > 1) break;
+    if (tx > 1 && (size_t)tx * nfld * ny * sizeof(double2) > 65536) break;
+    best = tx;
+  }
+  return best;
+}
+
+static bool size_ok(int nx, int ny) {
+  FftPlan a, b;
+  return nx >= 4 && ny >= 4 && nx % 2 == 0 && ny % 2 == 0 && make_plan(nx / 2, a) && make_plan(ny, b);
+}
+
+// ---------------------------------------------------------------------------
+// Tier A
+// ---------------------------------------------------------------------------
+struct fsibe_spectral {
+  int nx, ny, H;
+  double xl, yl, nu, pi, scalex, scaley;
+  FftPlan px, py;
+  hipStream_t stream;
+  double2 *twx, *twy, *tmp;
+  double *hbuf_in, *hbuf_out;  // device staging for host-pointer calls
+};
+
+static PackedDims packed_dims(const fsibe_spectral *c) {
+  PackedDims d;
+  d.nx = c->nx; d.ny = c->ny;
+  d.sx = 2.0 * c->pi / c->xl;
+  d.sy = 2.0 * c->pi / c->yl;
+  d.scalex = c->scalex; d.scaley = c->scaley;
+  return d;
+}
+
+extern "C" int fsibe_spectral_create(int nx, int ny, double xl, double yl, double nu, fsibe_spectral **out) {
+  if (!out) return set_err(FSIBE_EARG, "out is NULL");
+  *out = nullptr;
+  if (!size_ok(nx, ny))
+    return set_err(FSIBE_ESIZE, "grid %dx%d not supported (even sizes, nx/2 and ny = 2^a 3^b)", nx, ny);
+  if (!(xl > 0.0) || !(yl > 0.0)) return set_err(FSIBE_EARG, "domain size must be positive");
+  fsibe_spectral *c = new fsibe_spectral();
+  c->nx = nx; c->ny = ny; c->H = nx / 2;
+  c->xl = xl; c->yl = yl; c->nu = nu;
+  c->pi = 4.0 * atan(1.0);
+  { double a = 2.0 * c->pi / xl; c->scalex = a * a; }
+  { double a = 2.0 * c->pi / yl; c->scaley = a * a; }
+  make_plan(c->H, c->px);
+  make_plan(ny, c->py);
+  HIPCHK(hipStreamCreate(&c->stream));
+  auto tx = twiddles(nx), ty = twiddles(ny);
+  HIPCHK(hipMalloc(&c->twx, sizeof(double2) * nx));
+  HIPCHK(hipMalloc(&c->twy, sizeof(double2) * ny));
+  HIPCHK(hipMemcpy(c->twx, tx.data(), sizeof(double2) * nx, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->twy, ty.data(), sizeof(double2) * ny, hipMemcpyHostToDevice));
+  size_t N = (size_t)nx * ny;
+  HIPCHK(hipMalloc(&c->tmp, sizeof(double2) * (N / 2)));
+  HIPCHK(hipMalloc(&c->hbuf_in, sizeof(double) * N));
+  HIPCHK(hipMalloc(&c->hbuf_out, sizeof(double) * N));
+  *out = c;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_spectral_destroy(fsibe_spectral *c) {
+  if (!c) return FSIBE_OK;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->twx); (void)hipFree(c->twy); (void)hipFree(c->tmp);
+  (void)hipFree(c->hbuf_in); (void)hipFree(c->hbuf_out);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FSIBE_OK;
+}
+
+// real rows -> half spectrum (unnormalised), into tmp
+static int rows_r2c(int nx, int ny, const FftPlan &px, const double2 *twx, const double *f, double2 *out,
+                    hipStream_t st) {
+  const int H = nx / 2;
+  const int RB = std::max(1, std::min(8, 2048 / H));
+  const int E = emax_of((long long)RB * H);
+  const size_t lds = sizeof(double2) * RB * H;
+  const int grid = (ny + RB - 1) / RB;
+  LAUNCH_E(E, k_rows_r2c, grid, lds, st, f, out, px, twx, ny, RB);
+  return FSIBE_OK;
+}
+
+static int rows_c2r(int nx, int ny, const FftPlan &px, const double2 *twx, const double2 *in, double *f,
+                    hipStream_t st) {
+  const int H = nx / 2;
+  const int RB = std::max(1, std::min(8, 2048 / H));
+  const int E = emax_of((long long)RB * H);
+  const size_t lds = sizeof(double2) * RB * H;
+  const int grid = (ny + RB - 1) / RB;
+  LAUNCH_E(E, k_rows_c2r, grid, lds, st, in, f, px, twx, ny, RB);
+  return FSIBE_OK;
+}
+
+static int dev_coftxy(fsibe_spectral *c, const double *f, double *fk, hipStream_t st) {
+  CHK(rows_r2c(c->nx, c->ny, c->px, c->twx, f, c->tmp, st));
+  const int TX = pick_tx(c->H, c->ny, 1, 1);
+  const int E = emax_of((long long)TX * c->ny);
+  const size_t lds = sizeof(double2) * TX * c->ny;
+  const double norm = 1.0 / ((double)c->nx * (double)c->ny);
+  LAUNCH_E(E, k_cols_fwd_pack, c->H / TX, lds, st, c->tmp, fk, c->py, c->twy, c->H, TX, norm);
+  return FSIBE_OK;
+}
+
+static int dev_cofitxy(fsibe_spectral *c, const double *fk, double *f, hipStream_t st) {
+  const int TX = pick_tx(c->H, c->ny, 1, 1);
+  const int E = emax_of((long long)TX * c->ny);
+  const size_t lds = sizeof(double2) * TX * c->ny;
+  LAUNCH_E(E, k_cols_unpack_inv, c->H / TX, lds, st, fk, c->tmp, c->py, c->twy, c->H, TX);
+  CHK(rows_c2r(c->nx, c->ny, c->px, c->twx, c->tmp, f, st));
+  return FSIBE_OK;
+}
+
+static int grid1d(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 4096); }
+
+static int dev_op(fsibe_spectral *c, int which, const double *in, double *out, hipStream_t st) {
+  PackedDims d = packed_dims(c);
+  if (which == 0) {
+    k_cofdx<<<grid1d((size_t)c->H * c->ny), 256, 0, st>>>(in, out, d);
+  } else if (which == 1) {
+    k_cofdy<<<grid1d((size_t)c->nx * (c->ny / 2)), 256, 0, st>>>(in, out, d);
+  } else if (which >= 2 && which <= 5) {
+    k_block_op<<<grid1d((size_t)c->H * (c->ny / 2)), 256, 0, st>>>(in, out, d, which);
+  } else {
+    return set_err(FSIBE_EARG, "unknown operator %d", which);
+  }
+  HIPCHK(hipGetLastError());
+  return FSIBE_OK;
+}
+
+static int dev_vis(fsibe_spectral *c, double dt, double *work, hipStream_t st) {
+  PackedDims d = packed_dims(c);
+  const double coefx = -(dt * c->nu * c->scalex), coefy = -(dt * c->nu * c->scaley);
+  k_vis<<<grid1d((size_t)c->H * (c->ny / 2)), 256, 0, st>>>(work, d, coefx, coefy);
+  HIPCHK(hipGetLastError());
+  return FSIBE_OK;
+}
+
+static hipStream_t pick_stream(fsibe_spectral *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+
+extern "C" int fsibe_dev_coftxy(fsibe_spectral *c, const double *d_f, double *d_fk, void *s) {
+  if (!c || !d_f || !d_fk) return set_err(FSIBE_EARG, "null argument");
+  return dev_coftxy(c, d_f, d_fk, pick_stream(c, s));
+}
+extern "C" int fsibe_dev_cofitxy(fsibe_spectral *c, const double *d_fk, double *d_f, void *s) {
+  if (!c || !d_f || !d_fk) return set_err(FSIBE_EARG, "null argument");
+  return dev_cofitxy(c, d_fk, d_f, pick_stream(c, s));
+}
+extern "C" int fsibe_dev_spectral_op(fsibe_spectral *c, int which, const double *d_in, double *d_out, void *s) {
+  if (!c || !d_in || !d_out) return set_err(FSIBE_EARG, "null argument");
+  return dev_op(c, which, d_in, d_out, pick_stream(c, s));
+}
+extern "C" int fsibe_dev_vis(fsibe_spectral *c, double dt, double *d_work, void *s) {
+  if (!c || !d_work) return set_err(FSIBE_EARG, "null argument");
+  return dev_vis(c, dt, d_work, pick_stream(c, s));
+}
+extern "C" int fsibe_spectral_sync(fsibe_spectral *c) {
+  if (!c) return set_err(FSIBE_EARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FSIBE_OK;
+}
+
+// host-pointer wrappers: copy in, run, copy out
+static int host_call(fsibe_spectral *c, const double *in, double *out, int kind, int which, double dt) {
+  if (!c || !out || (kind != 3 && !in)) return set_err(FSIBE_EARG, "null argument");
+  const size_t bytes = sizeof(double) * (size_t)c->nx * c->ny;
+  if (kind != 3) HIPCHK(hipMemcpyAsync(c->hbuf_in, in, bytes, hipMemcpyHostToDevice, c->stream));
+  if (kind == 0) CHK(dev_coftxy(c, c->hbuf_in, c->hbuf_out, c->stream));
+  else if (kind == 1) CHK(dev_cofitxy(c, c->hbuf_in, c->hbuf_out, c->stream));
+  else if (kind == 2) CHK(dev_op(c, which, c->hbuf_in, c->hbuf_out, c->stream));
+  else CHK(dev_vis(c, dt, c->hbuf_out, c->stream));
+  HIPCHK(hipMemcpyAsync(out, c->hbuf_out, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_coftxy(fsibe_spectral *c, const double *f, double *fk) { return host_call(c, f, fk, 0, 0, 0); }
+extern "C" int fsibe_cofitxy(fsibe_spectral *c, const double *fk, double *f) { return host_call(c, fk, f, 1, 0, 0); }
+extern "C" int fsibe_cofdx(fsibe_spectral *c, const double *a, double *b) { return host_call(c, a, b, 2, 0, 0); }
+extern "C" int fsibe_cofdy(fsibe_spectral *c, const double *a, double *b) { return host_call(c, a, b, 2, 1, 0); }
+extern "C" int fsibe_cofdxdx(fsibe_spectral *c, const double *a, double *b) { return host_call(c, a, b, 2, 2, 0); }
+extern "C" int fsibe_cofdydy(fsibe_spectral *c, const double *a, double *b) { return host_call(c, a, b, 2, 3, 0); }
+extern "C" int fsibe_cofdxdy(fsibe_spectral *c, const double *a, double *b) { return host_call(c, a, b, 2, 4, 0); }
+extern "C" int fsibe_poisson(fsibe_spectral *c, const double *a, double *b) { return host_call(c, a, b, 2, 5, 0); }
+extern "C" int fsibe_vis(fsibe_spectral *c, double dt, double *w) { return host_call(c, nullptr, w, 3, 0, dt); }
+
+// dealiase_mask.f90:1-33 (host; the 0/1 pattern is computed once per run)
+static void dealias_host(int nx, int ny, double *d) {
+  const int kx_max = nx / 2 - 1, ky_max = ny / 2 - 1;
+  const double kx_trunc = 2.0 / 3.0 * (double)kx_max, ky_trunc = 2.0 / 3.0 * (double)ky_max;
+  for (size_t i = 0; i < (size_t)nx * ny; ++i) d[i] = 1.0;
+  for (int ky = 0; ky <= ky_max; ++ky)
+    for (int kx = 0; kx <= kx_max; ++kx) {
+      const double a = (double)kx, b = (double)ky;
+      if (a * a / (kx_trunc * kx_trunc) + b * b / (ky_trunc * ky_trunc) >= 1.0) {
+        d[(size_t)(2 * ky) * nx + 2 * kx] = 0.0;
+        d[(size_t)(2 * ky) * nx + 2 * kx + 1] = 0.0;
+        d[(size_t)(2 * ky + 1) * nx + 2 * kx] = 0.0;
+        d[(size_t)(2 * ky + 1) * nx + 2 * kx + 1] = 0.0;
+      }
+    }
+}
+
+extern "C" int fsibe_dealiase_mask(fsibe_spectral *c, double *d) {
+  if (!c || !d) return set_err(FSIBE_EARG, "null argument");
+  dealias_host(c->nx, c->ny, d);
+  return FSIBE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host kinematics: mouvement.f90:5-241 (+ cases 600, 601), integrate_position
+// ---------------------------------------------------------------------------
+static void mouvement_host(const fsibe_params &p, int im, double x0, double y0, double angle, double phase,
+                           double time, double &alpha, double &alpha_t, double &alpha_tt, double LE[6]) {
+  const double pi = 4.0 * atan(1.0);
+  for (int i = 0; i < 6; ++i) LE[i] = 0.0;
+  alpha = alpha_t = alpha_tt = 0.0;
+  if (im == 0) {
+    LE[0] = x0; LE[1] = y0;
+    alpha = angle * pi / 180.0;
+  } else if ((im >= 201 && im <= 213) || (im >= 300 && im <= 350)) {
+    static const double ts[13] = {1.0, 1.09, 1.20, 1.29, 1.33, 1.4, 1.5, 1.71, 2.0, 2.4, 3.0, 4.0, 1.62};
+    const double y_max = 0.5, t_max = (im < 300) ? ts[im - 201] : 1.62;
+    double k, kt, ktt;
+    if (time <= 1.0) {
+      const double a = -20.0, b = 70.0, c = -84.0, d = 35.0;
+      const double t2 = time * time, t3 = t2 * time, t4 = t3 * time, t5 = t4 * time, t6 = t5 * time,
+                   t7 = t6 * time;
+      k = a * t7 + b * t6 + c * t5 + d * t4;
+      kt = 7.0 * a * t6 + 6.0 * b * t5 + 5.0 * c * t4 + 4.0 * d * t3;
+      ktt = 42. * a * t5 + 30. * b * t4 + 20. * c * t3 + 12. * d * t2;
+    } else {
+      k = 1.0; kt = 0.0; ktt = 0.0;
+    }
+    const double arg = 2.0 * pi * time / t_max + pi / 2. + phase;
+    const double w = 2.0 * pi / t_max;
+    const double y = y_max * sin(arg), yt = y_max * (2.0 * pi / t_max) * cos(arg);
+    const double ytt = -(y_max * (w * w) * sin(arg));
+    LE[0] = x0; LE[1] = y0 + k * y; LE[3] = kt * y + yt * k; LE[5] = ktt * y + ytt * k + 2. * kt * yt;
+    alpha = angle * pi / 180.0;
+  } else if (im == 2) {
+    const double y_max = 0.5, t_max = 2.5, w = 2.0 * pi / t_max;
+    LE[0] = x0;
+    LE[1] = y0 + y_max * sin(2.0 * pi * time / t_max + pi / 2);
+    LE[3] = y_max * (2.0 * pi / t_max) * cos(2.0 * pi * time / t_max + pi / 2);
+    LE[5] = -(y_max * (w * w) * sin(2.0 * pi * time / t_max + pi / 2));
+    alpha = angle * pi / 180.0;
+  } else if (im == 3) {
+    const double angle_max = 80.0 * pi / 180.0, f = 0.5, w = 2.0 * pi * f;
+    LE[0] = x0; LE[1] = y0;
+    alpha = angle_max * sin(2.0 * pi * f * time);
+    alpha_t = angle_max * cos(2.0 * pi * f * time) * (2.0 * pi * f);
+    alpha_tt = -1.0 * angle_max * sin(2.0 * pi * f * time) * (w * w);
+  } else if (im == 4) {
+    const double u0 = -1.0;
+    LE[0] = x0 + u0 * time; LE[2] = u0; LE[1] = y0; LE[3] = 0.0;
+    alpha = angle * pi / 180.0;
+  } else if (im == 600) {
+    const double A = p.heave_amp, w = 2.0 * pi * p.heave_freq;
+    LE[0] = x0;
+    LE[1] = y0 + A * sin(w * time);
+    LE[3] = A * w * cos(w * time);
+    LE[5] = -(A * (w * w) * sin(w * time));
+    alpha = angle * pi / 180.0;
+  } else if (im == 601) {
+    const double *c = p.motion_coef;
+    const double w0 = 2.0 * pi * p.heave_freq;
+    double h = 0, ht = 0, htt = 0, a = 0, at = 0, att = 0;
+    for (int m = 1; m <= 2; ++m) {
+      const double wm = w0 * m, sn = sin(wm * time), cs = cos(wm * time);
+      const double ah = c[2 * (m - 1)], bh = c[2 * (m - 1) + 1];
+      const double ap = c[4 + 2 * (m - 1)], bp = c[4 + 2 * (m - 1) + 1];
+      h += ah * sn + bh * (cs - 1.0);
+      ht += wm * (ah * cs - bh * sn);
+      htt += -(wm * wm) * (ah * sn + bh * cs);
+      a += ap * sn + bp * (cs - 1.0);
+      at += wm * (ap * cs - bp * sn);
+      att += -(wm * wm) * (ap * sn + bp * cs);
+    }
+    LE[0] = x0; LE[1] = y0 + h; LE[3] = ht; LE[5] = htt;
+    alpha = angle * pi / 180.0 + a; alpha_t = at; alpha_tt = att;
+  }
+}
+
+extern "C" int fsibe_mouvement(const fsibe_params *p, int im, double x0, double y0, double angle, double phase,
+                               double time, double *alpha, double *alpha_t, double *alpha_tt, double LE[6]) {
+  if (!p || !alpha || !alpha_t || !alpha_tt || !LE) return set_err(FSIBE_EARG, "null argument");
+  mouvement_host(*p, im, x0, y0, angle, phase, time, *alpha, *alpha_t, *alpha_tt, LE);
+  return FSIBE_OK;
+}
+
+static void integrate_position_host(const fsibe_params &p, int im, double time, const double *theta,
+                                    const double *theta_dot, double *x, double *y, double *vx, double *vy) {
+  double alpha, at, att, LE[6];
+  mouvement_host(p, im, p.x0, p.y0, p.AngleBeam, 0.0, time, alpha, at, att, LE);
+  const int ns = p.ns;
+  const double ds = p.length / (double)(ns - 1);
+  for (int s = 0; s < ns; ++s) {
+    double xrel = 0.0, yrel = 0.0, vxrel = 0.0, vyrel = 0.0;
+    for (int i = 0; i <= s - 1; ++i) {
+      xrel = xrel + cos(theta[i]) * ds;
+      yrel = yrel + sin(theta[i]) * ds;
+      vxrel = vxrel - theta_dot[i] * sin(theta[i]) * ds;
+      vyrel = vyrel + theta_dot[i] * cos(theta[i]) * ds;
+    }
+    x[s] = LE[0] + xrel * cos(alpha) - yrel * sin(alpha);
+    y[s] = LE[1] + yrel * cos(alpha) + xrel * sin(alpha);
+    vx[s] = LE[2] + cos(alpha) * (vxrel - at * yrel) - sin(alpha) * (vyrel + at * xrel);
+    vy[s] = LE[3] + cos(alpha) * (vyrel + at * xrel) + sin(alpha) * (vxrel - at * yrel);
+  }
+}
+
+extern "C" int fsibe_integrate_position(const fsibe_params *p, int im, double time, const double *theta,
+                                        const double *theta_dot, double *x, double *y, double *vx, double *vy) {
+  if (!p || !theta || !theta_dot || !x || !y || !vx || !vy) return set_err(FSIBE_EARG, "null argument");
+  if (p->ns < 2) return set_err(FSIBE_EARG, "ns must be >= 2");
+  integrate_position_host(*p, im, time, theta, theta_dot, x, y, vx, vy);
+  return FSIBE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tier B
+// ---------------------------------------------------------------------------
+static constexpr int kNC = 128;  // BeamSurfaceIntegral circle points (BeamForces.f90:311)
+
+struct Box {
+  int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+  int w() const { return x1 >= x0 ? x1 - x0 + 1 : 0; }
+  int h() const { return y1 >= y0 ? y1 - y0 + 1 : 0; }
+  bool empty() const { return x1 < x0 || y1 < y0; }
+  void add(int a0, int a1, int b0, int b1) {
+    if (a1 < a0 || b1 < b0) return;
+    if (empty()) { x0 = a0; x1 = a1; y0 = b0; y1 = b1; return; }
+    x0 = std::min(x0, a0); x1 = std::max(x1, a1); y0 = std::min(y0, b0); y1 = std::max(y1, b1);
+  }
+};
+
+// kernel classes timed by fsibe_profile_* (HIP events on the sim's stream)
+enum ProfKernel {
+  P_MASK = 0, P_VEL, P_COLS_W0, P_ROWS_NL, P_SCALARS, P_COLS_AB2, P_COLS_W1, P_ROWS_PRESS, P_REDUCE,
+  P_COLS_PRESS, P_ROWS_BOX, P_INTERP, P_FORCES, P_NK
+};
+
+struct Prof {
+  bool on = false;
+  hipEvent_t ev[P_NK][2] = {};
+  bool rec[P_NK] = {};
+  double ms[P_NK] = {};
+  long long cnt[P_NK] = {};
+};
+
+struct fsibe_sim {
+  Prof prof;
+  fsibe_params p;
+  int nx, ny, H, ns;
+  double pi, dx, dy, ds, scalex, scaley, gerf;
+  bool sharp, penal_on, sponge_on, pressure_on, forces_on, visc;
+  StepConsts sc;
+  FlowParams fp;
+  FftPlan px, py;
+  hipStream_t stream;
+  hipEvent_t ev_upload;
+  bool upload_pending = false;
+  double2 *twx, *twy;
+  double2 *W[2], *NL[2], *T[7];
+  double *u1, *u2, *vor_mean, *ms, *wallrow;
+  double *sums, *umax_part;
+  DevScalars *dscal;
+  StepOutDev *dout;
+  double *mask_m, *mask_vx, *mask_vy;
+  double *fb_p, *fb_sa, *fb_sb;
+  double *d_pbeam, *d_tbeam;
+  // per-step upload area: prims | targets | node data | circle data | vals
+  char *h_stage, *d_stage;
+  size_t stage_bytes;
+  char *h_res;  // pinned result record
+  int n0 = 0, n1 = 1;
+  // geometry of the current step (set by create_mask)
+  bool have_beam = false, sponge_done = false;
+  double time_mask = 0.0, alpha = 0.0, LE[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> bx, by, bvx, bvy, btheta, btheta_dot;
+  Box mbox, fbox;
+  Window win;
+  int nprim = 0, ntg = 0;
+  size_t off_prim = 0, off_tg = 0, off_nd = 0, off_cd = 0, off_val = 0;
+  ForceGeom fg;
+  bool forces_ready = false;
+  bool press_box_valid = false;
+  // time loop state (fsibe_run)
+  double time = 0.0, dt0 = 0.0, dt1 = 0.0;
+  int it = 0;
+  bool startup_done = false;
+  fsibe_step_out last;
+  std::vector<double> pbeam, tbeam;
+};
+
+static void prof_begin(fsibe_sim *s, int k) {
+  if (s->prof.on) (void)hipEventRecord(s->prof.ev[k][0], s->stream);
+}
+static void prof_end(fsibe_sim *s, int k) {
+  if (s->prof.on) {
+    (void)hipEventRecord(s->prof.ev[k][1], s->stream);
+    s->prof.rec[k] = true;
+  }
+}
+// after a stream synchronisation: accumulate the recorded intervals
+static void prof_collect(fsibe_sim *s) {
+  if (!s->prof.on) return;
+  for (int k = 0; k < P_NK; ++k) {
+    if (!s->prof.rec[k]) continue;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, s->prof.ev[k][0], s->prof.ev[k][1]) == hipSuccess) {
+      s->prof.ms[k] += ms;
+      s->prof.cnt[k] += 1;
+    }
+    s->prof.rec[k] = false;
+  }
+}
+
+static double smooth_step_host(const fsibe_sim *s, double x, double t, double h) {
+  if (!s->sharp) {
+    const double delta = h * s->gerf;
+    return 0.5 * (erf((t - x) / delta) + erf((x + t) / delta));
+  }
+  return x <= t ? 1.0 : 0.0;
+}
+
+static inline int fnint(double x) { return (int)lround(x); }
+
+// create_sponge_mask, create_mask.f90:223-364 (types 0,1,2,3,10)
+static void sponge_host(const fsibe_sim *s, const double *mask_full, std::vector<double> &ms) {
+  const int nx = s->nx, ny = s->ny;
+  const double dx = s->dx, dy = s->dy, N = s->p.N_smooth, xl = s->p.xl, yl = s->p.yl;
+  ms.assign((size_t)nx * ny, 0.0);
+  const int isp = s->p.iSponge;
+  if (isp == 1) {
+    const double sss = s->p.SpongeSize, ssz = sss - 2.0 * N * dx, sb = xl - sss;
+    int ixmax = fnint((xl - 0.5 * sss) / dx), ixmin = fnint(sb / dx);
+    for (int ix = ixmin; ix <= ixmax; ++ix) {
+      const double tmp = smooth_step_host(s, xl - 0.5 * sss - (double)ix * dx, ssz * 0.5, N * dx);
+      for (int iy = 0; iy < ny; ++iy) ms[(size_t)iy * nx + ix] = tmp;
+    }
+    ixmin = fnint((xl - 0.5 * sss) / dx);
+    ixmax = nx - 1;
+    for (int ix = ixmin; ix <= ixmax; ++ix) {
+      const double tmp = smooth_step_host(s, (double)ix * dx - xl + 0.5 * sss - 1.0 * dx, ssz * 0.4, N * dx);
+      for (int iy = 0; iy < ny; ++iy) ms[(size_t)iy * nx + ix] = tmp;
+    }
+  } else if (isp == 10) {
+    const int ixmin = fnint((xl - s->p.SpongeSize) / dx);
+    for (int ix = std::max(ixmin, 0); ix < nx; ++ix)
+      for (int iy = 0; iy < ny; ++iy) ms[(size_t)iy * nx + ix] = 1.0;
+  } else if (isp == 2) {
+    const double sss = s->p.SpongeSize, ssz = sss - 2.0 * N * dx;
+    int ixmax = fnint((0.5 * sss) / dx);
+    for (int ix = 0; ix <= ixmax; ++ix) {
+      const double tmp = smooth_step_host(s, 0.5 * sss - (double)ix * dx, ssz * 0.5, N * dx);
+      for (int iy = 0; iy < ny; ++iy) ms[(size_t)iy * nx + ix] = tmp;
+    }
+    const int ixmin = fnint((0.5 * sss) / dx);
+    ixmax = fnint((1.0 * sss) / dx);
+    for (int ix = ixmin; ix <= ixmax; ++ix) {
+      const double tmp = smooth_step_host(s, (double)ix * dx - 0.5 * sss, ssz * 0.5, N * dx);
+      for (int iy = 0; iy < ny; ++iy) ms[(size_t)iy * nx + ix] = tmp;
+    }
+    for (int ix = 0; ix <= ixmax; ++ix)
+      for (int iy = 0; iy < ny; ++iy) {
+        const size_t k = (size_t)iy * nx + ix;
+        const double m = mask_full ? mask_full[k] : 0.0;
+        if (m * s->p.eps > 0.0) ms[k] = ms[k] * (1.0 - m * s->p.eps);
+      }
+  } else if (isp == 3) {
+    const double ssz = s->p.SpongeSize - N * dx;
+    for (int ix = 0; ix < nx; ++ix)
+      for (int iy = 0; iy < ny; ++iy) {
+        const double t1 = smooth_step_host(s, (double)ix * dx, ssz, N * dx);
+        const double t2 = smooth_step_host(s, xl - (double)ix * dx, ssz, N * dx);
+        const double t3 = smooth_step_host(s, (double)iy * dy, ssz, N * dy);
+        const double t4 = smooth_step_host(s, yl - (double)iy * dy, ssz, N * dy);
+        double v = t1 + t2 + t3 + t4;
+        if (v > 1.0) v = 1.0;
+        ms[(size_t)iy * nx + ix] = v;
+      }
+  }
+  for (auto &v : ms) v = v / s->p.eps_sponge;
+}
+
+extern "C" void *fsibe_sim_stream(fsibe_sim *s) { return s ? (void *)s->stream : nullptr; }
+
+extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
+  if (!s) return FSIBE_OK;
+  (void)hipStreamSynchronize(s->stream);
+  void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
+                  s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow, s->sums,
+                  s->umax_part, s->dscal, s->dout, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
+                  s->fb_sb, s->d_pbeam, s->d_tbeam, s->d_stage};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  if (s->h_stage) (void)hipHostFree(s->h_stage);
+  if (s->h_res) (void)hipHostFree(s->h_res);
+  (void)hipEventDestroy(s->ev_upload);
+  for (int k = 0; k < P_NK; ++k)
+    for (int j = 0; j < 2; ++j)
+      if (s->prof.ev[k][j]) (void)hipEventDestroy(s->prof.ev[k][j]);
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
+  if (!pp || !out) return set_err(FSIBE_EARG, "null argument");
+  *out = nullptr;
+  const fsibe_params &p = *pp;
+  if (!size_ok(p.nx, p.ny))
+    return set_err(FSIBE_ESIZE, "grid %dx%d not supported (even sizes, nx/2 and ny = 2^a 3^b)", p.nx, p.ny);
+  if (p.iFLUSI == 1)
+    return set_err(FSIBE_EUNSUPPORTED, "iFLUSI=1 needs the host solid solver; drive the step with "
+                                       "fsibe_create_mask/evolve/get_forces and solve the beam on the host");
+  if (p.iSponge == 4 || p.iSponge == 22)
+    return set_err(FSIBE_EUNSUPPORTED, "iSponge=%d (velocity sponge) is not implemented", p.iSponge);
+  if (p.iMeanVelocity == 11)
+    return set_err(FSIBE_EUNSUPPORTED, "iMeanVelocity=11 (zero mean pressure forcing) is not implemented");
+  if (p.iBeam > 1) return set_err(FSIBE_EUNSUPPORTED, "iBeam=%d: one beam is supported", p.iBeam);
+  if (p.iBeam == 1 && p.ns < 3) return set_err(FSIBE_EARG, "ns must be >= 3");
+  if (!(p.eps > 0.0) || !(p.xl > 0.0) || !(p.yl > 0.0)) return set_err(FSIBE_EARG, "eps, xl, yl must be > 0");
+  if ((p.iSponge > 0 && p.iSponge < 4) && !(p.eps_sponge > 0.0))
+    return set_err(FSIBE_EARG, "vorticity sponge needs eps_sponge > 0 (params.f90:185-188)");
+  fsibe_sim *s = new fsibe_sim();
+  s->p = p;
+  s->nx = p.nx; s->ny = p.ny; s->H = p.nx / 2; s->ns = p.iBeam ? p.ns : 0;
+  s->pi = 4.0 * atan(1.0);
+  s->dx = p.xl / (double)p.nx;
+  s->dy = p.yl / (double)p.ny;
+  s->ds = p.ns > 1 ? p.length / (double)(p.ns - 1) : 0.0;
+  { double a = 2.0 * s->pi / p.xl; s->scalex = a * a; }
+  { double a = 2.0 * s->pi / p.yl; s->scaley = a * a; }
+  s->gerf = fabs((exp(-(2.0 * 1.0) * (2.0 * 1.0)) - 1.0) / sqrt(s->pi));
+  s->sharp = fabs(p.N_smooth) < 1e-4;
+  if (s->p.U_mean_true == 0.0) s->p.U_mean_true = 1.0;
+  s->penal_on = p.iBeam > 0 || p.iWalls > 0 || p.iCylinder > 0;
+  s->sponge_on = p.iSponge > 0 && p.iSponge < 4;
+  s->pressure_on = p.iFLUSI == 1 || p.iSaveBeam > 0;
+  s->forces_on = s->pressure_on && p.iBeam > 0;
+  s->visc = p.iViscous == 1;
+  make_plan(s->H, s->px);
+  make_plan(s->ny, s->py);
+  StepConsts &c = s->sc;
+  c.nx = s->nx; c.ny = s->ny; c.H = s->H;
+  c.sx = 2.0 * s->pi / p.xl; c.sy = 2.0 * s->pi / p.yl;
+  c.scalex = s->scalex; c.scaley = s->scaley;
+  c.norm = 1.0 / ((double)s->nx * (double)s->ny);
+  {
+    const double kxt = 2.0 / 3.0 * (double)(s->nx / 2 - 1), kyt = 2.0 / 3.0 * (double)(s->ny / 2 - 1);
+    c.kxt2 = kxt * kxt; c.kyt2 = kyt * kyt;
+  }
+  c.idealis = p.idealis == 1;
+  c.nu = p.nu; c.eps = p.eps; c.dx = s->dx; c.dy = s->dy;
+  FlowParams &f = s->fp;
+  f.iMeanVelocity = p.iMeanVelocity; f.penal_on = s->penal_on;
+  f.theta_inf = p.theta_inf; f.T_fullspeed = p.T_fullspeed; f.U_mean_true = s->p.U_mean_true;
+  f.cfl = p.cfl; f.eps = p.eps; f.dt_fixed = p.dt_fixed; f.Time_end = p.Time_end; f.pi = s->pi;
+  f.dx = s->dx; f.dy = s->dy; f.nu = p.nu; f.scalex = s->scalex; f.scaley = s->scaley;
+
+  int rc = FSIBE_OK;
+  auto fail = [&](int code) { fsibe_sim_destroy(s); return code; };
+#define SIMCHK(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      rc = set_err(FSIBE_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+      return fail(rc);                                                                  \
+    }                                                                                   \
+  } while (0)
+  SIMCHK(hipStreamCreate(&s->stream));
+  SIMCHK(hipEventCreateWithFlags(&s->ev_upload, hipEventDisableTiming));
+  const size_t N = (size_t)s->nx * s->ny, NH = (size_t)s->H * s->ny;
+  auto tx = twiddles(s->nx), ty = twiddles(s->ny);
+  SIMCHK(hipMalloc(&s->twx, sizeof(double2) * s->nx));
+  SIMCHK(hipMalloc(&s->twy, sizeof(double2) * s->ny));
+  SIMCHK(hipMemcpy(s->twx, tx.data(), sizeof(double2) * s->nx, hipMemcpyHostToDevice));
+  SIMCHK(hipMemcpy(s->twy, ty.data(), sizeof(double2) * s->ny, hipMemcpyHostToDevice));
+  for (int i = 0; i < 2; ++i) {
+    SIMCHK(hipMalloc(&s->W[i], sizeof(double2) * NH));
+    SIMCHK(hipMalloc(&s->NL[i], sizeof(double2) * NH));
+    SIMCHK(hipMemset(s->W[i], 0, sizeof(double2) * NH));
+    SIMCHK(hipMemset(s->NL[i], 0, sizeof(double2) * NH));
+  }
+  for (int i = 0; i < 7; ++i) {
+    SIMCHK(hipMalloc(&s->T[i], sizeof(double2) * NH));
+    SIMCHK(hipMemset(s->T[i], 0, sizeof(double2) * NH));
+  }
+  SIMCHK(hipMalloc(&s->u1, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->u2, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->vor_mean, sizeof(double) * N));
+  SIMCHK(hipMemset(s->u1, 0, sizeof(double) * N));
+  SIMCHK(hipMemset(s->u2, 0, sizeof(double) * N));
+  SIMCHK(hipMemset(s->vor_mean, 0, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->sums, sizeof(double) * s->ny * kNSUM));
+  SIMCHK(hipMalloc(&s->umax_part, sizeof(double) * s->ny));
+  SIMCHK(hipMemset(s->umax_part, 0, sizeof(double) * s->ny));
+  SIMCHK(hipMalloc(&s->dscal, sizeof(DevScalars)));
+  SIMCHK(hipMemset(s->dscal, 0, sizeof(DevScalars)));
+  SIMCHK(hipMalloc(&s->dout, sizeof(StepOutDev)));
+  SIMCHK(hipMemset(s->dout, 0, sizeof(StepOutDev)));
+  SIMCHK(hipMalloc(&s->mask_m, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->mask_vx, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->mask_vy, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->fb_p, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->fb_sa, sizeof(double) * N));
+  SIMCHK(hipMalloc(&s->fb_sb, sizeof(double) * N));
+  const int nsb = std::max(s->ns, 1);
+  SIMCHK(hipMalloc(&s->d_pbeam, sizeof(double) * nsb));
+  SIMCHK(hipMalloc(&s->d_tbeam, sizeof(double) * nsb));
+  SIMCHK(hipMemset(s->d_pbeam, 0, sizeof(double) * nsb));
+  SIMCHK(hipMemset(s->d_tbeam, 0, sizeof(double) * nsb));
+  // staging layout
+  {
+    size_t off = 0;
+    auto al = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
+    const int nprim_max = 2 * nsb + 8;
+    const int ntg_max = 6 * nsb + 6 * kNC;
+    s->off_prim = al(sizeof(Prim) * nprim_max);
+    s->off_tg = al(sizeof(double) * 3 * ntg_max);
+    s->off_nd = al(sizeof(double) * 16 * nsb);
+    s->off_cd = al(sizeof(double) * 8 * 2 * kNC);
+    s->off_val = al(sizeof(double) * ntg_max);
+    s->stage_bytes = off;
+  }
+  SIMCHK(hipHostMalloc((void **)&s->h_stage, s->stage_bytes, hipHostMallocDefault));
+  SIMCHK(hipMalloc(&s->d_stage, s->stage_bytes));
+  SIMCHK(hipMemset(s->d_stage, 0, s->stage_bytes));
+  SIMCHK(hipHostMalloc((void **)&s->h_res, sizeof(StepOutDev) + 2 * sizeof(double) * nsb, hipHostMallocDefault));
+  s->ms = nullptr;
+  s->wallrow = nullptr;
+  if (p.iWalls > 0) {
+    std::vector<double> wr(s->ny, 0.0);
+    const double N_ = p.N_smooth, h_star = p.h_channel + N_ * s->dy;
+    const int iymax = fnint(h_star / s->dy);
+    for (int j = 0; j <= iymax && j < s->ny; ++j) wr[j] = smooth_step_host(s, (double)j * s->dy, p.h_channel, N_ * s->dy);
+    const int iymin = fnint((p.yl - h_star) / s->dy);
+    for (int j = s->ny - 1; j >= iymin && j >= 0; --j)
+      wr[j] = smooth_step_host(s, fabs((double)j * s->dy - p.yl), p.h_channel, N_ * s->dy);
+    SIMCHK(hipMalloc(&s->wallrow, sizeof(double) * s->ny));
+    SIMCHK(hipMemcpy(s->wallrow, wr.data(), sizeof(double) * s->ny, hipMemcpyHostToDevice));
+  }
+  s->bx.assign(nsb, 0.0); s->by.assign(nsb, 0.0); s->bvx.assign(nsb, 0.0); s->bvy.assign(nsb, 0.0);
+  s->btheta.assign(nsb, 0.0); s->btheta_dot.assign(nsb, 0.0);
+  s->pbeam.assign(nsb, 0.0); s->tbeam.assign(nsb, 0.0);
+  memset(&s->last, 0, sizeof(s->last));
+  s->win = Window{0, -1, 0, -1};
+#undef SIMCHK
+  *out = s;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_set_vortk(fsibe_sim *s, const double *vk) {
+  if (!s || !vk) return set_err(FSIBE_EARG, "null argument");
+  const size_t N = (size_t)s->nx * s->ny;
+  HIPCHK(hipMemcpyAsync(s->fb_p, vk, sizeof(double) * N, hipMemcpyHostToDevice, s->stream));
+  k_packed_to_half<<<grid1d((size_t)s->H * (s->ny / 2)), 256, 0, s->stream>>>(s->fb_p, s->W[s->n0], s->nx, s->ny);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->press_box_valid = false;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_swap_levels(fsibe_sim *s) {
+  if (!s) return set_err(FSIBE_EARG, "null argument");
+  std::swap(s->n0, s->n1);
+  return FSIBE_OK;
+}
+
+// ---- geometry of one step (host) ------------------------------------------
+static void add_prim(fsibe_sim *s, Prim &P, int &np) {
+  Prim *dst = reinterpret_cast<Prim *>(s->h_stage + s->off_prim);
+  dst[np++] = P;
+}
+
+static int build_geometry(fsibe_sim *s, double time) {
+  const int nx = s->nx, ny = s->ny, ns = s->ns;
+  const double dx = s->dx, dy = s->dy, N = s->p.N_smooth, t = s->p.t_beam;
+  const double mdx = std::max(dx, dy);
+  int np = 0;
+  Box mb;
+  if (s->p.iWalls > 0) {
+    const double h_star = s->p.h_channel + N * dy;
+    mb.add(0, nx - 1, 0, std::min(ny - 1, fnint(h_star / dy)));
+    mb.add(0, nx - 1, std::max(0, fnint((s->p.yl - h_star) / dy)), ny - 1);
+  }
+  const double delta = N * mdx * s->gerf;
+  auto dot = [&](double px_, double py_, double vx_, double vy_, double tt) {
+    Prim P{};
+    P.type = PRIM_DOT;
+    const double r = s->sharp ? tt + mdx : tt + 7.0 * delta + mdx;
+    P.ix0 = std::max(0, (int)floor((px_ - r) / dx) - 1);
+    P.ix1 = std::min(nx - 1, (int)ceil((px_ + r) / dx) + 1);
+    P.iy0 = std::max(0, (int)floor((py_ - r) / dy) - 1);
+    P.iy1 = std::min(ny - 1, (int)ceil((py_ + r) / dy) + 1);
+    P.a[0] = px_; P.a[1] = py_; P.a[2] = vx_; P.a[3] = vy_; P.a[4] = tt;
+    add_prim(s, P, np);
+    mb.add(P.ix0, P.ix1, P.iy0, P.iy1);
+  };
+  if (s->p.iCylinder == 1) dot(s->p.x0 - s->p.R_cylinder, s->p.y0, 0.0, 0.0, s->p.R_cylinder);
+  if (s->p.iBeam > 0) {
+    const double *X = s->bx.data(), *Y = s->by.data(), *VX = s->bvx.data(), *VY = s->bvy.data();
+    const double t_star = t + 2.0 * N * mdx;
+    for (int i = 0; i <= ns - 2; ++i) {  // DrawBeamSegment :740-761
+      Prim P{};
+      P.type = PRIM_SEG;
+      const double p1x = X[i], p1y = Y[i], p2x = X[i + 1], p2y = Y[i + 1];
+      const double al = -atan2((p2y - p1y), (p2x - p1x));
+      P.ix0 = std::max(fnint((std::min(p1x, p2x) - t_star) / dx), 0);
+      P.ix1 = std::min(fnint((std::max(p1x, p2x) + t_star) / dx), nx - 1);
+      P.iy0 = std::max(fnint((std::min(p1y, p2y) - t_star) / dy), 0);
+      P.iy1 = std::min(fnint((std::max(p1y, p2y) + t_star) / dy), ny - 1);
+      const double ca = cos(al), sa = sin(al);
+      P.a[0] = ca * p1x - sa * p1y; P.a[1] = sa * p1x + ca * p1y;
+      P.a[2] = ca * p2x - sa * p2y; P.a[3] = sa * p2x + ca * p2y;
+      P.a[4] = ca; P.a[5] = sa; P.a[6] = t_star;
+      P.a[7] = VX[i]; P.a[8] = VY[i]; P.a[9] = VX[i + 1]; P.a[10] = VY[i + 1];
+      add_prim(s, P, np);
+      mb.add(P.ix0, P.ix1, P.iy0, P.iy1);
+    }
+    if (s->p.iSharpTrailing == 1) {  // DrawSharpEnd :796-822
+      Prim P{};
+      P.type = PRIM_SHARP;
+      const double al = s->btheta[ns - 1] + s->alpha;
+      const double pex = X[ns - 1], pey = Y[ns - 1];
+      const double p1x = pex - 2.0 * mdx * cos(al), p1y = pey - 2.0 * mdx * sin(al);
+      const double p2x = pex + 4.0 * mdx * cos(al), p2y = pey + 4.0 * mdx * sin(al);
+      P.ix0 = std::max(fnint((std::min(p1x, p2x) - t_star) / dx), 0);
+      P.ix1 = std::min(fnint((std::max(p1x, p2x) + t_star) / dx), nx - 1);
+      P.iy0 = std::max(fnint((std::min(p1y, p2y) - t_star) / dy), 0);
+      P.iy1 = std::min(fnint((std::max(p1y, p2y) + t_star) / dy), ny - 1);
+      const double ca = cos(al), sa = sin(al);
+      P.a[0] = ca * p1x + sa * p1y; P.a[1] = -sa * p1x + ca * p1y;
+      P.a[2] = ca * p2x + sa * p2y; P.a[3] = -sa * p2x + ca * p2y;
+      P.a[4] = ca; P.a[5] = sa; P.a[6] = t_star; P.a[7] = VX[ns - 1]; P.a[8] = VY[ns - 1]; P.a[9] = mdx;
+      add_prim(s, P, np);
+      mb.add(P.ix0, P.ix1, P.iy0, P.iy1);
+    }
+    for (int i = 1; i <= ns - 2; ++i) {  // DrawHinge :579-592 via create_mask :103-107
+      Prim P{};
+      P.type = PRIM_HINGE;
+      const double a1 = atan2((Y[i] - Y[i - 1]), (X[i] - X[i - 1]));
+      const double a2 = atan2((Y[i + 1] - Y[i]), (X[i + 1] - X[i]));
+      const double ts2 = t + 2. * N * mdx;
+      P.ix0 = std::max(fnint((X[i] - ts2) / dx), 0);
+      P.ix1 = std::min(fnint((X[i] + ts2) / dx), nx - 1);
+      P.iy0 = std::max(fnint((Y[i] - ts2) / dy), 0);
+      P.iy1 = std::min(fnint((Y[i] + ts2) / dy), ny - 1);
+      const double c1 = a1 < 0.0 ? 2.0 * s->pi + a1 : a1, c2 = a2 < 0.0 ? 2.0 * s->pi + a2 : a2;
+      P.a[0] = X[i]; P.a[1] = Y[i];
+      P.a[2] = c1 * 180.0 / s->pi; P.a[3] = c2 * 180.0 / s->pi;
+      P.a[4] = ts2; P.a[5] = VX[i]; P.a[6] = VY[i];
+      add_prim(s, P, np);
+      mb.add(P.ix0, P.ix1, P.iy0, P.iy1);
+    }
+    if (Y[0] < 0.0) return set_err(FSIBE_EDOMAIN, "Mask:: Beam Hinge. Index out of region (create_mask.f90:685-688)");
+    dot(X[0], Y[0], VX[0], VY[0], t);
+    if (s->p.iSharpTrailing == 0) dot(X[ns - 1], Y[ns - 1], VX[ns - 1], VY[ns - 1], t);
+  }
+  s->nprim = np;
+  s->mbox = mb;
+
+  // cal_drag window (NavierStokes.f90:538-564)
+  {
+    int ixmin, ixmax, iymin, iymax;
+    const double *X = s->bx.data(), *Y = s->by.data();
+    if (s->p.iBeam > 0) {
+      double ymax = Y[0], ymin = Y[0], xmin = X[0], xmax = X[0];
+      for (int i = 1; i < ns; ++i) {
+        ymax = std::max(ymax, Y[i]); ymin = std::min(ymin, Y[i]);
+        xmin = std::min(xmin, X[i]); xmax = std::max(xmax, X[i]);
+      }
+      const double tb = s->p.t_beam, rc = s->p.R_cylinder;
+      iymax = (int)((std::max(ymax + 1.5 * tb, s->LE[1] + 2.0 * rc)) / dy);
+      iymin = (int)((std::min(ymin - 1.5 * tb, s->LE[1] - 2.0 * rc)) / dy);
+      ixmin = (int)(std::min((s->LE[0] - std::max(2.5 * rc, 1.5 * tb)), (xmin - 1.5 * tb)) / dx);
+      ixmax = (int)((xmax + 1.5 * tb) / dx);
+    } else {
+      iymax = ny - 1; iymin = 0; ixmax = nx - 1; ixmin = 0;
+    }
+    const double h_star = s->p.h_channel + s->p.N_smooth * dy;
+    if (s->p.iWalls > 0) {
+      if (ixmin < 0) ixmin = 0;
+      if (ixmax > nx - 1) ixmax = nx - 1;
+      if (iymin < fnint(h_star / dy) + 2) iymin = fnint(h_star / dy) + 2;
+      if (iymax > fnint((s->p.yl - h_star) / dy) - 2) iymax = fnint((s->p.yl - h_star) / dy) - 2;
+    }
+    ixmin = std::max(ixmin, 0); ixmax = std::min(ixmax, nx - 1);
+    iymin = std::max(iymin, 0); iymax = std::min(iymax, ny - 1);
+    s->win = Window{ixmin, ixmax, iymin, iymax};
+  }
+
+  // interpolation targets (BeamForces.f90:150-165, :351-362, :389-400)
+  s->forces_ready = false;
+  s->ntg = 0;
+  Box fb;
+  if (s->forces_on) {
+    double *tg = reinterpret_cast<double *>(s->h_stage + s->off_tg);
+    double *nd = reinterpret_cast<double *>(s->h_stage + s->off_nd);
+    double *cd = reinterpret_cast<double *>(s->h_stage + s->off_cd);
+    const double x2 = s->p.xl - dx, y2 = s->p.yl - dy;
+    const double dxi = (x2 - 0.0) / (double)(nx - 1), dyi = (y2 - 0.0) / (double)(ny - 1);
+    int nt = 0;
+    bool bad = false;
+    auto target = [&](double xt, double yt, int fid) {
+      if (xt > x2 || xt < 0.0 || yt > y2 || yt < 0.0) bad = true;
+      const int i = (int)((xt - 0.0) / dxi), j = (int)((yt - 0.0) / dyi);
+      if (i - 4 < 0 || i + 4 > nx - 1 || j - 4 < 0 || j + 4 > ny - 1) bad = true;
+      fb.add(i - 4, i + 4, j - 4, j + 4);
+      tg[3 * nt] = xt; tg[3 * nt + 1] = yt; tg[3 * nt + 2] = (double)fid;
+      ++nt;
+    };
+    const double *X = s->bx.data(), *Y = s->by.data();
+    const double rr = s->p.t_beam + s->p.N_smooth * std::max(dy, dx);
+    const double r1 = s->p.t_beam - 0.0 * s->p.N_smooth * std::max(dy, dx);
+    for (int n = 0; n < ns; ++n) {
+      const double th = s->btheta[n] + s->alpha;
+      const double st = sin(th), ct = cos(th);
+      nd[16 * n] = st; nd[16 * n + 1] = ct; nd[16 * n + 2] = sin(2.0 * th); nd[16 * n + 3] = cos(2.0 * th);
+      const double xu = X[n] - rr * st, yu = Y[n] + rr * ct, xb = X[n] + rr * st, yb = Y[n] - rr * ct;
+      const double xu1 = X[n] - r1 * st, yu1 = Y[n] + r1 * ct, xb1 = X[n] + r1 * st, yb1 = Y[n] - r1 * ct;
+      target(xu1, yu1, 0);
+      target(xb1, yb1, 0);
+      target(xu, yu, 1);
+      target(xu, yu, 2);
+      target(xb, yb, 1);
+      target(xb, yb, 2);
+    }
+    const bool bsi = s->visc && s->p.iCylinder == 0;
+    if (bsi) {
+      const double dphi = s->pi / (double)(kNC - 1);
+      double beta = s->btheta[ns - 1] + s->alpha;
+      for (int i = 0; i < kNC; ++i) {
+        const double phi = -(0.5 * s->pi - beta) + (double)i * dphi;
+        const double xc = X[ns - 1] + rr * cos(phi), yc = Y[ns - 1] + rr * sin(phi);
+        cd[8 * i] = cos(phi); cd[8 * i + 1] = sin(phi); cd[8 * i + 2] = cos(2.0 * phi); cd[8 * i + 3] = sin(2.0 * phi);
+        target(xc, yc, 1);
+        target(xc, yc, 2);
+        target(xc, yc, 0);
+      }
+      beta = s->btheta[1] + s->alpha;  // BeamForces.f90:390 takes node 1
+      for (int i = 0; i < kNC; ++i) {
+        const double phi = (0.5 * s->pi + beta) + (double)i * dphi;
+        const double xc = X[0] + rr * cos(phi), yc = Y[0] + rr * sin(phi);
+        const int q = kNC + i;
+        cd[8 * q] = cos(phi); cd[8 * q + 1] = sin(phi); cd[8 * q + 2] = cos(2.0 * phi); cd[8 * q + 3] = sin(2.0 * phi);
+        target(xc, yc, 1);
+        target(xc, yc, 2);
+        target(xc, yc, 0);
+      }
+    }
+    if (bad) return set_err(FSIBE_EDOMAIN, "DeltaInterpolation: target coordinates not in the field "
+                                           "(Interpolation.f90:19-24) or stencil leaves the grid");
+    s->ntg = nt;
+    ForceGeom &g = s->fg;
+    g.ns = ns; g.nc = bsi ? kNC : 0; g.nt = nt;
+    g.bx0 = fb.x0; g.by0 = fb.y0; g.bw = fb.w(); g.bh = fb.h();
+    g.dxi = dxi; g.dyi = dyi; g.pi = s->pi; g.ds = s->ds;
+    g.dL = (s->pi / (double)(kNC - 1)) * rr;
+    g.visc = s->visc; g.bsi_on = bsi;
+    // soft startup (BeamForces.f90:246-252)
+    const double Tr = s->p.T_release, tau = s->p.tau;
+    double soft;
+    if (time <= Tr) soft = 0.0;
+    else if (time > Tr && time < (Tr + tau)) {
+      const double a = time - Tr;
+      soft = (a * a * a) / (-0.5 * (tau * tau * tau)) + 3. * (a * a) / (tau * tau);
+    } else soft = 1.0;
+    g.soft = soft;
+    s->fbox = fb;
+    s->forces_ready = true;
+  } else if (s->pressure_on) {
+    s->fbox = Box();
+  }
+  return FSIBE_OK;
+}
+
+static MaskBox mask_box(const fsibe_sim *s) {
+  MaskBox m;
+  m.x0 = s->mbox.x0; m.x1 = s->mbox.x1; m.y0 = s->mbox.y0; m.y1 = s->mbox.y1; m.w = s->mbox.w();
+  if (s->mbox.empty()) { m.x0 = 0; m.x1 = -1; m.y0 = 0; m.y1 = -1; m.w = 0; }
+  m.m = s->mask_m; m.vx = s->mask_vx; m.vy = s->mask_vy;
+  return m;
+}
+
+static int ensure_sponge(fsibe_sim *s) {
+  if (s->sponge_done) return FSIBE_OK;
+  s->sponge_done = true;
+  if (!s->sponge_on) return FSIBE_OK;
+  std::vector<double> mfull;
+  if (s->p.iSponge == 2) {
+    mfull.assign((size_t)s->nx * s->ny, 0.0);
+    if (!s->mbox.empty()) {
+      std::vector<double> mb((size_t)s->mbox.w() * s->mbox.h());
+      HIPCHK(hipMemcpyAsync(mb.data(), s->mask_m, sizeof(double) * mb.size(), hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+      for (int y = 0; y < s->mbox.h(); ++y)
+        for (int x = 0; x < s->mbox.w(); ++x)
+          mfull[(size_t)(y + s->mbox.y0) * s->nx + x + s->mbox.x0] = mb[(size_t)y * s->mbox.w() + x];
+    }
+  }
+  std::vector<double> ms;
+  sponge_host(s, mfull.empty() ? nullptr : mfull.data(), ms);
+  HIPCHK(hipMalloc(&s->ms, sizeof(double) * ms.size()));
+  HIPCHK(hipMemcpy(s->ms, ms.data(), sizeof(double) * ms.size(), hipMemcpyHostToDevice));
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_create_mask(fsibe_sim *s, double time, const double *x, const double *y, const double *vx,
+                                 const double *vy, const double *theta, double alpha, const double LE[6]) {
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  if (s->ns > 0 && (!x || !y || !vx || !vy || !theta)) return set_err(FSIBE_EARG, "null beam array");
+  if (s->upload_pending) {
+    HIPCHK(hipEventSynchronize(s->ev_upload));
+    s->upload_pending = false;
+  }
+  for (int i = 0; i < s->ns; ++i) {
+    s->bx[i] = x[i]; s->by[i] = y[i]; s->bvx[i] = vx[i]; s->bvy[i] = vy[i]; s->btheta[i] = theta[i];
+  }
+  s->alpha = alpha;
+  for (int i = 0; i < 6; ++i) s->LE[i] = LE ? LE[i] : 0.0;
+  s->time_mask = time;
+  s->have_beam = true;
+  CHK(build_geometry(s, time));
+  HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->stage_bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipEventRecord(s->ev_upload, s->stream));
+  s->upload_pending = true;
+  if (!s->mbox.empty()) {
+    MaskConsts mc;
+    mc.dx = s->dx; mc.dy = s->dy; mc.t = s->p.t_beam; mc.h = s->p.N_smooth * std::max(s->dx, s->dy);
+    mc.gerf = s->gerf; mc.inv_eps_unused = 0.0; mc.ds = s->ds; mc.pi = s->pi; mc.sharp = s->sharp;
+    mc.eps = s->p.eps;
+    dim3 blk(64, 4);
+    dim3 grd((s->mbox.w() + 63) / 64, (s->mbox.h() + 3) / 4);
+    prof_begin(s, P_MASK);
+    k_mask<<<grd, blk, 0, s->stream>>>(reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim,
+                                       s->wallrow ? s->wallrow + 0 : nullptr, mc, s->mbox.x0, s->mbox.y0,
+                                       s->mbox.w(), s->mbox.h(), s->mask_m, s->mask_vx, s->mask_vy);
+    HIPCHK(hipGetLastError());
+    prof_end(s, P_MASK);
+  }
+  CHK(ensure_sponge(s));
+  return FSIBE_OK;
+}
+
+// ---- fluid step ----------------------------------------------------------
+static int cols_inv_fields(fsibe_sim *s, const double2 *W, double2 *const *outs, const int *fsel, int nf) {
+  const int TX = pick_tx(s->H, s->ny, 1, nf);
+  const int E = emax_of((long long)TX * s->ny);
+  const size_t lds = sizeof(double2) * TX * s->ny;
+  const int grid = (s->H / TX) * nf;
+  double2 *o[4] = {outs[0], nf > 1 ? outs[1] : nullptr, nf > 2 ? outs[2] : nullptr, nf > 3 ? outs[3] : nullptr};
+  int f[4] = {fsel[0], nf > 1 ? fsel[1] : 0, nf > 2 ? fsel[2] : 0, nf > 3 ? fsel[3] : 0};
+  LAUNCH_E(E, k_cols_inv_fields, grid, lds, s->stream, W, o[0], o[1], o[2], o[3], f[0], f[1], f[2], f[3], nf,
+           s->py, s->twy, s->sc, TX);
+  return FSIBE_OK;
+}
+
+// u = cal_velocity(time, vortk(n0)) and the CFL maxima (EvolveFluidExplicit :440)
+static int velocity_only(fsibe_sim *s, double time) {
+  double2 *outs[2] = {s->T[1], s->T[2]};
+  const int fsel[2] = {1, 2};
+  CHK(cols_inv_fields(s, s->W[s->n0], outs, fsel, 2));
+  const int E = emax_of(2LL * s->H);
+  const size_t lds = sizeof(double2) * 3 * s->H;
+  double mx, my;
+  {
+    // Mean_ux/Mean_uy at `time` (mean_velocity.f90), host copy of mean_u
+    const FlowParams &f = s->fp;
+    double tf = 1.0;
+    switch (f.iMeanVelocity) {
+      case 1:
+        if (time < f.T_fullspeed) {
+          const double t1 = f.T_fullspeed;
+          tf = (time * time * time) / (-0.5 * t1 * t1 * t1) + 3 * (time * time) / (t1 * t1);
+        }
+        mx = cos(f.theta_inf) * tf; my = sin(f.theta_inf) * tf; break;
+      case 2:
+        if (time < f.T_fullspeed) tf = 0.5 * (1.0 - cos(0.5 * f.pi * time / f.T_fullspeed));
+        mx = cos(f.theta_inf) * tf; my = sin(f.theta_inf) * tf; break;
+      case 3: mx = cos(f.theta_inf); my = sin(f.theta_inf); break;
+      case 6: mx = time; my = 0.0; break;
+      case 22: mx = 0.1; my = 0.0; break;
+      default: mx = 0.0; my = 0.0; break;
+    }
+  }
+  const double UMx = s->p.U_mean_true * mx, UMy = s->p.U_mean_true * my;
+  MaskBox mb = mask_box(s);
+  LAUNCH_E(E, k_rows_press, s->ny, lds, s->stream, nullptr, s->T[1], s->T[2], s->u1, s->u2, s->vor_mean, mb,
+           s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, UMx, UMy, 0, 0, s->px, s->twx, s->sc);
+  return FSIBE_OK;
+}
+
+static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
+  if (s->p.iBeam > 0 && !s->have_beam) return set_err(FSIBE_ESTATE, "fsibe_create_mask must be called first");
+  if (!s->have_beam)  // no beam: walls / cylinder only (create_mask.f90:34-69)
+    CHK(fsibe_create_mask(s, time, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr));
+  const int n0 = s->n0, n1 = s->n1;
+  if (explicit_step) {
+    prof_begin(s, P_VEL);
+    CHK(velocity_only(s, time));
+    prof_end(s, P_VEL);
+  }
+  // K1: A = Fy^-1[W0], B = Fy^-1[i ky W0]
+  {
+    double2 *outs[2] = {s->T[0], s->T[1]};
+    const int fsel[2] = {0, 4};
+    prof_begin(s, P_COLS_W0);
+    CHK(cols_inv_fields(s, s->W[n0], outs, fsel, 2));
+    prof_end(s, P_COLS_W0);
+  }
+  MaskBox mb = mask_box(s);
+  // K2
+  {
+    const int E = emax_of(3LL * s->H);
+    const size_t lds = sizeof(double2) * 3 * s->H;
+    prof_begin(s, P_ROWS_NL);
+    LAUNCH_E(E, k_rows_nl, s->ny, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2, s->sponge_on ? s->ms : nullptr,
+             mb, (int)s->penal_on, s->T[2], s->T[3], s->px, s->twx, s->sc);
+    prof_end(s, P_ROWS_NL);
+  }
+  // K3
+  prof_begin(s, P_SCALARS);
+  k_step_scalars<<<1, kNT, 0, s->stream>>>(s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step);
+  HIPCHK(hipGetLastError());
+  prof_end(s, P_SCALARS);
+  // K4
+  {
+    const int TX = pick_tx(s->H, s->ny, 2, 1);
+    const int E = emax_of(2LL * TX * s->ny);
+    const size_t lds = sizeof(double2) * 2 * TX * s->ny;
+    prof_begin(s, P_COLS_AB2);
+    LAUNCH_E(E, k_cols_ab2, s->H / TX, lds, s->stream, s->T[2], s->T[3], s->W[n0], s->NL[n1], s->NL[n0], s->W[n1],
+             s->dscal, (int)explicit_step, s->py, s->twy, s->sc, TX);
+    prof_end(s, P_COLS_AB2);
+  }
+  // K5: omega, psi, i ky psi (, -ky^2 psi) of vortk(n1)
+  {
+    const bool st = s->forces_on && s->visc;
+    double2 *outs[4] = {s->T[0], s->T[1], s->T[2], s->T[3]};
+    const int fsel[4] = {0, 1, 2, 3};
+    prof_begin(s, P_COLS_W1);
+    CHK(cols_inv_fields(s, s->W[n1], outs, fsel, st ? 4 : 3));
+    prof_end(s, P_COLS_W1);
+  }
+  // K6
+  {
+    const int E = emax_of(3LL * s->H);
+    const size_t lds = sizeof(double2) * 3 * s->H;
+    prof_begin(s, P_ROWS_PRESS);
+    LAUNCH_E(E, k_rows_press, s->ny, lds, s->stream, s->T[0], s->T[1], s->T[2], s->u1, s->u2, s->vor_mean, mb,
+             s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, 0.0, 0.0, 1, (int)s->pressure_on, s->px,
+             s->twx, s->sc);
+    prof_end(s, P_ROWS_PRESS);
+  }
+  // K9 reductions
+  prof_begin(s, P_REDUCE);
+  k_reduce_step<<<1, 64, 0, s->stream>>>(s->sums, s->ny, s->dscal, s->dout, s->dx * s->dy, s->p.nu, s->p.eps,
+                                         s->p.U_mean_true, (int)(s->p.iBeam > 0));
+  HIPCHK(hipGetLastError());
+  prof_end(s, P_REDUCE);
+  // K7 + K8 on the rows/columns the interpolation needs
+  s->press_box_valid = false;
+  if (s->forces_on && s->forces_ready && !s->fbox.empty()) {
+    const Box &fb = s->fbox;
+    {
+      const int TX = pick_tx(s->H, s->ny, 2, 1);
+      const int E = emax_of(2LL * TX * s->ny);
+      const size_t lds = sizeof(double2) * 2 * TX * s->ny;
+      prof_begin(s, P_COLS_PRESS);
+      LAUNCH_E(E, k_cols_press, s->H / TX, lds, s->stream, s->T[4], s->T[5], s->T[6], fb.y0, fb.y1, s->py,
+               s->twy, s->sc, TX);
+      prof_end(s, P_COLS_PRESS);
+    }
+    {
+      const int nf = s->visc ? 3 : 1;
+      const int E = emax_of((long long)nf * s->H);
+      const size_t lds = sizeof(double2) * 3 * s->H;
+      prof_begin(s, P_ROWS_BOX);
+      LAUNCH_E(E, k_rows_box, fb.h(), lds, s->stream, s->T[6], s->T[1], s->T[2], s->T[3], s->u1, s->u2, s->fb_p,
+               s->fb_sa, s->fb_sb, fb.y0, fb.x0, fb.x1, fb.w(), (int)s->visc, s->px, s->twx, s->sc);
+      prof_end(s, P_ROWS_BOX);
+    }
+    s->press_box_valid = true;
+  }
+  return FSIBE_OK;
+}
+
+static int launch_forces(fsibe_sim *s) {
+  if (!s->forces_on || !s->forces_ready || !s->press_box_valid) return FSIBE_OK;
+  const double *tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
+  double *val = reinterpret_cast<double *>(s->d_stage + s->off_val);
+  prof_begin(s, P_INTERP);
+  k_interp<<<(s->ntg + 127) / 128, 128, 0, s->stream>>>(tg, s->ntg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val);
+  HIPCHK(hipGetLastError());
+  prof_end(s, P_INTERP);
+  prof_begin(s, P_FORCES);
+  k_forces_final<<<1, 64, 0, s->stream>>>(reinterpret_cast<const double *>(s->d_stage + s->off_nd),
+                                          reinterpret_cast<const double *>(s->d_stage + s->off_cd), val, s->fg,
+                                          s->d_pbeam, s->d_tbeam, s->dout);
+  HIPCHK(hipGetLastError());
+  prof_end(s, P_FORCES);
+  return FSIBE_OK;
+}
+
+static int fetch_results(fsibe_sim *s) {
+  const int nsb = std::max(s->ns, 1);
+  HIPCHK(hipMemcpyAsync(s->h_res, s->dout, sizeof(StepOutDev), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(s->h_res + sizeof(StepOutDev), s->d_pbeam, sizeof(double) * nsb, hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipMemcpyAsync(s->h_res + sizeof(StepOutDev) + sizeof(double) * nsb, s->d_tbeam, sizeof(double) * nsb,
+                        hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->upload_pending = false;
+  prof_collect(s);
+  const StepOutDev *o = reinterpret_cast<const StepOutDev *>(s->h_res);
+  fsibe_step_out &r = s->last;
+  r.dt1 = o->dt1; r.u_max = o->u_max;
+  for (int i = 0; i < 2; ++i) { r.Force[i] = o->Force[i]; r.Force_unst[i] = o->Force_unst[i]; r.Force_press[i] = o->Force_press[i]; }
+  for (int i = 0; i < 6; ++i) r.energies[i] = o->energies[i];
+  for (int i = 0; i < 10; ++i) r.bsi[i] = o->bsi[i];
+  r.fx_int = o->fx_int; r.fy_int = o->fy_int;
+  r.status = 0;
+  const double *pb = reinterpret_cast<const double *>(s->h_res + sizeof(StepOutDev));
+  for (int i = 0; i < s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_evolve_explicit(fsibe_sim *s, double time, double *dt1) {
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  CHK(evolve(s, time, 0.0, true));
+  CHK(fetch_results(s));
+  if (dt1) *dt1 = s->last.dt1;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_evolve_ab2(fsibe_sim *s, double time, double dt0, double *dt1) {
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  if (!(dt0 > 0.0)) return set_err(FSIBE_EARG, "dt0 must be > 0 (take the startup step first)");
+  CHK(evolve(s, time, dt0, false));
+  CHK(fetch_results(s));
+  if (dt1) *dt1 = s->last.dt1;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_get_forces(fsibe_sim *s, double time, double *pressure_beam, double *tau_beam,
+                                double Force_press[2]) {
+  (void)time;  // the soft-startup factor uses the time given to fsibe_create_mask
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  if (!s->forces_on) return set_err(FSIBE_ESTATE, "forces need iSaveBeam>0 (pressure is not computed otherwise)");
+  if (!s->press_box_valid) return set_err(FSIBE_ESTATE, "no pressure of the current step (evolve first)");
+  CHK(launch_forces(s));
+  CHK(fetch_results(s));
+  for (int i = 0; i < s->ns; ++i) {
+    if (pressure_beam) pressure_beam[i] = s->pbeam[i];
+    if (tau_beam) tau_beam[i] = s->tbeam[i];
+  }
+  if (Force_press) { Force_press[0] = s->last.Force_press[0]; Force_press[1] = s->last.Force_press[1]; }
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_get_step_out(fsibe_sim *s, fsibe_step_out *out) {
+  if (!s || !out) return set_err(FSIBE_EARG, "null argument");
+  *out = s->last;
+  return FSIBE_OK;
+}
+
+// ---- time loop (time_step.f90:120-162, :178-341), prescribed kinematics ---
+static int run_one(fsibe_sim *s) {
+  const fsibe_params &p = s->p;
+  const bool moving = (p.iSponge >= 4 && s->time < p.T_fullspeed) || p.iFLUSI == 1 || p.iMotion > 0;
+  if (p.iBeam > 0 && (moving || !s->have_beam)) {
+    double alpha, at, att, LE[6];
+    mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, s->time, alpha, at, att, LE);
+    CHK(fsibe_create_mask(s, s->time, s->bx.data(), s->by.data(), s->bvx.data(), s->bvy.data(),
+                          s->btheta.data(), alpha, LE));
+  } else if (p.iBeam > 0) {
+    CHK(build_geometry(s, s->time));  // soft-startup factor and targets follow time
+    if (s->upload_pending) { HIPCHK(hipEventSynchronize(s->ev_upload)); s->upload_pending = false; }
+    HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->stage_bytes, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipEventRecord(s->ev_upload, s->stream));
+    s->upload_pending = true;
+  }
+  if (!s->startup_done) {
+    CHK(evolve(s, s->time, 0.0, true));
+  } else {
+    s->dt0 = s->dt1;
+    CHK(evolve(s, s->time, s->dt0, false));
+  }
+  if (s->forces_on) CHK(launch_forces(s));
+  CHK(fetch_results(s));
+  s->dt1 = s->last.dt1;
+  if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0)
+    integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
+                            s->by.data(), s->bvx.data(), s->bvy.data());
+  std::swap(s->n0, s->n1);
+  s->time = s->time + s->dt1;
+  if (s->startup_done) s->it++;
+  s->startup_done = true;
+  return FSIBE_OK;
+}
+
+static void init_beam_host(fsibe_sim *s) {
+  // init_beam.f90:130-167 for one beam
+  const fsibe_params &p = s->p;
+  double alpha, at, att, LE[6];
+  mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, 0.0, alpha, at, att, LE);
+  for (int n = 0; n < s->ns; ++n) {
+    s->bx[n] = LE[0] + (double)n * s->ds;
+    s->by[n] = LE[1];
+    s->bvx[n] = 0.0; s->bvy[n] = 0.0; s->btheta[n] = 0.0; s->btheta_dot[n] = 0.0;
+  }
+  integrate_position_host(p, p.iMotion, 0.0, s->btheta.data(), s->btheta_dot.data(), s->bx.data(), s->by.data(),
+                          s->bvx.data(), s->bvy.data());
+}
+
+extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  if (!s->startup_done && s->time == 0.0 && s->p.iBeam > 0 && !s->have_beam) init_beam_host(s);
+  for (int k = 0; k < nsteps; ++k) {
+    if (!(s->time < s->p.Time_end)) break;
+    CHK(run_one(s));
+  }
+  if (out) *out = s->last;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_get_time(fsibe_sim *s, double *time, double *dt1, int *it) {
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  if (time) *time = s->time;
+  if (dt1) *dt1 = s->dt1;
+  if (it) *it = s->it;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_get_beam(fsibe_sim *s, double *x, double *y, double *vx, double *vy, double *theta,
+                              double *pressure_new, double *tau_new) {
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  for (int i = 0; i < s->ns; ++i) {
+    if (x) x[i] = s->bx[i];
+    if (y) y[i] = s->by[i];
+    if (vx) vx[i] = s->bvx[i];
+    if (vy) vy[i] = s->bvy[i];
+    if (theta) theta[i] = s->btheta[i];
+    if (pressure_new) pressure_new[i] = s->pbeam[i];
+    if (tau_new) tau_new[i] = s->tbeam[i];
+  }
+  return FSIBE_OK;
+}
+
+// ---- field download --------------------------------------------------------
+static int box_to_full(fsibe_sim *s, const double *box, const Box &b, double *host) {
+  const size_t N = (size_t)s->nx * s->ny;
+  std::vector<double> tmp((size_t)std::max(1, b.w() * b.h()));
+  if (!b.empty()) {
+    HIPCHK(hipMemcpyAsync(tmp.data(), box, sizeof(double) * (size_t)b.w() * b.h(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  memset(host, 0, sizeof(double) * N);
+  for (int y = 0; y < b.h(); ++y)
+    memcpy(host + (size_t)(y + b.y0) * s->nx + b.x0, tmp.data() + (size_t)y * b.w(), sizeof(double) * b.w());
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_get_field(fsibe_sim *s, int which, double *out) {
+  if (!s || !out) return set_err(FSIBE_EARG, "null argument");
+  const size_t N = (size_t)s->nx * s->ny;
+  const int g = grid1d((size_t)s->H * (s->ny / 2));
+  switch (which) {
+    case FSIBE_FIELD_VORTK:
+    case FSIBE_FIELD_NLK: {
+      const double2 *src = which == FSIBE_FIELD_VORTK ? s->W[s->n0] : s->NL[s->n0];
+      k_half_to_packed<<<g, 256, 0, s->stream>>>(src, s->fb_p, s->nx, s->ny);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(out, s->fb_p, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+      s->press_box_valid = false;
+      break;
+    }
+    case FSIBE_FIELD_U1: HIPCHK(hipMemcpyAsync(out, s->u1, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream)); break;
+    case FSIBE_FIELD_U2: HIPCHK(hipMemcpyAsync(out, s->u2, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream)); break;
+    case FSIBE_FIELD_VOR_MEAN:
+      HIPCHK(hipMemcpyAsync(out, s->vor_mean, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+      break;
+    case FSIBE_FIELD_MASK_SPONGE:
+      if (s->ms) HIPCHK(hipMemcpyAsync(out, s->ms, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+      else memset(out, 0, sizeof(double) * N);
+      break;
+    case FSIBE_FIELD_MASK: return box_to_full(s, s->mask_m, s->mbox, out);
+    case FSIBE_FIELD_MASKVX: return box_to_full(s, s->mask_vx, s->mbox, out);
+    case FSIBE_FIELD_MASKVY: return box_to_full(s, s->mask_vy, s->mbox, out);
+    case FSIBE_FIELD_VORT: {
+      double2 *outs[1] = {s->T[6]};
+      const int fsel[1] = {0};
+      CHK(cols_inv_fields(s, s->W[s->n0], outs, fsel, 1));
+      CHK(rows_c2r(s->nx, s->ny, s->px, s->twx, s->T[6], s->fb_sb, s->stream));
+      HIPCHK(hipMemcpyAsync(out, s->fb_sb, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+      s->press_box_valid = false;
+      break;
+    }
+    case FSIBE_FIELD_PRESS: {
+      if (!s->pressure_on) { memset(out, 0, sizeof(double) * N); break; }
+      // full-grid pressure from the last step's right-hand side (T4, T5)
+      const int TX = pick_tx(s->H, s->ny, 2, 1);
+      const int E = emax_of(2LL * TX * s->ny);
+      const size_t lds = sizeof(double2) * 2 * TX * s->ny;
+      LAUNCH_E(E, k_cols_press, s->H / TX, lds, s->stream, s->T[4], s->T[5], s->T[6], 0, s->ny - 1, s->py, s->twy,
+               s->sc, TX);
+      const int E2 = emax_of((long long)s->H);
+      const size_t lds2 = sizeof(double2) * 3 * s->H;
+      LAUNCH_E(E2, k_rows_box, s->ny, lds2, s->stream, s->T[6], s->T[1], s->T[2], s->T[3], s->u1, s->u2, s->fb_sb,
+               s->fb_sa, s->fb_sa, 0, 0, s->nx - 1, s->nx, 0, s->px, s->twx, s->sc);
+      HIPCHK(hipMemcpyAsync(out, s->fb_sb, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+      s->press_box_valid = false;
+      break;
+    }
+    default: return set_err(FSIBE_EARG, "unknown field %d", which);
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_get_box(fsibe_sim *s, int which, double *out, int box[4]) {
+  if (!s || !box) return set_err(FSIBE_EARG, "null argument");
+  const Box &b = which < 3 ? s->fbox : s->mbox;
+  box[0] = b.x0; box[1] = b.x1; box[2] = b.y0; box[3] = b.y1;
+  if (!out || b.empty()) return FSIBE_OK;
+  const double *src = nullptr;
+  switch (which) {
+    case 0: src = s->fb_p; break;
+    case 1: src = s->fb_sa; break;
+    case 2: src = s->fb_sb; break;
+    case 3: src = s->mask_m; break;
+    case 4: src = s->mask_vx; break;
+    case 5: src = s->mask_vy; break;
+    default: return set_err(FSIBE_EARG, "unknown box field %d", which);
+  }
+  if (which < 3 && !s->press_box_valid) return set_err(FSIBE_ESTATE, "no pressure box of the current step");
+  HIPCHK(hipMemcpyAsync(out, src, sizeof(double) * (size_t)b.w() * b.h(), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_profile_enable(fsibe_sim *s, int on) {
+  if (!s) return set_err(FSIBE_EARG, "null sim");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < P_NK; ++k) {
+    for (int j = 0; j < 2; ++j)
+      if (!s->prof.ev[k][j]) HIPCHK(hipEventCreate(&s->prof.ev[k][j]));
+    s->prof.ms[k] = 0.0;
+    s->prof.cnt[k] = 0;
+    s->prof.rec[k] = false;
+  }
+  s->prof.on = on != 0;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_profile_read(fsibe_sim *s, double *total_ms, long long *count, int n) {
+  if (!s || !total_ms || !count) return set_err(FSIBE_EARG, "null argument");
+  for (int k = 0; k < n && k < P_NK; ++k) {
+    total_ms[k] = s->prof.ms[k];
+    count[k] = s->prof.cnt[k];
+  }
+  return P_NK;
+}

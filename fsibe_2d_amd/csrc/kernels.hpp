@@ -1,0 +1,322 @@
+// kernels.hpp -- HIP kernels of the FSIBE_2D fluid step for gfx950.
+//
+// Layouts in HBM
+//   real field     double  f[ny][nx]        (reference f(0:nx-1,0:ny-1))
+//   packed field   double  fk[ny][nx]       (reference coftxy layout)
+//   half spectrum  double2 c[ny][H], H=nx/2 (complex coefficient of mode
+//                  (kx, ky) at c[ky mod ny][kx]; kx=nx/2 and ky=ny/2 dropped)
+// The packed layout is only used at the Tier-A boundary; the step keeps its
+// spectral state as half spectra, where every spectral operator of the
+// reference (cofdx, cofdy, poisson, vis, dealiase) is a per-mode complex
+// scale: cofdx multiplies c(kx, +-ky) by i kx 2pi/xl, cofdy by i (+-ky) 2pi/yl.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_device.hpp"
+
+namespace fsibe {
+
+constexpr int kNT = 256;  // threads per workgroup (4 wavefronts)
+
+// signed y wavenumber of half-spectrum row idx; the Nyquist row maps to 0
+__device__ __forceinline__ int ky_signed(int idx, int ny) {
+  const int h = ny >> 1;
+  return idx < h ? idx : (idx > h ? idx - ny : 0);
+}
+
+// XCD-aware block index: the 8 XCDs receive workgroups round-robin, so
+// consecutive logical tiles (which share 128-byte lines of the column-major
+// access) are given to blocks that run on the same XCD and its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  if ((nblk & 7) != 0) return bid;
+  const int per = nblk >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+// ---- real <-> half-length complex conversions (in place, per row) -------------
+// Forward: Z = FFT_H(x[2n] + i x[2n+1]) -> X[k] = 1/2 (Z_k + conj Z_{H-k})
+//          - i/2 e^{-2 pi i k/nx} (Z_k - conj Z_{H-k}), k < H (X[H] dropped).
+__device__ __forceinline__ double2 r2c_one(double2 a, double2 b, double2 w) {
+#pragma clang fp contract(fast)
+  const double2 bc = cconj(b);
+  const double2 e = make_double2(0.5 * (a.x + bc.x), 0.5 * (a.y + bc.y));
+  const double2 d = make_double2(0.5 * (a.x - bc.x), 0.5 * (a.y - bc.y));
+  const double pr = w.x * d.x - w.y * d.y, pi = w.x * d.y + w.y * d.x;
+  return make_double2(e.x + pi, e.y - pr);
+}
+
+// Inverse (unnormalised c2r, X[H] = 0, Im X[0] ignored):
+// Z_k = (X_k + conj X_{H-k}) + i e^{+2 pi i k/nx} (X_k - conj X_{H-k}).
+__device__ __forceinline__ double2 c2r_one(double2 a, double2 b, double2 wconj) {
+#pragma clang fp contract(fast)
+  const double2 bc = cconj(b);
+  const double2 e = make_double2(a.x + bc.x, a.y + bc.y);
+  const double2 d = make_double2(a.x - bc.x, a.y - bc.y);
+  const double pr = wconj.x * d.x - wconj.y * d.y, pi = wconj.x * d.y + wconj.y * d.x;
+  return make_double2(e.x - pi, e.y + pr);
+}
+
+// rows: `nrows` sequences of length H at buf[r*H + k]
+__device__ void rows_r2c_post(double2 *buf, int nrows, int H, const double2 *__restrict__ twx,
+                              int tid, int nthr) {
+  const int hp = (H >> 1) + 1;
+  for (int e = tid; e < nrows * hp; e += nthr) {
+    const int r = e / hp, k = e - r * hp;
+    double2 *Z = buf + r * H;
+    const int q = (k == 0) ? 0 : H - k;
+    const double2 a = Z[k], b = Z[q];
+    const double2 xk = r2c_one(a, b, twx[k]);
+    if (q != k) Z[q] = r2c_one(b, a, twx[q]);
+    Z[k] = xk;
+  }
+  __syncthreads();
+}
+
+__device__ void rows_c2r_pre(double2 *buf, int nrows, int H, const double2 *__restrict__ twx,
+                             int tid, int nthr) {
+  const int hp = (H >> 1) + 1;
+  for (int e = tid; e < nrows * hp; e += nthr) {
+    const int r = e / hp, k = e - r * hp;
+    double2 *X = buf + r * H;
+    if (k == 0) {
+      const double a = X[0].x;  // x-DC: real part only, x-Nyquist is zero
+      X[0] = make_double2(a, a);
+    } else {
+      const int q = H - k;
+      const double2 a = X[k], b = X[q];
+      const double2 zk = c2r_one(a, b, cconj(twx[k]));
+      if (q != k) X[q] = c2r_one(b, a, cconj(twx[q]));
+      X[k] = zk;
+    }
+  }
+  __syncthreads();
+}
+
+// ============================================================================
+// Tier A kernels (packed layout at the boundary)
+// ============================================================================
+// real rows -> half spectra (unnormalised), nrow rows per block
+template <int EMAX>
+__global__ void __launch_bounds__(kNT) k_rows_r2c(const double *__restrict__ f, double2 *__restrict__ out,
+                                                  FftPlan plan, const double2 *__restrict__ twx, int ny,
+                                                  int RB) {
+  extern __shared__ double2 lds[];
+  const int H = plan.L, nx = 2 * H, tid = threadIdx.x;
+  const int row0 = blockIdx.x * RB;
+  const int nr = min(RB, ny - row0);
+  for (int e = tid; e < nr * H; e += kNT) {
+    const int r = e / H, n = e - r * H;
+    lds[e] = reinterpret_cast<const double2 *>(f + (size_t)(row0 + r) * nx)[n];
+  }
+  __syncthreads();
+  lds_fft<EMAX, false, false>(lds, plan, nr, H, 1, twx, 2, tid, kNT);
+  rows_r2c_post(lds, nr, H, twx, tid, kNT);
+  for (int e = tid; e < nr * H; e += kNT) out[(size_t)row0 * H + e] = lds[e];
+}
+
+template <int EMAX>
+__global__ void __launch_bounds__(kNT) k_rows_c2r(const double2 *__restrict__ in, double *__restrict__ f,
+                                                  FftPlan plan, const double2 *__restrict__ twx, int ny,
+                                                  int RB) {
+  extern __shared__ double2 lds[];
+  const int H = plan.L, nx = 2 * H, tid = threadIdx.x;
+  const int row0 = blockIdx.x * RB;
+  const int nr = min(RB, ny - row0);
+  for (int e = tid; e < nr * H; e += kNT) lds[e] = in[(size_t)row0 * H + e];
+  __syncthreads();
+  rows_c2r_pre(lds, nr, H, twx, tid, kNT);
+  lds_fft<EMAX, true, false>(lds, plan, nr, H, 1, twx, 2, tid, kNT);
+  for (int e = tid; e < nr * H; e += kNT) {
+    const int r = e / H, n = e - r * H;
+    reinterpret_cast<double2 *>(f + (size_t)(row0 + r) * nx)[n] = lds[e];
+  }
+}
+
+// half spectra -> column FFT -> x norm -> packed layout (cof_fftw33.f90:165-189)
+template <int EMAX>
+__global__ void __launch_bounds__(kNT) k_cols_fwd_pack(const double2 *__restrict__ in, double *__restrict__ fk,
+                                                       FftPlan plan, const double2 *__restrict__ twy, int H,
+                                                       int TX, double norm) {
+  extern __shared__ double2 lds[];
+  const int ny = plan.L, nx = 2 * H, tid = threadIdx.x;
+  const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
+  for (int e = tid; e < ny * TX; e += kNT) {
+    const int y = e / TX, c = e - y * TX;
+    lds[e] = in[(size_t)y * H + kx0 + c];
+  }
+  __syncthreads();
+  lds_fft<EMAX, false, true>(lds, plan, TX, 1, TX, twy, 1, tid, kNT);
+  const int hy = ny >> 1;
+  for (int e = tid; e < hy * TX; e += kNT) {
+    const int ky = e / TX, c = e - ky * TX, kx = kx0 + c;
+    double2 r0, r1;
+    if (ky == 0) {
+      const double2 v = lds[c];
+      r0 = make_double2(v.x * norm, v.y * norm);
+      r1 = make_double2(0.0, 0.0);
+    } else {
+      const double2 p = lds[ky * TX + c], m = lds[(ny - ky) * TX + c];
+      r0 = make_double2(0.5 * (p.x + m.x) * norm, 0.5 * (p.y + m.y) * norm);
+      r1 = make_double2(0.5 * (p.y - m.y) * norm, 0.5 * (-p.x + m.x) * norm);
+    }
+    reinterpret_cast<double2 *>(fk + (size_t)(2 * ky) * nx)[kx] = r0;
+    reinterpret_cast<double2 *>(fk + (size_t)(2 * ky + 1) * nx)[kx] = r1;
+  }
+}
+
+// packed layout -> half spectra (cof_fftw33.f90:211-231) -> inverse column FFT
+template <int EMAX>
+__global__ void __launch_bounds__(kNT) k_cols_unpack_inv(const double *__restrict__ fk, double2 *__restrict__ out,
+                                                         FftPlan plan, const double2 *__restrict__ twy, int H,
+                                                         int TX) {
+  extern __shared__ double2 lds[];
+  const int ny = plan.L, nx = 2 * H, tid = threadIdx.x;
+  const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
+  const int hy = ny >> 1;
+  for (int e = tid; e < hy * TX; e += kNT) {
+    const int ky = e / TX, c = e - ky * TX, kx = kx0 + c;
+    const double2 ab = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky) * nx)[kx];
+    const double2 cd = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky + 1) * nx)[kx];
+    if (ky == 0) {
+      lds[c] = ab;
+      lds[hy * TX + c] = make_double2(0.0, 0.0);
+    } else {
+      lds[ky * TX + c] = make_double2(ab.x - cd.y, cd.x + ab.y);
+      lds[(ny - ky) * TX + c] = make_double2(ab.x + cd.y, -cd.x + ab.y);
+    }
+  }
+  __syncthreads();
+  lds_fft<EMAX, true, true>(lds, plan, TX, 1, TX, twy, 1, tid, kNT);
+  for (int e = tid; e < ny * TX; e += kNT) {
+    const int y = e / TX, c = e - y * TX;
+    out[(size_t)y * H + kx0 + c] = lds[e];
+  }
+}
+
+// packed-layout operators, one thread per (even, odd) pair of x -------------
+struct PackedDims {
+  int nx, ny;
+  double sx, sy;          // 2 pi / xl, 2 pi / yl
+  double scalex, scaley;  // (2 pi / L)^2 as in params.f90:140-141
+};
+
+// cofdx.f90:19-26
+__global__ void k_cofdx(const double *__restrict__ fk, double *__restrict__ out, PackedDims d) {
+#pragma clang fp contract(off)
+  const int hx = d.nx >> 1;
+  const size_t n = (size_t)hx * d.ny;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(e % hx);
+    const double2 v = reinterpret_cast<const double2 *>(fk)[e];
+    const double k = (double)kx;
+    reinterpret_cast<double2 *>(out)[e] = make_double2(-v.y * k * d.sx, v.x * k * d.sx);
+  }
+}
+
+// cofdy.f90:19-27: pairs along y
+__global__ void k_cofdy(const double *__restrict__ fk, double *__restrict__ out, PackedDims d) {
+#pragma clang fp contract(off)
+  const int hy = d.ny >> 1;
+  const size_t n = (size_t)d.nx * hy;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(e % d.nx), l = (int)(e / d.nx);
+    const double a = fk[(size_t)(2 * l) * d.nx + x], b = fk[(size_t)(2 * l + 1) * d.nx + x];
+    const double k = (double)l;
+    out[(size_t)(2 * l) * d.nx + x] = -b * k * d.sy;
+    out[(size_t)(2 * l + 1) * d.nx + x] = a * k * d.sy;
+  }
+}
+
+// cofdxdx.f90, cofdydy.f90, cofdxdy.f90, poisson.f90: per 2x2 block
+// op 2: -kx^2 sx^2, 3: -ky^2 sy^2, 4: d/dxdy, 5: poisson
+__global__ void k_block_op(const double *__restrict__ fk, double *__restrict__ out, PackedDims d, int op) {
+#pragma clang fp contract(off)
+  const int hx = d.nx >> 1, hy = d.ny >> 1;
+  const size_t n = (size_t)hx * hy;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(e % hx), ky = (int)(e / hx);
+    const double2 r0 = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky) * d.nx)[kx];
+    const double2 r1 = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky + 1) * d.nx)[kx];
+    double2 o0, o1;
+    if (op == 2) {
+      const double k = (double)kx, k2 = k * k, sc = d.sx * d.sx;
+      o0 = make_double2(-r0.x * k2 * sc, -r0.y * k2 * sc);
+      o1 = make_double2(-r1.x * k2 * sc, -r1.y * k2 * sc);
+    } else if (op == 3) {
+      const double k = (double)ky, k2 = k * k, sc = d.sy * d.sy;
+      o0 = make_double2(-r0.x * k2 * sc, -r0.y * k2 * sc);
+      o1 = make_double2(-r1.x * k2 * sc, -r1.y * k2 * sc);
+    } else if (op == 4) {
+      const double fac = -(double)kx * (double)ky * (d.sx * d.sy);
+      // out(2kx,2ky)=fac*f(2kx+1,2ky+1); out(2kx+1,2ky)=-fac*f(2kx,2ky+1)
+      // out(2kx,2ky+1)=-fac*f(2kx+1,2ky); out(2kx+1,2ky+1)=fac*f(2kx,2ky)
+      o0 = make_double2(fac * r1.y, -fac * r1.x);
+      o1 = make_double2(-fac * r0.y, fac * r0.x);
+    } else {
+      if (kx == 0 && ky == 0) {
+        o0 = make_double2(0.0, 0.0);
+        o1 = o0;
+      } else {
+        const double quot = ((double)(kx * kx) * d.scalex + (double)(ky * ky) * d.scaley);
+        o0 = make_double2(r0.x / quot, r0.y / quot);
+        o1 = make_double2(r1.x / quot, r1.y / quot);
+      }
+    }
+    reinterpret_cast<double2 *>(out + (size_t)(2 * ky) * d.nx)[kx] = o0;
+    reinterpret_cast<double2 *>(out + (size_t)(2 * ky + 1) * d.nx)[kx] = o1;
+  }
+}
+
+// cal_vis.f90:16-33
+__global__ void k_vis(double *__restrict__ work, PackedDims d, double coefx, double coefy) {
+#pragma clang fp contract(off)
+  const int hx = d.nx >> 1, hy = d.ny >> 1;
+  const size_t n = (size_t)hx * hy;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(e % hx), ky = (int)(e / hx);
+    const double v = exp(coefx * (double)(kx * kx) + coefy * (double)(ky * ky));
+    reinterpret_cast<double2 *>(work + (size_t)(2 * ky) * d.nx)[kx] = make_double2(v, v);
+    reinterpret_cast<double2 *>(work + (size_t)(2 * ky + 1) * d.nx)[kx] = make_double2(v, v);
+  }
+}
+
+// packed <-> half spectrum conversions (no normalisation) ---------------------
+__global__ void k_packed_to_half(const double *__restrict__ fk, double2 *__restrict__ c, int nx, int ny) {
+#pragma clang fp contract(off)
+  const int H = nx >> 1, hy = ny >> 1;
+  const size_t n = (size_t)H * hy;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(e % H), ky = (int)(e / H);
+    const double2 ab = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky) * nx)[kx];
+    const double2 cd = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky + 1) * nx)[kx];
+    if (ky == 0) {
+      c[kx] = ab;
+      c[(size_t)hy * H + kx] = make_double2(0.0, 0.0);
+    } else {
+      c[(size_t)ky * H + kx] = make_double2(ab.x - cd.y, cd.x + ab.y);
+      c[(size_t)(ny - ky) * H + kx] = make_double2(ab.x + cd.y, -cd.x + ab.y);
+    }
+  }
+}
+
+__global__ void k_half_to_packed(const double2 *__restrict__ c, double *__restrict__ fk, int nx, int ny) {
+#pragma clang fp contract(off)
+  const int H = nx >> 1, hy = ny >> 1;
+  const size_t n = (size_t)H * hy;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(e % H), ky = (int)(e / H);
+    double2 r0, r1;
+    if (ky == 0) {
+      r0 = c[kx];
+      r1 = make_double2(0.0, 0.0);
+    } else {
+      const double2 p = c[(size_t)ky * H + kx], m = c[(size_t)(ny - ky) * H + kx];
+      r0 = make_double2(0.5 * (p.x + m.x), 0.5 * (p.y + m.y));
+      r1 = make_double2(0.5 * (p.y - m.y), 0.5 * (-p.x + m.x));
+    }
+    reinterpret_cast<double2 *>(fk + (size_t)(2 * ky) * nx)[kx] = r0;
+    reinterpret_cast<double2 *>(fk + (size_t)(2 * ky + 1) * nx)[kx] = r1;
+  }
+}
+
+}  // namespace fsibe

@@ -1,0 +1,103 @@
+"""Run parameters (the share_vars values the fluid step reads) and the
+workloads the benchmarks and tests use.
+
+Field names follow share_vars.f90:8-25 and the order PARAMS.m lists them
+(params.f90:9-81).  nu is the kinematic viscosity (params.f90:50-51 reads the
+Reynolds number and stores nu = 1/Re).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+from ._lib import FsibeParams
+
+MOTION_HEAVE = 600         # y = heave_amp sin(2 pi heave_freq t)  (no reference case)
+MOTION_HEAVE_PITCH = 601   # Fourier heave + pitch                   (no reference case)
+
+
+@dataclasses.dataclass
+class Params:
+    nx: int = 256
+    ny: int = 256
+    ns: int = 64
+    iMeanVelocity: int = 3
+    iSponge: int = 3
+    iBeam: int = 1
+    iWalls: int = 0
+    iCylinder: int = 0
+    iSharpTrailing: int = 0
+    iFLUSI: int = 0
+    iMotion: int = MOTION_HEAVE
+    iViscous: int = 1
+    iSaveBeam: int = 1
+    idealis: int = 1
+    xl: float = 4.0
+    yl: float = 4.0
+    nu: float = 1.0 / 200.0
+    eps: float = 1e-3
+    eps_sponge: float = 1e-3
+    cfl: float = 1.0
+    dt_fixed: float = 2e-4
+    Time_end: float = 1e6
+    N_smooth: float = 2.0
+    t_beam: float = 0.0312
+    length: float = 1.0
+    SpongeSize: float = 0.2
+    R_cylinder: float = 0.0
+    h_channel: float = 0.0
+    x0: float = 1.0
+    y0: float = 2.0
+    AngleBeam: float = 0.0
+    theta_inf: float = 0.0
+    T_fullspeed: float = 0.0
+    T_release: float = 0.0
+    tau: float = 0.0
+    U_mean_true: float = 1.0
+    heave_amp: float = 0.1
+    heave_freq: float = 1.0
+    motion_coef: tuple = (0.0,) * 8
+
+    def to_c(self) -> FsibeParams:
+        p = FsibeParams()
+        for f in dataclasses.fields(self):
+            if f.name == "motion_coef":
+                for i, c in enumerate(self.motion_coef):
+                    p.motion_coef[i] = float(c)
+            else:
+                setattr(p, f.name, getattr(self, f.name))
+        return p
+
+    def replace(self, **kw) -> "Params":
+        return dataclasses.replace(self, **kw)
+
+    @property
+    def dx(self) -> float:
+        return self.xl / self.nx
+
+    @property
+    def dy(self) -> float:
+        return self.yl / self.ny
+
+
+def fsi_256() -> Params:
+    """256^2, domain 4x4, Re=200, vorticity sponge 0.2 wide around the domain
+    (iSponge=3) with inflow U=1 (iMeanVelocity=3), beam of length 1 with 64
+    nodes clamped at (1,2), heave 0.1 sin(2 pi t), eta=1e-3, 2 smoothing cells,
+    2/3 dealiasing, dt=2e-4.  Choices left open by the workload description:
+    eps_sponge = eps = 1e-3, t_beam = 0.0312 (PARAMS.m default), tau = 0."""
+    return Params()
+
+
+def fsi_large(n: int = 1024) -> Params:
+    """The same physics at n^2 (1024 or 2048), Re=1000, 128 beam nodes,
+    dt = 5e-5."""
+    return Params(nx=n, ny=n, ns=128, nu=1.0 / 1000.0, dt_fixed=5e-5)
+
+
+def turbulence_4096(n: int = 4096) -> Params:
+    """Solver-only decaying 2-D turbulence: no mask, no sponge, fluid at rest
+    (iMeanVelocity=4), nu=1e-5, dt=1e-4, domain 2 pi x 2 pi."""
+    import math
+
+    return Params(nx=n, ny=n, ns=64, iBeam=0, iSponge=0, iSaveBeam=0, iMeanVelocity=4,
+                  nu=1e-5, dt_fixed=1e-4, cfl=1e6, xl=2 * math.pi, yl=2 * math.pi, iMotion=0)

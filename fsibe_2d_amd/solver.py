@@ -1,0 +1,149 @@
+"""Tier B: the device-resident fluid step with the reference's call sequence.
+
+    sim.create_mask(time, beam, alpha, LeadingEdge)  create_mask.f90:2-218
+    dt1 = sim.evolve_fluid_explicit(time)            NavierStokes.f90:422-459
+    dt1 = sim.evolve_fluid_ab2(time, dt0)            NavierStokes.f90:463-508
+    p, tau, Fp = sim.get_forces(time)                BeamForces.f90:6-288
+    sim.swap_levels()                                time_step.f90:152, 222
+    sim.run(nsteps)                                  time_step.f90:120-341
+                                                     (prescribed kinematics)
+
+State lives in GPU memory between calls.  Every method raises FsibeError on a
+nonzero status; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import FsibeStepOut, check, lib
+from .params import Params
+
+FIELDS = {
+    "vortk": 0, "nlk": 1, "u1": 3, "u2": 4, "press": 5, "mask": 6, "maskvx": 7,
+    "maskvy": 8, "mask_sponge": 9, "vor_mean": 10, "vort": 14,
+}
+BOX_FIELDS = {"press": 0, "stress_a": 1, "stress_b": 2, "mask": 3, "maskvx": 4, "maskvy": 5}
+
+
+def step_out_dict(o: FsibeStepOut) -> dict:
+    return dict(dt1=o.dt1, u_max=o.u_max, Force=tuple(o.Force), Force_unst=tuple(o.Force_unst),
+                Force_press=tuple(o.Force_press), energies=tuple(o.energies), bsi=tuple(o.bsi),
+                fx_int=o.fx_int, fy_int=o.fy_int)
+
+
+def mouvement(p: Params, time: float, iMouvement: int | None = None, phase: float = 0.0):
+    """(alpha, alpha_t, alpha_tt, LeadingEdge[6]) -- mouvement.f90:5-241."""
+    a, at, att = C.c_double(), C.c_double(), C.c_double()
+    le = np.zeros(6)
+    pc = p.to_c()
+    im = p.iMotion if iMouvement is None else iMouvement
+    check(lib().fsibe_mouvement(C.byref(pc), im, p.x0, p.y0, p.AngleBeam, phase, float(time),
+                                C.byref(a), C.byref(at), C.byref(att), le))
+    return a.value, at.value, att.value, le
+
+
+def integrate_position(p: Params, time: float, theta, theta_dot):
+    """Beam positions and velocities from theta (integrate_position.f90)."""
+    ns = p.ns
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    thd = np.ascontiguousarray(theta_dot, dtype=np.float64)
+    x, y, vx, vy = (np.zeros(ns) for _ in range(4))
+    pc = p.to_c()
+    check(lib().fsibe_integrate_position(C.byref(pc), p.iMotion, float(time), th, thd, x, y, vx, vy))
+    return x, y, vx, vy
+
+
+class FluidSim:
+    def __init__(self, params: Params):
+        self.p = params
+        self.nx, self.ny, self.ns = params.nx, params.ny, params.ns
+        self._pc = params.to_c()
+        h = C.c_void_p()
+        check(lib().fsibe_sim_create(C.byref(self._pc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fsibe_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state -----------------------------------------------------------------
+    def set_vortk(self, vortk_packed):
+        a = np.ascontiguousarray(vortk_packed, dtype=np.float64)
+        assert a.shape == (self.ny, self.nx)
+        check(lib().fsibe_set_vortk(self.h, a))
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.empty((self.ny, self.nx), dtype=np.float64)
+        check(lib().fsibe_get_field(self.h, FIELDS[name], out))
+        return out
+
+    def box(self, name: str):
+        """(array, (x0, x1, y0, y1)) of a field kept on the beam's box."""
+        b = (C.c_int * 4)()
+        check(lib().fsibe_get_box(self.h, BOX_FIELDS[name], None, b))
+        x0, x1, y0, y1 = b
+        if x1 < x0 or y1 < y0:
+            return np.zeros((0, 0)), (x0, x1, y0, y1)
+        out = np.empty((y1 - y0 + 1, x1 - x0 + 1), dtype=np.float64)
+        check(lib().fsibe_get_box(self.h, BOX_FIELDS[name], out.ctypes.data, b))
+        return out, (x0, x1, y0, y1)
+
+    # ---- the reference's per-step calls ----------------------------------------
+    def create_mask(self, time, x, y, vx, vy, theta, alpha, LeadingEdge):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().fsibe_create_mask(self.h, float(time), f(x), f(y), f(vx), f(vy), f(theta),
+                                      float(alpha), f(LeadingEdge)))
+
+    def evolve_fluid_explicit(self, time) -> float:
+        d = C.c_double()
+        check(lib().fsibe_evolve_explicit(self.h, float(time), C.byref(d)))
+        return d.value
+
+    def evolve_fluid_ab2(self, time, dt0) -> float:
+        d = C.c_double()
+        check(lib().fsibe_evolve_ab2(self.h, float(time), float(dt0), C.byref(d)))
+        return d.value
+
+    def get_forces(self, time):
+        pb, tb, fp = np.zeros(max(self.ns, 1)), np.zeros(max(self.ns, 1)), np.zeros(2)
+        check(lib().fsibe_get_forces(self.h, float(time), pb, tb, fp))
+        return pb[: self.ns], tb[: self.ns], fp
+
+    def step_out(self) -> dict:
+        o = FsibeStepOut()
+        check(lib().fsibe_get_step_out(self.h, C.byref(o)))
+        return step_out_dict(o)
+
+    def swap_levels(self):
+        check(lib().fsibe_swap_levels(self.h))
+
+    # ---- the time loop --------------------------------------------------------
+    def run(self, nsteps: int) -> dict:
+        o = FsibeStepOut()
+        check(lib().fsibe_run(self.h, int(nsteps), C.byref(o)))
+        return step_out_dict(o)
+
+    def time_state(self):
+        t, d, it = C.c_double(), C.c_double(), C.c_int()
+        check(lib().fsibe_get_time(self.h, C.byref(t), C.byref(d), C.byref(it)))
+        return t.value, d.value, it.value
+
+    def beam(self) -> dict:
+        n = max(self.ns, 1)
+        arrs = [np.zeros(n) for _ in range(7)]
+        check(lib().fsibe_get_beam(self.h, *arrs))
+        keys = ("x", "y", "vx", "vy", "theta", "pressure", "tau")
+        return {k: a[: self.ns] for k, a in zip(keys, arrs)}
+
+    @property
+    def stream(self):
+        return lib().fsibe_sim_stream(self.h)

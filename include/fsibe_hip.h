@@ -1,0 +1,219 @@
+/*
+ * fsibe_hip.h -- C ABI of libfsibe_hip.so, the MI355X (gfx950) implementation
+ * of the FSIBE_2D fluid time step.
+ *
+ * Conventions shared by every entry point
+ *  - Fields are IEEE binary64 arrays of nx*ny values in the reference's order
+ *    real(8) f(0:nx-1,0:ny-1), x fastest (reference NavierStokes.f90:19-28).
+ *  - Spectral fields in the public API use the reference's packed real layout
+ *    produced by coftxy (cof_fftw33.f90:142-193): four reals per (kx,ky) block,
+ *    both Nyquist modes dropped.
+ *  - Every call returns 0 on success or an FSIBE_E* code; fsibe_last_error()
+ *    gives the message of the last failure on the calling thread.  The
+ *    reference instead prints and STOPs (e.g. Interpolation.f90:19-24); the
+ *    Fortran binding in INTEGRATION.md turns a nonzero code into write+stop.
+ *  - Entry points without a _dev_ infix take host pointers, copy in, compute on
+ *    the GPU and copy out before returning.  _dev_ entry points take device
+ *    pointers and are asynchronous on the handle's stream.
+ *  - A handle must not be used from two threads at once (the reference is not
+ *    reentrant either: module globals and SAVE variables).
+ */
+#ifndef FSIBE_HIP_H
+#define FSIBE_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FSIBE_OK = 0,
+  FSIBE_EARG = 1,         /* null pointer or invalid argument */
+  FSIBE_ESIZE = 2,        /* grid size not supported (nx, ny even, n/2 and n of the form 2^a 3^b, b<=1) */
+  FSIBE_EHIP = 3,         /* HIP runtime error */
+  FSIBE_EUNSUPPORTED = 4, /* parameter combination outside the implemented path */
+  FSIBE_EDOMAIN = 5,      /* interpolation target outside the field (Interpolation.f90:19-24) */
+  FSIBE_ESTATE = 6        /* call sequence error */
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *fsibe_last_error(void);
+
+/* ========================================================================= */
+/* Tier A -- the FFT / spectral-operator plugin.                              */
+/* The reference selects its FFT backend at link time (makefile:40,75); every */
+/* backend exports fft_initialize, fft_free, coftxy, cofitxy with grid sizes  */
+/* taken from module share_vars.  This tier is the same plugin for a host    */
+/* that binds it through iso_c_binding.                      */
+/* ========================================================================= */
+typedef struct fsibe_spectral fsibe_spectral;
+
+/* replaces fft_initialize (cof_fftw33.f90:18-66); xl, yl, nu are the
+ * share_vars values the operators read (cofdx.f90:17, cal_vis.f90:20) */
+int fsibe_spectral_create(int nx, int ny, double xl, double yl, double nu,
+                          fsibe_spectral **out);
+/* replaces fft_free (cof_fftw33.f90:119-137) */
+int fsibe_spectral_destroy(fsibe_spectral *ctx);
+
+/* replaces coftxy(f, fk) (cof_fftw33.f90:142-193) */
+int fsibe_coftxy(fsibe_spectral *ctx, const double *f, double *fk);
+/* replaces cofitxy(fk, f) (cof_fftw33.f90:197-243) */
+int fsibe_cofitxy(fsibe_spectral *ctx, const double *fk, double *f);
+/* replace cofdx (cofdx.f90:1-29), cofdy (cofdy.f90:1-30), cofdxdx
+ * (cofdxdx.f90:1-27), cofdydy (cofdydy.f90:1-27), cofdxdy (cofdxdy.f90:1-40) */
+int fsibe_cofdx(fsibe_spectral *ctx, const double *fk, double *out);
+int fsibe_cofdy(fsibe_spectral *ctx, const double *fk, double *out);
+int fsibe_cofdxdx(fsibe_spectral *ctx, const double *fk, double *out);
+int fsibe_cofdydy(fsibe_spectral *ctx, const double *fk, double *out);
+int fsibe_cofdxdy(fsibe_spectral *ctx, const double *fk, double *out);
+/* replaces poisson(f, ans) (poisson.f90:1-30) */
+int fsibe_poisson(fsibe_spectral *ctx, const double *f, double *ans);
+/* replaces vis(dt, work) (cal_vis.f90:1-36) */
+int fsibe_vis(fsibe_spectral *ctx, double dt, double *work);
+/* replaces dealiase_mask (dealiase_mask.f90:1-33); writes the 0/1 mask */
+int fsibe_dealiase_mask(fsibe_spectral *ctx, double *dealiase);
+
+/* Device-pointer twins of the above, asynchronous on the context's stream
+ * (stream = NULL) or on the given hipStream_t. */
+int fsibe_dev_coftxy(fsibe_spectral *ctx, const double *d_f, double *d_fk, void *stream);
+int fsibe_dev_cofitxy(fsibe_spectral *ctx, const double *d_fk, double *d_f, void *stream);
+/* which: 0 cofdx, 1 cofdy, 2 cofdxdx, 3 cofdydy, 4 cofdxdy, 5 poisson */
+int fsibe_dev_spectral_op(fsibe_spectral *ctx, int which, const double *d_in, double *d_out,
+                          void *stream);
+int fsibe_dev_vis(fsibe_spectral *ctx, double dt, double *d_work, void *stream);
+int fsibe_spectral_sync(fsibe_spectral *ctx);
+
+/* ========================================================================= */
+/* Tier B -- device-resident fluid step (the performance path).               */
+/* Mirrors the reference's per-step calls (time_step.f90:187,193,199):       */
+/*   create_mask(time, beams)                       create_mask.f90:2-218    */
+/*   EvolveFluidAB2 / EvolveFluidExplicit           NavierStokes.f90:422-508 */
+/*   GetForces(time, beams, press, u, "new")        BeamForces.f90:6-51      */
+/* State (vortk, nlk, u, masks, vor_mean) lives in HBM between calls.         */
+/* ========================================================================= */
+
+/* Run parameters: the share_vars values the fluid step reads
+ * (share_vars.f90:8-25, params.f90:9-81).  U_mean_true as set by
+ * time_step.f90:96-100.  heave_* and motion_coef parameterise two motion
+ * cases the reference does not have (600: y = heave_amp sin(2 pi heave_freq t);
+ * 601: Fourier heave + pitch, see fsibe_mouvement). */
+typedef struct fsibe_params {
+  int32_t nx, ny, ns;
+  int32_t iMeanVelocity, iSponge, iBeam, iWalls, iCylinder, iSharpTrailing;
+  int32_t iFLUSI, iMotion, iViscous, iSaveBeam, idealis;
+  double xl, yl, nu, eps, eps_sponge, cfl, dt_fixed, Time_end;
+  double N_smooth, t_beam, length, SpongeSize, R_cylinder, h_channel;
+  double x0, y0, AngleBeam, theta_inf, T_fullspeed, T_release, tau;
+  double U_mean_true;
+  double heave_amp, heave_freq;
+  double motion_coef[8];
+} fsibe_params;
+
+/* Per-step results copied back to the host (type Solid fields filled by
+ * cal_drag NavierStokes.f90:567-589 and GetForcesInterp BeamForces.f90:254-270,
+ * plus the diagnostics the reference appends to files every step). */
+typedef struct fsibe_step_out {
+  double dt1, u_max;
+  double Force[2], Force_unst[2], Force_press[2];
+  double energies[6]; /* vor_rms kinetic dissipation porous_diss inflow_solid inflow_mean (NavierStokes.f90:236-256) */
+  double bsi[10];     /* drag lift drag_surf lift_surf Ft1 Fn1 Fp1 Ft2 Fn2 Fp2 (BeamForces.f90:485-496) */
+  double fx_int, fy_int; /* BeamForces.f90:279 */
+  int32_t status, pad;
+} fsibe_step_out;
+
+typedef struct fsibe_sim fsibe_sim;
+
+/* allocate a simulation (StartSimulation, dns.f90:183-201: dealiasing mask,
+ * FFT plans; create_sponge_mask is run by the first fsibe_create_mask) */
+int fsibe_sim_create(const fsibe_params *p, fsibe_sim **out);
+int fsibe_sim_destroy(fsibe_sim *sim);
+
+/* initial vorticity, packed spectral layout, written into time level n0
+ * (init_fields.f90:38-290 decides the values; the host supplies them) */
+int fsibe_set_vortk(fsibe_sim *sim, const double *vortk_packed);
+
+/* create_mask(time, beams): upload the beam state (arrays of length ns, the
+ * type Solid members x, y, vx, vy, theta) together with the leading-edge
+ * kinematics mouvement(time) returns (alpha, LeadingEdge(1:6)); the masks are
+ * rebuilt on the GPU.  The same beam state and kinematics are used by the
+ * next fsibe_evolve_* (cal_drag) and fsibe_get_forces calls. */
+int fsibe_create_mask(fsibe_sim *sim, double time, const double *x, const double *y,
+                      const double *vx, const double *vy, const double *theta, double alpha,
+                      const double LeadingEdge[6]);
+
+/* EvolveFluidExplicit(time, dt1, n0, n1, ...) -- NavierStokes.f90:422-459.
+ * Returns dt1 chosen by the CFL logic of cal_nlk (NavierStokes.f90:59-78). */
+int fsibe_evolve_explicit(fsibe_sim *sim, double time, double *dt1);
+/* EvolveFluidAB2(time, dt0, dt1, n0, n1, ...) -- NavierStokes.f90:463-508 */
+int fsibe_evolve_ab2(fsibe_sim *sim, double time, double dt0, double *dt1);
+/* GetForces(time, beams, press, u, "new") -- BeamForces.f90:6-288 with the
+ * hard-wired SPL=3 Delta interpolation.  pressure_beam, tau_beam: ns values. */
+int fsibe_get_forces(fsibe_sim *sim, double time, double *pressure_beam, double *tau_beam,
+                     double Force_press[2]);
+/* results of the last evolve / get_forces (cal_drag forces, energies, ...) */
+int fsibe_get_step_out(fsibe_sim *sim, fsibe_step_out *out);
+/* swap time levels n0 <-> n1 (time_step.f90:152,222) */
+int fsibe_swap_levels(fsibe_sim *sim);
+
+/* Field download.  which: see FSIBE_FIELD_*.  Spectral fields come back in the
+ * packed layout; physical fields as nx*ny reals. */
+enum {
+  FSIBE_FIELD_VORTK = 0, /* vortk(:,:,n0) packed */
+  FSIBE_FIELD_NLK = 1,   /* nlk(:,:,n0) packed */
+  FSIBE_FIELD_U1 = 3, FSIBE_FIELD_U2 = 4,
+  FSIBE_FIELD_PRESS = 5, /* static pressure (computed on request over the whole grid) */
+  FSIBE_FIELD_MASK = 6, FSIBE_FIELD_MASKVX = 7, FSIBE_FIELD_MASKVY = 8,
+  FSIBE_FIELD_MASK_SPONGE = 9, FSIBE_FIELD_VOR_MEAN = 10,
+  FSIBE_FIELD_VORT = 14 /* vorticity of vortk(:,:,n0) in physical space */
+};
+int fsibe_get_field(fsibe_sim *sim, int which, double *out);
+
+/* Fields kept only on the rectangle of grid points the step needs (the
+ * reference keeps them on the whole grid): which 0 static pressure, 1
+ * stress_a, 2 stress_b (the Delta-interpolation stencils of GetForcesInterp,
+ * BeamForces.f90:152-203), 3 mask, 4 maskvx, 5 maskvy (the support of the
+ * beam's mask; zero elsewhere).  box receives x0, x1, y0, y1 (inclusive);
+ * out (may be NULL to query the box) receives (x1-x0+1)*(y1-y0+1) values,
+ * x fastest. */
+int fsibe_get_box(fsibe_sim *sim, int which, double *out, int box[4]);
+
+/* ---- host-side kinematics (mouvement.f90:5-241, integrate_position.f90:1-47,
+ * init_beam.f90:48-169) for hosts that do not bring their own ------------- */
+int fsibe_mouvement(const fsibe_params *p, int iMouvement, double x0, double y0,
+                    double AngleBeam, double phase, double time, double *alpha,
+                    double *alpha_t, double *alpha_tt, double LeadingEdge[6]);
+int fsibe_integrate_position(const fsibe_params *p, int iMouvement, double time,
+                             const double *theta, const double *theta_dot, double *x, double *y,
+                             double *vx, double *vy);
+
+/* ---- the time loop of time_step.f90:120-341 for prescribed kinematics ------
+ * (iFLUSI=0, iMotion>0 or a fixed beam): the startup Euler step if not yet
+ * taken, then AB2 steps, each create_mask -> evolve -> GetForces ->
+ * integrate_position -> swap -> time += dt1.  Runs nsteps steps or until
+ * Time_end.  out (may be NULL) receives the last step's results. */
+int fsibe_run(fsibe_sim *sim, int nsteps, fsibe_step_out *out);
+/* current time, dt1 and number of AB2 steps taken by fsibe_run */
+int fsibe_get_time(fsibe_sim *sim, double *time, double *dt1, int *it);
+/* beam state held by fsibe_run (x, y, vx, vy, theta, pressure_new, tau_new) */
+int fsibe_get_beam(fsibe_sim *sim, double *x, double *y, double *vx, double *vy, double *theta,
+                   double *pressure_new, double *tau_new);
+/* HIP stream of the simulation (hipStream_t) for event timing */
+void *fsibe_sim_stream(fsibe_sim *sim);
+
+/* Per-kernel timing with HIP events recorded on the simulation's stream
+ * around each launch (no reference counterpart: the reference times whole
+ * routines with system_clock, PerformanceMeasurement.f90:26-121).
+ * fsibe_profile_enable resets the counters; fsibe_profile_read fills up to n
+ * totals (milliseconds) and launch counts and returns the number of kernel
+ * classes: 0 create_mask, 1 startup velocity, 2 columns of vortk(n0),
+ * 3 rows: nonlinear term, 4 dt/AB2 scalars, 5 columns: AB2 update,
+ * 6 columns of vortk(n1), 7 rows: velocity/pressure rhs, 8 reductions,
+ * 9 columns: pressure, 10 rows: pressure/stress box, 11 Delta interpolation,
+ * 12 beam forces. */
+int fsibe_profile_enable(fsibe_sim *sim, int on);
+int fsibe_profile_read(fsibe_sim *sim, double *total_ms, long long *count, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
