@@ -30,9 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["create_mask", "startup_velocity", "cols_vortk_n0", "rows_nonlinear", "step_scalars",
+# kernel classes of fsibe_profile_read (4 and 12 are no longer launched: the
+# dt/AB2 scalars run inside rows_nonlinear, the reductions inside
+# cols_pressure, the force sums inside the Delta interpolation kernel)
+KERNELS = ["create_mask", "startup_velocity", "cols_vortk_n0", "rows_nonlinear", "unused4",
            "cols_ab2_update", "cols_vortk_n1", "rows_velocity_pressure", "reductions",
-           "cols_pressure", "rows_pressure_box", "delta_interp", "beam_forces"]
+           "cols_pressure", "rows_pressure_box", "delta_interp_forces", "unused12"]
 
 
 def alg_bytes(name: str, nx: int, ny: int, visc: bool, box_rows: int, box_cols: int) -> float:
@@ -105,7 +108,20 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 20 device-to-device copies of MIB MiB (PMC traffic calibration)")
     args = ap.parse_args(argv)
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 20}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)

@@ -1,15 +1,17 @@
 // fft_device.hpp -- workgroup-level FP64 complex FFTs staged in LDS (gfx950).
 //
 // A workgroup transforms `batch` independent sequences of length L held in
-// LDS.  Element e of sequence b lives at buf[b*bs + e*es].  Each pass is one
-// in-place Stockham radix-R step: every thread loads its butterflies into
-// registers, the workgroup synchronises, and every thread writes the results
-// back to the autosorted positions, so a single LDS buffer suffices and the
-// result is in natural order after the last pass.
+// LDS.  Element e of sequence b lives at buf[b*bs + pidx(e)*es]; pidx inserts
+// one empty slot after every 8 elements, which spreads the stride-8 writes of
+// the first radix-8 pass over all 64 LDS banks.  Each pass is one in-place
+// Stockham radix-R step: every thread loads its butterflies into registers,
+// the workgroup synchronises, and every thread writes the results back to the
+// autosorted positions; the result is in natural order after the last pass.
 //
 // Radices 8, 4, 2 and 3 are supported; the host picks the pass sequence
-// (FftPlan).  Twiddles come from a global table tw[m] = exp(-2 pi i m / Ltab)
-// read with stride Ltab/L (L1/L2-resident: at most 64 KiB per length).
+// (FftPlan).  Twiddles: w = exp(-2 pi i k/(Ns R)) is read once per butterfly
+// from the table tw[m] = exp(-2 pi i m / Ltab) (stride Ltab/L) and its powers
+// w^2 .. w^(R-1) are formed by complex multiplication.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,13 +23,17 @@ struct FftPlan {
   int r[16];    // radices, pass order
 };
 
+// padded position of element e and padded length of a sequence of L elements
+__host__ __device__ __forceinline__ constexpr int pidx(int e) { return e + (e >> 3); }
+__host__ __device__ __forceinline__ constexpr int padlen(int L) { return L > 0 ? pidx(L - 1) + 1 : 0; }
+
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+#pragma clang fp contract(fast)
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 // multiply by -i (forward) or +i (inverse)
 template <bool INV>
 __device__ __forceinline__ double2 mul_mi(double2 a) {
@@ -55,6 +61,7 @@ template <bool INV> struct Dft<4, INV> {
 
 template <bool INV> struct Dft<8, INV> {
   __device__ __forceinline__ static void run(double2 *v) {
+#pragma clang fp contract(fast)
     const double h = 0.70710678118654752440;  // sqrt(1/2)
     double2 a[4] = {v[0], v[2], v[4], v[6]};
     double2 b[4] = {v[1], v[3], v[5], v[7]};
@@ -75,6 +82,7 @@ template <bool INV> struct Dft<8, INV> {
 
 template <bool INV> struct Dft<3, INV> {
   __device__ __forceinline__ static void run(double2 *v) {
+#pragma clang fp contract(fast)
     const double s3 = 0.86602540378443864676;  // sin(2 pi / 3)
     double2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
     double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
@@ -106,15 +114,18 @@ __device__ __forceinline__ void stockham_pass(double2 *__restrict__ buf, int L, 
       if (BFAST) { b = w % batch; j = w / batch; } else { j = w % m; b = w / m; }
       bb[it] = b; jj[it] = j;
       const int k = j % Ns;
-      double2 *p = buf + b * bs + j * es;
+      double2 *p = buf + b * bs;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = p[r * m * es];
+      for (int r = 0; r < R; ++r) v[it][r] = p[pidx(j + r * m) * es];
       if (Ns > 1) {
+        double2 w1 = tw[k * tstep];
+        if (INV) w1.y = -w1.y;
+        double2 wr = w1;
+        v[it][1] = cmul(v[it][1], wr);
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          double2 t = tw[r * k * tstep];
-          if (INV) t.y = -t.y;
-          v[it][r] = cmul(v[it][r], t);
+        for (int r = 2; r < R; ++r) {
+          wr = cmul(wr, w1);
+          v[it][r] = cmul(v[it][r], wr);
         }
       }
       Dft<R, INV>::run(v[it]);
@@ -128,9 +139,9 @@ __device__ __forceinline__ void stockham_pass(double2 *__restrict__ buf, int L, 
       const int j = jj[it];
       const int k = j % Ns;
       const int base = (j / Ns) * Ns * R + k;
-      double2 *p = buf + bb[it] * bs + base * es;
+      double2 *p = buf + bb[it] * bs;
 #pragma unroll
-      for (int r = 0; r < R; ++r) p[r * Ns * es] = v[it][r];
+      for (int r = 0; r < R; ++r) p[pidx(base + r * Ns) * es] = v[it][r];
     }
   }
   __syncthreads();
@@ -149,11 +160,140 @@ __device__ void lds_fft(double2 *__restrict__ buf, const FftPlan &plan, int batc
       case 8: stockham_pass<8, (EMAX + 7) / 8, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
       case 4: stockham_pass<4, (EMAX + 3) / 4, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
       case 2: stockham_pass<2, (EMAX + 1) / 2, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
-      case 3: stockham_pass<3, (EMAX + 2) / 3 + 1, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
+      case 3: stockham_pass<3, (EMAX + 2) / 3, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
       default: break;
     }
     Ns *= R;
   }
 }
+
+// ---- compile-time plans ------------------------------------------------------------
+// The same pass sequence as the host's runtime planner: factors 3 first, then
+// (4,4) or 2 or 4 for the power of two left after the eights, then 8s.
+__host__ __device__ constexpr int cp_n3(int L) {
+  int c = 0;
+  while (L > 0 && L % 3 == 0) { L /= 3; ++c; }
+  return c;
+}
+__host__ __device__ constexpr int cp_e2(int L) {
+  while (L > 0 && L % 3 == 0) L /= 3;
+  int c = 0;
+  while (L > 0 && L % 2 == 0) { L /= 2; ++c; }
+  return c;
+}
+__host__ __device__ constexpr int cp_np(int L) {
+  const int n3 = cp_n3(L), e2 = cp_e2(L), a = e2 / 3, b = e2 % 3;
+  return n3 + a + ((b == 1 && a >= 1) ? 1 : (b != 0 ? 1 : 0));
+}
+__host__ __device__ constexpr int cp_radix(int L, int p) {
+  const int n3 = cp_n3(L), e2 = cp_e2(L), a = e2 / 3, b = e2 % 3;
+  if (p < n3) return 3;
+  p -= n3;
+  if (b == 1 && a >= 1) return p < 2 ? 4 : 8;
+  if (b == 1) return p < 1 ? 2 : 8;
+  if (b == 2) return p < 1 ? 4 : 8;
+  return 8;
+}
+
+// In-place Stockham FFT with every size, stride and trip count known at
+// compile time (index arithmetic folds to shifts and constant multiplies).
+// Element e of sequence b at buf[b*BS + pidx(e)*ES]; NT threads take part.
+template <int L, bool INV, bool BFAST, int BATCH, int BS, int ES, int NT, int TWM, int P = 0, int NS = 1>
+__device__ __forceinline__ void fft_ct(double2 *__restrict__ buf, const double2 *__restrict__ tw, int tid) {
+  if constexpr (P < cp_np(L)) {
+    constexpr int R = cp_radix(L, P);
+    constexpr int M = L / R;
+    constexpr int TOTAL = M * BATCH;
+    constexpr int ITEMS = (TOTAL + NT - 1) / NT;
+    constexpr int TSTEP = (L / (NS * R)) * TWM;
+    double2 v[ITEMS][R];
+    int bq[ITEMS], jq[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int w = tid + it * NT;
+      if (TOTAL % NT == 0 || w < TOTAL) {
+        int b, j;
+        if constexpr (BFAST) { b = w % BATCH; j = w / BATCH; } else { j = w % M; b = w / M; }
+        bq[it] = b; jq[it] = j;
+        double2 *p = buf + b * BS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[it][r] = p[pidx(j + r * M) * ES];
+        if constexpr (NS > 1) {
+          double2 w1 = tw[(j % NS) * TSTEP];
+          if (INV) w1.y = -w1.y;
+          double2 wr = w1;
+          v[it][1] = cmul(v[it][1], wr);
+#pragma unroll
+          for (int r = 2; r < R; ++r) {
+            wr = cmul(wr, w1);
+            v[it][r] = cmul(v[it][r], wr);
+          }
+        }
+        Dft<R, INV>::run(v[it]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int w = tid + it * NT;
+      if (TOTAL % NT == 0 || w < TOTAL) {
+        const int j = jq[it];
+        const int base = (j / NS) * NS * R + (j % NS);
+        double2 *p = buf + bq[it] * BS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) p[pidx(base + r * NS) * ES] = v[it][r];
+      }
+    }
+    __syncthreads();
+    fft_ct<L, INV, BFAST, BATCH, BS, ES, NT, TWM, P + 1, NS * R>(buf, tw, tid);
+  }
+}
+
+// ---- transform policies: the kernels are written once against these -------------
+// Row transforms (length H, sequences of padlen(H) back to back, twiddle table
+// of length 2H).  Runtime plan:
+template <int EMAX> struct RowRT {
+  static constexpr int LB = EMAX <= 8 ? 1024 : 256;
+  FftPlan plan;
+  __device__ __forceinline__ int H() const { return plan.L; }
+  template <bool INV, int NSEQ>
+  __device__ __forceinline__ void fft(double2 *lds, const double2 *tw, int tid, int nt) const {
+    lds_fft<EMAX, INV, false>(lds, plan, NSEQ, padlen(plan.L), 1, tw, 2, tid, nt);
+  }
+};
+// compile-time plan, launched with NTC threads
+template <int HC, int NTC> struct RowCT {
+  static constexpr int LB = NTC;
+  FftPlan plan;
+  __device__ __forceinline__ int H() const { return HC; }
+  template <bool INV, int NSEQ>
+  __device__ __forceinline__ void fft(double2 *lds, const double2 *tw, int tid, int) const {
+    fft_ct<HC, INV, false, NSEQ, padlen(HC), 1, NTC, 2>(lds, tw, tid);
+  }
+};
+// Column tiles of TX columns (length NY); a call transforms BM*TX sequences
+// with element stride EM*TX (column c of field f at offset f*TX + c).
+template <int EMAX> struct ColRT {
+  static constexpr int LB = EMAX <= 8 ? 1024 : 256;
+  FftPlan plan;
+  int TX;
+  __device__ __forceinline__ int NY() const { return plan.L; }
+  __device__ __forceinline__ int tx() const { return TX; }
+  template <bool INV, int BM, int EM>
+  __device__ __forceinline__ void fft(double2 *lds, const double2 *tw, int tid, int nt) const {
+    lds_fft<EMAX, INV, true>(lds, plan, BM * TX, 1, EM * TX, tw, 1, tid, nt);
+  }
+};
+template <int NYC, int TXC, int NTC> struct ColCT {
+  static constexpr int LB = NTC;
+  FftPlan plan;
+  int TX;
+  __device__ __forceinline__ int NY() const { return NYC; }
+  __device__ __forceinline__ int tx() const { return TXC; }
+  template <bool INV, int BM, int EM>
+  __device__ __forceinline__ void fft(double2 *lds, const double2 *tw, int tid, int) const {
+    fft_ct<NYC, INV, true, BM * TXC, 1, EM * TXC, NTC, 1>(lds, tw, tid);
+  }
+};
 
 }  // namespace fsibe

@@ -91,13 +91,26 @@ static std::vector<double2> twiddles(int n) {
   return t;
 }
 
-static int emax_of(long long elems) {
-  long long e = (elems + kNT - 1) / kNT;
-  if (e <= 4) return 4;
-  if (e <= 8) return 8;
-  if (e <= 16) return 16;
-  if (e <= 32) return 32;
-  return -1;
+// Launch shape of a transform kernel holding `elems` complex values in LDS:
+// the workgroup grows to 1024 threads while every thread still holds at least
+// 4 values (more waves per CU to hide LDS and memory latency); EMAX is the
+// per-thread bound rounded up to an instantiated value.
+struct Shape {
+  int nt, emax;
+};
+
+static Shape shape_of(long long elems) {
+  // one radix-8 butterfly per thread: nt ~ elems / 8, between 256 and 1024
+  int nt = kNT;
+  while (nt < 1024 && elems >= 16LL * nt) nt *= 2;
+  long long e = (elems + nt - 1) / nt;
+  int em = e <= 4 ? 4 : e <= 8 ? 8 : e <= 16 ? 16 : e <= 32 ? 32 : -1;
+  if (em > 8 && nt > kNT) {  // EMAX 16/32 instances are bounded to 256 threads
+    nt = kNT;
+    e = (elems + nt - 1) / nt;
+    em = e <= 16 ? 16 : e <= 32 ? 32 : -1;
+  }
+  return Shape{nt, em};
 }
 
 static int lds_limit() {
@@ -124,15 +137,65 @@ static int prep_lds(K kernel, size_t lds) {
   return FSIBE_OK;
 }
 
-#define LAUNCH_E(E, KT, grid, lds, st, ...)                                                    \
-  do {                                                                                        \
-    const int e__ = (E);                                                                      \
-    if (e__ == 4) { CHK(prep_lds(KT<4>, lds)); KT<4><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
-    else if (e__ == 8) { CHK(prep_lds(KT<8>, lds)); KT<8><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
-    else if (e__ == 16) { CHK(prep_lds(KT<16>, lds)); KT<16><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
-    else if (e__ == 32) { CHK(prep_lds(KT<32>, lds)); KT<32><<<grid, kNT, lds, st>>>(__VA_ARGS__); } \
-    else return set_err(FSIBE_ESIZE, "transform tile too large for one workgroup");         \
-    HIPCHK(hipGetLastError());                                                                \
+// ---- policy dispatch --------------------------------------------------------
+// Sizes with a compile-time plan: the transform's index arithmetic folds to
+// constants.  Other sizes (any 2^a 3^b) run the runtime-plan kernels.
+constexpr int ct_nt(int butterflies, int cap) {
+  int nt = 64;
+  while (nt < butterflies && nt < cap) nt *= 2;
+  return nt;
+}
+constexpr int ct_tx(int ny) { return (ny == 1024 || ny == 2048) ? 2 : 1; }
+
+// fn(policy, nthreads) for row transforms of length H holding NSEQ sequences
+template <int NSEQ, class Fn>
+static int with_rows(const FftPlan &px, Fn &&fn) {
+  switch (px.L) {
+#define ROWCASE(HH) \
+    case HH: return fn(RowCT<HH, ct_nt(HH * NSEQ / 8, 256)>{px}, ct_nt(HH * NSEQ / 8, 256));
+    ROWCASE(64) ROWCASE(128) ROWCASE(192) ROWCASE(256) ROWCASE(384) ROWCASE(512) ROWCASE(1024) ROWCASE(2048)
+#undef ROWCASE
+    default: break;
+  }
+  const Shape sh = shape_of((long long)NSEQ * px.L);
+  switch (sh.emax) {
+    case 4: return fn(RowRT<4>{px}, sh.nt);
+    case 8: return fn(RowRT<8>{px}, sh.nt);
+    case 16: return fn(RowRT<16>{px}, sh.nt);
+    case 32: return fn(RowRT<32>{px}, sh.nt);
+    default: return set_err(FSIBE_ESIZE, "row transform of %d x %d too large", NSEQ, px.L);
+  }
+}
+
+// fn(policy, nthreads, TX) for column tiles of length NY, BM fields per tile
+template <int BM, class Fn>
+static int with_cols(const FftPlan &py, int H, int rt_tx, Fn &&fn) {
+  switch (py.L) {
+#define COLCASE(NN)                                                                              \
+    case NN:                                                                                     \
+      if (H % ct_tx(NN) == 0)                                                                    \
+        return fn(ColCT<NN, ct_tx(NN), ct_nt(NN * ct_tx(NN) * BM / 8, 1024)>{py, ct_tx(NN)},     \
+                  ct_nt(NN * ct_tx(NN) * BM / 8, 1024), ct_tx(NN));                              \
+      break;
+    COLCASE(128) COLCASE(256) COLCASE(384) COLCASE(512) COLCASE(768) COLCASE(1024) COLCASE(2048) COLCASE(4096)
+#undef COLCASE
+    default: break;
+  }
+  const Shape sh = shape_of((long long)BM * rt_tx * py.L);
+  switch (sh.emax) {
+    case 4: return fn(ColRT<4>{py, rt_tx}, sh.nt, rt_tx);
+    case 8: return fn(ColRT<8>{py, rt_tx}, sh.nt, rt_tx);
+    case 16: return fn(ColRT<16>{py, rt_tx}, sh.nt, rt_tx);
+    case 32: return fn(ColRT<32>{py, rt_tx}, sh.nt, rt_tx);
+    default: return set_err(FSIBE_ESIZE, "column tile of %d x %d too large", BM * rt_tx, py.L);
+  }
+}
+
+#define KLAUNCH(KT, grid, nt, lds, st, ...)                         \
+  do {                                                              \
+    CHK(prep_lds(KT, lds));                                         \
+    KT<<<grid, nt, lds, st>>>(__VA_ARGS__);                         \
+    HIPCHK(hipGetLastError());                                      \
   } while (0)
 
 static int env_int(const char *name, int dflt) {
@@ -148,9 +211,9 @@ static int pick_tx(int H, int ny, int nfld, int split) {
   for (int tx = 1; tx <= 16; tx *= 2) {
     if (H % tx) break;
     if ((long long)tx * nfld * ny > 32LL * kNT) break;
-    if ((size_t)tx * nfld * ny * sizeof(double2) > (size_t)lds_limit()) break;
+    if ((size_t)tx * nfld * padlen(ny) * sizeof(double2) > (size_t)lds_limit()) break;
     if ((long long)(H / tx) * split < 256 && tx > 1) break;
-    if (tx > 1 && (size_t)tx * nfld * ny * sizeof(double2) > 65536) break;
+    if (tx > 1 && (size_t)tx * nfld * padlen(ny) * sizeof(double2) > 73728) break;
     best = tx;
   }
   return best;
@@ -220,44 +283,41 @@ extern "C" int fsibe_spectral_destroy(fsibe_spectral *c) {
   return FSIBE_OK;
 }
 
-// real rows -> half spectrum (unnormalised), into tmp
+// real rows -> half spectrum (unnormalised)
 static int rows_r2c(int nx, int ny, const FftPlan &px, const double2 *twx, const double *f, double2 *out,
                     hipStream_t st) {
-  const int H = nx / 2;
-  const int RB = std::max(1, std::min(8, 2048 / H));
-  const int E = emax_of((long long)RB * H);
-  const size_t lds = sizeof(double2) * RB * H;
-  const int grid = (ny + RB - 1) / RB;
-  LAUNCH_E(E, k_rows_r2c, grid, lds, st, f, out, px, twx, ny, RB);
-  return FSIBE_OK;
+  const size_t lds = sizeof(double2) * padlen(nx / 2);
+  return with_rows<1>(px, [&](auto F, int nt) -> int {
+    KLAUNCH(k_rows_r2c<decltype(F)>, ny, nt, lds, st, f, out, F, twx);
+    return FSIBE_OK;
+  });
 }
 
 static int rows_c2r(int nx, int ny, const FftPlan &px, const double2 *twx, const double2 *in, double *f,
                     hipStream_t st) {
-  const int H = nx / 2;
-  const int RB = std::max(1, std::min(8, 2048 / H));
-  const int E = emax_of((long long)RB * H);
-  const size_t lds = sizeof(double2) * RB * H;
-  const int grid = (ny + RB - 1) / RB;
-  LAUNCH_E(E, k_rows_c2r, grid, lds, st, in, f, px, twx, ny, RB);
-  return FSIBE_OK;
+  const size_t lds = sizeof(double2) * padlen(nx / 2);
+  return with_rows<1>(px, [&](auto F, int nt) -> int {
+    KLAUNCH(k_rows_c2r<decltype(F)>, ny, nt, lds, st, in, f, F, twx);
+    return FSIBE_OK;
+  });
 }
 
 static int dev_coftxy(fsibe_spectral *c, const double *f, double *fk, hipStream_t st) {
   CHK(rows_r2c(c->nx, c->ny, c->px, c->twx, f, c->tmp, st));
-  const int TX = pick_tx(c->H, c->ny, 1, 1);
-  const int E = emax_of((long long)TX * c->ny);
-  const size_t lds = sizeof(double2) * TX * c->ny;
   const double norm = 1.0 / ((double)c->nx * (double)c->ny);
-  LAUNCH_E(E, k_cols_fwd_pack, c->H / TX, lds, st, c->tmp, fk, c->py, c->twy, c->H, TX, norm);
-  return FSIBE_OK;
+  return with_cols<1>(c->py, c->H, pick_tx(c->H, c->ny, 1, 1), [&](auto F, int nt, int TX) -> int {
+    const size_t lds = sizeof(double2) * TX * padlen(c->ny);
+    KLAUNCH(k_cols_fwd_pack<decltype(F)>, c->H / TX, nt, lds, st, c->tmp, fk, F, c->twy, c->H, norm);
+    return FSIBE_OK;
+  });
 }
 
 static int dev_cofitxy(fsibe_spectral *c, const double *fk, double *f, hipStream_t st) {
-  const int TX = pick_tx(c->H, c->ny, 1, 1);
-  const int E = emax_of((long long)TX * c->ny);
-  const size_t lds = sizeof(double2) * TX * c->ny;
-  LAUNCH_E(E, k_cols_unpack_inv, c->H / TX, lds, st, fk, c->tmp, c->py, c->twy, c->H, TX);
+  CHK(with_cols<1>(c->py, c->H, pick_tx(c->H, c->ny, 1, 1), [&](auto F, int nt, int TX) -> int {
+    const size_t lds = sizeof(double2) * TX * padlen(c->ny);
+    KLAUNCH(k_cols_unpack_inv<decltype(F)>, c->H / TX, nt, lds, st, fk, c->tmp, F, c->twy, c->H);
+    return FSIBE_OK;
+  }));
   CHK(rows_c2r(c->nx, c->ny, c->px, c->twx, c->tmp, f, st));
   return FSIBE_OK;
 }
@@ -440,24 +500,30 @@ extern "C" int fsibe_mouvement(const fsibe_params *p, int im, double x0, double 
   return FSIBE_OK;
 }
 
+// integrate_position.f90:1-47.  The reference recomputes each node's prefix
+// sums from i = 0; carrying them from node s to s+1 performs the same
+// additions in the same order, so the result is identical at O(ns) cost.
 static void integrate_position_host(const fsibe_params &p, int im, double time, const double *theta,
                                     const double *theta_dot, double *x, double *y, double *vx, double *vy) {
   double alpha, at, att, LE[6];
   mouvement_host(p, im, p.x0, p.y0, p.AngleBeam, 0.0, time, alpha, at, att, LE);
   const int ns = p.ns;
   const double ds = p.length / (double)(ns - 1);
+  const double ca = cos(alpha), sa = sin(alpha);
+  double xrel = 0.0, yrel = 0.0, vxrel = 0.0, vyrel = 0.0;
   for (int s = 0; s < ns; ++s) {
-    double xrel = 0.0, yrel = 0.0, vxrel = 0.0, vyrel = 0.0;
-    for (int i = 0; i <= s - 1; ++i) {
-      xrel = xrel + cos(theta[i]) * ds;
-      yrel = yrel + sin(theta[i]) * ds;
-      vxrel = vxrel - theta_dot[i] * sin(theta[i]) * ds;
-      vyrel = vyrel + theta_dot[i] * cos(theta[i]) * ds;
+    if (s > 0) {
+      const int i = s - 1;
+      const double c = cos(theta[i]), sn = sin(theta[i]);
+      xrel = xrel + c * ds;
+      yrel = yrel + sn * ds;
+      vxrel = vxrel - theta_dot[i] * sn * ds;
+      vyrel = vyrel + theta_dot[i] * c * ds;
     }
-    x[s] = LE[0] + xrel * cos(alpha) - yrel * sin(alpha);
-    y[s] = LE[1] + yrel * cos(alpha) + xrel * sin(alpha);
-    vx[s] = LE[2] + cos(alpha) * (vxrel - at * yrel) - sin(alpha) * (vyrel + at * xrel);
-    vy[s] = LE[3] + cos(alpha) * (vyrel + at * xrel) + sin(alpha) * (vxrel - at * yrel);
+    x[s] = LE[0] + xrel * ca - yrel * sa;
+    y[s] = LE[1] + yrel * ca + xrel * sa;
+    vx[s] = LE[2] + ca * (vxrel - at * yrel) - sa * (vyrel + at * xrel);
+    vy[s] = LE[3] + ca * (vyrel + at * xrel) + sa * (vxrel - at * yrel);
   }
 }
 
@@ -518,6 +584,7 @@ struct fsibe_sim {
   double *sums, *umax_part;
   DevScalars *dscal;
   StepOutDev *dout;
+  unsigned int *d_done;  // block counter of k_forces (zero between launches)
   double *mask_m, *mask_vx, *mask_vy;
   double *fb_p, *fb_sa, *fb_sb;
   double *d_pbeam, *d_tbeam;
@@ -643,8 +710,8 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   (void)hipStreamSynchronize(s->stream);
   void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
                   s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow, s->sums,
-                  s->umax_part, s->dscal, s->dout, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
-                  s->fb_sb, s->d_pbeam, s->d_tbeam, s->d_stage};
+                  s->umax_part, s->dscal, s->dout, s->d_done, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
+                  s->fb_sb, s->d_stage};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -673,6 +740,8 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     return set_err(FSIBE_EUNSUPPORTED, "iMeanVelocity=11 (zero mean pressure forcing) is not implemented");
   if (p.iBeam > 1) return set_err(FSIBE_EUNSUPPORTED, "iBeam=%d: one beam is supported", p.iBeam);
   if (p.iBeam == 1 && p.ns < 3) return set_err(FSIBE_EARG, "ns must be >= 3");
+  if (p.iBeam == 1 && 2 * p.ns + 8 > kMaxTilePrims)
+    return set_err(FSIBE_EUNSUPPORTED, "ns=%d: at most %d beam nodes", p.ns, (kMaxTilePrims - 8) / 2);
   if (!(p.eps > 0.0) || !(p.xl > 0.0) || !(p.yl > 0.0)) return set_err(FSIBE_EARG, "eps, xl, yl must be > 0");
   if ((p.iSponge > 0 && p.iSponge < 4) && !(p.eps_sponge > 0.0))
     return set_err(FSIBE_EARG, "vorticity sponge needs eps_sponge > 0 (params.f90:185-188)");
@@ -751,8 +820,14 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipMemset(s->umax_part, 0, sizeof(double) * s->ny));
   SIMCHK(hipMalloc(&s->dscal, sizeof(DevScalars)));
   SIMCHK(hipMemset(s->dscal, 0, sizeof(DevScalars)));
-  SIMCHK(hipMalloc(&s->dout, sizeof(StepOutDev)));
-  SIMCHK(hipMemset(s->dout, 0, sizeof(StepOutDev)));
+  SIMCHK(hipMalloc(&s->d_done, sizeof(unsigned int)));
+  SIMCHK(hipMemset(s->d_done, 0, sizeof(unsigned int)));
+  const int nsb0 = std::max(s->ns, 1);
+  const size_t res_bytes = sizeof(StepOutDev) + 2 * sizeof(double) * nsb0;
+  SIMCHK(hipMalloc(&s->dout, res_bytes));
+  SIMCHK(hipMemset(s->dout, 0, res_bytes));
+  s->d_pbeam = reinterpret_cast<double *>(reinterpret_cast<char *>(s->dout) + sizeof(StepOutDev));
+  s->d_tbeam = s->d_pbeam + nsb0;
   SIMCHK(hipMalloc(&s->mask_m, sizeof(double) * N));
   SIMCHK(hipMalloc(&s->mask_vx, sizeof(double) * N));
   SIMCHK(hipMalloc(&s->mask_vy, sizeof(double) * N));
@@ -760,10 +835,6 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipMalloc(&s->fb_sa, sizeof(double) * N));
   SIMCHK(hipMalloc(&s->fb_sb, sizeof(double) * N));
   const int nsb = std::max(s->ns, 1);
-  SIMCHK(hipMalloc(&s->d_pbeam, sizeof(double) * nsb));
-  SIMCHK(hipMalloc(&s->d_tbeam, sizeof(double) * nsb));
-  SIMCHK(hipMemset(s->d_pbeam, 0, sizeof(double) * nsb));
-  SIMCHK(hipMemset(s->d_tbeam, 0, sizeof(double) * nsb));
   // staging layout
   {
     size_t off = 0;
@@ -772,8 +843,8 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     const int ntg_max = 6 * nsb + 6 * kNC;
     s->off_prim = al(sizeof(Prim) * nprim_max);
     s->off_tg = al(sizeof(double) * 3 * ntg_max);
-    s->off_nd = al(sizeof(double) * 16 * nsb);
-    s->off_cd = al(sizeof(double) * 8 * 2 * kNC);
+    s->off_nd = al(sizeof(double) * 4 * nsb);
+    s->off_cd = al(sizeof(double) * 4 * 2 * kNC);
     s->off_val = al(sizeof(double) * ntg_max);
     s->stage_bytes = off;
   }
@@ -904,6 +975,7 @@ static int build_geometry(fsibe_sim *s, double time) {
       P.a[0] = X[i]; P.a[1] = Y[i];
       P.a[2] = c1 * 180.0 / s->pi; P.a[3] = c2 * 180.0 / s->pi;
       P.a[4] = ts2; P.a[5] = VX[i]; P.a[6] = VY[i];
+      P.a[8] = cos(a1); P.a[9] = sin(a1); P.a[10] = cos(a2); P.a[11] = sin(a2);
       add_prim(s, P, np);
       mb.add(P.ix0, P.ix1, P.iy0, P.iy1);
     }
@@ -970,7 +1042,7 @@ static int build_geometry(fsibe_sim *s, double time) {
     for (int n = 0; n < ns; ++n) {
       const double th = s->btheta[n] + s->alpha;
       const double st = sin(th), ct = cos(th);
-      nd[16 * n] = st; nd[16 * n + 1] = ct; nd[16 * n + 2] = sin(2.0 * th); nd[16 * n + 3] = cos(2.0 * th);
+      nd[4 * n] = st; nd[4 * n + 1] = ct; nd[4 * n + 2] = sin(2.0 * th); nd[4 * n + 3] = cos(2.0 * th);
       const double xu = X[n] - rr * st, yu = Y[n] + rr * ct, xb = X[n] + rr * st, yb = Y[n] - rr * ct;
       const double xu1 = X[n] - r1 * st, yu1 = Y[n] + r1 * ct, xb1 = X[n] + r1 * st, yb1 = Y[n] - r1 * ct;
       target(xu1, yu1, 0);
@@ -987,7 +1059,7 @@ static int build_geometry(fsibe_sim *s, double time) {
       for (int i = 0; i < kNC; ++i) {
         const double phi = -(0.5 * s->pi - beta) + (double)i * dphi;
         const double xc = X[ns - 1] + rr * cos(phi), yc = Y[ns - 1] + rr * sin(phi);
-        cd[8 * i] = cos(phi); cd[8 * i + 1] = sin(phi); cd[8 * i + 2] = cos(2.0 * phi); cd[8 * i + 3] = sin(2.0 * phi);
+        cd[4 * i] = cos(phi); cd[4 * i + 1] = sin(phi); cd[4 * i + 2] = cos(2.0 * phi); cd[4 * i + 3] = sin(2.0 * phi);
         target(xc, yc, 1);
         target(xc, yc, 2);
         target(xc, yc, 0);
@@ -997,7 +1069,7 @@ static int build_geometry(fsibe_sim *s, double time) {
         const double phi = (0.5 * s->pi + beta) + (double)i * dphi;
         const double xc = X[0] + rr * cos(phi), yc = Y[0] + rr * sin(phi);
         const int q = kNC + i;
-        cd[8 * q] = cos(phi); cd[8 * q + 1] = sin(phi); cd[8 * q + 2] = cos(2.0 * phi); cd[8 * q + 3] = sin(2.0 * phi);
+        cd[4 * q] = cos(phi); cd[4 * q + 1] = sin(phi); cd[4 * q + 2] = cos(2.0 * phi); cd[4 * q + 3] = sin(2.0 * phi);
         target(xc, yc, 1);
         target(xc, yc, 2);
         target(xc, yc, 0);
@@ -1076,7 +1148,7 @@ extern "C" int fsibe_create_mask(fsibe_sim *s, double time, const double *x, con
   s->time_mask = time;
   s->have_beam = true;
   CHK(build_geometry(s, time));
-  HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->stage_bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipEventRecord(s->ev_upload, s->stream));
   s->upload_pending = true;
   if (!s->mbox.empty()) {
@@ -1084,8 +1156,8 @@ extern "C" int fsibe_create_mask(fsibe_sim *s, double time, const double *x, con
     mc.dx = s->dx; mc.dy = s->dy; mc.t = s->p.t_beam; mc.h = s->p.N_smooth * std::max(s->dx, s->dy);
     mc.gerf = s->gerf; mc.inv_eps_unused = 0.0; mc.ds = s->ds; mc.pi = s->pi; mc.sharp = s->sharp;
     mc.eps = s->p.eps;
-    dim3 blk(64, 4);
-    dim3 grd((s->mbox.w() + 63) / 64, (s->mbox.h() + 3) / 4);
+    dim3 blk(kMaskThreads);
+    dim3 grd((s->mbox.w() + kMaskTile - 1) / kMaskTile, (s->mbox.h() + kMaskTile - 1) / kMaskTile);
     prof_begin(s, P_MASK);
     k_mask<<<grd, blk, 0, s->stream>>>(reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim,
                                        s->wallrow ? s->wallrow + 0 : nullptr, mc, s->mbox.x0, s->mbox.y0,
@@ -1099,15 +1171,15 @@ extern "C" int fsibe_create_mask(fsibe_sim *s, double time, const double *x, con
 
 // ---- fluid step ----------------------------------------------------------
 static int cols_inv_fields(fsibe_sim *s, const double2 *W, double2 *const *outs, const int *fsel, int nf) {
-  const int TX = pick_tx(s->H, s->ny, 1, nf);
-  const int E = emax_of((long long)TX * s->ny);
-  const size_t lds = sizeof(double2) * TX * s->ny;
-  const int grid = (s->H / TX) * nf;
   double2 *o[4] = {outs[0], nf > 1 ? outs[1] : nullptr, nf > 2 ? outs[2] : nullptr, nf > 3 ? outs[3] : nullptr};
   int f[4] = {fsel[0], nf > 1 ? fsel[1] : 0, nf > 2 ? fsel[2] : 0, nf > 3 ? fsel[3] : 0};
-  LAUNCH_E(E, k_cols_inv_fields, grid, lds, s->stream, W, o[0], o[1], o[2], o[3], f[0], f[1], f[2], f[3], nf,
-           s->py, s->twy, s->sc, TX);
-  return FSIBE_OK;
+  return with_cols<1>(s->py, s->H, pick_tx(s->H, s->ny, 1, nf), [&](auto F, int nt, int TX) -> int {
+    const size_t lds = sizeof(double2) * TX * padlen(s->ny);
+    const int grid = (s->H / TX) * nf;
+    KLAUNCH(k_cols_inv_fields<decltype(F)>, grid, nt, lds, s->stream, W, o[0], o[1], o[2], o[3], f[0], f[1], f[2],
+            f[3], nf, F, s->twy, s->sc);
+    return FSIBE_OK;
+  });
 }
 
 // u = cal_velocity(time, vortk(n0)) and the CFL maxima (EvolveFluidExplicit :440)
@@ -1115,8 +1187,7 @@ static int velocity_only(fsibe_sim *s, double time) {
   double2 *outs[2] = {s->T[1], s->T[2]};
   const int fsel[2] = {1, 2};
   CHK(cols_inv_fields(s, s->W[s->n0], outs, fsel, 2));
-  const int E = emax_of(2LL * s->H);
-  const size_t lds = sizeof(double2) * 3 * s->H;
+  const size_t lds = sizeof(double2) * 3 * padlen(s->H);
   double mx, my;
   {
     // Mean_ux/Mean_uy at `time` (mean_velocity.f90), host copy of mean_u
@@ -1140,9 +1211,12 @@ static int velocity_only(fsibe_sim *s, double time) {
   }
   const double UMx = s->p.U_mean_true * mx, UMy = s->p.U_mean_true * my;
   MaskBox mb = mask_box(s);
-  LAUNCH_E(E, k_rows_press, s->ny, lds, s->stream, nullptr, s->T[1], s->T[2], s->u1, s->u2, s->vor_mean, mb,
-           s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, UMx, UMy, 0, 0, s->px, s->twx, s->sc);
-  return FSIBE_OK;
+  return with_rows<3>(s->px, [&](auto F, int nt) -> int {
+    KLAUNCH(k_rows_press<decltype(F)>, s->ny, nt, lds, s->stream, nullptr, s->T[1], s->T[2], s->u1, s->u2,
+            s->vor_mean, mb, s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, UMx, UMy, 0, 0, F, s->twx,
+            s->sc);
+    return FSIBE_OK;
+  });
 }
 
 static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
@@ -1164,28 +1238,27 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     prof_end(s, P_COLS_W0);
   }
   MaskBox mb = mask_box(s);
-  // K2
+  // K2 (block 0 also computes the K3 scalars from the previous u_max partials)
   {
-    const int E = emax_of(3LL * s->H);
-    const size_t lds = sizeof(double2) * 3 * s->H;
+    const size_t lds = sizeof(double2) * 3 * padlen(s->H);
     prof_begin(s, P_ROWS_NL);
-    LAUNCH_E(E, k_rows_nl, s->ny, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2, s->sponge_on ? s->ms : nullptr,
-             mb, (int)s->penal_on, s->T[2], s->T[3], s->px, s->twx, s->sc);
+    CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+      KLAUNCH(k_rows_nl<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
+              s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
+              s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step);
+      return FSIBE_OK;
+    }));
     prof_end(s, P_ROWS_NL);
   }
-  // K3
-  prof_begin(s, P_SCALARS);
-  k_step_scalars<<<1, kNT, 0, s->stream>>>(s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step);
-  HIPCHK(hipGetLastError());
-  prof_end(s, P_SCALARS);
   // K4
   {
-    const int TX = pick_tx(s->H, s->ny, 2, 1);
-    const int E = emax_of(2LL * TX * s->ny);
-    const size_t lds = sizeof(double2) * 2 * TX * s->ny;
     prof_begin(s, P_COLS_AB2);
-    LAUNCH_E(E, k_cols_ab2, s->H / TX, lds, s->stream, s->T[2], s->T[3], s->W[n0], s->NL[n1], s->NL[n0], s->W[n1],
-             s->dscal, (int)explicit_step, s->py, s->twy, s->sc, TX);
+    CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+      const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
+      KLAUNCH(k_cols_ab2<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[2], s->T[3], s->W[n0], s->NL[n1],
+              s->NL[n0], s->W[n1], s->dscal, (int)explicit_step, F, s->twy, s->sc);
+      return FSIBE_OK;
+    }));
     prof_end(s, P_COLS_AB2);
   }
   // K5: omega, psi, i ky psi (, -ky^2 psi) of vortk(n1)
@@ -1199,43 +1272,50 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
   }
   // K6
   {
-    const int E = emax_of(3LL * s->H);
-    const size_t lds = sizeof(double2) * 3 * s->H;
+    const size_t lds = sizeof(double2) * 3 * padlen(s->H);
     prof_begin(s, P_ROWS_PRESS);
-    LAUNCH_E(E, k_rows_press, s->ny, lds, s->stream, s->T[0], s->T[1], s->T[2], s->u1, s->u2, s->vor_mean, mb,
-             s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, 0.0, 0.0, 1, (int)s->pressure_on, s->px,
-             s->twx, s->sc);
+    CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+      KLAUNCH(k_rows_press<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->T[2], s->u1, s->u2,
+              s->vor_mean, mb, s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, 0.0, 0.0, 1,
+              (int)s->pressure_on, F, s->twx, s->sc);
+      return FSIBE_OK;
+    }));
     prof_end(s, P_ROWS_PRESS);
   }
-  // K9 reductions
-  prof_begin(s, P_REDUCE);
-  k_reduce_step<<<1, 64, 0, s->stream>>>(s->sums, s->ny, s->dscal, s->dout, s->dx * s->dy, s->p.nu, s->p.eps,
-                                         s->p.U_mean_true, (int)(s->p.iBeam > 0));
-  HIPCHK(hipGetLastError());
-  prof_end(s, P_REDUCE);
+  // K9 reductions: in block 0 of K7 when K7 runs, else on their own
+  ReduceArgs ra;
+  ra.sums = s->sums; ra.ny = s->ny; ra.beam_on = (int)(s->p.iBeam > 0); ra.s = s->dscal; ra.out = s->dout;
+  ra.dxdy = s->dx * s->dy; ra.nu = s->p.nu; ra.eps = s->p.eps; ra.U_mean_true = s->p.U_mean_true;
   // K7 + K8 on the rows/columns the interpolation needs
   s->press_box_valid = false;
   if (s->forces_on && s->forces_ready && !s->fbox.empty()) {
     const Box &fb = s->fbox;
     {
-      const int TX = pick_tx(s->H, s->ny, 2, 1);
-      const int E = emax_of(2LL * TX * s->ny);
-      const size_t lds = sizeof(double2) * 2 * TX * s->ny;
       prof_begin(s, P_COLS_PRESS);
-      LAUNCH_E(E, k_cols_press, s->H / TX, lds, s->stream, s->T[4], s->T[5], s->T[6], fb.y0, fb.y1, s->py,
-               s->twy, s->sc, TX);
+      CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+        const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
+        KLAUNCH(k_cols_press<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[4], s->T[5], s->T[6], fb.y0, fb.y1,
+                F, s->twy, s->sc, ra, 1);
+        return FSIBE_OK;
+      }));
       prof_end(s, P_COLS_PRESS);
     }
     {
-      const int nf = s->visc ? 3 : 1;
-      const int E = emax_of((long long)nf * s->H);
-      const size_t lds = sizeof(double2) * 3 * s->H;
+      const size_t lds = sizeof(double2) * 3 * padlen(s->H);
       prof_begin(s, P_ROWS_BOX);
-      LAUNCH_E(E, k_rows_box, fb.h(), lds, s->stream, s->T[6], s->T[1], s->T[2], s->T[3], s->u1, s->u2, s->fb_p,
-               s->fb_sa, s->fb_sb, fb.y0, fb.x0, fb.x1, fb.w(), (int)s->visc, s->px, s->twx, s->sc);
+      CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+        KLAUNCH(k_rows_box<decltype(F)>, fb.h(), nt, lds, s->stream, s->T[6], s->T[1], s->T[2], s->T[3], s->u1,
+                s->u2, s->fb_p, s->fb_sa, s->fb_sb, fb.y0, fb.x0, fb.x1, fb.w(), (int)s->visc, F, s->twx, s->sc);
+        return FSIBE_OK;
+      }));
       prof_end(s, P_ROWS_BOX);
     }
     s->press_box_valid = true;
+  } else {
+    prof_begin(s, P_REDUCE);
+    k_reduce_step<<<1, kNT, 0, s->stream>>>(ra);
+    HIPCHK(hipGetLastError());
+    prof_end(s, P_REDUCE);
   }
   return FSIBE_OK;
 }
@@ -1244,26 +1324,20 @@ static int launch_forces(fsibe_sim *s) {
   if (!s->forces_on || !s->forces_ready || !s->press_box_valid) return FSIBE_OK;
   const double *tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
   double *val = reinterpret_cast<double *>(s->d_stage + s->off_val);
+  if (s->ntg <= 0) return set_err(FSIBE_ESTATE, "no interpolation targets");
   prof_begin(s, P_INTERP);
-  k_interp<<<(s->ntg + 127) / 128, 128, 0, s->stream>>>(tg, s->ntg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val);
+  k_forces<<<(s->ntg + kInterpPerBlock - 1) / kInterpPerBlock, kInterpThreads, 0, s->stream>>>(
+      tg, s->ntg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
+      reinterpret_cast<const double *>(s->d_stage + s->off_cd), s->d_pbeam, s->d_tbeam, s->dout, s->d_done);
   HIPCHK(hipGetLastError());
   prof_end(s, P_INTERP);
-  prof_begin(s, P_FORCES);
-  k_forces_final<<<1, 64, 0, s->stream>>>(reinterpret_cast<const double *>(s->d_stage + s->off_nd),
-                                          reinterpret_cast<const double *>(s->d_stage + s->off_cd), val, s->fg,
-                                          s->d_pbeam, s->d_tbeam, s->dout);
-  HIPCHK(hipGetLastError());
-  prof_end(s, P_FORCES);
   return FSIBE_OK;
 }
 
 static int fetch_results(fsibe_sim *s) {
   const int nsb = std::max(s->ns, 1);
-  HIPCHK(hipMemcpyAsync(s->h_res, s->dout, sizeof(StepOutDev), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipMemcpyAsync(s->h_res + sizeof(StepOutDev), s->d_pbeam, sizeof(double) * nsb, hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(s->h_res, s->dout, sizeof(StepOutDev) + 2 * sizeof(double) * nsb, hipMemcpyDeviceToHost,
                         s->stream));
-  HIPCHK(hipMemcpyAsync(s->h_res + sizeof(StepOutDev) + sizeof(double) * nsb, s->d_tbeam, sizeof(double) * nsb,
-                        hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   s->upload_pending = false;
   prof_collect(s);
@@ -1331,7 +1405,7 @@ static int run_one(fsibe_sim *s) {
   } else if (p.iBeam > 0) {
     CHK(build_geometry(s, s->time));  // soft-startup factor and targets follow time
     if (s->upload_pending) { HIPCHK(hipEventSynchronize(s->ev_upload)); s->upload_pending = false; }
-    HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->stage_bytes, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipEventRecord(s->ev_upload, s->stream));
     s->upload_pending = true;
   }
@@ -1454,15 +1528,18 @@ extern "C" int fsibe_get_field(fsibe_sim *s, int which, double *out) {
     case FSIBE_FIELD_PRESS: {
       if (!s->pressure_on) { memset(out, 0, sizeof(double) * N); break; }
       // full-grid pressure from the last step's right-hand side (T4, T5)
-      const int TX = pick_tx(s->H, s->ny, 2, 1);
-      const int E = emax_of(2LL * TX * s->ny);
-      const size_t lds = sizeof(double2) * 2 * TX * s->ny;
-      LAUNCH_E(E, k_cols_press, s->H / TX, lds, s->stream, s->T[4], s->T[5], s->T[6], 0, s->ny - 1, s->py, s->twy,
-               s->sc, TX);
-      const int E2 = emax_of((long long)s->H);
-      const size_t lds2 = sizeof(double2) * 3 * s->H;
-      LAUNCH_E(E2, k_rows_box, s->ny, lds2, s->stream, s->T[6], s->T[1], s->T[2], s->T[3], s->u1, s->u2, s->fb_sb,
-               s->fb_sa, s->fb_sa, 0, 0, s->nx - 1, s->nx, 0, s->px, s->twx, s->sc);
+      CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+        const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
+        KLAUNCH(k_cols_press<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[4], s->T[5], s->T[6], 0, s->ny - 1,
+                F, s->twy, s->sc, ReduceArgs{}, 0);
+        return FSIBE_OK;
+      }));
+      const size_t lds2 = sizeof(double2) * 3 * padlen(s->H);
+      CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+        KLAUNCH(k_rows_box<decltype(F)>, s->ny, nt, lds2, s->stream, s->T[6], s->T[1], s->T[2], s->T[3], s->u1,
+                s->u2, s->fb_sb, s->fb_sa, s->fb_sa, 0, 0, s->nx - 1, s->nx, 0, F, s->twx, s->sc);
+        return FSIBE_OK;
+      }));
       HIPCHK(hipMemcpyAsync(out, s->fb_sb, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
       s->press_box_valid = false;
       break;

@@ -56,37 +56,41 @@ __device__ __forceinline__ double2 c2r_one(double2 a, double2 b, double2 wconj) 
   return make_double2(e.x - pi, e.y + pr);
 }
 
-// rows: `nrows` sequences of length H at buf[r*H + k]
-__device__ void rows_r2c_post(double2 *buf, int nrows, int H, const double2 *__restrict__ twx,
+// Launch bounds: up to 1024 threads when at most 8 elements per thread are
+// held in registers during a pass, 256 threads otherwise.
+#define FSIBE_LB(EMAX) __launch_bounds__((EMAX) <= 8 ? 1024 : 256)
+
+// rows: `nrows` sequences of length H, sequence r at buf + r*PL (PL = padlen(H))
+__device__ void rows_r2c_post(double2 *buf, int nrows, int H, int PL, const double2 *__restrict__ twx,
                               int tid, int nthr) {
   const int hp = (H >> 1) + 1;
   for (int e = tid; e < nrows * hp; e += nthr) {
     const int r = e / hp, k = e - r * hp;
-    double2 *Z = buf + r * H;
+    double2 *Z = buf + r * PL;
     const int q = (k == 0) ? 0 : H - k;
-    const double2 a = Z[k], b = Z[q];
+    const double2 a = Z[pidx(k)], b = Z[pidx(q)];
     const double2 xk = r2c_one(a, b, twx[k]);
-    if (q != k) Z[q] = r2c_one(b, a, twx[q]);
-    Z[k] = xk;
+    if (q != k) Z[pidx(q)] = r2c_one(b, a, twx[q]);
+    Z[pidx(k)] = xk;
   }
   __syncthreads();
 }
 
-__device__ void rows_c2r_pre(double2 *buf, int nrows, int H, const double2 *__restrict__ twx,
+__device__ void rows_c2r_pre(double2 *buf, int nrows, int H, int PL, const double2 *__restrict__ twx,
                              int tid, int nthr) {
   const int hp = (H >> 1) + 1;
   for (int e = tid; e < nrows * hp; e += nthr) {
     const int r = e / hp, k = e - r * hp;
-    double2 *X = buf + r * H;
+    double2 *X = buf + r * PL;
     if (k == 0) {
       const double a = X[0].x;  // x-DC: real part only, x-Nyquist is zero
       X[0] = make_double2(a, a);
     } else {
       const int q = H - k;
-      const double2 a = X[k], b = X[q];
+      const double2 a = X[pidx(k)], b = X[pidx(q)];
       const double2 zk = c2r_one(a, b, cconj(twx[k]));
-      if (q != k) X[q] = c2r_one(b, a, cconj(twx[q]));
-      X[k] = zk;
+      if (q != k) X[pidx(q)] = c2r_one(b, a, cconj(twx[q]));
+      X[pidx(k)] = zk;
     }
   }
   __syncthreads();
@@ -95,59 +99,50 @@ __device__ void rows_c2r_pre(double2 *buf, int nrows, int H, const double2 *__re
 // ============================================================================
 // Tier A kernels (packed layout at the boundary)
 // ============================================================================
-// real rows -> half spectra (unnormalised), nrow rows per block
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_rows_r2c(const double *__restrict__ f, double2 *__restrict__ out,
-                                                  FftPlan plan, const double2 *__restrict__ twx, int ny,
-                                                  int RB) {
+// real rows -> half spectra (unnormalised), one row per block
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_r2c(const double *__restrict__ f, double2 *__restrict__ out, FX F,
+                                                     const double2 *__restrict__ twx) {
   extern __shared__ double2 lds[];
-  const int H = plan.L, nx = 2 * H, tid = threadIdx.x;
-  const int row0 = blockIdx.x * RB;
-  const int nr = min(RB, ny - row0);
-  for (int e = tid; e < nr * H; e += kNT) {
-    const int r = e / H, n = e - r * H;
-    lds[e] = reinterpret_cast<const double2 *>(f + (size_t)(row0 + r) * nx)[n];
-  }
+  const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x;
+  const int row = blockIdx.x;
+  for (int n = tid; n < H; n += nt) lds[pidx(n)] = reinterpret_cast<const double2 *>(f + (size_t)row * nx)[n];
   __syncthreads();
-  lds_fft<EMAX, false, false>(lds, plan, nr, H, 1, twx, 2, tid, kNT);
-  rows_r2c_post(lds, nr, H, twx, tid, kNT);
-  for (int e = tid; e < nr * H; e += kNT) out[(size_t)row0 * H + e] = lds[e];
+  F.template fft<false, 1>(lds, twx, tid, nt);
+  rows_r2c_post(lds, 1, H, padlen(H), twx, tid, nt);
+  for (int n = tid; n < H; n += nt) out[(size_t)row * H + n] = lds[pidx(n)];
 }
 
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_rows_c2r(const double2 *__restrict__ in, double *__restrict__ f,
-                                                  FftPlan plan, const double2 *__restrict__ twx, int ny,
-                                                  int RB) {
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_c2r(const double2 *__restrict__ in, double *__restrict__ f, FX F,
+                                                     const double2 *__restrict__ twx) {
   extern __shared__ double2 lds[];
-  const int H = plan.L, nx = 2 * H, tid = threadIdx.x;
-  const int row0 = blockIdx.x * RB;
-  const int nr = min(RB, ny - row0);
-  for (int e = tid; e < nr * H; e += kNT) lds[e] = in[(size_t)row0 * H + e];
+  const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x;
+  const int row = blockIdx.x;
+  for (int n = tid; n < H; n += nt) lds[pidx(n)] = in[(size_t)row * H + n];
   __syncthreads();
-  rows_c2r_pre(lds, nr, H, twx, tid, kNT);
-  lds_fft<EMAX, true, false>(lds, plan, nr, H, 1, twx, 2, tid, kNT);
-  for (int e = tid; e < nr * H; e += kNT) {
-    const int r = e / H, n = e - r * H;
-    reinterpret_cast<double2 *>(f + (size_t)(row0 + r) * nx)[n] = lds[e];
-  }
+  rows_c2r_pre(lds, 1, H, padlen(H), twx, tid, nt);
+  F.template fft<true, 1>(lds, twx, tid, nt);
+  for (int n = tid; n < H; n += nt) reinterpret_cast<double2 *>(f + (size_t)row * nx)[n] = lds[pidx(n)];
 }
 
 // half spectra -> column FFT -> x norm -> packed layout (cof_fftw33.f90:165-189)
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_cols_fwd_pack(const double2 *__restrict__ in, double *__restrict__ fk,
-                                                       FftPlan plan, const double2 *__restrict__ twy, int H,
-                                                       int TX, double norm) {
+// column tile layout: element y of column c at lds[pidx(y)*TX + c]
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_fwd_pack(const double2 *__restrict__ in, double *__restrict__ fk,
+                                                          FY F, const double2 *__restrict__ twy, int H,
+                                                          double norm) {
   extern __shared__ double2 lds[];
-  const int ny = plan.L, nx = 2 * H, tid = threadIdx.x;
+  const int ny = F.NY(), TX = F.tx(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x;
   const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
-  for (int e = tid; e < ny * TX; e += kNT) {
+  for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, c = e - y * TX;
-    lds[e] = in[(size_t)y * H + kx0 + c];
+    lds[pidx(y) * TX + c] = in[(size_t)y * H + kx0 + c];
   }
   __syncthreads();
-  lds_fft<EMAX, false, true>(lds, plan, TX, 1, TX, twy, 1, tid, kNT);
+  F.template fft<false, 1, 1>(lds, twy, tid, nt);
   const int hy = ny >> 1;
-  for (int e = tid; e < hy * TX; e += kNT) {
+  for (int e = tid; e < hy * TX; e += nt) {
     const int ky = e / TX, c = e - ky * TX, kx = kx0 + c;
     double2 r0, r1;
     if (ky == 0) {
@@ -155,7 +150,7 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_pack(const double2 *__restrict
       r0 = make_double2(v.x * norm, v.y * norm);
       r1 = make_double2(0.0, 0.0);
     } else {
-      const double2 p = lds[ky * TX + c], m = lds[(ny - ky) * TX + c];
+      const double2 p = lds[pidx(ky) * TX + c], m = lds[pidx(ny - ky) * TX + c];
       r0 = make_double2(0.5 * (p.x + m.x) * norm, 0.5 * (p.y + m.y) * norm);
       r1 = make_double2(0.5 * (p.y - m.y) * norm, 0.5 * (-p.x + m.x) * norm);
     }
@@ -165,31 +160,30 @@ __global__ void __launch_bounds__(kNT) k_cols_fwd_pack(const double2 *__restrict
 }
 
 // packed layout -> half spectra (cof_fftw33.f90:211-231) -> inverse column FFT
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_cols_unpack_inv(const double *__restrict__ fk, double2 *__restrict__ out,
-                                                         FftPlan plan, const double2 *__restrict__ twy, int H,
-                                                         int TX) {
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_unpack_inv(const double *__restrict__ fk, double2 *__restrict__ out,
+                                                            FY F, const double2 *__restrict__ twy, int H) {
   extern __shared__ double2 lds[];
-  const int ny = plan.L, nx = 2 * H, tid = threadIdx.x;
+  const int ny = F.NY(), TX = F.tx(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x;
   const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
   const int hy = ny >> 1;
-  for (int e = tid; e < hy * TX; e += kNT) {
+  for (int e = tid; e < hy * TX; e += nt) {
     const int ky = e / TX, c = e - ky * TX, kx = kx0 + c;
     const double2 ab = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky) * nx)[kx];
     const double2 cd = reinterpret_cast<const double2 *>(fk + (size_t)(2 * ky + 1) * nx)[kx];
     if (ky == 0) {
       lds[c] = ab;
-      lds[hy * TX + c] = make_double2(0.0, 0.0);
+      lds[pidx(hy) * TX + c] = make_double2(0.0, 0.0);
     } else {
-      lds[ky * TX + c] = make_double2(ab.x - cd.y, cd.x + ab.y);
-      lds[(ny - ky) * TX + c] = make_double2(ab.x + cd.y, -cd.x + ab.y);
+      lds[pidx(ky) * TX + c] = make_double2(ab.x - cd.y, cd.x + ab.y);
+      lds[pidx(ny - ky) * TX + c] = make_double2(ab.x + cd.y, -cd.x + ab.y);
     }
   }
   __syncthreads();
-  lds_fft<EMAX, true, true>(lds, plan, TX, 1, TX, twy, 1, tid, kNT);
-  for (int e = tid; e < ny * TX; e += kNT) {
+  F.template fft<true, 1, 1>(lds, twy, tid, nt);
+  for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, c = e - y * TX;
-    out[(size_t)y * H + kx0 + c] = lds[e];
+    out[(size_t)y * H + kx0 + c] = lds[pidx(y) * TX + c];
   }
 }
 

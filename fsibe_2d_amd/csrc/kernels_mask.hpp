@@ -44,18 +44,53 @@ __device__ __forceinline__ double angle_deg(double g, double pi) {
   return a * 180.0 / pi;
 }
 
-__global__ void k_mask(const Prim *__restrict__ prims, int np, const double *__restrict__ wallrow, MaskConsts c,
-                       int bx0, int by0, int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
-                       double *__restrict__ ovy) {
+// One workgroup per 16x16 tile of the mask box.  The primitives whose boxes
+// meet the tile are first compacted, in drawing order, into an LDS list
+// (ballot + prefix count per chunk of 256), so each pixel visits only those.
+constexpr int kMaskTile = 8;
+constexpr int kMaskThreads = kMaskTile * kMaskTile;
+constexpr int kMaxTilePrims = 1024;  // >= number of primitives (host checks)
+
+__global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ prims, int np,
+                                              const double *__restrict__ wallrow, MaskConsts c, int bx0, int by0,
+                                              int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
+                                              double *__restrict__ ovy) {
 #pragma clang fp contract(off)
-  const int lx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ly = blockIdx.y * blockDim.y + threadIdx.y;
+  __shared__ int list[kMaxTilePrims];
+  __shared__ int wcount[kMaskThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tx0 = bx0 + blockIdx.x * kMaskTile, ty0 = by0 + blockIdx.y * kMaskTile;
+  const int tx1 = tx0 + kMaskTile - 1, ty1 = ty0 + kMaskTile - 1;
+  int nlist = 0;
+  for (int c0 = 0; c0 < np; c0 += kMaskThreads) {
+    const int p = c0 + tid;
+    bool f = false;
+    if (p < np) {
+      const Prim &P = prims[p];
+      f = !(P.ix1 < tx0 || P.ix0 > tx1 || P.iy1 < ty0 || P.iy0 > ty1);
+    }
+    const unsigned long long b = __ballot(f);
+    const int pre = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[wid] = __popcll(b);
+    __syncthreads();
+    int off = nlist;
+    for (int w = 0; w < wid; ++w) off += wcount[w];
+    if (f && off + pre < kMaxTilePrims) list[off + pre] = p;
+    int tot = 0;
+    for (int w = 0; w < kMaskThreads / 64; ++w) tot += wcount[w];
+    __syncthreads();
+    nlist += tot;
+  }
+  nlist = nlist < kMaxTilePrims ? nlist : kMaxTilePrims;
+  const int lx = blockIdx.x * kMaskTile + (tid & (kMaskTile - 1));
+  const int ly = blockIdx.y * kMaskTile + (tid / kMaskTile);
   if (lx >= bw || ly >= bh) return;
   const int i = bx0 + lx, j = by0 + ly;
   double m = wallrow ? wallrow[j] : 0.0, mvx = 0.0, mvy = 0.0;
   const double xi = (double)i * c.dx, yj = (double)j * c.dy;
-  for (int p = 0; p < np; ++p) {
-    const Prim &P = prims[p];
+  for (int q = 0; q < nlist; ++q) {
+    // every lane of the wave reads the same entry: keep it in a scalar register
+    const Prim &P = prims[__builtin_amdgcn_readfirstlane(list[q])];
     if (i < P.ix0 || i > P.ix1 || j < P.iy0 || j > P.iy1) continue;
     const double *a = P.a;
     if (P.type == PRIM_SEG) {
@@ -71,10 +106,16 @@ __global__ void k_mask(const Prim *__restrict__ prims, int np, const double *__r
         }
       }
     } else if (P.type == PRIM_HINGE) {
-      // DrawHinge :595-653; a = px py beta1 beta2 t_star vx vy
+      // DrawHinge :595-653; a = px py beta1 beta2 t_star vx vy d1(2) d2(2)
       const double ddx = xi - a[0], ddy = yj - a[1];
       const double R = sqrt(ddx * ddx + ddy * ddy);
-      if (R <= a[4]) {
+      // The sector test below is "angle to segment 1 <= 90 deg and angle to
+      // segment 2 >= 90 deg", i.e. d1.v >= 0 and d2.v <= 0 with d1, d2 the
+      // unit directions of the two segments.  Pixels that fail it by more
+      // than 1e-9 R (far beyond the rounding of the degree comparisons) are
+      // skipped; the rest take the reference's exact atan2 test.
+      const double dot1 = a[8] * ddx + a[9] * ddy, dot2 = a[10] * ddx + a[11] * ddy;
+      if (R <= a[4] && !(dot1 < -1e-9 * R || dot2 > 1e-9 * R)) {
         const double gamma = angle_deg(atan2(yj - a[1], xi - a[0]), c.pi);
         const double b1 = a[2], b2 = a[3];
         bool in1 = false, in2 = false;

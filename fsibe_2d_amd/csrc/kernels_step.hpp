@@ -11,14 +11,15 @@
 //   K2 rows  A,B -> omega, d_x omega, d_y omega (3 c2r), with u, mask, sponge:
 //            a = -u.grad(omega) - chi_s omega, (px,py) = -chi (u - u_s);
 //            Rc = Fx[a] + i kx Fx[py], Rpx = Fx[px]          (3 r2c)
-//   K3 1 WG  CFL dt1, AB2 weights, mean flow                  (cal_nlk :59-78)
+//   K3       CFL dt1, AB2 weights, mean flow (block 0 of K2)  (cal_nlk :59-78)
 //   K4 cols  nlk = N(Rc) - i ky N(Rpx); vortk(n1) = AB2 update (:478-498)
 //   K5 cols  vortk(n1) -> Fy^-1 of omega, psi, i ky psi, -ky^2 psi (cal_velocity)
 //   K6 rows  omega, u1, u2; vor_mean; energy/drag/penal partial sums;
 //            pressure rhs q = chi(u-u_s) + omega x u -> Fx  (pressure :305-370)
 //   K7 cols  div rhs -> poisson -> Fy^-1 on the rows near the beam
 //   K8 rows  p = q - |u|^2/2 and the stresses on the rows near the beam
-//   K9/10    reductions, Delta interpolation at the beam (GetForcesInterp)
+//   K9       reductions of the K6 row sums (block 0 of K7)
+//   K10      Delta interpolation at the beam and the force sums (GetForcesInterp)
 #pragma once
 #include "kernels.hpp"
 
@@ -125,21 +126,21 @@ __device__ void block_sum(double (&v)[NV], double *red) {
 // derived fields.  Grid: ntiles * nfield blocks, one (tile, field) each.
 // field 0: F, 1: psi = F/|k|^2, 2: i ky psi, 3: -ky^2 psi, 4: i ky F
 // ============================================================================
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_cols_inv_fields(const double2 *__restrict__ W, double2 *__restrict__ o0,
-                                                         double2 *__restrict__ o1, double2 *__restrict__ o2,
-                                                         double2 *__restrict__ o3, int fsel0, int fsel1,
-                                                         int fsel2, int fsel3, int nfield, FftPlan plan,
-                                                         const double2 *__restrict__ twy, StepConsts c, int TX) {
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_inv_fields(const double2 *__restrict__ W, double2 *__restrict__ o0,
+                                                            double2 *__restrict__ o1, double2 *__restrict__ o2,
+                                                            double2 *__restrict__ o3, int fsel0, int fsel1,
+                                                            int fsel2, int fsel3, int nfield, FY F,
+                                                            const double2 *__restrict__ twy, StepConsts c) {
   extern __shared__ double2 lds[];
-  const int ny = plan.L, H = c.H, tid = threadIdx.x;
+  const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = lb / nfield, slot = lb - tile * nfield;
   const int kx0 = tile * TX;
   const int fsel = slot == 0 ? fsel0 : slot == 1 ? fsel1 : slot == 2 ? fsel2 : fsel3;
   double2 *out = slot == 0 ? o0 : slot == 1 ? o1 : slot == 2 ? o2 : o3;
   const int hy = ny >> 1;
-  for (int e = tid; e < ny * TX; e += kNT) {
+  for (int e = tid; e < ny * TX; e += nt) {
     const int idx = e / TX, cc = e - idx * TX, kx = kx0 + cc;
     const double2 w = W[(size_t)idx * H + kx];
     double2 v;
@@ -163,88 +164,25 @@ __global__ void __launch_bounds__(kNT) k_cols_inv_fields(const double2 *__restri
       else if (fsel == 2) v = make_double2(-kyf * psi.y, kyf * psi.x);
       else v = make_double2(-(kyf * kyf) * psi.x, -(kyf * kyf) * psi.y);
     }
-    lds[e] = v;
+    lds[pidx(idx) * TX + cc] = v;
   }
   __syncthreads();
-  lds_fft<EMAX, true, true>(lds, plan, TX, 1, TX, twy, 1, tid, kNT);
-  for (int e = tid; e < ny * TX; e += kNT) {
+  F.template fft<true, 1, 1>(lds, twy, tid, nt);
+  for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
-    out[(size_t)y * H + kx0 + cc] = lds[e];
-  }
-}
-
-// ============================================================================
-// K2: nonlinear term, penalisation and sponge in physical space (one row/WG)
-// cal_nlk, NavierStokes.f90:39-52 and :88-164
-// ============================================================================
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_rows_nl(const double2 *__restrict__ A, const double2 *__restrict__ B,
-                                                 const double *__restrict__ u1, const double *__restrict__ u2,
-                                                 const double *__restrict__ ms, MaskBox mb, int penal_on,
-                                                 double2 *__restrict__ Rc, double2 *__restrict__ Rpx, FftPlan plan,
-                                                 const double2 *__restrict__ twx, StepConsts c) {
-  extern __shared__ double2 lds[];
-  const int H = c.H, nx = c.nx, tid = threadIdx.x;
-  const int y = blockIdx.x;
-  double2 *f0 = lds, *f1 = lds + H, *f2 = lds + 2 * H;
-  const bool prow = penal_on && mb.x1 >= mb.x0 && y >= mb.y0 && y <= mb.y1;
-  for (int k = tid; k < H; k += kNT) {
-    const double2 a = A[(size_t)y * H + k];
-    const double kf = (double)k * c.sx;
-    f0[k] = a;
-    f1[k] = B[(size_t)y * H + k];
-    f2[k] = make_double2(-kf * a.y, kf * a.x);
-  }
-  __syncthreads();
-  rows_c2r_pre(lds, 3, H, twx, tid, kNT);
-  lds_fft<EMAX, true, false>(lds, plan, 3, H, 1, twx, 2, tid, kNT);
-  for (int n = tid; n < H; n += kNT) {
-    const double2 w = f0[n], wy = f1[n], wx = f2[n];
-    const double2 a1 = reinterpret_cast<const double2 *>(u1 + (size_t)y * nx)[n];
-    const double2 a2 = reinterpret_cast<const double2 *>(u2 + (size_t)y * nx)[n];
-    double2 s = make_double2(0.0, 0.0);
-    if (ms) s = reinterpret_cast<const double2 *>(ms + (size_t)y * nx)[n];
-    double2 r;
-    r.x = (-(a1.x * wx.x) - a2.x * wy.x) - s.x * w.x;
-    r.y = (-(a1.y * wx.y) - a2.y * wy.y) - s.y * w.y;
-    double2 px = make_double2(0.0, 0.0), py = make_double2(0.0, 0.0);
-    if (prow) {
-      const int x = 2 * n;
-      if (mb.inside(x, y)) {
-        const size_t q = (size_t)(y - mb.y0) * mb.w + (x - mb.x0);
-        px.x = -(mb.m[q] * (a1.x - mb.vx[q]));
-        py.x = -(mb.m[q] * (a2.x - mb.vy[q]));
-      }
-      if (mb.inside(x + 1, y)) {
-        const size_t q = (size_t)(y - mb.y0) * mb.w + (x + 1 - mb.x0);
-        px.y = -(mb.m[q] * (a1.y - mb.vx[q]));
-        py.y = -(mb.m[q] * (a2.y - mb.vy[q]));
-      }
-    }
-    f0[n] = r;
-    f1[n] = px;
-    f2[n] = py;
-  }
-  __syncthreads();
-  lds_fft<EMAX, false, false>(lds, plan, 3, H, 1, twx, 2, tid, kNT);
-  rows_r2c_post(lds, 3, H, twx, tid, kNT);
-  for (int k = tid; k < H; k += kNT) {
-    const double2 xa = f0[k], xpx = f1[k], xpy = f2[k];
-    const double kf = (double)k * c.sx;
-    Rc[(size_t)y * H + k] = make_double2(xa.x - kf * xpy.y, xa.y + kf * xpy.x);
-    Rpx[(size_t)y * H + k] = xpx;
+    out[(size_t)y * H + kx0 + cc] = lds[pidx(y) * TX + cc];
   }
 }
 
 // ============================================================================
 // K3: time step and AB2 weights (cal_nlk NavierStokes.f90:59-78,
-// EvolveFluidAB2 :488-489, vis cal_vis.f90:20-21), one workgroup
+// EvolveFluidAB2 :488-489, vis cal_vis.f90:20-21); run by block 0 of K2,
+// whose inputs (the previous step's CFL maxima) are complete at launch
 // ============================================================================
-__global__ void __launch_bounds__(kNT) k_step_scalars(const double *__restrict__ umax_part, int nparts,
-                                                      DevScalars *__restrict__ s, FlowParams f, double time,
-                                                      double dt0, int explicit_step) {
+// Block-wide (any multiple of 64 threads); red holds >= blockDim/64 doubles.
+__device__ void step_scalars(const double *__restrict__ umax_part, int nparts, DevScalars *__restrict__ s,
+                             const FlowParams &f, double time, double dt0, int explicit_step, double *red) {
 #pragma clang fp contract(off)
-  __shared__ double red[kNT / 64];
   double m = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmax(m, umax_part[i]);
   m = wave_max(m);
@@ -280,33 +218,104 @@ __global__ void __launch_bounds__(kNT) k_step_scalars(const double *__restrict__
     s->UMx = f.U_mean_true * mx;
     s->UMy = f.U_mean_true * my;
   }
+  __syncthreads();
+}
+
+// ============================================================================
+// K2: nonlinear term, penalisation and sponge in physical space (one row/WG)
+// cal_nlk, NavierStokes.f90:39-52 and :88-164
+// ============================================================================
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                    const double *__restrict__ u1, const double *__restrict__ u2,
+                                                    const double *__restrict__ ms, MaskBox mb, int penal_on,
+                                                    double2 *__restrict__ Rc, double2 *__restrict__ Rpx, FX F,
+                                                    const double2 *__restrict__ twx, StepConsts c,
+                                                    const double *__restrict__ umax_part, int nparts,
+                                                    DevScalars *__restrict__ sc, FlowParams fl, double time,
+                                                    double dt0, int explicit_step) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[16];
+  if (blockIdx.x == 0) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red);
+  const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
+  const int y = blockIdx.x;
+  double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
+  const bool prow = penal_on && mb.x1 >= mb.x0 && y >= mb.y0 && y <= mb.y1;
+  for (int k = tid; k < H; k += nt) {
+    const double2 a = A[(size_t)y * H + k];
+    const double kf = (double)k * c.sx;
+    const int q = pidx(k);
+    f0[q] = a;
+    f1[q] = B[(size_t)y * H + k];
+    f2[q] = make_double2(-kf * a.y, kf * a.x);
+  }
+  __syncthreads();
+  rows_c2r_pre(lds, 3, H, PL, twx, tid, nt);
+  F.template fft<true, 3>(lds, twx, tid, nt);
+  for (int n = tid; n < H; n += nt) {
+    const int q = pidx(n);
+    const double2 w = f0[q], wy = f1[q], wx = f2[q];
+    const double2 a1 = reinterpret_cast<const double2 *>(u1 + (size_t)y * nx)[n];
+    const double2 a2 = reinterpret_cast<const double2 *>(u2 + (size_t)y * nx)[n];
+    double2 s = make_double2(0.0, 0.0);
+    if (ms) s = reinterpret_cast<const double2 *>(ms + (size_t)y * nx)[n];
+    double2 r;
+    r.x = (-(a1.x * wx.x) - a2.x * wy.x) - s.x * w.x;
+    r.y = (-(a1.y * wx.y) - a2.y * wy.y) - s.y * w.y;
+    double2 px = make_double2(0.0, 0.0), py = make_double2(0.0, 0.0);
+    if (prow) {
+      const int x = 2 * n;
+      if (mb.inside(x, y)) {
+        const size_t qq = (size_t)(y - mb.y0) * mb.w + (x - mb.x0);
+        px.x = -(mb.m[qq] * (a1.x - mb.vx[qq]));
+        py.x = -(mb.m[qq] * (a2.x - mb.vy[qq]));
+      }
+      if (mb.inside(x + 1, y)) {
+        const size_t qq = (size_t)(y - mb.y0) * mb.w + (x + 1 - mb.x0);
+        px.y = -(mb.m[qq] * (a1.y - mb.vx[qq]));
+        py.y = -(mb.m[qq] * (a2.y - mb.vy[qq]));
+      }
+    }
+    f0[q] = r;
+    f1[q] = px;
+    f2[q] = py;
+  }
+  __syncthreads();
+  F.template fft<false, 3>(lds, twx, tid, nt);
+  rows_r2c_post(lds, 3, H, PL, twx, tid, nt);
+  for (int k = tid; k < H; k += nt) {
+    const int q = pidx(k);
+    const double2 xa = f0[q], xpx = f1[q], xpy = f2[q];
+    const double kf = (double)k * c.sx;
+    Rc[(size_t)y * H + k] = make_double2(xa.x - kf * xpy.y, xa.y + kf * xpy.x);
+    Rpx[(size_t)y * H + k] = xpx;
+  }
 }
 
 // ============================================================================
 // K4: nlk(n0) and the integrating-factor AB2 / Euler update of vortk
 // (EvolveFluidAB2 NavierStokes.f90:478-498, EvolveFluidExplicit :444-449)
 // ============================================================================
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_cols_ab2(const double2 *__restrict__ Rc, const double2 *__restrict__ Rpx,
-                                                  const double2 *__restrict__ W0, const double2 *__restrict__ NL1,
-                                                  double2 *__restrict__ NL0, double2 *__restrict__ W1,
-                                                  const DevScalars *__restrict__ s, int explicit_step,
-                                                  FftPlan plan, const double2 *__restrict__ twy, StepConsts c,
-                                                  int TX) {
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_ab2(const double2 *__restrict__ Rc, const double2 *__restrict__ Rpx,
+                                                     const double2 *__restrict__ W0, const double2 *__restrict__ NL1,
+                                                     double2 *__restrict__ NL0, double2 *__restrict__ W1,
+                                                     const DevScalars *__restrict__ s, int explicit_step, FY F,
+                                                     const double2 *__restrict__ twy, StepConsts c) {
   extern __shared__ double2 lds[];
-  const int ny = plan.L, H = c.H, tid = threadIdx.x;
+  const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
   const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
   const int TT = 2 * TX;
-  for (int e = tid; e < ny * TX; e += kNT) {
+  for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
-    lds[y * TT + cc] = Rc[(size_t)y * H + kx0 + cc];
-    lds[y * TT + TX + cc] = Rpx[(size_t)y * H + kx0 + cc];
+    lds[pidx(y) * TT + cc] = Rc[(size_t)y * H + kx0 + cc];
+    lds[pidx(y) * TT + TX + cc] = Rpx[(size_t)y * H + kx0 + cc];
   }
   __syncthreads();
-  lds_fft<EMAX, false, true>(lds, plan, TT, 1, TT, twy, 1, tid, kNT);
+  F.template fft<false, 2, 2>(lds, twy, tid, nt);
   const double b10 = s->b10, b11 = s->b11, cx1 = s->cx1, cy1 = s->cy1, cx0 = s->cx0, cy0 = s->cy0;
   const int hy = ny >> 1;
-  for (int e = tid; e < ny * TX; e += kNT) {
+  for (int e = tid; e < ny * TX; e += nt) {
     const int idx = e / TX, cc = e - idx * TX, kx = kx0 + cc;
     const size_t g = (size_t)idx * H + kx;
     double2 nl = make_double2(0.0, 0.0), w1 = make_double2(0.0, 0.0);
@@ -314,7 +323,7 @@ __global__ void __launch_bounds__(kNT) k_cols_ab2(const double2 *__restrict__ Rc
       const int ky = ky_signed(idx, ny);
       const int aky = ky < 0 ? -ky : ky;
       const double kyf = (double)ky * c.sy;
-      const double2 fc = lds[idx * TT + cc], fp = lds[idx * TT + TX + cc];
+      const double2 fc = lds[pidx(idx) * TT + cc], fp = lds[pidx(idx) * TT + TX + cc];
       const double2 Fc = make_double2(fc.x * c.norm, fc.y * c.norm);
       const double2 Fp = make_double2(fp.x * c.norm, fp.y * c.norm);
       nl = make_double2(Fc.x + kyf * Fp.y, Fc.y - kyf * Fp.x);
@@ -352,32 +361,34 @@ __global__ void __launch_bounds__(kNT) k_cols_ab2(const double2 *__restrict__ Rc
 constexpr int kNSUM = 11;
 struct Window { int x0, x1, y0, y1; };
 
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_rows_press(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
                                                     const double2 *__restrict__ Cpsiy, double *__restrict__ u1,
                                                     double *__restrict__ u2, double *__restrict__ vor_mean,
                                                     MaskBox mb, Window win, double *__restrict__ sums,
                                                     double *__restrict__ umax_part, double2 *__restrict__ Rq,
                                                     double2 *__restrict__ Rqy, const DevScalars *__restrict__ s,
-                                                    double UMx0, double UMy0, int mode, int pres, FftPlan plan,
+                                                    double UMx0, double UMy0, int mode, int pres, FX F,
                                                     const double2 *__restrict__ twx, StepConsts c) {
   extern __shared__ double2 lds[];
-  __shared__ double red[kNSUM * (kNT / 64)];
-  const int H = c.H, nx = c.nx, tid = threadIdx.x;
+  __shared__ double red[kNSUM * 16];
+  const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
   const int y = blockIdx.x;
-  double2 *f0 = lds, *f1 = lds + H, *f2 = lds + 2 * H;
+  double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const int nf = mode ? 3 : 2;
   double2 *g1 = mode ? f1 : f0, *g2 = mode ? f2 : f1;  // psi_x, psi_y slots
-  for (int k = tid; k < H; k += kNT) {
+  for (int k = tid; k < H; k += nt) {
     const double kf = (double)k * c.sx;
     const double2 p = Cpsi[(size_t)y * H + k];
-    if (mode) f0[k] = Cw[(size_t)y * H + k];
-    g1[k] = make_double2(-kf * p.y, kf * p.x);
-    g2[k] = Cpsiy[(size_t)y * H + k];
+    const int q = pidx(k);
+    if (mode) f0[q] = Cw[(size_t)y * H + k];
+    g1[q] = make_double2(-kf * p.y, kf * p.x);
+    g2[q] = Cpsiy[(size_t)y * H + k];
   }
   __syncthreads();
-  rows_c2r_pre(lds, nf, H, twx, tid, kNT);
-  lds_fft<EMAX, true, false>(lds, plan, nf, H, 1, twx, 2, tid, kNT);
+  rows_c2r_pre(lds, nf, H, PL, twx, tid, nt);
+  if (mode) F.template fft<true, 3>(lds, twx, tid, nt);
+  else F.template fft<true, 2>(lds, twx, tid, nt);
   double UMx = UMx0, UMy = UMy0, Mx = 0.0, My = 0.0, time = 0.0, dt1 = 0.0;
   if (mode) {
     UMx = s->UMx; UMy = s->UMy; Mx = s->Mx; My = s->My; time = s->time; dt1 = s->dt1;
@@ -388,8 +399,9 @@ __global__ void __launch_bounds__(kNT) k_rows_press(const double2 *__restrict__ 
 #pragma unroll
   for (int i = 0; i < kNSUM; ++i) acc[i] = 0.0;
   double um = 0.0;
-  for (int n = tid; n < H; n += kNT) {
-    const double2 gx = g1[n], uy1 = g2[n];
+  for (int n = tid; n < H; n += nt) {
+    const int qn = pidx(n);
+    const double2 gx = g1[qn], uy1 = g2[qn];
     double2 a1, a2;
     {
 #pragma clang fp contract(off)
@@ -400,7 +412,7 @@ __global__ void __launch_bounds__(kNT) k_rows_press(const double2 *__restrict__ 
     reinterpret_cast<double2 *>(u2 + (size_t)y * nx)[n] = a2;
     um = fmax(um, fmax(a1.x * a1.x + a2.x * a2.x, a1.y * a1.y + a2.y * a2.y));
     if (!mode) continue;
-    const double2 w = f0[n];
+    const double2 w = f0[qn];
     {
       double2 *vm = reinterpret_cast<double2 *>(vor_mean + (size_t)y * nx) + n;
       const double2 v = *vm;
@@ -438,8 +450,8 @@ __global__ void __launch_bounds__(kNT) k_rows_press(const double2 *__restrict__ 
     }
     if (pres) {
       // penal(:,:,1) - vor*uy, penal(:,:,2) + vor*ux   (:357-358)
-      f0[n] = make_double2(q1.x - w.x * a2.x, q1.y - w.y * a2.y);
-      f1[n] = make_double2(q2.x + w.x * a1.x, q2.y + w.y * a1.y);
+      f0[qn] = make_double2(q1.x - w.x * a2.x, q1.y - w.y * a2.y);
+      f1[qn] = make_double2(q2.x + w.x * a1.x, q2.y + w.y * a1.y);
     }
   }
   // u_max partial (next step's CFL)
@@ -450,7 +462,7 @@ __global__ void __launch_bounds__(kNT) k_rows_press(const double2 *__restrict__ 
     __syncthreads();
     if (tid == 0) {
       double m = 0.0;
-      for (int w = 0; w < kNT / 64; ++w) m = fmax(m, red[w]);
+      for (int w = 0; w < nt / 64; ++w) m = fmax(m, red[w]);
       umax_part[y] = m;
     }
     __syncthreads();
@@ -460,114 +472,17 @@ __global__ void __launch_bounds__(kNT) k_rows_press(const double2 *__restrict__ 
   if (tid == 0)
     for (int i = 0; i < kNSUM; ++i) sums[(size_t)y * kNSUM + i] = acc[i];
   if (!pres) return;
-  lds_fft<EMAX, false, false>(lds, plan, 2, H, 1, twx, 2, tid, kNT);
-  rows_r2c_post(lds, 2, H, twx, tid, kNT);
-  for (int k = tid; k < H; k += kNT) {
-    const double2 xq = f0[k];
+  F.template fft<false, 2>(lds, twx, tid, nt);
+  rows_r2c_post(lds, 2, H, PL, twx, tid, nt);
+  for (int k = tid; k < H; k += nt) {
+    const int q = pidx(k);
+    const double2 xq = f0[q];
     const double kf = (double)k * c.sx;
     Rq[(size_t)y * H + k] = make_double2(-kf * xq.y, kf * xq.x);
-    Rqy[(size_t)y * H + k] = f1[k];
+    Rqy[(size_t)y * H + k] = f1[q];
   }
 }
 
-// ============================================================================
-// K7: divergence + poisson (pressure NavierStokes.f90:366-382), then the
-// inverse column transform; rows [r0, r1] are written.
-// ============================================================================
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_cols_press(const double2 *__restrict__ Rq, const double2 *__restrict__ Rqy,
-                                                    double2 *__restrict__ Cq, int r0, int r1, FftPlan plan,
-                                                    const double2 *__restrict__ twy, StepConsts c, int TX) {
-  extern __shared__ double2 lds[];
-  const int ny = plan.L, H = c.H, tid = threadIdx.x;
-  const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
-  const int TT = 2 * TX;
-  for (int e = tid; e < ny * TX; e += kNT) {
-    const int y = e / TX, cc = e - y * TX;
-    lds[y * TT + cc] = Rq[(size_t)y * H + kx0 + cc];
-    lds[y * TT + TX + cc] = Rqy[(size_t)y * H + kx0 + cc];
-  }
-  __syncthreads();
-  lds_fft<EMAX, false, true>(lds, plan, TT, 1, TT, twy, 1, tid, kNT);
-  const int hy = ny >> 1;
-  for (int e = tid; e < ny * TX; e += kNT) {
-    const int idx = e / TX, cc = e - idx * TX, kx = kx0 + cc;
-    const int ky = ky_signed(idx, ny);
-    const int aky = ky < 0 ? -ky : ky;
-    double2 q = make_double2(0.0, 0.0);
-    if (idx != hy && !(kx == 0 && ky == 0)) {
-      const double kyf = (double)ky * c.sy;
-      const double2 a = lds[idx * TT + cc], b = lds[idx * TT + TX + cc];
-      const double2 fa = make_double2(a.x * c.norm, a.y * c.norm);
-      const double2 fb = make_double2(b.x * c.norm, b.y * c.norm);
-      const double2 rhs = make_double2(fa.x - kyf * fb.y, fa.y + kyf * fb.x);
-      const double quot = ((double)(kx * kx) * c.scalex + (double)(aky * aky) * c.scaley);
-      q = make_double2(rhs.x / quot, rhs.y / quot);
-    }
-    // this thread is the only reader and writer of slot (idx, cc)
-    lds[idx * TT + cc] = q;
-  }
-  __syncthreads();
-  lds_fft<EMAX, true, true>(lds, plan, TX, 1, TT, twy, 1, tid, kNT);
-  for (int e = tid; e < ny * TX; e += kNT) {
-    const int y = e / TX, cc = e - y * TX;
-    if (y >= r0 && y <= r1) Cq[(size_t)y * H + kx0 + cc] = lds[y * TT + cc];
-  }
-}
-
-// ============================================================================
-// K8: static pressure p = q - |u|^2/2 (:390) and the viscous stresses
-// stress_a = 2 nu d_x u1, stress_b = nu (d_y u1 + d_x u2) (BeamForces.f90:
-// 105-117) on rows r0+blockIdx.x, written for columns [bx0, bx1] with pitch bw.
-// Spectrally d_x u1 = i kx (i ky psi), d_y u1 + d_x u2 = (kx^2 - ky^2) psi.
-// ============================================================================
-template <int EMAX>
-__global__ void __launch_bounds__(kNT) k_rows_box(const double2 *__restrict__ Cq, const double2 *__restrict__ Cpsi,
-                                                  const double2 *__restrict__ Cpsiy,
-                                                  const double2 *__restrict__ Cpsiyy, const double *__restrict__ u1,
-                                                  const double *__restrict__ u2, double *__restrict__ press,
-                                                  double *__restrict__ sa, double *__restrict__ sb, int r0, int bx0,
-                                                  int bx1, int bw, int visc, FftPlan plan,
-                                                  const double2 *__restrict__ twx, StepConsts c) {
-  extern __shared__ double2 lds[];
-  const int H = c.H, nx = c.nx, tid = threadIdx.x;
-  const int y = r0 + blockIdx.x;
-  double2 *f0 = lds, *f1 = lds + H, *f2 = lds + 2 * H;
-  const int nf = visc ? 3 : 1;
-  for (int k = tid; k < H; k += kNT) {
-    f0[k] = Cq[(size_t)y * H + k];
-    if (visc) {
-      const double kf = (double)k * c.sx;
-      const double2 py = Cpsiy[(size_t)y * H + k];
-      const double2 p = Cpsi[(size_t)y * H + k], pyy = Cpsiyy[(size_t)y * H + k];
-      f1[k] = make_double2(-kf * py.y, kf * py.x);
-      f2[k] = make_double2((kf * kf) * p.x + pyy.x, (kf * kf) * p.y + pyy.y);
-    }
-  }
-  __syncthreads();
-  rows_c2r_pre(lds, nf, H, twx, tid, kNT);
-  lds_fft<EMAX, true, false>(lds, plan, nf, H, 1, twx, 2, tid, kNT);
-  const double *q = reinterpret_cast<const double *>(f0);
-  const double *A = reinterpret_cast<const double *>(f1);
-  const double *Bv = reinterpret_cast<const double *>(f2);
-  const size_t row = (size_t)blockIdx.x * bw;
-  for (int x = bx0 + tid; x <= bx1; x += kNT) {
-#pragma clang fp contract(off)
-    const double v1 = u1[(size_t)y * nx + x], v2 = u2[(size_t)y * nx + x];
-    press[row + (x - bx0)] = q[x] - 0.5 * (v1 * v1 + v2 * v2);
-    if (visc) {
-      sa[row + (x - bx0)] = 2.0 * c.nu * A[x];
-      sb[row + (x - bx0)] = c.nu * Bv[x];
-    }
-  }
-}
-
-// ============================================================================
-// K9: reductions of the per-row partial sums: cal_drag (Force, Force_unst,
-// NavierStokes.f90:567-589), cal_energy (:236-256), BeamSurfaceIntegral
-// drag/lift (BeamForces.f90:483-486).  One workgroup; thread i sums quantity
-// i over rows in order.
-// ============================================================================
 struct StepOutDev {
   double dt1, u_max;
   double Force[2], Force_unst[2], Force_press[2];
@@ -577,19 +492,37 @@ struct StepOutDev {
   int status, pad;
 };
 
-__global__ void k_reduce_step(const double *__restrict__ sums, int ny, DevScalars *__restrict__ s,
-                              StepOutDev *__restrict__ out, double dxdy, double nu, double eps,
-                              double U_mean_true, int beam_on) {
+// ============================================================================
+// K9: reductions of the per-row partial sums: cal_drag (Force, Force_unst,
+// NavierStokes.f90:567-589), cal_energy (:236-256), BeamSurfaceIntegral
+// drag/lift (BeamForces.f90:483-486).  Block-wide tree sums in a fixed order;
+// run by block 0 of K7 (or alone when K7 is not needed).
+// ============================================================================
+struct ReduceArgs {
+  const double *sums;
+  int ny, beam_on;
+  DevScalars *s;
+  StepOutDev *out;
+  double dxdy, nu, eps, U_mean_true;
+};
+
+__device__ void reduce_step(const ReduceArgs &r) {
 #pragma clang fp contract(off)
-  __shared__ double tot[kNSUM];
-  const int i = threadIdx.x;
-  if (i < kNSUM) {
-    double a = 0.0;
-    for (int y = 0; y < ny; ++y) a += sums[(size_t)y * kNSUM + i];
-    tot[i] = a;
-  }
-  __syncthreads();
-  if (i == 0) {
+  __shared__ double red[kNSUM * 16];
+  const double *__restrict__ sums = r.sums;
+  const int ny = r.ny, beam_on = r.beam_on;
+  DevScalars *__restrict__ s = r.s;
+  StepOutDev *__restrict__ out = r.out;
+  const double dxdy = r.dxdy, nu = r.nu, eps = r.eps, U_mean_true = r.U_mean_true;
+  double acc[kNSUM];
+#pragma unroll
+  for (int i = 0; i < kNSUM; ++i) acc[i] = 0.0;
+  for (int y = threadIdx.x; y < ny; y += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < kNSUM; ++i) acc[i] += sums[(size_t)y * kNSUM + i];
+  block_sum<kNSUM>(acc, red);
+  if (threadIdx.x == 0) {
+    const double *tot = acc;
     out->dt1 = s->dt1;
     out->u_max = s->u_max;
     out->energies[0] = dxdy * sqrt(tot[0]);
@@ -614,6 +547,102 @@ __global__ void k_reduce_step(const double *__restrict__ sums, int ny, DevScalar
       s->drag_unst_old = dn;
       s->lift_unst_old = ln;
       s->ready = 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kNT) k_reduce_step(ReduceArgs r) { reduce_step(r); }
+
+// ============================================================================
+// K7: divergence + poisson (pressure NavierStokes.f90:366-382), then the
+// inverse column transform; rows [r0, r1] are written.
+// ============================================================================
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_press(const double2 *__restrict__ Rq, const double2 *__restrict__ Rqy,
+                                                       double2 *__restrict__ Cq, int r0, int r1, FY F,
+                                                       const double2 *__restrict__ twy, StepConsts c,
+                                                       ReduceArgs ra, int do_reduce) {
+  extern __shared__ double2 lds[];
+  if (do_reduce && blockIdx.x == 0) reduce_step(ra);
+  const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
+  const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
+  const int TT = 2 * TX;
+  for (int e = tid; e < ny * TX; e += nt) {
+    const int y = e / TX, cc = e - y * TX;
+    lds[pidx(y) * TT + cc] = Rq[(size_t)y * H + kx0 + cc];
+    lds[pidx(y) * TT + TX + cc] = Rqy[(size_t)y * H + kx0 + cc];
+  }
+  __syncthreads();
+  F.template fft<false, 2, 2>(lds, twy, tid, nt);
+  const int hy = ny >> 1;
+  for (int e = tid; e < ny * TX; e += nt) {
+    const int idx = e / TX, cc = e - idx * TX, kx = kx0 + cc;
+    const int ky = ky_signed(idx, ny);
+    const int aky = ky < 0 ? -ky : ky;
+    double2 q = make_double2(0.0, 0.0);
+    if (idx != hy && !(kx == 0 && ky == 0)) {
+      const double kyf = (double)ky * c.sy;
+      const double2 a = lds[pidx(idx) * TT + cc], b = lds[pidx(idx) * TT + TX + cc];
+      const double2 fa = make_double2(a.x * c.norm, a.y * c.norm);
+      const double2 fb = make_double2(b.x * c.norm, b.y * c.norm);
+      const double2 rhs = make_double2(fa.x - kyf * fb.y, fa.y + kyf * fb.x);
+      const double quot = ((double)(kx * kx) * c.scalex + (double)(aky * aky) * c.scaley);
+      q = make_double2(rhs.x / quot, rhs.y / quot);
+    }
+    // this thread is the only reader and writer of slot (idx, cc)
+    lds[pidx(idx) * TT + cc] = q;
+  }
+  __syncthreads();
+  F.template fft<true, 1, 2>(lds, twy, tid, nt);
+  for (int e = tid; e < ny * TX; e += nt) {
+    const int y = e / TX, cc = e - y * TX;
+    if (y >= r0 && y <= r1) Cq[(size_t)y * H + kx0 + cc] = lds[pidx(y) * TT + cc];
+  }
+}
+
+// ============================================================================
+// K8: static pressure p = q - |u|^2/2 (:390) and the viscous stresses
+// stress_a = 2 nu d_x u1, stress_b = nu (d_y u1 + d_x u2) (BeamForces.f90:
+// 105-117) on rows r0+blockIdx.x, written for columns [bx0, bx1] with pitch bw.
+// Spectrally d_x u1 = i kx (i ky psi), d_y u1 + d_x u2 = (kx^2 - ky^2) psi.
+// ============================================================================
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_box(const double2 *__restrict__ Cq, const double2 *__restrict__ Cpsi,
+                                          const double2 *__restrict__ Cpsiy, const double2 *__restrict__ Cpsiyy,
+                                          const double *__restrict__ u1, const double *__restrict__ u2,
+                                          double *__restrict__ press, double *__restrict__ sa,
+                                          double *__restrict__ sb, int r0, int bx0, int bx1, int bw, int visc,
+                                          FX F, const double2 *__restrict__ twx, StepConsts c) {
+  extern __shared__ double2 lds[];
+  const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
+  const int y = r0 + blockIdx.x;
+  double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
+  const int nf = visc ? 3 : 1;
+  for (int k = tid; k < H; k += nt) {
+    const int q = pidx(k);
+    f0[q] = Cq[(size_t)y * H + k];
+    if (visc) {
+      const double kf = (double)k * c.sx;
+      const double2 py = Cpsiy[(size_t)y * H + k];
+      const double2 p = Cpsi[(size_t)y * H + k], pyy = Cpsiyy[(size_t)y * H + k];
+      f1[q] = make_double2(-kf * py.y, kf * py.x);
+      f2[q] = make_double2((kf * kf) * p.x + pyy.x, (kf * kf) * p.y + pyy.y);
+    }
+  }
+  __syncthreads();
+  rows_c2r_pre(lds, nf, H, PL, twx, tid, nt);
+  if (visc) F.template fft<true, 3>(lds, twx, tid, nt);
+  else F.template fft<true, 1>(lds, twx, tid, nt);
+  const size_t row = (size_t)blockIdx.x * bw;
+  for (int x = bx0 + tid; x <= bx1; x += nt) {
+#pragma clang fp contract(off)
+    const int q = pidx(x >> 1), h = x & 1;
+    const double qv = h ? f0[q].y : f0[q].x;
+    const double v1 = u1[(size_t)y * nx + x], v2 = u2[(size_t)y * nx + x];
+    press[row + (x - bx0)] = qv - 0.5 * (v1 * v1 + v2 * v2);
+    if (visc) {
+      sa[row + (x - bx0)] = 2.0 * c.nu * (h ? f1[q].y : f1[q].x);
+      sb[row + (x - bx0)] = c.nu * (h ? f2[q].y : f2[q].x);
     }
   }
 }
@@ -643,93 +672,144 @@ __device__ __forceinline__ double phi4star(double x, double dx, double pi) {
   return 0.0;
 }
 
-// targets: tg[3*t] = x, tg[3*t+1] = y, tg[3*t+2] = field id (0 press, 1 sa, 2 sb)
-__global__ void k_interp(const double *__restrict__ tg, int nt, const double *__restrict__ press,
-                         const double *__restrict__ sa, const double *__restrict__ sb, ForceGeom g,
-                         double *__restrict__ val) {
+// targets: tg[3*t] = x, tg[3*t+1] = y, tg[3*t+2] = field id (0 press, 1 sa, 2 sb).
+// DeltaInterpolation (Interpolation.f90:26-44).  One block of 64 threads
+// handles 7 targets: thread (t, a) evaluates the kernel weights wx[a], wy[a]
+// and the 9 products of stencil column a; one thread per target then adds
+// the 81 products in the reference's order (ix outer, iy inner).
+constexpr int kInterpThreads = 64;
+constexpr int kInterpPerBlock = 7;
+
+__device__ void interp_targets(const double *__restrict__ tg, int nt, const double *__restrict__ press,
+                               const double *__restrict__ sa, const double *__restrict__ sb, const ForceGeom &g,
+                               double *__restrict__ val) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  const double xt = tg[3 * t], yt = tg[3 * t + 1];
-  const int fid = (int)tg[3 * t + 2];
-  const double *f = fid == 0 ? press : fid == 1 ? sa : sb;
-  const int i = (int)((xt - 0.0) / g.dxi), j = (int)((yt - 0.0) / g.dyi);
-  double R1 = 0.0;
-  for (int ix = i - 4; ix <= i + 4; ++ix) {
-    const double xl = (double)ix * g.dxi + 0.0;
-    const double px = phi4star(fabs(xl - xt), g.dxi, g.pi);
-    for (int iy = j - 4; iy <= j + 4; ++iy) {
-      const double yl = (double)iy * g.dyi + 0.0;
-      R1 = R1 + px * phi4star(fabs(yl - yt), g.dyi, g.pi) * f[(size_t)(iy - g.by0) * g.bw + (ix - g.bx0)];
-    }
+  __shared__ double wxs[kInterpPerBlock][9], wys[kInterpPerBlock][9], prod[kInterpPerBlock][81];
+  const int lt = threadIdx.x / 9, a = threadIdx.x - 9 * lt;
+  const int t = blockIdx.x * kInterpPerBlock + lt;
+  const bool act = lt < kInterpPerBlock && t < nt;
+  int i = 0, j = 0;
+  const double *f = press;
+  if (act) {
+    const double xt = tg[3 * t], yt = tg[3 * t + 1];
+    const int fid = (int)tg[3 * t + 2];
+    f = fid == 0 ? press : fid == 1 ? sa : sb;
+    i = (int)((xt - 0.0) / g.dxi);
+    j = (int)((yt - 0.0) / g.dyi);
+    wxs[lt][a] = phi4star(fabs(((double)(i - 4 + a) * g.dxi + 0.0) - xt), g.dxi, g.pi);
+    wys[lt][a] = phi4star(fabs(((double)(j - 4 + a) * g.dyi + 0.0) - yt), g.dyi, g.pi);
   }
-  val[t] = R1;
+  __syncthreads();
+  if (act) {
+    const double *col = f + (size_t)(j - 4 - g.by0) * g.bw + (i - 4 - g.bx0 + a);
+    const double wx = wxs[lt][a];
+#pragma unroll
+    for (int b = 0; b < 9; ++b) prod[lt][9 * a + b] = wx * wys[lt][b] * col[(size_t)b * g.bw];
+  }
+  __syncthreads();
+  if (act && a == 0) {
+    double R1 = 0.0;
+    for (int q = 0; q < 81; ++q) R1 = R1 + prod[lt][q];
+    val[t] = R1;
+  }
 }
 
-// node data nd[16*n + ...]: 0 sin th, 1 cos th, 2 sin 2th, 3 cos 2th
-// circle data cd[8*i + ...] (trailing i < nc, leading nc <= i < 2nc):
+// node data nd[4*n + ...]: 0 sin th, 1 cos th, 2 sin 2th, 3 cos 2th
+// circle data cd[4*i + ...] (trailing i < nc, leading nc <= i < 2nc):
 //   0 cos phi, 1 sin phi, 2 cos 2phi, 3 sin 2phi
 // interpolated values val: node n: 6n+0 P(u1) 6n+1 P(b1) 6n+2 A(u) 6n+3 B(u)
 //   6n+4 A(b) 6n+5 B(b); circle point i: 6ns + 3i + {0 A, 1 B, 2 P}
-__global__ void k_forces_final(const double *__restrict__ nd, const double *__restrict__ cd,
-                               const double *__restrict__ val, ForceGeom g, double *__restrict__ pbeam,
-                               double *__restrict__ tbeam, StepOutDev *__restrict__ out) {
+// One workgroup: every node / circle point is a term, the eight running
+// sums of the reference (fx, fy, Ft, Fn, Fp) become block tree sums.
+__device__ void forces_final(const double *__restrict__ nd, const double *__restrict__ cd,
+                             const double *__restrict__ val, const ForceGeom &g, double *__restrict__ pbeam,
+                             double *__restrict__ tbeam, StepOutDev *__restrict__ out) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double red[8 * 16];
   const int ns = g.ns;
-  double fx = 0.0, fy = 0.0;
-  for (int n = 0; n < ns; ++n) {
-    const double st = nd[16 * n], ct = nd[16 * n + 1], s2 = nd[16 * n + 2], c2 = nd[16 * n + 3];
+  double acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc[i] = 0.0;
+  // acc: 0 fx, 1 fy, 2 Ft0, 3 Ft1, 4 Fn0, 5 Fn1, 6 Fp0, 7 Fp1
+  for (int n = threadIdx.x; n < ns; n += blockDim.x) {
+    const double st = nd[4 * n], ct = nd[4 * n + 1], s2 = nd[4 * n + 2], c2 = nd[4 * n + 3];
     double pb = val[6 * n] - val[6 * n + 1];
     double tb = 0.0;
-    if (g.visc) {
-      const double A1 = val[6 * n + 2], B1 = val[6 * n + 3], A2 = val[6 * n + 4], B2 = val[6 * n + 5];
+    const double A1 = val[6 * n + 2], B1 = val[6 * n + 3], A2 = val[6 * n + 4], B2 = val[6 * n + 5];
+    if (g.visc) {  // BeamForces.f90:206-215, :237
       const double sn1 = -(A1 * c2) - B1 * s2, st1 = -(A1 * s2) + B1 * c2;
       const double sn2 = -(A2 * c2) - B2 * s2, st2 = -(A2 * s2) + B2 * c2;
       tb = st1 - st2;
       pb = pb + (-sn1 + sn2);
     }
-    pb = pb * g.soft;
+    pb = pb * g.soft;  // :254-255
     tb = tb * g.soft;
     pbeam[n] = pb;
     tbeam[n] = tb;
-    fx = fx - (st * pb + ct * tb) * g.ds;
-    fy = fy + (ct * pb + st * tb) * g.ds;
+    acc[0] -= (st * pb + ct * tb) * g.ds;  // :265-266
+    acc[1] += (ct * pb + st * tb) * g.ds;
+    if (g.bsi_on) {
+      {  // top side, :433-452
+        const double A = A1, B = B1, P = val[6 * n];
+        const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
+        acc[2] += g.ds * stt * (+ct); acc[3] += g.ds * stt * (+st);
+        acc[4] += g.ds * sn * (-st);  acc[5] += g.ds * sn * (+ct);
+        acc[6] += g.ds * spp * (-st); acc[7] += g.ds * spp * (+ct);
+      }
+      {  // bottom side, :462-480
+        const double A = A2, B = B2, P = val[6 * n + 1];
+        const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
+        acc[2] += g.ds * stt * (-ct); acc[3] += g.ds * stt * (-st);
+        acc[4] += g.ds * sn * (+st);  acc[5] += g.ds * sn * (-ct);
+        acc[6] += g.ds * spp * (+st); acc[7] += g.ds * spp * (-ct);
+      }
+    }
   }
-  out->Force_press[0] = fx;
-  out->Force_press[1] = fy;
-  out->fx_int = fx;
-  out->fy_int = fy;
-  if (!g.bsi_on) return;
-  double Ft0 = 0, Ft1 = 0, Fn0 = 0, Fn1 = 0, Fp0 = 0, Fp1 = 0;
-  for (int i = 0; i < 2 * g.nc; ++i) {
-    const double cp = cd[8 * i], sp = cd[8 * i + 1], c2 = cd[8 * i + 2], s2 = cd[8 * i + 3];
-    const double A = val[6 * ns + 3 * i], B = val[6 * ns + 3 * i + 1], P = val[6 * ns + 3 * i + 2];
-    const double sn = +A * c2 + B * s2, stt = -(A * s2) + B * c2, spp = -P;
-    Ft0 = Ft0 + g.dL * stt * (-sp); Ft1 = Ft1 + g.dL * stt * (+cp);
-    Fn0 = Fn0 + g.dL * sn * (+cp);  Fn1 = Fn1 + g.dL * sn * (+sp);
-    Fp0 = Fp0 + g.dL * spp * (+cp); Fp1 = Fp1 + g.dL * spp * (+sp);
+  if (g.bsi_on) {
+    for (int i = threadIdx.x; i < 2 * g.nc; i += blockDim.x) {  // half circles, :354-420
+      const double cp = cd[4 * i], sp = cd[4 * i + 1], c2 = cd[4 * i + 2], s2 = cd[4 * i + 3];
+      const double A = val[6 * ns + 3 * i], B = val[6 * ns + 3 * i + 1], P = val[6 * ns + 3 * i + 2];
+      const double sn = +A * c2 + B * s2, stt = -(A * s2) + B * c2, spp = -P;
+      acc[2] += g.dL * stt * (-sp); acc[3] += g.dL * stt * (+cp);
+      acc[4] += g.dL * sn * (+cp);  acc[5] += g.dL * sn * (+sp);
+      acc[6] += g.dL * spp * (+cp); acc[7] += g.dL * spp * (+sp);
+    }
   }
-  for (int n = 0; n < ns; ++n) {  // top side, BeamForces.f90:433-452
-    const double st = nd[16 * n], ct = nd[16 * n + 1], s2 = nd[16 * n + 2], c2 = nd[16 * n + 3];
-    const double A = val[6 * n + 2], B = val[6 * n + 3], P = val[6 * n];
-    const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
-    Ft0 = Ft0 + g.ds * stt * (+ct); Ft1 = Ft1 + g.ds * stt * (+st);
-    Fn0 = Fn0 + g.ds * sn * (-st);  Fn1 = Fn1 + g.ds * sn * (+ct);
-    Fp0 = Fp0 + g.ds * spp * (-st); Fp1 = Fp1 + g.ds * spp * (+ct);
+  block_sum<8>(acc, red);
+  if (threadIdx.x == 0) {
+    out->Force_press[0] = acc[0];
+    out->Force_press[1] = acc[1];
+    out->fx_int = acc[0];
+    out->fy_int = acc[1];
+    if (g.bsi_on) {
+      out->bsi[2] = acc[2] + acc[4] + acc[6];
+      out->bsi[3] = acc[3] + acc[5] + acc[7];
+      out->bsi[4] = acc[2]; out->bsi[5] = acc[4]; out->bsi[6] = acc[6];
+      out->bsi[7] = acc[3]; out->bsi[8] = acc[5]; out->bsi[9] = acc[7];
+    }
   }
-  for (int n = 0; n < ns; ++n) {  // bottom side, :462-480
-    const double st = nd[16 * n], ct = nd[16 * n + 1], s2 = nd[16 * n + 2], c2 = nd[16 * n + 3];
-    const double A = val[6 * n + 4], B = val[6 * n + 5], P = val[6 * n + 1];
-    const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
-    Ft0 = Ft0 + g.ds * stt * (-ct); Ft1 = Ft1 + g.ds * stt * (-st);
-    Fn0 = Fn0 + g.ds * sn * (+st);  Fn1 = Fn1 + g.ds * sn * (-ct);
-    Fp0 = Fp0 + g.ds * spp * (+st); Fp1 = Fp1 + g.ds * spp * (-ct);
-  }
-  out->bsi[2] = Ft0 + Fn0 + Fp0;
-  out->bsi[3] = Ft1 + Fn1 + Fp1;
-  out->bsi[4] = Ft0; out->bsi[5] = Fn0; out->bsi[6] = Fp0;
-  out->bsi[7] = Ft1; out->bsi[8] = Fn1; out->bsi[9] = Fp1;
+}
+
+// Delta interpolation of every target; the last block to finish (counter in
+// *done, reset by that block) forms the beam loads and force sums.
+__global__ void __launch_bounds__(kInterpThreads) k_forces(const double *__restrict__ tg, int nt,
+                                                           const double *__restrict__ press,
+                                                           const double *__restrict__ sa,
+                                                           const double *__restrict__ sb, ForceGeom g,
+                                                           double *__restrict__ val, const double *__restrict__ nd,
+                                                           const double *__restrict__ cd, double *__restrict__ pbeam,
+                                                           double *__restrict__ tbeam, StepOutDev *__restrict__ out,
+                                                           unsigned int *__restrict__ done) {
+  __shared__ unsigned int last;
+  interp_targets(tg, nt, press, sa, sb, g, val);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1u : 0u;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  forces_final(nd, cd, val, g, pbeam, tbeam, out);
+  if (threadIdx.x == 0) *done = 0u;
 }
 
 }  // namespace fsibe

@@ -206,10 +206,11 @@ void *fsibe_sim_stream(fsibe_sim *sim);
  * fsibe_profile_enable resets the counters; fsibe_profile_read fills up to n
  * totals (milliseconds) and launch counts and returns the number of kernel
  * classes: 0 create_mask, 1 startup velocity, 2 columns of vortk(n0),
- * 3 rows: nonlinear term, 4 dt/AB2 scalars, 5 columns: AB2 update,
- * 6 columns of vortk(n1), 7 rows: velocity/pressure rhs, 8 reductions,
- * 9 columns: pressure, 10 rows: pressure/stress box, 11 Delta interpolation,
- * 12 beam forces. */
+ * 3 rows: nonlinear term (with the dt/AB2 scalars), 4 unused,
+ * 5 columns: AB2 update, 6 columns of vortk(n1), 7 rows: velocity/pressure
+ * rhs, 8 reductions (only when class 9 does not run), 9 columns: pressure
+ * (with the reductions), 10 rows: pressure/stress box, 11 Delta
+ * interpolation and beam forces, 12 unused. */
 int fsibe_profile_enable(fsibe_sim *sim, int on);
 int fsibe_profile_read(fsibe_sim *sim, double *total_ms, long long *count, int n);
 
