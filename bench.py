@@ -79,6 +79,83 @@ def read_traffic(kernel: str, n: int):
     return float(m.group(1)) if m else None
 
 
+def _csv_rows(path):
+    import csv
+
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find(d: str, suffix: str) -> str:
+    for root, _, files in os.walk(d):
+        for fn in sorted(files):
+            if fn.endswith(suffix):
+                return os.path.join(root, fn)
+    raise FileNotFoundError(f"no *{suffix} under {d}")
+
+
+def _short(name: str) -> str:
+    """fsibe::k_rows_press<fsibe::RowCT<512, 256> >(...) -> k_rows_press<RowCT<512, 256>>"""
+    name = name.split("(")[0].replace("fsibe::", "").replace(" >", ">")
+    return name.replace("void ", "").strip()
+
+
+def summarize_trace(d: str) -> str:
+    """Per-kernel dispatch count / total / mean duration from a
+    `rocprofv3 --kernel-trace --stats --output-format csv` directory; kernels
+    are keyed by name and grid size (one kernel serves several launches)."""
+    rows = _csv_rows(_find(d, "kernel_trace.csv"))
+    acc = {}
+    for r in rows:
+        g = r.get("Grid_Size") or "x".join(r.get(f"Grid_Size_{c}", "1") for c in "XYZ")
+        k = f"{_short(r['Kernel_Name'])} grid={g}"
+        v = acc.setdefault(k, [0, 0.0])
+        v[0] += 1
+        v[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    tot = sum(v[1] for v in acc.values())
+    out = [" calls   total_us    avg_us    pct  kernel"]
+    for k, v in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"{v[0]:6d} {v[1]:10.1f} {v[1] / v[0]:9.3f} {100 * v[1] / tot:6.2f}  {k}")
+    return "\n".join(out)
+
+
+def _pmc_means(d: str, counter: str):
+    """mean counter value (KiB) per dispatch for each kernel name"""
+    acc = {}
+    for r in _csv_rows(_find(d, "counter_collection.csv")):
+        if r["Counter_Name"] != counter:
+            continue
+        k = f"{_short(r['Kernel_Name'])} grid={r['Grid_Size']}"
+        v = acc.setdefault(k, [0.0, 0])
+        v[0] += float(r["Counter_Value"])
+        v[1] += 1
+    return {k: (v[0] / v[1], v[1]) for k, v in acc.items()}
+
+
+def summarize_pmc(fetch_dir: str, write_dir: str, cfetch_dir: str, cwrite_dir: str, copy_mib: int) -> str:
+    """HBM bytes per dispatch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    runs, each scaled by the factor that makes a device-to-device copy of
+    copy_mib MiB read and write exactly copy_mib MiB."""
+    ref_kib = copy_mib * 1024.0
+
+    def factor(d, counter):
+        m = _pmc_means(d, counter)
+        big = max(v[0] for v in m.values())  # the copy dispatches dominate
+        return ref_kib / big, big
+
+    ff, fk = factor(cfetch_dir, "FETCH_SIZE")
+    wf, wk = factor(cwrite_dir, "WRITE_SIZE")
+    fm, wm = _pmc_means(fetch_dir, "FETCH_SIZE"), _pmc_means(write_dir, "WRITE_SIZE")
+    out = [f"calibration: copy of {copy_mib} MiB reads FETCH_SIZE {fk:.1f} KiB -> factor {ff:.4f}; "
+           f"WRITE_SIZE {wk:.1f} KiB -> factor {wf:.4f}",
+           " dispatches  fetch_KiB  write_KiB  traffic_bytes_per_dispatch  kernel"]
+    for k in sorted(fm, key=lambda k: -fm[k][0]):
+        f = fm[k][0] * ff
+        w = wm.get(k, (0.0, 0))[0] * wf
+        out.append(f"{fm[k][1]:11d} {f:10.1f} {w:10.1f} {1024.0 * (f + w):27.0f}  {k}")
+    return "\n".join(out)
+
+
 def cpu_baseline(p, seconds: float, threads: int):
     from oracle import oracle
 
@@ -110,7 +187,18 @@ def main(argv=None) -> int:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 20 device-to-device copies of MIB MiB (PMC traffic calibration)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace --stats csv directory")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="print HBM bytes per dispatch from four rocprofv3 --pmc csv directories")
+    ap.add_argument("--copy-mib", type=int, default=64)
     args = ap.parse_args(argv)
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace))
+        return 0
+    if args.summarize_pmc:
+        print(summarize_pmc(*args.summarize_pmc, args.copy_mib))
+        return 0
     if args.calibrate_copy:
         import torch
 

@@ -421,6 +421,15 @@ extern "C" int fsibe_dealiase_mask(fsibe_spectral *c, double *d) {
 // ---------------------------------------------------------------------------
 // host kinematics: mouvement.f90:5-241 (+ cases 600, 601), integrate_position
 // ---------------------------------------------------------------------------
+// periods of the heaving cases 401..413 (mouvement.f90:81-94)
+static const double kTs400[13] = {0.51, 0.34, 0.25, 0.20, 0.17, 0.14, 0.13, 0.11, 0.10, 0.09, 0.08, 0.08, 0.07};
+
+// soft-startup time: cases 401..413 overwrite the global tau with their
+// period on every mouvement call (mouvement.f90:102-104)
+static double tau_eff(const fsibe_params &p) {
+  return (p.iMotion >= 401 && p.iMotion <= 413) ? kTs400[p.iMotion - 401] : p.tau;
+}
+
 static void mouvement_host(const fsibe_params &p, int im, double x0, double y0, double angle, double phase,
                            double time, double &alpha, double &alpha_t, double &alpha_tt, double LE[6]) {
   const double pi = 4.0 * atan(1.0);
@@ -448,6 +457,13 @@ static void mouvement_host(const fsibe_params &p, int im, double x0, double y0, 
     const double y = y_max * sin(arg), yt = y_max * (2.0 * pi / t_max) * cos(arg);
     const double ytt = -(y_max * (w * w) * sin(arg));
     LE[0] = x0; LE[1] = y0 + k * y; LE[3] = kt * y + yt * k; LE[5] = ktt * y + ytt * k + 2. * kt * yt;
+    alpha = angle * pi / 180.0;
+  } else if (im >= 401 && im <= 413) {
+    const double y_max = 0.0333, t_max = kTs400[im - 401], w = 2.0 * pi / t_max;
+    LE[0] = x0;
+    LE[1] = y0 + y_max * sin(2.0 * pi * time / t_max + pi / 2);
+    LE[3] = y_max * (2.0 * pi / t_max) * cos(2.0 * pi * time / t_max + pi / 2);
+    LE[5] = -(y_max * (w * w) * sin(2.0 * pi * time / t_max + pi / 2));
     alpha = angle * pi / 180.0;
   } else if (im == 2) {
     const double y_max = 0.5, t_max = 2.5, w = 2.0 * pi / t_max;
@@ -583,15 +599,21 @@ struct fsibe_sim {
   double *u1, *u2, *vor_mean, *ms, *wallrow;
   double *sums, *umax_part;
   DevScalars *dscal;
-  StepOutDev *dout;
   unsigned int *d_done;  // block counter of k_forces (zero between launches)
   double *mask_m, *mask_vx, *mask_vy;
   double *fb_p, *fb_sa, *fb_sb;
-  double *d_pbeam, *d_tbeam;
   // per-step upload area: prims | targets | node data | circle data | vals
   char *h_stage, *d_stage;
   size_t stage_bytes;
-  char *h_res;  // pinned result record
+  // two result slots (StepOutDev, p_beam[nsb], tau_beam[nsb]) in pinned,
+  // host-coherent memory that the kernels write directly; d_res is the
+  // device address of h_res
+  char *h_res = nullptr, *d_res = nullptr;
+  size_t res_stride = 0;
+  int slot = 0;
+  bool res_pending = false;  // pipelined run: results of the previous step not yet read
+  bool pipe = false;         // record ev_dt after the nonlinear-term kernel
+  hipEvent_t ev_dt;
   int n0 = 0, n1 = 1;
   // geometry of the current step (set by create_mask)
   bool have_beam = false, sponge_done = false;
@@ -633,6 +655,14 @@ static void prof_collect(fsibe_sim *s) {
     }
     s->prof.rec[k] = false;
   }
+}
+
+// device addresses of the result slot k
+static StepOutDev *res_dev(const fsibe_sim *s, int k) {
+  return reinterpret_cast<StepOutDev *>(s->d_res + (size_t)k * s->res_stride);
+}
+static double *pbeam_dev(const fsibe_sim *s, int k) {
+  return reinterpret_cast<double *>(s->d_res + (size_t)k * s->res_stride + sizeof(StepOutDev));
 }
 
 static double smooth_step_host(const fsibe_sim *s, double x, double t, double h) {
@@ -710,13 +740,14 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   (void)hipStreamSynchronize(s->stream);
   void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
                   s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow, s->sums,
-                  s->umax_part, s->dscal, s->dout, s->d_done, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
+                  s->umax_part, s->dscal, s->d_done, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
                   s->fb_sb, s->d_stage};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
   if (s->h_res) (void)hipHostFree(s->h_res);
   (void)hipEventDestroy(s->ev_upload);
+  (void)hipEventDestroy(s->ev_dt);
   for (int k = 0; k < P_NK; ++k)
     for (int j = 0; j < 2; ++j)
       if (s->prof.ev[k][j]) (void)hipEventDestroy(s->prof.ev[k][j]);
@@ -793,6 +824,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   } while (0)
   SIMCHK(hipStreamCreate(&s->stream));
   SIMCHK(hipEventCreateWithFlags(&s->ev_upload, hipEventDisableTiming));
+  SIMCHK(hipEventCreateWithFlags(&s->ev_dt, hipEventDisableTiming));
   const size_t N = (size_t)s->nx * s->ny, NH = (size_t)s->H * s->ny;
   auto tx = twiddles(s->nx), ty = twiddles(s->ny);
   SIMCHK(hipMalloc(&s->twx, sizeof(double2) * s->nx));
@@ -824,10 +856,10 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipMemset(s->d_done, 0, sizeof(unsigned int)));
   const int nsb0 = std::max(s->ns, 1);
   const size_t res_bytes = sizeof(StepOutDev) + 2 * sizeof(double) * nsb0;
-  SIMCHK(hipMalloc(&s->dout, res_bytes));
-  SIMCHK(hipMemset(s->dout, 0, res_bytes));
-  s->d_pbeam = reinterpret_cast<double *>(reinterpret_cast<char *>(s->dout) + sizeof(StepOutDev));
-  s->d_tbeam = s->d_pbeam + nsb0;
+  s->res_stride = (res_bytes + 255) & ~(size_t)255;
+  SIMCHK(hipHostMalloc((void **)&s->h_res, 2 * s->res_stride, hipHostMallocMapped | hipHostMallocCoherent));
+  memset(s->h_res, 0, 2 * s->res_stride);
+  SIMCHK(hipHostGetDevicePointer((void **)&s->d_res, s->h_res, 0));
   SIMCHK(hipMalloc(&s->mask_m, sizeof(double) * N));
   SIMCHK(hipMalloc(&s->mask_vx, sizeof(double) * N));
   SIMCHK(hipMalloc(&s->mask_vy, sizeof(double) * N));
@@ -851,7 +883,6 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipHostMalloc((void **)&s->h_stage, s->stage_bytes, hipHostMallocDefault));
   SIMCHK(hipMalloc(&s->d_stage, s->stage_bytes));
   SIMCHK(hipMemset(s->d_stage, 0, s->stage_bytes));
-  SIMCHK(hipHostMalloc((void **)&s->h_res, sizeof(StepOutDev) + 2 * sizeof(double) * nsb, hipHostMallocDefault));
   s->ms = nullptr;
   s->wallrow = nullptr;
   if (p.iWalls > 0) {
@@ -1085,7 +1116,7 @@ static int build_geometry(fsibe_sim *s, double time) {
     g.dL = (s->pi / (double)(kNC - 1)) * rr;
     g.visc = s->visc; g.bsi_on = bsi;
     // soft startup (BeamForces.f90:246-252)
-    const double Tr = s->p.T_release, tau = s->p.tau;
+    const double Tr = s->p.T_release, tau = tau_eff(s->p);
     double soft;
     if (time <= Tr) soft = 0.0;
     else if (time > Tr && time < (Tr + tau)) {
@@ -1245,10 +1276,11 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
       KLAUNCH(k_rows_nl<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
               s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
-              s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step);
+              s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot));
       return FSIBE_OK;
     }));
     prof_end(s, P_ROWS_NL);
+    if (s->pipe) HIPCHK(hipEventRecord(s->ev_dt, s->stream));
   }
   // K4
   {
@@ -1284,7 +1316,7 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
   }
   // K9 reductions: in block 0 of K7 when K7 runs, else on their own
   ReduceArgs ra;
-  ra.sums = s->sums; ra.ny = s->ny; ra.beam_on = (int)(s->p.iBeam > 0); ra.s = s->dscal; ra.out = s->dout;
+  ra.sums = s->sums; ra.ny = s->ny; ra.beam_on = (int)(s->p.iBeam > 0); ra.s = s->dscal; ra.out = res_dev(s, s->slot);
   ra.dxdy = s->dx * s->dy; ra.nu = s->p.nu; ra.eps = s->p.eps; ra.U_mean_true = s->p.U_mean_true;
   // K7 + K8 on the rows/columns the interpolation needs
   s->press_box_valid = false;
@@ -1328,20 +1360,18 @@ static int launch_forces(fsibe_sim *s) {
   prof_begin(s, P_INTERP);
   k_forces<<<(s->ntg + kInterpPerBlock - 1) / kInterpPerBlock, kInterpThreads, 0, s->stream>>>(
       tg, s->ntg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
-      reinterpret_cast<const double *>(s->d_stage + s->off_cd), s->d_pbeam, s->d_tbeam, s->dout, s->d_done);
+      reinterpret_cast<const double *>(s->d_stage + s->off_cd), pbeam_dev(s, s->slot),
+      pbeam_dev(s, s->slot) + std::max(s->ns, 1), res_dev(s, s->slot), s->d_done);
   HIPCHK(hipGetLastError());
   prof_end(s, P_INTERP);
   return FSIBE_OK;
 }
 
-static int fetch_results(fsibe_sim *s) {
+// copy result slot k (complete: its kernels have finished) into s->last
+static void parse_results(fsibe_sim *s, int k) {
   const int nsb = std::max(s->ns, 1);
-  HIPCHK(hipMemcpyAsync(s->h_res, s->dout, sizeof(StepOutDev) + 2 * sizeof(double) * nsb, hipMemcpyDeviceToHost,
-                        s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  s->upload_pending = false;
-  prof_collect(s);
-  const StepOutDev *o = reinterpret_cast<const StepOutDev *>(s->h_res);
+  const char *base = s->h_res + (size_t)k * s->res_stride;
+  const StepOutDev *o = reinterpret_cast<const StepOutDev *>(base);
   fsibe_step_out &r = s->last;
   r.dt1 = o->dt1; r.u_max = o->u_max;
   for (int i = 0; i < 2; ++i) { r.Force[i] = o->Force[i]; r.Force_unst[i] = o->Force_unst[i]; r.Force_press[i] = o->Force_press[i]; }
@@ -1349,8 +1379,15 @@ static int fetch_results(fsibe_sim *s) {
   for (int i = 0; i < 10; ++i) r.bsi[i] = o->bsi[i];
   r.fx_int = o->fx_int; r.fy_int = o->fy_int;
   r.status = 0;
-  const double *pb = reinterpret_cast<const double *>(s->h_res + sizeof(StepOutDev));
+  const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
   for (int i = 0; i < s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
+}
+
+static int fetch_results(fsibe_sim *s) {
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->upload_pending = false;
+  prof_collect(s);
+  parse_results(s, s->slot);
   return FSIBE_OK;
 }
 
@@ -1394,7 +1431,12 @@ extern "C" int fsibe_get_step_out(fsibe_sim *s, fsibe_step_out *out) {
 }
 
 // ---- time loop (time_step.f90:120-162, :178-341), prescribed kinematics ---
-static int run_one(fsibe_sim *s) {
+// One step of the loop.  pipe: do not wait for the step to finish; wait only
+// for dt1 (written by the nonlinear-term kernel), read the previous step's
+// results, and return so that the host prepares the next step's geometry
+// while this one runs.
+static int run_one(fsibe_sim *s, bool pipe) {
+  s->pipe = pipe;
   const fsibe_params &p = s->p;
   const bool moving = (p.iSponge >= 4 && s->time < p.T_fullspeed) || p.iFLUSI == 1 || p.iMotion > 0;
   if (p.iBeam > 0 && (moving || !s->have_beam)) {
@@ -1416,8 +1458,17 @@ static int run_one(fsibe_sim *s) {
     CHK(evolve(s, s->time, s->dt0, false));
   }
   if (s->forces_on) CHK(launch_forces(s));
-  CHK(fetch_results(s));
-  s->dt1 = s->last.dt1;
+  if (!pipe) {
+    CHK(fetch_results(s));
+    s->dt1 = s->last.dt1;
+  } else {
+    HIPCHK(hipEventSynchronize(s->ev_dt));
+    s->upload_pending = false;  // the geometry upload precedes the nonlinear-term kernel
+    s->dt1 = *reinterpret_cast<volatile const double *>(s->h_res + (size_t)s->slot * s->res_stride);
+    if (s->res_pending) parse_results(s, s->slot ^ 1);  // finished before this step started
+    s->res_pending = true;
+    s->slot ^= 1;
+  }
   if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0)
     integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
                             s->by.data(), s->bvx.data(), s->bvy.data());
@@ -1445,10 +1496,21 @@ static void init_beam_host(fsibe_sim *s) {
 extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
   if (!s->startup_done && s->time == 0.0 && s->p.iBeam > 0 && !s->have_beam) init_beam_host(s);
+  const bool pipe = !s->prof.on;  // kernel timing needs a synchronisation per step
+  int rc = FSIBE_OK;
   for (int k = 0; k < nsteps; ++k) {
     if (!(s->time < s->p.Time_end)) break;
-    CHK(run_one(s));
+    rc = run_one(s, pipe);
+    if (rc != FSIBE_OK) break;
   }
+  s->pipe = false;
+  if (s->res_pending) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->upload_pending = false;
+    parse_results(s, s->slot ^ 1);
+    s->res_pending = false;
+  }
+  if (rc != FSIBE_OK) return rc;
   if (out) *out = s->last;
   return FSIBE_OK;
 }

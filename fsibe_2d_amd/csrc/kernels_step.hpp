@@ -54,6 +54,17 @@ struct DevScalars {
   int ready, pad;
 };
 
+// per-step results; lives in pinned host memory that the kernels write
+// directly (two slots, so the host reads one step while the next runs)
+struct StepOutDev {
+  double dt1, u_max;
+  double Force[2], Force_unst[2], Force_press[2];
+  double energies[6];
+  double bsi[10];
+  double fx_int, fy_int;
+  int status, pad;
+};
+
 struct FlowParams {
   int iMeanVelocity, penal_on;
   double theta_inf, T_fullspeed, U_mean_true, cfl, eps, dt_fixed, Time_end, pi, dx, dy, nu,
@@ -181,7 +192,8 @@ __global__ void __launch_bounds__(FY::LB) k_cols_inv_fields(const double2 *__res
 // ============================================================================
 // Block-wide (any multiple of 64 threads); red holds >= blockDim/64 doubles.
 __device__ void step_scalars(const double *__restrict__ umax_part, int nparts, DevScalars *__restrict__ s,
-                             const FlowParams &f, double time, double dt0, int explicit_step, double *red) {
+                             const FlowParams &f, double time, double dt0, int explicit_step, double *red,
+                             StepOutDev *__restrict__ res) {
 #pragma clang fp contract(off)
   double m = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmax(m, umax_part[i]);
@@ -217,6 +229,10 @@ __device__ void step_scalars(const double *__restrict__ umax_part, int nparts, D
     s->My = my;
     s->UMx = f.U_mean_true * mx;
     s->UMy = f.U_mean_true * my;
+    // the host reads dt1 as soon as this kernel has finished
+    res->dt1 = dt1;
+    res->u_max = u_max;
+    __threadfence_system();
   }
   __syncthreads();
 }
@@ -233,10 +249,10 @@ __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ 
                                                     const double2 *__restrict__ twx, StepConsts c,
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
-                                                    double dt0, int explicit_step) {
+                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res) {
   extern __shared__ double2 lds[];
   __shared__ double red[16];
-  if (blockIdx.x == 0) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red);
+  if (blockIdx.x == 0) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
   const int y = blockIdx.x;
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
@@ -483,14 +499,6 @@ __global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict
   }
 }
 
-struct StepOutDev {
-  double dt1, u_max;
-  double Force[2], Force_unst[2], Force_press[2];
-  double energies[6];
-  double bsi[10];
-  double fx_int, fy_int;
-  int status, pad;
-};
 
 // ============================================================================
 // K9: reductions of the per-row partial sums: cal_drag (Force, Force_unst,
@@ -548,6 +556,7 @@ __device__ void reduce_step(const ReduceArgs &r) {
       s->lift_unst_old = ln;
       s->ready = 1;
     }
+    __threadfence_system();
   }
 }
 
@@ -788,6 +797,7 @@ __device__ void forces_final(const double *__restrict__ nd, const double *__rest
       out->bsi[7] = acc[3]; out->bsi[8] = acc[5]; out->bsi[9] = acc[7];
     }
   }
+  __threadfence_system();
 }
 
 // Delta interpolation of every target; the last block to finish (counter in

@@ -577,6 +577,15 @@ double o_mean_uy(o_sim *s, double time) {
 
 /* mouvement.f90:5-241, plus two prescribed-motion cases that the reference
  * does not have (600: sinusoidal heave, 601: Fourier heave + pitch). */
+/* periods of the heaving cases 401..413, mouvement.f90:81-94 */
+static const double o_ts400[13] = {0.51, 0.34, 0.25, 0.20, 0.17, 0.14, 0.13, 0.11, 0.10, 0.09, 0.08, 0.08, 0.07};
+
+/* soft-startup time: the cases 401..413 overwrite the global tau with their
+   period on every mouvement call (mouvement.f90:102-104) */
+static double o_tau(const o_sim *s, int im) {
+  return (im >= 401 && im <= 413) ? o_ts400[im - 401] : s->p.tau;
+}
+
 void o_mouvement(o_sim *s, double time, const o_beam *b, double *alpha,
                  double *alpha_t, double *alpha_tt, double LE[6]) {
   const double pi = s->pi;
@@ -604,6 +613,13 @@ void o_mouvement(o_sim *s, double time, const o_beam *b, double *alpha,
     double ytt = -(y_max * (w * w) * sin(arg));
     LE[0] = b->x0; LE[1] = b->y0 + k * y; LE[2] = 0.0;
     LE[3] = kt * y + yt * k; LE[4] = 0.0; LE[5] = ktt * y + ytt * k + 2. * kt * yt;
+    *alpha = b->AngleBeam * pi / 180.0;
+  } else if (im >= 401 && im <= 413) { /* :77-117 (also sets tau = t_max, see o_tau) */
+    double y_max = 0.0333, t_max = o_ts400[im - 401], w = 2.0 * pi / t_max;
+    LE[0] = b->x0;
+    LE[1] = b->y0 + y_max * sin(2.0 * pi * time / t_max + pi / 2);
+    LE[3] = y_max * (2.0 * pi / t_max) * cos(2.0 * pi * time / t_max + pi / 2);
+    LE[5] = -(y_max * (w * w) * sin(2.0 * pi * time / t_max + pi / 2));
     *alpha = b->AngleBeam * pi / 180.0;
   } else if (im == 2) { /* :118-133 */
     double y_max = 0.5, t_max = 2.5, w = 2.0 * pi / t_max;
@@ -1350,7 +1366,7 @@ void o_get_forces(o_sim *s, double time, o_beam *b, const double *press,
     for (int n = 0; n < ns; n++) tau_beam[n] = 0.0;
   }
   double soft; /* :246-252 */
-  const double Tr = s->p.T_release, tau = s->p.tau;
+  const double Tr = s->p.T_release, tau = o_tau(s, b->iMouvement);
   if (time <= Tr) soft = 0.0;
   else if (time > Tr && time < (Tr + tau)) {
     double a = time - Tr;
