@@ -117,6 +117,7 @@ def lib():
         L.o_delta_phi4star.argtypes = [C.c_double, C.c_double, C.c_double]
         L.o_delta_interp.restype = C.c_double
         L.o_delta_interp.argtypes = [vp, C.c_double, C.c_double, dp, C.POINTER(C.c_int)]
+        L.o_get_forces.argtypes = [vp, C.c_double, C.POINTER(OBeam), dp, dp]
         L.o_set_vortk.argtypes = [vp, dp]
         L.o_run.argtypes = [vp, C.c_int]
         L.o_startup_step.argtypes = [vp]
@@ -285,6 +286,12 @@ class Oracle:
         if err.value:
             raise ValueError("DeltaInterpolation: target outside field")
         return v
+
+    def get_forces(self, time, press, u):
+        """GetForces (BeamForces.f90:6-51) on the given pressure and velocity
+        (u: (2, ny, nx)); results land in the beam (pressure, tau, Force_press)."""
+        self.L.o_get_forces(self.h, time, self.L.o_get_beam(self.h), self._as(press),
+                            np.ascontiguousarray(u, dtype=np.float64))
 
     # ---- solver ---------------------------------------------------------------
     def set_vortk(self, vk):

@@ -1,0 +1,141 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point that
+include/fsibe_hip.h declares, rejects bad arguments before touching the
+device, and its host-side kinematics (fsibe_mouvement, fsibe_integrate_position)
+agree bit for bit with the reference's mouvement.f90 / integrate_position.f90
+(oracle/_ref) and the oracle.  Also the synthetic-input generator against the
+published SplitMix64 sequence."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from refbuild import Ref, needs_ref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "fsibe_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fsibe_\w+)\s*\(", src)))
+
+
+def test_header_declares_the_python_export_list():
+    from fsibe_2d_amd._lib import EXPORTS
+
+    assert set(_declared()) == set(EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    from fsibe_2d_amd._lib import lib
+
+    L = lib()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _err(L):
+    return L.fsibe_last_error().decode()
+
+
+def test_argument_errors_before_any_device_call():
+    from fsibe_2d_amd._lib import FsibeParams, lib
+    from fsibe_2d_amd.params import Params
+
+    L = lib()
+    h = C.c_void_p()
+    assert L.fsibe_spectral_create(100, 64, 4.0, 4.0, 1e-2, C.byref(h)) == 2  # FSIBE_ESIZE
+    assert "not supported" in _err(L)
+    assert L.fsibe_spectral_create(64, 64, -1.0, 4.0, 1e-2, C.byref(h)) == 1  # FSIBE_EARG
+    assert L.fsibe_sim_create(None, C.byref(h)) == 1
+
+    def create(**kw):
+        p = Params(nx=64, ny=64, **kw).to_c()
+        return L.fsibe_sim_create(C.byref(p), C.byref(h))
+
+    assert create(iFLUSI=1) == 4 and "host solid solver" in _err(L)
+    assert create(iSponge=4) == 4
+    assert create(iMeanVelocity=11) == 4
+    assert create(iBeam=2) == 4
+    assert create(eps=0.0) == 1
+    assert create(iSponge=3, eps_sponge=0.0) == 1
+    assert isinstance(FsibeParams(), C.Structure)
+    assert L.fsibe_run(None, 1, None) == 1
+
+
+MOTIONS = [0, 2, 3, 4, 201, 209, 213, 300, 350, 401, 413]
+
+
+@needs_ref
+@pytest.mark.parametrize("im", MOTIONS)
+def test_host_mouvement_matches_reference(im):
+    from fsibe_2d_amd.params import Params
+    from fsibe_2d_amd.solver import mouvement
+
+    r = Ref(16, 16, ns=8)
+    p = Params(nx=64, ny=64, ns=8, iMotion=im, x0=0.9, y0=2.1, AngleBeam=-15.0)
+    for t in (0.0, 0.2, 0.9999, 1.0, 2.75):
+        a, at, att, le = mouvement(p, t, phase=0.3)
+        ra, rat, ratt, rle = r.mouvement(im, 0.9, 2.1, -15.0, 0.3, t)
+        assert (a, at, att) == (ra, rat, ratt) and np.array_equal(le, rle), t
+
+
+@needs_ref
+@pytest.mark.parametrize("im,ns", [(0, 3), (3, 64), (201, 128), (4, 17)])
+def test_host_integrate_position_matches_reference(im, ns):
+    """the host sums in O(ns) what the reference sums in O(ns^2), bit for bit"""
+    from fsibe_2d_amd.params import Params
+    from fsibe_2d_amd.solver import integrate_position
+
+    r = Ref(16, 16, ns=ns, length=1.0)
+    p = Params(nx=64, ny=64, ns=ns, iMotion=im, x0=1.0, y0=2.0, AngleBeam=10.0, length=1.0)
+    rng = np.random.default_rng(ns)
+    for t in (0.0, 0.4, 1.3):
+        th, thd = 0.3 * rng.normal(size=ns), rng.normal(size=ns)
+        got = integrate_position(p, t, th, thd)
+        ref = r.integrate_position(im, 1.0, 2.0, 10.0, 0.0, t, th, thd)
+        for g, e in zip(got, ref):
+            assert np.array_equal(g, e), t
+
+
+@pytest.mark.parametrize("im", [600, 601])
+def test_host_motion_cases_without_reference_counterpart(oracle_mod, im):
+    """heave (600) and Fourier heave + pitch (601): host and oracle agree bit
+    for bit, and the velocity is the time derivative of the position"""
+    from fsibe_2d_amd.params import Params
+    from fsibe_2d_amd.solver import mouvement
+
+    coef = [0.05, -0.02, 0.01, 0.03, 0.1, -0.04, 0.02, 0.05]
+    p = Params(nx=64, ny=64, ns=8, iMotion=im, heave_amp=0.1, heave_freq=1.3, motion_coef=coef)
+    o = oracle_mod.Oracle(nx=16, ny=16, ns=8, iMotion=im, heave_amp=0.1, heave_freq=1.3, motion_coef=coef,
+                          x0=p.x0, y0=p.y0, AngleBeam=p.AngleBeam)
+    for t in (0.0, 0.31, 1.7):
+        a, at, att, le = mouvement(p, t)
+        oa, oat, oatt, ole = o.mouvement(t)
+        assert (a, at, att) == (oa, oat, oatt) and np.array_equal(le, ole)
+        h = 1e-6
+        le_p, le_m = mouvement(p, t + h)[3], mouvement(p, t - h)[3]
+        assert abs((le_p[1] - le_m[1]) / (2 * h) - le[3]) < 1e-8
+        assert abs((le_p[3] - le_m[3]) / (2 * h) - le[5]) < 1e-6
+        ap, am = mouvement(p, t + h)[0], mouvement(p, t - h)[0]
+        assert abs((ap - am) / (2 * h) - at) < 1e-8
+
+
+def test_splitmix64_published_sequence():
+    from fsibe_2d_amd.inputs import splitmix64
+
+    z = splitmix64(0, 4)
+    assert [int(v) for v in z] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F,
+                                   0xF88BB8A8724C81EC]
+
+
+def test_normals_match_the_oracle(oracle_mod):
+    """numpy and the C oracle draw the same normals (Box-Muller on the same
+    uniforms; numpy's log/sin may differ in the last bit)"""
+    from fsibe_2d_amd.inputs import normals
+
+    a, b = normals(3, 10001), oracle_mod.normals(3, 10001)
+    assert np.abs(a - b).max() < 4e-15
+    assert abs(a.mean()) < 0.03 and abs(a.std() - 1.0) < 0.03
