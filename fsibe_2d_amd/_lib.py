@@ -66,7 +66,7 @@ EXPORTS = (
     "fsibe_evolve_explicit", "fsibe_evolve_ab2", "fsibe_get_forces", "fsibe_get_step_out",
     "fsibe_swap_levels", "fsibe_get_field", "fsibe_get_box", "fsibe_mouvement",
     "fsibe_integrate_position", "fsibe_run", "fsibe_get_time", "fsibe_get_beam",
-    "fsibe_sim_stream", "fsibe_profile_enable", "fsibe_profile_read",
+    "fsibe_sim_stream", "fsibe_profile_enable", "fsibe_profile_read", "fsibe_cofts", "fsibe_cofits",
 )
 
 _lib = None
@@ -119,6 +119,8 @@ def lib():
     L.fsibe_sim_stream.restype = vp
     L.fsibe_sim_stream.argtypes = [vp]
     L.fsibe_profile_enable.argtypes = [vp, C.c_int]
+    L.fsibe_cofts.argtypes = [dp, dp, C.c_int, C.c_int]
+    L.fsibe_cofits.argtypes = [dp, dp, C.c_int, C.c_int]
     L.fsibe_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
     _lib = L
     return L

@@ -509,6 +509,73 @@ static void mouvement_host(const fsibe_params &p, int im, double x0, double y0, 
   }
 }
 
+// ---- 1-D transforms of the FFT plugin (cofts / cofits) ---------------------
+// Host-side: their only caller is the beam-curvature filter of the host solid
+// solver (SolidSolver.f90:1388,1417) on L = ns + 2 n_ext points, far too
+// small for a device launch.  Each column is a real DFT of length L summed in
+// long double with twiddles exp(-2 pi i jk/L) from a table of L entries.
+static void twiddle_table(int L, std::vector<long double> &c, std::vector<long double> &s) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  c.resize(L);
+  s.resize(L);
+  for (int m = 0; m < L; ++m) {
+    c[m] = cosl(2.0L * pi * (long double)m / (long double)L);
+    s[m] = sinl(2.0L * pi * (long double)m / (long double)L);
+  }
+}
+
+extern "C" int fsibe_cofts(const double *f, double *fk, int L, int n) {
+  if (!f || !fk || L < 1 || n < 0) return set_err(FSIBE_EARG, "cofts: bad argument");
+  std::vector<long double> c, s;
+  twiddle_table(L, c, s);
+  std::vector<double> ft((size_t)L + 2);
+  for (int j = 0; j < n; ++j) {
+    const double *x = f + (size_t)j * L;
+    // ft = r2c(x) as interleaved (Re, Im) for k = 0 .. L/2
+    for (int k = 0; k <= L / 2; ++k) {
+      long double re = 0.0L, im = 0.0L;
+      for (int m = 0; m < L; ++m) {
+        const int q = (int)(((long long)k * m) % L);
+        re += (long double)x[m] * c[q];
+        im -= (long double)x[m] * s[q];
+      }
+      ft[2 * k] = (double)re;
+      ft[2 * k + 1] = (double)im;
+    }
+    // fk = ft(0:L-1) / L: the highest coefficient pair is cut (cof_fftw33.f90:277-285)
+    const double norm = 1.0 / (double)L;
+    for (int m = 0; m < L; ++m) fk[(size_t)j * L + m] = ft[m] * norm;
+  }
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_cofits(const double *fk, double *f, int L, int n) {
+  if (!f || !fk || L < 1 || n < 0) return set_err(FSIBE_EARG, "cofits: bad argument");
+  std::vector<long double> c, s;
+  twiddle_table(L, c, s);
+  std::vector<double> ft((size_t)L + 2);
+  for (int j = 0; j < n; ++j) {
+    // ft(0:L-1) = fk, ft(L:L+1) = 0 (cof_fftw33.f90:311-316), then the
+    // unnormalised c2r, which reads Re only of the k = 0 (and, for even L,
+    // the k = L/2) coefficient
+    for (int m = 0; m < L; ++m) ft[m] = fk[(size_t)j * L + m];
+    ft[L] = 0.0;
+    ft[L + 1] = 0.0;
+    const int K = L / 2;  // last index of the half spectrum
+    for (int m = 0; m < L; ++m) {
+      long double acc = (long double)ft[0];
+      for (int k = 1; k <= K; ++k) {
+        const int q = (int)(((long long)k * m) % L);
+        const long double re = ft[2 * k], im = ft[2 * k + 1];
+        const long double t = re * c[q] - im * s[q];
+        acc += (2 * k == L) ? (long double)re * c[q] : 2.0L * t;
+      }
+      f[(size_t)j * L + m] = (double)acc;
+    }
+  }
+  return FSIBE_OK;
+}
+
 extern "C" int fsibe_mouvement(const fsibe_params *p, int im, double x0, double y0, double angle, double phase,
                                double time, double *alpha, double *alpha_t, double *alpha_tt, double LE[6]) {
   if (!p || !alpha || !alpha_t || !alpha_tt || !LE) return set_err(FSIBE_EARG, "null argument");
@@ -576,6 +643,7 @@ enum ProfKernel {
 
 struct Prof {
   bool on = false;
+  unsigned mask = 0;  // kernel classes that are timed
   hipEvent_t ev[P_NK][2] = {};
   bool rec[P_NK] = {};
   double ms[P_NK] = {};
@@ -635,10 +703,10 @@ struct fsibe_sim {
 };
 
 static void prof_begin(fsibe_sim *s, int k) {
-  if (s->prof.on) (void)hipEventRecord(s->prof.ev[k][0], s->stream);
+  if (s->prof.on && (s->prof.mask >> k & 1u)) (void)hipEventRecord(s->prof.ev[k][0], s->stream);
 }
 static void prof_end(fsibe_sim *s, int k) {
-  if (s->prof.on) {
+  if (s->prof.on && (s->prof.mask >> k & 1u)) {
     (void)hipEventRecord(s->prof.ev[k][1], s->stream);
     s->prof.rec[k] = true;
   }
@@ -1633,7 +1701,7 @@ extern "C" int fsibe_get_box(fsibe_sim *s, int which, double *out, int box[4]) {
   return FSIBE_OK;
 }
 
-extern "C" int fsibe_profile_enable(fsibe_sim *s, int on) {
+extern "C" int fsibe_profile_enable(fsibe_sim *s, int classes) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
   HIPCHK(hipStreamSynchronize(s->stream));
   for (int k = 0; k < P_NK; ++k) {
@@ -1643,7 +1711,8 @@ extern "C" int fsibe_profile_enable(fsibe_sim *s, int on) {
     s->prof.cnt[k] = 0;
     s->prof.rec[k] = false;
   }
-  s->prof.on = on != 0;
+  s->prof.on = classes != 0;
+  s->prof.mask = (unsigned)classes & ((1u << P_NK) - 1u);
   return FSIBE_OK;
 }
 

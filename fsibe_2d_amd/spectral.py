@@ -92,3 +92,20 @@ def fft_initialize(nx, ny, xl, yl, nu=0.0) -> Spectral:
 
 def fft_free(ctx: Spectral):
     ctx.close()
+
+
+def cofts(f, L: int, n: int = 1):
+    """n real sequences of length L (columns of f, shape (n, L)) -> the
+    truncated r2c coefficients / L (cof_fftw33.f90:248-296), host-side."""
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(n, L)
+    fk = np.empty_like(f)
+    check(lib().fsibe_cofts(f, fk, L, n))
+    return fk
+
+
+def cofits(fk, L: int, n: int = 1):
+    """inverse of cofts without the 1/L (cof_fftw33.f90:300-340), host-side."""
+    fk = np.ascontiguousarray(fk, dtype=np.float64).reshape(n, L)
+    f = np.empty_like(fk)
+    check(lib().fsibe_cofits(fk, f, L, n))
+    return f

@@ -73,6 +73,14 @@ int fsibe_vis(fsibe_spectral *ctx, double dt, double *work);
 /* replaces dealiase_mask (dealiase_mask.f90:1-33); writes the 0/1 mask */
 int fsibe_dealiase_mask(fsibe_spectral *ctx, double *dealiase);
 
+/* replace cofts(f, fk, L, n) and cofits(fk, f, L, n) (cof_fftw33.f90:248-340):
+ * n real sequences of length L in columns, r2c scaled by 1/L and truncated
+ * to L reals (Re0, Im0, Re1, Im1, ...), and the unnormalised inverse.  These
+ * run on the host: their only caller is the host solid solver's curvature
+ * filter (SolidSolver.f90:1388,1417) on a few hundred points. */
+int fsibe_cofts(const double *f, double *fk, int L, int n);
+int fsibe_cofits(const double *fk, double *f, int L, int n);
+
 /* Device-pointer twins of the above, asynchronous on the context's stream
  * (stream = NULL) or on the given hipStream_t. */
 int fsibe_dev_coftxy(fsibe_spectral *ctx, const double *d_f, double *d_fk, void *stream);
@@ -203,7 +211,9 @@ void *fsibe_sim_stream(fsibe_sim *sim);
 /* Per-kernel timing with HIP events recorded on the simulation's stream
  * around each launch (no reference counterpart: the reference times whole
  * routines with system_clock, PerformanceMeasurement.f90:26-121).
- * fsibe_profile_enable resets the counters; fsibe_profile_read fills up to n
+ * fsibe_profile_enable resets the counters and times the kernel classes whose
+ * bits are set in `classes` (-1: all, 0: off; timing synchronises once per
+ * step); fsibe_profile_read fills up to n
  * totals (milliseconds) and launch counts and returns the number of kernel
  * classes: 0 create_mask, 1 startup velocity, 2 columns of vortk(n0),
  * 3 rows: nonlinear term (with the dt/AB2 scalars), 4 unused,
@@ -211,7 +221,7 @@ void *fsibe_sim_stream(fsibe_sim *sim);
  * rhs, 8 reductions (only when class 9 does not run), 9 columns: pressure
  * (with the reductions), 10 rows: pressure/stress box, 11 Delta
  * interpolation and beam forces, 12 unused. */
-int fsibe_profile_enable(fsibe_sim *sim, int on);
+int fsibe_profile_enable(fsibe_sim *sim, int classes);
 int fsibe_profile_read(fsibe_sim *sim, double *total_ms, long long *count, int n);
 
 #ifdef __cplusplus

@@ -869,7 +869,7 @@ static void draw_sharp_end(o_sim *s, double pex, double pey, double alpha,
  * are not restated) */
 void o_create_mask(o_sim *s, double time, o_beam *b) {
   const int nx = s->nx, ny = s->ny, ns = s->ns;
-  const double dx = s->dx, dy = s->dy, N = s->p.N_smooth;
+  const double dy = s->dy, N = s->p.N_smooth;
   size_t NN = (size_t)nx * ny;
   double h_star = s->p.h_channel + N * dy;
   memset(s->mask, 0, NN * sizeof(double));

@@ -14,6 +14,21 @@
  *
  * Array convention (reference NavierStokes.f90:19-28): real(8) f(0:nx-1,0:ny-1)
  * column-major, x fastest, i.e. f[iy*nx + ix] in C.
+ *
+ * How each part is pinned (tests/test_oracle_*.py, tests/test_golden.py):
+ * the FFT and packing by a brute-force DFT, numpy and single-mode closed
+ * forms; the operators, vis, dealiase_mask, Mean_ux/uy, mouvement and
+ * integrate_position bit for bit by the reference's own Fortran (oracle/_ref)
+ * and its golden vectors; cal_velocity, cal_nlk (nonlinear, penalisation and
+ * sponge terms), the CFL step, cal_energy, time_avg_vort and the integrator
+ * by closed forms (Taylor-Green decay, AB2 order); SmoothStep, DrawADot, a
+ * straight beam and the type-3 sponge by erf closed forms and symmetry;
+ * phi*, DeltaInterpolation and GetForcesInterp by moment conditions and
+ * linear / sinusoidal fields.
+ * Parity unpinned (no independent check; GPU parity against this code only):
+ * cal_drag's Force / Force_unst window sums, BeamSurfaceIntegral, the bent
+ * beam parts of create_mask (DrawHinge sectors, DrawSharpEnd), walls, and the
+ * sponge types 1, 2 and 10.
  */
 #ifndef FSIBE_ORACLE_H
 #define FSIBE_ORACLE_H

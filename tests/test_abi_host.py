@@ -139,3 +139,30 @@ def test_normals_match_the_oracle(oracle_mod):
     a, b = normals(3, 10001), oracle_mod.normals(3, 10001)
     assert np.abs(a - b).max() < 4e-15
     assert abs(a.mean()) < 0.03 and abs(a.std() - 1.0) < 0.03
+
+
+@pytest.mark.parametrize("L", [8, 9, 100, 131])
+def test_host_1d_transforms_against_numpy(L):
+    """cofts = rfft / L truncated to L reals (Re0, Im0, Re1, ...); cofits =
+    unnormalised c2r of those L reals with the cut pair zero
+    (cof_fftw33.f90:248-340)"""
+    from fsibe_2d_amd.spectral import cofits, cofts
+
+    n = 3
+    f = np.random.default_rng(L).normal(size=(n, L))
+    fk = cofts(f, L, n)
+    c = np.fft.rfft(f, axis=1) / L
+    inter = np.empty((n, 2 * c.shape[1]))
+    inter[:, 0::2], inter[:, 1::2] = c.real, c.imag
+    assert np.abs(fk - inter[:, :L]).max() < 1e-15
+    # inverse of the truncated spectrum
+    pad = np.zeros((n, 2 * (L // 2 + 1)))
+    pad[:, :L] = fk
+    cc = pad[:, 0::2] + 1j * pad[:, 1::2]
+    ref = np.fft.irfft(cc[:, : L // 2 + 1], n=L, axis=1) * L
+    assert np.abs(cofits(fk, L, n) - ref).max() < 1e-13
+    if L % 2 == 1:  # odd L: only the last imaginary part is cut
+        g = cofits(cofts(f, L, n), L, n)
+        k = (L - 1) // 2
+        lost = -2 * c[:, k].imag[:, None] * np.sin(2 * np.pi * k * np.arange(L) / L)[None, :]
+        assert np.abs(g - (f - lost)).max() < 1e-13

@@ -114,3 +114,37 @@ def test_ab2_second_order(oracle_mod):
     error of startup Euler + AB2 falls by 4 when dt halves"""
     e1, e2, e3 = (_advected_error(oracle_mod, dt) for dt in (0.02, 0.01, 0.005))
     assert 3.6 < e1 / e2 < 4.4 and 3.6 < e2 / e3 < 4.4, (e1, e2, e3)
+
+
+def test_energy_diagnostics_and_running_mean(oracle_mod):
+    """cal_energy (NavierStokes.f90:222-263) and time_avg_vort (:670-683) for
+    omega = a cos x after one step: omega = a g cos x, u = (0, a g sin x),
+    g = exp(-nu dt); no mask, so the three penalisation terms are 0, and the
+    running mean after the first step is the field itself"""
+    n, a, nu, dt = 32, 0.7, 0.01, 1e-3
+    o = _flow(oracle_mod, n, nu=nu, dt_fixed=dt)
+    o.set_vortk(_packed(n, {(0, 2): 0.5 * a}))
+    o.run(1)
+    e = list(o.diag().energies)
+    dxdy, g = (TWO_PI / n) ** 2, np.exp(-nu * dt)
+    s2 = a * a * g * g * n * n / 2  # sum of omega^2 = sum of |u|^2 over the grid
+    ref = [dxdy * np.sqrt(s2), 0.5 * dxdy * s2, nu * dxdy * s2, 0.0, 0.0, 0.0]
+    assert np.allclose(e, ref, rtol=1e-14, atol=0.0)
+    X = np.arange(n) * TWO_PI / n
+    assert np.abs(o.field("vor_mean") - a * g * np.cos(X)[None, :]).max() < 1e-15
+
+
+def test_sponge_term_closed_form(oracle_mod):
+    """the vorticity sponge adds -F[chi_s omega] (cal_nlk NavierStokes.f90:
+    139-164); chi_s = c + d cos x with omega = cos x + cos 2y gives
+    chi_s omega = c cos x + c cos 2y + d/2 + d/2 cos 2x + d cos x cos 2y"""
+    n, c, d = 32, 2.0, 0.6
+    vk = _packed(n, {(0, 2): 0.5, (4, 0): 0.5})
+    o = _flow(oracle_mod, n, iSponge=3, eps_sponge=1.0)
+    X = np.meshgrid(np.arange(n) * TWO_PI / n, np.arange(n) * TWO_PI / n)[0]
+    o.set_field("mask_sponge", c + d * np.cos(X))
+    u, _ = o.cal_velocity(0.0, vk)
+    nlk, _ = o.cal_nlk(0.0, vk, u)
+    ref = _packed(n, {(5, 3): 0.375, (0, 2): -0.5 * c, (4, 0): -0.5 * c, (0, 0): -0.5 * d,
+                      (0, 4): -0.25 * d, (4, 2): -0.25 * d})
+    assert np.abs(nlk - ref).max() < 1e-13
