@@ -830,9 +830,6 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   const fsibe_params &p = *pp;
   if (!size_ok(p.nx, p.ny))
     return set_err(FSIBE_ESIZE, "grid %dx%d not supported (even sizes, nx/2 and ny = 2^a 3^b)", p.nx, p.ny);
-  if (p.iFLUSI == 1)
-    return set_err(FSIBE_EUNSUPPORTED, "iFLUSI=1 needs the host solid solver; drive the step with "
-                                       "fsibe_create_mask/evolve/get_forces and solve the beam on the host");
   if (p.iSponge == 4 || p.iSponge == 22)
     return set_err(FSIBE_EUNSUPPORTED, "iSponge=%d (velocity sponge) is not implemented", p.iSponge);
   if (p.iMeanVelocity == 11)
@@ -842,7 +839,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   if (p.iBeam == 1 && 2 * p.ns + 8 > kMaxTilePrims)
     return set_err(FSIBE_EUNSUPPORTED, "ns=%d: at most %d beam nodes", p.ns, (kMaxTilePrims - 8) / 2);
   if (!(p.eps > 0.0) || !(p.xl > 0.0) || !(p.yl > 0.0)) return set_err(FSIBE_EARG, "eps, xl, yl must be > 0");
-  if ((p.iSponge > 0 && p.iSponge < 4) && !(p.eps_sponge > 0.0))
+  if (((p.iSponge > 0 && p.iSponge < 4) || p.iSponge == 10) && !(p.eps_sponge > 0.0))
     return set_err(FSIBE_EARG, "vorticity sponge needs eps_sponge > 0 (params.f90:185-188)");
   fsibe_sim *s = new fsibe_sim();
   s->p = p;
@@ -1211,7 +1208,7 @@ static MaskBox mask_box(const fsibe_sim *s) {
 static int ensure_sponge(fsibe_sim *s) {
   if (s->sponge_done) return FSIBE_OK;
   s->sponge_done = true;
-  if (!s->sponge_on) return FSIBE_OK;
+  if (!s->sponge_on && s->p.iSponge != 10) return FSIBE_OK;
   std::vector<double> mfull;
   if (s->p.iSponge == 2) {
     mfull.assign((size_t)s->nx * s->ny, 0.0);
@@ -1328,12 +1325,14 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     CHK(velocity_only(s, time));
     prof_end(s, P_VEL);
   }
-  // K1: A = Fy^-1[W0], B = Fy^-1[i ky W0]
+  // the outflow sponge of iSponge = 10 acts inside the penalisation term
+  const bool sp10 = s->p.iSponge == 10 && s->penal_on && s->ms;
+  // K1: A = Fy^-1[W0], B = Fy^-1[i ky W0] (and Fy^-1[i ky psi] for iSponge = 10)
   {
-    double2 *outs[2] = {s->T[0], s->T[1]};
-    const int fsel[2] = {0, 4};
+    double2 *outs[3] = {s->T[0], s->T[1], s->T[4]};
+    const int fsel[3] = {0, 4, 2};
     prof_begin(s, P_COLS_W0);
-    CHK(cols_inv_fields(s, s->W[n0], outs, fsel, 2));
+    CHK(cols_inv_fields(s, s->W[n0], outs, fsel, sp10 ? 3 : 2));
     prof_end(s, P_COLS_W0);
   }
   MaskBox mb = mask_box(s);
@@ -1349,6 +1348,19 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     }));
     prof_end(s, P_ROWS_NL);
     if (s->pipe) HIPCHK(hipEventRecord(s->ev_dt, s->stream));
+  }
+  if (sp10) {
+    // NavierStokes.f90:108-109: ixmin = nint((xl - SpongeSize)/dx), ixmax = nx - 1 - 15
+    const int ixmin = (int)lround((s->p.xl - s->p.SpongeSize) / s->dx), ixmax = s->nx - 1 - 15;
+    if (ixmin < 0) return set_err(FSIBE_EARG, "iSponge=10: SpongeSize exceeds the domain");
+    if (ixmin <= ixmax) {
+      const size_t lds = sizeof(double2) * 2 * padlen(s->H);
+      CHK(with_rows<1>(s->px, [&](auto F, int nt) -> int {
+        KLAUNCH(k_rows_sponge10<decltype(F)>, s->ny, nt, lds, s->stream, s->T[4], s->ms, s->T[3], ixmin, ixmax, F,
+                s->twx, s->sc);
+        return FSIBE_OK;
+      }));
+    }
   }
   // K4
   {
@@ -1563,6 +1575,10 @@ static void init_beam_host(fsibe_sim *s) {
 
 extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
+  if (s->p.iFLUSI == 1)
+    return set_err(FSIBE_EUNSUPPORTED, "iFLUSI=1 needs the host solid solver: drive the step with "
+                                       "fsibe_create_mask / fsibe_evolve_* / fsibe_get_forces and solve the "
+                                       "beam on the host");
   if (!s->startup_done && s->time == 0.0 && s->p.iBeam > 0 && !s->have_beam) init_beam_host(s);
   const bool pipe = !s->prof.on;  // kernel timing needs a synchronisation per step
   int rc = FSIBE_OK;

@@ -186,6 +186,60 @@ __global__ void __launch_bounds__(FY::LB) k_cols_inv_fields(const double2 *__res
 }
 
 // ============================================================================
+// K2b (iSponge = 10 only): the outflow sponge of cal_nlk NavierStokes.f90:
+// 103-121.  Row y of Cpy = Fy^-1[i ky psi] gives du1/dx = Fx^-1[i kx Cpy];
+// w = -chi_s du1/dx; e(ix) = sum_{is=ixmin}^{ix-1} (w(is)+w(is+1))/2 dx for
+// ixmin <= ix <= ixmax (summed by one thread in the reference's order),
+// e = 0 elsewhere; then Rpx += Fx[e] (Rpx holds Fx[-chi (u1 - u_s)]).
+// ============================================================================
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_sponge10(const double2 *__restrict__ Cpy,
+                                                          const double *__restrict__ ms,
+                                                          double2 *__restrict__ Rpx, int ixmin, int ixmax,
+                                                          FX F, const double2 *__restrict__ twx, StepConsts c) {
+  extern __shared__ double2 lds[];
+  const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
+  const int y = blockIdx.x;
+  double2 *f0 = lds, *f1 = lds + PL;
+  for (int k = tid; k < H; k += nt) {
+    const double2 p = Cpy[(size_t)y * H + k];
+    const double kf = (double)k * c.sx;
+    f0[pidx(k)] = make_double2(-kf * p.y, kf * p.x);
+    f1[pidx(k)] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  rows_c2r_pre(lds, 1, H, PL, twx, tid, nt);
+  F.template fft<true, 1>(lds, twx, tid, nt);
+  for (int n = tid; n < H; n += nt) {
+#pragma clang fp contract(off)
+    const double2 d = f0[pidx(n)];
+    const double2 m = reinterpret_cast<const double2 *>(ms + (size_t)y * nx)[n];
+    f0[pidx(n)] = make_double2(-(m.x * d.x), -(m.y * d.y));
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma clang fp contract(off)
+    const double *w = reinterpret_cast<const double *>(f0);
+    double *e = reinterpret_cast<double *>(f1);
+    double summe = 0.0;
+    for (int ix = ixmin; ix <= ixmax; ++ix) {
+      e[2 * pidx(ix >> 1) + (ix & 1)] = summe;
+      const int i1 = ix + 1;
+      summe = summe + 0.5 * (w[2 * pidx(ix >> 1) + (ix & 1)] + w[2 * pidx(i1 >> 1) + (i1 & 1)]) * c.dx;
+    }
+  }
+  __syncthreads();
+  for (int n = tid; n < H; n += nt) f0[pidx(n)] = f1[pidx(n)];
+  __syncthreads();
+  F.template fft<false, 1>(lds, twx, tid, nt);
+  rows_r2c_post(lds, 1, H, PL, twx, tid, nt);
+  for (int k = tid; k < H; k += nt) {
+    const double2 a = Rpx[(size_t)y * H + k], b = f0[pidx(k)];
+    Rpx[(size_t)y * H + k] = make_double2(a.x + b.x, a.y + b.y);
+  }
+}
+
+// ============================================================================
 // K3: time step and AB2 weights (cal_nlk NavierStokes.f90:59-78,
 // EvolveFluidAB2 :488-489, vis cal_vis.f90:20-21); run by block 0 of K2,
 // whose inputs (the previous step's CFL maxima) are complete at launch

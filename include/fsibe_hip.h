@@ -198,7 +198,9 @@ int fsibe_integrate_position(const fsibe_params *p, int iMouvement, double time,
  * (iFLUSI=0, iMotion>0 or a fixed beam): the startup Euler step if not yet
  * taken, then AB2 steps, each create_mask -> evolve -> GetForces ->
  * integrate_position -> swap -> time += dt1.  Runs nsteps steps or until
- * Time_end.  out (may be NULL) receives the last step's results. */
+ * Time_end.  out (may be NULL) receives the last step's results.  iFLUSI=1
+ * (beam solved by the host's solid solver) is FSIBE_EUNSUPPORTED here: such
+ * a host calls the per-step entry points above itself. */
 int fsibe_run(fsibe_sim *sim, int nsteps, fsibe_step_out *out);
 /* current time, dt1 and number of AB2 steps taken by fsibe_run */
 int fsibe_get_time(fsibe_sim *sim, double *time, double *dt1, int *it);

@@ -1030,6 +1030,20 @@ void o_cal_nlk(o_sim *s, double time, double *dt1, double *nlk,
       pen1[k] = -(s->mask[k] * (u1[k] - s->maskvx[k]));
       pen2[k] = -(s->mask[k] * (u2[k] - s->maskvy[k]));
     }
+    if (p->iSponge == 10) { /* :103-121: trapezoid integral of -chi_s du1/dx along x */
+      const int nx = s->nx, ny = s->ny;
+      o_coftxy(s, u1, work2); o_cofdx(s, work2, penvortk); o_cofitxy(s, penvortk, work2);
+      for (size_t k = 0; k < N; k++) work2[k] = -(s->mask_sponge[k] * work2[k]);
+      const int ixmin = fnint((p->xl - p->SpongeSize) / dx), ixmax = nx - 1 - 15;
+      for (int iy = 0; iy < ny; iy++) {
+        const double *w = work2 + (size_t)iy * nx;
+        for (int ix = ixmin; ix <= ixmax; ix++) {
+          double summe = 0.0;
+          for (int is = ixmin; is <= ix - 1; is++) summe = summe + 0.5 * (w[is] + w[is + 1]) * dx;
+          pen1[(size_t)iy * nx + ix] = pen1[(size_t)iy * nx + ix] + summe;
+        }
+      }
+    }
     o_coftxy(s, pen2, work2); o_cofdx(s, work2, pen2);
     o_coftxy(s, pen1, work2); o_cofdy(s, work2, pen1);
   } else {
@@ -1431,6 +1445,28 @@ void o_ab2_step(o_sim *s) {
   int t = s->n1; s->n1 = s->n0; s->n0 = t;
   s->time = s->time + s->dt1;
   s->it++;
+}
+
+/* The per-step calls of time_step.f90:187-199 for a host that moves the beam
+ * itself (iFLUSI=1: its solid solver writes the beam arrays between steps). */
+void o_host_create_mask(o_sim *s, double time) {
+  if (!s->started) {
+    o_create_sponge_mask(s);
+    s->started = 1;
+  }
+  o_create_mask(s, time, &s->beam);
+}
+double o_host_evolve(o_sim *s, double time, double dt0, int explicit_step) {
+  double dt1 = 0.0;
+  if (explicit_step)
+    o_evolve_explicit(s, time, &dt1, s->n0, s->n1, &s->beam);
+  else
+    o_evolve_ab2(s, time, dt0, &dt1, s->n0, s->n1, &s->beam);
+  return dt1;
+}
+void o_host_get_forces(o_sim *s, double time) { o_get_forces(s, time, &s->beam, s->press, s->u); }
+void o_host_swap(o_sim *s) {
+  int t = s->n1; s->n1 = s->n0; s->n0 = t;
 }
 
 void o_run(o_sim *s, int nsteps) {

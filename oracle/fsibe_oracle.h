@@ -134,6 +134,11 @@ void o_set_vortk(o_sim *s, const double *vortk_P); /* into slab n0 */
 void o_startup_step(o_sim *s);
 void o_ab2_step(o_sim *s);
 void o_run(o_sim *s, int nsteps); /* startup (if needed) + AB2 steps */
+/* per-step calls for a host that moves the beam itself (iFLUSI=1) */
+void o_host_create_mask(o_sim *s, double time);
+double o_host_evolve(o_sim *s, double time, double dt0, int explicit_step);
+void o_host_get_forces(o_sim *s, double time);
+void o_host_swap(o_sim *s);
 
 /* ---- state access --------------------------------------------------------*/
 enum {

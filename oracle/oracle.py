@@ -119,6 +119,11 @@ def lib():
         L.o_delta_interp.argtypes = [vp, C.c_double, C.c_double, dp, C.POINTER(C.c_int)]
         L.o_get_forces.argtypes = [vp, C.c_double, C.POINTER(OBeam), dp, dp]
         L.o_set_vortk.argtypes = [vp, dp]
+        L.o_host_create_mask.argtypes = [vp, C.c_double]
+        L.o_host_evolve.restype = C.c_double
+        L.o_host_evolve.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+        L.o_host_get_forces.argtypes = [vp, C.c_double]
+        L.o_host_swap.argtypes = [vp]
         L.o_run.argtypes = [vp, C.c_int]
         L.o_startup_step.argtypes = [vp]
         L.o_ab2_step.argtypes = [vp]
@@ -292,6 +297,19 @@ class Oracle:
         (u: (2, ny, nx)); results land in the beam (pressure, tau, Force_press)."""
         self.L.o_get_forces(self.h, time, self.L.o_get_beam(self.h), self._as(press),
                             np.ascontiguousarray(u, dtype=np.float64))
+
+    # per-step calls for a host that moves the beam itself (iFLUSI=1)
+    def host_create_mask(self, time):
+        self.L.o_host_create_mask(self.h, time)
+
+    def host_evolve(self, time, dt0=0.0, explicit=False):
+        return self.L.o_host_evolve(self.h, time, dt0, int(explicit))
+
+    def host_get_forces(self, time):
+        self.L.o_host_get_forces(self.h, time)
+
+    def host_swap(self):
+        self.L.o_host_swap(self.h)
 
     # ---- solver ---------------------------------------------------------------
     def set_vortk(self, vk):
