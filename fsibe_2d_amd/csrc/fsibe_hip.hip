@@ -668,6 +668,7 @@ struct fsibe_sim {
   double *sums, *umax_part;
   DevScalars *dscal;
   unsigned int *d_done;  // block counter of k_forces (zero between launches)
+  double *d_part;        // per-unit partial force sums of k_forces
   double *mask_m, *mask_vx, *mask_vy;
   double *fb_p, *fb_sa, *fb_sb;
   // per-step upload area: prims | targets | node data | circle data | vals
@@ -808,7 +809,7 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   (void)hipStreamSynchronize(s->stream);
   void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
                   s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow, s->sums,
-                  s->umax_part, s->dscal, s->d_done, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
+                  s->umax_part, s->dscal, s->d_done, s->d_part, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
                   s->fb_sb, s->d_stage};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -918,6 +919,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipMalloc(&s->dscal, sizeof(DevScalars)));
   SIMCHK(hipMemset(s->dscal, 0, sizeof(DevScalars)));
   SIMCHK(hipMalloc(&s->d_done, sizeof(unsigned int)));
+  SIMCHK(hipMalloc(&s->d_part, sizeof(double) * 8 * (std::max(s->ns, 1) + kNC)));
   SIMCHK(hipMemset(s->d_done, 0, sizeof(unsigned int)));
   const int nsb0 = std::max(s->ns, 1);
   const size_t res_bytes = sizeof(StepOutDev) + 2 * sizeof(double) * nsb0;
@@ -1437,11 +1439,15 @@ static int launch_forces(fsibe_sim *s) {
   const double *tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
   double *val = reinterpret_cast<double *>(s->d_stage + s->off_val);
   if (s->ntg <= 0) return set_err(FSIBE_ESTATE, "no interpolation targets");
+  // one block per beam node and per pair of circle points
+  const int units = s->fg.ns + s->fg.nc;
+  if (s->ntg != 6 * s->fg.ns + 3 * 2 * s->fg.nc || s->fg.nc > kNC)
+    return set_err(FSIBE_ESTATE, "interpolation targets do not match the beam");
   prof_begin(s, P_INTERP);
-  k_forces<<<(s->ntg + kInterpPerBlock - 1) / kInterpPerBlock, kInterpThreads, 0, s->stream>>>(
-      tg, s->ntg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
+  k_forces<<<units, kForceThreads, 0, s->stream>>>(
+      tg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
       reinterpret_cast<const double *>(s->d_stage + s->off_cd), pbeam_dev(s, s->slot),
-      pbeam_dev(s, s->slot) + std::max(s->ns, 1), res_dev(s, s->slot), s->d_done);
+      pbeam_dev(s, s->slot) + std::max(s->ns, 1), s->d_part, res_dev(s, s->slot), s->d_done);
   HIPCHK(hipGetLastError());
   prof_end(s, P_INTERP);
   return FSIBE_OK;

@@ -44,118 +44,187 @@ __device__ __forceinline__ double angle_deg(double g, double pi) {
   return a * 180.0 / pi;
 }
 
-// One workgroup per 16x16 tile of the mask box.  The primitives whose boxes
-// meet the tile are first compacted, in drawing order, into an LDS list
-// (ballot + prefix count per chunk of 256), so each pixel visits only those.
+// One workgroup of four waves per 8x8 tile of the mask box (lane = pixel).
+// The bounding boxes of all primitives are first copied to LDS.  Then, in
+// passes over the primitive list, the ones whose boxes meet the tile are
+// compacted in drawing order (ballot) and staged in LDS; the four waves
+// evaluate the staged primitives' profile values at the tile's pixels in
+// parallel (the erf / sqrt / atan2 work, which does not depend on the running
+// maximum), and wave 0 applies them in drawing order with the reference's
+// replacement rule.  A value of -1 marks "primitive does not touch this
+// pixel" (every profile value is >= 0 and so is the running maximum); the
+// sharp end marks "not touched" with -2 and otherwise overwrites.
 constexpr int kMaskTile = 8;
-constexpr int kMaskThreads = kMaskTile * kMaskTile;
+constexpr int kMaskPix = kMaskTile * kMaskTile;  // pixels per tile = lanes of a wave
+constexpr int kMaskWaves = 4;
+constexpr int kMaskThreads = kMaskPix * kMaskWaves;
 constexpr int kMaxTilePrims = 1024;  // >= number of primitives (host checks)
+constexpr int kMaskStage = 32;       // primitives staged per pass
+
+// profile value of primitive P at (xi, yj), or -1 / -2 (see above)
+__device__ __forceinline__ double prim_value(const Prim &P, const MaskConsts &c, int i, int j, double xi,
+                                             double yj) {
+#pragma clang fp contract(off)
+  const double none = P.type == PRIM_SHARP ? -2.0 : -1.0;
+  if (i < P.ix0 || i > P.ix1 || j < P.iy0 || j > P.iy1) return none;
+  const double *a = P.a;
+  if (P.type == PRIM_SEG) {
+    // DrawBeamSegment :764-778; a = p1s(2) p2s(2) cos sin t_star v1(2) v2(2)
+    const double xs = xi * a[4] - yj * a[5];
+    const double ys = xi * a[5] + yj * a[4];
+    if (xs >= a[0] && xs <= a[2] && ys <= (a[1] + a[6]) && ys >= (a[3] - a[6]))
+      return smooth_step(c, fabs(ys - a[1]), c.t, c.h);
+    return -1.0;
+  }
+  if (P.type == PRIM_HINGE) {
+    // DrawHinge :595-653; a = px py beta1 beta2 t_star vx vy d1(2) d2(2)
+    const double ddx = xi - a[0], ddy = yj - a[1];
+    const double R = sqrt(ddx * ddx + ddy * ddy);
+    // The sector test below is "angle to segment 1 <= 90 deg and angle to
+    // segment 2 >= 90 deg", i.e. d1.v >= 0 and d2.v <= 0 with d1, d2 the
+    // unit directions of the two segments.  Pixels that fail it by more
+    // than 1e-9 R (far beyond the rounding of the degree comparisons) are
+    // skipped; the rest take the reference's exact atan2 test.
+    const double dot1 = a[8] * ddx + a[9] * ddy, dot2 = a[10] * ddx + a[11] * ddy;
+    if (!(R <= a[4]) || dot1 < -1e-9 * R || dot2 > 1e-9 * R) return -1.0;
+    const double gamma = angle_deg(atan2(yj - a[1], xi - a[0]), c.pi);
+    const double b1 = a[2], b2 = a[3];
+    bool in1 = false, in2 = false;
+    if (b1 < 90.0) {
+      in1 = (gamma >= 0.0 && gamma <= 90.0 + b1) || (gamma >= 270.0 + b1 && gamma <= 360.0);
+    } else if (b1 >= 90.0 && b1 < 270.0) {
+      in1 = (gamma >= 270.0 + b1 - 360.0 && gamma <= 90.0 + b1);
+    } else if (b1 >= 270.0) {
+      in1 = (gamma >= 0.0 && gamma <= 90.0 + b1 - 360.0) || (gamma >= 270.0 + b1 - 360.0 && gamma <= 360.0);
+    }
+    if (b2 <= 90.0) {
+      in2 = (gamma >= 90.0 + b2 && gamma < 270.0 + b2);
+    } else if (b2 >= 90.0 && b2 < 270.0) {
+      in2 = (gamma >= 0.0 && gamma <= 270.0 + b2 - 360.0) || (gamma >= 90.0 + b2 && gamma <= 360.0);
+    } else if (b2 >= 270.0) {
+      in2 = (gamma >= 90.0 + b2 - 360.0 && gamma < 270.0 + b2 - 360.0);
+    }
+    return (in1 && in2) ? smooth_step(c, R, c.t, c.h) : -1.0;
+  }
+  if (P.type == PRIM_DOT) {
+    // DrawADot :690-701; a = px py vx vy t
+    const double ddx = xi - a[0], ddy = yj - a[1];
+    const double R = sqrt(ddx * ddx + ddy * ddy);
+    return smooth_step(c, R, a[4], c.h);
+  }
+  // PRIM_SHARP, DrawSharpEnd :825-847; a = p1s(2) p2s(2) cos sin t_star vx vy mdx
+  const double xs = xi * a[4] + yj * a[5];
+  const double ys = -(xi * a[5]) + yj * a[4];
+  if (xs >= a[0] && xs <= a[2] && ys <= (a[1] + a[6]) && ys >= (a[3] - a[6])) {
+    const double tmp2 = smooth_step(c, xs - a[0], 2.0 * a[9], c.h);
+    const double temp = smooth_step(c, fabs(ys - a[1]), c.t, c.h);
+    return temp * tmp2;
+  }
+  return -2.0;
+}
 
 __global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ prims, int np,
                                               const double *__restrict__ wallrow, MaskConsts c, int bx0, int by0,
                                               int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
                                               double *__restrict__ ovy) {
 #pragma clang fp contract(off)
-  __shared__ int list[kMaxTilePrims];
-  __shared__ int wcount[kMaskThreads / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  __shared__ int4 bnd[kMaxTilePrims];  // ix0, ix1, iy0, iy1
+  __shared__ Prim sp[kMaskStage];
+  __shared__ int list[kMaskStage];
+  __shared__ double val[kMaskStage][kMaskPix];
+  __shared__ int nstage, pnext;
+  const int tid = threadIdx.x, lane = tid & (kMaskPix - 1), wv = tid / kMaskPix;
   const int tx0 = bx0 + blockIdx.x * kMaskTile, ty0 = by0 + blockIdx.y * kMaskTile;
   const int tx1 = tx0 + kMaskTile - 1, ty1 = ty0 + kMaskTile - 1;
-  int nlist = 0;
-  for (int c0 = 0; c0 < np; c0 += kMaskThreads) {
-    const int p = c0 + tid;
-    bool f = false;
-    if (p < np) {
-      const Prim &P = prims[p];
-      f = !(P.ix1 < tx0 || P.ix0 > tx1 || P.iy1 < ty0 || P.iy0 > ty1);
-    }
-    const unsigned long long b = __ballot(f);
-    const int pre = __popcll(b & ((1ull << lane) - 1ull));
-    if (lane == 0) wcount[wid] = __popcll(b);
-    __syncthreads();
-    int off = nlist;
-    for (int w = 0; w < wid; ++w) off += wcount[w];
-    if (f && off + pre < kMaxTilePrims) list[off + pre] = p;
-    int tot = 0;
-    for (int w = 0; w < kMaskThreads / 64; ++w) tot += wcount[w];
-    __syncthreads();
-    nlist += tot;
+  for (int p = tid; p < np; p += kMaskThreads) {
+    const int4 h = reinterpret_cast<const int4 *>(prims + p)[0];  // type ix0 ix1 iy0
+    bnd[p] = make_int4(h.y, h.z, h.w, prims[p].iy1);
   }
-  nlist = nlist < kMaxTilePrims ? nlist : kMaxTilePrims;
-  const int lx = blockIdx.x * kMaskTile + (tid & (kMaskTile - 1));
-  const int ly = blockIdx.y * kMaskTile + (tid / kMaskTile);
-  if (lx >= bw || ly >= bh) return;
+  if (tid == 0) pnext = 0;
+  __syncthreads();
+  const int lx = blockIdx.x * kMaskTile + (lane & (kMaskTile - 1));
+  const int ly = blockIdx.y * kMaskTile + (lane / kMaskTile);
+  const bool valid = lx < bw && ly < bh;
   const int i = bx0 + lx, j = by0 + ly;
-  double m = wallrow ? wallrow[j] : 0.0, mvx = 0.0, mvy = 0.0;
   const double xi = (double)i * c.dx, yj = (double)j * c.dy;
-  for (int q = 0; q < nlist; ++q) {
-    // every lane of the wave reads the same entry: keep it in a scalar register
-    const Prim &P = prims[__builtin_amdgcn_readfirstlane(list[q])];
-    if (i < P.ix0 || i > P.ix1 || j < P.iy0 || j > P.iy1) continue;
-    const double *a = P.a;
-    if (P.type == PRIM_SEG) {
-      // DrawBeamSegment :764-778; a = p1s(2) p2s(2) cos sin t_star v1(2) v2(2)
-      const double xs = xi * a[4] - yj * a[5];
-      const double ys = xi * a[5] + yj * a[4];
-      if (xs >= a[0] && xs <= a[2] && ys <= (a[1] + a[6]) && ys >= (a[3] - a[6])) {
-        const double temp = smooth_step(c, fabs(ys - a[1]), c.t, c.h);
-        if (temp > m) {
-          m = temp;
-          mvx = (a[7] + (xs - a[0]) * (a[9] - a[7]) / c.ds);
-          mvy = (a[8] + (xs - a[0]) * (a[10] - a[8]) / c.ds);
+  double m = (valid && wallrow) ? wallrow[j] : 0.0, mvx = 0.0, mvy = 0.0;
+  while (true) {
+    // wave 0 compacts the next primitives that meet the tile, in order
+    if (wv == 0) {
+      int p0 = pnext, n = 0;
+      while (p0 < np) {
+        const int p = p0 + lane;
+        bool f = false;
+        if (p < np) {
+          const int4 b = bnd[p];
+          f = !(b.y < tx0 || b.x > tx1 || b.w < ty0 || b.z > ty1);
         }
+        const unsigned long long bal = __ballot(f);
+        const int cnt = __popcll(bal);
+        if (n + cnt > kMaskStage) {  // take the first kMaskStage - n of this group
+          const int take = kMaskStage - n;
+          const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+          if (f && pre < take) list[n + pre] = p;
+          // resume after the last primitive taken
+          unsigned long long bb = bal;
+          int lastlane = -1;
+          for (int k = 0; k < take; ++k) {
+            lastlane = __ffsll((long long)bb) - 1;
+            bb &= bb - 1ull;
+          }
+          p0 = p0 + lastlane + 1;
+          n = kMaskStage;
+          break;
+        }
+        if (f) list[n + __popcll(bal & ((1ull << lane) - 1ull))] = p;
+        n += cnt;
+        p0 += kMaskPix;
       }
-    } else if (P.type == PRIM_HINGE) {
-      // DrawHinge :595-653; a = px py beta1 beta2 t_star vx vy d1(2) d2(2)
-      const double ddx = xi - a[0], ddy = yj - a[1];
-      const double R = sqrt(ddx * ddx + ddy * ddy);
-      // The sector test below is "angle to segment 1 <= 90 deg and angle to
-      // segment 2 >= 90 deg", i.e. d1.v >= 0 and d2.v <= 0 with d1, d2 the
-      // unit directions of the two segments.  Pixels that fail it by more
-      // than 1e-9 R (far beyond the rounding of the degree comparisons) are
-      // skipped; the rest take the reference's exact atan2 test.
-      const double dot1 = a[8] * ddx + a[9] * ddy, dot2 = a[10] * ddx + a[11] * ddy;
-      if (R <= a[4] && !(dot1 < -1e-9 * R || dot2 > 1e-9 * R)) {
-        const double gamma = angle_deg(atan2(yj - a[1], xi - a[0]), c.pi);
-        const double b1 = a[2], b2 = a[3];
-        bool in1 = false, in2 = false;
-        if (b1 < 90.0) {
-          in1 = (gamma >= 0.0 && gamma <= 90.0 + b1) || (gamma >= 270.0 + b1 && gamma <= 360.0);
-        } else if (b1 >= 90.0 && b1 < 270.0) {
-          in1 = (gamma >= 270.0 + b1 - 360.0 && gamma <= 90.0 + b1);
-        } else if (b1 >= 270.0) {
-          in1 = (gamma >= 0.0 && gamma <= 90.0 + b1 - 360.0) || (gamma >= 270.0 + b1 - 360.0 && gamma <= 360.0);
-        }
-        if (b2 <= 90.0) {
-          in2 = (gamma >= 90.0 + b2 && gamma < 270.0 + b2);
-        } else if (b2 >= 90.0 && b2 < 270.0) {
-          in2 = (gamma >= 0.0 && gamma <= 270.0 + b2 - 360.0) || (gamma >= 90.0 + b2 && gamma <= 360.0);
-        } else if (b2 >= 270.0) {
-          in2 = (gamma >= 90.0 + b2 - 360.0 && gamma < 270.0 + b2 - 360.0);
-        }
-        if (in1 && in2) {
-          const double temp = smooth_step(c, R, c.t, c.h);
-          if (temp > m) { m = temp; mvx = a[5]; mvy = a[6]; }
-        }
-      }
-    } else if (P.type == PRIM_DOT) {
-      // DrawADot :690-701; a = px py vx vy t
-      const double ddx = xi - a[0], ddy = yj - a[1];
-      const double R = sqrt(ddx * ddx + ddy * ddy);
-      const double temp = smooth_step(c, R, a[4], c.h);
-      if (temp > m) { m = temp; mvx = a[2]; mvy = a[3]; }
-    } else if (P.type == PRIM_SHARP) {
-      // DrawSharpEnd :825-847; a = p1s(2) p2s(2) cos sin t_star vx vy mdx
-      const double xs = xi * a[4] + yj * a[5];
-      const double ys = -(xi * a[5]) + yj * a[4];
-      if (xs >= a[0] && xs <= a[2] && ys <= (a[1] + a[6]) && ys >= (a[3] - a[6])) {
-        const double tmp2 = smooth_step(c, xs - a[0], 2.0 * a[9], c.h);
-        const double temp = smooth_step(c, fabs(ys - a[1]), c.t, c.h);
-        m = temp * tmp2;
-        if (xs - a[0] >= 2.0 * a[9]) { mvx = a[7]; mvy = a[8]; }
+      if (lane == 0) {
+        nstage = n;
+        pnext = p0 < np ? p0 : np;
       }
     }
+    __syncthreads();
+    // read once per pass: wave 0 rewrites both at the start of the next pass
+    const int n = nstage, pn = pnext;
+    if (n == 0) break;
+    for (int e = tid; e < n * (int)(sizeof(Prim) / 8); e += kMaskThreads) {
+      const int q = e / (int)(sizeof(Prim) / 8), w = e - q * (int)(sizeof(Prim) / 8);
+      reinterpret_cast<double *>(sp + q)[w] = reinterpret_cast<const double *>(prims + list[q])[w];
+    }
+    __syncthreads();
+    for (int q = wv; q < n; q += kMaskWaves) val[q][lane] = prim_value(sp[q], c, i, j, xi, yj);
+    __syncthreads();
+    if (wv == 0 && valid) {
+      for (int q = 0; q < n; ++q) {
+        const Prim &P = sp[q];
+        const double v = val[q][lane];
+        const double *a = P.a;
+        if (P.type == PRIM_SHARP) {
+          if (v > -1.5) {
+            m = v;
+            const double xs = xi * a[4] + yj * a[5];
+            if (xs - a[0] >= 2.0 * a[9]) { mvx = a[7]; mvy = a[8]; }
+          }
+        } else if (v > m) {
+          m = v;
+          if (P.type == PRIM_SEG) {
+            const double xs = xi * a[4] - yj * a[5];
+            mvx = (a[7] + (xs - a[0]) * (a[9] - a[7]) / c.ds);
+            mvy = (a[8] + (xs - a[0]) * (a[10] - a[8]) / c.ds);
+          } else if (P.type == PRIM_HINGE) {
+            mvx = a[5]; mvy = a[6];
+          } else {
+            mvx = a[2]; mvy = a[3];
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next pass overwrites list, sp and val
+    if (pn >= np) break;
   }
+  if (wv != 0 || !valid) return;
   const size_t q = (size_t)ly * bw + lx;
   om[q] = m / c.eps;  // :210-214
   ovx[q] = mvx;

@@ -735,25 +735,34 @@ __device__ __forceinline__ double phi4star(double x, double dx, double pi) {
   return 0.0;
 }
 
-// targets: tg[3*t] = x, tg[3*t+1] = y, tg[3*t+2] = field id (0 press, 1 sa, 2 sb).
-// DeltaInterpolation (Interpolation.f90:26-44).  One block of 64 threads
-// handles 7 targets: thread (t, a) evaluates the kernel weights wx[a], wy[a]
-// and the 9 products of stencil column a; one thread per target then adds
-// the 81 products in the reference's order (ix outer, iy inner).
-constexpr int kInterpThreads = 64;
-constexpr int kInterpPerBlock = 7;
+// targets: tg[3*t] = x, tg[3*t+1] = y, tg[3*t+2] = field id (0 press, 1 sa, 2 sb);
+// node n owns targets 6n .. 6n+5 (P(u1) P(b1) A(u) B(u) A(b) B(b)), circle
+// point i (trailing i < nc, leading nc <= i < 2nc) owns 6ns + 3i + {0 A, 1 B, 2 P}.
+// node data nd[4n + ...]: sin th, cos th, sin 2th, cos 2th; circle data
+// cd[4i + ...]: cos phi, sin phi, cos 2phi, sin 2phi.
+//
+// One block of 64 threads per unit: a beam node (6 targets) or a pair of
+// circle points (6 targets).  Thread (t, a) evaluates the kernel weights
+// wx[a], wy[a] and the 9 products of stencil column a (DeltaInterpolation,
+// Interpolation.f90:26-44); one thread per target adds the 81 products in
+// the reference's order (ix outer, iy inner).  Thread 0 then forms the
+// unit's loads and its terms of the eight force sums (GetForcesInterp
+// BeamForces.f90:204-267, BeamSurfaceIntegral :354-480) and writes them to
+// part[8 unit + k]; the last block to finish adds the parts in unit order.
+constexpr int kForceThreads = 64;
 
-__device__ void interp_targets(const double *__restrict__ tg, int nt, const double *__restrict__ press,
-                               const double *__restrict__ sa, const double *__restrict__ sb, const ForceGeom &g,
-                               double *__restrict__ val) {
+__device__ __forceinline__ void interp6(const double *__restrict__ tg, int t0, int ntu,
+                                        const double *__restrict__ press, const double *__restrict__ sa,
+                                        const double *__restrict__ sb, const ForceGeom &g, double *vals,
+                                        double *__restrict__ val) {
 #pragma clang fp contract(off)
-  __shared__ double wxs[kInterpPerBlock][9], wys[kInterpPerBlock][9], prod[kInterpPerBlock][81];
+  __shared__ double wxs[6][9], wys[6][9], prod[6][81];
   const int lt = threadIdx.x / 9, a = threadIdx.x - 9 * lt;
-  const int t = blockIdx.x * kInterpPerBlock + lt;
-  const bool act = lt < kInterpPerBlock && t < nt;
+  const bool act = lt < ntu;
   int i = 0, j = 0;
   const double *f = press;
   if (act) {
+    const int t = t0 + lt;
     const double xt = tg[3 * t], yt = tg[3 * t + 1];
     const int fid = (int)tg[3 * t + 2];
     f = fid == 0 ? press : fid == 1 ? sa : sb;
@@ -773,107 +782,118 @@ __device__ void interp_targets(const double *__restrict__ tg, int nt, const doub
   if (act && a == 0) {
     double R1 = 0.0;
     for (int q = 0; q < 81; ++q) R1 = R1 + prod[lt][q];
-    val[t] = R1;
+    vals[lt] = R1;
+    val[t0 + lt] = R1;
   }
+  __syncthreads();
 }
 
-// node data nd[4*n + ...]: 0 sin th, 1 cos th, 2 sin 2th, 3 cos 2th
-// circle data cd[4*i + ...] (trailing i < nc, leading nc <= i < 2nc):
-//   0 cos phi, 1 sin phi, 2 cos 2phi, 3 sin 2phi
-// interpolated values val: node n: 6n+0 P(u1) 6n+1 P(b1) 6n+2 A(u) 6n+3 B(u)
-//   6n+4 A(b) 6n+5 B(b); circle point i: 6ns + 3i + {0 A, 1 B, 2 P}
-// One workgroup: every node / circle point is a term, the eight running
-// sums of the reference (fx, fy, Ft, Fn, Fp) become block tree sums.
-__device__ void forces_final(const double *__restrict__ nd, const double *__restrict__ cd,
-                             const double *__restrict__ val, const ForceGeom &g, double *__restrict__ pbeam,
-                             double *__restrict__ tbeam, StepOutDev *__restrict__ out) {
+__global__ void __launch_bounds__(kForceThreads) k_forces(const double *__restrict__ tg, const double *__restrict__ press,
+                                                          const double *__restrict__ sa, const double *__restrict__ sb,
+                                                          ForceGeom g, double *__restrict__ val,
+                                                          const double *__restrict__ nd, const double *__restrict__ cd,
+                                                          double *__restrict__ pbeam, double *__restrict__ tbeam,
+                                                          double *__restrict__ part, StepOutDev *__restrict__ out,
+                                                          unsigned int *__restrict__ done) {
 #pragma clang fp contract(off)
-  __shared__ double red[8 * 16];
-  const int ns = g.ns;
-  double acc[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) acc[i] = 0.0;
-  // acc: 0 fx, 1 fy, 2 Ft0, 3 Ft1, 4 Fn0, 5 Fn1, 6 Fp0, 7 Fp1
-  for (int n = threadIdx.x; n < ns; n += blockDim.x) {
-    const double st = nd[4 * n], ct = nd[4 * n + 1], s2 = nd[4 * n + 2], c2 = nd[4 * n + 3];
-    double pb = val[6 * n] - val[6 * n + 1];
-    double tb = 0.0;
-    const double A1 = val[6 * n + 2], B1 = val[6 * n + 3], A2 = val[6 * n + 4], B2 = val[6 * n + 5];
-    if (g.visc) {  // BeamForces.f90:206-215, :237
-      const double sn1 = -(A1 * c2) - B1 * s2, st1 = -(A1 * s2) + B1 * c2;
-      const double sn2 = -(A2 * c2) - B2 * s2, st2 = -(A2 * s2) + B2 * c2;
-      tb = st1 - st2;
-      pb = pb + (-sn1 + sn2);
-    }
-    pb = pb * g.soft;  // :254-255
-    tb = tb * g.soft;
-    pbeam[n] = pb;
-    tbeam[n] = tb;
-    acc[0] -= (st * pb + ct * tb) * g.ds;  // :265-266
-    acc[1] += (ct * pb + st * tb) * g.ds;
-    if (g.bsi_on) {
-      {  // top side, :433-452
-        const double A = A1, B = B1, P = val[6 * n];
-        const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
-        acc[2] += g.ds * stt * (+ct); acc[3] += g.ds * stt * (+st);
-        acc[4] += g.ds * sn * (-st);  acc[5] += g.ds * sn * (+ct);
-        acc[6] += g.ds * spp * (-st); acc[7] += g.ds * spp * (+ct);
-      }
-      {  // bottom side, :462-480
-        const double A = A2, B = B2, P = val[6 * n + 1];
-        const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
-        acc[2] += g.ds * stt * (-ct); acc[3] += g.ds * stt * (-st);
-        acc[4] += g.ds * sn * (+st);  acc[5] += g.ds * sn * (-ct);
-        acc[6] += g.ds * spp * (+st); acc[7] += g.ds * spp * (-ct);
-      }
-    }
-  }
-  if (g.bsi_on) {
-    for (int i = threadIdx.x; i < 2 * g.nc; i += blockDim.x) {  // half circles, :354-420
-      const double cp = cd[4 * i], sp = cd[4 * i + 1], c2 = cd[4 * i + 2], s2 = cd[4 * i + 3];
-      const double A = val[6 * ns + 3 * i], B = val[6 * ns + 3 * i + 1], P = val[6 * ns + 3 * i + 2];
-      const double sn = +A * c2 + B * s2, stt = -(A * s2) + B * c2, spp = -P;
-      acc[2] += g.dL * stt * (-sp); acc[3] += g.dL * stt * (+cp);
-      acc[4] += g.dL * sn * (+cp);  acc[5] += g.dL * sn * (+sp);
-      acc[6] += g.dL * spp * (+cp); acc[7] += g.dL * spp * (+sp);
-    }
-  }
-  block_sum<8>(acc, red);
-  if (threadIdx.x == 0) {
-    out->Force_press[0] = acc[0];
-    out->Force_press[1] = acc[1];
-    out->fx_int = acc[0];
-    out->fy_int = acc[1];
-    if (g.bsi_on) {
-      out->bsi[2] = acc[2] + acc[4] + acc[6];
-      out->bsi[3] = acc[3] + acc[5] + acc[7];
-      out->bsi[4] = acc[2]; out->bsi[5] = acc[4]; out->bsi[6] = acc[6];
-      out->bsi[7] = acc[3]; out->bsi[8] = acc[5]; out->bsi[9] = acc[7];
-    }
-  }
-  __threadfence_system();
-}
-
-// Delta interpolation of every target; the last block to finish (counter in
-// *done, reset by that block) forms the beam loads and force sums.
-__global__ void __launch_bounds__(kInterpThreads) k_forces(const double *__restrict__ tg, int nt,
-                                                           const double *__restrict__ press,
-                                                           const double *__restrict__ sa,
-                                                           const double *__restrict__ sb, ForceGeom g,
-                                                           double *__restrict__ val, const double *__restrict__ nd,
-                                                           const double *__restrict__ cd, double *__restrict__ pbeam,
-                                                           double *__restrict__ tbeam, StepOutDev *__restrict__ out,
-                                                           unsigned int *__restrict__ done) {
+  __shared__ double vals[6];
+  __shared__ double tot[8];
+  __shared__ double red[8][kForceThreads];
   __shared__ unsigned int last;
-  interp_targets(tg, nt, press, sa, sb, g, val);
+  const int u = blockIdx.x, ns = g.ns;
+  const bool node = u < ns;
+  const int t0 = node ? 6 * u : 6 * ns + 6 * (u - ns);
+  interp6(tg, t0, 6, press, sa, sb, g, vals, val);
+  if (threadIdx.x == 0) {
+    // acc: 0 fx, 1 fy, 2 Ft0, 3 Ft1, 4 Fn0, 5 Fn1, 6 Fp0, 7 Fp1
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (node) {
+      const double st = nd[4 * u], ct = nd[4 * u + 1], s2 = nd[4 * u + 2], c2 = nd[4 * u + 3];
+      double pb = vals[0] - vals[1];
+      double tb = 0.0;
+      const double A1 = vals[2], B1 = vals[3], A2 = vals[4], B2 = vals[5];
+      if (g.visc) {  // BeamForces.f90:206-215, :237
+        const double sn1 = -(A1 * c2) - B1 * s2, st1 = -(A1 * s2) + B1 * c2;
+        const double sn2 = -(A2 * c2) - B2 * s2, st2 = -(A2 * s2) + B2 * c2;
+        tb = st1 - st2;
+        pb = pb + (-sn1 + sn2);
+      }
+      pb = pb * g.soft;  // :254-255
+      tb = tb * g.soft;
+      pbeam[u] = pb;
+      tbeam[u] = tb;
+      acc[0] = -((st * pb + ct * tb) * g.ds);  // :265-266
+      acc[1] = (ct * pb + st * tb) * g.ds;
+      if (g.bsi_on) {
+        {  // top side, :433-452
+          const double A = A1, B = B1, P = vals[0];
+          const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
+          acc[2] += g.ds * stt * (+ct); acc[3] += g.ds * stt * (+st);
+          acc[4] += g.ds * sn * (-st);  acc[5] += g.ds * sn * (+ct);
+          acc[6] += g.ds * spp * (-st); acc[7] += g.ds * spp * (+ct);
+        }
+        {  // bottom side, :462-480
+          const double A = A2, B = B2, P = vals[1];
+          const double sn = -(A * c2) - B * s2, stt = -(A * s2) + B * c2, spp = -P;
+          acc[2] += g.ds * stt * (-ct); acc[3] += g.ds * stt * (-st);
+          acc[4] += g.ds * sn * (+st);  acc[5] += g.ds * sn * (-ct);
+          acc[6] += g.ds * spp * (+st); acc[7] += g.ds * spp * (-ct);
+        }
+      }
+    } else {
+      for (int k = 0; k < 2; ++k) {  // half circles, :354-420
+        const int i = 2 * (u - ns) + k;
+        const double cp = cd[4 * i], sp = cd[4 * i + 1], c2 = cd[4 * i + 2], s2 = cd[4 * i + 3];
+        const double A = vals[3 * k], B = vals[3 * k + 1], P = vals[3 * k + 2];
+        const double sn = +A * c2 + B * s2, stt = -(A * s2) + B * c2, spp = -P;
+        acc[2] += g.dL * stt * (-sp); acc[3] += g.dL * stt * (+cp);
+        acc[4] += g.dL * sn * (+cp);  acc[5] += g.dL * sn * (+sp);
+        acc[6] += g.dL * spp * (+cp); acc[7] += g.dL * spp * (+sp);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) part[8 * u + k] = acc[k];
+  }
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1u : 0u;
   __syncthreads();
   if (!last) return;
   __threadfence();
-  forces_final(nd, cd, val, g, pbeam, tbeam, out);
-  if (threadIdx.x == 0) *done = 0u;
+  {
+    // thread t adds units t, t + 64, t + 128, ... (independent loads), then
+    // threads 0..7 add the 64 thread sums of one quantity in thread order
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int v = threadIdx.x; v < (int)gridDim.x; v += kForceThreads) {
+      const double4 lo = reinterpret_cast<const double4 *>(part + 8 * v)[0];
+      const double4 hi = reinterpret_cast<const double4 *>(part + 8 * v)[1];
+      acc[0] += lo.x; acc[1] += lo.y; acc[2] += lo.z; acc[3] += lo.w;
+      acc[4] += hi.x; acc[5] += hi.y; acc[6] += hi.z; acc[7] += hi.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double sum = 0.0;
+    for (int t = 0; t < kForceThreads; ++t) sum += red[threadIdx.x][t];
+    tot[threadIdx.x] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out->Force_press[0] = tot[0];
+    out->Force_press[1] = tot[1];
+    out->fx_int = tot[0];
+    out->fy_int = tot[1];
+    if (g.bsi_on) {
+      out->bsi[2] = tot[2] + tot[4] + tot[6];
+      out->bsi[3] = tot[3] + tot[5] + tot[7];
+      out->bsi[4] = tot[2]; out->bsi[5] = tot[4]; out->bsi[6] = tot[6];
+      out->bsi[7] = tot[3]; out->bsi[8] = tot[5]; out->bsi[9] = tot[7];
+    }
+    __threadfence_system();
+    *done = 0u;
+  }
 }
 
 }  // namespace fsibe
