@@ -30,11 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# kernel classes of fsibe_profile_read (4 and 12 are no longer launched: the
-# dt/AB2 scalars run inside rows_nonlinear, the reductions inside
-# cols_pressure, the force sums inside the Delta interpolation kernel)
+# kernel classes of fsibe_profile_read (4, 8 and 12 are not launched: the
+# dt/AB2 scalars run inside rows_nonlinear, and the host adds the row sums
+# and per-node force terms the kernels leave in the pinned result record)
 KERNELS = ["create_mask", "startup_velocity", "cols_vortk_n0", "rows_nonlinear", "unused4",
-           "cols_ab2_update", "cols_vortk_n1", "rows_velocity_pressure", "reductions",
+           "cols_ab2_update", "cols_vortk_n1", "rows_velocity_pressure", "unused8",
            "cols_pressure", "rows_pressure_box", "delta_interp_forces", "unused12"]
 
 

@@ -665,10 +665,8 @@ struct fsibe_sim {
   double2 *twx, *twy;
   double2 *W[2], *NL[2], *T[7];
   double *u1, *u2, *vor_mean, *ms, *wallrow;
-  double *sums, *umax_part;
+  double *umax_part;
   DevScalars *dscal;
-  unsigned int *d_done;  // block counter of k_forces (zero between launches)
-  double *d_part;        // per-unit partial force sums of k_forces
   double *mask_m, *mask_vx, *mask_vy;
   double *fb_p, *fb_sa, *fb_sb;
   // per-step upload area: prims | targets | node data | circle data | vals
@@ -683,6 +681,13 @@ struct fsibe_sim {
   bool res_pending = false;  // pipelined run: results of the previous step not yet read
   bool pipe = false;         // record ev_dt after the nonlinear-term kernel
   hipEvent_t ev_dt;
+  // what each slot holds: the fluid step's row sums, the force terms
+  bool slot_fluid[2] = {false, false}, slot_forces[2] = {false, false};
+  int slot_units[2] = {0, 0};
+  // cal_drag's previous penalisation sums (type Solid drag_unst_old,
+  // lift_unst_old, UnsteadyCorrectionsReady; NavierStokes.f90:579-589)
+  double drag_unst_old = 0.0, lift_unst_old = 0.0;
+  bool unst_ready = false;
   int n0 = 0, n1 = 1;
   // geometry of the current step (set by create_mask)
   bool have_beam = false, sponge_done = false;
@@ -732,6 +737,14 @@ static StepOutDev *res_dev(const fsibe_sim *s, int k) {
 }
 static double *pbeam_dev(const fsibe_sim *s, int k) {
   return reinterpret_cast<double *>(s->d_res + (size_t)k * s->res_stride + sizeof(StepOutDev));
+}
+static size_t rows_off(const fsibe_sim *s) { return sizeof(StepOutDev) + sizeof(double) * 2 * std::max(s->ns, 1); }
+static size_t parts_off(const fsibe_sim *s) { return rows_off(s) + sizeof(double) * (size_t)s->ny * kNSUM; }
+static double *rows_dev(const fsibe_sim *s, int k) {
+  return reinterpret_cast<double *>(s->d_res + (size_t)k * s->res_stride + rows_off(s));
+}
+static double *parts_dev(const fsibe_sim *s, int k) {
+  return reinterpret_cast<double *>(s->d_res + (size_t)k * s->res_stride + parts_off(s));
 }
 
 static double smooth_step_host(const fsibe_sim *s, double x, double t, double h) {
@@ -808,8 +821,8 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   if (!s) return FSIBE_OK;
   (void)hipStreamSynchronize(s->stream);
   void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
-                  s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow, s->sums,
-                  s->umax_part, s->dscal, s->d_done, s->d_part, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
+                  s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow,
+                  s->umax_part, s->dscal, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
                   s->fb_sb, s->d_stage};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -913,16 +926,14 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipMemset(s->u1, 0, sizeof(double) * N));
   SIMCHK(hipMemset(s->u2, 0, sizeof(double) * N));
   SIMCHK(hipMemset(s->vor_mean, 0, sizeof(double) * N));
-  SIMCHK(hipMalloc(&s->sums, sizeof(double) * s->ny * kNSUM));
   SIMCHK(hipMalloc(&s->umax_part, sizeof(double) * s->ny));
   SIMCHK(hipMemset(s->umax_part, 0, sizeof(double) * s->ny));
   SIMCHK(hipMalloc(&s->dscal, sizeof(DevScalars)));
   SIMCHK(hipMemset(s->dscal, 0, sizeof(DevScalars)));
-  SIMCHK(hipMalloc(&s->d_done, sizeof(unsigned int)));
-  SIMCHK(hipMalloc(&s->d_part, sizeof(double) * 8 * (std::max(s->ns, 1) + kNC)));
-  SIMCHK(hipMemset(s->d_done, 0, sizeof(unsigned int)));
   const int nsb0 = std::max(s->ns, 1);
-  const size_t res_bytes = sizeof(StepOutDev) + 2 * sizeof(double) * nsb0;
+  // slot: StepOutDev | p_beam[nsb] | tau_beam[nsb] | row sums[ny][kNSUM] | force terms[8][nsb + kNC]
+  const size_t res_bytes = sizeof(StepOutDev) + sizeof(double) * (2 * nsb0 + (size_t)s->ny * kNSUM +
+                                                                   8 * (size_t)(nsb0 + kNC));
   s->res_stride = (res_bytes + 255) & ~(size_t)255;
   SIMCHK(hipHostMalloc((void **)&s->h_res, 2 * s->res_stride, hipHostMallocMapped | hipHostMallocCoherent));
   memset(s->h_res, 0, 2 * s->res_stride);
@@ -1311,7 +1322,8 @@ static int velocity_only(fsibe_sim *s, double time) {
   MaskBox mb = mask_box(s);
   return with_rows<3>(s->px, [&](auto F, int nt) -> int {
     KLAUNCH(k_rows_press<decltype(F)>, s->ny, nt, lds, s->stream, nullptr, s->T[1], s->T[2], s->u1, s->u2,
-            s->vor_mean, mb, s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, UMx, UMy, 0, 0, F, s->twx,
+            s->vor_mean, mb, s->win, rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal, UMx, UMy, 0, 0,
+            F, s->twx,
             s->sc);
     return FSIBE_OK;
   });
@@ -1390,16 +1402,13 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     prof_begin(s, P_ROWS_PRESS);
     CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
       KLAUNCH(k_rows_press<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->T[2], s->u1, s->u2,
-              s->vor_mean, mb, s->win, s->sums, s->umax_part, s->T[4], s->T[5], s->dscal, 0.0, 0.0, 1,
+              s->vor_mean, mb, s->win, rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal, 0.0, 0.0, 1,
               (int)s->pressure_on, F, s->twx, s->sc);
       return FSIBE_OK;
     }));
     prof_end(s, P_ROWS_PRESS);
   }
-  // K9 reductions: in block 0 of K7 when K7 runs, else on their own
-  ReduceArgs ra;
-  ra.sums = s->sums; ra.ny = s->ny; ra.beam_on = (int)(s->p.iBeam > 0); ra.s = s->dscal; ra.out = res_dev(s, s->slot);
-  ra.dxdy = s->dx * s->dy; ra.nu = s->p.nu; ra.eps = s->p.eps; ra.U_mean_true = s->p.U_mean_true;
+  s->slot_fluid[s->slot] = true;  // the host adds the row sums of K6
   // K7 + K8 on the rows/columns the interpolation needs
   s->press_box_valid = false;
   if (s->forces_on && s->forces_ready && !s->fbox.empty()) {
@@ -1409,7 +1418,7 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
       CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
         const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
         KLAUNCH(k_cols_press<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[4], s->T[5], s->T[6], fb.y0, fb.y1,
-                F, s->twy, s->sc, ra, 1);
+                F, s->twy, s->sc);
         return FSIBE_OK;
       }));
       prof_end(s, P_COLS_PRESS);
@@ -1425,11 +1434,6 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
       prof_end(s, P_ROWS_BOX);
     }
     s->press_box_valid = true;
-  } else {
-    prof_begin(s, P_REDUCE);
-    k_reduce_step<<<1, kNT, 0, s->stream>>>(ra);
-    HIPCHK(hipGetLastError());
-    prof_end(s, P_REDUCE);
   }
   return FSIBE_OK;
 }
@@ -1447,26 +1451,79 @@ static int launch_forces(fsibe_sim *s) {
   k_forces<<<units, kForceThreads, 0, s->stream>>>(
       tg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
       reinterpret_cast<const double *>(s->d_stage + s->off_cd), pbeam_dev(s, s->slot),
-      pbeam_dev(s, s->slot) + std::max(s->ns, 1), s->d_part, res_dev(s, s->slot), s->d_done);
+      pbeam_dev(s, s->slot) + std::max(s->ns, 1), parts_dev(s, s->slot));
   HIPCHK(hipGetLastError());
   prof_end(s, P_INTERP);
+  s->slot_forces[s->slot] = true;
+  s->slot_units[s->slot] = units;
   return FSIBE_OK;
 }
 
-// copy result slot k (complete: its kernels have finished) into s->last
+// Read result slot k (its kernels have finished) into s->last: dt1, u_max;
+// the fluid step's row sums (energies cal_energy NavierStokes.f90:236-256,
+// cal_drag Force / Force_unst :567-589, BeamSurfaceIntegral drag/lift
+// BeamForces.f90:483-486) added over rows in order; the force terms of
+// k_forces (Force_press :261-267, surface integrals :485-496) added over
+// units in order.  Each part is read once per step.
 static void parse_results(fsibe_sim *s, int k) {
   const int nsb = std::max(s->ns, 1);
   const char *base = s->h_res + (size_t)k * s->res_stride;
   const StepOutDev *o = reinterpret_cast<const StepOutDev *>(base);
   fsibe_step_out &r = s->last;
-  r.dt1 = o->dt1; r.u_max = o->u_max;
-  for (int i = 0; i < 2; ++i) { r.Force[i] = o->Force[i]; r.Force_unst[i] = o->Force_unst[i]; r.Force_press[i] = o->Force_press[i]; }
-  for (int i = 0; i < 6; ++i) r.energies[i] = o->energies[i];
-  for (int i = 0; i < 10; ++i) r.bsi[i] = o->bsi[i];
-  r.fx_int = o->fx_int; r.fy_int = o->fy_int;
+  r.dt1 = o->dt1;
+  r.u_max = o->u_max;
   r.status = 0;
-  const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
-  for (int i = 0; i < s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
+  const double dxdy = s->dx * s->dy;
+  if (s->slot_fluid[k]) {
+    s->slot_fluid[k] = false;
+    const double *rows = reinterpret_cast<const double *>(base + rows_off(s));
+    double tot[kNSUM];
+    for (int i = 0; i < kNSUM; ++i) tot[i] = 0.0;
+    for (int y = 0; y < s->ny; ++y)
+      for (int i = 0; i < kNSUM; ++i) tot[i] += rows[(size_t)y * kNSUM + i];
+    r.energies[0] = dxdy * sqrt(tot[0]);
+    r.energies[1] = 0.5 * dxdy * tot[1];
+    r.energies[2] = s->p.nu * dxdy * tot[0];
+    r.energies[3] = dxdy * tot[2];
+    r.energies[4] = dxdy * tot[3];
+    r.energies[5] = dxdy * s->p.U_mean_true * tot[4];
+    r.bsi[0] = tot[5] * dxdy;
+    r.bsi[1] = tot[6] * dxdy;
+    if (s->p.iBeam > 0) {
+      r.Force[0] = tot[7] * dxdy;
+      r.Force[1] = tot[8] * dxdy;
+      const double dn = tot[9] * dxdy * s->p.eps, ln = tot[10] * dxdy * s->p.eps;
+      if (s->unst_ready) {
+        r.Force_unst[0] = (dn - s->drag_unst_old) / o->dt1;
+        r.Force_unst[1] = (ln - s->lift_unst_old) / o->dt1;
+      } else {
+        r.Force_unst[0] = 0.0;
+        r.Force_unst[1] = 0.0;
+      }
+      s->drag_unst_old = dn;
+      s->lift_unst_old = ln;
+      s->unst_ready = true;
+    }
+  }
+  if (s->slot_forces[k]) {
+    s->slot_forces[k] = false;
+    const double *part = reinterpret_cast<const double *>(base + parts_off(s));
+    double tot[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int u = 0; u < s->slot_units[k]; ++u)
+      for (int i = 0; i < 8; ++i) tot[i] += part[8 * u + i];
+    r.Force_press[0] = tot[0];
+    r.Force_press[1] = tot[1];
+    r.fx_int = tot[0];
+    r.fy_int = tot[1];
+    if (s->fg.bsi_on) {
+      r.bsi[2] = tot[2] + tot[4] + tot[6];
+      r.bsi[3] = tot[3] + tot[5] + tot[7];
+      r.bsi[4] = tot[2]; r.bsi[5] = tot[4]; r.bsi[6] = tot[6];
+      r.bsi[7] = tot[3]; r.bsi[8] = tot[5]; r.bsi[9] = tot[7];
+    }
+    const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
+    for (int i = 0; i < s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
+  }
 }
 
 static int fetch_results(fsibe_sim *s) {
@@ -1683,7 +1740,7 @@ extern "C" int fsibe_get_field(fsibe_sim *s, int which, double *out) {
       CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
         const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
         KLAUNCH(k_cols_press<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[4], s->T[5], s->T[6], 0, s->ny - 1,
-                F, s->twy, s->sc, ReduceArgs{}, 0);
+                F, s->twy, s->sc);
         return FSIBE_OK;
       }));
       const size_t lds2 = sizeof(double2) * 3 * padlen(s->H);

@@ -18,8 +18,10 @@
 //            pressure rhs q = chi(u-u_s) + omega x u -> Fx  (pressure :305-370)
 //   K7 cols  div rhs -> poisson -> Fy^-1 on the rows near the beam
 //   K8 rows  p = q - |u|^2/2 and the stresses on the rows near the beam
-//   K9       reductions of the K6 row sums (block 0 of K7)
-//   K10      Delta interpolation at the beam and the force sums (GetForcesInterp)
+//   K10      Delta interpolation at the beam and the per-node force terms
+//            (GetForcesInterp); the host adds the per-row sums of K6 and
+//            the per-node terms of K10, which the kernels write into the
+//            pinned result record, in a fixed order
 #pragma once
 #include "kernels.hpp"
 
@@ -50,8 +52,6 @@ struct DevScalars {
   double UMx, UMy;             // U_mean_true * Mean_u(time+dt1)
   double Mx, My;               // Mean_u(time+dt1)
   double time, dt0;
-  double drag_unst_old, lift_unst_old;
-  int ready, pad;
 };
 
 // per-step results; lives in pinned host memory that the kernels write
@@ -555,78 +555,14 @@ __global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict
 
 
 // ============================================================================
-// K9: reductions of the per-row partial sums: cal_drag (Force, Force_unst,
-// NavierStokes.f90:567-589), cal_energy (:236-256), BeamSurfaceIntegral
-// drag/lift (BeamForces.f90:483-486).  Block-wide tree sums in a fixed order;
-// run by block 0 of K7 (or alone when K7 is not needed).
-// ============================================================================
-struct ReduceArgs {
-  const double *sums;
-  int ny, beam_on;
-  DevScalars *s;
-  StepOutDev *out;
-  double dxdy, nu, eps, U_mean_true;
-};
-
-__device__ void reduce_step(const ReduceArgs &r) {
-#pragma clang fp contract(off)
-  __shared__ double red[kNSUM * 16];
-  const double *__restrict__ sums = r.sums;
-  const int ny = r.ny, beam_on = r.beam_on;
-  DevScalars *__restrict__ s = r.s;
-  StepOutDev *__restrict__ out = r.out;
-  const double dxdy = r.dxdy, nu = r.nu, eps = r.eps, U_mean_true = r.U_mean_true;
-  double acc[kNSUM];
-#pragma unroll
-  for (int i = 0; i < kNSUM; ++i) acc[i] = 0.0;
-  for (int y = threadIdx.x; y < ny; y += blockDim.x)
-#pragma unroll
-    for (int i = 0; i < kNSUM; ++i) acc[i] += sums[(size_t)y * kNSUM + i];
-  block_sum<kNSUM>(acc, red);
-  if (threadIdx.x == 0) {
-    const double *tot = acc;
-    out->dt1 = s->dt1;
-    out->u_max = s->u_max;
-    out->energies[0] = dxdy * sqrt(tot[0]);
-    out->energies[1] = 0.5 * dxdy * tot[1];
-    out->energies[2] = nu * dxdy * tot[0];
-    out->energies[3] = dxdy * tot[2];
-    out->energies[4] = dxdy * tot[3];
-    out->energies[5] = dxdy * U_mean_true * tot[4];
-    out->bsi[0] = tot[5] * dxdy;
-    out->bsi[1] = tot[6] * dxdy;
-    if (beam_on) {
-      out->Force[0] = tot[7] * dxdy;
-      out->Force[1] = tot[8] * dxdy;
-      const double dn = tot[9] * dxdy * eps, ln = tot[10] * dxdy * eps;
-      if (s->ready) {
-        out->Force_unst[0] = (dn - s->drag_unst_old) / s->dt1;
-        out->Force_unst[1] = (ln - s->lift_unst_old) / s->dt1;
-      } else {
-        out->Force_unst[0] = 0.0;
-        out->Force_unst[1] = 0.0;
-      }
-      s->drag_unst_old = dn;
-      s->lift_unst_old = ln;
-      s->ready = 1;
-    }
-    __threadfence_system();
-  }
-}
-
-__global__ void __launch_bounds__(kNT) k_reduce_step(ReduceArgs r) { reduce_step(r); }
-
-// ============================================================================
 // K7: divergence + poisson (pressure NavierStokes.f90:366-382), then the
 // inverse column transform; rows [r0, r1] are written.
 // ============================================================================
 template <class FY>
 __global__ void __launch_bounds__(FY::LB) k_cols_press(const double2 *__restrict__ Rq, const double2 *__restrict__ Rqy,
                                                        double2 *__restrict__ Cq, int r0, int r1, FY F,
-                                                       const double2 *__restrict__ twy, StepConsts c,
-                                                       ReduceArgs ra, int do_reduce) {
+                                                       const double2 *__restrict__ twy, StepConsts c) {
   extern __shared__ double2 lds[];
-  if (do_reduce && blockIdx.x == 0) reduce_step(ra);
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
   const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
   const int TT = 2 * TX;
@@ -748,7 +684,8 @@ __device__ __forceinline__ double phi4star(double x, double dx, double pi) {
 // the reference's order (ix outer, iy inner).  Thread 0 then forms the
 // unit's loads and its terms of the eight force sums (GetForcesInterp
 // BeamForces.f90:204-267, BeamSurfaceIntegral :354-480) and writes them to
-// part[8 unit + k]; the last block to finish adds the parts in unit order.
+// part[8 unit + k] in the pinned result record; the host adds them in unit
+// order.
 constexpr int kForceThreads = 64;
 
 __device__ __forceinline__ void interp6(const double *__restrict__ tg, int t0, int ntu,
@@ -793,13 +730,9 @@ __global__ void __launch_bounds__(kForceThreads) k_forces(const double *__restri
                                                           ForceGeom g, double *__restrict__ val,
                                                           const double *__restrict__ nd, const double *__restrict__ cd,
                                                           double *__restrict__ pbeam, double *__restrict__ tbeam,
-                                                          double *__restrict__ part, StepOutDev *__restrict__ out,
-                                                          unsigned int *__restrict__ done) {
+                                                          double *__restrict__ part) {
 #pragma clang fp contract(off)
   __shared__ double vals[6];
-  __shared__ double tot[8];
-  __shared__ double red[8][kForceThreads];
-  __shared__ unsigned int last;
   const int u = blockIdx.x, ns = g.ns;
   const bool node = u < ns;
   const int t0 = node ? 6 * u : 6 * ns + 6 * (u - ns);
@@ -853,46 +786,6 @@ __global__ void __launch_bounds__(kForceThreads) k_forces(const double *__restri
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) part[8 * u + k] = acc[k];
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1u : 0u;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  {
-    // thread t adds units t, t + 64, t + 128, ... (independent loads), then
-    // threads 0..7 add the 64 thread sums of one quantity in thread order
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int v = threadIdx.x; v < (int)gridDim.x; v += kForceThreads) {
-      const double4 lo = reinterpret_cast<const double4 *>(part + 8 * v)[0];
-      const double4 hi = reinterpret_cast<const double4 *>(part + 8 * v)[1];
-      acc[0] += lo.x; acc[1] += lo.y; acc[2] += lo.z; acc[3] += lo.w;
-      acc[4] += hi.x; acc[5] += hi.y; acc[6] += hi.z; acc[7] += hi.w;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = acc[k];
-  }
-  __syncthreads();
-  if (threadIdx.x < 8) {
-    double sum = 0.0;
-    for (int t = 0; t < kForceThreads; ++t) sum += red[threadIdx.x][t];
-    tot[threadIdx.x] = sum;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    out->Force_press[0] = tot[0];
-    out->Force_press[1] = tot[1];
-    out->fx_int = tot[0];
-    out->fy_int = tot[1];
-    if (g.bsi_on) {
-      out->bsi[2] = tot[2] + tot[4] + tot[6];
-      out->bsi[3] = tot[3] + tot[5] + tot[7];
-      out->bsi[4] = tot[2]; out->bsi[5] = tot[4]; out->bsi[6] = tot[6];
-      out->bsi[7] = tot[3]; out->bsi[8] = tot[5]; out->bsi[9] = tot[7];
-    }
-    __threadfence_system();
-    *done = 0u;
   }
 }
 

@@ -220,9 +220,9 @@ void *fsibe_sim_stream(fsibe_sim *sim);
  * classes: 0 create_mask, 1 startup velocity, 2 columns of vortk(n0),
  * 3 rows: nonlinear term (with the dt/AB2 scalars), 4 unused,
  * 5 columns: AB2 update, 6 columns of vortk(n1), 7 rows: velocity/pressure
- * rhs, 8 reductions (only when class 9 does not run), 9 columns: pressure
- * (with the reductions), 10 rows: pressure/stress box, 11 Delta
- * interpolation and beam forces, 12 unused. */
+ * rhs and row sums, 8 unused, 9 columns: pressure, 10 rows: pressure/stress
+ * box, 11 Delta interpolation and per-node beam loads, 12 unused (the host
+ * adds the row sums and the per-node terms when it reads a step). */
 int fsibe_profile_enable(fsibe_sim *sim, int classes);
 int fsibe_profile_read(fsibe_sim *sim, double *total_ms, long long *count, int n);
 
