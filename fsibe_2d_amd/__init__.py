@@ -6,6 +6,6 @@ path: `spectral` (the FFT/operator plugin) and `solver` (the device-resident
 step and the time loop with prescribed beam kinematics).
 """
 from ._lib import FsibeError, lib  # noqa: F401
-from .params import Params, fsi_256, fsi_large, turbulence_4096  # noqa: F401
+from .params import Params, batch_configs, fsi_256, fsi_large, turbulence_4096  # noqa: F401
 
-__all__ = ["FsibeError", "lib", "Params", "fsi_256", "fsi_large", "turbulence_4096"]
+__all__ = ["FsibeError", "lib", "Params", "batch_configs", "fsi_256", "fsi_large", "turbulence_4096"]

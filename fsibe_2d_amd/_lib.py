@@ -67,6 +67,7 @@ EXPORTS = (
     "fsibe_swap_levels", "fsibe_get_field", "fsibe_get_box", "fsibe_mouvement",
     "fsibe_integrate_position", "fsibe_run", "fsibe_get_time", "fsibe_get_beam",
     "fsibe_sim_stream", "fsibe_profile_enable", "fsibe_profile_read", "fsibe_cofts", "fsibe_cofits",
+    "fsibe_batch_create", "fsibe_batch_destroy", "fsibe_batch_run", "fsibe_batch_size", "fsibe_batch_member",
 )
 
 _lib = None
@@ -119,6 +120,12 @@ def lib():
     L.fsibe_sim_stream.restype = vp
     L.fsibe_sim_stream.argtypes = [vp]
     L.fsibe_profile_enable.argtypes = [vp, C.c_int]
+    L.fsibe_batch_create.argtypes = [C.POINTER(FsibeParams), C.c_int, C.POINTER(C.c_void_p), C.POINTER(vp)]
+    L.fsibe_batch_destroy.argtypes = [vp]
+    L.fsibe_batch_run.argtypes = [vp, C.c_int]
+    L.fsibe_batch_size.argtypes = [vp]
+    L.fsibe_batch_member.restype = vp
+    L.fsibe_batch_member.argtypes = [vp, C.c_int]
     L.fsibe_cofts.argtypes = [dp, dp, C.c_int, C.c_int]
     L.fsibe_cofits.argtypes = [dp, dp, C.c_int, C.c_int]
     L.fsibe_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]

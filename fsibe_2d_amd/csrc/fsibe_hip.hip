@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1802,4 +1803,381 @@ extern "C" int fsibe_profile_read(fsibe_sim *s, double *total_ms, long long *cou
     count[k] = s->prof.cnt[k];
   }
   return P_NK;
+}
+
+// ===========================================================================
+// Batched simulations (no reference counterpart: the reference runs
+// simulations one after another, dns.f90:70-101).  A batch owns ordinary
+// simulations grouped by grid size.  Each member takes its startup step on
+// its own; every AB2 step after that is one launch per stage and size group,
+// with one argument record per member at blockIdx.y (kb_* kernels).  Members
+// keep their own time, dt, beam and parameters.  The loop is pipelined like
+// fsibe_run: the host waits only for the nonlinear-term stage, reads every
+// member's dt1, and prepares the next step's geometry while the rest runs.
+// ===========================================================================
+enum BatchStage { B_MASK, B_K1, B_K2, B_K4, B_K5, B_K6, B_K7, B_K8, B_K10, B_NST };
+
+static const size_t kBatchArgSize[B_NST] = {sizeof(MaskArgs),      sizeof(ColsInvArgs),   sizeof(RowsNlArgs),
+                                            sizeof(ColsAb2Args),   sizeof(ColsInvArgs),   sizeof(RowsPressArgs),
+                                            sizeof(ColsPressArgs), sizeof(RowsBoxArgs),   sizeof(ForcesArgs)};
+
+struct BatchGroup {
+  int nx, ny, H, nf5;
+  bool pres;
+  std::vector<int> mem;
+  size_t off[B_NST];  // argument arrays in the batch's argument buffer
+  int tx[B_NST];      // columns per tile of the column stages
+  int nt[B_NST];      // threads per block
+};
+
+struct fsibe_batch {
+  std::vector<fsibe_sim *> m;
+  std::vector<BatchGroup> groups;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_dt = nullptr;
+  char *h_args = nullptr, *d_args = nullptr;
+  size_t args_bytes = 0;
+  bool res_pending = false;
+};
+
+extern "C" int fsibe_batch_destroy(fsibe_batch *b) {
+  if (!b) return FSIBE_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (fsibe_sim *s : b->m) fsibe_sim_destroy(s);
+  if (b->h_args) (void)hipHostFree(b->h_args);
+  if (b->d_args) (void)hipFree(b->d_args);
+  if (b->ev_dt) (void)hipEventDestroy(b->ev_dt);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const double *const *vortk,
+                                  fsibe_batch **out) {
+  if (!params || !out || n < 1) return set_err(FSIBE_EARG, "batch: bad argument");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    const fsibe_params &p = params[i];
+    if (p.iFLUSI == 1) return set_err(FSIBE_EUNSUPPORTED, "batch member %d: iFLUSI=1 (host-solved beam)", i);
+    if (p.iSponge == 10) return set_err(FSIBE_EUNSUPPORTED, "batch member %d: iSponge=10", i);
+  }
+  fsibe_batch *b = new fsibe_batch();
+  auto fail = [&](int rc) { fsibe_batch_destroy(b); return rc; };
+  for (int i = 0; i < n; ++i) {
+    fsibe_sim *s = nullptr;
+    int rc = fsibe_sim_create(&params[i], &s);
+    if (rc != FSIBE_OK) return fail(rc);
+    b->m.push_back(s);
+    if (vortk && vortk[i]) {
+      rc = fsibe_set_vortk(s, vortk[i]);
+      if (rc != FSIBE_OK) return fail(rc);
+    }
+  }
+  // groups: same grid and the same launch shapes
+  for (int i = 0; i < n; ++i) {
+    const fsibe_sim *s = b->m[i];
+    const int nf5 = (s->forces_on && s->visc) ? 4 : 3;
+    BatchGroup *g = nullptr;
+    for (auto &q : b->groups)
+      if (q.nx == s->nx && q.ny == s->ny && q.nf5 == nf5 && q.pres == s->pressure_on) g = &q;
+    if (!g) {
+      b->groups.push_back(BatchGroup{});
+      g = &b->groups.back();
+      g->nx = s->nx; g->ny = s->ny; g->H = s->H; g->nf5 = nf5; g->pres = s->pressure_on;
+    }
+    g->mem.push_back(i);
+  }
+  size_t off = 0;
+  for (auto &g : b->groups)
+    for (int k = 0; k < B_NST; ++k) {
+      g.off[k] = off;
+      off += (kBatchArgSize[k] * g.mem.size() + 255) & ~(size_t)255;
+    }
+  b->args_bytes = off;
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->ev_dt, hipEventDisableTiming) != hipSuccess ||
+      hipHostMalloc((void **)&b->h_args, off, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&b->d_args, off) != hipSuccess)
+    return fail(set_err(FSIBE_EHIP, "batch: allocation failed"));
+  memset(b->h_args, 0, off);
+  // launch shapes of the column stages (the same choices the single path makes)
+  for (auto &g : b->groups) {
+    const fsibe_sim *s = b->m[g.mem[0]];
+    auto col = [&](int k, int BM, int rt_tx) {
+      int tx = 1, nt = 0;
+      int rc = BM == 1 ? with_cols<1>(s->py, s->H, rt_tx, [&](auto, int nt_, int TX) -> int { tx = TX; nt = nt_; return FSIBE_OK; })
+                       : with_cols<2>(s->py, s->H, rt_tx, [&](auto, int nt_, int TX) -> int { tx = TX; nt = nt_; return FSIBE_OK; });
+      g.tx[k] = tx; g.nt[k] = nt;
+      return rc;
+    };
+    int rc = col(B_K1, 1, pick_tx(s->H, s->ny, 1, 2));
+    if (rc == FSIBE_OK) rc = col(B_K4, 2, pick_tx(s->H, s->ny, 2, 1));
+    if (rc == FSIBE_OK) rc = col(B_K5, 1, pick_tx(s->H, s->ny, 1, g.nf5));
+    if (rc == FSIBE_OK) rc = col(B_K7, 2, pick_tx(s->H, s->ny, 2, 1));
+    if (rc != FSIBE_OK) return fail(rc);
+  }
+  *out = b;
+  return FSIBE_OK;
+}
+
+// the host part of fsibe_create_mask: beam state and geometry for `time`
+static int stage_geometry(fsibe_sim *s, double time) {
+  const fsibe_params &p = s->p;
+  const bool moving = (p.iSponge >= 4 && s->time < p.T_fullspeed) || p.iFLUSI == 1 || p.iMotion > 0;
+  if (p.iBeam > 0 && moving) {
+    double at, att;
+    mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, time, s->alpha, at, att, s->LE);
+    s->time_mask = time;
+  }
+  return build_geometry(s, time);
+}
+
+static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, int maxblk[B_NST]) {
+  char *A = b->h_args;
+  const int n0 = s->n0, n1 = s->n1;
+  const MaskBox mb = mask_box(s);
+  {  // create_mask
+    MaskArgs &a = reinterpret_cast<MaskArgs *>(A + g.off[B_MASK])[j];
+    a.prims = reinterpret_cast<const Prim *>(s->d_stage + s->off_prim);
+    a.np = s->nprim;
+    a.wallrow = s->wallrow;
+    a.om = s->mask_m; a.ovx = s->mask_vx; a.ovy = s->mask_vy;
+    MaskConsts &mc = a.c;
+    mc.dx = s->dx; mc.dy = s->dy; mc.t = s->p.t_beam; mc.h = s->p.N_smooth * std::max(s->dx, s->dy);
+    mc.gerf = s->gerf; mc.inv_eps_unused = 0.0; mc.ds = s->ds; mc.pi = s->pi; mc.sharp = s->sharp;
+    mc.eps = s->p.eps;
+    a.bx0 = s->mbox.x0; a.by0 = s->mbox.y0; a.bw = s->mbox.w(); a.bh = s->mbox.h();
+    a.tiles_x = (a.bw + kMaskTile - 1) / kMaskTile;
+    a.tiles = s->mbox.empty() ? 0 : a.tiles_x * ((a.bh + kMaskTile - 1) / kMaskTile);
+    maxblk[B_MASK] = std::max(maxblk[B_MASK], a.tiles);
+  }
+  {  // K1: A = Fy^-1[W0], B = Fy^-1[i ky W0]
+    ColsInvArgs &a = reinterpret_cast<ColsInvArgs *>(A + g.off[B_K1])[j];
+    a.W = s->W[n0]; a.o[0] = s->T[0]; a.o[1] = s->T[1]; a.o[2] = nullptr; a.o[3] = nullptr;
+    a.fsel[0] = 0; a.fsel[1] = 4; a.fsel[2] = 0; a.fsel[3] = 0; a.nfield = 2;
+    a.nblk = (s->H / g.tx[B_K1]) * 2;
+    a.c = s->sc;
+    maxblk[B_K1] = std::max(maxblk[B_K1], a.nblk);
+  }
+  {  // K2 + K3
+    RowsNlArgs &a = reinterpret_cast<RowsNlArgs *>(A + g.off[B_K2])[j];
+    a.A = s->T[0]; a.B = s->T[1]; a.u1 = s->u1; a.u2 = s->u2; a.ms = s->sponge_on ? s->ms : nullptr;
+    a.mb = mb; a.penal_on = s->penal_on; a.nparts = s->ny; a.explicit_step = 0; a.nblk = s->ny;
+    a.Rc = s->T[2]; a.Rpx = s->T[3]; a.c = s->sc; a.umax_part = s->umax_part; a.sc = s->dscal; a.fl = s->fp;
+    a.time = s->time; a.dt0 = s->dt0; a.res = res_dev(s, s->slot);
+    maxblk[B_K2] = std::max(maxblk[B_K2], a.nblk);
+  }
+  {  // K4
+    ColsAb2Args &a = reinterpret_cast<ColsAb2Args *>(A + g.off[B_K4])[j];
+    a.Rc = s->T[2]; a.Rpx = s->T[3]; a.W0 = s->W[n0]; a.NL1 = s->NL[n1]; a.NL0 = s->NL[n0]; a.W1 = s->W[n1];
+    a.s = s->dscal; a.explicit_step = 0; a.nblk = s->H / g.tx[B_K4]; a.c = s->sc;
+    maxblk[B_K4] = std::max(maxblk[B_K4], a.nblk);
+  }
+  {  // K5
+    ColsInvArgs &a = reinterpret_cast<ColsInvArgs *>(A + g.off[B_K5])[j];
+    a.W = s->W[n1];
+    for (int k = 0; k < 4; ++k) { a.o[k] = k < g.nf5 ? s->T[k] : nullptr; a.fsel[k] = k < g.nf5 ? k : 0; }
+    a.nfield = g.nf5; a.nblk = (s->H / g.tx[B_K5]) * g.nf5; a.c = s->sc;
+    maxblk[B_K5] = std::max(maxblk[B_K5], a.nblk);
+  }
+  {  // K6
+    RowsPressArgs &a = reinterpret_cast<RowsPressArgs *>(A + g.off[B_K6])[j];
+    a.Cw = s->T[0]; a.Cpsi = s->T[1]; a.Cpsiy = s->T[2]; a.u1 = s->u1; a.u2 = s->u2; a.vor_mean = s->vor_mean;
+    a.sums = rows_dev(s, s->slot); a.umax_part = s->umax_part; a.mb = mb; a.win = s->win;
+    a.Rq = s->T[4]; a.Rqy = s->T[5]; a.s = s->dscal; a.UMx0 = 0.0; a.UMy0 = 0.0; a.mode = 1;
+    a.pres = s->pressure_on; a.nblk = s->ny; a.c = s->sc;
+    maxblk[B_K6] = std::max(maxblk[B_K6], a.nblk);
+  }
+  s->slot_fluid[s->slot] = true;
+  const bool fz = s->forces_on && s->forces_ready && !s->fbox.empty();
+  const Box &fb = s->fbox;
+  {  // K7
+    ColsPressArgs &a = reinterpret_cast<ColsPressArgs *>(A + g.off[B_K7])[j];
+    a.Rq = s->T[4]; a.Rqy = s->T[5]; a.Cq = s->T[6]; a.r0 = fb.y0; a.r1 = fb.y1;
+    a.nblk = fz ? s->H / g.tx[B_K7] : 0; a.c = s->sc;
+    maxblk[B_K7] = std::max(maxblk[B_K7], a.nblk);
+  }
+  {  // K8
+    RowsBoxArgs &a = reinterpret_cast<RowsBoxArgs *>(A + g.off[B_K8])[j];
+    a.Cq = s->T[6]; a.Cpsi = s->T[1]; a.Cpsiy = s->T[2]; a.Cpsiyy = s->T[3]; a.u1 = s->u1; a.u2 = s->u2;
+    a.press = s->fb_p; a.sa = s->fb_sa; a.sb = s->fb_sb; a.r0 = fb.y0; a.bx0 = fb.x0; a.bx1 = fb.x1;
+    a.bw = fb.w(); a.visc = s->visc; a.nblk = fz ? fb.h() : 0; a.c = s->sc;
+    maxblk[B_K8] = std::max(maxblk[B_K8], a.nblk);
+  }
+  s->press_box_valid = fz;
+  {  // K10
+    ForcesArgs &a = reinterpret_cast<ForcesArgs *>(A + g.off[B_K10])[j];
+    const bool ok = fz && s->ntg > 0 && s->ntg == 6 * s->fg.ns + 6 * s->fg.nc;
+    a.tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
+    a.press = s->fb_p; a.sa = s->fb_sa; a.sb = s->fb_sb;
+    a.nd = reinterpret_cast<const double *>(s->d_stage + s->off_nd);
+    a.cd = reinterpret_cast<const double *>(s->d_stage + s->off_cd);
+    a.val = reinterpret_cast<double *>(s->d_stage + s->off_val);
+    a.pbeam = pbeam_dev(s, s->slot); a.tbeam = pbeam_dev(s, s->slot) + std::max(s->ns, 1);
+    a.part = parts_dev(s, s->slot); a.g = s->fg;
+    a.nblk = ok ? s->fg.ns + s->fg.nc : 0;
+    maxblk[B_K10] = std::max(maxblk[B_K10], a.nblk);
+    s->slot_forces[s->slot] = ok;
+    s->slot_units[s->slot] = a.nblk;
+  }
+}
+
+#define BLAUNCH(KT, gx, gy, nt, lds, st, ...)                      \
+  do {                                                              \
+    if ((gx) > 0) {                                                 \
+      CHK(prep_lds(KT, lds));                                       \
+      KT<<<dim3((gx), (gy)), nt, lds, st>>>(__VA_ARGS__);           \
+      HIPCHK(hipGetLastError());                                    \
+    }                                                               \
+  } while (0)
+
+// one AB2 step of every member
+static int batch_step(fsibe_batch *b) {
+  for (fsibe_sim *s : b->m) {
+    if (!(s->time < s->p.Time_end)) return set_err(FSIBE_ESTATE, "batch: a member reached Time_end");
+    s->dt0 = s->dt1;
+    CHK(stage_geometry(s, s->time));
+    HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, b->stream));
+  }
+  std::vector<std::array<int, B_NST>> maxblk(b->groups.size());
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    BatchGroup &g = b->groups[gi];
+    int mx[B_NST] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < g.mem.size(); ++j) fill_batch_args(b, g, (int)j, b->m[g.mem[j]], mx);
+    for (int k = 0; k < B_NST; ++k) maxblk[gi][k] = mx[k];
+  }
+  HIPCHK(hipMemcpyAsync(b->d_args, b->h_args, b->args_bytes, hipMemcpyHostToDevice, b->stream));
+  auto arr = [&](const BatchGroup &g, int k) { return b->d_args + g.off[k]; };
+  // stage by stage, each stage for every group
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    BLAUNCH(kb_mask, maxblk[gi][B_MASK], (int)g.mem.size(), kMaskThreads, 0, b->stream,
+            reinterpret_cast<const MaskArgs *>(arr(g, B_MASK)));
+  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    const int nm = (int)g.mem.size();
+    CHK(with_cols<1>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 1, 2), [&](auto F, int nt, int TX) -> int {
+      BLAUNCH(kb_cols_inv_fields<decltype(F)>, maxblk[gi][B_K1], nm, nt, sizeof(double2) * TX * padlen(s0->ny),
+              b->stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K1)), F, s0->twy);
+      return FSIBE_OK;
+    }));
+  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    const int nm = (int)g.mem.size();
+    CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
+      BLAUNCH(kb_rows_nl<decltype(F)>, maxblk[gi][B_K2], nm, nt, sizeof(double2) * 3 * padlen(s0->H), b->stream,
+              reinterpret_cast<const RowsNlArgs *>(arr(g, B_K2)), F, s0->twx);
+      return FSIBE_OK;
+    }));
+  }
+  HIPCHK(hipEventRecord(b->ev_dt, b->stream));
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    const int nm = (int)g.mem.size();
+    CHK(with_cols<2>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+      BLAUNCH(kb_cols_ab2<decltype(F)>, maxblk[gi][B_K4], nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny),
+              b->stream, reinterpret_cast<const ColsAb2Args *>(arr(g, B_K4)), F, s0->twy);
+      return FSIBE_OK;
+    }));
+  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    const int nm = (int)g.mem.size();
+    CHK(with_cols<1>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 1, g.nf5), [&](auto F, int nt, int TX) -> int {
+      BLAUNCH(kb_cols_inv_fields<decltype(F)>, maxblk[gi][B_K5], nm, nt, sizeof(double2) * TX * padlen(s0->ny),
+              b->stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K5)), F, s0->twy);
+      return FSIBE_OK;
+    }));
+  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    const int nm = (int)g.mem.size();
+    CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
+      BLAUNCH(kb_rows_press<decltype(F)>, maxblk[gi][B_K6], nm, nt, sizeof(double2) * 3 * padlen(s0->H), b->stream,
+              reinterpret_cast<const RowsPressArgs *>(arr(g, B_K6)), F, s0->twx);
+      return FSIBE_OK;
+    }));
+  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    const int nm = (int)g.mem.size();
+    CHK(with_cols<2>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+      BLAUNCH(kb_cols_press<decltype(F)>, maxblk[gi][B_K7], nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny),
+              b->stream, reinterpret_cast<const ColsPressArgs *>(arr(g, B_K7)), F, s0->twy);
+      return FSIBE_OK;
+    }));
+  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    const int nm = (int)g.mem.size();
+    CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
+      BLAUNCH(kb_rows_box<decltype(F)>, maxblk[gi][B_K8], nm, nt, sizeof(double2) * 3 * padlen(s0->H), b->stream,
+              reinterpret_cast<const RowsBoxArgs *>(arr(g, B_K8)), F, s0->twx);
+      return FSIBE_OK;
+    }));
+  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    BLAUNCH(kb_forces, maxblk[gi][B_K10], (int)g.mem.size(), kForceThreads, 0, b->stream,
+            reinterpret_cast<const ForcesArgs *>(arr(g, B_K10)));
+  }
+  // dt1 of every member, then the previous step's results
+  HIPCHK(hipEventSynchronize(b->ev_dt));
+  for (fsibe_sim *s : b->m) {
+    s->dt1 = *reinterpret_cast<volatile const double *>(s->h_res + (size_t)s->slot * s->res_stride);
+    if (b->res_pending) parse_results(s, s->slot ^ 1);
+    s->slot ^= 1;
+    const fsibe_params &p = s->p;
+    if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0)
+      integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
+                              s->by.data(), s->bvx.data(), s->bvy.data());
+    std::swap(s->n0, s->n1);
+    s->time = s->time + s->dt1;
+    s->it++;
+  }
+  b->res_pending = true;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_batch_run(fsibe_batch *b, int nsteps) {
+  if (!b) return set_err(FSIBE_EARG, "null batch");
+  // the startup (Euler) step of members that have not taken it, on their
+  // own; it counts as the first of the nsteps steps
+  int done = 0;
+  if (nsteps > 0) {
+    bool any = false;
+    for (fsibe_sim *s : b->m)
+      if (!s->startup_done) {
+        CHK(fsibe_run(s, 1, nullptr));
+        any = true;
+      }
+    done = any ? 1 : 0;
+  }
+  for (fsibe_sim *s : b->m) HIPCHK(hipStreamSynchronize(s->stream));
+  int rc = FSIBE_OK;
+  for (int k = done; k < nsteps; ++k) {
+    rc = batch_step(b);
+    if (rc != FSIBE_OK) break;
+  }
+  if (b->res_pending) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (fsibe_sim *s : b->m) parse_results(s, s->slot ^ 1);
+    b->res_pending = false;
+  }
+  return rc;
+}
+
+extern "C" int fsibe_batch_size(fsibe_batch *b) { return b ? (int)b->m.size() : 0; }
+
+extern "C" fsibe_sim *fsibe_batch_member(fsibe_batch *b, int i) {
+  if (!b || i < 0 || i >= (int)b->m.size()) return nullptr;
+  return b->m[i];
 }

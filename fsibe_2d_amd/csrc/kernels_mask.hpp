@@ -123,10 +123,10 @@ __device__ __forceinline__ double prim_value(const Prim &P, const MaskConsts &c,
   return -2.0;
 }
 
-__global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ prims, int np,
-                                              const double *__restrict__ wallrow, MaskConsts c, int bx0, int by0,
-                                              int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
-                                              double *__restrict__ ovy) {
+__device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np, const double *__restrict__ wallrow,
+                                          const MaskConsts &c, int bx0, int by0, int bw, int bh,
+                                          double *__restrict__ om, double *__restrict__ ovx,
+                                          double *__restrict__ ovy, int tbx, int tby) {
 #pragma clang fp contract(off)
   __shared__ int4 bnd[kMaxTilePrims];  // ix0, ix1, iy0, iy1
   __shared__ Prim sp[kMaskStage];
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ 
   __shared__ double val[kMaskStage][kMaskPix];
   __shared__ int nstage, pnext;
   const int tid = threadIdx.x, lane = tid & (kMaskPix - 1), wv = tid / kMaskPix;
-  const int tx0 = bx0 + blockIdx.x * kMaskTile, ty0 = by0 + blockIdx.y * kMaskTile;
+  const int tx0 = bx0 + tbx * kMaskTile, ty0 = by0 + tby * kMaskTile;
   const int tx1 = tx0 + kMaskTile - 1, ty1 = ty0 + kMaskTile - 1;
   for (int p = tid; p < np; p += kMaskThreads) {
     const int4 h = reinterpret_cast<const int4 *>(prims + p)[0];  // type ix0 ix1 iy0
@@ -142,8 +142,8 @@ __global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ 
   }
   if (tid == 0) pnext = 0;
   __syncthreads();
-  const int lx = blockIdx.x * kMaskTile + (lane & (kMaskTile - 1));
-  const int ly = blockIdx.y * kMaskTile + (lane / kMaskTile);
+  const int lx = tbx * kMaskTile + (lane & (kMaskTile - 1));
+  const int ly = tby * kMaskTile + (lane / kMaskTile);
   const bool valid = lx < bw && ly < bh;
   const int i = bx0 + lx, j = by0 + ly;
   const double xi = (double)i * c.dx, yj = (double)j * c.dy;
@@ -229,6 +229,28 @@ __global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ 
   om[q] = m / c.eps;  // :210-214
   ovx[q] = mvx;
   ovy[q] = mvy;
+}
+
+__global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ prims, int np,
+                                              const double *__restrict__ wallrow, MaskConsts c, int bx0, int by0,
+                                              int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
+                                              double *__restrict__ ovy) {
+  mask_body(prims, np, wallrow, c, bx0, by0, bw, bh, om, ovx, ovy, blockIdx.x, blockIdx.y);
+}
+
+// batched: one record per simulation at blockIdx.y, tiles flattened on x
+struct MaskArgs {
+  const Prim *prims;
+  const double *wallrow;
+  double *om, *ovx, *ovy;
+  MaskConsts c;
+  int np, bx0, by0, bw, bh, tiles_x, tiles, pad;
+};
+__global__ void __launch_bounds__(kMaskThreads) kb_mask(const MaskArgs *__restrict__ arr) {
+  const MaskArgs &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.tiles) return;
+  const int tby = blockIdx.x / a.tiles_x, tbx = blockIdx.x - tby * a.tiles_x;
+  mask_body(a.prims, a.np, a.wallrow, a.c, a.bx0, a.by0, a.bw, a.bh, a.om, a.ovx, a.ovy, tbx, tby);
 }
 
 }  // namespace fsibe

@@ -138,14 +138,14 @@ __device__ void block_sum(double (&v)[NV], double *red) {
 // field 0: F, 1: psi = F/|k|^2, 2: i ky psi, 3: -ky^2 psi, 4: i ky F
 // ============================================================================
 template <class FY>
-__global__ void __launch_bounds__(FY::LB) k_cols_inv_fields(const double2 *__restrict__ W, double2 *__restrict__ o0,
+__device__ __forceinline__ void cols_inv_fields_body(const double2 *__restrict__ W, double2 *__restrict__ o0,
                                                             double2 *__restrict__ o1, double2 *__restrict__ o2,
                                                             double2 *__restrict__ o3, int fsel0, int fsel1,
                                                             int fsel2, int fsel3, int nfield, FY F,
-                                                            const double2 *__restrict__ twy, StepConsts c) {
+                                                            const double2 *__restrict__ twy, StepConsts c, int bid, int nblk) {
   extern __shared__ double2 lds[];
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int lb = xcd_remap(bid, nblk);
   const int tile = lb / nfield, slot = lb - tile * nfield;
   const int kx0 = tile * TX;
   const int fsel = slot == 0 ? fsel0 : slot == 1 ? fsel1 : slot == 2 ? fsel2 : fsel3;
@@ -296,19 +296,19 @@ __device__ void step_scalars(const double *__restrict__ umax_part, int nparts, D
 // cal_nlk, NavierStokes.f90:39-52 and :88-164
 // ============================================================================
 template <class FX>
-__global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ A, const double2 *__restrict__ B,
+__device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, const double2 *__restrict__ B,
                                                     const double *__restrict__ u1, const double *__restrict__ u2,
                                                     const double *__restrict__ ms, MaskBox mb, int penal_on,
                                                     double2 *__restrict__ Rc, double2 *__restrict__ Rpx, FX F,
                                                     const double2 *__restrict__ twx, StepConsts c,
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
-                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res) {
+                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res, int bid, int nblk) {
   extern __shared__ double2 lds[];
   __shared__ double red[16];
-  if (blockIdx.x == 0) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
+  if (bid == 0) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
-  const int y = blockIdx.x;
+  const int y = bid;
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const bool prow = penal_on && mb.x1 >= mb.x0 && y >= mb.y0 && y <= mb.y1;
   for (int k = tid; k < H; k += nt) {
@@ -367,14 +367,14 @@ __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ 
 // (EvolveFluidAB2 NavierStokes.f90:478-498, EvolveFluidExplicit :444-449)
 // ============================================================================
 template <class FY>
-__global__ void __launch_bounds__(FY::LB) k_cols_ab2(const double2 *__restrict__ Rc, const double2 *__restrict__ Rpx,
+__device__ __forceinline__ void cols_ab2_body(const double2 *__restrict__ Rc, const double2 *__restrict__ Rpx,
                                                      const double2 *__restrict__ W0, const double2 *__restrict__ NL1,
                                                      double2 *__restrict__ NL0, double2 *__restrict__ W1,
                                                      const DevScalars *__restrict__ s, int explicit_step, FY F,
-                                                     const double2 *__restrict__ twy, StepConsts c) {
+                                                     const double2 *__restrict__ twy, StepConsts c, int bid, int nblk) {
   extern __shared__ double2 lds[];
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
-  const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
+  const int kx0 = xcd_remap(bid, nblk) * TX;
   const int TT = 2 * TX;
   for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
@@ -432,18 +432,18 @@ constexpr int kNSUM = 11;
 struct Window { int x0, x1, y0, y1; };
 
 template <class FX>
-__global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
+__device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
                                                     const double2 *__restrict__ Cpsiy, double *__restrict__ u1,
                                                     double *__restrict__ u2, double *__restrict__ vor_mean,
                                                     MaskBox mb, Window win, double *__restrict__ sums,
                                                     double *__restrict__ umax_part, double2 *__restrict__ Rq,
                                                     double2 *__restrict__ Rqy, const DevScalars *__restrict__ s,
                                                     double UMx0, double UMy0, int mode, int pres, FX F,
-                                                    const double2 *__restrict__ twx, StepConsts c) {
+                                                    const double2 *__restrict__ twx, StepConsts c, int bid, int nblk) {
   extern __shared__ double2 lds[];
   __shared__ double red[kNSUM * 16];
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
-  const int y = blockIdx.x;
+  const int y = bid;
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const int nf = mode ? 3 : 2;
   double2 *g1 = mode ? f1 : f0, *g2 = mode ? f2 : f1;  // psi_x, psi_y slots
@@ -559,12 +559,12 @@ __global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict
 // inverse column transform; rows [r0, r1] are written.
 // ============================================================================
 template <class FY>
-__global__ void __launch_bounds__(FY::LB) k_cols_press(const double2 *__restrict__ Rq, const double2 *__restrict__ Rqy,
+__device__ __forceinline__ void cols_press_body(const double2 *__restrict__ Rq, const double2 *__restrict__ Rqy,
                                                        double2 *__restrict__ Cq, int r0, int r1, FY F,
-                                                       const double2 *__restrict__ twy, StepConsts c) {
+                                                       const double2 *__restrict__ twy, StepConsts c, int bid, int nblk) {
   extern __shared__ double2 lds[];
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
-  const int kx0 = xcd_remap(blockIdx.x, gridDim.x) * TX;
+  const int kx0 = xcd_remap(bid, nblk) * TX;
   const int TT = 2 * TX;
   for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
@@ -606,15 +606,15 @@ __global__ void __launch_bounds__(FY::LB) k_cols_press(const double2 *__restrict
 // Spectrally d_x u1 = i kx (i ky psi), d_y u1 + d_x u2 = (kx^2 - ky^2) psi.
 // ============================================================================
 template <class FX>
-__global__ void __launch_bounds__(FX::LB) k_rows_box(const double2 *__restrict__ Cq, const double2 *__restrict__ Cpsi,
+__device__ __forceinline__ void rows_box_body(const double2 *__restrict__ Cq, const double2 *__restrict__ Cpsi,
                                           const double2 *__restrict__ Cpsiy, const double2 *__restrict__ Cpsiyy,
                                           const double *__restrict__ u1, const double *__restrict__ u2,
                                           double *__restrict__ press, double *__restrict__ sa,
                                           double *__restrict__ sb, int r0, int bx0, int bx1, int bw, int visc,
-                                          FX F, const double2 *__restrict__ twx, StepConsts c) {
+                                          FX F, const double2 *__restrict__ twx, StepConsts c, int bid, int nblk) {
   extern __shared__ double2 lds[];
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
-  const int y = r0 + blockIdx.x;
+  const int y = r0 + bid;
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const int nf = visc ? 3 : 1;
   for (int k = tid; k < H; k += nt) {
@@ -632,7 +632,7 @@ __global__ void __launch_bounds__(FX::LB) k_rows_box(const double2 *__restrict__
   rows_c2r_pre(lds, nf, H, PL, twx, tid, nt);
   if (visc) F.template fft<true, 3>(lds, twx, tid, nt);
   else F.template fft<true, 1>(lds, twx, tid, nt);
-  const size_t row = (size_t)blockIdx.x * bw;
+  const size_t row = (size_t)bid * bw;
   for (int x = bx0 + tid; x <= bx1; x += nt) {
 #pragma clang fp contract(off)
     const int q = pidx(x >> 1), h = x & 1;
@@ -725,15 +725,15 @@ __device__ __forceinline__ void interp6(const double *__restrict__ tg, int t0, i
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kForceThreads) k_forces(const double *__restrict__ tg, const double *__restrict__ press,
+__device__ __forceinline__ void forces_body(const double *__restrict__ tg, const double *__restrict__ press,
                                                           const double *__restrict__ sa, const double *__restrict__ sb,
                                                           ForceGeom g, double *__restrict__ val,
                                                           const double *__restrict__ nd, const double *__restrict__ cd,
                                                           double *__restrict__ pbeam, double *__restrict__ tbeam,
-                                                          double *__restrict__ part) {
+                                                          double *__restrict__ part, int bid, int nblk) {
 #pragma clang fp contract(off)
   __shared__ double vals[6];
-  const int u = blockIdx.x, ns = g.ns;
+  const int u = bid, ns = g.ns;
   const bool node = u < ns;
   const int t0 = node ? 6 * u : 6 * ns + 6 * (u - ns);
   interp6(tg, t0, 6, press, sa, sb, g, vals, val);
@@ -787,6 +787,188 @@ __global__ void __launch_bounds__(kForceThreads) k_forces(const double *__restri
 #pragma unroll
     for (int k = 0; k < 8; ++k) part[8 * u + k] = acc[k];
   }
+}
+
+// ============================================================================
+// Entry points.  k_*: one simulation, arguments by value (grid.x = rows or
+// tiles).  kb_*: a batch of simulations of one grid size, one argument
+// record per simulation at blockIdx.y; grid.x is the largest per-simulation
+// block count and blocks beyond a simulation's own count return at once.
+// ============================================================================
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_inv_fields(const double2 *__restrict__ W, double2 *__restrict__ o0,
+                                                            double2 *__restrict__ o1, double2 *__restrict__ o2,
+                                                            double2 *__restrict__ o3, int fsel0, int fsel1,
+                                                            int fsel2, int fsel3, int nfield, FY F,
+                                                            const double2 *__restrict__ twy, StepConsts c) {
+  cols_inv_fields_body(W, o0, o1, o2, o3, fsel0, fsel1, fsel2, fsel3, nfield, F, twy, c, blockIdx.x, gridDim.x);
+}
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                    const double *__restrict__ u1, const double *__restrict__ u2,
+                                                    const double *__restrict__ ms, MaskBox mb, int penal_on,
+                                                    double2 *__restrict__ Rc, double2 *__restrict__ Rpx, FX F,
+                                                    const double2 *__restrict__ twx, StepConsts c,
+                                                    const double *__restrict__ umax_part, int nparts,
+                                                    DevScalars *__restrict__ sc, FlowParams fl, double time,
+                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res) {
+  rows_nl_body(A, B, u1, u2, ms, mb, penal_on, Rc, Rpx, F, twx, c, umax_part, nparts, sc, fl, time, dt0,
+               explicit_step, res, blockIdx.x, gridDim.x);
+}
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_ab2(const double2 *__restrict__ Rc, const double2 *__restrict__ Rpx,
+                                                     const double2 *__restrict__ W0, const double2 *__restrict__ NL1,
+                                                     double2 *__restrict__ NL0, double2 *__restrict__ W1,
+                                                     const DevScalars *__restrict__ s, int explicit_step, FY F,
+                                                     const double2 *__restrict__ twy, StepConsts c) {
+  cols_ab2_body(Rc, Rpx, W0, NL1, NL0, W1, s, explicit_step, F, twy, c, blockIdx.x, gridDim.x);
+}
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
+                                                       const double2 *__restrict__ Cpsiy, double *__restrict__ u1,
+                                                       double *__restrict__ u2, double *__restrict__ vor_mean,
+                                                       MaskBox mb, Window win, double *__restrict__ sums,
+                                                       double *__restrict__ umax_part, double2 *__restrict__ Rq,
+                                                       double2 *__restrict__ Rqy, const DevScalars *__restrict__ s,
+                                                       double UMx0, double UMy0, int mode, int pres, FX F,
+                                                       const double2 *__restrict__ twx, StepConsts c) {
+  rows_press_body(Cw, Cpsi, Cpsiy, u1, u2, vor_mean, mb, win, sums, umax_part, Rq, Rqy, s, UMx0, UMy0, mode, pres,
+                  F, twx, c, blockIdx.x, gridDim.x);
+}
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) k_cols_press(const double2 *__restrict__ Rq, const double2 *__restrict__ Rqy,
+                                                       double2 *__restrict__ Cq, int r0, int r1, FY F,
+                                                       const double2 *__restrict__ twy, StepConsts c) {
+  cols_press_body(Rq, Rqy, Cq, r0, r1, F, twy, c, blockIdx.x, gridDim.x);
+}
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_box(const double2 *__restrict__ Cq, const double2 *__restrict__ Cpsi,
+                                                     const double2 *__restrict__ Cpsiy,
+                                                     const double2 *__restrict__ Cpsiyy, const double *__restrict__ u1,
+                                                     const double *__restrict__ u2, double *__restrict__ press,
+                                                     double *__restrict__ sa, double *__restrict__ sb, int r0, int bx0,
+                                                     int bx1, int bw, int visc, FX F,
+                                                     const double2 *__restrict__ twx, StepConsts c) {
+  rows_box_body(Cq, Cpsi, Cpsiy, Cpsiyy, u1, u2, press, sa, sb, r0, bx0, bx1, bw, visc, F, twx, c, blockIdx.x,
+                gridDim.x);
+}
+__global__ void __launch_bounds__(kForceThreads) k_forces(const double *__restrict__ tg, const double *__restrict__ press,
+                                                          const double *__restrict__ sa, const double *__restrict__ sb,
+                                                          ForceGeom g, double *__restrict__ val,
+                                                          const double *__restrict__ nd, const double *__restrict__ cd,
+                                                          double *__restrict__ pbeam, double *__restrict__ tbeam,
+                                                          double *__restrict__ part) {
+  forces_body(tg, press, sa, sb, g, val, nd, cd, pbeam, tbeam, part, blockIdx.x, gridDim.x);
+}
+
+// ---- per-simulation argument records of the batched entry points ----------
+struct ColsInvArgs {
+  const double2 *W;
+  double2 *o[4];
+  int fsel[4], nfield, nblk;
+  StepConsts c;
+};
+struct RowsNlArgs {
+  const double2 *A, *B;
+  const double *u1, *u2, *ms;
+  MaskBox mb;
+  int penal_on, nparts, explicit_step, nblk;
+  double2 *Rc, *Rpx;
+  StepConsts c;
+  const double *umax_part;
+  DevScalars *sc;
+  FlowParams fl;
+  double time, dt0;
+  StepOutDev *res;
+};
+struct ColsAb2Args {
+  const double2 *Rc, *Rpx, *W0, *NL1;
+  double2 *NL0, *W1;
+  const DevScalars *s;
+  int explicit_step, nblk;
+  StepConsts c;
+};
+struct RowsPressArgs {
+  const double2 *Cw, *Cpsi, *Cpsiy;
+  double *u1, *u2, *vor_mean, *sums, *umax_part;
+  MaskBox mb;
+  Window win;
+  double2 *Rq, *Rqy;
+  const DevScalars *s;
+  double UMx0, UMy0;
+  int mode, pres, nblk, pad;
+  StepConsts c;
+};
+struct ColsPressArgs {
+  const double2 *Rq, *Rqy;
+  double2 *Cq;
+  int r0, r1, nblk, pad;
+  StepConsts c;
+};
+struct RowsBoxArgs {
+  const double2 *Cq, *Cpsi, *Cpsiy, *Cpsiyy;
+  const double *u1, *u2;
+  double *press, *sa, *sb;
+  int r0, bx0, bx1, bw, visc, nblk;
+  StepConsts c;
+};
+struct ForcesArgs {
+  const double *tg, *press, *sa, *sb, *nd, *cd;
+  double *val, *pbeam, *tbeam, *part;
+  ForceGeom g;
+  int nblk, pad;
+};
+
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) kb_cols_inv_fields(const ColsInvArgs *__restrict__ arr, FY F,
+                                                             const double2 *__restrict__ twy) {
+  const ColsInvArgs &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  cols_inv_fields_body(a.W, a.o[0], a.o[1], a.o[2], a.o[3], a.fsel[0], a.fsel[1], a.fsel[2], a.fsel[3], a.nfield, F,
+                       twy, a.c, blockIdx.x, a.nblk);
+}
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) kb_rows_nl(const RowsNlArgs *__restrict__ arr, FX F,
+                                                     const double2 *__restrict__ twx) {
+  const RowsNlArgs &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  rows_nl_body(a.A, a.B, a.u1, a.u2, a.ms, a.mb, a.penal_on, a.Rc, a.Rpx, F, twx, a.c, a.umax_part, a.nparts, a.sc,
+               a.fl, a.time, a.dt0, a.explicit_step, a.res, blockIdx.x, a.nblk);
+}
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) kb_cols_ab2(const ColsAb2Args *__restrict__ arr, FY F,
+                                                      const double2 *__restrict__ twy) {
+  const ColsAb2Args &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  cols_ab2_body(a.Rc, a.Rpx, a.W0, a.NL1, a.NL0, a.W1, a.s, a.explicit_step, F, twy, a.c, blockIdx.x, a.nblk);
+}
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) kb_rows_press(const RowsPressArgs *__restrict__ arr, FX F,
+                                                        const double2 *__restrict__ twx) {
+  const RowsPressArgs &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  rows_press_body(a.Cw, a.Cpsi, a.Cpsiy, a.u1, a.u2, a.vor_mean, a.mb, a.win, a.sums, a.umax_part, a.Rq, a.Rqy, a.s,
+                  a.UMx0, a.UMy0, a.mode, a.pres, F, twx, a.c, blockIdx.x, a.nblk);
+}
+template <class FY>
+__global__ void __launch_bounds__(FY::LB) kb_cols_press(const ColsPressArgs *__restrict__ arr, FY F,
+                                                        const double2 *__restrict__ twy) {
+  const ColsPressArgs &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  cols_press_body(a.Rq, a.Rqy, a.Cq, a.r0, a.r1, F, twy, a.c, blockIdx.x, a.nblk);
+}
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) kb_rows_box(const RowsBoxArgs *__restrict__ arr, FX F,
+                                                      const double2 *__restrict__ twx) {
+  const RowsBoxArgs &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  rows_box_body(a.Cq, a.Cpsi, a.Cpsiy, a.Cpsiyy, a.u1, a.u2, a.press, a.sa, a.sb, a.r0, a.bx0, a.bx1, a.bw, a.visc, F,
+                twx, a.c, blockIdx.x, a.nblk);
+}
+__global__ void __launch_bounds__(kForceThreads) kb_forces(const ForcesArgs *__restrict__ arr) {
+  const ForcesArgs &a = arr[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  forces_body(a.tg, a.press, a.sa, a.sb, a.g, a.val, a.nd, a.cd, a.pbeam, a.tbeam, a.part, blockIdx.x, a.nblk);
 }
 
 }  // namespace fsibe

@@ -17,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import FsibeStepOut, check, lib
+from ._lib import FsibeParams, FsibeStepOut, check, lib
 from .params import Params
 
 FIELDS = {
@@ -147,3 +147,54 @@ class FluidSim:
     @property
     def stream(self):
         return lib().fsibe_sim_stream(self.h)
+
+
+class _Member(FluidSim):
+    """a member of a Batch (owned by the batch, not destroyed on its own)"""
+
+    def __init__(self, params: Params, handle):
+        self.p = params
+        self.nx, self.ny, self.ns = params.nx, params.ny, params.ns
+        self.h = C.c_void_p(handle)
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Independent simulations advanced together: every AB2 step is one
+    launch per stage and grid size (fsibe_batch_*).  Member i behaves like a
+    FluidSim with params[i] run alone."""
+
+    def __init__(self, params, vortk=None):
+        n = len(params)
+        self.params = list(params)
+        arr = (FsibeParams * n)(*[p.to_c() for p in params])
+        self._keep = []
+        ptrs = (C.c_void_p * n)()
+        for i in range(n):
+            if vortk is not None and vortk[i] is not None:
+                a = np.ascontiguousarray(vortk[i], dtype=np.float64)
+                assert a.shape == (params[i].ny, params[i].nx)
+                self._keep.append(a)
+                ptrs[i] = a.ctypes.data
+        h = C.c_void_p()
+        check(lib().fsibe_batch_create(arr, n, ptrs, C.byref(h)))
+        self.h = h
+        self._keep = []
+        self.members = [_Member(params[i], lib().fsibe_batch_member(h, i)) for i in range(n)]
+
+    def run(self, nsteps: int):
+        check(lib().fsibe_batch_run(self.h, int(nsteps)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fsibe_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

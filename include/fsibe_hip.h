@@ -207,6 +207,24 @@ int fsibe_get_time(fsibe_sim *sim, double *time, double *dt1, int *it);
 /* beam state held by fsibe_run (x, y, vx, vy, theta, pressure_new, tau_new) */
 int fsibe_get_beam(fsibe_sim *sim, double *x, double *y, double *vx, double *vy, double *theta,
                    double *pressure_new, double *tau_new);
+/* ---- batches of independent simulations --------------------------------
+ * No reference counterpart (the reference runs one simulation per process;
+ * sequential runs at several resolutions, dns.f90:70-101).  n members with
+ * their own parameters (grid sizes may differ; iFLUSI=0 and iSponge!=10),
+ * vortk[i] the packed initial vorticity of member i (or NULL).  Every AB2
+ * step of the batch is one launch per stage and grid size; each member's
+ * results are those of fsibe_run on that member alone.  The members are
+ * ordinary simulations: fsibe_batch_member returns one for fsibe_get_field,
+ * fsibe_get_step_out, fsibe_get_time, fsibe_get_beam (after fsibe_batch_run
+ * returns); they belong to the batch and are destroyed with it. */
+typedef struct fsibe_batch fsibe_batch;
+int fsibe_batch_create(const fsibe_params *params, int n, const double *const *vortk, fsibe_batch **out);
+int fsibe_batch_destroy(fsibe_batch *batch);
+/* advance every member nsteps steps (the first of a member is its startup step) */
+int fsibe_batch_run(fsibe_batch *batch, int nsteps);
+int fsibe_batch_size(fsibe_batch *batch);
+fsibe_sim *fsibe_batch_member(fsibe_batch *batch, int i);
+
 /* HIP stream of the simulation (hipStream_t) for event timing */
 void *fsibe_sim_stream(fsibe_sim *sim);
 

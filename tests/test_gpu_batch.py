@@ -1,0 +1,96 @@
+"""Batched simulations (fsibe_batch_*): members of uneven grid sizes and
+parameters advanced together must reproduce, bit for bit, the same
+simulations run one at a time through fsibe_run (the batched kernels run the
+same device code with the same launch shapes), and the oracle to the usual
+bars."""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _members():
+    from fsibe_2d_amd import Params
+
+    return [
+        Params(nx=64, ny=64, ns=32, dt_fixed=2e-4),
+        Params(nx=96, ny=64, ns=40, nu=2e-3, heave_freq=1.7, dt_fixed=1.5e-4),
+        Params(nx=128, ny=128, ns=64, nu=1e-3, heave_freq=0.6, dt_fixed=1e-4),
+        Params(nx=64, ny=64, ns=48, nu=4e-3, heave_amp=0.05, dt_fixed=2e-4),
+        Params(nx=128, ny=128, ns=70, iViscous=0, dt_fixed=1e-4),
+        Params(nx=96, ny=64, ns=33, iMotion=3, AngleBeam=8.0, dt_fixed=1.5e-4),
+    ]
+
+
+def _noise(oracle_mod, p, seed):
+    o = oracle_mod.Oracle(nx=p.nx, ny=p.ny, iBeam=0, iSponge=0, xl=p.xl, yl=p.yl)
+    return o.init_noise(seed=seed, amp=1e-3, kmax2=64)
+
+
+def test_batch_equals_single_runs(oracle_mod):
+    from fsibe_2d_amd.solver import Batch, FluidSim
+
+    ps = _members()
+    vks = [_noise(oracle_mod, p, i) for i, p in enumerate(ps)]
+    b = Batch(ps, vks)
+    steps = 9
+    b.run(steps)
+    for i, (p, vk) in enumerate(zip(ps, vks)):
+        g = FluidSim(p)
+        g.set_vortk(vk)
+        g.run(steps)
+        m = b.members[i]
+        print(f"[batch member {i}] {p.nx}x{p.ny} time {m.time_state()[0]:.6f}", flush=True)
+        assert m.time_state() == g.time_state()
+        for k in ("vortk", "u1", "u2", "vor_mean", "mask", "maskvx", "maskvy"):
+            assert np.array_equal(m.field(k), g.field(k)), (i, k)
+        so, go = m.step_out(), g.step_out()
+        for k in ("dt1", "Force", "Force_unst", "Force_press", "energies", "bsi"):
+            assert so[k] == go[k], (i, k, so[k], go[k])
+        mb, gb = m.beam(), g.beam()
+        for k in ("x", "y", "pressure", "tau"):
+            assert np.array_equal(mb[k], gb[k]), (i, k)
+        g.close()
+    b.close()
+
+
+def test_batch_member_against_oracle(oracle_mod):
+    """one member of a mixed batch against the oracle's plain run"""
+    from fsibe_2d_amd.solver import Batch
+
+    ps = _members()
+    vks = [_noise(oracle_mod, p, i) for i, p in enumerate(ps)]
+    b = Batch(ps, vks)
+    b.run(12)
+    p = ps[2]
+    o = oracle_mod.Oracle(oracle_mod.make_params(**{k: getattr(p, k) for k in (
+        "nx", "ny", "ns", "nu", "dt_fixed", "iMotion", "heave_amp", "heave_freq", "iViscous")}))
+    o.set_vortk(vks[2])
+    o.run(12)
+    err = rel_l2(b.members[2].field("vortk"), o.field("vortk"))
+    print(f"[batch vs oracle] vortk rel L2 {err:.2e}", flush=True)
+    assert err < 1e-10
+    f_g, f_o = np.array(b.members[2].step_out()["Force"]), np.array(o.beam_arrays()["Force"])
+    assert np.all(np.abs(f_g - f_o) <= 1e-9 * np.maximum(np.abs(f_o), 1e-6 * np.abs(f_o).max()))
+    b.close()
+
+
+def test_batch_of_benchmark_members(oracle_mod):
+    """8 members of the batched workload (grids 256..768) against single runs"""
+    from fsibe_2d_amd import batch_configs
+    from fsibe_2d_amd.solver import Batch, FluidSim
+
+    ps = batch_configs(8)
+    vks = [_noise(oracle_mod, p, 100 + i) for i, p in enumerate(ps)]
+    b = Batch(ps, vks)
+    b.run(5)
+    for i in (0, 3, 7):
+        g = FluidSim(ps[i])
+        g.set_vortk(vks[i])
+        g.run(5)
+        assert np.array_equal(b.members[i].field("vortk"), g.field("vortk")), i
+        assert b.members[i].step_out()["Force"] == g.step_out()["Force"], i
+        g.close()
+    b.close()
