@@ -11,7 +11,12 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -1815,6 +1820,70 @@ extern "C" int fsibe_profile_read(fsibe_sim *s, double *total_ms, long long *cou
 // fsibe_run: the host waits only for the nonlinear-term stage, reads every
 // member's dt1, and prepares the next step's geometry while the rest runs.
 // ===========================================================================
+// A few host threads for the per-member host work of a batch step (beam
+// geometry, reading results): for(i = 0 .. n-1) fn(i), spread over the pool
+// and the calling thread.  Each member's host state is touched by one thread.
+class HostPool {
+ public:
+  explicit HostPool(int nthreads) {
+    for (int t = 0; t < nthreads; ++t) workers_.emplace_back([this] { loop(); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &w : workers_) w.join();
+  }
+  void run(int n, const std::function<void(int)> &fn) {
+    if (workers_.empty() || n <= 1) {
+      for (int i = 0; i < n; ++i) fn(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &fn;
+      n_ = n;
+      next_.store(0);
+      active_ = (int)workers_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> g(mu_);
+    done_cv_.wait(g, [this] { return active_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (int i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) (*fn_)(i);
+  }
+  void loop() {
+    unsigned long long seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+      }
+      work();
+      std::lock_guard<std::mutex> g(mu_);
+      if (--active_ == 0) done_cv_.notify_one();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)> *fn_ = nullptr;
+  std::atomic<int> next_{0};
+  int n_ = 0, active_ = 0;
+  unsigned long long gen_ = 0;
+  bool stop_ = false;
+};
+
 enum BatchStage { B_MASK, B_K1, B_K2, B_K4, B_K5, B_K6, B_K7, B_K8, B_K10, B_NST };
 
 static const size_t kBatchArgSize[B_NST] = {sizeof(MaskArgs),      sizeof(ColsInvArgs),   sizeof(RowsNlArgs),
@@ -1825,6 +1894,8 @@ struct BatchGroup {
   int nx, ny, H, nf5;
   bool pres;
   std::vector<int> mem;
+  hipStream_t stream = nullptr;  // each group runs its stages on its own stream
+  hipEvent_t ev_dt = nullptr;    // after the group's nonlinear-term stage
   size_t off[B_NST];  // argument arrays in the batch's argument buffer
   int tx[B_NST];      // columns per tile of the column stages
   int nt[B_NST];      // threads per block
@@ -1833,21 +1904,29 @@ struct BatchGroup {
 struct fsibe_batch {
   std::vector<fsibe_sim *> m;
   std::vector<BatchGroup> groups;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev_dt = nullptr;
+  hipStream_t stream = nullptr;  // argument upload
+  hipEvent_t ev_dt = nullptr, ev_args = nullptr;
   char *h_args = nullptr, *d_args = nullptr;
   size_t args_bytes = 0;
   bool res_pending = false;
+  HostPool *pool = nullptr;
 };
 
 extern "C" int fsibe_batch_destroy(fsibe_batch *b) {
   if (!b) return FSIBE_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto &g : b->groups) {
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    if (g.ev_dt) (void)hipEventDestroy(g.ev_dt);
+  }
+  if (b->ev_args) (void)hipEventDestroy(b->ev_args);
   for (fsibe_sim *s : b->m) fsibe_sim_destroy(s);
   if (b->h_args) (void)hipHostFree(b->h_args);
   if (b->d_args) (void)hipFree(b->d_args);
   if (b->ev_dt) (void)hipEventDestroy(b->ev_dt);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b->pool;
   delete b;
   return FSIBE_OK;
 }
@@ -1894,11 +1973,22 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
       off += (kBatchArgSize[k] * g.mem.size() + 255) & ~(size_t)255;
     }
   b->args_bytes = off;
+  for (auto &g : b->groups)
+    if (hipStreamCreate(&g.stream) != hipSuccess || hipEventCreateWithFlags(&g.ev_dt, hipEventDisableTiming) != hipSuccess)
+      return fail(set_err(FSIBE_EHIP, "batch: stream creation failed"));
   if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->ev_dt, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_args, hipEventDisableTiming) != hipSuccess ||
       hipHostMalloc((void **)&b->h_args, off, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&b->d_args, off) != hipSuccess)
     return fail(set_err(FSIBE_EHIP, "batch: allocation failed"));
   memset(b->h_args, 0, off);
+  {
+    // host threads: FSIBE_HOST_THREADS, else up to 15 helpers beside the caller
+    const int hw = (int)std::thread::hardware_concurrency();
+    int nt = env_int("FSIBE_HOST_THREADS", std::min(16, std::max(1, hw))) - 1;
+    nt = std::max(0, std::min(nt, n - 1));
+    b->pool = new HostPool(nt);
+  }
   // launch shapes of the column stages (the same choices the single path makes)
   for (auto &g : b->groups) {
     const fsibe_sim *s = b->m[g.mem[0]];
@@ -2032,12 +2122,30 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
 
 // one AB2 step of every member
 static int batch_step(fsibe_batch *b) {
-  for (fsibe_sim *s : b->m) {
+  for (fsibe_sim *s : b->m)
     if (!(s->time < s->p.Time_end)) return set_err(FSIBE_ESTATE, "batch: a member reached Time_end");
+  std::atomic<int> bad{-1};
+  std::vector<std::string> errs(b->m.size());
+  b->pool->run((int)b->m.size(), [&](int i) {
+    fsibe_sim *s = b->m[i];
     s->dt0 = s->dt1;
-    CHK(stage_geometry(s, s->time));
-    HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, b->stream));
+    if (stage_geometry(s, s->time) != FSIBE_OK) {
+      errs[i] = fsibe_last_error();
+      bad.store(i);
+    }
+  });
+  if (bad.load() >= 0) return set_err(FSIBE_EDOMAIN, "batch member %d: %s", bad.load(), errs[bad.load()].c_str());
+  // the previous step must be done with the argument buffer before it is
+  // overwritten: the upload stream waits for every group's last stage
+  for (auto &g : b->groups) {
+    HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, g.ev_dt, 0));
   }
+  for (auto &g : b->groups)
+    for (int i : g.mem) {
+      fsibe_sim *s = b->m[i];
+      HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, g.stream));
+    }
   std::vector<std::array<int, B_NST>> maxblk(b->groups.size());
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     BatchGroup &g = b->groups[gi];
@@ -2046,11 +2154,13 @@ static int batch_step(fsibe_batch *b) {
     for (int k = 0; k < B_NST; ++k) maxblk[gi][k] = mx[k];
   }
   HIPCHK(hipMemcpyAsync(b->d_args, b->h_args, b->args_bytes, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipEventRecord(b->ev_args, b->stream));
+  for (auto &g : b->groups) HIPCHK(hipStreamWaitEvent(g.stream, b->ev_args, 0));
   auto arr = [&](const BatchGroup &g, int k) { return b->d_args + g.off[k]; };
   // stage by stage, each stage for every group
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     const BatchGroup &g = b->groups[gi];
-    BLAUNCH(kb_mask, maxblk[gi][B_MASK], (int)g.mem.size(), kMaskThreads, 0, b->stream,
+    BLAUNCH(kb_mask, maxblk[gi][B_MASK], (int)g.mem.size(), kMaskThreads, 0, g.stream,
             reinterpret_cast<const MaskArgs *>(arr(g, B_MASK)));
   }
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
@@ -2059,7 +2169,7 @@ static int batch_step(fsibe_batch *b) {
     const int nm = (int)g.mem.size();
     CHK(with_cols<1>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 1, 2), [&](auto F, int nt, int TX) -> int {
       BLAUNCH(kb_cols_inv_fields<decltype(F)>, maxblk[gi][B_K1], nm, nt, sizeof(double2) * TX * padlen(s0->ny),
-              b->stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K1)), F, s0->twy);
+              g.stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K1)), F, s0->twy);
       return FSIBE_OK;
     }));
   }
@@ -2068,19 +2178,19 @@ static int batch_step(fsibe_batch *b) {
     const fsibe_sim *s0 = b->m[g.mem[0]];
     const int nm = (int)g.mem.size();
     CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
-      BLAUNCH(kb_rows_nl<decltype(F)>, maxblk[gi][B_K2], nm, nt, sizeof(double2) * 3 * padlen(s0->H), b->stream,
+      BLAUNCH(kb_rows_nl<decltype(F)>, maxblk[gi][B_K2], nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream,
               reinterpret_cast<const RowsNlArgs *>(arr(g, B_K2)), F, s0->twx);
       return FSIBE_OK;
     }));
   }
-  HIPCHK(hipEventRecord(b->ev_dt, b->stream));
+  for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     const BatchGroup &g = b->groups[gi];
     const fsibe_sim *s0 = b->m[g.mem[0]];
     const int nm = (int)g.mem.size();
     CHK(with_cols<2>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
       BLAUNCH(kb_cols_ab2<decltype(F)>, maxblk[gi][B_K4], nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny),
-              b->stream, reinterpret_cast<const ColsAb2Args *>(arr(g, B_K4)), F, s0->twy);
+              g.stream, reinterpret_cast<const ColsAb2Args *>(arr(g, B_K4)), F, s0->twy);
       return FSIBE_OK;
     }));
   }
@@ -2090,7 +2200,7 @@ static int batch_step(fsibe_batch *b) {
     const int nm = (int)g.mem.size();
     CHK(with_cols<1>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 1, g.nf5), [&](auto F, int nt, int TX) -> int {
       BLAUNCH(kb_cols_inv_fields<decltype(F)>, maxblk[gi][B_K5], nm, nt, sizeof(double2) * TX * padlen(s0->ny),
-              b->stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K5)), F, s0->twy);
+              g.stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K5)), F, s0->twy);
       return FSIBE_OK;
     }));
   }
@@ -2099,7 +2209,7 @@ static int batch_step(fsibe_batch *b) {
     const fsibe_sim *s0 = b->m[g.mem[0]];
     const int nm = (int)g.mem.size();
     CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
-      BLAUNCH(kb_rows_press<decltype(F)>, maxblk[gi][B_K6], nm, nt, sizeof(double2) * 3 * padlen(s0->H), b->stream,
+      BLAUNCH(kb_rows_press<decltype(F)>, maxblk[gi][B_K6], nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream,
               reinterpret_cast<const RowsPressArgs *>(arr(g, B_K6)), F, s0->twx);
       return FSIBE_OK;
     }));
@@ -2110,7 +2220,7 @@ static int batch_step(fsibe_batch *b) {
     const int nm = (int)g.mem.size();
     CHK(with_cols<2>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
       BLAUNCH(kb_cols_press<decltype(F)>, maxblk[gi][B_K7], nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny),
-              b->stream, reinterpret_cast<const ColsPressArgs *>(arr(g, B_K7)), F, s0->twy);
+              g.stream, reinterpret_cast<const ColsPressArgs *>(arr(g, B_K7)), F, s0->twy);
       return FSIBE_OK;
     }));
   }
@@ -2119,21 +2229,23 @@ static int batch_step(fsibe_batch *b) {
     const fsibe_sim *s0 = b->m[g.mem[0]];
     const int nm = (int)g.mem.size();
     CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
-      BLAUNCH(kb_rows_box<decltype(F)>, maxblk[gi][B_K8], nm, nt, sizeof(double2) * 3 * padlen(s0->H), b->stream,
+      BLAUNCH(kb_rows_box<decltype(F)>, maxblk[gi][B_K8], nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream,
               reinterpret_cast<const RowsBoxArgs *>(arr(g, B_K8)), F, s0->twx);
       return FSIBE_OK;
     }));
   }
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     const BatchGroup &g = b->groups[gi];
-    BLAUNCH(kb_forces, maxblk[gi][B_K10], (int)g.mem.size(), kForceThreads, 0, b->stream,
+    BLAUNCH(kb_forces, maxblk[gi][B_K10], (int)g.mem.size(), kForceThreads, 0, g.stream,
             reinterpret_cast<const ForcesArgs *>(arr(g, B_K10)));
   }
   // dt1 of every member, then the previous step's results
-  HIPCHK(hipEventSynchronize(b->ev_dt));
-  for (fsibe_sim *s : b->m) {
+  for (auto &g : b->groups) HIPCHK(hipEventSynchronize(g.ev_dt));
+  const bool pending = b->res_pending;
+  b->pool->run((int)b->m.size(), [&](int i) {
+    fsibe_sim *s = b->m[i];
     s->dt1 = *reinterpret_cast<volatile const double *>(s->h_res + (size_t)s->slot * s->res_stride);
-    if (b->res_pending) parse_results(s, s->slot ^ 1);
+    if (pending) parse_results(s, s->slot ^ 1);
     s->slot ^= 1;
     const fsibe_params &p = s->p;
     if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0)
@@ -2142,7 +2254,7 @@ static int batch_step(fsibe_batch *b) {
     std::swap(s->n0, s->n1);
     s->time = s->time + s->dt1;
     s->it++;
-  }
+  });
   b->res_pending = true;
   return FSIBE_OK;
 }
@@ -2169,6 +2281,7 @@ extern "C" int fsibe_batch_run(fsibe_batch *b, int nsteps) {
   }
   if (b->res_pending) {
     HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto &g : b->groups) HIPCHK(hipStreamSynchronize(g.stream));
     for (fsibe_sim *s : b->m) parse_results(s, s->slot ^ 1);
     b->res_pending = false;
   }
