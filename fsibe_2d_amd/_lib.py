@@ -43,6 +43,8 @@ class FsibeParams(C.Structure):
         ("tau", C.c_double), ("U_mean_true", C.c_double),
         ("heave_amp", C.c_double), ("heave_freq", C.c_double),
         ("motion_coef", C.c_double * 8),
+        ("mue", C.c_double), ("eta", C.c_double), ("sigma", C.c_double), ("grav", C.c_double),
+        ("TimeMethodSolid", C.c_int32), ("iImpulse", C.c_int32),
     ]
 
 
@@ -52,9 +54,22 @@ class FsibeStepOut(C.Structure):
         ("Force", C.c_double * 2), ("Force_unst", C.c_double * 2), ("Force_press", C.c_double * 2),
         ("energies", C.c_double * 6), ("bsi", C.c_double * 10),
         ("fx_int", C.c_double), ("fy_int", C.c_double),
-        ("status", C.c_int32), ("pad", C.c_int32),
+        ("status", C.c_int32), ("ibes_iter", C.c_int32),
     ]
 
+
+class FsibeSolidInfo(C.Structure):
+    """fsibe_solid_info: scalars of type(solid) and of the solid solver."""
+
+    _fields_ = [
+        ("dt_old", C.c_double), ("grav", C.c_double), ("E_kinetic", C.c_double), ("E_pot", C.c_double),
+        ("E_elastic", C.c_double), ("Inertial_Force", C.c_double * 2),
+        ("StartupStep", C.c_int32), ("iter", C.c_int32), ("brake", C.c_int32), ("method", C.c_int32),
+    ]
+
+
+SOLID_ARRAYS = ("x", "y", "vx", "vy", "theta", "theta_dot", "ax", "ay", "pressure_old", "pressure_new",
+                "tau_old", "tau_new", "oldold")
 
 # every symbol include/fsibe_hip.h declares
 EXPORTS = (
@@ -68,6 +83,9 @@ EXPORTS = (
     "fsibe_integrate_position", "fsibe_run", "fsibe_get_time", "fsibe_get_beam",
     "fsibe_sim_stream", "fsibe_profile_enable", "fsibe_profile_read", "fsibe_cofts", "fsibe_cofits",
     "fsibe_batch_create", "fsibe_batch_destroy", "fsibe_batch_run", "fsibe_batch_size", "fsibe_batch_member",
+    "fsibe_solid_create", "fsibe_solid_destroy", "fsibe_solid_get", "fsibe_solid_set", "fsibe_solid_get_info",
+    "fsibe_solid_set_info", "fsibe_solid_step", "fsibe_solid_guess", "fsibe_solid_residual",
+    "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid",
 )
 
 _lib = None
@@ -129,6 +147,18 @@ def lib():
     L.fsibe_cofts.argtypes = [dp, dp, C.c_int, C.c_int]
     L.fsibe_cofits.argtypes = [dp, dp, C.c_int, C.c_int]
     L.fsibe_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
+    L.fsibe_solid_create.argtypes = [C.POINTER(FsibeParams), pvp]
+    L.fsibe_solid_destroy.argtypes = [vp]
+    L.fsibe_solid_get.argtypes = [vp, C.c_int, dp]
+    L.fsibe_solid_set.argtypes = [vp, C.c_int, dp]
+    L.fsibe_solid_get_info.argtypes = [vp, C.POINTER(FsibeSolidInfo)]
+    L.fsibe_solid_set_info.argtypes = [vp, C.POINTER(FsibeSolidInfo)]
+    L.fsibe_solid_step.argtypes = [vp, C.c_double, C.c_double]
+    L.fsibe_solid_guess.argtypes = [vp, C.c_double, C.c_double, dp]
+    L.fsibe_solid_residual.argtypes = [vp, C.c_double, C.c_double, dp, dp]
+    L.fsibe_solid_jacobian.argtypes = [vp, C.c_double, C.c_double, dp, dp]
+    L.fsibe_solid_rhs.argtypes = [vp, C.c_double, dp, dp, dp, dp, dp, dp, dp]
+    L.fsibe_sim_solid.argtypes = [vp, pvp]
     _lib = L
     return L
 

@@ -28,6 +28,8 @@
 #include "kernels.hpp"
 #include "kernels_mask.hpp"
 #include "kernels_step.hpp"
+#include "kinematics.hpp"
+#include "solid.hpp"
 
 using namespace fsibe;
 
@@ -424,97 +426,6 @@ extern "C" int fsibe_dealiase_mask(fsibe_spectral *c, double *d) {
   return FSIBE_OK;
 }
 
-// ---------------------------------------------------------------------------
-// host kinematics: mouvement.f90:5-241 (+ cases 600, 601), integrate_position
-// ---------------------------------------------------------------------------
-// periods of the heaving cases 401..413 (mouvement.f90:81-94)
-static const double kTs400[13] = {0.51, 0.34, 0.25, 0.20, 0.17, 0.14, 0.13, 0.11, 0.10, 0.09, 0.08, 0.08, 0.07};
-
-// soft-startup time: cases 401..413 overwrite the global tau with their
-// period on every mouvement call (mouvement.f90:102-104)
-static double tau_eff(const fsibe_params &p) {
-  return (p.iMotion >= 401 && p.iMotion <= 413) ? kTs400[p.iMotion - 401] : p.tau;
-}
-
-static void mouvement_host(const fsibe_params &p, int im, double x0, double y0, double angle, double phase,
-                           double time, double &alpha, double &alpha_t, double &alpha_tt, double LE[6]) {
-  const double pi = 4.0 * atan(1.0);
-  for (int i = 0; i < 6; ++i) LE[i] = 0.0;
-  alpha = alpha_t = alpha_tt = 0.0;
-  if (im == 0) {
-    LE[0] = x0; LE[1] = y0;
-    alpha = angle * pi / 180.0;
-  } else if ((im >= 201 && im <= 213) || (im >= 300 && im <= 350)) {
-    static const double ts[13] = {1.0, 1.09, 1.20, 1.29, 1.33, 1.4, 1.5, 1.71, 2.0, 2.4, 3.0, 4.0, 1.62};
-    const double y_max = 0.5, t_max = (im < 300) ? ts[im - 201] : 1.62;
-    double k, kt, ktt;
-    if (time <= 1.0) {
-      const double a = -20.0, b = 70.0, c = -84.0, d = 35.0;
-      const double t2 = time * time, t3 = t2 * time, t4 = t3 * time, t5 = t4 * time, t6 = t5 * time,
-                   t7 = t6 * time;
-      k = a * t7 + b * t6 + c * t5 + d * t4;
-      kt = 7.0 * a * t6 + 6.0 * b * t5 + 5.0 * c * t4 + 4.0 * d * t3;
-      ktt = 42. * a * t5 + 30. * b * t4 + 20. * c * t3 + 12. * d * t2;
-    } else {
-      k = 1.0; kt = 0.0; ktt = 0.0;
-    }
-    const double arg = 2.0 * pi * time / t_max + pi / 2. + phase;
-    const double w = 2.0 * pi / t_max;
-    const double y = y_max * sin(arg), yt = y_max * (2.0 * pi / t_max) * cos(arg);
-    const double ytt = -(y_max * (w * w) * sin(arg));
-    LE[0] = x0; LE[1] = y0 + k * y; LE[3] = kt * y + yt * k; LE[5] = ktt * y + ytt * k + 2. * kt * yt;
-    alpha = angle * pi / 180.0;
-  } else if (im >= 401 && im <= 413) {
-    const double y_max = 0.0333, t_max = kTs400[im - 401], w = 2.0 * pi / t_max;
-    LE[0] = x0;
-    LE[1] = y0 + y_max * sin(2.0 * pi * time / t_max + pi / 2);
-    LE[3] = y_max * (2.0 * pi / t_max) * cos(2.0 * pi * time / t_max + pi / 2);
-    LE[5] = -(y_max * (w * w) * sin(2.0 * pi * time / t_max + pi / 2));
-    alpha = angle * pi / 180.0;
-  } else if (im == 2) {
-    const double y_max = 0.5, t_max = 2.5, w = 2.0 * pi / t_max;
-    LE[0] = x0;
-    LE[1] = y0 + y_max * sin(2.0 * pi * time / t_max + pi / 2);
-    LE[3] = y_max * (2.0 * pi / t_max) * cos(2.0 * pi * time / t_max + pi / 2);
-    LE[5] = -(y_max * (w * w) * sin(2.0 * pi * time / t_max + pi / 2));
-    alpha = angle * pi / 180.0;
-  } else if (im == 3) {
-    const double angle_max = 80.0 * pi / 180.0, f = 0.5, w = 2.0 * pi * f;
-    LE[0] = x0; LE[1] = y0;
-    alpha = angle_max * sin(2.0 * pi * f * time);
-    alpha_t = angle_max * cos(2.0 * pi * f * time) * (2.0 * pi * f);
-    alpha_tt = -1.0 * angle_max * sin(2.0 * pi * f * time) * (w * w);
-  } else if (im == 4) {
-    const double u0 = -1.0;
-    LE[0] = x0 + u0 * time; LE[2] = u0; LE[1] = y0; LE[3] = 0.0;
-    alpha = angle * pi / 180.0;
-  } else if (im == 600) {
-    const double A = p.heave_amp, w = 2.0 * pi * p.heave_freq;
-    LE[0] = x0;
-    LE[1] = y0 + A * sin(w * time);
-    LE[3] = A * w * cos(w * time);
-    LE[5] = -(A * (w * w) * sin(w * time));
-    alpha = angle * pi / 180.0;
-  } else if (im == 601) {
-    const double *c = p.motion_coef;
-    const double w0 = 2.0 * pi * p.heave_freq;
-    double h = 0, ht = 0, htt = 0, a = 0, at = 0, att = 0;
-    for (int m = 1; m <= 2; ++m) {
-      const double wm = w0 * m, sn = sin(wm * time), cs = cos(wm * time);
-      const double ah = c[2 * (m - 1)], bh = c[2 * (m - 1) + 1];
-      const double ap = c[4 + 2 * (m - 1)], bp = c[4 + 2 * (m - 1) + 1];
-      h += ah * sn + bh * (cs - 1.0);
-      ht += wm * (ah * cs - bh * sn);
-      htt += -(wm * wm) * (ah * sn + bh * cs);
-      a += ap * sn + bp * (cs - 1.0);
-      at += wm * (ap * cs - bp * sn);
-      att += -(wm * wm) * (ap * sn + bp * cs);
-    }
-    LE[0] = x0; LE[1] = y0 + h; LE[3] = ht; LE[5] = htt;
-    alpha = angle * pi / 180.0 + a; alpha_t = at; alpha_tt = att;
-  }
-}
-
 // ---- 1-D transforms of the FFT plugin (cofts / cofits) ---------------------
 // Host-side: their only caller is the beam-curvature filter of the host solid
 // solver (SolidSolver.f90:1388,1417) on L = ns + 2 n_ext points, far too
@@ -589,38 +500,143 @@ extern "C" int fsibe_mouvement(const fsibe_params *p, int im, double x0, double 
   return FSIBE_OK;
 }
 
-// integrate_position.f90:1-47.  The reference recomputes each node's prefix
-// sums from i = 0; carrying them from node s to s+1 performs the same
-// additions in the same order, so the result is identical at O(ns) cost.
-static void integrate_position_host(const fsibe_params &p, int im, double time, const double *theta,
-                                    const double *theta_dot, double *x, double *y, double *vx, double *vy) {
-  double alpha, at, att, LE[6];
-  mouvement_host(p, im, p.x0, p.y0, p.AngleBeam, 0.0, time, alpha, at, att, LE);
-  const int ns = p.ns;
-  const double ds = p.length / (double)(ns - 1);
-  const double ca = cos(alpha), sa = sin(alpha);
-  double xrel = 0.0, yrel = 0.0, vxrel = 0.0, vyrel = 0.0;
-  for (int s = 0; s < ns; ++s) {
-    if (s > 0) {
-      const int i = s - 1;
-      const double c = cos(theta[i]), sn = sin(theta[i]);
-      xrel = xrel + c * ds;
-      yrel = yrel + sn * ds;
-      vxrel = vxrel - theta_dot[i] * sn * ds;
-      vyrel = vyrel + theta_dot[i] * c * ds;
-    }
-    x[s] = LE[0] + xrel * ca - yrel * sa;
-    y[s] = LE[1] + yrel * ca + xrel * sa;
-    vx[s] = LE[2] + ca * (vxrel - at * yrel) - sa * (vyrel + at * xrel);
-    vy[s] = LE[3] + ca * (vyrel + at * xrel) + sa * (vxrel - at * yrel);
-  }
-}
-
 extern "C" int fsibe_integrate_position(const fsibe_params *p, int im, double time, const double *theta,
                                         const double *theta_dot, double *x, double *y, double *vx, double *vy) {
   if (!p || !theta || !theta_dot || !x || !y || !vx || !vy) return set_err(FSIBE_EARG, "null argument");
   if (p->ns < 2) return set_err(FSIBE_EARG, "ns must be >= 2");
   integrate_position_host(*p, im, time, theta, theta_dot, x, y, vx, vy);
+  return FSIBE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host solid solver (solid.hpp)
+// ---------------------------------------------------------------------------
+struct fsibe_solid {
+  ibes::Solver sv;
+  ibes::Beam b;
+};
+
+extern "C" int fsibe_solid_create(const fsibe_params *p, fsibe_solid **out) {
+  if (!p || !out) return set_err(FSIBE_EARG, "null argument");
+  *out = nullptr;
+  if (p->iBeam < 1) return set_err(FSIBE_EARG, "the solid solver needs a beam (iBeam >= 1)");
+  if (!(p->length > 0.0)) return set_err(FSIBE_EARG, "beam length must be > 0");
+  fsibe_solid *b = new fsibe_solid();
+  if (!b->sv.init(*p)) {
+    const std::string e = b->sv.err;
+    delete b;
+    return set_err(FSIBE_EARG, "%s", e.c_str());
+  }
+  ibes::init_beam(*p, b->b);
+  *out = b;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_destroy(fsibe_solid *b) {
+  delete b;
+  return FSIBE_OK;
+}
+
+static std::vector<double> *solid_array(fsibe_solid *b, int which) {
+  ibes::Beam &m = b->b;
+  std::vector<double> *v[FSIBE_SOLID_OLDOLD + 1] = {&m.x, &m.y, &m.vx, &m.vy, &m.theta, &m.theta_dot, &m.ax,
+                                                    &m.ay, &m.pressure_old, &m.pressure_new, &m.tau_old,
+                                                    &m.tau_new, &m.oldold};
+  return (which >= 0 && which <= FSIBE_SOLID_OLDOLD) ? v[which] : nullptr;
+}
+
+extern "C" int fsibe_solid_get(fsibe_solid *b, int which, double *out) {
+  if (!b || !out) return set_err(FSIBE_EARG, "null argument");
+  std::vector<double> *v = solid_array(b, which);
+  if (!v) return set_err(FSIBE_EARG, "unknown solid array %d", which);
+  std::copy(v->begin(), v->end(), out);
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_set(fsibe_solid *b, int which, const double *in) {
+  if (!b || !in) return set_err(FSIBE_EARG, "null argument");
+  std::vector<double> *v = solid_array(b, which);
+  if (!v) return set_err(FSIBE_EARG, "unknown solid array %d", which);
+  std::copy(in, in + v->size(), v->begin());
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_get_info(fsibe_solid *b, fsibe_solid_info *o) {
+  if (!b || !o) return set_err(FSIBE_EARG, "null argument");
+  const ibes::Beam &m = b->b;
+  o->dt_old = m.dt_old;
+  o->grav = b->sv.grav;
+  o->E_kinetic = m.E_kinetic;
+  o->E_pot = m.E_pot;
+  o->E_elastic = m.E_elastic;
+  o->Inertial_Force[0] = m.Inertial_Force[0];
+  o->Inertial_Force[1] = m.Inertial_Force[1];
+  o->StartupStep = m.StartupStep ? 1 : 0;
+  o->iter = b->sv.iter;
+  o->brake = b->sv.brake ? 1 : 0;
+  o->method = b->sv.step_method(m);
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_set_info(fsibe_solid *b, const fsibe_solid_info *in) {
+  if (!b || !in) return set_err(FSIBE_EARG, "null argument");
+  b->b.dt_old = in->dt_old;
+  b->sv.grav = in->grav;
+  b->b.StartupStep = in->StartupStep != 0;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_step(fsibe_solid *b, double time, double dt) {
+  if (!b) return set_err(FSIBE_EARG, "null solid");
+  if (!(dt > 0.0)) return set_err(FSIBE_EARG, "dt must be > 0");
+  if (b->sv.step_method(b->b) == ibes::kBDF2 && !(b->b.dt_old > 0.0))
+    return set_err(FSIBE_ESTATE, "BDF2 step without a previous step (dt_old = 0)");
+  if (!b->sv.step(time, dt, b->b)) return set_err(FSIBE_EDOMAIN, "%s", b->sv.err.c_str());
+  return FSIBE_OK;
+}
+
+// context of the next step without taking it (test hooks)
+static int solid_context(fsibe_solid *b, double time, double dt) {
+  if (!(dt > 0.0)) return set_err(FSIBE_EARG, "dt must be > 0");
+  const int m = b->sv.step_method(b->b);
+  if (m == ibes::kBDF2 && !(b->b.dt_old > 0.0)) return set_err(FSIBE_ESTATE, "BDF2 step without dt_old");
+  if (!b->sv.begin(time, dt, b->b, m)) return set_err(FSIBE_EDOMAIN, "%s", b->sv.err.c_str());
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_guess(fsibe_solid *b, double time, double dt, double *x) {
+  if (!b || !x) return set_err(FSIBE_EARG, "null argument");
+  if (!(dt > 0.0)) return set_err(FSIBE_EARG, "dt must be > 0");
+  if (!b->sv.guess(time, dt, b->b, x)) return set_err(FSIBE_EDOMAIN, "%s", b->sv.err.c_str());
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_residual(fsibe_solid *b, double time, double dt, const double *x, double *F) {
+  if (!b || !x || !F) return set_err(FSIBE_EARG, "null argument");
+  CHK(solid_context(b, time, dt));
+  b->sv.residual(x, [&](int k, double v) { F[k - 1] = v; });
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_jacobian(fsibe_solid *b, double time, double dt, const double *x, double *J) {
+  if (!b || !x || !J) return set_err(FSIBE_EARG, "null argument");
+  CHK(solid_context(b, time, dt));
+  const size_t n = (size_t)b->sv.nx;
+  std::fill(J, J + n * n, 0.0);
+  b->sv.jacobian(x, [&](int k, int l, double v) { J[(size_t)(k - 1) * n + (l - 1)] = v; });
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_rhs(fsibe_solid *b, double time, const double *theta, const double *theta_dot,
+                               const double *pressure, const double *tau, double *theta_t, double *theta_tt,
+                               double *T) {
+  if (!b || !theta || !theta_dot || !pressure || !tau || !theta_t || !theta_tt || !T)
+    return set_err(FSIBE_EARG, "null argument");
+  const int ns = b->sv.ns;
+  std::copy(theta, theta + ns, theta_t);
+  std::copy(theta_dot, theta_dot + ns, theta_tt);
+  if (!b->sv.rhs_beameqn(time, theta_t, theta_tt, pressure, tau, T))
+    return set_err(FSIBE_EDOMAIN, "%s", b->sv.err.c_str());
   return FSIBE_OK;
 }
 
@@ -712,6 +728,7 @@ struct fsibe_sim {
   bool startup_done = false;
   fsibe_step_out last;
   std::vector<double> pbeam, tbeam;
+  fsibe_solid *solid = nullptr;  // iFLUSI=1: the beam solved by the library (fsibe_run)
 };
 
 static void prof_begin(fsibe_sim *s, int k) {
@@ -840,6 +857,7 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
     for (int j = 0; j < 2; ++j)
       if (s->prof.ev[k][j]) (void)hipEventDestroy(s->prof.ev[k][j]);
   (void)hipStreamDestroy(s->stream);
+  delete s->solid;
   delete s;
   return FSIBE_OK;
 }
@@ -1479,6 +1497,7 @@ static void parse_results(fsibe_sim *s, int k) {
   r.dt1 = o->dt1;
   r.u_max = o->u_max;
   r.status = 0;
+  r.ibes_iter = 0;
   const double dxdy = s->dx * s->dy;
   if (s->slot_fluid[k]) {
     s->slot_fluid[k] = false;
@@ -1579,7 +1598,38 @@ extern "C" int fsibe_get_step_out(fsibe_sim *s, fsibe_step_out *out) {
   return FSIBE_OK;
 }
 
-// ---- time loop (time_step.f90:120-162, :178-341), prescribed kinematics ---
+// ---- time loop (time_step.f90:120-162, :178-341) ---------------------------
+// the beam of the library's solid solver, for iFLUSI=1
+static int ensure_solid(fsibe_sim *s) {
+  if (s->solid) return FSIBE_OK;
+  if (s->p.iBeam < 1) return set_err(FSIBE_EARG, "iFLUSI=1 without a beam");
+  return fsibe_solid_create(&s->p, &s->solid);
+}
+
+static void beam_from_solid(fsibe_sim *s) {
+  const ibes::Beam &b = s->solid->b;
+  s->bx = b.x; s->by = b.y; s->bvx = b.vx; s->bvy = b.vy; s->btheta = b.theta; s->btheta_dot = b.theta_dot;
+}
+
+// GetForces' loads -> GravityImpulse + SolidSolverWrapper(time, dt1) ->
+// pressure_old = pressure_new (time_step.f90:199-228)
+static int solid_step(fsibe_sim *s) {
+  ibes::Beam &b = s->solid->b;
+  for (int i = 0; i < s->ns; ++i) { b.pressure_new[i] = s->pbeam[i]; b.tau_new[i] = s->tbeam[i]; }
+  if (!s->solid->sv.step(s->time, s->dt1, b)) return set_err(FSIBE_EDOMAIN, "%s", s->solid->sv.err.c_str());
+  beam_from_solid(s);
+  s->last.ibes_iter = s->solid->sv.iter;
+  s->last.status = s->solid->sv.brake ? 1 : 0;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_sim_solid(fsibe_sim *s, fsibe_solid **out) {
+  if (!s || !out) return set_err(FSIBE_EARG, "null argument");
+  CHK(ensure_solid(s));
+  *out = s->solid;
+  return FSIBE_OK;
+}
+
 // One step of the loop.  pipe: do not wait for the step to finish; wait only
 // for dt1 (written by the nonlinear-term kernel), read the previous step's
 // results, and return so that the host prepares the next step's geometry
@@ -1618,9 +1668,12 @@ static int run_one(fsibe_sim *s, bool pipe) {
     s->res_pending = true;
     s->slot ^= 1;
   }
-  if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0)
+  if (p.iFLUSI == 1 && p.iBeam > 0) {
+    CHK(solid_step(s));
+  } else if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0) {
     integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
                             s->by.data(), s->bvx.data(), s->bvy.data());
+  }
   std::swap(s->n0, s->n1);
   s->time = s->time + s->dt1;
   if (s->startup_done) s->it++;
@@ -1644,17 +1697,22 @@ static void init_beam_host(fsibe_sim *s) {
 
 extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
-  if (s->p.iFLUSI == 1)
-    return set_err(FSIBE_EUNSUPPORTED, "iFLUSI=1 needs the host solid solver: drive the step with "
-                                       "fsibe_create_mask / fsibe_evolve_* / fsibe_get_forces and solve the "
-                                       "beam on the host");
-  if (!s->startup_done && s->time == 0.0 && s->p.iBeam > 0 && !s->have_beam) init_beam_host(s);
-  const bool pipe = !s->prof.on;  // kernel timing needs a synchronisation per step
+  const bool fsi = s->p.iFLUSI == 1 && s->p.iBeam > 0;
+  if (fsi) {
+    CHK(ensure_solid(s));
+    if (!s->startup_done && !s->have_beam) beam_from_solid(s);
+  } else if (!s->startup_done && s->time == 0.0 && s->p.iBeam > 0 && !s->have_beam) {
+    init_beam_host(s);
+  }
+  // kernel timing needs a synchronisation per step; so does the solid
+  // solver, which needs the step's loads before the next step's mask
+  const bool pipe = !s->prof.on && !fsi;
   int rc = FSIBE_OK;
   for (int k = 0; k < nsteps; ++k) {
     if (!(s->time < s->p.Time_end)) break;
     rc = run_one(s, pipe);
     if (rc != FSIBE_OK) break;
+    if (s->last.status == 1) break;  // continue_timestep = .false.
   }
   s->pipe = false;
   if (s->res_pending) {

@@ -34,6 +34,8 @@ contains
     p%T_fullspeed = T_fullspeed; p%T_release = T_release; p%tau = tau
     p%U_mean_true = U_mean_true
     p%heave_amp = 0.0_c_double; p%heave_freq = 0.0_c_double; p%motion_coef = 0.0_c_double
+    p%mue = mue; p%eta = eta; p%sigma = sigma; p%grav = grav
+    p%TimeMethodSolid = TimeMethodSolid; p%iImpulse = iImpulse
     call fsibe_check(fsibe_sim_create(p, fsibe_sim), "fsibe_sim_create")
     call fsibe_check(fsibe_set_vortk(fsibe_sim, vortk_n0), "fsibe_set_vortk")
   end subroutine hip_fluid_create

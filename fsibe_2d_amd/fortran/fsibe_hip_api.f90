@@ -29,6 +29,8 @@ module fsibe_hip_api
     real(c_double) :: U_mean_true
     real(c_double) :: heave_amp, heave_freq
     real(c_double) :: motion_coef(8)
+    real(c_double) :: mue, eta, sigma, grav
+    integer(c_int32_t) :: TimeMethodSolid, iImpulse
   end type
 
   ! struct fsibe_step_out
@@ -38,7 +40,7 @@ module fsibe_hip_api
     real(c_double) :: energies(6)
     real(c_double) :: bsi(10)
     real(c_double) :: fx_int, fy_int
-    integer(c_int32_t) :: status, pad
+    integer(c_int32_t) :: status, ibes_iter
   end type
 
   interface

@@ -56,6 +56,14 @@ class Params:
     heave_amp: float = 0.1
     heave_freq: float = 1.0
     motion_coef: tuple = (0.0,) * 8
+    # beam material for the host solid solver (iFLUSI=1): mass ratio,
+    # stiffness, damping, gravity, time scheme (5 = BDF2), gravity impulse
+    mue: float = 1.0
+    eta: float = 0.1
+    sigma: float = 0.0
+    grav: float = 0.0
+    TimeMethodSolid: int = 5
+    iImpulse: int = 0
 
     def to_c(self) -> FsibeParams:
         p = FsibeParams()
@@ -79,19 +87,23 @@ class Params:
         return self.yl / self.ny
 
 
-def fsi_256() -> Params:
+def fsi_256(coupled: bool = True) -> Params:
     """256^2, domain 4x4, Re=200, vorticity sponge 0.2 wide around the domain
     (iSponge=3) with inflow U=1 (iMeanVelocity=3), beam of length 1 with 64
     nodes clamped at (1,2), heave 0.1 sin(2 pi t), eta=1e-3, 2 smoothing cells,
-    2/3 dealiasing, dt=2e-4.  Choices left open by the workload description:
-    eps_sponge = eps = 1e-3, t_beam = 0.0312 (PARAMS.m default), tau = 0."""
-    return Params()
+    2/3 dealiasing, dt=2e-4.  coupled: the flexible beam (mass ratio 1,
+    stiffness 0.1) solved by the solid solver (iFLUSI=1); otherwise the beam
+    follows the heave rigidly (iFLUSI=0).  Choices left open by the workload
+    description: eps_sponge = eps = 1e-3, t_beam = 0.0312 (PARAMS.m default),
+    tau = 0, no structural damping (sigma = 0), BDF2 for the beam."""
+    return Params(iFLUSI=1 if coupled else 0, mue=1.0, eta=0.1)
 
 
-def fsi_large(n: int = 1024) -> Params:
+def fsi_large(n: int = 1024, coupled: bool = True) -> Params:
     """The same physics at n^2 (1024 or 2048), Re=1000, 128 beam nodes,
-    dt = 5e-5."""
-    return Params(nx=n, ny=n, ns=128, nu=1.0 / 1000.0, dt_fixed=5e-5)
+    dt = 5e-5, beam mass ratio 0.5 and stiffness 0.05."""
+    return Params(nx=n, ny=n, ns=128, nu=1.0 / 1000.0, dt_fixed=5e-5, iFLUSI=1 if coupled else 0,
+                  mue=0.5, eta=0.05)
 
 
 def turbulence_4096(n: int = 4096) -> Params:

@@ -6,7 +6,8 @@
     p, tau, Fp = sim.get_forces(time)                BeamForces.f90:6-288
     sim.swap_levels()                                time_step.f90:152, 222
     sim.run(nsteps)                                  time_step.f90:120-341
-                                                     (prescribed kinematics)
+                                                     (prescribed kinematics, or
+                                                     iFLUSI=1 with sim.solid)
 
 State lives in GPU memory between calls.  Every method raises FsibeError on a
 nonzero status; there is no CPU path.
@@ -30,7 +31,7 @@ BOX_FIELDS = {"press": 0, "stress_a": 1, "stress_b": 2, "mask": 3, "maskvx": 4, 
 def step_out_dict(o: FsibeStepOut) -> dict:
     return dict(dt1=o.dt1, u_max=o.u_max, Force=tuple(o.Force), Force_unst=tuple(o.Force_unst),
                 Force_press=tuple(o.Force_press), energies=tuple(o.energies), bsi=tuple(o.bsi),
-                fx_int=o.fx_int, fy_int=o.fy_int)
+                fx_int=o.fx_int, fy_int=o.fy_int, status=o.status, ibes_iter=o.ibes_iter)
 
 
 def mouvement(p: Params, time: float, iMouvement: int | None = None, phase: float = 0.0):
@@ -147,6 +148,15 @@ class FluidSim:
     @property
     def stream(self):
         return lib().fsibe_sim_stream(self.h)
+
+    @property
+    def solid(self):
+        """the solid solver fsibe_run uses for iFLUSI=1 (fsibe_sim_solid)"""
+        from .solid import Solid
+
+        h = C.c_void_p()
+        check(lib().fsibe_sim_solid(self.h, C.byref(h)))
+        return Solid(self.p, handle=h.value, owner=self)
 
 
 class _Member(FluidSim):

@@ -115,6 +115,11 @@ typedef struct fsibe_params {
   double U_mean_true;
   double heave_amp, heave_freq;
   double motion_coef[8];
+  /* the beam's material for the host solid solver (share_vars.f90:17-21):
+   * mass ratio, stiffness, damping, gravity, time scheme (1 implicit Euler,
+   * 2 Crank-Nicolson, 5 BDF2), gravity impulse case */
+  double mue, eta, sigma, grav;
+  int32_t TimeMethodSolid, iImpulse;
 } fsibe_params;
 
 /* Per-step results copied back to the host (type Solid fields filled by
@@ -126,7 +131,8 @@ typedef struct fsibe_step_out {
   double energies[6]; /* vor_rms kinetic dissipation porous_diss inflow_solid inflow_mean (NavierStokes.f90:236-256) */
   double bsi[10];     /* drag lift drag_surf lift_surf Ft1 Fn1 Fp1 Ft2 Fn2 Fp2 (BeamForces.f90:485-496) */
   double fx_int, fy_int; /* BeamForces.f90:279 */
-  int32_t status, pad;
+  int32_t status;    /* 1: the solid solver stopped the run (emergency brake) */
+  int32_t ibes_iter; /* Newton iterations of the step's solid solve (iFLUSI=1) */
 } fsibe_step_out;
 
 typedef struct fsibe_sim fsibe_sim;
@@ -194,13 +200,78 @@ int fsibe_integrate_position(const fsibe_params *p, int iMouvement, double time,
                              const double *theta, const double *theta_dot, double *x, double *y,
                              double *vx, double *vy);
 
-/* ---- the time loop of time_step.f90:120-341 for prescribed kinematics ------
- * (iFLUSI=0, iMotion>0 or a fixed beam): the startup Euler step if not yet
- * taken, then AB2 steps, each create_mask -> evolve -> GetForces ->
- * integrate_position -> swap -> time += dt1.  Runs nsteps steps or until
- * Time_end.  out (may be NULL) receives the last step's results.  iFLUSI=1
- * (beam solved by the host's solid solver) is FSIBE_EUNSUPPORTED here: such
- * a host calls the per-step entry points above itself. */
+/* ---- host solid solver (SolidSolver.f90) -----------------------------------
+ * One beam: the Newton solver IBES_solver (SolidSolver.f90:99-414) for the
+ * angle theta and tension T of the inextensible beam, time scheme
+ * TimeMethodSolid (1 implicit Euler, 2 Crank-Nicolson, 5 BDF2 starting with
+ * one Crank-Nicolson step), material mue, eta, sigma, grav of fsibe_params.
+ * A host without the reference's solver (it links MKL) uses this one; the
+ * Fortran module fsibe_2d_amd/fortran/solid_hip.f90 wraps it as
+ * InitializeSolidSolver / SolidSolverWrapper. */
+typedef struct fsibe_solid fsibe_solid;
+/* init_beams for one beam (init_beam.f90:80-120: straight beam at the leading
+ * edge of mouvement(0), integrate_position(0)) and InitializeSolidSolver
+ * (SolidSolver.f90:54-67).  Needs ns >= 8. */
+int fsibe_solid_create(const fsibe_params *p, fsibe_solid **out);
+int fsibe_solid_destroy(fsibe_solid *b);
+/* arrays of type(solid) (share_vars.f90:50-63), ns values each; OLDOLD is
+ * beam_oldold(0:ns-1, 1:6), 6*ns values, column-major (x y vx vy theta
+ * theta_dot) */
+enum {
+  FSIBE_SOLID_X = 0, FSIBE_SOLID_Y = 1, FSIBE_SOLID_VX = 2, FSIBE_SOLID_VY = 3, FSIBE_SOLID_THETA = 4,
+  FSIBE_SOLID_THETA_DOT = 5, FSIBE_SOLID_AX = 6, FSIBE_SOLID_AY = 7, FSIBE_SOLID_PRESSURE_OLD = 8,
+  FSIBE_SOLID_PRESSURE_NEW = 9, FSIBE_SOLID_TAU_OLD = 10, FSIBE_SOLID_TAU_NEW = 11, FSIBE_SOLID_OLDOLD = 12
+};
+int fsibe_solid_get(fsibe_solid *b, int which, double *out);
+int fsibe_solid_set(fsibe_solid *b, int which, const double *in);
+/* scalars of type(solid) and of the solver */
+typedef struct fsibe_solid_info {
+  double dt_old;             /* previous step, for BDF2 */
+  double grav;               /* gravity after GravityImpulse (SolidSolver.f90:874-911) */
+  double E_kinetic, E_pot, E_elastic; /* SolidEnergies (SolidSolver.f90:76-92) */
+  double Inertial_Force[2];
+  int32_t StartupStep;       /* 1: the next IBES step is the first */
+  int32_t iter;              /* Newton iterations of the last step (file IBES_iter) */
+  int32_t brake;             /* 1: the last step tripped the emergency brake
+                              * max|theta_dot - theta_dot_old| > 100 (SolidSolver.f90:370-380) */
+  int32_t method;            /* scheme of the next IBES step */
+} fsibe_solid_info;
+int fsibe_solid_get_info(fsibe_solid *b, fsibe_solid_info *out);
+/* restore dt_old, grav and StartupStep (e.g. from a backup file) */
+int fsibe_solid_set_info(fsibe_solid *b, const fsibe_solid_info *in);
+/* GravityImpulse(time); SolidSolverWrapper(time, dt, beams): an IBES step if
+ * time > T_release, else integrate_position(time + dt); SolidEnergies; then
+ * pressure_old = pressure_new, tau_old = tau_new (time_step.f90:209-228).
+ * The loads pressure_new, tau_new are those GetForces wrote for this step
+ * (set them with fsibe_solid_set).  FSIBE_EDOMAIN if Newton does not
+ * converge in 49 iterations (the reference stops, SolidSolver.f90:311-314). */
+int fsibe_solid_step(fsibe_solid *b, double time, double dt);
+/* The Newton system of the next IBES step t -> t + dt, x of 2*ns+4 values
+ * [theta(-1..ns+1), T(-1..ns-1)]: the predictor (EE1 and the ghost values,
+ * SolidSolver.f90:167-209), F_nonlinear (:552-667) and the Jacobian
+ * J[k*(2ns+4) + l] = dF_k/dx_l, which is transpose(J) of Jacobi (:671-863),
+ * the matrix Solve_LGS factors. */
+int fsibe_solid_guess(fsibe_solid *b, double time, double dt, double *x);
+int fsibe_solid_residual(fsibe_solid *b, double time, double dt, const double *x, double *F);
+int fsibe_solid_jacobian(fsibe_solid *b, double time, double dt, const double *x, double *J);
+/* RHS_beameqn (SolidSolver.f90:918-994) at time: theta_t (= theta_dot with
+ * theta_dot(0) = 0), theta_tt and the tension T (Tension, :1000-1081) */
+int fsibe_solid_rhs(fsibe_solid *b, double time, const double *theta, const double *theta_dot,
+                    const double *pressure, const double *tau, double *theta_t, double *theta_tt,
+                    double *T);
+/* the solver fsibe_run uses for iFLUSI=1 (created by its first call) */
+int fsibe_sim_solid(fsibe_sim *sim, fsibe_solid **out);
+
+/* ---- the time loop of time_step.f90:120-341 --------------------------------
+ * The startup Euler step if not yet taken, then AB2 steps, each create_mask
+ * -> evolve -> GetForces -> beam -> swap -> time += dt1, where the beam is
+ * integrate_position for prescribed kinematics (iFLUSI=0) or, for iFLUSI=1,
+ * GravityImpulse + SolidSolverWrapper of the library's solid solver
+ * (fsibe_sim_solid).  Runs nsteps steps, or until Time_end, or until the
+ * solid solver's emergency brake (then out->status = 1, as the reference
+ * ends its loop with continue_timestep = .false.).  out (may be NULL)
+ * receives the last step's results; out->ibes_iter the Newton iterations of
+ * its solid step. */
 int fsibe_run(fsibe_sim *sim, int nsteps, fsibe_step_out *out);
 /* current time, dt1 and number of AB2 steps taken by fsibe_run */
 int fsibe_get_time(fsibe_sim *sim, double *time, double *dt1, int *it);

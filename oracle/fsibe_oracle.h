@@ -52,6 +52,9 @@ typedef struct o_params {
   /* motion case 600 (sinusoidal heave) and 601 (Fourier heave + pitch) */
   double heave_amp, heave_freq;
   double motion_coef[8];
+  /* beam material (used by the host solid solver, not by the oracle) */
+  double mue, eta, sigma, grav;
+  int32_t TimeMethodSolid, iImpulse;
 } o_params;
 
 /* One beam (reference share_vars.f90:50-63, type Solid); arrays length ns. */

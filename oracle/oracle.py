@@ -38,6 +38,8 @@ class OParams(C.Structure):
         ("tau", C.c_double), ("U_mean_true", C.c_double),
         ("heave_amp", C.c_double), ("heave_freq", C.c_double),
         ("motion_coef", C.c_double * 8),
+        ("mue", C.c_double), ("eta", C.c_double), ("sigma", C.c_double), ("grav", C.c_double),
+        ("TimeMethodSolid", C.c_int32), ("iImpulse", C.c_int32),
     ]
 
 
