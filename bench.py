@@ -1,20 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: FSI steps/s of the FP64 fluid step at 1024^2 on one MI355X.
+"""Benchmark: FSI steps/s of the FP64 fluid-structure step at 1024^2 on one MI355X.
 
 A step is one pass of the reference's time loop body (time_step.f90:178-341)
 over the 1024^2 workload: create_mask -> EvolveFluidAB2 (cal_nlk, AB2 update,
 pressure, cal_drag, cal_energy, time_avg_vort) -> GetForces (stresses, Delta
-interpolation, surface integral) -> leading-edge kinematics -> swap, with all
-fields resident in HBM.  The beam is driven by prescribed heave
-0.1 sin(2 pi t) (the Newton solve of the flexible beam stays on the host and
-is not part of this path).  Inputs: band-limited Gaussian vorticity noise
-(seed 0), synthetic.
+interpolation, surface integral) -> GravityImpulse + SolidSolverWrapper (the
+flexible beam, mass ratio 0.5, stiffness 0.05, clamped at a leading edge
+heaving 0.1 sin(2 pi t), solved by Newton iterations on the host) -> swap,
+with all fields resident in HBM.  Inputs: band-limited Gaussian vorticity
+noise (seed 0), synthetic.  Also reported: the same step with the beam
+following the heave rigidly (prescribed kinematics, no solid solve), and the
+host solid solver's own time per step.
 
 Prints one JSON line.  roofline: the kernel with the largest total time,
 its algorithmic bytes per launch (compulsory reads + writes of its operands)
 divided by its mean duration measured with HIP events on the simulation's
-stream.  cpu_baseline: the CPU oracle (C/OpenMP restatement of the same step)
-on a bounded sample of the same workload, on this host's cores.
+stream.  cpu_baseline: the CPU oracle (C/OpenMP restatement of the fluid step
+and numpy restatement of the solid solver) on a bounded sample of the same
+workload, on this host's cores.
 """
 from __future__ import annotations
 
@@ -30,12 +33,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# kernel classes of fsibe_profile_read (4, 8 and 12 are not launched: the
+# kernel classes of fsibe_profile_read; 4, 8 and 12 are host phases (the
 # dt/AB2 scalars run inside rows_nonlinear, and the host adds the row sums
 # and per-node force terms the kernels leave in the pinned result record)
-KERNELS = ["create_mask", "startup_velocity", "cols_vortk_n0", "rows_nonlinear", "unused4",
-           "cols_ab2_update", "cols_vortk_n1", "rows_velocity_pressure", "unused8",
-           "cols_pressure", "rows_pressure_box", "delta_interp_forces", "unused12"]
+KERNELS = ["create_mask", "startup_velocity", "cols_vortk_n0", "rows_nonlinear", "host_beam_solve",
+           "cols_ab2_update", "cols_vortk_n1", "rows_velocity_pressure", "host_mask_geometry",
+           "cols_pressure", "rows_pressure_box", "delta_interp_forces", "host_wait_results"]
 
 
 def alg_bytes(name: str, nx: int, ny: int, visc: bool, box_rows: int, box_cols: int,
@@ -213,23 +216,66 @@ def fft_anchor(n: int, threads: int, reps: int = 10):
 
 
 def cpu_baseline(p, seconds: float, threads: int):
+    """coupled steps/s of the oracle: its fluid step (C/OpenMP) and its solid
+    solver (numpy) in the loop of time_step.f90:178-230"""
     from oracle import oracle
+    from oracle.solid import SolidOracle
 
     oracle.set_threads(threads)
     o = oracle.Oracle(oracle.make_params(**{k: getattr(p, k) for k in (
-        "nx", "ny", "ns", "nu", "dt_fixed", "iMotion", "heave_amp", "heave_freq")}))
-    vk = o.init_noise(seed=0, amp=1e-3, kmax2=64)
-    o.set_vortk(vk)
-    o.run(2)  # startup + one AB2 step (warm-up, not timed)
+        "nx", "ny", "ns", "nu", "dt_fixed", "iMotion", "heave_amp", "heave_freq", "iFLUSI")}))
+    so = SolidOracle(o, mue=p.mue, eta=p.eta, sigma=p.sigma, grav=p.grav, TimeMethodSolid=p.TimeMethodSolid,
+                     iImpulse=p.iImpulse)
+    o.set_vortk(o.init_noise(seed=0, amp=1e-3, kmax2=64))
+    state = {"time": 0.0, "dt0": 0.0, "k": 0}
+
+    def step():
+        t = state["time"]
+        o.set_beam_arrays(x=so.x, y=so.y, vx=so.vx, vy=so.vy, theta=so.theta, theta_dot=so.theta_dot)
+        o.host_create_mask(t)
+        dt1 = o.host_evolve(t, explicit=True) if state["k"] == 0 else o.host_evolve(t, state["dt0"])
+        o.host_get_forces(t)
+        b = o.beam_arrays()
+        so.pressure_new, so.tau_new = b["pressure"].copy(), b["tau"].copy()
+        so.step(t, dt1)
+        o.host_swap()
+        state.update(time=t + dt1, dt0=dt1, k=state["k"] + 1)
+
+    step()
+    step()  # startup + one AB2 step (warm-up, not timed)
     steps, t0 = 0, time.perf_counter()
     while True:
-        o.run(1)
+        step()
         steps += 1
         el = time.perf_counter() - t0
         if el >= seconds or steps >= 50:
             break
     o.close()
     return steps / el, steps, el
+
+
+def solid_step_us(p, steps: int = 2000):
+    """host time of one solid-solver step (GravityImpulse + SolidSolverWrapper)
+    of the workload's beam under a steady load, on this host"""
+    import numpy as np
+
+    from fsibe_2d_amd.solid import Solid
+
+    b = Solid(p)
+    s = np.linspace(0.0, 1.0, p.ns)
+    b.set("pressure_new", 0.3 * s)
+    t = 0.0
+    for _ in range(20):
+        b.step(t, p.dt_fixed)
+        t += p.dt_fixed
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step(t, p.dt_fixed)
+        t += p.dt_fixed
+    el = time.perf_counter() - t0
+    it = b.info()["iter"]
+    b.close()
+    return round(1e6 * el / steps, 2), it
 
 
 def batch_bench(nsims: int, steps: int, warmup: int = 3):
@@ -339,6 +385,19 @@ def main(argv=None) -> int:
     el = time.perf_counter() - t0
     steps_per_s = args.steps / el
 
+    # the same step with the beam following the heave rigidly (iFLUSI = 0)
+    pr = fsi_large(n, coupled=False)
+    simr = FluidSim(pr)
+    simr.set_vortk(vk)
+    simr.run(args.warmup)
+    hip_sync()
+    t0 = time.perf_counter()
+    simr.run(args.steps)
+    hip_sync()
+    el_r = time.perf_counter() - t0
+    simr.close()
+    solid_us, solid_iter = solid_step_us(p)
+
     roofline = None
     prof = {}
     if not args.no_profile:
@@ -394,8 +453,8 @@ def main(argv=None) -> int:
         anchor = fft_anchor(n, threads)
         ratio = anchor["oracle_over_scipy"]
         cpu = {"value": round(rate, 4), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"{k} AB2 steps of the same {n}^2 FSI workload after startup + 1 step, "
-                         f"{sec:.1f} s, C/OpenMP oracle with {threads} threads"
+               "sample": f"{k} coupled AB2 steps of the same {n}^2 FSI workload after startup + 1 step, "
+                         f"{sec:.1f} s, C/OpenMP fluid oracle with {threads} threads, numpy solid oracle"
                          + (f"; one {n}^2 r2c FFT: oracle {anchor['oracle_rfft2_ms']} ms, "
                             f"scipy.fft.rfft2 ({threads} workers) {anchor['scipy_rfft2_ms']} ms, "
                             f"time ratio {ratio}" if ratio else "")}
@@ -416,9 +475,17 @@ def main(argv=None) -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"fsi_{n}: {n}^2 periodic grid, domain 4x4, Re=1000, beam 128 nodes "
-                               f"heaving 0.1 sin(2 pi t), dt=5e-5, iSponge=3, iViscous=1, forces every step",
-                   "grid": n, "ns": 128, "dt": 5e-5},
+        "config": {"workload": f"fsi_{n}: {n}^2 periodic grid, domain 4x4, Re=1000, flexible beam of 128 "
+                               f"nodes (mass ratio 0.5, stiffness 0.05, BDF2) clamped at a leading edge "
+                               f"heaving 0.1 sin(2 pi t), dt=5e-5, iSponge=3, iViscous=1, forces and beam "
+                               f"solve every step",
+                   "grid": n, "ns": 128, "dt": 5e-5, "iFLUSI": 1},
+        "prescribed_motion": {"value": round(args.steps / el_r, 3), "unit": "steps/s",
+                              "ms_per_step": round(1e3 * el_r / args.steps, 5),
+                              "note": "same step, beam following the heave rigidly (iFLUSI=0): the host "
+                                      "prepares step n+1 while step n runs"},
+        "host_solid_step_us": solid_us,
+        "host_solid_newton_iters": solid_iter,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_fft_anchor": anchor,

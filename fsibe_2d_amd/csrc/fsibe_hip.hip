@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <functional>
@@ -693,6 +694,7 @@ struct fsibe_sim {
   double *fb_p, *fb_sa, *fb_sb;
   // per-step upload area: prims | targets | node data | circle data | vals
   char *h_stage, *d_stage;
+  char *m_stage = nullptr;  // device address of h_stage (mapped)
   size_t stage_bytes;
   // two result slots (StepOutDev, p_beam[nsb], tau_beam[nsb]) in pinned,
   // host-coherent memory that the kernels write directly; d_res is the
@@ -703,6 +705,11 @@ struct fsibe_sim {
   bool res_pending = false;  // pipelined run: results of the previous step not yet read
   bool pipe = false;         // record ev_dt after the nonlinear-term kernel
   hipEvent_t ev_dt;
+  hipEvent_t ev_step;        // after a step's last kernel (iFLUSI=1 loop)
+  // the next step's first column pass (it reads only vortk) already runs:
+  // launched for time level k1_level before the host solves the beam
+  bool k1_ready = false;
+  int k1_level = -1;
   // what each slot holds: the fluid step's row sums, the force terms
   bool slot_fluid[2] = {false, false}, slot_forces[2] = {false, false};
   int slot_units[2] = {0, 0};
@@ -730,6 +737,32 @@ struct fsibe_sim {
   std::vector<double> pbeam, tbeam;
   fsibe_solid *solid = nullptr;  // iFLUSI=1: the beam solved by the library (fsibe_run)
 };
+
+// host phases timed with the steady clock into the otherwise unused classes
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+static void prof_host(fsibe_sim *s, int k, double t0) {
+  if (s->prof.on && (s->prof.mask >> k & 1u)) {
+    s->prof.ms[k] += now_ms() - t0;
+    s->prof.cnt[k] += 1;
+  }
+}
+
+// stage bytes [off, off + bytes) of the host record into device memory, on
+// the step's stream; then ev_upload marks the host record free again
+static int upload_stage(fsibe_sim *s, size_t off, size_t bytes) {
+  const int n = (int)(bytes / sizeof(double2));
+  if (n > 0) {
+    const int blocks = std::min(64, (n + 255) / 256);
+    k_stage_copy<<<blocks, 256, 0, s->stream>>>(reinterpret_cast<const double2 *>(s->m_stage + off),
+                                              reinterpret_cast<double2 *>(s->d_stage + off), n);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(s->ev_upload, s->stream));
+  s->upload_pending = true;
+  return FSIBE_OK;
+}
 
 static void prof_begin(fsibe_sim *s, int k) {
   if (s->prof.on && (s->prof.mask >> k & 1u)) (void)hipEventRecord(s->prof.ev[k][0], s->stream);
@@ -853,6 +886,7 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   if (s->h_res) (void)hipHostFree(s->h_res);
   (void)hipEventDestroy(s->ev_upload);
   (void)hipEventDestroy(s->ev_dt);
+  (void)hipEventDestroy(s->ev_step);
   for (int k = 0; k < P_NK; ++k)
     for (int j = 0; j < 2; ++j)
       if (s->prof.ev[k][j]) (void)hipEventDestroy(s->prof.ev[k][j]);
@@ -928,6 +962,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipStreamCreate(&s->stream));
   SIMCHK(hipEventCreateWithFlags(&s->ev_upload, hipEventDisableTiming));
   SIMCHK(hipEventCreateWithFlags(&s->ev_dt, hipEventDisableTiming));
+  SIMCHK(hipEventCreateWithFlags(&s->ev_step, hipEventDisableTiming));
   const size_t N = (size_t)s->nx * s->ny, NH = (size_t)s->H * s->ny;
   auto tx = twiddles(s->nx), ty = twiddles(s->ny);
   SIMCHK(hipMalloc(&s->twx, sizeof(double2) * s->nx));
@@ -982,7 +1017,8 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     s->off_val = al(sizeof(double) * ntg_max);
     s->stage_bytes = off;
   }
-  SIMCHK(hipHostMalloc((void **)&s->h_stage, s->stage_bytes, hipHostMallocDefault));
+  SIMCHK(hipHostMalloc((void **)&s->h_stage, s->stage_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+  SIMCHK(hipHostGetDevicePointer((void **)&s->m_stage, s->h_stage, 0));
   SIMCHK(hipMalloc(&s->d_stage, s->stage_bytes));
   SIMCHK(hipMemset(s->d_stage, 0, s->stage_bytes));
   s->ms = nullptr;
@@ -1031,7 +1067,8 @@ static void add_prim(fsibe_sim *s, Prim &P, int &np) {
   dst[np++] = P;
 }
 
-static int build_geometry(fsibe_sim *s, double time) {
+// the mask's primitives and box, and the cal_drag window (create_mask.f90)
+static int build_mask_geometry(fsibe_sim *s) {
   const int nx = s->nx, ny = s->ny, ns = s->ns;
   const double dx = s->dx, dy = s->dy, N = s->p.N_smooth, t = s->p.t_beam;
   const double mdx = std::max(dx, dy);
@@ -1148,8 +1185,14 @@ static int build_geometry(fsibe_sim *s, double time) {
     iymin = std::max(iymin, 0); iymax = std::min(iymax, ny - 1);
     s->win = Window{ixmin, ixmax, iymin, iymax};
   }
+  return FSIBE_OK;
+}
 
-  // interpolation targets (BeamForces.f90:150-165, :351-362, :389-400)
+// interpolation targets (BeamForces.f90:150-165, :351-362, :389-400), node
+// and circle data of GetForces, and the pressure box they need
+static int build_force_geometry(fsibe_sim *s, double time) {
+  const int nx = s->nx, ny = s->ny, ns = s->ns;
+  const double dx = s->dx, dy = s->dy;
   s->forces_ready = false;
   s->ntg = 0;
   Box fb;
@@ -1234,6 +1277,11 @@ static int build_geometry(fsibe_sim *s, double time) {
   return FSIBE_OK;
 }
 
+static int build_geometry(fsibe_sim *s, double time) {
+  CHK(build_mask_geometry(s));
+  return build_force_geometry(s, time);
+}
+
 static MaskBox mask_box(const fsibe_sim *s) {
   MaskBox m;
   m.x0 = s->mbox.x0; m.x1 = s->mbox.x1; m.y0 = s->mbox.y0; m.y1 = s->mbox.y1; m.w = s->mbox.w();
@@ -1265,8 +1313,29 @@ static int ensure_sponge(fsibe_sim *s) {
   return FSIBE_OK;
 }
 
+// with_forces = false: only the mask's part of the geometry is built and
+// uploaded; the caller builds and uploads the force part (force_geometry)
+// before the pressure and force kernels
+static int create_mask_impl(fsibe_sim *s, double time, const double *x, const double *y, const double *vx,
+                            const double *vy, const double *theta, double alpha, const double LE[6],
+                            bool with_forces);
+
 extern "C" int fsibe_create_mask(fsibe_sim *s, double time, const double *x, const double *y, const double *vx,
                                  const double *vy, const double *theta, double alpha, const double LE[6]) {
+  return create_mask_impl(s, time, x, y, vx, vy, theta, alpha, LE, true);
+}
+
+static int force_geometry(fsibe_sim *s, double time) {
+  const double th0 = now_ms();
+  CHK(build_force_geometry(s, time));
+  CHK(upload_stage(s, s->off_tg, s->off_val - s->off_tg));
+  prof_host(s, P_REDUCE, th0);
+  return FSIBE_OK;
+}
+
+static int create_mask_impl(fsibe_sim *s, double time, const double *x, const double *y, const double *vx,
+                            const double *vy, const double *theta, double alpha, const double LE[6],
+                            bool with_forces) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
   if (s->ns > 0 && (!x || !y || !vx || !vy || !theta)) return set_err(FSIBE_EARG, "null beam array");
   if (s->upload_pending) {
@@ -1280,10 +1349,15 @@ extern "C" int fsibe_create_mask(fsibe_sim *s, double time, const double *x, con
   for (int i = 0; i < 6; ++i) s->LE[i] = LE ? LE[i] : 0.0;
   s->time_mask = time;
   s->have_beam = true;
-  CHK(build_geometry(s, time));
-  HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipEventRecord(s->ev_upload, s->stream));
-  s->upload_pending = true;
+  const double th0 = now_ms();
+  if (with_forces) {
+    CHK(build_geometry(s, time));
+  } else {
+    CHK(build_mask_geometry(s));
+    s->forces_ready = false;
+  }
+  CHK(upload_stage(s, 0, with_forces ? s->off_val : s->off_tg));
+  prof_host(s, P_REDUCE, th0);
   if (!s->mbox.empty()) {
     MaskConsts mc;
     mc.dx = s->dx; mc.dy = s->dy; mc.t = s->p.t_beam; mc.h = s->p.N_smooth * std::max(s->dx, s->dy);
@@ -1353,7 +1427,22 @@ static int velocity_only(fsibe_sim *s, double time) {
   });
 }
 
-static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
+// K1: A = Fy^-1[W0], B = Fy^-1[i ky W0] (and Fy^-1[i ky psi] for iSponge =
+// 10, whose outflow sponge acts inside the penalisation term).  It reads only
+// the vorticity of time level `level`.
+static int launch_k1(fsibe_sim *s, int level) {
+  const bool sp10 = s->p.iSponge == 10 && s->penal_on && s->ms;
+  double2 *outs[3] = {s->T[0], s->T[1], s->T[4]};
+  const int fsel[3] = {0, 4, 2};
+  prof_begin(s, P_COLS_W0);
+  CHK(cols_inv_fields(s, s->W[level], outs, fsel, sp10 ? 3 : 2));
+  prof_end(s, P_COLS_W0);
+  return FSIBE_OK;
+}
+
+// K1 .. K6 of a step: the new vorticity, velocity, pressure right-hand
+// sides and the row sums
+static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_step) {
   if (s->p.iBeam > 0 && !s->have_beam) return set_err(FSIBE_ESTATE, "fsibe_create_mask must be called first");
   if (!s->have_beam)  // no beam: walls / cylinder only (create_mask.f90:34-69)
     CHK(fsibe_create_mask(s, time, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr));
@@ -1363,15 +1452,12 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     CHK(velocity_only(s, time));
     prof_end(s, P_VEL);
   }
-  // the outflow sponge of iSponge = 10 acts inside the penalisation term
   const bool sp10 = s->p.iSponge == 10 && s->penal_on && s->ms;
-  // K1: A = Fy^-1[W0], B = Fy^-1[i ky W0] (and Fy^-1[i ky psi] for iSponge = 10)
-  {
-    double2 *outs[3] = {s->T[0], s->T[1], s->T[4]};
-    const int fsel[3] = {0, 4, 2};
-    prof_begin(s, P_COLS_W0);
-    CHK(cols_inv_fields(s, s->W[n0], outs, fsel, sp10 ? 3 : 2));
-    prof_end(s, P_COLS_W0);
+  if (s->k1_ready && s->k1_level == n0) {
+    s->k1_ready = false;  // K1 of this step was launched ahead (run_one)
+  } else {
+    s->k1_ready = false;
+    CHK(launch_k1(s, n0));
   }
   MaskBox mb = mask_box(s);
   // K2 (block 0 also computes the K3 scalars from the previous u_max partials)
@@ -1433,7 +1519,12 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     prof_end(s, P_ROWS_PRESS);
   }
   s->slot_fluid[s->slot] = true;  // the host adds the row sums of K6
-  // K7 + K8 on the rows/columns the interpolation needs
+  return FSIBE_OK;
+}
+
+// K7 + K8: the static pressure and the stresses on the rows/columns the
+// interpolation needs (the box of the force geometry)
+static int evolve_pressure_box(fsibe_sim *s) {
   s->press_box_valid = false;
   if (s->forces_on && s->forces_ready && !s->fbox.empty()) {
     const Box &fb = s->fbox;
@@ -1460,6 +1551,11 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
     s->press_box_valid = true;
   }
   return FSIBE_OK;
+}
+
+static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
+  CHK(evolve_fluid(s, time, dt0, explicit_step));
+  return evolve_pressure_box(s);
 }
 
 static int launch_forces(fsibe_sim *s) {
@@ -1634,31 +1730,46 @@ extern "C" int fsibe_sim_solid(fsibe_sim *s, fsibe_solid **out) {
 // for dt1 (written by the nonlinear-term kernel), read the previous step's
 // results, and return so that the host prepares the next step's geometry
 // while this one runs.
-static int run_one(fsibe_sim *s, bool pipe) {
+static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
   s->pipe = pipe;
   const fsibe_params &p = s->p;
   const bool moving = (p.iSponge >= 4 && s->time < p.T_fullspeed) || p.iFLUSI == 1 || p.iMotion > 0;
+  // iFLUSI=1 waits for every step: the host builds the force geometry
+  // (interpolation targets) while the GPU already runs the fluid kernels
+  const bool split = !pipe && p.iFLUSI == 1 && p.iBeam > 0 && s->forces_on;
   if (p.iBeam > 0 && (moving || !s->have_beam)) {
     double alpha, at, att, LE[6];
     mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, s->time, alpha, at, att, LE);
-    CHK(fsibe_create_mask(s, s->time, s->bx.data(), s->by.data(), s->bvx.data(), s->bvy.data(),
-                          s->btheta.data(), alpha, LE));
+    CHK(create_mask_impl(s, s->time, s->bx.data(), s->by.data(), s->bvx.data(), s->bvy.data(),
+                         s->btheta.data(), alpha, LE, !split));
   } else if (p.iBeam > 0) {
     CHK(build_geometry(s, s->time));  // soft-startup factor and targets follow time
     if (s->upload_pending) { HIPCHK(hipEventSynchronize(s->ev_upload)); s->upload_pending = false; }
-    HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipEventRecord(s->ev_upload, s->stream));
-    s->upload_pending = true;
+    CHK(upload_stage(s, 0, s->off_val));
   }
-  if (!s->startup_done) {
-    CHK(evolve(s, s->time, 0.0, true));
+  if (s->startup_done) s->dt0 = s->dt1;
+  if (split) {
+    CHK(evolve_fluid(s, s->time, s->startup_done ? s->dt0 : 0.0, !s->startup_done));
+    CHK(force_geometry(s, s->time));
+    CHK(evolve_pressure_box(s));
   } else {
-    s->dt0 = s->dt1;
-    CHK(evolve(s, s->time, s->dt0, false));
+    CHK(evolve(s, s->time, s->startup_done ? s->dt0 : 0.0, !s->startup_done));
   }
   if (s->forces_on) CHK(launch_forces(s));
-  if (!pipe) {
+  if (!pipe && k1_ahead) {
+    // iFLUSI=1: the next step's K1 runs while the host solves the beam
+    HIPCHK(hipEventRecord(s->ev_step, s->stream));
+    CHK(launch_k1(s, s->n1));
+    s->k1_ready = true;
+    s->k1_level = s->n1;
+    HIPCHK(hipEventSynchronize(s->ev_step));
+    s->upload_pending = false;
+    parse_results(s, s->slot);
+    s->dt1 = s->last.dt1;
+  } else if (!pipe) {
+    const double th0 = now_ms();
     CHK(fetch_results(s));
+    prof_host(s, P_FORCES, th0);
     s->dt1 = s->last.dt1;
   } else {
     HIPCHK(hipEventSynchronize(s->ev_dt));
@@ -1669,7 +1780,9 @@ static int run_one(fsibe_sim *s, bool pipe) {
     s->slot ^= 1;
   }
   if (p.iFLUSI == 1 && p.iBeam > 0) {
+    const double th0 = now_ms();
     CHK(solid_step(s));
+    prof_host(s, P_SCALARS, th0);
   } else if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0) {
     integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
                             s->by.data(), s->bvx.data(), s->bvy.data());
@@ -1710,10 +1823,14 @@ extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
   int rc = FSIBE_OK;
   for (int k = 0; k < nsteps; ++k) {
     if (!(s->time < s->p.Time_end)) break;
-    rc = run_one(s, pipe);
+    // the next step's first kernel goes ahead of the host's beam solve when
+    // there is a next step in this call (event timing keeps the plain order)
+    const bool ahead = fsi && !s->prof.on && k + 1 < nsteps;
+    rc = run_one(s, pipe, ahead);
     if (rc != FSIBE_OK) break;
     if (s->last.status == 1) break;  // continue_timestep = .false.
   }
+  s->k1_ready = false;
   s->pipe = false;
   if (s->res_pending) {
     HIPCHK(hipStreamSynchronize(s->stream));

@@ -313,4 +313,12 @@ __global__ void k_half_to_packed(const double2 *__restrict__ c, double *__restri
   }
 }
 
+// Copy n 16-byte words of the per-step staging record from pinned, mapped
+// host memory to device memory.  A kernel on the step's own stream instead of
+// a DMA copy: the step's next kernel then follows without a hand-off between
+// the copy engine and the compute queue.
+__global__ void k_stage_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 }  // namespace fsibe

@@ -14,8 +14,9 @@
 // :502-520).  The system is banded once the unknowns are interleaved node by
 // node (theta(j), T(j)) and each equation is placed at the node it belongs
 // to: 7 sub- and 8 super-diagonals.  This solver factors that band with row
-// interchanges (partial pivoting), O(ns) per Newton iteration instead of
-// O(ns^3); the Newton iterate differs from the dense solve's by rounding only.
+// interchanges (partial pivoting), O(ns) instead of O(ns^3), once per time
+// step as a rule (see the Newton loop); the Newton iterates differ from the
+// dense solves' by rounding only.
 //
 // Expressions keep the reference's operand order (Fortran evaluates a+b+c as
 // (a+b)+c and x**2 as x*x); the library is compiled without floating-point
@@ -169,23 +170,39 @@ inline bool tridiag_solve(int n, double *dl, double *d, double *du, double *b) {
 // Row r keeps columns r - kKL .. r + kKL + kKU (room for the fill of the row
 // interchanges): row(r)[c] is entry (r, c).  last[r] is the last column of
 // row r that may be nonzero; elimination and back substitution stop there.
+// factor() keeps the multipliers of step j below the diagonal of column j
+// and the interchange of step j in piv[j] (the storage of LAPACK's dgbtrf),
+// so that a factorisation solves further right-hand sides.
 struct Band {
   int n = 0;
   std::vector<double> a;
-  std::vector<int> last;
+  std::vector<int> last, piv;
+  std::vector<double> inv;  // 1 / U(j, j) after factor()
   void reset(int nn) {
     n = nn;
     a.assign((size_t)n * kBW, 0.0);
-    last.assign(n, 0);
+    last.resize(n);
+    piv.resize(n);
+    inv.resize(n);
     for (int r = 0; r < n; ++r) last[r] = r;
   }
   double *row(int r) { return a.data() + (size_t)r * kBW + kKL - r; }
+  const double *row(int r) const { return a.data() + (size_t)r * kBW + kKL - r; }
   double &at(int r, int c) {
     if (c > last[r]) last[r] = c;
     return row(r)[c];
   }
-  // factor and solve in place (b becomes the solution); false if singular
-  bool solve(double *b) {
+  // y = A x (before factor())
+  void matvec(const double *x, double *y) const {
+    for (int r = 0; r < n; ++r) {
+      const double *rr = row(r);
+      double s = 0.0;
+      for (int c = std::max(0, r - kKL); c <= last[r]; ++c) s = s + rr[c] * x[c];
+      y[r] = s;
+    }
+  }
+  // LU with row interchanges in place; false if singular
+  bool factor() {
     for (int j = 0; j < n; ++j) {
       const int lst = std::min(n - 1, j + kKL);
       int p = j;
@@ -195,33 +212,47 @@ struct Band {
         if (v > best) { best = v; p = i; }
       }
       if (best == 0.0) return false;
+      piv[j] = p;
       if (p != j) {
         double *rj = row(j), *rp = row(p);
         const int ce = std::max(last[j], last[p]);
         for (int c = j; c <= ce; ++c) std::swap(rj[c], rp[c]);
         std::swap(last[j], last[p]);
-        std::swap(b[j], b[p]);
       }
       const double *rj = row(j);
-      const double piv = rj[j];
+      const double pv = rj[j];
       const int ce = last[j];
       for (int i = j + 1; i <= lst; ++i) {
         double *ri = row(i);
         if (ri[j] == 0.0) continue;
-        const double l = ri[j] / piv;
-        ri[j] = 0.0;
+        const double l = ri[j] / pv;
+        ri[j] = l;
         for (int c = j + 1; c <= ce; ++c) ri[c] = ri[c] - l * rj[c];
         if (ce > last[i]) last[i] = ce;
-        b[i] = b[i] - l * b[j];
       }
     }
+    for (int j = 0; j < n; ++j) inv[j] = 1.0 / row(j)[j];
+    return true;
+  }
+  // b <- A^-1 b with the factors of factor()
+  void solve_factored(double *b) const {
+    for (int j = 0; j < n; ++j) {
+      if (piv[j] != j) std::swap(b[j], b[piv[j]]);
+      const int lst = std::min(n - 1, j + kKL);
+      for (int i = j + 1; i <= lst; ++i) {
+        const double l = row(i)[j];
+        if (l != 0.0) b[i] = b[i] - l * b[j];
+      }
+    }
+    // back substitution; the terms of b[j+2..] are added first so that
+    // only the last one waits for b[j+1]
     for (int j = n - 1; j >= 0; --j) {
       const double *rj = row(j);
       double s = b[j];
-      for (int c = j + 1; c <= last[j]; ++c) s = s - rj[c] * b[c];
-      b[j] = s / rj[j];
+      for (int c = last[j]; c >= j + 2; --c) s = s - rj[c] * b[c];
+      if (last[j] >= j + 1) s = s - rj[j + 1] * b[j + 1];
+      b[j] = s * inv[j];
     }
-    return true;
   }
 };
 
@@ -232,7 +263,8 @@ struct Solver {
   double ds = 0.0, grav = 0.0;
   int method = kBDF2;     // TimeMethodSolid (restored after a startup step)
   Diff dn, de;            // stencils on ns points and on the ns + 3 extended points
-  Band band;
+  Band band, lu;          // the Jacobian being assembled; the factorisation in use
+  int refactored = 0;     // Newton iterations after the first that factored afresh
   std::vector<int> rowpos, colpos;  // reference equation k / unknown l (1-based) -> band row / column
   // per-step context shared by F_nonlinear and Jacobi
   struct Ctx {
@@ -264,6 +296,7 @@ struct Solver {
     dn.init(ds);
     de.init(ds);
     band.reset(nx);
+    lu.reset(nx);
     colpos.assign(nx + 1, 0);
     rowpos.assign(nx + 1, 0);
     // unknowns: theta(j) = x(j+2) at 2(j+1), T(j) = x(ns+5+j) at 2(j+1)+1,
@@ -685,24 +718,60 @@ struct Solver {
     const int m = step_method(b);
     b.StartupStep = false;
     const std::vector<double> x_old(b.x), y_old(b.y), vx_old(b.vx), vy_old(b.vy);
-    std::vector<double> x(nx), dx(nx);
+    std::vector<double> x(nx), dx(nx), rhs(nx), tmp(nx);
     if (!guess(time, dt, b, x.data())) return false;
     if (!begin(time, dt, b, m)) return false;
     const std::vector<double> &th_old = th_old_, &thd_old = thd_old_;
     const double *thg = x.data() + 1;
-    // Newton (SolidSolver.f90:229-316)
+    // Newton (SolidSolver.f90:229-316).  The first iteration factors its
+    // Jacobian.  Later iterations solve J_k dx = -F_k with that factorisation
+    // and one step of iterative refinement against the assembled J_k; the
+    // refinement's correction c measures ||J_0^-1 (J_k - J_0)||, and the
+    // refined step differs from the exact solve by about that factor times
+    // |c|.  Within one time step the Jacobian moves little (|c| ~ 1e-8 |dx|),
+    // so the Newton iterates are those of a fresh factorisation to rounding;
+    // if |c| > 1e-6 |dx| (and c is not below the rounding of x itself,
+    // 1e-14 max|x|, as in a converged last iteration) the iteration factors
+    // J_k instead.
     iter = 0;
+    refactored = 0;
     for (;;) {
       band.reset(nx);
       bool outside = false;
-      residual(x.data(), [&](int k, double v) { dx[rowpos[k]] = -1.0 * v; });
+      residual(x.data(), [&](int k, double v) { rhs[rowpos[k]] = -1.0 * v; });
       jacobian(x.data(), [&](int k, int l, double v) {
         const int r = rowpos[k], c = colpos[l];
         if (c < r - kKL || c > r + kKU) outside = true;
         else band.at(r, c) = v;
       });
       if (outside) { err = "IBES: Jacobian entry outside the band"; return false; }
-      if (!band.solve(dx.data())) { err = "IBES: singular Newton matrix"; return false; }
+      bool fresh = iter == 0;
+      if (!fresh) {
+        dx = rhs;
+        lu.solve_factored(dx.data());
+        band.matvec(dx.data(), tmp.data());
+        double cm = 0.0, dm = 0.0;
+        for (int r = 0; r < nx; ++r) tmp[r] = rhs[r] - tmp[r];
+        lu.solve_factored(tmp.data());
+        for (int r = 0; r < nx; ++r) {
+          cm = std::max(cm, std::fabs(tmp[r]));
+          dm = std::max(dm, std::fabs(dx[r]));
+        }
+        double xm = 0.0;
+        for (int l = 0; l < nx; ++l) xm = std::max(xm, std::fabs(x[l]));
+        if (cm <= 1e-6 * dm || cm <= 1e-14 * xm) {
+          for (int r = 0; r < nx; ++r) dx[r] = dx[r] + tmp[r];
+        } else {
+          fresh = true;
+          ++refactored;
+        }
+      }
+      if (fresh) {
+        std::swap(lu, band);
+        if (!lu.factor()) { err = "IBES: singular Newton matrix"; return false; }
+        dx = rhs;
+        lu.solve_factored(dx.data());
+      }
       ++iter;
       // dx is in band order; the update and the norms run in the reference's order
       double s2 = 0.0, x2 = 0.0;

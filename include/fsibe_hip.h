@@ -307,11 +307,13 @@ void *fsibe_sim_stream(fsibe_sim *sim);
  * step); fsibe_profile_read fills up to n
  * totals (milliseconds) and launch counts and returns the number of kernel
  * classes: 0 create_mask, 1 startup velocity, 2 columns of vortk(n0),
- * 3 rows: nonlinear term (with the dt/AB2 scalars), 4 unused,
- * 5 columns: AB2 update, 6 columns of vortk(n1), 7 rows: velocity/pressure
- * rhs and row sums, 8 unused, 9 columns: pressure, 10 rows: pressure/stress
- * box, 11 Delta interpolation and per-node beam loads, 12 unused (the host
- * adds the row sums and the per-node terms when it reads a step). */
+ * 3 rows: nonlinear term (with the dt/AB2 scalars), 5 columns: AB2 update,
+ * 6 columns of vortk(n1), 7 rows: velocity/pressure rhs and row sums,
+ * 9 columns: pressure, 10 rows: pressure/stress box, 11 Delta interpolation
+ * and per-node beam loads.  Classes 4, 8 and 12 time host phases with the
+ * steady clock: 4 the solid solver's step (iFLUSI=1), 8 the mask geometry
+ * and its upload, 12 the wait for a step's results (the host adds the row
+ * sums and the per-node terms when it reads a step). */
 int fsibe_profile_enable(fsibe_sim *sim, int classes);
 int fsibe_profile_read(fsibe_sim *sim, double *total_ms, long long *count, int n);
 
