@@ -2081,7 +2081,7 @@ struct fsibe_batch {
   std::vector<BatchGroup> groups;
   hipStream_t stream = nullptr;  // argument upload
   hipEvent_t ev_dt = nullptr, ev_args = nullptr;
-  char *h_args = nullptr, *d_args = nullptr;
+  char *h_args = nullptr, *d_args = nullptr, *m_args = nullptr;  // m_args: device address of h_args
   size_t args_bytes = 0;
   bool res_pending = false;
   HostPool *pool = nullptr;
@@ -2153,7 +2153,8 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
       return fail(set_err(FSIBE_EHIP, "batch: stream creation failed"));
   if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->ev_dt, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_args, hipEventDisableTiming) != hipSuccess ||
-      hipHostMalloc((void **)&b->h_args, off, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&b->h_args, off, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&b->m_args, b->h_args, 0) != hipSuccess ||
       hipMalloc(&b->d_args, off) != hipSuccess)
     return fail(set_err(FSIBE_EHIP, "batch: allocation failed"));
   memset(b->h_args, 0, off);
@@ -2316,11 +2317,6 @@ static int batch_step(fsibe_batch *b) {
     HIPCHK(hipEventRecord(g.ev_dt, g.stream));
     HIPCHK(hipStreamWaitEvent(b->stream, g.ev_dt, 0));
   }
-  for (auto &g : b->groups)
-    for (int i : g.mem) {
-      fsibe_sim *s = b->m[i];
-      HIPCHK(hipMemcpyAsync(s->d_stage, s->h_stage, s->off_val, hipMemcpyHostToDevice, g.stream));
-    }
   std::vector<std::array<int, B_NST>> maxblk(b->groups.size());
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     BatchGroup &g = b->groups[gi];
@@ -2328,7 +2324,23 @@ static int batch_step(fsibe_batch *b) {
     for (size_t j = 0; j < g.mem.size(); ++j) fill_batch_args(b, g, (int)j, b->m[g.mem[j]], mx);
     for (int k = 0; k < B_NST; ++k) maxblk[gi][k] = mx[k];
   }
-  HIPCHK(hipMemcpyAsync(b->d_args, b->h_args, b->args_bytes, hipMemcpyHostToDevice, b->stream));
+  // the argument records and every member's geometry, copied by kernels on
+  // the upload stream (no copy-engine hand-off before the first stage)
+  {
+    std::vector<StageSeg> segs;
+    segs.push_back(StageSeg{reinterpret_cast<const double2 *>(b->m_args), reinterpret_cast<double2 *>(b->d_args),
+                            (long long)(b->args_bytes / sizeof(double2))});
+    for (fsibe_sim *s : b->m)
+      segs.push_back(StageSeg{reinterpret_cast<const double2 *>(s->m_stage), reinterpret_cast<double2 *>(s->d_stage),
+                              (long long)(s->off_val / sizeof(double2))});
+    for (size_t k0 = 0; k0 < segs.size(); k0 += kMaxStageSegs) {
+      StageSegs ss{};
+      const int ns = (int)std::min(segs.size() - k0, (size_t)kMaxStageSegs);
+      for (int k = 0; k < ns; ++k) ss.s[k] = segs[k0 + k];
+      k_stage_gather<<<dim3(8, ns), 256, 0, b->stream>>>(ss);
+      HIPCHK(hipGetLastError());
+    }
+  }
   HIPCHK(hipEventRecord(b->ev_args, b->stream));
   for (auto &g : b->groups) HIPCHK(hipStreamWaitEvent(g.stream, b->ev_args, 0));
   auto arr = [&](const BatchGroup &g, int k) { return b->d_args + g.off[k]; };

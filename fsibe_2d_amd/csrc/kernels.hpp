@@ -321,4 +321,21 @@ __global__ void k_stage_copy(const double2 *__restrict__ src, double2 *__restric
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
+// Several staging copies in one launch: segment blockIdx.y copies n 16-byte
+// words from src (mapped host memory) to dst (device memory).
+struct StageSeg {
+  const double2 *src;
+  double2 *dst;
+  long long n;
+};
+constexpr int kMaxStageSegs = 128;
+struct StageSegs {
+  StageSeg s[kMaxStageSegs];
+};
+__global__ void k_stage_gather(StageSegs segs) {
+  const StageSeg g = segs.s[blockIdx.y];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < g.n; i += (long long)gridDim.x * blockDim.x)
+    g.dst[i] = g.src[i];
+}
+
 }  // namespace fsibe
