@@ -85,7 +85,7 @@ EXPORTS = (
     "fsibe_batch_create", "fsibe_batch_destroy", "fsibe_batch_run", "fsibe_batch_size", "fsibe_batch_member",
     "fsibe_solid_create", "fsibe_solid_destroy", "fsibe_solid_get", "fsibe_solid_set", "fsibe_solid_get_info",
     "fsibe_solid_set_info", "fsibe_solid_step", "fsibe_solid_guess", "fsibe_solid_residual",
-    "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid",
+    "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid", "fsibe_solid_wrapper", "fsibe_gravity_impulse",
 )
 
 _lib = None
@@ -159,6 +159,8 @@ def lib():
     L.fsibe_solid_jacobian.argtypes = [vp, C.c_double, C.c_double, dp, dp]
     L.fsibe_solid_rhs.argtypes = [vp, C.c_double, dp, dp, dp, dp, dp, dp, dp]
     L.fsibe_sim_solid.argtypes = [vp, pvp]
+    L.fsibe_solid_wrapper.argtypes = [vp, C.c_double, C.c_double]
+    L.fsibe_gravity_impulse.argtypes = [C.POINTER(FsibeParams), C.c_double, C.POINTER(C.c_double)]
     _lib = L
     return L
 

@@ -590,9 +590,24 @@ extern "C" int fsibe_solid_set_info(fsibe_solid *b, const fsibe_solid_info *in) 
 extern "C" int fsibe_solid_step(fsibe_solid *b, double time, double dt) {
   if (!b) return set_err(FSIBE_EARG, "null solid");
   if (!(dt > 0.0)) return set_err(FSIBE_EARG, "dt must be > 0");
-  if (b->sv.step_method(b->b) == ibes::kBDF2 && !(b->b.dt_old > 0.0))
+  if (time > b->sv.p.T_release && b->sv.step_method(b->b) == ibes::kBDF2 && !(b->b.dt_old > 0.0))
     return set_err(FSIBE_ESTATE, "BDF2 step without a previous step (dt_old = 0)");
   if (!b->sv.step(time, dt, b->b)) return set_err(FSIBE_EDOMAIN, "%s", b->sv.err.c_str());
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_solid_wrapper(fsibe_solid *b, double time, double dt) {
+  if (!b) return set_err(FSIBE_EARG, "null solid");
+  if (!(dt > 0.0)) return set_err(FSIBE_EARG, "dt must be > 0");
+  if (time > b->sv.p.T_release && b->sv.step_method(b->b) == ibes::kBDF2 && !(b->b.dt_old > 0.0))
+    return set_err(FSIBE_ESTATE, "BDF2 step without a previous step (dt_old = 0)");
+  if (!b->sv.wrapper(time, dt, b->b)) return set_err(FSIBE_EDOMAIN, "%s", b->sv.err.c_str());
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_gravity_impulse(const fsibe_params *p, double time, double *grav) {
+  if (!p || !grav) return set_err(FSIBE_EARG, "null argument");
+  *grav = ibes::Solver::gravity_at(p->iImpulse, time, *grav);
   return FSIBE_OK;
 }
 

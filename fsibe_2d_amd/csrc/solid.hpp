@@ -324,14 +324,15 @@ struct Solver {
   }
 
   // GravityImpulse (SolidSolver.f90:874-911): grav as a function of time
-  void gravity_impulse(double time) {
+  void gravity_impulse(double time) { grav = gravity_at(p.iImpulse, time, grav); }
+  static double gravity_at(int iImpulse, double time, double g) {
     const double pi = 4.0 * atan(1.0);
-    if (p.iImpulse == 1 || p.iImpulse == 2) {
-      const double T0 = 0.75, T1 = 0.85, amp = p.iImpulse == 1 ? 1.00 : 4.00;
-      grav = amp * sin(pi * (time - T0) / (T1 - T0));
-      if (time < T0) grav = 0.0;
-      if (time > T1) grav = 0.0;
-    } else if (p.iImpulse == 3) {
+    if (iImpulse == 1 || iImpulse == 2) {
+      const double T0 = 0.75, T1 = 0.85, amp = iImpulse == 1 ? 1.00 : 4.00;
+      g = amp * sin(pi * (time - T0) / (T1 - T0));
+      if (time < T0) g = 0.0;
+      if (time > T1) g = 0.0;
+    } else if (iImpulse == 3) {
       const double T0 = 2.0, T1 = 3.0;
       double k;
       if (time <= T0) {
@@ -344,8 +345,9 @@ struct Solver {
       } else {
         k = 1.0;
       }
-      grav = 0.7 * k;
+      g = 0.7 * k;
     }
+    return g;
   }
 
   // Tension (SolidSolver.f90:1000-1081): T and T_s of the angle field theta
@@ -813,17 +815,24 @@ struct Solver {
   // (time_step.f90:209-210, SolidSolver.f90:13-44); then the loads advance,
   // pressure_old = pressure_new (time_step.f90:225-228)
   bool step(double time, double dt, Beam &b) {
+    gravity_impulse(time);
+    if (!wrapper(time, dt, b)) return false;
+    b.pressure_old = b.pressure_new;
+    b.tau_old = b.tau_new;
+    return true;
+  }
+
+  // SolidSolverWrapper (SolidSolver.f90:13-44) alone: an IBES step once the
+  // beam is released, else the leading edge's motion only; SolidEnergies
+  bool wrapper(double time, double dt, Beam &b) {
     brake = false;
     iter = 0;
-    gravity_impulse(time);
     if (time > p.T_release) {
       if (!ibes(time, dt, b)) return false;
     } else {
       integrate(time + dt, b);
     }
     energies(b);
-    b.pressure_old = b.pressure_new;
-    b.tau_old = b.tau_new;
     return true;
   }
 };

@@ -43,6 +43,17 @@ module fsibe_hip_api
     integer(c_int32_t) :: status, ibes_iter
   end type
 
+  ! struct fsibe_solid_info
+  type, bind(C) :: fsibe_solid_info_t
+    real(c_double) :: dt_old, grav, E_kinetic, E_pot, E_elastic, Inertial_Force(2)
+    integer(c_int32_t) :: StartupStep, iter, brake, method
+  end type
+
+  integer(c_int), parameter :: FSIBE_SOLID_X = 0, FSIBE_SOLID_Y = 1, FSIBE_SOLID_VX = 2, FSIBE_SOLID_VY = 3, &
+                               FSIBE_SOLID_THETA = 4, FSIBE_SOLID_THETA_DOT = 5, FSIBE_SOLID_AX = 6, &
+                               FSIBE_SOLID_AY = 7, FSIBE_SOLID_PRESSURE_OLD = 8, FSIBE_SOLID_PRESSURE_NEW = 9, &
+                               FSIBE_SOLID_TAU_OLD = 10, FSIBE_SOLID_TAU_NEW = 11, FSIBE_SOLID_OLDOLD = 12
+
   interface
     function fsibe_last_error() bind(C, name="fsibe_last_error") result(p)
       import :: c_ptr
@@ -217,6 +228,64 @@ module fsibe_hip_api
       type(c_ptr), value :: sim
       integer(c_int), value :: nsteps
       type(fsibe_step_out_t), intent(out) :: out
+      integer(c_int) :: rc
+    end function
+
+    ! ---- host solid solver ----------------------------------------------------
+    function fsibe_solid_create(p, b) bind(C, name="fsibe_solid_create") result(rc)
+      import :: c_int, c_ptr, fsibe_params_t
+      type(fsibe_params_t), intent(in) :: p
+      type(c_ptr), intent(out) :: b
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_destroy(b) bind(C, name="fsibe_solid_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_get(b, which, out) bind(C, name="fsibe_solid_get") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: which
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_set(b, which, in) bind(C, name="fsibe_solid_set") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: which
+      real(c_double), intent(in) :: in(*)
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_get_info(b, info) bind(C, name="fsibe_solid_get_info") result(rc)
+      import :: c_int, c_ptr, fsibe_solid_info_t
+      type(c_ptr), value :: b
+      type(fsibe_solid_info_t), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_set_info(b, info) bind(C, name="fsibe_solid_set_info") result(rc)
+      import :: c_int, c_ptr, fsibe_solid_info_t
+      type(c_ptr), value :: b
+      type(fsibe_solid_info_t), intent(in) :: info
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_step(b, time, dt) bind(C, name="fsibe_solid_step") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double), value :: time, dt
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_wrapper(b, time, dt) bind(C, name="fsibe_solid_wrapper") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double), value :: time, dt
+      integer(c_int) :: rc
+    end function
+    function fsibe_gravity_impulse(p, time, grav) bind(C, name="fsibe_gravity_impulse") result(rc)
+      import :: c_int, c_double, fsibe_params_t
+      type(fsibe_params_t), intent(in) :: p
+      real(c_double), value :: time
+      real(c_double), intent(inout) :: grav
       integer(c_int) :: rc
     end function
   end interface

@@ -246,6 +246,13 @@ int fsibe_solid_set_info(fsibe_solid *b, const fsibe_solid_info *in);
  * (set them with fsibe_solid_set).  FSIBE_EDOMAIN if Newton does not
  * converge in 49 iterations (the reference stops, SolidSolver.f90:311-314). */
 int fsibe_solid_step(fsibe_solid *b, double time, double dt);
+/* SolidSolverWrapper(time, dt, beams) alone (SolidSolver.f90:13-44), for a
+ * host that runs GravityImpulse and advances the loads itself (the Fortran
+ * module solid_hip.f90): grav is the solver's (fsibe_solid_set_info). */
+int fsibe_solid_wrapper(fsibe_solid *b, double time, double dt);
+/* GravityImpulse(time) (SolidSolver.f90:874-911): *grav in the share_vars
+ * variable's role, overwritten for iImpulse 1..3, unchanged otherwise */
+int fsibe_gravity_impulse(const fsibe_params *p, double time, double *grav);
 /* The Newton system of the next IBES step t -> t + dt, x of 2*ns+4 values
  * [theta(-1..ns+1), T(-1..ns-1)]: the predictor (EE1 and the ghost values,
  * SolidSolver.f90:167-209), F_nonlinear (:552-667) and the Jacobian
