@@ -2099,6 +2099,7 @@ struct fsibe_batch {
   char *h_args = nullptr, *d_args = nullptr, *m_args = nullptr;  // m_args: device address of h_args
   size_t args_bytes = 0;
   bool res_pending = false;
+  bool fsi = false;  // some member's beam is solved (iFLUSI=1): every step waits for its loads
   HostPool *pool = nullptr;
 };
 
@@ -2127,7 +2128,6 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
   *out = nullptr;
   for (int i = 0; i < n; ++i) {
     const fsibe_params &p = params[i];
-    if (p.iFLUSI == 1) return set_err(FSIBE_EUNSUPPORTED, "batch member %d: iFLUSI=1 (host-solved beam)", i);
     if (p.iSponge == 10) return set_err(FSIBE_EUNSUPPORTED, "batch member %d: iSponge=10", i);
   }
   fsibe_batch *b = new fsibe_batch();
@@ -2140,6 +2140,11 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
     if (vortk && vortk[i]) {
       rc = fsibe_set_vortk(s, vortk[i]);
       if (rc != FSIBE_OK) return fail(rc);
+    }
+    if (s->p.iFLUSI == 1 && s->p.iBeam > 0) {
+      rc = ensure_solid(s);
+      if (rc != FSIBE_OK) return fail(rc);
+      b->fsi = true;
     }
   }
   // groups: same grid and the same launch shapes
@@ -2385,7 +2390,8 @@ static int batch_step(fsibe_batch *b) {
       return FSIBE_OK;
     }));
   }
-  for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+  if (!b->fsi)
+    for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
   for (size_t gi = 0; gi < b->groups.size(); ++gi) {
     const BatchGroup &g = b->groups[gi];
     const fsibe_sim *s0 = b->m[g.mem[0]];
@@ -2441,6 +2447,34 @@ static int batch_step(fsibe_batch *b) {
     BLAUNCH(kb_forces, maxblk[gi][B_K10], (int)g.mem.size(), kForceThreads, 0, g.stream,
             reinterpret_cast<const ForcesArgs *>(arr(g, B_K10)));
   }
+  if (b->fsi) {
+    // the beams of step n+1 need the loads of step n: wait for every group,
+    // then each member reads its results, and its beam is solved (or moved)
+    for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+    for (auto &g : b->groups) HIPCHK(hipEventSynchronize(g.ev_dt));
+    b->pool->run((int)b->m.size(), [&](int i) {
+      fsibe_sim *s = b->m[i];
+      s->upload_pending = false;
+      parse_results(s, s->slot);
+      s->dt1 = s->last.dt1;
+      s->slot ^= 1;
+      const fsibe_params &p = s->p;
+      if (p.iFLUSI == 1 && p.iBeam > 0) {
+        if (solid_step(s) != FSIBE_OK) {
+          errs[i] = fsibe_last_error();
+          bad.store(i);
+        }
+      } else if (p.iMotion > 0 && p.iBeam > 0) {
+        integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
+                                s->by.data(), s->bvx.data(), s->bvy.data());
+      }
+      std::swap(s->n0, s->n1);
+      s->time = s->time + s->dt1;
+      s->it++;
+    });
+    if (bad.load() >= 0) return set_err(FSIBE_EDOMAIN, "batch member %d: %s", bad.load(), errs[bad.load()].c_str());
+    return FSIBE_OK;
+  }
   // dt1 of every member, then the previous step's results
   for (auto &g : b->groups) HIPCHK(hipEventSynchronize(g.ev_dt));
   const bool pending = b->res_pending;
@@ -2480,6 +2514,11 @@ extern "C" int fsibe_batch_run(fsibe_batch *b, int nsteps) {
   for (int k = done; k < nsteps; ++k) {
     rc = batch_step(b);
     if (rc != FSIBE_OK) break;
+    // a member's solid solver tripped its emergency brake: the batch stops
+    // after this step, as that member's own loop would
+    bool brake = false;
+    for (fsibe_sim *s : b->m) brake = brake || s->last.status == 1;
+    if (brake) break;
   }
   if (b->res_pending) {
     HIPCHK(hipStreamSynchronize(b->stream));

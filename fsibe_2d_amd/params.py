@@ -115,24 +115,25 @@ def turbulence_4096(n: int = 4096) -> Params:
                   nu=1e-5, dt_fixed=1e-4, cfl=1e6, xl=2 * math.pi, yl=2 * math.pi, iMotion=0)
 
 
-def batch_configs(n: int = 64, seed: int = 2) -> list:
+def batch_configs(n: int = 64, seed: int = 2, coupled: bool = True) -> list:
     """The batched workload: n independent FSI simulations with uneven grids
     and parameters, drawn per member in this order from SplitMix64(seed)
     uniforms u = (x >> 11) 2^-53: grid in {256, 384, 512, 768}, beam
     stiffness log-uniform in [0.01, 1], mass ratio uniform in [0.2, 2], heave
     frequency uniform in [0.5, 2], Re uniform in [100, 1000], nodes uniform
     in {64, ..., 128}.  Physics otherwise as fsi_256(); dt = 2e-4 (256/N).
-    The beam is driven by the heave (stiffness and mass ratio are drawn to
-    keep the sequence but only matter to a host solid solver)."""
+    coupled: each member's flexible beam is solved by the solid solver
+    (iFLUSI=1); otherwise it follows the heave rigidly."""
     from .inputs import splitmix64
 
     z = splitmix64(seed, 6 * n)
     u = [float(int(v) >> 11) * (1.0 / 9007199254740992.0) for v in z]
     out = []
     for i in range(n):
-        g, _eta, _mu, f, re, nsu = u[6 * i: 6 * i + 6]
+        g, eta, mu, f, re, nsu = u[6 * i: 6 * i + 6]
         N = (256, 384, 512, 768)[min(int(g * 4), 3)]
         ns = 64 + min(int(nsu * 65), 64)
         out.append(Params(nx=N, ny=N, ns=ns, nu=1.0 / (100.0 + 900.0 * re), heave_freq=0.5 + 1.5 * f,
-                          dt_fixed=2e-4 * 256.0 / N))
+                          dt_fixed=2e-4 * 256.0 / N, iFLUSI=1 if coupled else 0,
+                          eta=0.01 * 100.0 ** eta, mue=0.2 + 1.8 * mu))
     return out

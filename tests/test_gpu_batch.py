@@ -2,7 +2,7 @@
 parameters advanced together must reproduce, bit for bit, the same
 simulations run one at a time through fsibe_run (the batched kernels run the
 same device code with the same launch shapes), and the oracle to the usual
-bars."""
+bars; with prescribed beams and with beams solved by the solid solver."""
 import numpy as np
 import pytest
 
@@ -77,12 +77,13 @@ def test_batch_member_against_oracle(oracle_mod):
     b.close()
 
 
-def test_batch_of_benchmark_members(oracle_mod):
+@pytest.mark.parametrize("coupled", [False, True])
+def test_batch_of_benchmark_members(coupled, oracle_mod):
     """8 members of the batched workload (grids 256..768) against single runs"""
     from fsibe_2d_amd import batch_configs
     from fsibe_2d_amd.solver import Batch, FluidSim
 
-    ps = batch_configs(8)
+    ps = batch_configs(8, coupled=coupled)
     vks = [_noise(oracle_mod, p, 100 + i) for i, p in enumerate(ps)]
     b = Batch(ps, vks)
     b.run(5)
@@ -92,5 +93,40 @@ def test_batch_of_benchmark_members(oracle_mod):
         g.run(5)
         assert np.array_equal(b.members[i].field("vortk"), g.field("vortk")), i
         assert b.members[i].step_out()["Force"] == g.step_out()["Force"], i
+        assert np.array_equal(b.members[i].beam()["theta"], g.beam()["theta"]), i
+        g.close()
+    b.close()
+
+
+def test_batch_with_solved_beams(oracle_mod):
+    """members whose beams the solid solver moves (iFLUSI = 1) beside
+    prescribed ones: each bit-identical to its own coupled run"""
+    from fsibe_2d_amd import Params
+    from fsibe_2d_amd.solver import Batch, FluidSim
+
+    ps = _members()
+    ps[0] = ps[0].replace(iFLUSI=1, mue=0.6, eta=0.05)
+    ps[2] = ps[2].replace(iFLUSI=1, mue=1.5, eta=0.2, TimeMethodSolid=2, sigma=0.3)
+    ps[5] = ps[5].replace(iFLUSI=1, iMotion=600, AngleBeam=0.0)
+    vks = [_noise(oracle_mod, p, 20 + i) for i, p in enumerate(ps)]
+    b = Batch(ps, vks)
+    steps = 11
+    b.run(steps)
+    for i in (0, 1, 2, 5):
+        g = FluidSim(ps[i])
+        g.set_vortk(vks[i])
+        g.run(steps)
+        m = b.members[i]
+        print(f"[batch fsi member {i}] iFLUSI {ps[i].iFLUSI} theta max {np.abs(g.beam()['theta']).max():.3e}",
+              flush=True)
+        assert m.time_state() == g.time_state()
+        assert np.array_equal(m.field("vortk"), g.field("vortk")), i
+        for k in ("x", "y", "theta", "pressure", "tau"):
+            assert np.array_equal(m.beam()[k], g.beam()[k]), (i, k)
+        so, go = m.step_out(), g.step_out()
+        for k in ("dt1", "Force", "Force_unst", "Force_press", "ibes_iter"):
+            assert so[k] == go[k], (i, k)
+        if ps[i].iFLUSI == 1:
+            assert np.array_equal(m.solid.get("theta_dot"), g.solid.get("theta_dot"))
         g.close()
     b.close()
