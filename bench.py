@@ -322,6 +322,52 @@ def batch_bench(nsims: int, steps: int, warmup: int = 3):
             "grids": {str(k): v for k, v in sorted(grids.items())}, "dtype": "f64", "data": "synthetic"}
 
 
+def adjoint_bench(steps: int = 1000, chk: int = 50):
+    """the differentiation workload (adjoint_512: 512^2, Re 500, dt 1e-4,
+    motion 601 with coefficients 0.05 N(0,1) of seed 3): wall time of J
+    alone and of J + dJ/dc (forward with checkpoints every chk steps,
+    recompute of each segment, reverse sweep), and the gradient along a
+    random unit direction against the central difference of J (h = 1e-6)"""
+    import numpy as np
+
+    from fsibe_2d_amd.adjoint import Adjoint
+    from fsibe_2d_amd.inputs import band_limited_noise, normals
+    from fsibe_2d_amd.params import adjoint_512, adjoint_coefficients
+    from fsibe_2d_amd.spectral import Spectral
+
+    p = adjoint_512()
+    sp = Spectral(p.nx, p.ny, p.xl, p.yl, p.nu)
+    vk = band_limited_noise(sp, sp.dealiase_mask(), seed=0, amp=1e-3, kmax2=64)
+    sp.close()
+    c = np.array(adjoint_coefficients(3))
+    a = Adjoint(p, vk, steps, chk)
+    a.gradient(c)  # warm-up: first launches and LDS attributes
+    t_f = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        J = a.objective(c)
+        t_f.append(time.perf_counter() - t0)
+    t_g = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        Jg, g = a.gradient(c)
+        t_g.append(time.perf_counter() - t0)
+    tm = a.timings()
+    d = np.array(normals(4, 8))
+    d /= np.linalg.norm(d)
+    h = 1e-6
+    fd = (a.objective(c + h * d) - a.objective(c - h * d)) / (2 * h)
+    a.close()
+    f, gr = min(t_f), min(t_g)
+    return {"workload": f"{p.nx}^2, Re=500, dt=1e-4, {steps} steps, motion 601 (8 coefficients, "
+                        f"0.05 N(0,1) seed 3), J = (1/T) sum(-Force(1) dt), checkpoints every {chk} steps",
+            "J": J, "J_of_gradient_run": Jg, "grad": [float(v) for v in g],
+            "forward_ms": round(1e3 * f, 3), "gradient_ms": round(1e3 * gr, 3),
+            "cost_ratio": round(gr / f, 3), "reverse_ms": round(tm["reverse_ms"], 3),
+            "directional_fd": {"h": h, "adjoint": float(g @ d), "fd": fd,
+                               "rel_err": abs(float(g @ d) - fd) / abs(fd)}}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,6 +379,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--batch-sims", type=int, default=64, help="members of the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--adjoint-steps", type=int, default=1000, help="steps of the adjoint leg (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 20 device-to-device copies of MIB MiB (PMC traffic calibration)")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -460,6 +507,7 @@ def main(argv=None) -> int:
                             f"time ratio {ratio}" if ratio else "")}
 
     batched = batch_bench(args.batch_sims, args.batch_steps) if args.batch_sims > 0 else None
+    adjoint = adjoint_bench(args.adjoint_steps) if args.adjoint_steps > 0 else None
 
     step_bytes = 848.0 * n * n  # reference's per-step algorithmic bytes (19 transforms + 30 passes)
     rec = {
@@ -490,6 +538,7 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "cpu_fft_anchor": anchor,
         "batched": batched,
+        "adjoint": adjoint,
         "step_alg_GBps": round(step_bytes * steps_per_s / 1e9, 1),
         "step_alg_frac": round(step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
         "last_step": {"dt1": out["dt1"], "Force": out["Force"], "Force_press": out["Force_press"]},

@@ -86,6 +86,8 @@ EXPORTS = (
     "fsibe_solid_create", "fsibe_solid_destroy", "fsibe_solid_get", "fsibe_solid_set", "fsibe_solid_get_info",
     "fsibe_solid_set_info", "fsibe_solid_step", "fsibe_solid_guess", "fsibe_solid_residual",
     "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid", "fsibe_solid_wrapper", "fsibe_gravity_impulse",
+    "fsibe_adjoint_create", "fsibe_adjoint_destroy", "fsibe_adjoint_objective", "fsibe_adjoint_gradient",
+    "fsibe_adjoint_timings",
 )
 
 _lib = None
@@ -161,6 +163,11 @@ def lib():
     L.fsibe_sim_solid.argtypes = [vp, pvp]
     L.fsibe_solid_wrapper.argtypes = [vp, C.c_double, C.c_double]
     L.fsibe_gravity_impulse.argtypes = [C.POINTER(FsibeParams), C.c_double, C.POINTER(C.c_double)]
+    L.fsibe_adjoint_create.argtypes = [C.POINTER(FsibeParams), vp, C.c_int, C.c_int, pvp]
+    L.fsibe_adjoint_destroy.argtypes = [vp]
+    L.fsibe_adjoint_objective.argtypes = [vp, dp, C.POINTER(C.c_double)]
+    L.fsibe_adjoint_gradient.argtypes = [vp, dp, C.POINTER(C.c_double), dp]
+    L.fsibe_adjoint_timings.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 

@@ -1,0 +1,411 @@
+// adjoint_host.hpp -- host side of the discrete adjoint (fsibe_adjoint_*),
+// part of fsibe_hip.hip's translation unit (included at its end).
+//
+// Objective (the north star's differentiation form; no reference
+// counterpart): J = (1/T) sum_n (-Force(1)^n) dt1^n over nsteps steps of a
+// prescribed-motion simulation, T = sum dt1, Force(1) the cal_drag force
+// (NavierStokes.f90:567-575); parameters: the 8 coefficients of motion 601
+// (two harmonics of heave and of pitch).  Gradient by a reverse sweep over the
+// adjoint kernels of kernels_adjoint.hpp with two-level checkpointing: the
+// forward run stores the simulation's state every `chk` steps; the reverse
+// sweep restores each checkpoint, recomputes its `chk` steps recording what
+// the adjoint needs (the tape), and runs the adjoint steps of that segment
+// backwards.  Memory: nsteps/chk checkpoints plus one tape of chk steps.
+
+static void adj_copy_d(hipStream_t st, const double *src, double *dst, size_t n) {
+  if (n == 0) return;
+  const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+  ak_copy_d<<<blocks, 256, 0, st>>>(src, dst, (long long)n);
+}
+static void adj_copy_i(hipStream_t st, const int *src, int *dst, size_t n) {
+  if (n == 0) return;
+  const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+  ak_copy_i<<<blocks, 256, 0, st>>>(src, dst, (long long)n);
+}
+static void adj_zero(hipStream_t st, double *dst, size_t n) {
+  if (n == 0) return;
+  const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+  ak_zero<<<blocks, 256, 0, st>>>(dst, (long long)n);
+}
+
+struct AdjTape {
+  int cap = 0, n = 0;
+  size_t NH = 0, N = 0;
+  double2 *A = nullptr, *B = nullptr;
+  double *u1 = nullptr, *u2 = nullptr, *m = nullptr, *vx = nullptr, *vy = nullptr;
+  int *win = nullptr;
+  DevScalars *sc = nullptr;
+  struct Rec {
+    Box mbox;
+    Window win;
+    double tau = 0.0, dt1 = 0.0;
+    int explicit_step = 0;
+  };
+  std::vector<Rec> rec;
+};
+
+// after create_mask: the mask box (values, velocities, winning primitives),
+// the cal_drag window and the time of the kinematics the beam arrays hold
+static void tape_mask(fsibe_sim *s) {
+  AdjTape &t = *s->tape;
+  if (t.n >= t.cap) return;
+  const int j = t.n;
+  AdjTape::Rec &r = t.rec[j];
+  r.mbox = s->mbox;
+  r.win = s->win;
+  r.tau = s->beam_time;
+  const size_t nb = s->mbox.empty() ? 0 : (size_t)s->mbox.w() * s->mbox.h();
+  adj_copy_d(s->stream, s->mask_m, t.m + j * t.N, nb);
+  adj_copy_d(s->stream, s->mask_vx, t.vx + j * t.N, nb);
+  adj_copy_d(s->stream, s->mask_vy, t.vy + j * t.N, nb);
+  if (s->mask_win) adj_copy_i(s->stream, s->mask_win, t.win + j * t.N, nb);
+}
+
+// after the nonlinear-term kernel: A, B (its column inputs), u of this step
+// (it is overwritten by the velocity kernel) and the step's scalars
+static int tape_fluid(fsibe_sim *s, bool explicit_step) {
+  AdjTape &t = *s->tape;
+  if (t.n >= t.cap) return set_err(FSIBE_ESTATE, "adjoint tape full");
+  const int j = t.n;
+  adj_copy_d(s->stream, reinterpret_cast<const double *>(s->T[0]), reinterpret_cast<double *>(t.A + j * t.NH),
+             2 * t.NH);
+  adj_copy_d(s->stream, reinterpret_cast<const double *>(s->T[1]), reinterpret_cast<double *>(t.B + j * t.NH),
+             2 * t.NH);
+  adj_copy_d(s->stream, s->u1, t.u1 + j * t.N, t.N);
+  adj_copy_d(s->stream, s->u2, t.u2 + j * t.N, t.N);
+  adj_copy_d(s->stream, reinterpret_cast<const double *>(s->dscal), reinterpret_cast<double *>(t.sc + j),
+             sizeof(DevScalars) / sizeof(double));
+  HIPCHK(hipGetLastError());
+  t.rec[j].explicit_step = explicit_step ? 1 : 0;
+  t.n = j + 1;
+  return FSIBE_OK;
+}
+
+// the dynamic state of a simulation between two steps
+struct SimSnap {
+  char *dev = nullptr;
+  double time = 0, dt0 = 0, dt1 = 0, beam_time = 0, jsum_f = 0, jsum_t = 0, drag_unst_old = 0, lift_unst_old = 0;
+  int it = 0, n0 = 0, n1 = 1, slot = 0;
+  bool startup_done = false, have_beam = false, unst_ready = false;
+  std::vector<double> bx, by, bvx, bvy, btheta, btheta_dot;
+};
+
+static size_t snap_bytes(const fsibe_sim *s) {
+  const size_t NH = (size_t)s->H * s->ny, N = (size_t)s->nx * s->ny;
+  return sizeof(double2) * 4 * NH + sizeof(double) * (3 * N + s->ny) + sizeof(DevScalars);
+}
+
+// copy (to = true: sim -> snap) the device state, in the order W0 W1 NL0 NL1 u1 u2 vor_mean umax dscal
+static void snap_dev(fsibe_sim *s, SimSnap &q, bool to) {
+  const size_t NH = (size_t)s->H * s->ny, N = (size_t)s->nx * s->ny;
+  double *p = reinterpret_cast<double *>(q.dev);
+  double *fld[9] = {reinterpret_cast<double *>(s->W[0]), reinterpret_cast<double *>(s->W[1]),
+                    reinterpret_cast<double *>(s->NL[0]), reinterpret_cast<double *>(s->NL[1]),
+                    s->u1, s->u2, s->vor_mean, s->umax_part, reinterpret_cast<double *>(s->dscal)};
+  const size_t len[9] = {2 * NH, 2 * NH, 2 * NH, 2 * NH, N, N, N, (size_t)s->ny, sizeof(DevScalars) / sizeof(double)};
+  for (int k = 0; k < 9; ++k) {
+    if (to) adj_copy_d(s->stream, fld[k], p, len[k]);
+    else adj_copy_d(s->stream, p, fld[k], len[k]);
+    p += len[k];
+  }
+}
+
+static int snap_save(fsibe_sim *s, SimSnap &q) {
+  snap_dev(s, q, true);
+  HIPCHK(hipGetLastError());
+  q.time = s->time; q.dt0 = s->dt0; q.dt1 = s->dt1; q.beam_time = s->beam_time;
+  q.jsum_f = s->jsum_f; q.jsum_t = s->jsum_t; q.drag_unst_old = s->drag_unst_old; q.lift_unst_old = s->lift_unst_old;
+  q.it = s->it; q.n0 = s->n0; q.n1 = s->n1; q.slot = s->slot;
+  q.startup_done = s->startup_done; q.have_beam = s->have_beam; q.unst_ready = s->unst_ready;
+  q.bx = s->bx; q.by = s->by; q.bvx = s->bvx; q.bvy = s->bvy; q.btheta = s->btheta; q.btheta_dot = s->btheta_dot;
+  return FSIBE_OK;
+}
+
+static int snap_restore(fsibe_sim *s, SimSnap &q) {
+  HIPCHK(hipStreamSynchronize(s->stream));
+  snap_dev(s, q, false);
+  HIPCHK(hipGetLastError());
+  s->time = q.time; s->dt0 = q.dt0; s->dt1 = q.dt1; s->beam_time = q.beam_time;
+  s->jsum_f = q.jsum_f; s->jsum_t = q.jsum_t; s->drag_unst_old = q.drag_unst_old; s->lift_unst_old = q.lift_unst_old;
+  s->it = q.it; s->n0 = q.n0; s->n1 = q.n1; s->slot = q.slot;
+  s->startup_done = q.startup_done; s->have_beam = q.have_beam; s->unst_ready = q.unst_ready;
+  s->bx = q.bx; s->by = q.by; s->bvx = q.bvx; s->bvy = q.bvy; s->btheta = q.btheta; s->btheta_dot = q.btheta_dot;
+  s->res_pending = false;
+  s->slot_fluid[0] = s->slot_fluid[1] = false;
+  s->slot_forces[0] = s->slot_forces[1] = false;
+  s->k1_ready = false;
+  s->upload_pending = false;
+  return FSIBE_OK;
+}
+
+struct fsibe_adjoint {
+  fsibe_sim *s = nullptr;
+  int nsteps = 0, chk = 0, nseg = 0;
+  SimSnap init;
+  std::vector<SimSnap> snaps;
+  AdjTape tape;
+  double2 *lW = nullptr, *lNL = nullptr, *T[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double *lu1 = nullptr, *lu2 = nullptr, *lm = nullptr, *lvx = nullptr, *lvy = nullptr;
+  double *part = nullptr, *h_part = nullptr;
+  int part_blocks = 0;
+  double *d_nodes = nullptr;
+  std::vector<char *> allocs;
+  double ms_forward = 0.0, ms_gradient = 0.0, ms_reverse = 0.0;
+};
+
+static char *adj_alloc(fsibe_adjoint *a, size_t bytes) {
+  char *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
+  a->allocs.push_back(p);
+  return p;
+}
+
+extern "C" int fsibe_adjoint_destroy(fsibe_adjoint *a) {
+  if (!a) return FSIBE_OK;
+  if (a->s) (void)hipStreamSynchronize(a->s->stream);
+  for (char *p : a->allocs) (void)hipFree(p);
+  if (a->h_part) (void)hipHostFree(a->h_part);
+  if (a->s) fsibe_sim_destroy(a->s);
+  delete a;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, int nsteps, int chk,
+                                    fsibe_adjoint **out) {
+  if (!p || !out || nsteps < 1 || chk < 1) return set_err(FSIBE_EARG, "adjoint: bad argument");
+  *out = nullptr;
+  if (p->iFLUSI != 0 || p->iBeam != 1 || p->iMotion != 601)
+    return set_err(FSIBE_EUNSUPPORTED, "adjoint: needs prescribed motion 601 of one beam (iFLUSI=0, iBeam=1)");
+  if (p->iSharpTrailing != 0 || p->iSponge == 10)
+    return set_err(FSIBE_EUNSUPPORTED, "adjoint: sharp trailing edge and iSponge=10 are not differentiated");
+  fsibe_adjoint *a = new fsibe_adjoint();
+  auto fail = [&](int rc) { fsibe_adjoint_destroy(a); return rc; };
+  int rc = fsibe_sim_create(p, &a->s);
+  if (rc != FSIBE_OK) return fail(rc);
+  fsibe_sim *s = a->s;
+  if (vortk) {
+    rc = fsibe_set_vortk(s, vortk);
+    if (rc != FSIBE_OK) return fail(rc);
+  }
+  a->nsteps = nsteps;
+  a->chk = std::min(chk, nsteps);
+  a->nseg = (nsteps + a->chk - 1) / a->chk;
+  const size_t NH = (size_t)s->H * s->ny, N = (size_t)s->nx * s->ny;
+  const int cap = a->chk;
+  AdjTape &t = a->tape;
+  t.cap = cap; t.NH = NH; t.N = N;
+  t.rec.resize(cap);
+  bool ok = true;
+  auto D = [&](size_t bytes) { char *q = adj_alloc(a, bytes); ok = ok && q; return q; };
+  if (hipMalloc(&s->mask_win, sizeof(int) * N) != hipSuccess) ok = false;  // freed with the simulation
+  t.A = reinterpret_cast<double2 *>(D(sizeof(double2) * NH * cap));
+  t.B = reinterpret_cast<double2 *>(D(sizeof(double2) * NH * cap));
+  t.u1 = reinterpret_cast<double *>(D(sizeof(double) * N * cap));
+  t.u2 = reinterpret_cast<double *>(D(sizeof(double) * N * cap));
+  t.m = reinterpret_cast<double *>(D(sizeof(double) * N * cap));
+  t.vx = reinterpret_cast<double *>(D(sizeof(double) * N * cap));
+  t.vy = reinterpret_cast<double *>(D(sizeof(double) * N * cap));
+  t.win = reinterpret_cast<int *>(D(sizeof(int) * N * cap));
+  t.sc = reinterpret_cast<DevScalars *>(D(sizeof(DevScalars) * cap));
+  a->lW = reinterpret_cast<double2 *>(D(sizeof(double2) * NH));
+  a->lNL = reinterpret_cast<double2 *>(D(sizeof(double2) * NH));
+  for (int k = 0; k < 6; ++k) a->T[k] = reinterpret_cast<double2 *>(D(sizeof(double2) * NH));
+  a->lu1 = reinterpret_cast<double *>(D(sizeof(double) * N));
+  a->lu2 = reinterpret_cast<double *>(D(sizeof(double) * N));
+  a->lm = reinterpret_cast<double *>(D(sizeof(double) * N));
+  a->lvx = reinterpret_cast<double *>(D(sizeof(double) * N));
+  a->lvy = reinterpret_cast<double *>(D(sizeof(double) * N));
+  a->part_blocks = 256;
+  a->part = reinterpret_cast<double *>(D(sizeof(double) * 4 * a->part_blocks * cap));
+  a->d_nodes = reinterpret_cast<double *>(D(sizeof(double) * std::max(s->ns, 1)));
+  a->init.dev = D(snap_bytes(s));
+  a->snaps.resize(a->nseg);
+  for (auto &q : a->snaps) q.dev = D(snap_bytes(s));
+  if (!ok) return fail(set_err(FSIBE_EHIP, "adjoint: device allocation failed"));
+  if (hipHostMalloc((void **)&a->h_part, sizeof(double) * 4 * a->part_blocks * cap, hipHostMallocDefault) != hipSuccess)
+    return fail(set_err(FSIBE_EHIP, "adjoint: host allocation failed"));
+  // node arc lengths: integrate_position's prefix sums for theta = 0
+  std::vector<double> sn(std::max(s->ns, 1), 0.0);
+  for (int i = 1; i < s->ns; ++i) sn[i] = sn[i - 1] + 1.0 * s->ds;
+  if (hipMemcpy(a->d_nodes, sn.data(), sizeof(double) * sn.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(FSIBE_EHIP, "adjoint: upload failed"));
+  rc = snap_save(s, a->init);
+  if (rc == FSIBE_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = set_err(FSIBE_EHIP, "adjoint: sync failed");
+  if (rc != FSIBE_OK) return fail(rc);
+  *out = a;
+  return FSIBE_OK;
+}
+
+// the initial state with the motion coefficients c
+static int adj_begin(fsibe_adjoint *a, const double coef[8]) {
+  fsibe_sim *s = a->s;
+  for (int k = 0; k < 8; ++k) s->p.motion_coef[k] = coef[k];
+  CHK(snap_restore(s, a->init));
+  s->have_beam = false;
+  init_beam_host(s);
+  s->jsum_f = 0.0;
+  s->jsum_t = 0.0;
+  return FSIBE_OK;
+}
+
+static int adj_forward(fsibe_adjoint *a, bool save, double *J) {
+  fsibe_sim *s = a->s;
+  for (int k = 0; k < a->nseg; ++k) {
+    if (save) CHK(snap_save(s, a->snaps[k]));
+    const int len = std::min(a->chk, a->nsteps - k * a->chk);
+    fsibe_step_out o;
+    CHK(fsibe_run(s, len, &o));
+    if (s->it + 1 < k * a->chk + len) return set_err(FSIBE_ESTATE, "adjoint: the run reached Time_end");
+  }
+  if (!(s->jsum_t > 0.0)) return set_err(FSIBE_ESTATE, "adjoint: no step taken");
+  *J = s->jsum_f / s->jsum_t;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_adjoint_objective(fsibe_adjoint *a, const double coef[8], double *J) {
+  if (!a || !coef || !J) return set_err(FSIBE_EARG, "null argument");
+  const double t0 = now_ms();
+  CHK(adj_begin(a, coef));
+  CHK(adj_forward(a, false, J));
+  a->ms_forward = now_ms() - t0;
+  return FSIBE_OK;
+}
+
+// d(LEy, alpha, hdot, alpha_t)/d coef at time t for motion 601 (kinematics.hpp)
+static void motion601_jac(const fsibe_params &p, double t, double jac[4][8]) {
+  const double pi = 4.0 * atan(1.0), w0 = 2.0 * pi * p.heave_freq;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 8; ++j) jac[i][j] = 0.0;
+  for (int m = 1; m <= 2; ++m) {
+    const double wm = w0 * m, sn = sin(wm * t), cs = cos(wm * t);
+    const int ja = 2 * (m - 1), jb = ja + 1;
+    jac[0][ja] = sn;             // LEy = y0 + sum a sin + b (cos - 1)
+    jac[0][jb] = cs - 1.0;
+    jac[2][ja] = wm * cs;        // hdot
+    jac[2][jb] = -(wm * sn);
+    jac[1][4 + ja] = sn;         // alpha = AngleBeam pi/180 + sum ...
+    jac[1][4 + jb] = cs - 1.0;
+    jac[3][4 + ja] = wm * cs;    // alpha_t
+    jac[3][4 + jb] = -(wm * sn);
+  }
+}
+
+static int adj_reverse_step(fsibe_adjoint *a, int j, const double *u1n, double cJ) {
+  fsibe_sim *s = a->s;
+  AdjTape &t = a->tape;
+  const AdjTape::Rec &r = t.rec[j];
+  MaskBox mb;
+  mb.x0 = r.mbox.x0; mb.x1 = r.mbox.x1; mb.y0 = r.mbox.y0; mb.y1 = r.mbox.y1; mb.w = r.mbox.w();
+  if (r.mbox.empty()) { mb.x0 = 0; mb.x1 = -1; mb.y0 = 0; mb.y1 = -1; mb.w = 0; }
+  mb.m = t.m + j * t.N; mb.vx = t.vx + j * t.N; mb.vy = t.vy + j * t.N;
+  double2 *Ta = a->T[0], *Tb = a->T[1], *Tc = a->T[2], *Td = a->T[3], *Te = a->T[4], *Tf = a->T[5];
+  const size_t nbox = r.mbox.empty() ? 0 : (size_t)r.mbox.w() * r.mbox.h();
+  adj_zero(s->stream, a->lm, nbox);
+  adj_zero(s->stream, a->lvx, nbox);
+  adj_zero(s->stream, a->lvy, nbox);
+  CHK(with_rows<2>(s->px, [&](auto F, int nt) -> int {
+    KLAUNCH(ak_rows_vel<decltype(F)>, s->ny, nt, sizeof(double2) * 2 * padlen(s->H), s->stream, a->lu1, a->lu2,
+            u1n, mb, r.win, cJ, a->lm, a->lvx, a->lvy, Ta, Tb, F, s->twx, s->sc);
+    return FSIBE_OK;
+  }));
+  CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+    KLAUNCH(ak_cols_ab2<decltype(F)>, s->H / TX, nt, sizeof(double2) * 2 * TX * padlen(s->ny), s->stream, Ta, Tb,
+            a->lW, a->lNL, Tc, Td, t.sc + j, r.explicit_step, F, s->twy, s->sc);
+    return FSIBE_OK;
+  }));
+  CHK(with_rows<6>(s->px, [&](auto F, int nt) -> int {
+    KLAUNCH(ak_rows_nl<decltype(F)>, s->ny, nt, sizeof(double2) * 6 * padlen(s->H), s->stream, Tc, Td,
+            t.A + j * t.NH, t.B + j * t.NH, t.u1 + j * t.N, t.u2 + j * t.N,
+            s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, a->lu1, a->lu2, a->lm, a->lvx, a->lvy, Te, Tf,
+            F, s->twx, s->sc);
+    return FSIBE_OK;
+  }));
+  CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+    KLAUNCH(ak_cols_w0<decltype(F)>, s->H / TX, nt, sizeof(double2) * 2 * TX * padlen(s->ny), s->stream, Te, Tf,
+            a->lW, F, s->twy, s->sc);
+    return FSIBE_OK;
+  }));
+  double *part = a->part + (size_t)j * 4 * a->part_blocks;
+  if (nbox > 0) {
+    double alpha, at, att, LE[6];
+    mouvement_host(s->p, s->p.iMotion, s->p.x0, s->p.y0, s->p.AngleBeam, 0.0, r.tau, alpha, at, att, LE);
+    MaskConsts mc;
+    mc.dx = s->dx; mc.dy = s->dy; mc.t = s->p.t_beam; mc.h = s->p.N_smooth * std::max(s->dx, s->dy);
+    mc.gerf = s->gerf; mc.inv_eps_unused = 0.0; mc.ds = s->ds; mc.pi = s->pi; mc.sharp = s->sharp;
+    mc.eps = s->p.eps;
+    RigidBeam rb;
+    rb.LEx = LE[0]; rb.LEy = LE[1]; rb.alpha = alpha; rb.hdot = LE[3]; rb.at = at;
+    rb.s = a->d_nodes; rb.ns = s->ns; rb.off = s->p.iCylinder == 1 ? 1 : 0;
+    ak_mask<<<a->part_blocks, 256, 0, s->stream>>>(t.win + j * t.N, a->lm, a->lvx, a->lvy, mc, rb, r.mbox.x0,
+                                                  r.mbox.y0, r.mbox.w(), r.mbox.h(), part);
+    HIPCHK(hipGetLastError());
+  } else {
+    adj_zero(s->stream, part, 4 * (size_t)a->part_blocks);
+  }
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_adjoint_gradient(fsibe_adjoint *a, const double coef[8], double *J, double grad[8]) {
+  if (!a || !coef || !J || !grad) return set_err(FSIBE_EARG, "null argument");
+  fsibe_sim *s = a->s;
+  const double t0 = now_ms();
+  CHK(adj_begin(a, coef));
+  CHK(adj_forward(a, true, J));
+  const double t1 = now_ms();
+  const double T = s->jsum_t;
+  const size_t NH = (size_t)s->H * s->ny, N = (size_t)s->nx * s->ny;
+  adj_zero(s->stream, reinterpret_cast<double *>(a->lW), 2 * NH);
+  adj_zero(s->stream, reinterpret_cast<double *>(a->lNL), 2 * NH);
+  adj_zero(s->stream, a->lu1, N);
+  adj_zero(s->stream, a->lu2, N);
+  double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double dxdy = s->dx * s->dy;
+  for (int k = a->nseg - 1; k >= 0; --k) {
+    const int len = std::min(a->chk, a->nsteps - k * a->chk);
+    // recompute the segment, recording it
+    CHK(snap_restore(s, a->snaps[k]));
+    AdjTape &t = a->tape;
+    t.n = 0;
+    s->tape = &t;
+    int rc = FSIBE_OK;
+    for (int j = 0; j < len && rc == FSIBE_OK; ++j) {
+      rc = run_one(s, false, false);
+      if (rc == FSIBE_OK) {
+        t.rec[j].dt1 = s->dt1;
+        if (t.n != j + 1) rc = set_err(FSIBE_ESTATE, "adjoint: step %d was not recorded", k * a->chk + j);
+      }
+    }
+    s->tape = nullptr;
+    CHK(rc);
+    for (int j = len - 1; j >= 0; --j) {
+      const double *u1n = j == len - 1 ? s->u1 : t.u1 + (size_t)(j + 1) * t.N;
+      CHK(adj_reverse_step(a, j, u1n, -(t.rec[j].dt1 * dxdy) / T));
+    }
+    HIPCHK(hipMemcpyAsync(a->h_part, a->part, sizeof(double) * 4 * a->part_blocks * len, hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int j = len - 1; j >= 0; --j) {
+      double q[4] = {0.0, 0.0, 0.0, 0.0};
+      const double *pp = a->h_part + (size_t)j * 4 * a->part_blocks;
+      for (int b = 0; b < a->part_blocks; ++b)
+        for (int i = 0; i < 4; ++i) q[i] += pp[4 * b + i];
+      double jac[4][8];
+      motion601_jac(s->p, t.rec[j].tau, jac);
+      for (int c = 0; c < 8; ++c)
+        for (int i = 0; i < 4; ++i) g[c] += q[i] * jac[i][c];
+    }
+  }
+  for (int c = 0; c < 8; ++c) grad[c] = g[c];
+  const double t2 = now_ms();
+  a->ms_gradient = t2 - t0;
+  a->ms_reverse = t2 - t1;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_adjoint_timings(fsibe_adjoint *a, double *forward_ms, double *gradient_ms, double *reverse_ms) {
+  if (!a) return set_err(FSIBE_EARG, "null adjoint");
+  if (forward_ms) *forward_ms = a->ms_forward;
+  if (gradient_ms) *gradient_ms = a->ms_gradient;
+  if (reverse_ms) *reverse_ms = a->ms_reverse;
+  return FSIBE_OK;
+}

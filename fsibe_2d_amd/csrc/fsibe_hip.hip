@@ -29,6 +29,7 @@
 #include "kernels.hpp"
 #include "kernels_mask.hpp"
 #include "kernels_step.hpp"
+#include "kernels_adjoint.hpp"
 #include "kinematics.hpp"
 #include "solid.hpp"
 
@@ -706,6 +707,7 @@ struct fsibe_sim {
   double *umax_part;
   DevScalars *dscal;
   double *mask_m, *mask_vx, *mask_vy;
+  int *mask_win = nullptr;  // winning primitive per mask pixel (adjoint runs only)
   double *fb_p, *fb_sa, *fb_sb;
   // per-step upload area: prims | targets | node data | circle data | vals
   char *h_stage, *d_stage;
@@ -751,7 +753,16 @@ struct fsibe_sim {
   fsibe_step_out last;
   std::vector<double> pbeam, tbeam;
   fsibe_solid *solid = nullptr;  // iFLUSI=1: the beam solved by the library (fsibe_run)
+  // time of the kinematics in the beam arrays (integrate_position's time)
+  double beam_time = 0.0;
+  // sums of -Force(1) dt1 and dt1 over the steps read so far (the objective
+  // of the adjoint, J = (1/T) sum (-Force(1)) dt)
+  double jsum_f = 0.0, jsum_t = 0.0;
+  struct AdjTape *tape = nullptr;  // records the forward data of each step for the adjoint
 };
+
+static void tape_mask(fsibe_sim *s);
+static int tape_fluid(fsibe_sim *s, bool explicit_step);
 
 // host phases timed with the steady clock into the otherwise unused classes
 static double now_ms() {
@@ -907,6 +918,7 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
       if (s->prof.ev[k][j]) (void)hipEventDestroy(s->prof.ev[k][j]);
   (void)hipStreamDestroy(s->stream);
   delete s->solid;
+  if (s->mask_win) (void)hipFree(s->mask_win);
   delete s;
   return FSIBE_OK;
 }
@@ -1383,10 +1395,11 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
     prof_begin(s, P_MASK);
     k_mask<<<grd, blk, 0, s->stream>>>(reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim,
                                        s->wallrow ? s->wallrow + 0 : nullptr, mc, s->mbox.x0, s->mbox.y0,
-                                       s->mbox.w(), s->mbox.h(), s->mask_m, s->mask_vx, s->mask_vy);
+                                       s->mbox.w(), s->mbox.h(), s->mask_m, s->mask_vx, s->mask_vy, s->mask_win);
     HIPCHK(hipGetLastError());
     prof_end(s, P_MASK);
   }
+  if (s->tape) tape_mask(s);
   CHK(ensure_sponge(s));
   return FSIBE_OK;
 }
@@ -1488,6 +1501,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     prof_end(s, P_ROWS_NL);
     if (s->pipe) HIPCHK(hipEventRecord(s->ev_dt, s->stream));
   }
+  if (s->tape) CHK(tape_fluid(s, explicit_step));
   if (sp10) {
     // NavierStokes.f90:108-109: ixmin = nint((xl - SpongeSize)/dx), ixmax = nx - 1 - 15
     const int ixmin = (int)lround((s->p.xl - s->p.SpongeSize) / s->dx), ixmax = s->nx - 1 - 15;
@@ -1628,6 +1642,8 @@ static void parse_results(fsibe_sim *s, int k) {
     if (s->p.iBeam > 0) {
       r.Force[0] = tot[7] * dxdy;
       r.Force[1] = tot[8] * dxdy;
+      s->jsum_f += -r.Force[0] * o->dt1;
+      s->jsum_t += o->dt1;
       const double dn = tot[9] * dxdy * s->p.eps, ln = tot[10] * dxdy * s->p.eps;
       if (s->unst_ready) {
         r.Force_unst[0] = (dn - s->drag_unst_old) / o->dt1;
@@ -1801,6 +1817,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
   } else if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0) {
     integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
                             s->by.data(), s->bvx.data(), s->bvy.data());
+    s->beam_time = s->time;
   }
   std::swap(s->n0, s->n1);
   s->time = s->time + s->dt1;
@@ -1821,6 +1838,7 @@ static void init_beam_host(fsibe_sim *s) {
   }
   integrate_position_host(p, p.iMotion, 0.0, s->btheta.data(), s->btheta_dot.data(), s->bx.data(), s->by.data(),
                           s->bvx.data(), s->bvy.data());
+  s->beam_time = 0.0;
 }
 
 extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
@@ -2535,3 +2553,6 @@ extern "C" fsibe_sim *fsibe_batch_member(fsibe_batch *b, int i) {
   if (!b || i < 0 || i >= (int)b->m.size()) return nullptr;
   return b->m[i];
 }
+
+// ---- discrete adjoint (fsibe_adjoint_*) -------------------------------------
+#include "adjoint_host.hpp"

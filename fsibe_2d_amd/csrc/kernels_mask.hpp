@@ -123,10 +123,12 @@ __device__ __forceinline__ double prim_value(const Prim &P, const MaskConsts &c,
   return -2.0;
 }
 
+// owin (may be null): index of the primitive that set each pixel's value
+// (-1: none, the wall or zero), read by the adjoint of the mask
 __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np, const double *__restrict__ wallrow,
                                           const MaskConsts &c, int bx0, int by0, int bw, int bh,
                                           double *__restrict__ om, double *__restrict__ ovx,
-                                          double *__restrict__ ovy, int tbx, int tby) {
+                                          double *__restrict__ ovy, int *__restrict__ owin, int tbx, int tby) {
 #pragma clang fp contract(off)
   __shared__ int4 bnd[kMaxTilePrims];  // ix0, ix1, iy0, iy1
   __shared__ Prim sp[kMaskStage];
@@ -148,6 +150,7 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
   const int i = bx0 + lx, j = by0 + ly;
   const double xi = (double)i * c.dx, yj = (double)j * c.dy;
   double m = (valid && wallrow) ? wallrow[j] : 0.0, mvx = 0.0, mvy = 0.0;
+  int wi = -1;
   while (true) {
     // wave 0 compacts the next primitives that meet the tile, in order
     if (wv == 0) {
@@ -204,11 +207,13 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
         if (P.type == PRIM_SHARP) {
           if (v > -1.5) {
             m = v;
+            wi = list[q];
             const double xs = xi * a[4] + yj * a[5];
             if (xs - a[0] >= 2.0 * a[9]) { mvx = a[7]; mvy = a[8]; }
           }
         } else if (v > m) {
           m = v;
+          wi = list[q];
           if (P.type == PRIM_SEG) {
             const double xs = xi * a[4] - yj * a[5];
             mvx = (a[7] + (xs - a[0]) * (a[9] - a[7]) / c.ds);
@@ -229,13 +234,14 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
   om[q] = m / c.eps;  // :210-214
   ovx[q] = mvx;
   ovy[q] = mvy;
+  if (owin) owin[q] = wi;
 }
 
 __global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ prims, int np,
                                               const double *__restrict__ wallrow, MaskConsts c, int bx0, int by0,
                                               int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
-                                              double *__restrict__ ovy) {
-  mask_body(prims, np, wallrow, c, bx0, by0, bw, bh, om, ovx, ovy, blockIdx.x, blockIdx.y);
+                                              double *__restrict__ ovy, int *__restrict__ owin) {
+  mask_body(prims, np, wallrow, c, bx0, by0, bw, bh, om, ovx, ovy, owin, blockIdx.x, blockIdx.y);
 }
 
 // batched: one record per simulation at blockIdx.y, tiles flattened on x
@@ -250,7 +256,7 @@ __global__ void __launch_bounds__(kMaskThreads) kb_mask(const MaskArgs *__restri
   const MaskArgs &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.tiles) return;
   const int tby = blockIdx.x / a.tiles_x, tbx = blockIdx.x - tby * a.tiles_x;
-  mask_body(a.prims, a.np, a.wallrow, a.c, a.bx0, a.by0, a.bw, a.bh, a.om, a.ovx, a.ovy, tbx, tby);
+  mask_body(a.prims, a.np, a.wallrow, a.c, a.bx0, a.by0, a.bw, a.bh, a.om, a.ovx, a.ovy, nullptr, tbx, tby);
 }
 
 }  // namespace fsibe

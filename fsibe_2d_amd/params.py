@@ -115,6 +115,23 @@ def turbulence_4096(n: int = 4096) -> Params:
                   nu=1e-5, dt_fixed=1e-4, cfl=1e6, xl=2 * math.pi, yl=2 * math.pi, iMotion=0)
 
 
+def adjoint_512(n: int = 512) -> Params:
+    """The differentiation workload: n^2, Re = 500, dt = 1e-4 fixed, the beam
+    following prescribed rigid heave + pitch (motion 601: two harmonics each,
+    base frequency 1), no pressure or stresses; physics otherwise as
+    fsi_256()."""
+    return Params(nx=n, ny=n, ns=64, nu=1.0 / 500.0, dt_fixed=1e-4, cfl=100.0, iMotion=MOTION_HEAVE_PITCH,
+                  heave_freq=1.0, iSaveBeam=0, iFLUSI=0)
+
+
+def adjoint_coefficients(seed: int = 3) -> list:
+    """the 8 motion coefficients of the differentiation workload: N(0, 0.05)
+    draws of the SplitMix64 / Box-Muller generator"""
+    from .inputs import normals
+
+    return [0.05 * float(v) for v in normals(seed, 8)]
+
+
 def batch_configs(n: int = 64, seed: int = 2, coupled: bool = True) -> list:
     """The batched workload: n independent FSI simulations with uneven grids
     and parameters, drawn per member in this order from SplitMix64(seed)

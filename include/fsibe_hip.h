@@ -307,6 +307,26 @@ int fsibe_batch_run(fsibe_batch *batch, int nsteps);
 int fsibe_batch_size(fsibe_batch *batch);
 fsibe_sim *fsibe_batch_member(fsibe_batch *batch, int i);
 
+/* ---- discrete adjoint ---------------------------------------------------
+ * No reference counterpart (the reference has no adjoint).  For a
+ * prescribed-motion simulation (iFLUSI=0, one beam, motion case 601: two
+ * harmonics of heave and of pitch, coefficients motion_coef[8]):
+ *     J(c) = (1/T) sum_n (-Force(1)^n) dt1^n,  T = sum_n dt1^n
+ * over nsteps steps from the initial vorticity vortk (packed layout), where
+ * Force(1) is cal_drag's penalisation force (NavierStokes.f90:567-575).
+ * fsibe_adjoint_gradient returns J and dJ/dc by the discrete adjoint of the
+ * GPU step (reverse sweep over recomputed segments of chk steps, one
+ * checkpoint per segment); fsibe_adjoint_objective runs the forward only.
+ * The time step is taken as fixed (dt_fixed, CFL not binding). */
+typedef struct fsibe_adjoint fsibe_adjoint;
+int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, int nsteps, int chk, fsibe_adjoint **out);
+int fsibe_adjoint_destroy(fsibe_adjoint *adj);
+int fsibe_adjoint_objective(fsibe_adjoint *adj, const double coef[8], double *J);
+int fsibe_adjoint_gradient(fsibe_adjoint *adj, const double coef[8], double *J, double grad[8]);
+/* wall times of the last objective / gradient call (the gradient's time
+ * includes its forward run; reverse_ms is the part after it) */
+int fsibe_adjoint_timings(fsibe_adjoint *adj, double *forward_ms, double *gradient_ms, double *reverse_ms);
+
 /* HIP stream of the simulation (hipStream_t) for event timing */
 void *fsibe_sim_stream(fsibe_sim *sim);
 

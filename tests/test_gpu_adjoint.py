@@ -1,0 +1,130 @@
+"""The discrete adjoint (fsibe_adjoint_*): J(c) = (1/T) sum_n (-Force(1)^n) dt1^n
+over a prescribed-motion run (motion 601, c = its 8 Fourier coefficients) and
+dJ/dc by the reverse sweep.  The reference has no adjoint; the gradient is
+pinned by central finite differences of the same GPU objective, whose
+Force(1) terms are the cal_drag forces (NavierStokes.f90:567-575) that the
+step's own parity tests hold to the oracle, and J itself is tied to the
+step_out record of a plain FluidSim run."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n, ns, seed=0):
+    from fsibe_2d_amd import Params
+    from fsibe_2d_amd.inputs import band_limited_noise
+    from fsibe_2d_amd.spectral import Spectral
+
+    p = Params(nx=n, ny=n, ns=ns, iMotion=601, heave_freq=2.0, dt_fixed=2e-4, cfl=100.0, nu=1 / 500,
+               iSaveBeam=0)
+    sp = Spectral(n, n, p.xl, p.yl, p.nu)
+    vk = band_limited_noise(sp, sp.dealiase_mask(), seed=seed, amp=1e-3, kmax2=64)
+    sp.close()
+    return p, vk
+
+
+def _coef(seed=3):
+    from fsibe_2d_amd.params import adjoint_coefficients
+
+    return np.array(adjoint_coefficients(seed))
+
+
+@pytest.mark.parametrize("n,ns,steps,chk", [(64, 24, 30, 8), (128, 48, 60, 16)])
+def test_gradient_against_finite_differences(n, ns, steps, chk):
+    """each component of dJ/dc against (J(c + h e_k) - J(c - h e_k)) / 2h,
+    h = 1e-6: relative error within 1e-6 (central differences of a smooth
+    J carry O(h^2) truncation and O(eps J / h) rounding, ~1e-9 here)"""
+    from fsibe_2d_amd.adjoint import Adjoint
+
+    p, vk = _case(n, ns)
+    c = _coef()
+    a = Adjoint(p, vk, steps, chk)
+    J, g = a.gradient(c)
+    assert a.objective(c) == J
+    h = 1e-6
+    fd = np.zeros(8)
+    for k in range(8):
+        e = np.zeros(8)
+        e[k] = h
+        fd[k] = (a.objective(c + e) - a.objective(c - e)) / (2 * h)
+    rel = np.abs(fd - g) / np.abs(fd)
+    print(f"[adjoint {n}] J {J:.10e} grad {g} rel {rel}", flush=True)
+    assert np.all(np.abs(g) > 1e-3)
+    assert rel.max() <= 1e-6
+    a.close()
+
+
+def test_gradient_independent_of_checkpoint_spacing():
+    """the checkpoint spacing changes only what is recomputed, not the
+    arithmetic: J and dJ/dc are bit-identical for chk = 1, 7 and nsteps"""
+    from fsibe_2d_amd.adjoint import Adjoint
+
+    p, vk = _case(64, 24, seed=5)
+    c = _coef(7)
+    res = []
+    for chk in (1, 7, 21):
+        a = Adjoint(p, vk, 21, chk)
+        res.append(a.gradient(c))
+        a.close()
+    for J, g in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(g, res[0][1])
+
+
+def test_objective_is_the_force_record_of_a_plain_run():
+    """J = (1/T) sum (-Force(1) dt1) from the step_out records of FluidSim
+    stepping the same run one step at a time"""
+    from fsibe_2d_amd.adjoint import Adjoint
+    from fsibe_2d_amd.solver import FluidSim
+
+    p, vk = _case(64, 24, seed=1)
+    c = _coef(11)
+    p.motion_coef = list(c)
+    steps = 17
+    a = Adjoint(p, vk, steps, 5)
+    J = a.objective(c)
+    a.close()
+    g = FluidSim(p)
+    g.set_vortk(vk)
+    num = den = 0.0
+    for _ in range(steps):
+        out = g.run(1)
+        num += -out["Force"][0] * out["dt1"]
+        den += out["dt1"]
+    g.close()
+    assert J == pytest.approx(num / den, rel=1e-13)
+
+
+def test_full_size_directional_derivative():
+    """the differentiation workload's grid and physics (512^2, Re 500,
+    dt 1e-4) over 100 steps with checkpoints every 50: the gradient along a
+    random direction d against the central difference of J along d"""
+    from fsibe_2d_amd.adjoint import Adjoint
+    from fsibe_2d_amd.inputs import band_limited_noise, normals
+    from fsibe_2d_amd.params import adjoint_512
+    from fsibe_2d_amd.spectral import Spectral
+
+    p = adjoint_512()
+    sp = Spectral(p.nx, p.ny, p.xl, p.yl, p.nu)
+    vk = band_limited_noise(sp, sp.dealiase_mask(), seed=0, amp=1e-3, kmax2=64)
+    sp.close()
+    c = _coef()
+    d = np.array(normals(4, 8))
+    d /= np.linalg.norm(d)
+    a = Adjoint(p, vk, 100, 50)
+    J, g = a.gradient(c)
+    h = 1e-6
+    fd = (a.objective(c + h * d) - a.objective(c - h * d)) / (2 * h)
+    print(f"[adjoint 512] J {J:.10e} g.d {g @ d:.10e} fd {fd:.10e}", flush=True)
+    assert abs(g @ d - fd) <= 1e-6 * abs(fd)
+    a.close()
+
+
+def test_unsupported_configurations_are_refused():
+    from fsibe_2d_amd import FsibeError, Params
+    from fsibe_2d_amd.adjoint import Adjoint
+
+    for kw in (dict(iFLUSI=1), dict(iMotion=600), dict(iSharpTrailing=1)):
+        with pytest.raises(FsibeError):
+            Adjoint(Params(nx=64, ny=64, ns=24, **({"iMotion": 601} | kw)), None, 10, 5)
