@@ -322,12 +322,13 @@ def batch_bench(nsims: int, steps: int, warmup: int = 3):
             "grids": {str(k): v for k, v in sorted(grids.items())}, "dtype": "f64", "data": "synthetic"}
 
 
-def adjoint_bench(steps: int = 1000, chk: int = 50):
+def adjoint_bench(steps: int = 1000, chk: int = 50, smooth: bool = True):
     """the differentiation workload (adjoint_512: 512^2, Re 500, dt 1e-4,
     motion 601 with coefficients 0.05 N(0,1) of seed 3): wall time of J
     alone and of J + dJ/dc (forward with checkpoints every chk steps,
-    recompute of each segment, reverse sweep), and the gradient along a
-    random unit direction against the central difference of J (h = 1e-6)"""
+    recompute of each segment, reverse sweep), each gradient component
+    against the central difference of J with h = 1e-6 and 1e-7, and the
+    gradient along a random unit direction against the same (h = 1e-6)"""
     import numpy as np
 
     from fsibe_2d_amd.adjoint import Adjoint
@@ -335,7 +336,7 @@ def adjoint_bench(steps: int = 1000, chk: int = 50):
     from fsibe_2d_amd.params import adjoint_512, adjoint_coefficients
     from fsibe_2d_amd.spectral import Spectral
 
-    p = adjoint_512()
+    p = adjoint_512(smooth_cutoff=smooth)
     sp = Spectral(p.nx, p.ny, p.xl, p.yl, p.nu)
     vk = band_limited_noise(sp, sp.dealiase_mask(), seed=0, amp=1e-3, kmax2=64)
     sp.close()
@@ -357,13 +358,24 @@ def adjoint_bench(steps: int = 1000, chk: int = 50):
     d /= np.linalg.norm(d)
     h = 1e-6
     fd = (a.objective(c + h * d) - a.objective(c - h * d)) / (2 * h)
+    comp = {}
+    for hh in (1e-6, 1e-7):
+        fdk = np.zeros(8)
+        for k in range(8):
+            e = np.zeros(8)
+            e[k] = hh
+            fdk[k] = (a.objective(c + e) - a.objective(c - e)) / (2 * hh)
+        comp[f"{hh:g}"] = float(np.max(np.abs(fdk - g) / np.abs(fdk)))
     a.close()
     f, gr = min(t_f), min(t_g)
     return {"workload": f"{p.nx}^2, Re=500, dt=1e-4, {steps} steps, motion 601 (8 coefficients, "
-                        f"0.05 N(0,1) seed 3), J = (1/T) sum(-Force(1) dt), checkpoints every {chk} steps",
-            "J": J, "J_of_gradient_run": Jg, "grad": [float(v) for v in g],
+                        f"0.05 N(0,1) seed 3), J = (1/T) sum(-Force(1) dt), checkpoints every {chk} steps, "
+                        + ("mask cut-offs and cal_drag window moved out (iSmoothCutoff=1)" if smooth
+                           else "the reference's mask cut-offs and cal_drag window"),
+            "iSmoothCutoff": int(smooth), "J": J, "J_of_gradient_run": Jg, "grad": [float(v) for v in g],
             "forward_ms": round(1e3 * f, 3), "gradient_ms": round(1e3 * gr, 3),
             "cost_ratio": round(gr / f, 3), "reverse_ms": round(tm["reverse_ms"], 3),
+            "fd_max_rel_err_per_component": comp,
             "directional_fd": {"h": h, "adjoint": float(g @ d), "fd": fd,
                                "rel_err": abs(float(g @ d) - fd) / abs(fd)}}
 
@@ -507,7 +519,10 @@ def main(argv=None) -> int:
                             f"time ratio {ratio}" if ratio else "")}
 
     batched = batch_bench(args.batch_sims, args.batch_steps) if args.batch_sims > 0 else None
-    adjoint = adjoint_bench(args.adjoint_steps) if args.adjoint_steps > 0 else None
+    adjoint = adjoint_ref = None
+    if args.adjoint_steps > 0:
+        adjoint = adjoint_bench(args.adjoint_steps, smooth=True)
+        adjoint_ref = adjoint_bench(args.adjoint_steps, smooth=False)
 
     step_bytes = 848.0 * n * n  # reference's per-step algorithmic bytes (19 transforms + 30 passes)
     rec = {
@@ -539,6 +554,7 @@ def main(argv=None) -> int:
         "cpu_fft_anchor": anchor,
         "batched": batched,
         "adjoint": adjoint,
+        "adjoint_reference_cutoffs": adjoint_ref,
         "step_alg_GBps": round(step_bytes * steps_per_s / 1e9, 1),
         "step_alg_frac": round(step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
         "last_step": {"dt1": out["dt1"], "Force": out["Force"], "Force_press": out["Force_press"]},

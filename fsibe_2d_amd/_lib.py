@@ -45,6 +45,7 @@ class FsibeParams(C.Structure):
         ("motion_coef", C.c_double * 8),
         ("mue", C.c_double), ("eta", C.c_double), ("sigma", C.c_double), ("grav", C.c_double),
         ("TimeMethodSolid", C.c_int32), ("iImpulse", C.c_int32),
+        ("iSmoothCutoff", C.c_int32),
     ]
 
 

@@ -178,6 +178,8 @@ extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, 
     return set_err(FSIBE_EUNSUPPORTED, "adjoint: needs prescribed motion 601 of one beam (iFLUSI=0, iBeam=1)");
   if (p->iSharpTrailing != 0 || p->iSponge == 10)
     return set_err(FSIBE_EUNSUPPORTED, "adjoint: sharp trailing edge and iSponge=10 are not differentiated");
+  if (!(p->dt_fixed > 0.0))
+    return set_err(FSIBE_EUNSUPPORTED, "adjoint: needs a fixed time step (dt_fixed > 0); dt1 is not differentiated");
   fsibe_adjoint *a = new fsibe_adjoint();
   auto fail = [&](int rc) { fsibe_adjoint_destroy(a); return rc; };
   int rc = fsibe_sim_create(p, &a->s);
@@ -373,6 +375,9 @@ extern "C" int fsibe_adjoint_gradient(fsibe_adjoint *a, const double coef[8], do
       if (rc == FSIBE_OK) {
         t.rec[j].dt1 = s->dt1;
         if (t.n != j + 1) rc = set_err(FSIBE_ESTATE, "adjoint: step %d was not recorded", k * a->chk + j);
+        else if (s->dt1 != s->p.dt_fixed)
+          rc = set_err(FSIBE_EDOMAIN, "adjoint: step %d took dt1 = %g from the CFL bound or Time_end, not dt_fixed",
+                       k * a->chk + j, s->dt1);
       }
     }
     s->tape = nullptr;

@@ -1107,6 +1107,9 @@ static int build_mask_geometry(fsibe_sim *s) {
     mb.add(0, nx - 1, std::max(0, fnint((s->p.yl - h_star) / dy)), ny - 1);
   }
   const double delta = N * mdx * s->gerf;
+  // segments and hinges: create_mask's t* = t + 2 N max(dx,dy), or with
+  // iSmoothCutoff = 1 the end dots' reach t + 7 delta (smooth step < 1e-22)
+  const double t_cut = s->p.iSmoothCutoff ? t + 7.0 * delta : t + 2.0 * N * mdx;
   auto dot = [&](double px_, double py_, double vx_, double vy_, double tt) {
     Prim P{};
     P.type = PRIM_DOT;
@@ -1122,7 +1125,7 @@ static int build_mask_geometry(fsibe_sim *s) {
   if (s->p.iCylinder == 1) dot(s->p.x0 - s->p.R_cylinder, s->p.y0, 0.0, 0.0, s->p.R_cylinder);
   if (s->p.iBeam > 0) {
     const double *X = s->bx.data(), *Y = s->by.data(), *VX = s->bvx.data(), *VY = s->bvy.data();
-    const double t_star = t + 2.0 * N * mdx;
+    const double t_star = t_cut;
     for (int i = 0; i <= ns - 2; ++i) {  // DrawBeamSegment :740-761
       Prim P{};
       P.type = PRIM_SEG;
@@ -1163,7 +1166,7 @@ static int build_mask_geometry(fsibe_sim *s) {
       P.type = PRIM_HINGE;
       const double a1 = atan2((Y[i] - Y[i - 1]), (X[i] - X[i - 1]));
       const double a2 = atan2((Y[i + 1] - Y[i]), (X[i + 1] - X[i]));
-      const double ts2 = t + 2. * N * mdx;
+      const double ts2 = t_cut;
       P.ix0 = std::max(fnint((X[i] - ts2) / dx), 0);
       P.ix1 = std::min(fnint((X[i] + ts2) / dx), nx - 1);
       P.iy0 = std::max(fnint((Y[i] - ts2) / dy), 0);
@@ -1200,6 +1203,19 @@ static int build_mask_geometry(fsibe_sim *s) {
       ixmax = (int)((xmax + 1.5 * tb) / dx);
     } else {
       iymax = ny - 1; iymin = 0; ixmax = nx - 1; ixmin = 0;
+    }
+    if (s->p.iBeam > 0 && s->p.iSmoothCutoff) {
+      // every pixel within t + 7 delta + max(dx,dy) of a node
+      double ymax = Y[0], ymin = Y[0], xmin = X[0], xmax = X[0];
+      for (int i = 1; i < ns; ++i) {
+        ymax = std::max(ymax, Y[i]); ymin = std::min(ymin, Y[i]);
+        xmin = std::min(xmin, X[i]); xmax = std::max(xmax, X[i]);
+      }
+      const double r = t + 7.0 * delta + mdx;
+      ixmin = std::min(ixmin, fnint((xmin - r) / dx));
+      ixmax = std::max(ixmax, fnint((xmax + r) / dx));
+      iymin = std::min(iymin, fnint((ymin - r) / dy));
+      iymax = std::max(iymax, fnint((ymax + r) / dy));
     }
     const double h_star = s->p.h_channel + s->p.N_smooth * dy;
     if (s->p.iWalls > 0) {

@@ -36,6 +36,7 @@ contains
     p%heave_amp = 0.0_c_double; p%heave_freq = 0.0_c_double; p%motion_coef = 0.0_c_double
     p%mue = mue; p%eta = eta; p%sigma = sigma; p%grav = grav
     p%TimeMethodSolid = TimeMethodSolid; p%iImpulse = iImpulse
+    p%iSmoothCutoff = 0
     call fsibe_check(fsibe_sim_create(p, fsibe_sim), "fsibe_sim_create")
     call fsibe_check(fsibe_set_vortk(fsibe_sim, vortk_n0), "fsibe_set_vortk")
   end subroutine hip_fluid_create

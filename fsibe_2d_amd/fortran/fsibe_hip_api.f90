@@ -31,6 +31,7 @@ module fsibe_hip_api
     real(c_double) :: motion_coef(8)
     real(c_double) :: mue, eta, sigma, grav
     integer(c_int32_t) :: TimeMethodSolid, iImpulse
+    integer(c_int32_t) :: iSmoothCutoff
   end type
 
   ! struct fsibe_step_out
@@ -286,6 +287,35 @@ module fsibe_hip_api
       type(fsibe_params_t), intent(in) :: p
       real(c_double), value :: time
       real(c_double), intent(inout) :: grav
+      integer(c_int) :: rc
+    end function
+    ! discrete adjoint (no reference counterpart): J = (1/T) sum(-Force(1) dt)
+    ! over nsteps steps of motion 601 and dJ/dcoef(8), checkpoints every chk
+    function fsibe_adjoint_create(p, vortk, nsteps, chk, adj) bind(C, name="fsibe_adjoint_create") result(rc)
+      import :: c_int, c_double, c_ptr, fsibe_params_t
+      type(fsibe_params_t), intent(in) :: p
+      real(c_double), intent(in) :: vortk(*)
+      integer(c_int), value :: nsteps, chk
+      type(c_ptr), intent(out) :: adj
+      integer(c_int) :: rc
+    end function
+    function fsibe_adjoint_destroy(adj) bind(C, name="fsibe_adjoint_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: adj
+      integer(c_int) :: rc
+    end function
+    function fsibe_adjoint_objective(adj, coef, J) bind(C, name="fsibe_adjoint_objective") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: adj
+      real(c_double), intent(in) :: coef(8)
+      real(c_double), intent(out) :: J
+      integer(c_int) :: rc
+    end function
+    function fsibe_adjoint_gradient(adj, coef, J, grad) bind(C, name="fsibe_adjoint_gradient") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: adj
+      real(c_double), intent(in) :: coef(8)
+      real(c_double), intent(out) :: J, grad(8)
       integer(c_int) :: rc
     end function
   end interface

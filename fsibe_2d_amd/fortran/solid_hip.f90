@@ -36,6 +36,7 @@ contains
     p%heave_amp = 0.0_c_double; p%heave_freq = 0.0_c_double; p%motion_coef = 0.0_c_double
     p%mue = mue; p%eta = eta; p%sigma = sigma; p%grav = grav
     p%TimeMethodSolid = TimeMethodSolid; p%iImpulse = iImpulse
+    p%iSmoothCutoff = 0
   end subroutine solid_params
 
   ! InitializeSolidSolver (SolidSolver.f90:54-67): the next step is the first

@@ -64,6 +64,9 @@ class Params:
     grav: float = 0.0
     TimeMethodSolid: int = 5
     iImpulse: int = 0
+    # 1: the mask's cut-offs and cal_drag's window moved to where the smooth
+    # step is below rounding (continuous Force; no reference counterpart)
+    iSmoothCutoff: int = 0
 
     def to_c(self) -> FsibeParams:
         p = FsibeParams()
@@ -115,13 +118,16 @@ def turbulence_4096(n: int = 4096) -> Params:
                   nu=1e-5, dt_fixed=1e-4, cfl=1e6, xl=2 * math.pi, yl=2 * math.pi, iMotion=0)
 
 
-def adjoint_512(n: int = 512) -> Params:
+def adjoint_512(n: int = 512, smooth_cutoff: bool = True) -> Params:
     """The differentiation workload: n^2, Re = 500, dt = 1e-4 fixed, the beam
     following prescribed rigid heave + pitch (motion 601: two harmonics each,
     base frequency 1), no pressure or stresses; physics otherwise as
-    fsi_256()."""
+    fsi_256().  smooth_cutoff: the mask's cut-offs and cal_drag's window
+    moved to where the smooth step is below rounding (iSmoothCutoff = 1), so
+    that J is a continuous function of the coefficients; with the reference's
+    cut-offs J jumps where a pixel crosses t* or the window's edge moves."""
     return Params(nx=n, ny=n, ns=64, nu=1.0 / 500.0, dt_fixed=1e-4, cfl=100.0, iMotion=MOTION_HEAVE_PITCH,
-                  heave_freq=1.0, iSaveBeam=0, iFLUSI=0)
+                  heave_freq=1.0, iSaveBeam=0, iFLUSI=0, iSmoothCutoff=1 if smooth_cutoff else 0)
 
 
 def adjoint_coefficients(seed: int = 3) -> list:

@@ -120,6 +120,15 @@ typedef struct fsibe_params {
    * 2 Crank-Nicolson, 5 BDF2), gravity impulse case */
   double mue, eta, sigma, grav;
   int32_t TimeMethodSolid, iImpulse;
+  /* 0: create_mask's cut-offs (segments and hinges drawn out to
+   * t* = t_beam + 2 N_smooth max(dx,dy), where the smooth step is still
+   * ~2e-7; cal_drag's window 1.5 t_beam around the beam, inside the mask's
+   * smooth zone).  1 (no reference counterpart; for differentiation):
+   * segments and hinges drawn out to t_beam + 7 delta, where the smooth
+   * step is below the rounding of 1, and the cal_drag window grown to hold
+   * every pixel within t_beam + 7 delta + max(dx,dy) of a node, so that the
+   * mask and Force are continuous functions of the beam's position. */
+  int32_t iSmoothCutoff;
 } fsibe_params;
 
 /* Per-step results copied back to the host (type Solid fields filled by

@@ -757,13 +757,22 @@ static void draw_a_dot(o_sim *s, double px, double py, double vx, double vy,
     }
 }
 
+/* the reach of segments and hinges: create_mask.f90:579, :737
+ * (t* = t + 2 N max(dx,dy)); with iSmoothCutoff = 1 (no reference
+ * counterpart) t + 7 delta, delta = N max(dx,dy) GradientERF (:511-512) */
+static double o_t_cut(const o_sim *s, double t, double N) {
+  if (!s->p.iSmoothCutoff) return t + 2.0 * N * fmax(s->dx, s->dy);
+  double GradientERF = fabs((exp(-(2.0 * 1.0) * (2.0 * 1.0)) - 1.0) / sqrt(s->pi));
+  return t + 7.0 * (N * fmax(s->dx, s->dy) * GradientERF);
+}
+
 /* DrawHinge: create_mask.f90:563-656 */
 static void draw_hinge(o_sim *s, double px, double py, double alpha1,
                        double alpha2, double vx, double vy, double t,
                        double N) {
   const int nx = s->nx, ny = s->ny;
   const double dx = s->dx, dy = s->dy, pi = s->pi;
-  double t_star = t + 2. * N * fmax(dx, dy);
+  double t_star = o_t_cut(s, t, N);
   int ixmin = imax(fnint((px - t_star) / dx), 0);
   int ixmax = imin(fnint((px + t_star) / dx), nx - 1);
   int iymin = imax(fnint((py - t_star) / dy), 0);
@@ -808,7 +817,7 @@ static void draw_beam_segment(o_sim *s, double p1x, double p1y, double p2x,
                               double v2y, double t, double N) {
   const int nx = s->nx, ny = s->ny;
   const double dx = s->dx, dy = s->dy, ds = s->ds;
-  double t_star = t + 2.0 * N * fmax(dx, dy);
+  double t_star = o_t_cut(s, t, N);
   double alpha = -atan2((p2y - p1y), (p2x - p1x));
   int ixmin = imax(fnint((fmin(p1x, p2x) - t_star) / dx), 0);
   int ixmax = imin(fnint((fmax(p1x, p2x) + t_star) / dx), nx - 1);
@@ -1083,6 +1092,13 @@ void o_cal_drag(o_sim *s, double time, double dt1, const double *penal,
     iymin = (int)((fmin(ymin - 1.5 * p->t_beam, LE[1] - 2.0 * p->R_cylinder)) / dy);
     ixmin = (int)(fmin((LE[0] - fmax(2.5 * p->R_cylinder, 1.5 * p->t_beam)), (xmin - 1.5 * p->t_beam)) / dx);
     ixmax = (int)((xmax + 1.5 * p->t_beam) / dx);
+    if (p->iSmoothCutoff) { /* every pixel within t + 7 delta + max(dx,dy) of a node */
+      double r = o_t_cut(s, p->t_beam, p->N_smooth) + fmax(dx, dy);
+      ixmin = imin(ixmin, fnint((xmin - r) / dx));
+      ixmax = imax(ixmax, fnint((xmax + r) / dx));
+      iymin = imin(iymin, fnint((ymin - r) / dy));
+      iymax = imax(iymax, fnint((ymax + r) / dy));
+    }
   } else {
     iymax = ny - 1; iymin = 0; ixmax = nx - 1; ixmin = 0;
   }

@@ -55,6 +55,9 @@ typedef struct o_params {
   /* beam material (used by the host solid solver, not by the oracle) */
   double mue, eta, sigma, grav;
   int32_t TimeMethodSolid, iImpulse;
+  /* 1: segments and hinges drawn out to t + 7 delta and the cal_drag window
+   * grown to the beam's whole mask (a form the reference does not have) */
+  int32_t iSmoothCutoff;
 } o_params;
 
 /* One beam (reference share_vars.f90:50-63, type Solid); arrays length ns. */

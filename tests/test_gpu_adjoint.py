@@ -11,13 +11,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _case(n, ns, seed=0):
+def _case(n, ns, seed=0, smooth=0):
     from fsibe_2d_amd import Params
     from fsibe_2d_amd.inputs import band_limited_noise
     from fsibe_2d_amd.spectral import Spectral
 
     p = Params(nx=n, ny=n, ns=ns, iMotion=601, heave_freq=2.0, dt_fixed=2e-4, cfl=100.0, nu=1 / 500,
-               iSaveBeam=0)
+               iSaveBeam=0, iSmoothCutoff=smooth)
     sp = Spectral(n, n, p.xl, p.yl, p.nu)
     vk = band_limited_noise(sp, sp.dealiase_mask(), seed=seed, amp=1e-3, kmax2=64)
     sp.close()
@@ -30,26 +30,33 @@ def _coef(seed=3):
     return np.array(adjoint_coefficients(seed))
 
 
-@pytest.mark.parametrize("n,ns,steps,chk", [(64, 24, 30, 8), (128, 48, 60, 16)])
-def test_gradient_against_finite_differences(n, ns, steps, chk):
-    """each component of dJ/dc against (J(c + h e_k) - J(c - h e_k)) / 2h,
-    h = 1e-6: relative error within 1e-6 (central differences of a smooth
-    J carry O(h^2) truncation and O(eps J / h) rounding, ~1e-9 here)"""
-    from fsibe_2d_amd.adjoint import Adjoint
-
-    p, vk = _case(n, ns)
-    c = _coef()
-    a = Adjoint(p, vk, steps, chk)
-    J, g = a.gradient(c)
-    assert a.objective(c) == J
-    h = 1e-6
+def _fd(a, c, h):
     fd = np.zeros(8)
     for k in range(8):
         e = np.zeros(8)
         e[k] = h
         fd[k] = (a.objective(c + e) - a.objective(c - e)) / (2 * h)
+    return fd
+
+
+@pytest.mark.parametrize("n,ns,steps,chk,smooth", [(64, 24, 30, 8, 0), (128, 48, 60, 16, 0),
+                                                   (128, 48, 60, 16, 1)])
+def test_gradient_against_finite_differences(n, ns, steps, chk, smooth):
+    """each component of dJ/dc against (J(c + h e_k) - J(c - h e_k)) / 2h,
+    h = 1e-6: relative error within 1e-6 (central differences of a smooth
+    J carry O(h^2) truncation and O(eps J / h) rounding, ~1e-9 here).  Over
+    these short runs the beam moves too little for a pixel to cross the
+    reference's cut-offs, so both cut-off forms are smooth here."""
+    from fsibe_2d_amd.adjoint import Adjoint
+
+    p, vk = _case(n, ns, smooth=smooth)
+    c = _coef()
+    a = Adjoint(p, vk, steps, chk)
+    J, g = a.gradient(c)
+    assert a.objective(c) == J
+    fd = _fd(a, c, 1e-6)
     rel = np.abs(fd - g) / np.abs(fd)
-    print(f"[adjoint {n}] J {J:.10e} grad {g} rel {rel}", flush=True)
+    print(f"[adjoint {n} cutoff {smooth}] J {J:.10e} grad {g} rel {rel}", flush=True)
     assert np.all(np.abs(g) > 1e-3)
     assert rel.max() <= 1e-6
     a.close()
@@ -96,19 +103,45 @@ def test_objective_is_the_force_record_of_a_plain_run():
     assert J == pytest.approx(num / den, rel=1e-13)
 
 
-def test_full_size_directional_derivative():
-    """the differentiation workload's grid and physics (512^2, Re 500,
-    dt 1e-4) over 100 steps with checkpoints every 50: the gradient along a
-    random direction d against the central difference of J along d"""
-    from fsibe_2d_amd.adjoint import Adjoint
-    from fsibe_2d_amd.inputs import band_limited_noise, normals
+def _workload(smooth):
+    from fsibe_2d_amd.inputs import band_limited_noise
     from fsibe_2d_amd.params import adjoint_512
     from fsibe_2d_amd.spectral import Spectral
 
-    p = adjoint_512()
+    p = adjoint_512(smooth_cutoff=smooth)
     sp = Spectral(p.nx, p.ny, p.xl, p.yl, p.nu)
     vk = band_limited_noise(sp, sp.dealiase_mask(), seed=0, amp=1e-3, kmax2=64)
     sp.close()
+    return p, vk
+
+
+def test_differentiation_workload():
+    """the differentiation workload as specified (512^2, Re 500, dt 1e-4,
+    1000 steps, checkpoints every 50, coefficients 0.05 N(0,1) of seed 3)
+    with the continuous mask (iSmoothCutoff = 1): every component of the
+    gradient against the central difference with h = 1e-6, within 1e-6"""
+    from fsibe_2d_amd.adjoint import Adjoint
+
+    p, vk = _workload(True)
+    c = _coef()
+    a = Adjoint(p, vk, 1000, 50)
+    J, g = a.gradient(c)
+    fd = _fd(a, c, 1e-6)
+    rel = np.abs(fd - g) / np.abs(fd)
+    print(f"[adjoint 512 x 1000] J {J:.10e} grad {g} rel {rel}", flush=True)
+    assert rel.max() <= 1e-6
+    a.close()
+
+
+def test_full_size_directional_derivative():
+    """the reference's cut-offs at the workload's grid and physics over 100
+    steps (before the beam has moved far enough for J to jump): the
+    gradient along a random direction d against the central difference of J
+    along d"""
+    from fsibe_2d_amd.adjoint import Adjoint
+    from fsibe_2d_amd.inputs import normals
+
+    p, vk = _workload(False)
     c = _coef()
     d = np.array(normals(4, 8))
     d /= np.linalg.norm(d)
@@ -125,6 +158,6 @@ def test_unsupported_configurations_are_refused():
     from fsibe_2d_amd import FsibeError, Params
     from fsibe_2d_amd.adjoint import Adjoint
 
-    for kw in (dict(iFLUSI=1), dict(iMotion=600), dict(iSharpTrailing=1)):
+    for kw in (dict(iFLUSI=1), dict(iMotion=600), dict(iSharpTrailing=1), dict(dt_fixed=0.0)):
         with pytest.raises(FsibeError):
             Adjoint(Params(nx=64, ny=64, ns=24, **({"iMotion": 601} | kw)), None, 10, 5)

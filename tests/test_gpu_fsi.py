@@ -14,7 +14,7 @@ PKEYS = ("nx", "ny", "ns", "iMeanVelocity", "iSponge", "iBeam", "iWalls", "iCyli
          "iFLUSI", "iMotion", "iViscous", "iSaveBeam", "idealis", "xl", "yl", "nu", "eps", "eps_sponge", "cfl",
          "dt_fixed", "Time_end", "N_smooth", "t_beam", "length", "SpongeSize", "R_cylinder", "h_channel", "x0",
          "y0", "AngleBeam", "theta_inf", "T_fullspeed", "T_release", "tau", "U_mean_true", "heave_amp",
-         "heave_freq", "motion_coef")
+         "heave_freq", "motion_coef", "iSmoothCutoff")
 
 
 def oracle_fsi(oracle_mod, p, vk, nsteps):

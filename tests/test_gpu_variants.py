@@ -2,7 +2,8 @@
 the device path implements (create_mask.f90 walls :34-62, cylinder :67-69,
 sharp trailing edge :97-99; create_sponge_mask types 1, 2, 10; mean flow
 ramps mean_velocity.f90 cases 1, 2; iViscous = 0; the sharp mask; flapping
-and heave + pitch kinematics; no beam at all; non-square grids), a bent beam
+and heave + pitch kinematics; the mask's cut-offs moved out (iSmoothCutoff);
+no beam at all; non-square grids), a bent beam
 moved by the host between steps (iFLUSI = 1, the per-step entry points of
 time_step.f90:187-199), the full-size 1024^2 and 2048^2 FSI steps, and the
 4096^2 solver-only case through the exact Taylor-Green decay."""
@@ -17,7 +18,7 @@ PKEYS = ("nx", "ny", "ns", "iMeanVelocity", "iSponge", "iBeam", "iWalls", "iCyli
          "iFLUSI", "iMotion", "iViscous", "iSaveBeam", "idealis", "xl", "yl", "nu", "eps", "eps_sponge", "cfl",
          "dt_fixed", "Time_end", "N_smooth", "t_beam", "length", "SpongeSize", "R_cylinder", "h_channel", "x0",
          "y0", "AngleBeam", "theta_inf", "T_fullspeed", "T_release", "tau", "U_mean_true", "heave_amp",
-         "heave_freq", "motion_coef")
+         "heave_freq", "motion_coef", "iSmoothCutoff")
 
 
 def _pair(oracle_mod, **kw):
@@ -71,6 +72,9 @@ VARIANTS = {
     "flapping": dict(iMotion=3, AngleBeam=10.0),
     "heave_pitch": dict(iMotion=601, heave_freq=2.0, motion_coef=(0.05, -0.02, 0.01, 0.03, 0.1, -0.04, 0.02, 0.05)),
     "soft_startup": dict(T_release=0.001, tau=0.002),
+    "smooth_cutoff": dict(iSmoothCutoff=1),
+    "smooth_cutoff_heave_pitch": dict(iSmoothCutoff=1, iMotion=601, heave_freq=2.0,
+                                      motion_coef=(0.05, -0.02, 0.01, 0.03, 0.1, -0.04, 0.02, 0.05)),
     "no_beam": dict(iBeam=0, iSponge=0, iSaveBeam=0, iMeanVelocity=0, xl=2 * np.pi, yl=2 * np.pi, nu=1e-3),
 }
 

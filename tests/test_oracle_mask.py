@@ -74,3 +74,32 @@ def test_sponge_type3(oracle_mod):
                      + smooth(4.0 - Y, th, h)) / es
     assert np.abs(ms - ref).max() < 1e-15 / es
     assert ms[0, 0] == 1.0 / es and ms[n // 2, n // 2] < 1e-12
+
+
+def test_smooth_cutoff_extends_the_reference_mask(oracle_mod):
+    """iSmoothCutoff = 1 draws segments and hinges out to t + 7 delta: on a
+    straight beam each column through the beam's interior is then the erf
+    profile of the distance to the centre line on every row, where the
+    reference's t* = t + 2 N dx cuts it at SS(t*) ~ 2e-7, and the mask never
+    decreases; Force's window then holds every pixel where the mask is not 0"""
+    n, ns, t, N, eps, A = 128, 32, 0.05, 2.0, 1e-3, 0.1
+    kw = dict(nx=n, ny=n, ns=ns, iBeam=1, iMotion=600, heave_amp=A, heave_freq=1.0, x0=1.0, y0=2.0, t_beam=t,
+              N_smooth=N, eps=eps, iSponge=0)
+    dx = 4.0 / n
+    ms = {}
+    for cut in (0, 1):
+        o = oracle_mod.Oracle(iSmoothCutoff=cut, **kw)
+        o.integrate_position(0.0)
+        o.create_mask(0.0)
+        ms[cut] = o.field("mask") * eps
+    ref, sm = ms[0], ms[1]
+    j0, i_mid = 64, int(round(1.5 / dx))
+    d = np.abs(np.arange(n) - j0) * dx
+    assert np.all(np.abs(sm[:, i_mid] - np.maximum(smooth(d, t, N * dx), 0.0)) < 1e-16)
+    t_star = t + 2 * N * dx
+    inner = d <= t_star
+    assert np.array_equal(ref[inner, i_mid], sm[inner, i_mid])
+    assert not ref[~inner, i_mid].any()
+    step = smooth(t_star, t, N * dx)
+    assert 1e-7 < step < 1e-6 and sm[~inner, i_mid].max() > 1e-9
+    assert np.all(sm >= ref) and (sm - ref).max() <= step
