@@ -134,6 +134,7 @@ static int snap_restore(fsibe_sim *s, SimSnap &q) {
   s->slot_fluid[0] = s->slot_fluid[1] = false;
   s->slot_forces[0] = s->slot_forces[1] = false;
   s->k1_ready = false;
+  s->k2_ready = false;
   s->upload_pending = false;
   return FSIBE_OK;
 }

@@ -726,6 +726,9 @@ struct fsibe_sim {
   // the next step's first column pass (it reads only vortk) already runs:
   // launched for time level k1_level before the host solves the beam
   bool k1_ready = false;
+  // K2's rows outside k2_b0 .. k2_b1 launched ahead with K1 (iFLUSI=1 loop)
+  bool k2_ready = false;
+  int k2_b0 = 0, k2_b1 = -1;
   int k1_level = -1;
   // what each slot holds: the fluid step's row sums, the force terms
   bool slot_fluid[2] = {false, false}, slot_forces[2] = {false, false};
@@ -739,6 +742,7 @@ struct fsibe_sim {
   bool have_beam = false, sponge_done = false;
   double time_mask = 0.0, alpha = 0.0, LE[6] = {0, 0, 0, 0, 0, 0};
   std::vector<double> bx, by, bvx, bvy, btheta, btheta_dot;
+  std::vector<double> seg_ang, seg_c, seg_s;  // mask geometry scratch, one per segment
   Box mbox, fbox;
   Window win;
   int nprim = 0, ntg = 0;
@@ -765,6 +769,16 @@ static void tape_mask(fsibe_sim *s);
 static int tape_fluid(fsibe_sim *s, bool explicit_step);
 
 // host phases timed with the steady clock into the otherwise unused classes
+// wait for an event by polling: the waiting thread stays on its core, so the
+// host work that follows (the beam solve, the next geometry) starts at once
+// and with warm caches, where a blocking wait may let the core sleep
+static hipError_t wait_event(hipEvent_t ev) {
+  for (;;) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e != hipErrorNotReady) return e;
+  }
+}
+
 static double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -1063,6 +1077,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   }
   s->bx.assign(nsb, 0.0); s->by.assign(nsb, 0.0); s->bvx.assign(nsb, 0.0); s->bvy.assign(nsb, 0.0);
   s->btheta.assign(nsb, 0.0); s->btheta_dot.assign(nsb, 0.0);
+  s->seg_ang.assign(nsb, 0.0); s->seg_c.assign(nsb, 0.0); s->seg_s.assign(nsb, 0.0);
   s->pbeam.assign(nsb, 0.0); s->tbeam.assign(nsb, 0.0);
   memset(&s->last, 0, sizeof(s->last));
   s->win = Window{0, -1, 0, -1};
@@ -1126,16 +1141,22 @@ static int build_mask_geometry(fsibe_sim *s) {
   if (s->p.iBeam > 0) {
     const double *X = s->bx.data(), *Y = s->by.data(), *VX = s->bvx.data(), *VY = s->bvy.data();
     const double t_star = t_cut;
+    // the segments' directions atan2(dy, dx), shared by the hinges (DrawHinge
+    // takes the same atan2 of the same differences, create_mask.f90:104-105)
+    double *seg_ang = s->seg_ang.data(), *seg_c = s->seg_c.data(), *seg_s = s->seg_s.data();
     for (int i = 0; i <= ns - 2; ++i) {  // DrawBeamSegment :740-761
       Prim P{};
       P.type = PRIM_SEG;
       const double p1x = X[i], p1y = Y[i], p2x = X[i + 1], p2y = Y[i + 1];
-      const double al = -atan2((p2y - p1y), (p2x - p1x));
+      seg_ang[i] = atan2((p2y - p1y), (p2x - p1x));
+      const double al = -seg_ang[i];
       P.ix0 = std::max(fnint((std::min(p1x, p2x) - t_star) / dx), 0);
       P.ix1 = std::min(fnint((std::max(p1x, p2x) + t_star) / dx), nx - 1);
       P.iy0 = std::max(fnint((std::min(p1y, p2y) - t_star) / dy), 0);
       P.iy1 = std::min(fnint((std::max(p1y, p2y) + t_star) / dy), ny - 1);
       const double ca = cos(al), sa = sin(al);
+      seg_c[i] = ca;
+      seg_s[i] = -sa;
       P.a[0] = ca * p1x - sa * p1y; P.a[1] = sa * p1x + ca * p1y;
       P.a[2] = ca * p2x - sa * p2y; P.a[3] = sa * p2x + ca * p2y;
       P.a[4] = ca; P.a[5] = sa; P.a[6] = t_star;
@@ -1164,8 +1185,7 @@ static int build_mask_geometry(fsibe_sim *s) {
     for (int i = 1; i <= ns - 2; ++i) {  // DrawHinge :579-592 via create_mask :103-107
       Prim P{};
       P.type = PRIM_HINGE;
-      const double a1 = atan2((Y[i] - Y[i - 1]), (X[i] - X[i - 1]));
-      const double a2 = atan2((Y[i + 1] - Y[i]), (X[i + 1] - X[i]));
+      const double a1 = seg_ang[i - 1], a2 = seg_ang[i];
       const double ts2 = t_cut;
       P.ix0 = std::max(fnint((X[i] - ts2) / dx), 0);
       P.ix1 = std::min(fnint((X[i] + ts2) / dx), nx - 1);
@@ -1175,7 +1195,8 @@ static int build_mask_geometry(fsibe_sim *s) {
       P.a[0] = X[i]; P.a[1] = Y[i];
       P.a[2] = c1 * 180.0 / s->pi; P.a[3] = c2 * 180.0 / s->pi;
       P.a[4] = ts2; P.a[5] = VX[i]; P.a[6] = VY[i];
-      P.a[8] = cos(a1); P.a[9] = sin(a1); P.a[10] = cos(a2); P.a[11] = sin(a2);
+      // unit directions of the two segments, for the sector pre-test only
+      P.a[8] = seg_c[i - 1]; P.a[9] = seg_s[i - 1]; P.a[10] = seg_c[i]; P.a[11] = seg_s[i];
       add_prim(s, P, np);
       mb.add(P.ix0, P.ix1, P.iy0, P.iy1);
     }
@@ -1484,6 +1505,35 @@ static int launch_k1(fsibe_sim *s, int level) {
   return FSIBE_OK;
 }
 
+// K2 of the next step on the rows that no mask can reach: outside the rows
+// of this step's mask box widened by kK2Margin rows on each side (the beam
+// moves far less than a cell per step; a mask box that leaves the band only
+// makes evolve_fluid recompute more rows).  These rows read K1's output, the
+// velocity and the sponge, none of which the beam solve changes, and skip
+// the penalisation term (empty mask box); the K3 scalars wait for the rest.
+constexpr int kK2Margin = 4;
+static int launch_k2_ahead(fsibe_sim *s) {
+  s->k2_ready = false;
+  if (s->mbox.empty()) return FSIBE_OK;
+  const int b0 = std::max(0, s->mbox.y0 - kK2Margin), b1 = std::min(s->ny - 1, s->mbox.y1 + kK2Margin);
+  const int na = b0, nb = s->ny - 1 - b1;
+  if (na + nb <= 0) return FSIBE_OK;
+  MaskBox none;
+  none.x0 = 0; none.x1 = -1; none.y0 = 0; none.y1 = -1; none.w = 0;
+  none.m = none.vx = none.vy = nullptr;
+  const size_t lds = sizeof(double2) * 3 * padlen(s->H);
+  CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+    KLAUNCH(k_rows_nl<decltype(F)>, na + nb, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
+            s->sponge_on ? s->ms : nullptr, none, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
+            s->umax_part, s->ny, s->dscal, s->fp, 0.0, 0.0, 0, res_dev(s, s->slot), 0, na, b1 + 1, 0);
+    return FSIBE_OK;
+  }));
+  s->k2_b0 = b0;
+  s->k2_b1 = b1;
+  s->k2_ready = true;
+  return FSIBE_OK;
+}
+
 // K1 .. K6 of a step: the new vorticity, velocity, pressure right-hand
 // sides and the row sums
 static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_step) {
@@ -1497,6 +1547,8 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     prof_end(s, P_VEL);
   }
   const bool sp10 = s->p.iSponge == 10 && s->penal_on && s->ms;
+  const bool k2_ahead = s->k2_ready && s->k1_ready && s->k1_level == n0;
+  s->k2_ready = false;
   if (s->k1_ready && s->k1_level == n0) {
     s->k1_ready = false;  // K1 of this step was launched ahead (run_one)
   } else {
@@ -1504,14 +1556,24 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     CHK(launch_k1(s, n0));
   }
   MaskBox mb = mask_box(s);
-  // K2 (block 0 also computes the K3 scalars from the previous u_max partials)
+  // K2 (block 0 also computes the K3 scalars from the previous u_max partials).
+  // When the rows outside a band around the previous mask box went ahead,
+  // the band and the rows of this mask box remain.
   {
+    int ra = 0, na = s->ny;
+    if (k2_ahead) {
+      int lo = s->k2_b0, hi = s->k2_b1;
+      if (mb.x1 >= mb.x0) { lo = std::min(lo, mb.y0); hi = std::max(hi, mb.y1); }
+      ra = lo;
+      na = hi - lo + 1;
+    }
     const size_t lds = sizeof(double2) * 3 * padlen(s->H);
     prof_begin(s, P_ROWS_NL);
     CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
-      KLAUNCH(k_rows_nl<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
+      KLAUNCH(k_rows_nl<decltype(F)>, na, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
               s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
-              s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot));
+              s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), ra, na,
+              0, 1);
       return FSIBE_OK;
     }));
     prof_end(s, P_ROWS_NL);
@@ -1809,7 +1871,8 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     CHK(launch_k1(s, s->n1));
     s->k1_ready = true;
     s->k1_level = s->n1;
-    HIPCHK(hipEventSynchronize(s->ev_step));
+    CHK(launch_k2_ahead(s));
+    HIPCHK(wait_event(s->ev_step));
     s->upload_pending = false;
     parse_results(s, s->slot);
     s->dt1 = s->last.dt1;
@@ -1819,7 +1882,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     prof_host(s, P_FORCES, th0);
     s->dt1 = s->last.dt1;
   } else {
-    HIPCHK(hipEventSynchronize(s->ev_dt));
+    HIPCHK(wait_event(s->ev_dt));
     s->upload_pending = false;  // the geometry upload precedes the nonlinear-term kernel
     s->dt1 = *reinterpret_cast<volatile const double *>(s->h_res + (size_t)s->slot * s->res_stride);
     if (s->res_pending) parse_results(s, s->slot ^ 1);  // finished before this step started
@@ -1880,6 +1943,7 @@ extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
     if (s->last.status == 1) break;  // continue_timestep = .false.
   }
   s->k1_ready = false;
+  s->k2_ready = false;
   s->pipe = false;
   if (s->res_pending) {
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -2485,7 +2549,7 @@ static int batch_step(fsibe_batch *b) {
     // the beams of step n+1 need the loads of step n: wait for every group,
     // then each member reads its results, and its beam is solved (or moved)
     for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-    for (auto &g : b->groups) HIPCHK(hipEventSynchronize(g.ev_dt));
+    for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
     b->pool->run((int)b->m.size(), [&](int i) {
       fsibe_sim *s = b->m[i];
       s->upload_pending = false;
@@ -2510,7 +2574,7 @@ static int batch_step(fsibe_batch *b) {
     return FSIBE_OK;
   }
   // dt1 of every member, then the previous step's results
-  for (auto &g : b->groups) HIPCHK(hipEventSynchronize(g.ev_dt));
+  for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
   const bool pending = b->res_pending;
   b->pool->run((int)b->m.size(), [&](int i) {
     fsibe_sim *s = b->m[i];

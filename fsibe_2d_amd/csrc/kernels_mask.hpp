@@ -59,7 +59,7 @@ constexpr int kMaskPix = kMaskTile * kMaskTile;  // pixels per tile = lanes of a
 constexpr int kMaskWaves = 4;
 constexpr int kMaskThreads = kMaskPix * kMaskWaves;
 constexpr int kMaxTilePrims = 1024;  // >= number of primitives (host checks)
-constexpr int kMaskStage = 32;       // primitives staged per pass
+constexpr int kMaskStage = 64;       // primitives staged per pass (one pass for a tile of the beam)
 
 // profile value of primitive P at (xi, yj), or -1 / -2 (see above)
 __device__ __forceinline__ double prim_value(const Prim &P, const MaskConsts &c, int i, int j, double xi,

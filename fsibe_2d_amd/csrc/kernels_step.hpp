@@ -303,12 +303,12 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
                                                     const double2 *__restrict__ twx, StepConsts c,
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
-                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res, int bid, int nblk) {
+                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res, int y,
+                                                    bool scal) {
   extern __shared__ double2 lds[];
   __shared__ double red[16];
-  if (bid == 0) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
+  if (scal) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
-  const int y = bid;
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const bool prow = penal_on && mb.x1 >= mb.x0 && y >= mb.y0 && y <= mb.y1;
   for (int k = tid; k < H; k += nt) {
@@ -811,9 +811,12 @@ __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ 
                                                     const double2 *__restrict__ twx, StepConsts c,
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
-                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res) {
+                                                    double dt0, int explicit_step, StepOutDev *__restrict__ res,
+                                                    int ra, int na, int rb, int scal) {
+  // rows ra .. ra+na-1, then rb onwards (two row ranges; one when rb is unused)
+  const int b = blockIdx.x;
   rows_nl_body(A, B, u1, u2, ms, mb, penal_on, Rc, Rpx, F, twx, c, umax_part, nparts, sc, fl, time, dt0,
-               explicit_step, res, blockIdx.x, gridDim.x);
+               explicit_step, res, b < na ? ra + b : rb + (b - na), scal && b == 0);
 }
 template <class FY>
 __global__ void __launch_bounds__(FY::LB) k_cols_ab2(const double2 *__restrict__ Rc, const double2 *__restrict__ Rpx,
@@ -933,7 +936,7 @@ __global__ void __launch_bounds__(FX::LB) kb_rows_nl(const RowsNlArgs *__restric
   const RowsNlArgs &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
   rows_nl_body(a.A, a.B, a.u1, a.u2, a.ms, a.mb, a.penal_on, a.Rc, a.Rpx, F, twx, a.c, a.umax_part, a.nparts, a.sc,
-               a.fl, a.time, a.dt0, a.explicit_step, a.res, blockIdx.x, a.nblk);
+               a.fl, a.time, a.dt0, a.explicit_step, a.res, (int)blockIdx.x, blockIdx.x == 0);
 }
 template <class FY>
 __global__ void __launch_bounds__(FY::LB) kb_cols_ab2(const ColsAb2Args *__restrict__ arr, FY F,

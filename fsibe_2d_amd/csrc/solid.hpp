@@ -727,14 +727,15 @@ struct Solver {
     const double *thg = x.data() + 1;
     // Newton (SolidSolver.f90:229-316).  The first iteration factors its
     // Jacobian.  Later iterations solve J_k dx = -F_k with that factorisation
-    // and one step of iterative refinement against the assembled J_k; the
-    // refinement's correction c measures ||J_0^-1 (J_k - J_0)||, and the
-    // refined step differs from the exact solve by about that factor times
-    // |c|.  Within one time step the Jacobian moves little (|c| ~ 1e-8 |dx|),
-    // so the Newton iterates are those of a fresh factorisation to rounding;
-    // if |c| > 1e-6 |dx| (and c is not below the rounding of x itself,
-    // 1e-14 max|x|, as in a converged last iteration) the iteration factors
-    // J_k instead.
+    // and one step of iterative refinement against the assembled J_k.  With
+    // rho = ||J_0^-1 (J_k - J_0)||, the correction c is about rho |dx| and
+    // the refined step differs from the exact solve by about rho^2 |dx|.
+    // Within one time step rho is 1e-8 .. 1e-5 (larger when the fluid loads
+    // move the iterate more), so a refined step is the exact Newton step to
+    // 1e-10 of its size, and the next iteration's Newton correction removes
+    // even that.  If |c| > 1e-4 |dx| (and c is not below the rounding of x
+    // itself, 1e-14 max|x|, as in a converged last iteration) the iteration
+    // factors J_k instead.
     iter = 0;
     refactored = 0;
     for (;;) {
@@ -761,7 +762,7 @@ struct Solver {
         }
         double xm = 0.0;
         for (int l = 0; l < nx; ++l) xm = std::max(xm, std::fabs(x[l]));
-        if (cm <= 1e-6 * dm || cm <= 1e-14 * xm) {
+        if (cm <= 1e-4 * dm || cm <= 1e-14 * xm) {
           for (int r = 0; r < nx; ++r) dx[r] = dx[r] + tmp[r];
         } else {
           fresh = true;
