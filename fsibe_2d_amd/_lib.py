@@ -88,7 +88,8 @@ EXPORTS = (
     "fsibe_solid_set_info", "fsibe_solid_step", "fsibe_solid_guess", "fsibe_solid_residual",
     "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid", "fsibe_solid_wrapper", "fsibe_gravity_impulse",
     "fsibe_adjoint_create", "fsibe_adjoint_destroy", "fsibe_adjoint_objective", "fsibe_adjoint_gradient",
-    "fsibe_adjoint_timings",
+    "fsibe_adjoint_timings", "fsibe_backup_write", "fsibe_backup_read", "fsibe_inicond_write",
+    "fsibe_inicond_read",
 )
 
 _lib = None
@@ -169,6 +170,11 @@ def lib():
     L.fsibe_adjoint_objective.argtypes = [vp, dp, C.POINTER(C.c_double)]
     L.fsibe_adjoint_gradient.argtypes = [vp, dp, C.POINTER(C.c_double), dp]
     L.fsibe_adjoint_timings.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.fsibe_backup_write.argtypes = [vp, C.c_char_p, C.c_int, C.c_double, C.c_int]
+    L.fsibe_backup_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int)]
+    L.fsibe_inicond_write.argtypes = [vp, C.c_char_p]
+    L.fsibe_inicond_read.argtypes = [vp, C.c_char_p]
     _lib = L
     return L
 

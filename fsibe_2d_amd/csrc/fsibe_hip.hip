@@ -2636,3 +2636,4 @@ extern "C" fsibe_sim *fsibe_batch_member(fsibe_batch *b, int i) {
 
 // ---- discrete adjoint (fsibe_adjoint_*) -------------------------------------
 #include "adjoint_host.hpp"
+#include "backup_host.hpp"

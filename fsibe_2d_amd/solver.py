@@ -15,6 +15,7 @@ nonzero status; there is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -144,6 +145,26 @@ class FluidSim:
         check(lib().fsibe_get_beam(self.h, *arrs))
         keys = ("x", "y", "vx", "vy", "theta", "pressure", "tau")
         return {k: a[: self.ns] for k, a in zip(keys, arrs)}
+
+    def backup_write(self, path, ivideo: int = 1, colorscale: float = 0.0, colorscale_done: bool = False):
+        """MakeRuntimeBackup (save_fields.f90:109-152) into `path`"""
+        check(lib().fsibe_backup_write(self.h, os.fsencode(path), int(ivideo), float(colorscale),
+                                       1 if colorscale_done else 0))
+
+    def backup_read(self, path) -> dict:
+        """init_fields with inicond = 2 from one backup file; returns the
+        host's counters stored with it"""
+        iv, cs, cd = C.c_int(), C.c_double(), C.c_int()
+        check(lib().fsibe_backup_read(self.h, os.fsencode(path), C.byref(iv), C.byref(cs), C.byref(cd)))
+        return {"ivideo": iv.value, "colorscale": cs.value, "colorscale_done": bool(cd.value)}
+
+    def inicond_write(self, path):
+        """the final vorticity file of time_step.f90:350-355"""
+        check(lib().fsibe_inicond_write(self.h, os.fsencode(path)))
+
+    def inicond_read(self, path):
+        """init_fields with inicond = 111 (before the first step)"""
+        check(lib().fsibe_inicond_read(self.h, os.fsencode(path)))
 
     @property
     def stream(self):

@@ -336,6 +336,32 @@ int fsibe_adjoint_gradient(fsibe_adjoint *adj, const double coef[8], double *J, 
  * includes its forward run; reverse_ms is the part after it) */
 int fsibe_adjoint_timings(fsibe_adjoint *adj, double *forward_ms, double *gradient_ms, double *reverse_ms);
 
+/* ---- restart files --------------------------------------------------------
+ * MakeRuntimeBackup (save_fields.f90:109-152): the runtime_backup<n>.in file
+ * of the time loop's state after the last step, as the reference writes it
+ * (Fortran unformatted sequential records, 4-byte markers, REAL 8 bytes,
+ * INTEGER/LOGICAL 4): time; 123.0; n1, dt1, vortk, nlk, workvis, mask,
+ * maskvx, maskvy, ivideo, u; ns; colorscale, colorscale_done; the beam's six
+ * records; vor_mean.  ivideo, colorscale, colorscale_done are the host's
+ * output counters, stored as given.  The spectral arrays omit the Nyquist
+ * modes the library does not hold (zero after dealiasing; nlk's are read
+ * by nothing).  Returns FSIBE_EARG if the file cannot be written. */
+int fsibe_backup_write(fsibe_sim *sim, const char *path, int ivideo, double colorscale, int colorscale_done);
+/* init_fields with inicond = 2 (init_fields.f90:44-97) for one file: the
+ * state is restored so that the next fsibe_run / step continues the time
+ * loop (AB2, no startup step; create_mask runs again from the stored beam).
+ * The beam's StartupStep is not in the file and is set, so a BDF2 solid
+ * restarts with a Crank-Nicolson step, as in the reference.  FSIBE_EDOMAIN
+ * on a wrong checksum, ns or record size.  Choosing the newer of the two
+ * backup files (init_fields.f90:47-52) is the host's part. */
+int fsibe_backup_read(fsibe_sim *sim, const char *path, int *ivideo, double *colorscale, int *colorscale_done);
+/* the final file of time_step.f90:350-355: nx, ny and cofitxy(vortk) of
+ * Fortran level 0 (the whole vortk array passed as a 2-D argument) */
+int fsibe_inicond_write(fsibe_sim *sim, const char *path);
+/* init_fields with inicond = 111 (init_fields.f90:109-128): vortk(:,:,0) =
+ * coftxy of the file's vorticity; only before the first step */
+int fsibe_inicond_read(fsibe_sim *sim, const char *path);
+
 /* HIP stream of the simulation (hipStream_t) for event timing */
 void *fsibe_sim_stream(fsibe_sim *sim);
 
