@@ -311,6 +311,37 @@ module fsibe_hip_api
       real(c_double), intent(out) :: J
       integer(c_int) :: rc
     end function
+    ! restart files (MakeRuntimeBackup, init_fields inicond = 2 and 111)
+    function fsibe_backup_write(sim, path, ivideo, colorscale, colorscale_done) &
+        bind(C, name="fsibe_backup_write") result(rc)
+      import :: c_int, c_double, c_ptr, c_char
+      type(c_ptr), value :: sim
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: ivideo, colorscale_done
+      real(c_double), value :: colorscale
+      integer(c_int) :: rc
+    end function
+    function fsibe_backup_read(sim, path, ivideo, colorscale, colorscale_done) &
+        bind(C, name="fsibe_backup_read") result(rc)
+      import :: c_int, c_double, c_ptr, c_char
+      type(c_ptr), value :: sim
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: ivideo, colorscale_done
+      real(c_double), intent(out) :: colorscale
+      integer(c_int) :: rc
+    end function
+    function fsibe_inicond_write(sim, path) bind(C, name="fsibe_inicond_write") result(rc)
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: sim
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function
+    function fsibe_inicond_read(sim, path) bind(C, name="fsibe_inicond_read") result(rc)
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: sim
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function
     function fsibe_adjoint_gradient(adj, coef, J, grad) bind(C, name="fsibe_adjoint_gradient") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: adj
