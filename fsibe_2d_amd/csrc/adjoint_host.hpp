@@ -177,8 +177,9 @@ extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, 
   *out = nullptr;
   if (p->iFLUSI != 0 || p->iBeam != 1 || p->iMotion != 601)
     return set_err(FSIBE_EUNSUPPORTED, "adjoint: needs prescribed motion 601 of one beam (iFLUSI=0, iBeam=1)");
-  if (p->iSharpTrailing != 0 || p->iSponge == 10)
-    return set_err(FSIBE_EUNSUPPORTED, "adjoint: sharp trailing edge and iSponge=10 are not differentiated");
+  if (p->iSharpTrailing != 0 || p->iSponge == 10 || p->iSponge == 4 || p->iSponge == 22 || p->iMeanVelocity == 11)
+    return set_err(FSIBE_EUNSUPPORTED,
+                   "adjoint: sharp trailing edge, iSponge 4/10/22 and iMeanVelocity 11 are not differentiated");
   if (!(p->dt_fixed > 0.0))
     return set_err(FSIBE_EUNSUPPORTED, "adjoint: needs a fixed time step (dt_fixed > 0); dt1 is not differentiated");
   fsibe_adjoint *a = new fsibe_adjoint();

@@ -704,6 +704,8 @@ struct fsibe_sim {
   double2 *twx, *twy;
   double2 *W[2], *NL[2], *T[7];
   double *u1, *u2, *vor_mean, *ms, *wallrow;
+  double *frame = nullptr;   // iSponge = 22: the frame mask (SmoothFrameAroundDomain), full grid
+  double *rowpen = nullptr;  // iMeanVelocity = 11: row sums of the penalisation term (2 per row)
   double *umax_part;
   DevScalars *dscal;
   double *mask_m, *mask_vx, *mask_vy;
@@ -853,7 +855,71 @@ static double smooth_step_host(const fsibe_sim *s, double x, double t, double h)
 
 static inline int fnint(double x) { return (int)lround(x); }
 
-// create_sponge_mask, create_mask.f90:223-364 (types 0,1,2,3,10)
+// SmoothFrameAroundDomain, create_mask.f90:370-463: a level set phi of the
+// distance to the domain's edges, with corners rounded to radius
+// 2 thickness (thickness = 4 t_beam), and the cosine step SmoothStep2
+// (:523-544) of phi over thickness/2; phi stays 1000 where no loop sets it
+static void frame_host(const fsibe_sim *s, std::vector<double> &fr) {
+  const int nx = s->nx, ny = s->ny;
+  const double dx = s->dx, dy = s->dy, xl = s->p.xl, yl = s->p.yl, pi = s->pi;
+  std::vector<double> phi((size_t)nx * ny, 1000.0);
+  const double thickness = 4.0 * s->p.t_beam, R1 = 0.0, R2 = 2.0 * thickness, RR = R1 + R2;
+  const int ixmin = fnint(RR / dx), ixmax = nx - fnint(RR / dx);
+  const int iymin = fnint(RR / dy), iymax = ny - fnint(RR / dy);
+  auto P = [&](int ix, int iy) -> double & { return phi[(size_t)iy * nx + ix]; };
+  for (int ix = std::max(ixmin, 0); ix <= std::min(ixmax, nx - 1); ++ix)
+    for (int iy = 0; iy < ny; ++iy) {
+      const double y = (double)iy * dy;
+      const double m = std::min(P(ix, iy), std::min(fabs(y - R1), fabs(y - (yl - R1))));
+      P(ix, iy) = (iy <= fnint(R1 / dy) || iy >= ny - 1 - fnint(R1 / dy)) ? -m : m;
+    }
+  for (int ix = 0; ix < nx; ++ix)
+    for (int iy = std::max(iymin, 0); iy <= std::min(iymax, ny - 1); ++iy) {
+      const double x = (double)ix * dx;
+      const double m = std::min(P(ix, iy), std::min(fabs(x - R1), fabs(x - (xl - R1))));
+      P(ix, iy) = (ix <= fnint(R1 / dx) || ix >= nx - 1 - fnint(R1 / dx)) ? -m : m;
+    }
+  const double cxs[4] = {RR, RR, xl - RR, xl - RR}, cys[4] = {RR, yl - RR, yl - RR, RR};
+  const int xa[4] = {0, 0, ixmax, ixmax}, xb[4] = {ixmin, ixmin, nx - 1, nx - 1};
+  const int ya[4] = {0, iymax, iymax, 0}, yb[4] = {iymin, ny - 1, ny - 1, iymin};
+  for (int c = 0; c < 4; ++c)
+    for (int ix = std::max(xa[c], 0); ix <= std::min(xb[c], nx - 1); ++ix)
+      for (int iy = std::max(ya[c], 0); iy <= std::min(yb[c], ny - 1); ++iy) {
+        const double a = (double)ix * dx - cxs[c], b = (double)iy * dy - cys[c];
+        P(ix, iy) = std::min(P(ix, iy), -(sqrt(a * a + b * b) - R2));
+      }
+  const double t = thickness / 2., h = thickness / 2.;
+  fr.resize(phi.size());
+  for (size_t k = 0; k < phi.size(); ++k) {
+    const double x = phi[k];
+    if (x <= t - h) fr[k] = 1.0;
+    else if ((t - h) < x && x < (t + h)) fr[k] = 0.5 * (1. + cos((x - t + h) * pi / (2.0 * h)));
+    else fr[k] = 0.0;
+  }
+}
+
+// Mean_ux, Mean_uy (mean_velocity.f90) at `time` on the host; case 11
+// (u_mean) is the device's state and reads as 0 here
+static void mean_u_host(const FlowParams &f, double time, double &mx, double &my) {
+  double tf = 1.0;
+  switch (f.iMeanVelocity) {
+    case 1:
+      if (time < f.T_fullspeed) {
+        const double t1 = f.T_fullspeed;
+        tf = (time * time * time) / (-0.5 * t1 * t1 * t1) + 3 * (time * time) / (t1 * t1);
+      }
+      mx = cos(f.theta_inf) * tf; my = sin(f.theta_inf) * tf; break;
+    case 2:
+      if (time < f.T_fullspeed) tf = 0.5 * (1.0 - cos(0.5 * f.pi * time / f.T_fullspeed));
+      mx = cos(f.theta_inf) * tf; my = sin(f.theta_inf) * tf; break;
+    case 3: mx = cos(f.theta_inf); my = sin(f.theta_inf); break;
+    case 6: mx = time; my = 0.0; break;
+    case 22: mx = 0.1; my = 0.0; break;
+    default: mx = 0.0; my = 0.0; break;
+  }
+}
+
+// create_sponge_mask, create_mask.f90:223-364 (types 0,1,2,3,10,22)
 static void sponge_host(const fsibe_sim *s, const double *mask_full, std::vector<double> &ms) {
   const int nx = s->nx, ny = s->ny;
   const double dx = s->dx, dy = s->dy, N = s->p.N_smooth, xl = s->p.xl, yl = s->p.yl;
@@ -872,6 +938,17 @@ static void sponge_host(const fsibe_sim *s, const double *mask_full, std::vector
       const double tmp = smooth_step_host(s, (double)ix * dx - xl + 0.5 * sss - 1.0 * dx, ssz * 0.4, N * dx);
       for (int iy = 0; iy < ny; ++iy) ms[(size_t)iy * nx + ix] = tmp;
     }
+  } else if (isp == 22) {
+    // :333-349: 1 outside the inner rectangle, 0 inside and wherever the
+    // startup mask (frame included) is positive
+    const int ixmin = fnint(s->p.SpongeSize / dx), iymin = fnint(s->p.SpongeSize / dy);
+    const int ixmax = fnint((xl - s->p.SpongeSize) / dx), iymax = fnint((yl - s->p.SpongeSize) / dy);
+    for (auto &v : ms) v = 1.0;
+    for (int iy = std::max(iymin, 0); iy <= std::min(iymax, ny - 1); ++iy)
+      for (int ix = std::max(ixmin, 0); ix <= std::min(ixmax, nx - 1); ++ix) ms[(size_t)iy * nx + ix] = 0.0;
+    if (mask_full)
+      for (size_t k = 0; k < ms.size(); ++k)
+        if (mask_full[k] > 0.0) ms[k] = 0.0;
   } else if (isp == 10) {
     const int ixmin = fnint((xl - s->p.SpongeSize) / dx);
     for (int ix = std::max(ixmin, 0); ix < nx; ++ix)
@@ -919,7 +996,7 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
                   s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow,
                   s->umax_part, s->dscal, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
-                  s->fb_sb, s->d_stage};
+                  s->fb_sb, s->d_stage, s->frame, s->rowpen};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -943,16 +1020,14 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   const fsibe_params &p = *pp;
   if (!size_ok(p.nx, p.ny))
     return set_err(FSIBE_ESIZE, "grid %dx%d not supported (even sizes, nx/2 and ny = 2^a 3^b)", p.nx, p.ny);
-  if (p.iSponge == 4 || p.iSponge == 22)
-    return set_err(FSIBE_EUNSUPPORTED, "iSponge=%d (velocity sponge) is not implemented", p.iSponge);
-  if (p.iMeanVelocity == 11)
-    return set_err(FSIBE_EUNSUPPORTED, "iMeanVelocity=11 (zero mean pressure forcing) is not implemented");
+  if (!(p.iSponge >= 0 && p.iSponge <= 4) && p.iSponge != 10 && p.iSponge != 22)
+    return set_err(FSIBE_EUNSUPPORTED, "iSponge=%d is not a sponge type of create_mask.f90", p.iSponge);
   if (p.iBeam > 1) return set_err(FSIBE_EUNSUPPORTED, "iBeam=%d: one beam is supported", p.iBeam);
   if (p.iBeam == 1 && p.ns < 3) return set_err(FSIBE_EARG, "ns must be >= 3");
   if (p.iBeam == 1 && 2 * p.ns + 8 > kMaxTilePrims)
     return set_err(FSIBE_EUNSUPPORTED, "ns=%d: at most %d beam nodes", p.ns, (kMaxTilePrims - 8) / 2);
   if (!(p.eps > 0.0) || !(p.xl > 0.0) || !(p.yl > 0.0)) return set_err(FSIBE_EARG, "eps, xl, yl must be > 0");
-  if (((p.iSponge > 0 && p.iSponge < 4) || p.iSponge == 10) && !(p.eps_sponge > 0.0))
+  if (((p.iSponge > 0 && p.iSponge < 4) || p.iSponge == 10 || p.iSponge == 22) && !(p.eps_sponge > 0.0))
     return set_err(FSIBE_EARG, "vorticity sponge needs eps_sponge > 0 (params.f90:185-188)");
   fsibe_sim *s = new fsibe_sim();
   s->p = p;
@@ -966,8 +1041,10 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   s->gerf = fabs((exp(-(2.0 * 1.0) * (2.0 * 1.0)) - 1.0) / sqrt(s->pi));
   s->sharp = fabs(p.N_smooth) < 1e-4;
   if (s->p.U_mean_true == 0.0) s->p.U_mean_true = 1.0;
-  s->penal_on = p.iBeam > 0 || p.iWalls > 0 || p.iCylinder > 0;
-  s->sponge_on = p.iSponge > 0 && p.iSponge < 4;
+  // cal_nlk's penalisation term (NavierStokes.f90:88) and vorticity sponge
+  // (:142); the time-step limit 0.9 eps leaves iSponge = 22 out (:71)
+  s->penal_on = p.iBeam > 0 || p.iWalls > 0 || p.iCylinder > 0 || p.iSponge == 22;
+  s->sponge_on = (p.iSponge > 0 && p.iSponge < 4) || p.iSponge == 22;
   s->pressure_on = p.iFLUSI == 1 || p.iSaveBeam > 0;
   s->forces_on = s->pressure_on && p.iBeam > 0;
   s->visc = p.iViscous == 1;
@@ -985,7 +1062,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   c.idealis = p.idealis == 1;
   c.nu = p.nu; c.eps = p.eps; c.dx = s->dx; c.dy = s->dy;
   FlowParams &f = s->fp;
-  f.iMeanVelocity = p.iMeanVelocity; f.penal_on = s->penal_on;
+  f.iMeanVelocity = p.iMeanVelocity; f.penal_on = p.iBeam > 0 || p.iWalls > 0 || p.iCylinder > 0;
   f.theta_inf = p.theta_inf; f.T_fullspeed = p.T_fullspeed; f.U_mean_true = s->p.U_mean_true;
   f.cfl = p.cfl; f.eps = p.eps; f.dt_fixed = p.dt_fixed; f.Time_end = p.Time_end; f.pi = s->pi;
   f.dx = s->dx; f.dy = s->dy; f.nu = p.nu; f.scalex = s->scalex; f.scaley = s->scaley;
@@ -1075,6 +1152,16 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     SIMCHK(hipMalloc(&s->wallrow, sizeof(double) * s->ny));
     SIMCHK(hipMemcpy(s->wallrow, wr.data(), sizeof(double) * s->ny, hipMemcpyHostToDevice));
   }
+  if (p.iSponge == 22) {
+    std::vector<double> fr;
+    frame_host(s, fr);
+    SIMCHK(hipMalloc(&s->frame, sizeof(double) * fr.size()));
+    SIMCHK(hipMemcpy(s->frame, fr.data(), sizeof(double) * fr.size(), hipMemcpyHostToDevice));
+  }
+  if (p.iMeanVelocity == 11) {
+    SIMCHK(hipMalloc(&s->rowpen, sizeof(double) * 2 * s->ny));
+    SIMCHK(hipMemset(s->rowpen, 0, sizeof(double) * 2 * s->ny));
+  }
   s->bx.assign(nsb, 0.0); s->by.assign(nsb, 0.0); s->bvx.assign(nsb, 0.0); s->bvy.assign(nsb, 0.0);
   s->btheta.assign(nsb, 0.0); s->btheta_dot.assign(nsb, 0.0);
   s->seg_ang.assign(nsb, 0.0); s->seg_c.assign(nsb, 0.0); s->seg_s.assign(nsb, 0.0);
@@ -1121,6 +1208,10 @@ static int build_mask_geometry(fsibe_sim *s) {
     mb.add(0, nx - 1, 0, std::min(ny - 1, fnint(h_star / dy)));
     mb.add(0, nx - 1, std::max(0, fnint((s->p.yl - h_star) / dy)), ny - 1);
   }
+  // the velocity sponge's columns (create_mask.f90:137-180) and the frame
+  // (:187-202) on every row
+  if (s->p.iSponge == 4) mb.add(0, std::min(nx - 1, fnint((1.0 * s->p.SpongeSize) / dx)), 0, ny - 1);
+  if (s->p.iSponge == 22) mb.add(0, nx - 1, 0, ny - 1);
   const double delta = N * mdx * s->gerf;
   // segments and hinges: create_mask's t* = t + 2 N max(dx,dy), or with
   // iSmoothCutoff = 1 the end dots' reach t + 7 delta (smooth step < 1e-22)
@@ -1359,7 +1450,7 @@ static int ensure_sponge(fsibe_sim *s) {
   s->sponge_done = true;
   if (!s->sponge_on && s->p.iSponge != 10) return FSIBE_OK;
   std::vector<double> mfull;
-  if (s->p.iSponge == 2) {
+  if (s->p.iSponge == 2 || s->p.iSponge == 22) {
     mfull.assign((size_t)s->nx * s->ny, 0.0);
     if (!s->mbox.empty()) {
       std::vector<double> mb((size_t)s->mbox.w() * s->mbox.h());
@@ -1427,6 +1518,28 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
     mc.dx = s->dx; mc.dy = s->dy; mc.t = s->p.t_beam; mc.h = s->p.N_smooth * std::max(s->dx, s->dy);
     mc.gerf = s->gerf; mc.inv_eps_unused = 0.0; mc.ds = s->ds; mc.pi = s->pi; mc.sharp = s->sharp;
     mc.eps = s->p.eps;
+    if (s->p.iSponge == 4 || s->p.iSponge == 22) {
+      mean_u_host(s->fp, time, mc.mux, mc.muy);
+      if (s->p.iMeanVelocity == 11)  // device address of DevScalars::u_mean
+        mc.mean_dev = reinterpret_cast<const double *>(reinterpret_cast<const char *>(s->dscal) +
+                                                        offsetof(DevScalars, u_mean));
+      mc.N = s->p.N_smooth;
+    }
+    if (s->p.iSponge == 4) {
+      const double S = s->p.SpongeSize;
+      mc.vs4 = 1;
+      mc.vs_S = S;
+      mc.vs_ssz = S - 2.0 * s->p.N_smooth * s->dx;
+      mc.vs_Heff = s->p.yl - 2.0 * s->p.h_channel;
+      mc.h_channel = s->p.h_channel;
+      mc.vs_ixmid = fnint((0.5 * S) / s->dx);
+      mc.vs_ixmax = fnint((1.0 * S) / s->dx);
+    }
+    if (s->p.iSponge == 22) {
+      mc.frame = s->frame;
+      mc.nx_frame = s->nx;
+      mc.frame_M = std::max(1.0e-3, s->p.eps);
+    }
     dim3 blk(kMaskThreads);
     dim3 grd((s->mbox.w() + kMaskTile - 1) / kMaskTile, (s->mbox.h() + kMaskTile - 1) / kMaskTile);
     prof_begin(s, P_MASK);
@@ -1462,24 +1575,9 @@ static int velocity_only(fsibe_sim *s, double time) {
   const size_t lds = sizeof(double2) * 3 * padlen(s->H);
   double mx, my;
   {
-    // Mean_ux/Mean_uy at `time` (mean_velocity.f90), host copy of mean_u
-    const FlowParams &f = s->fp;
-    double tf = 1.0;
-    switch (f.iMeanVelocity) {
-      case 1:
-        if (time < f.T_fullspeed) {
-          const double t1 = f.T_fullspeed;
-          tf = (time * time * time) / (-0.5 * t1 * t1 * t1) + 3 * (time * time) / (t1 * t1);
-        }
-        mx = cos(f.theta_inf) * tf; my = sin(f.theta_inf) * tf; break;
-      case 2:
-        if (time < f.T_fullspeed) tf = 0.5 * (1.0 - cos(0.5 * f.pi * time / f.T_fullspeed));
-        mx = cos(f.theta_inf) * tf; my = sin(f.theta_inf) * tf; break;
-      case 3: mx = cos(f.theta_inf); my = sin(f.theta_inf); break;
-      case 6: mx = time; my = 0.0; break;
-      case 22: mx = 0.1; my = 0.0; break;
-      default: mx = 0.0; my = 0.0; break;
-    }
+    // Mean_ux/Mean_uy at `time`; only the startup step comes here, when
+    // case 11's u_mean is still 0
+    mean_u_host(s->fp, time, mx, my);
   }
   const double UMx = s->p.U_mean_true * mx, UMy = s->p.U_mean_true * my;
   MaskBox mb = mask_box(s);
@@ -1525,7 +1623,8 @@ static int launch_k2_ahead(fsibe_sim *s) {
   CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
     KLAUNCH(k_rows_nl<decltype(F)>, na + nb, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
             s->sponge_on ? s->ms : nullptr, none, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
-            s->umax_part, s->ny, s->dscal, s->fp, 0.0, 0.0, 0, res_dev(s, s->slot), 0, na, b1 + 1, 0);
+            s->umax_part, s->ny, s->dscal, s->fp, 0.0, 0.0, 0, res_dev(s, s->slot), 0, na, b1 + 1, 0,
+            s->penal_on ? s->rowpen : nullptr);
     return FSIBE_OK;
   }));
   s->k2_b0 = b0;
@@ -1573,13 +1672,19 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
       KLAUNCH(k_rows_nl<decltype(F)>, na, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
               s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
               s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), ra, na,
-              0, 1);
+              0, 1, s->penal_on ? s->rowpen : nullptr);
       return FSIBE_OK;
     }));
     prof_end(s, P_ROWS_NL);
     if (s->pipe) HIPCHK(hipEventRecord(s->ev_dt, s->stream));
   }
   if (s->tape) CHK(tape_fluid(s, explicit_step));
+  if (s->p.iMeanVelocity == 11) {
+    k_mean11<<<1, 256, 0, s->stream>>>(s->penal_on ? s->rowpen : nullptr, s->ny,
+                                       1.0 / ((double)s->nx * (double)s->ny), s->dscal, n0, n1, (int)explicit_step,
+                                       s->p.U_mean_true);
+    HIPCHK(hipGetLastError());
+  }
   if (sp10) {
     // NavierStokes.f90:108-109: ixmin = nint((xl - SpongeSize)/dx), ixmax = nx - 1 - 15
     const int ixmin = (int)lround((s->p.xl - s->p.SpongeSize) / s->dx), ixmax = s->nx - 1 - 15;
@@ -2226,7 +2331,9 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
   *out = nullptr;
   for (int i = 0; i < n; ++i) {
     const fsibe_params &p = params[i];
-    if (p.iSponge == 10) return set_err(FSIBE_EUNSUPPORTED, "batch member %d: iSponge=10", i);
+    if (p.iSponge == 10 || p.iSponge == 4 || p.iSponge == 22 || p.iMeanVelocity == 11)
+      return set_err(FSIBE_EUNSUPPORTED, "batch member %d: iSponge=%d / iMeanVelocity=%d is not batched", i,
+                     p.iSponge, p.iMeanVelocity);
   }
   fsibe_batch *b = new fsibe_batch();
   auto fail = [&](int rc) { fsibe_batch_destroy(b); return rc; };

@@ -25,6 +25,17 @@ struct MaskConsts {
   double dx, dy, t, h, gerf, inv_eps_unused, ds, pi;
   int sharp;
   double eps;
+  // velocity sponge of a channel inflow (iSponge = 4, create_mask.f90:130-184)
+  int vs4 = 0, vs_ixmid = 0, vs_ixmax = 0;
+  double vs_S = 0.0, vs_ssz = 0.0, vs_Heff = 0.0, h_channel = 0.0, N = 0.0;
+  // frame around the domain (iSponge = 22, :187-202): mask += frame w
+  const double *frame = nullptr;
+  int nx_frame = 0;
+  double frame_M = 1.0;  // max(1e-3, eps)
+  // Mean_u(time) of the mask's time; mean_dev (2 doubles on the device)
+  // instead when iMeanVelocity = 11 (u_mean lives on the device)
+  double mux = 0.0, muy = 0.0;
+  const double *mean_dev = nullptr;
 };
 
 // SmoothStep, create_mask.f90:475-520 (gerf = |(exp(-4)-1)/sqrt(pi)|)
@@ -230,6 +241,33 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
     if (pn >= np) break;
   }
   if (wv != 0 || !valid) return;
+  if (c.vs4 || c.frame) {
+    const double mux = c.mean_dev ? c.mean_dev[0] : c.mux, muy = c.mean_dev ? c.mean_dev[1] : c.muy;
+    if (c.vs4) {
+      // left half (x = 0.75 S - i dx, step of half width S/4), then the right
+      // half (x = i dx - 0.5 S, the smoothing layer); the middle column gets
+      // both; maskvx is the channel's parabola wherever the sponge is drawn
+      const double y_chan = (double)j * c.dy - c.h_channel;
+      const double prof = (y_chan > 0.0 && y_chan < c.vs_Heff)
+                              ? 1.5 * mux * y_chan * (c.vs_Heff - y_chan) / ((0.5 * c.vs_Heff) * (0.5 * c.vs_Heff))
+                              : 0.0;
+      if (i >= 0 && i <= c.vs_ixmid) {
+        const double tmp = smooth_step(c, 0.75 * c.vs_S - (double)i * c.dx, c.vs_ssz * 0.5, 0.25 * c.vs_ssz);
+        if (m < tmp) m = tmp;
+        mvx = prof;
+      }
+      if (i >= c.vs_ixmid && i <= c.vs_ixmax) {
+        const double tmp = smooth_step(c, (double)i * c.dx - 0.5 * c.vs_S, c.vs_ssz * 0.5, c.N * c.dx);
+        if (m < tmp) m = tmp;
+        mvx = prof;
+      }
+    }
+    if (c.frame) {
+      const double fr = c.frame[(size_t)j * (size_t)c.nx_frame + i];
+      m = m + (fr * c.eps) / c.frame_M;
+      if (fr > 0.0) { mvx = mux; mvy = muy; }
+    }
+  }
   const size_t q = (size_t)ly * bw + lx;
   om[q] = m / c.eps;  // :210-214
   ovx[q] = mvx;

@@ -52,6 +52,11 @@ struct DevScalars {
   double UMx, UMy;             // U_mean_true * Mean_u(time+dt1)
   double Mx, My;               // Mean_u(time+dt1)
   double time, dt0;
+  // iMeanVelocity = 11: the mean velocity u_mean and the mean penalisation
+  // force fmean of the two time levels (share_vars.f90:23-24), carried
+  // from step to step (the step kernels leave them alone)
+  double u_mean[2];
+  double fmean[2][2];
 };
 
 // per-step results; lives in pinned host memory that the kernels write
@@ -71,7 +76,8 @@ struct FlowParams {
       scalex, scaley;
 };
 
-// mean_velocity.f90:4-84 (case 11 is rejected on the host)
+// mean_velocity.f90:4-84; case 11 (u_mean, zero mean pressure) returns 0
+// here and k_mean11 sets the step's mean velocity once u_mean is advanced
 __device__ __forceinline__ void mean_u(const FlowParams &f, double time, double &ux, double &uy) {
 #pragma clang fp contract(off)
   double tf = 1.0;
@@ -304,13 +310,14 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
                                                     double dt0, int explicit_step, StepOutDev *__restrict__ res, int y,
-                                                    bool scal) {
+                                                    bool scal, double *__restrict__ rowpen) {
   extern __shared__ double2 lds[];
   __shared__ double red[16];
   if (scal) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const bool prow = penal_on && mb.x1 >= mb.x0 && y >= mb.y0 && y <= mb.y1;
+  double ps1 = 0.0, ps2 = 0.0;  // the row's sums of the penalisation term (rowpen)
   for (int k = tid; k < H; k += nt) {
     const double2 a = A[(size_t)y * H + k];
     const double kf = (double)k * c.sx;
@@ -345,10 +352,25 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
         px.y = -(mb.m[qq] * (a1.y - mb.vx[qq]));
         py.y = -(mb.m[qq] * (a2.y - mb.vy[qq]));
       }
+      ps1 += px.x + px.y;
+      ps2 += py.x + py.y;
     }
     f0[q] = r;
     f1[q] = px;
     f2[q] = py;
+  }
+  if (rowpen) {  // f_mean of cal_nlk, NavierStokes.f90:97-100: per-row sums
+    __shared__ double rp[32];
+    ps1 = wave_sum(ps1);
+    ps2 = wave_sum(ps2);
+    if ((tid & 63) == 0) { rp[2 * (tid >> 6)] = ps1; rp[2 * (tid >> 6) + 1] = ps2; }
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0.0, b = 0.0;
+      for (int w = 0; w < nt / 64; ++w) { a += rp[2 * w]; b += rp[2 * w + 1]; }
+      rowpen[2 * y] = a;
+      rowpen[2 * y + 1] = b;
+    }
   }
   __syncthreads();
   F.template fft<false, 3>(lds, twx, tid, nt);
@@ -812,11 +834,49 @@ __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ 
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
                                                     double dt0, int explicit_step, StepOutDev *__restrict__ res,
-                                                    int ra, int na, int rb, int scal) {
+                                                    int ra, int na, int rb, int scal, double *__restrict__ rowpen) {
   // rows ra .. ra+na-1, then rb onwards (two row ranges; one when rb is unused)
   const int b = blockIdx.x;
   rows_nl_body(A, B, u1, u2, ms, mb, penal_on, Rc, Rpx, F, twx, c, umax_part, nparts, sc, fl, time, dt0,
-               explicit_step, res, b < na ? ra + b : rb + (b - na), scal && b == 0);
+               explicit_step, res, b < na ? ra + b : rb + (b - na), scal && b == 0, rowpen);
+}
+
+// iMeanVelocity = 11 (zero mean pressure): f_mean = sum(penal) / (nx ny)
+// from the row sums (cal_nlk, NavierStokes.f90:97-100) into fmean(:,n0),
+// then u_mean advanced as EvolveFluidExplicit (:453-456) or EvolveFluidAB2
+// (:501-503) do, and the step's mean velocity Mean_u(time+dt1) = u_mean set
+// for the velocity kernel.  One block of 256 threads.
+__global__ void __launch_bounds__(256) k_mean11(const double *__restrict__ rowpen, int ny, double inv_n,
+                                                DevScalars *__restrict__ s, int n0, int n1, int explicit_step,
+                                                double U_mean_true) {
+#pragma clang fp contract(off)
+  __shared__ double rp[8];
+  double a = 0.0, b = 0.0;
+  if (rowpen)
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) { a += rowpen[2 * y]; b += rowpen[2 * y + 1]; }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) { rp[2 * (threadIdx.x >> 6)] = a; rp[2 * (threadIdx.x >> 6) + 1] = b; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (rowpen) {  // penalisation off: cal_nlk leaves f_mean as it was
+    s->fmean[n0][0] = ((rp[0] + rp[2]) + (rp[4] + rp[6])) * inv_n;
+    s->fmean[n0][1] = ((rp[1] + rp[3]) + (rp[5] + rp[7])) * inv_n;
+  }
+  double u0 = s->u_mean[0], u1 = s->u_mean[1];
+  if (explicit_step) {
+    u0 = u0 + s->fmean[n0][0] * s->dt1;
+    u1 = u1 + s->fmean[n0][1] * s->dt1;
+  } else {
+    u0 = u0 + s->b10 * s->fmean[n0][0] + s->b11 * s->fmean[n1][0];
+    u1 = u1 + s->b10 * s->fmean[n0][1] + s->b11 * s->fmean[n1][1];
+  }
+  s->u_mean[0] = u0;
+  s->u_mean[1] = u1;
+  s->Mx = u0;
+  s->My = u1;
+  s->UMx = U_mean_true * u0;
+  s->UMy = U_mean_true * u1;
 }
 template <class FY>
 __global__ void __launch_bounds__(FY::LB) k_cols_ab2(const double2 *__restrict__ Rc, const double2 *__restrict__ Rpx,
@@ -936,7 +996,7 @@ __global__ void __launch_bounds__(FX::LB) kb_rows_nl(const RowsNlArgs *__restric
   const RowsNlArgs &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
   rows_nl_body(a.A, a.B, a.u1, a.u2, a.ms, a.mb, a.penal_on, a.Rc, a.Rpx, F, twx, a.c, a.umax_part, a.nparts, a.sc,
-               a.fl, a.time, a.dt0, a.explicit_step, a.res, (int)blockIdx.x, blockIdx.x == 0);
+               a.fl, a.time, a.dt0, a.explicit_step, a.res, (int)blockIdx.x, blockIdx.x == 0, nullptr);
 }
 template <class FY>
 __global__ void __launch_bounds__(FY::LB) kb_cols_ab2(const ColsAb2Args *__restrict__ arr, FY F,

@@ -172,6 +172,9 @@ struct o_sim {
   double energies[6], bsi[11], fx_int, fy_int;
   double u_mean[2];
   double fmean[2][2];
+  double fm_new[2]; /* f_mean of the last cal_nlk (NavierStokes.f90:97-100) */
+  int fm_set;       /* 1 when cal_nlk computed it (penalisation on) */
+  int frame_done;   /* mask_frame set up (create_mask.f90:188-192) */
 };
 
 static int g_threads = 0;
@@ -876,6 +879,58 @@ static void draw_sharp_end(o_sim *s, double pex, double pey, double alpha,
 
 /* create_mask: create_mask.f90:2-218 (walls, cylinder, beam; sponges 4 and 22
  * are not restated) */
+/* SmoothStep2: create_mask.f90:523-544 (cosine step of half width h) */
+static double o_smoothstep2(const o_sim *s, double x, double t, double h) {
+  if (x <= t - h) return 1.0;
+  if ((t - h) < x && x < (t + h)) return 0.5 * (1. + cos((x - t + h) * s->pi / (2.0 * h)));
+  return 0.0;
+}
+
+/* SmoothFrameAroundDomain: create_mask.f90:370-463.  A level set phi of the
+ * distance to the domain's edges (rounded corners of radius 2 thickness),
+ * then mask1 = SmoothStep2(phi; thickness/2, thickness/2), thickness =
+ * 4 t_beam.  Unset phi is 1000. */
+void o_frame_mask(o_sim *s, double *mask1) {
+  const int nx = s->nx, ny = s->ny;
+  const double dx = s->dx, dy = s->dy, xl = s->p.xl, yl = s->p.yl;
+  const size_t NN = (size_t)nx * ny;
+  double *phi = dalloc(NN);
+  for (size_t k = 0; k < NN; k++) phi[k] = 1000.0;
+  const double thickness = 4.0 * s->p.t_beam, R1 = 0.0, R2 = 2.0 * thickness, RR = R1 + R2;
+  const int ixmin = fnint(RR / dx), ixmax = nx - fnint(RR / dx);
+  const int iymin = fnint(RR / dy), iymax = ny - fnint(RR / dy);
+#define PHI(ix, iy) phi[(size_t)(iy) * nx + (ix)]
+  for (int ix = ixmin; ix <= ixmax; ix++)
+    for (int iy = 0; iy <= ny - 1; iy++) {
+      if (ix < 0 || ix >= nx) continue;
+      const double y = (double)iy * dy;
+      const double m = fmin(PHI(ix, iy), fmin(fabs(y - R1), fabs(y - (yl - R1))));
+      PHI(ix, iy) = (iy <= fnint(R1 / dy) || iy >= ny - 1 - fnint(R1 / dy)) ? -m : +m;
+    }
+  for (int ix = 0; ix <= nx - 1; ix++)
+    for (int iy = iymin; iy <= iymax; iy++) {
+      if (iy < 0 || iy >= ny) continue;
+      const double x = (double)ix * dx;
+      const double m = fmin(PHI(ix, iy), fmin(fabs(x - R1), fabs(x - (xl - R1))));
+      PHI(ix, iy) = (ix <= fnint(R1 / dx) || ix >= nx - 1 - fnint(R1 / dx)) ? -m : +m;
+    }
+  /* the four corners */
+  const double cx[4] = {RR, RR, xl - RR, xl - RR}, cy[4] = {RR, yl - RR, yl - RR, RR};
+  const int x0s[4] = {0, 0, ixmax, ixmax}, x1s[4] = {ixmin, ixmin, nx - 1, nx - 1};
+  const int y0s[4] = {0, iymax, iymax, 0}, y1s[4] = {iymin, ny - 1, ny - 1, iymin};
+  for (int c = 0; c < 4; c++)
+    for (int ix = x0s[c]; ix <= x1s[c]; ix++)
+      for (int iy = y0s[c]; iy <= y1s[c]; iy++) {
+        if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) continue;
+        const double x = (double)ix * dx, y = (double)iy * dy;
+        const double a = x - cx[c], b = y - cy[c];
+        PHI(ix, iy) = fmin(PHI(ix, iy), -(sqrt(a * a + b * b) - R2));
+      }
+  for (size_t k = 0; k < NN; k++) mask1[k] = o_smoothstep2(s, phi[k], thickness / 2., thickness / 2.);
+#undef PHI
+  free(phi);
+}
+
 void o_create_mask(o_sim *s, double time, o_beam *b) {
   const int nx = s->nx, ny = s->ny, ns = s->ns;
   const double dy = s->dy, N = s->p.N_smooth;
@@ -919,6 +974,35 @@ void o_create_mask(o_sim *s, double time, o_beam *b) {
     if (s->p.iSharpTrailing == 0)
       draw_a_dot(s, b->x[ns - 1], b->y[ns - 1], b->vx[ns - 1], b->vy[ns - 1], t, N);
   }
+  if (s->p.iSponge == 4) { /* velocity sponge of a channel inflow, :130-184 */
+    const double dx = s->dx, S = s->p.SpongeSize, ssz = S - 2.0 * N * dx;
+    const double H_eff = s->p.yl - 2.0 * s->p.h_channel, mux = o_mean_ux(s, time);
+    for (int half = 0; half < 2; half++) {
+      const int ixmin = half ? fnint((0.5 * S) / dx) : 0;
+      const int ixmax = half ? fnint((1.0 * S) / dx) : fnint((0.5 * S) / dx);
+      for (int ix = ixmin; ix <= ixmax; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+          double tmp;
+          const double y_chan = (double)iy * dy - s->p.h_channel;
+          if (half == 0) o_smoothstep(s, &tmp, 0.75 * S - (double)ix * dx, ssz * 0.5, 0.25 * ssz);
+          else o_smoothstep(s, &tmp, (double)ix * dx - 0.5 * S, ssz * 0.5, N * dx);
+          const size_t k = (size_t)iy * nx + ix;
+          if (s->mask[k] < tmp) s->mask[k] = tmp;
+          if (y_chan > 0.0 && y_chan < H_eff)
+            s->maskvx[k] = 1.5 * mux * y_chan * (H_eff - y_chan) / ((0.5 * H_eff) * (0.5 * H_eff));
+          else
+            s->maskvx[k] = 0.0;
+        }
+    }
+  }
+  if (s->p.iSponge == 22) { /* frame around the domain, :187-202 */
+    if (!s->frame_done) { o_frame_mask(s, s->mask_frame); s->frame_done = 1; }
+    const double w = s->p.eps / fmax(1.0e-3, s->p.eps);
+    const double mux = o_mean_ux(s, time), muy = o_mean_uy(s, time);
+    for (size_t k = 0; k < NN; k++) s->mask[k] = s->mask[k] + s->mask_frame[k] * w;
+    for (size_t k = 0; k < NN; k++)
+      if (s->mask_frame[k] > 0.0) { s->maskvx[k] = mux; s->maskvy[k] = muy; }
+  }
   for (size_t k = 0; k < NN; k++) s->mask[k] = s->mask[k] / s->p.eps; /* :210-214 */
 }
 
@@ -944,6 +1028,15 @@ void o_create_sponge_mask(o_sim *s) {
       o_smoothstep(s, &tmp, x, ssz * 0.4, N * dx);
       for (int iy = 0; iy < ny; iy++) ms[(size_t)iy * nx + ix] = tmp;
     }
+  } else if (isp == 22) { /* :333-349, after the startup mask */
+    const int ixmin = fnint(s->p.SpongeSize / dx), iymin = fnint(s->p.SpongeSize / dy);
+    const int ixmax = fnint((xl - s->p.SpongeSize) / dx), iymax = fnint((yl - s->p.SpongeSize) / dy);
+    for (size_t k = 0; k < NN; k++) ms[k] = 1.0;
+    for (int iy = iymin; iy <= iymax; iy++)
+      for (int ix = ixmin; ix <= ixmax; ix++)
+        if (ix >= 0 && ix < nx && iy >= 0 && iy < ny) ms[(size_t)iy * nx + ix] = 0.0;
+    for (size_t k = 0; k < NN; k++)
+      if (s->mask[k] > 0.0) ms[k] = 0.0;
   } else if (isp == 10) {
     int ixmin = fnint((xl - s->p.SpongeSize) / dx);
     for (int ix = ixmin; ix < nx; ix++)
@@ -1038,6 +1131,14 @@ void o_cal_nlk(o_sim *s, double time, double *dt1, double *nlk,
     for (size_t k = 0; k < N; k++) {
       pen1[k] = -(s->mask[k] * (u1[k] - s->maskvx[k]));
       pen2[k] = -(s->mask[k] * (u2[k] - s->maskvy[k]));
+    }
+    s->fm_set = 0;
+    if (p->iMeanVelocity == 11) { /* :97-100 */
+      double a = 0.0, c = 0.0;
+      for (size_t k = 0; k < N; k++) { a = a + pen1[k]; c = c + pen2[k]; }
+      s->fm_new[0] = a / (double)N;
+      s->fm_new[1] = c / (double)N;
+      s->fm_set = 1;
     }
     if (p->iSponge == 10) { /* :103-121: trapezoid integral of -chi_s du1/dx along x */
       const int nx = s->nx, ny = s->ny;
@@ -1210,6 +1311,11 @@ void o_evolve_explicit(o_sim *s, double time, double *dt1, int n0, int n1,
   o_vis(s, *dt1, s->workvis[n1]);
   for (size_t k = 0; k < N; k++)
     s->vortk[n1][k] = (s->vortk[n0][k] + *dt1 * s->nlk[n0][k]) * s->workvis[n1][k] * s->dealiase[k];
+  if (s->p.iMeanVelocity == 11) { /* :453-456 */
+    if (s->fm_set) { s->fmean[n0][0] = s->fm_new[0]; s->fmean[n0][1] = s->fm_new[1]; }
+    s->u_mean[0] = s->u_mean[0] + s->fmean[n0][0] * *dt1;
+    s->u_mean[1] = s->u_mean[1] + s->fmean[n0][1] * *dt1;
+  }
   o_pressure(s, time, *dt1, s->vortk[n1], s->press, s->u, b);
   free(stream);
 }
@@ -1227,6 +1333,10 @@ void o_evolve_ab2(o_sim *s, double time, double dt0, double *dt1, int n0,
   for (size_t k = 0; k < N; k++)
     s->vortk[n1][k] = (s->vortk[n0][k] + b10 * s->nlk[n0][k] + b11 * s->nlk[n1][k]) *
                       s->workvis[n1][k] * s->dealiase[k];
+  if (s->p.iMeanVelocity == 11) { /* :501-503 */
+    if (s->fm_set) { s->fmean[n0][0] = s->fm_new[0]; s->fmean[n0][1] = s->fm_new[1]; }
+    for (int c = 0; c < 2; c++) s->u_mean[c] = s->u_mean[c] + b10 * s->fmean[n0][c] + b11 * s->fmean[n1][c];
+  }
   o_pressure(s, time, d1, s->vortk[n1], s->press, s->u, b);
 }
 
@@ -1466,11 +1576,11 @@ void o_ab2_step(o_sim *s) {
 /* The per-step calls of time_step.f90:187-199 for a host that moves the beam
  * itself (iFLUSI=1: its solid solver writes the beam arrays between steps). */
 void o_host_create_mask(o_sim *s, double time) {
-  if (!s->started) {
+  o_create_mask(s, time, &s->beam);
+  if (!s->started) { /* after the startup mask (time_step.f90:84,90): sponge 22 reads it */
     o_create_sponge_mask(s);
     s->started = 1;
   }
-  o_create_mask(s, time, &s->beam);
 }
 double o_host_evolve(o_sim *s, double time, double dt0, int explicit_step) {
   double dt1 = 0.0;
@@ -1538,6 +1648,7 @@ void o_get_diag(o_sim *s, o_diag *d) {
   memcpy(d->bsi, s->bsi, sizeof(d->bsi));
   d->fx_int = s->fx_int; d->fy_int = s->fy_int;
   d->n0 = s->n0; d->it = s->it; d->started = s->started;
+  d->u_mean[0] = s->u_mean[0]; d->u_mean[1] = s->u_mean[1];
 }
 
 /* ---- synthetic inputs ---------------------------------------------------- */

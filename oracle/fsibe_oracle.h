@@ -112,6 +112,7 @@ double o_mean_uy(o_sim *s, double time);
 void o_smoothstep(o_sim *s, double *f, double x, double t, double h);
 void o_create_mask(o_sim *s, double time, o_beam *b);
 void o_create_sponge_mask(o_sim *s);
+void o_frame_mask(o_sim *s, double *mask1);
 
 /* ---- fluid solver (NavierStokes.f90) -------------------------------------*/
 void o_cal_velocity(o_sim *s, double time, const double *vortk, double *u,
@@ -163,6 +164,7 @@ typedef struct o_diag {
   double bsi[11];     /* drag lift drag_surf lift_surf Ft1 Fn1 Fp1 Ft2 Fn2 Fp2 (10 used) */
   double fx_int, fy_int;
   int32_t n0, it, started, pad;
+  double u_mean[2];   /* iMeanVelocity = 11 */
 } o_diag;
 void o_get_diag(o_sim *s, o_diag *d);
 

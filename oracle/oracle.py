@@ -66,6 +66,7 @@ class ODiag(C.Structure):
         ("u_max", C.c_double), ("energies", C.c_double * 6), ("bsi", C.c_double * 11),
         ("fx_int", C.c_double), ("fy_int", C.c_double),
         ("n0", C.c_int32), ("it", C.c_int32), ("started", C.c_int32), ("pad", C.c_int32),
+        ("u_mean", C.c_double * 2),
     ]
 
 

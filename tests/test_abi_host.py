@@ -55,8 +55,8 @@ def test_argument_errors_before_any_device_call():
         p = Params(nx=64, ny=64, **kw).to_c()
         return L.fsibe_sim_create(C.byref(p), C.byref(h))
 
-    assert create(iSponge=4) == 4 and "velocity sponge" in _err(L)
-    assert create(iMeanVelocity=11) == 4
+    assert create(iSponge=7) == 4 and "sponge type" in _err(L)
+    assert create(iSponge=22, eps_sponge=0.0) == 1
     assert create(iBeam=2) == 4
     assert create(eps=0.0) == 1
     assert create(iSponge=3, eps_sponge=0.0) == 1

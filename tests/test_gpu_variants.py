@@ -1,7 +1,9 @@
 """GPU parity beyond the benchmark configuration: every mask and flow variant
 the device path implements (create_mask.f90 walls :34-62, cylinder :67-69,
-sharp trailing edge :97-99; create_sponge_mask types 1, 2, 10; mean flow
-ramps mean_velocity.f90 cases 1, 2; iViscous = 0; the sharp mask; flapping
+sharp trailing edge :97-99, the velocity sponge of a channel :130-184 and
+the frame around the domain :187-202; create_sponge_mask types 1, 2, 10, 22;
+mean flow ramps mean_velocity.f90 cases 1, 2 and the zero-mean-pressure
+forcing 11 (NavierStokes.f90:97-100,453-456,501-503); iViscous = 0; the sharp mask; flapping
 and heave + pitch kinematics; the mask's cut-offs moved out (iSmoothCutoff);
 no beam at all; non-square grids), a bent beam
 moved by the host between steps (iFLUSI = 1, the per-step entry points of
@@ -72,6 +74,11 @@ VARIANTS = {
     "flapping": dict(iMotion=3, AngleBeam=10.0),
     "heave_pitch": dict(iMotion=601, heave_freq=2.0, motion_coef=(0.05, -0.02, 0.01, 0.03, 0.1, -0.04, 0.02, 0.05)),
     "soft_startup": dict(T_release=0.001, tau=0.002),
+    "velocity_sponge_channel": dict(iWalls=1, h_channel=0.4, U_mean_true=0.8, iSponge=4, SpongeSize=0.5,
+                                    iMeanVelocity=1, T_fullspeed=0.01),
+    "frame_sponge": dict(iSponge=22, SpongeSize=0.4),
+    "zero_mean_pressure": dict(iMeanVelocity=11),
+    "frame_zero_mean_pressure": dict(iSponge=22, SpongeSize=0.4, iMeanVelocity=11),
     "smooth_cutoff": dict(iSmoothCutoff=1),
     "smooth_cutoff_heave_pitch": dict(iSmoothCutoff=1, iMotion=601, heave_freq=2.0,
                                       motion_coef=(0.05, -0.02, 0.01, 0.03, 0.1, -0.04, 0.02, 0.05)),

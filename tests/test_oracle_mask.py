@@ -3,6 +3,7 @@
 one dot (:67-69, DrawADot :660-701), up-down symmetry and rigid velocity of a
 straight heaving beam, and the type-3 sponge (:315-330)."""
 import numpy as np
+import pytest
 from scipy.special import erf
 
 GERF = abs((np.exp(-4.0) - 1.0) / np.sqrt(np.pi))
@@ -103,3 +104,95 @@ def test_smooth_cutoff_extends_the_reference_mask(oracle_mod):
     step = smooth(t_star, t, N * dx)
     assert 1e-7 < step < 1e-6 and sm[~inner, i_mid].max() > 1e-9
     assert np.all(sm >= ref) and (sm - ref).max() <= step
+
+
+def _ss2(x, t, h):
+    """SmoothStep2, create_mask.f90:523-544 (cosine step)"""
+    x = np.asarray(x, float)
+    return np.where(x <= t - h, 1.0, np.where(x < t + h, 0.5 * (1.0 + np.cos((x - t + h) * np.pi / (2.0 * h))), 0.0))
+
+
+def test_frame_around_the_domain(oracle_mod):
+    """iSponge = 22: the frame is SmoothStep2 of the distance to the nearest
+    edge over 4 t_beam (away from the rounded corners), 1 on the edge rows
+    and in the corners, added to the mask at weight 1 (eps = 1e-3) with the
+    mean velocity; the vorticity sponge is 1/eps_sponge in the outer band
+    and 0 where the mask is positive or inside"""
+    n, tb, S = 64, 0.05, 0.3
+    o = oracle_mod.Oracle(nx=n, ny=n, iBeam=0, iSponge=22, SpongeSize=S, t_beam=tb, eps=1e-3, eps_sponge=1e-3,
+                          iMeanVelocity=3, theta_inf=0.0)
+    o.create_mask(0.0)
+    o.create_sponge_mask()
+    dx = 4.0 / n
+    T = 4 * tb
+    fr = o.field("mask") * 1e-3
+    mid = n // 2
+    x = np.arange(n) * dx
+    near = x < 2 * T - 1e-12  # columns that only the edge distance reaches on the middle row
+    np.testing.assert_allclose(fr[mid, near], _ss2(x[near], T / 2, T / 2), rtol=0, atol=1e-15)
+    assert fr[mid, mid] == 0.0 and fr[0, mid] == 1.0 and fr[0, 0] == 1.0 and fr[n - 1, n - 1] == 1.0
+    assert np.array_equal(fr, fr.T)  # square domain
+    vx = o.field("maskvx")
+    assert np.all(vx[fr > 0] == 1.0) and np.all(vx[fr == 0] == 0.0)
+    ms = o.field("mask_sponge") * 1e-3
+    k = int(round(0.25 / dx))  # inside the sponge band, beyond the frame's reach
+    assert fr[mid, k] == 0.0 and ms[mid, k] == pytest.approx(1.0, rel=1e-15)
+    assert ms[mid, mid] == 0.0 and ms[0, mid] == 0.0
+
+
+def test_velocity_sponge_of_a_channel(oracle_mod):
+    """iSponge = 4: over the first S of the channel the mask is at least the
+    erf step of the sponge (left half: x = 0.75 S - x_i over a quarter of
+    the core size; right half: x_i - 0.5 S over the smoothing layer) and
+    maskvx is the parabola 1.5 Mean_ux y (H - y) / (H/2)^2 inside the
+    channel, 0 in the walls"""
+    nx, ny, h, S, N = 128, 64, 0.4, 0.5, 2.0
+    o = oracle_mod.Oracle(nx=nx, ny=ny, iBeam=0, iWalls=1, h_channel=h, iSponge=4, SpongeSize=S, N_smooth=N,
+                          iMeanVelocity=3, theta_inf=0.0, eps=1e-3)
+    o.create_mask(0.0)
+    m, vx = o.field("mask") * 1e-3, o.field("maskvx")
+    dx, dy = 4.0 / nx, 4.0 / ny
+    H = 4.0 - 2 * h
+    ssz = S - 2 * N * dx
+    yc = np.arange(ny) * dy - h
+    prof = np.where((yc > 0) & (yc < H), 1.5 * yc * (H - yc) / (0.5 * H) ** 2, 0.0)
+    imid, imax = int(round(0.5 * S / dx)), int(round(S / dx))
+    for ix in (0, 3, imid - 1):
+        step = float(smooth(0.75 * S - ix * dx, ssz * 0.5, 0.25 * ssz))
+        assert np.all(m[:, ix] >= step - 1e-16)
+        np.testing.assert_allclose(vx[:, ix], prof, rtol=1e-15, atol=0)
+        inside = (yc > 2 * h) & (yc < H - 2 * h)  # rows the walls do not reach
+        np.testing.assert_allclose(m[inside, ix], step, rtol=1e-15)
+    for ix in (imid + 1, imax):
+        step = float(smooth(ix * dx - 0.5 * S, ssz * 0.5, N * dx))
+        inside = (yc > 2 * h) & (yc < H - 2 * h)
+        np.testing.assert_allclose(m[inside, ix], step, rtol=1e-15, atol=1e-300)
+        np.testing.assert_allclose(vx[:, ix], prof, rtol=1e-15, atol=0)
+    assert np.all(vx[:, imax + 1:] == 0.0)
+
+
+def test_zero_mean_pressure_forcing(oracle_mod):
+    """iMeanVelocity = 11: u_mean advances with the mean penalisation force
+    f = mean(-mask (u - u_s)) of each step's cal_nlk: u_mean = f dt1 after
+    the startup step, u_mean + b10 f_n + b11 f_(n-1) after an AB2 step"""
+    n = 64
+    o = oracle_mod.Oracle(nx=n, ny=n, iBeam=0, iCylinder=1, R_cylinder=0.3, x0=2.0, y0=2.0, iSponge=0,
+                          iMeanVelocity=11, iMotion=0, dt_fixed=2e-4)
+    vk = o.init_noise(seed=5, amp=1e-2, kmax2=64)
+    o.set_vortk(vk)
+    u0, _ = o.cal_velocity(0.0, vk)
+    o.run(1)
+    mask, mvx, mvy = o.field("mask"), o.field("maskvx"), o.field("maskvy")
+    f1 = np.array([np.mean(-(mask * (u0[0] - mvx))), np.mean(-(mask * (u0[1] - mvy)))])
+    d = o.diag()
+    um1 = np.array(d.u_mean)
+    np.testing.assert_allclose(um1, f1 * d.dt1, rtol=1e-12, atol=1e-18)
+    assert np.abs(um1).max() > 0
+    u1 = np.stack([o.field("u1"), o.field("u2")])
+    o.run(1)
+    f2 = np.array([np.mean(-(mask * (u1[0] - mvx))), np.mean(-(mask * (u1[1] - mvy)))])
+    d2 = o.diag()
+    dt1, dt0 = d2.dt1, d2.dt0
+    b10 = dt1 / dt0 * (0.5 * dt1 + dt0)
+    b11 = -(0.5 * dt1 * dt1) / dt0
+    np.testing.assert_allclose(np.array(d2.u_mean), um1 + b10 * f2 + b11 * f1, rtol=1e-12, atol=1e-18)
