@@ -318,6 +318,7 @@ extern "C" int fsibe_backup_read(fsibe_sim *s, const char *path, int *ivideo, do
       // not in the file: init_beam's .true. holds (init_beam.f90:107), so a
       // BDF2 run restarts with a Crank-Nicolson step, as the reference does
       b.StartupStep = true;
+      s->solid->sv.pre_ok_ = false;
     }
   }
   if (ivideo) *ivideo = iv;

@@ -560,6 +560,7 @@ extern "C" int fsibe_solid_set(fsibe_solid *b, int which, const double *in) {
   std::vector<double> *v = solid_array(b, which);
   if (!v) return set_err(FSIBE_EARG, "unknown solid array %d", which);
   std::copy(in, in + v->size(), v->begin());
+  b->sv.pre_ok_ = false;
   return FSIBE_OK;
 }
 
@@ -585,6 +586,7 @@ extern "C" int fsibe_solid_set_info(fsibe_solid *b, const fsibe_solid_info *in) 
   b->b.dt_old = in->dt_old;
   b->sv.grav = in->grav;
   b->b.StartupStep = in->StartupStep != 0;
+  b->sv.pre_ok_ = false;
   return FSIBE_OK;
 }
 
@@ -1977,6 +1979,10 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     s->k1_ready = true;
     s->k1_level = s->n1;
     CHK(launch_k2_ahead(s));
+    // the beam solver's predictor needs only the state at t_n: it runs while
+    // the step's fluid kernels finish
+    if (p.iFLUSI == 1 && p.iBeam > 0 && s->solid && s->time > p.T_release)
+      (void)s->solid->sv.predict(s->time, s->solid->b);
     HIPCHK(wait_event(s->ev_step));
     s->upload_pending = false;
     parse_results(s, s->slot);

@@ -276,6 +276,11 @@ struct Solver {
     std::vector<double> tau_s, old_rhs, thd_new;
   } cx;
   std::vector<double> th_old_, thd_old_;  // theta, theta_dot at t_n
+  // the predictor's RHS_beameqn at t_n (pre_th1_ = theta_dot, pre_T_ =
+  // tension), computed by predict() before the new loads are known
+  std::vector<double> pre_th1_, pre_T_;
+  double pre_time_ = 0.0;
+  bool pre_ok_ = false;
   std::string err;
   int iter = 0;            // Newton iterations of the last step (IBES_iter)
   bool brake = false;      // the step tripped the reference's emergency brake
@@ -689,11 +694,30 @@ struct Solver {
 
   // initial guess: an explicit Euler step with the loads at t_n, then the
   // ghost values of the boundary conditions (SolidSolver.f90:167-209)
+  // The part of the predictor that needs only the state at t_n and the old
+  // loads (RHS_beameqn with pressure_old, tau_old, gravity at t_n), so that
+  // a host can compute it while the fluid step that brings the new loads
+  // still runs.  guess() uses it when its time matches, else computes it.
+  bool predict(double time, const Beam &b) {
+    pre_ok_ = false;
+    gravity_impulse(time);
+    pre_th1_ = b.theta;
+    std::vector<double> thd1(b.theta_dot);
+    pre_T_.assign(ns, 0.0);
+    if (!rhs_beameqn(time, pre_th1_.data(), thd1.data(), b.pressure_old.data(), b.tau_old.data(), pre_T_.data()))
+      return false;
+    pre_time_ = time;
+    pre_ok_ = true;
+    return true;
+  }
+
   bool guess(double time, double dt, const Beam &b, double *x) {
     double alpha, at, att, LE[6];
     motion(time + dt, alpha, at, att, LE);
-    std::vector<double> th1(b.theta), thd1(b.theta_dot), T(ns), tg(ns);
-    if (!rhs_beameqn(time, th1.data(), thd1.data(), b.pressure_old.data(), b.tau_old.data(), T.data())) return false;
+    if (!(pre_ok_ && pre_time_ == time) && !predict(time, b)) return false;
+    pre_ok_ = false;  // used once
+    const std::vector<double> &th1 = pre_th1_, &T = pre_T_;
+    std::vector<double> tg(ns);
     for (int i = 0; i < ns; ++i) tg[i] = b.theta[i] + dt * th1[i];
     const double eta = p.eta, mue = p.mue, ds2 = ds * ds;
     double *thg = x + 1;  // thg[-1] .. thg[ns+1]
