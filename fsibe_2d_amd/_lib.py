@@ -89,7 +89,7 @@ EXPORTS = (
     "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid", "fsibe_solid_wrapper", "fsibe_gravity_impulse",
     "fsibe_adjoint_create", "fsibe_adjoint_destroy", "fsibe_adjoint_objective", "fsibe_adjoint_gradient",
     "fsibe_adjoint_timings", "fsibe_backup_write", "fsibe_backup_read", "fsibe_inicond_write",
-    "fsibe_inicond_read",
+    "fsibe_inicond_read", "fsibe_pressure_snapshot", "fsibe_save_field",
 )
 
 _lib = None
@@ -175,6 +175,9 @@ def lib():
                                     C.POINTER(C.c_int)]
     L.fsibe_inicond_write.argtypes = [vp, C.c_char_p]
     L.fsibe_inicond_read.argtypes = [vp, C.c_char_p]
+    L.fsibe_pressure_snapshot.argtypes = [vp, C.c_double, C.c_int, dp]
+    L.fsibe_save_field.argtypes = [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_char_p,
+                                   C.c_char_p]
     _lib = L
     return L
 

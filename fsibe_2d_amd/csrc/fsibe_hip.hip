@@ -708,6 +708,8 @@ struct fsibe_sim {
   double *u1, *u2, *vor_mean, *ms, *wallrow;
   double *frame = nullptr;   // iSponge = 22: the frame mask (SmoothFrameAroundDomain), full grid
   double *rowpen = nullptr;  // iMeanVelocity = 11: row sums of the penalisation term (2 per row)
+  fsibe_spectral *spec = nullptr;  // packed-layout operators for pressure snapshots (made on first use)
+  double *snap = nullptr;          // their scratch, 9 fields
   double *umax_part;
   DevScalars *dscal;
   double *mask_m, *mask_vx, *mask_vy;
@@ -998,7 +1000,8 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
                   s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow,
                   s->umax_part, s->dscal, s->mask_m, s->mask_vx, s->mask_vy, s->fb_p, s->fb_sa,
-                  s->fb_sb, s->d_stage, s->frame, s->rowpen};
+                  s->fb_sb, s->d_stage, s->frame, s->rowpen, s->snap};
+  if (s->spec) fsibe_spectral_destroy(s->spec);
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
@@ -2750,3 +2753,4 @@ extern "C" fsibe_sim *fsibe_batch_member(fsibe_batch *b, int i) {
 // ---- discrete adjoint (fsibe_adjoint_*) -------------------------------------
 #include "adjoint_host.hpp"
 #include "backup_host.hpp"
+#include "outputs_host.hpp"

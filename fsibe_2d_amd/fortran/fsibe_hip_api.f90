@@ -342,6 +342,24 @@ module fsibe_hip_api
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int) :: rc
     end function
+    ! ComputePressureSnapshot and SaveField
+    function fsibe_pressure_snapshot(sim, time, level, press) bind(C, name="fsibe_pressure_snapshot") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      real(c_double), value :: time
+      integer(c_int), value :: level
+      real(c_double), intent(out) :: press(*)
+      integer(c_int) :: rc
+    end function
+    function fsibe_save_field(path, field, nx, ny, nsave, xl, yl, fieldtype, header) &
+        bind(C, name="fsibe_save_field") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*), fieldtype(*), header(*)
+      real(c_double), intent(in) :: field(*)
+      integer(c_int), value :: nx, ny, nsave
+      real(c_double), value :: xl, yl
+      integer(c_int) :: rc
+    end function
     function fsibe_adjoint_gradient(adj, coef, J, grad) bind(C, name="fsibe_adjoint_gradient") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: adj

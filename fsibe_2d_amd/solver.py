@@ -166,9 +166,25 @@ class FluidSim:
         """init_fields with inicond = 111 (before the first step)"""
         check(lib().fsibe_inicond_read(self.h, os.fsencode(path)))
 
+    def pressure_snapshot(self, time: float, level: int) -> np.ndarray:
+        """ComputePressureSnapshot(time, vortk(:,:,level)) (NavierStokes.f90:598-665)"""
+        out = np.zeros((self.p.ny, self.p.nx))
+        check(lib().fsibe_pressure_snapshot(self.h, float(time), int(level), out))
+        return out
+
     @property
     def stream(self):
         return lib().fsibe_sim_stream(self.h)
+
+
+def save_field(path, field, nsave: int = 1, xl: float = 4.0, yl: float = 4.0, fieldtype: str = "vorticity",
+               header: str | None = None):
+    """SaveField (FieldExport.f90:612-648): the reference's text field file of
+    a host field [ny][nx]; header: the PARAMS lines written before it"""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    ny, nx = f.shape
+    check(lib().fsibe_save_field(os.fsencode(path), f, nx, ny, int(nsave), float(xl), float(yl),
+                                 fieldtype.encode(), header.encode() if header is not None else None))
 
     @property
     def solid(self):

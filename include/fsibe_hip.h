@@ -362,6 +362,20 @@ int fsibe_inicond_write(fsibe_sim *sim, const char *path);
  * coftxy of the file's vorticity; only before the first step */
 int fsibe_inicond_read(fsibe_sim *sim, const char *path);
 
+/* ---- field outputs ----------------------------------------------------------
+ * ComputePressureSnapshot(time, vortk(:,:,level), press) (NavierStokes.f90:
+ * 598-665): the static pressure of Fortran level `level` (0/1; the time
+ * loop passes n1) at `time`, with the current mask, into press[nx*ny].
+ * Leaves the simulation's state alone.  Not a step kernel: built from the
+ * packed-layout spectral operators on the device. */
+int fsibe_pressure_snapshot(fsibe_sim *sim, double time, int level, double *press);
+/* SaveField(filename, field, NSave, lx, ly, fieldtype) (FieldExport.f90:
+ * 612-648) for a host field [ny][nx]: header = the PARAMS lines that
+ * WriteHeaderToFile writes (BeamIO.f90:364-376), '\n'-separated, each
+ * written blank-padded to 256 characters (NULL: none).  No GPU involved. */
+int fsibe_save_field(const char *path, const double *field, int nx, int ny, int nsave, double xl, double yl,
+                     const char *fieldtype, const char *header);
+
 /* HIP stream of the simulation (hipStream_t) for event timing */
 void *fsibe_sim_stream(fsibe_sim *sim);
 

@@ -1320,6 +1320,32 @@ void o_evolve_explicit(o_sim *s, double time, double *dt1, int n0, int n1,
   free(stream);
 }
 
+/* ComputePressureSnapshot: NavierStokes.f90:596-665, for vortk of Fortran
+ * level `level` at `time`, with the current mask */
+void o_pressure_snapshot(o_sim *s, double time, int level, double *press) {
+  const size_t N = (size_t)s->nx * s->ny;
+  double *u = dalloc(2 * N), *work1 = dalloc(N), *work2 = dalloc(N), *wdx = dalloc(N), *wdy = dalloc(N);
+  double *pen1 = dalloc(N), *pen2 = dalloc(N);
+  const double *vortk = s->vortk[level];
+  o_cal_velocity(s, time, vortk, u, work1);                                  /* :622 */
+  const double *u1 = u, *u2 = u + N;
+  for (size_t k = 0; k < N; k++) {                                           /* :625-626 */
+    pen1[k] = s->mask[k] * (u1[k] - s->maskvx[k]);
+    pen2[k] = s->mask[k] * (u2[k] - s->maskvy[k]);
+  }
+  o_cofitxy(s, vortk, work1);                                                /* :629 */
+  o_coftxy(s, pen1, work2); o_cofdx(s, work2, pen1);                         /* :635-636 */
+  o_coftxy(s, pen2, work2); o_cofdy(s, work2, pen2);                         /* :638-639 */
+  for (size_t k = 0; k < N; k++) { wdx[k] = -work1[k] * u2[k]; wdy[k] = +work1[k] * u1[k]; } /* :644-645 */
+  o_coftxy(s, wdx, work1); o_cofdx(s, work1, wdx);                           /* :648-649 */
+  o_coftxy(s, wdy, work1); o_cofdy(s, work1, wdy);                           /* :650-651 */
+  for (size_t k = 0; k < N; k++) work1[k] = ((pen1[k] + pen2[k]) + wdy[k]) + wdx[k]; /* :654-655 */
+  o_poisson(s, work1, work2);                                                /* :659 */
+  o_cofitxy(s, work2, work1);                                                /* :660 */
+  for (size_t k = 0; k < N; k++) press[k] = work1[k] - 0.5 * (u1[k] * u1[k] + u2[k] * u2[k]); /* :661 */
+  free(u); free(work1); free(work2); free(wdx); free(wdy); free(pen1); free(pen2);
+}
+
 /* EvolveFluidAB2: NavierStokes.f90:463-508 */
 void o_evolve_ab2(o_sim *s, double time, double dt0, double *dt1, int n0,
                   int n1, o_beam *b) {

@@ -113,6 +113,7 @@ void o_smoothstep(o_sim *s, double *f, double x, double t, double h);
 void o_create_mask(o_sim *s, double time, o_beam *b);
 void o_create_sponge_mask(o_sim *s);
 void o_frame_mask(o_sim *s, double *mask1);
+void o_pressure_snapshot(o_sim *s, double time, int level, double *press);
 
 /* ---- fluid solver (NavierStokes.f90) -------------------------------------*/
 void o_cal_velocity(o_sim *s, double time, const double *vortk, double *u,

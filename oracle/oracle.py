@@ -105,6 +105,7 @@ def lib():
         L.o_dealiase_mask.argtypes = [vp, dp]
         L.o_create_mask.argtypes = [vp, C.c_double, C.POINTER(OBeam)]
         L.o_create_sponge_mask.argtypes = [vp]
+        L.o_pressure_snapshot.argtypes = [vp, C.c_double, C.c_int, dp]
         L.o_cal_velocity.argtypes = [vp, C.c_double, dp, dp, dp]
         L.o_cal_nlk.argtypes = [vp, C.c_double, C.POINTER(C.c_double), dp, dp, dp]
         L.o_get_beam.restype = C.POINTER(OBeam)
@@ -345,6 +346,11 @@ class Oracle:
         st = self._new()
         self.L.o_cal_velocity(self.h, time, self._as(vk), u, st)
         return u, st
+
+    def pressure_snapshot(self, time, level):
+        out = self._new()
+        self.L.o_pressure_snapshot(self.h, C.c_double(time), int(level), out)
+        return out
 
     def cal_nlk(self, time, vk, u):
         nlk = self._new()
