@@ -419,7 +419,8 @@ def main(argv=None) -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU decomposition is described in DESIGN.md", file=sys.stderr)
+              "a large simulation would shard by row slabs and column slabs with one all-to-all between "
+              "passes, and batches by member, neither of which is built here", file=sys.stderr)
         return 2
 
     from fsibe_2d_amd import fsi_large

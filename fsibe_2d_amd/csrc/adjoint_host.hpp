@@ -1,9 +1,9 @@
 // adjoint_host.hpp -- host side of the discrete adjoint (fsibe_adjoint_*),
 // part of fsibe_hip.hip's translation unit (included at its end).
 //
-// Objective (the north star's differentiation form; no reference
-// counterpart): J = (1/T) sum_n (-Force(1)^n) dt1^n over nsteps steps of a
-// prescribed-motion simulation, T = sum dt1, Force(1) the cal_drag force
+// Objective (no reference counterpart): J = (1/T) sum_n (-Force(1)^n) dt1^n
+// over nsteps steps of a prescribed-motion simulation, T = sum dt1,
+// Force(1) the cal_drag force
 // (NavierStokes.f90:567-575); parameters: the 8 coefficients of motion 601
 // (two harmonics of heave and of pitch).  Gradient by a reverse sweep over the
 // adjoint kernels of kernels_adjoint.hpp with two-level checkpointing: the

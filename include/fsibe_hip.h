@@ -11,7 +11,8 @@
  *  - Every call returns 0 on success or an FSIBE_E* code; fsibe_last_error()
  *    gives the message of the last failure on the calling thread.  The
  *    reference instead prints and STOPs (e.g. Interpolation.f90:19-24); the
- *    Fortran binding in INTEGRATION.md turns a nonzero code into write+stop.
+ *    Fortran binding (fsibe_2d_amd/fortran, fsibe_check) turns a nonzero
+ *    code into write+stop.
  *  - Entry points without a _dev_ infix take host pointers, copy in, compute on
  *    the GPU and copy out before returning.  _dev_ entry points take device
  *    pointers and are asynchronous on the handle's stream.
@@ -44,7 +45,7 @@ const char *fsibe_last_error(void);
 /* The reference selects its FFT backend at link time (makefile:40,75); every */
 /* backend exports fft_initialize, fft_free, coftxy, cofitxy with grid sizes  */
 /* taken from module share_vars.  This tier is the same plugin for a host    */
-/* that binds it through iso_c_binding.                      */
+/* that binds it through iso_c_binding (fsibe_2d_amd/fortran/cof_hip.f90).   */
 /* ========================================================================= */
 typedef struct fsibe_spectral fsibe_spectral;
 

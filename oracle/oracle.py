@@ -144,7 +144,7 @@ def lib():
 
 
 def make_params(**kw) -> OParams:
-    """Defaults follow the survey's config 0 (256^2, domain 4x4, Re=200,
+    """Defaults: the 256^2 workload (domain 4x4, Re=200,
     sponge 3, beam of 64 nodes at (1,2), heave 0.1 sin(2 pi t))."""
     d = dict(
         nx=256, ny=256, ns=64, iMeanVelocity=3, iSponge=3, iBeam=1, iWalls=0,

@@ -2,7 +2,7 @@
 restatement of the reference's call sequence (time_step.f90:120-341 with the
 beam heaving as prescribed kinematics).
 
-Bars (north star): vorticity relative L2 <= 1e-10 after 100 steps; forces and
+Bars: vorticity relative L2 <= 1e-10 after 100 steps; forces and
 beam loads <= 1e-9 relative, with an absolute floor of 1e-6 x the largest
 magnitude of that quantity (lift is ~0 by symmetry early in the run).
 """
