@@ -10,7 +10,8 @@ OUT = os.path.join(HERE, "libfsibe_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off keeps the reference's rounding in elementwise code; the FFT
-# device code re-enables contraction locally.  Packed FP32 ops are disabled as
+# device code writes its fused multiply-adds out (fma), so every kernel
+# instance of a transform rounds alike.  Packed FP32 ops are disabled as
 # required on these GPUs (the library computes in FP64 throughout).
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

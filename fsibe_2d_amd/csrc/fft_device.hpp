@@ -29,9 +29,12 @@ __host__ __device__ __forceinline__ constexpr int padlen(int L) { return L > 0 ?
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// Complex product with its two fused multiply-adds written out.  The
+// library builds with contraction off, so every fused operation of the FFT
+// is explicit and every instantiation (compile-time or runtime plan, any
+// tile width) rounds exactly alike.
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-#pragma clang fp contract(fast)
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // multiply by -i (forward) or +i (inverse)
@@ -61,36 +64,37 @@ template <bool INV> struct Dft<4, INV> {
 
 template <bool INV> struct Dft<8, INV> {
   __device__ __forceinline__ static void run(double2 *v) {
-#pragma clang fp contract(fast)
     const double h = 0.70710678118654752440;  // sqrt(1/2)
     double2 a[4] = {v[0], v[2], v[4], v[6]};
     double2 b[4] = {v[1], v[3], v[5], v[7]};
     Dft<4, INV>::run(a);
     Dft<4, INV>::run(b);
-    // b1 *= w8, b2 *= w8^2, b3 *= w8^3 with w8 = exp(-+ i pi/4)
-    double2 t1 = INV ? make_double2(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y))
-                     : make_double2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
-    double2 t2 = mul_mi<INV>(b[2]);
-    double2 t3 = INV ? make_double2(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y))
-                     : make_double2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
+    // b1 *= w8 and b3 *= w8^3 (w8 = exp(-+ i pi/4)) as h * s1, h * s3, each
+    // folded into the sums below by a fused multiply-add; b2 *= w8^2 = -+i
+    const double2 s1 = INV ? make_double2(b[1].x - b[1].y, b[1].x + b[1].y)
+                           : make_double2(b[1].x + b[1].y, b[1].y - b[1].x);
+    const double2 s3 = INV ? make_double2(-(b[3].x + b[3].y), b[3].x - b[3].y)
+                           : make_double2(b[3].y - b[3].x, -(b[3].x + b[3].y));
+    const double2 t2 = mul_mi<INV>(b[2]);
     v[0] = cadd(a[0], b[0]); v[4] = csub(a[0], b[0]);
-    v[1] = cadd(a[1], t1);   v[5] = csub(a[1], t1);
+    v[1] = make_double2(fma(h, s1.x, a[1].x), fma(h, s1.y, a[1].y));
+    v[5] = make_double2(fma(-h, s1.x, a[1].x), fma(-h, s1.y, a[1].y));
     v[2] = cadd(a[2], t2);   v[6] = csub(a[2], t2);
-    v[3] = cadd(a[3], t3);   v[7] = csub(a[3], t3);
+    v[3] = make_double2(fma(h, s3.x, a[3].x), fma(h, s3.y, a[3].y));
+    v[7] = make_double2(fma(-h, s3.x, a[3].x), fma(-h, s3.y, a[3].y));
   }
 };
 
 template <bool INV> struct Dft<3, INV> {
   __device__ __forceinline__ static void run(double2 *v) {
-#pragma clang fp contract(fast)
     const double s3 = 0.86602540378443864676;  // sin(2 pi / 3)
-    double2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-    double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
-    // e = (sign i s3) d, sign = -1 forward
-    double2 e = INV ? make_double2(-s3 * d.y, s3 * d.x) : make_double2(s3 * d.y, -s3 * d.x);
+    const double2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    const double2 m = make_double2(fma(-0.5, t.x, v[0].x), fma(-0.5, t.y, v[0].y));
+    // v1 = m + e, v2 = m - e with e = (sign i s3) d, sign = -1 forward
+    const double ex = INV ? -s3 : s3;  // e.x = ex * d.y, e.y = -ex * d.x
     v[0] = cadd(v[0], t);
-    v[1] = cadd(m, e);
-    v[2] = csub(m, e);
+    v[1] = make_double2(fma(ex, d.y, m.x), fma(-ex, d.x, m.y));
+    v[2] = make_double2(fma(-ex, d.y, m.x), fma(ex, d.x, m.y));
   }
 };
 

@@ -36,23 +36,22 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 // ---- real <-> half-length complex conversions (in place, per row) -------------
 // Forward: Z = FFT_H(x[2n] + i x[2n+1]) -> X[k] = 1/2 (Z_k + conj Z_{H-k})
 //          - i/2 e^{-2 pi i k/nx} (Z_k - conj Z_{H-k}), k < H (X[H] dropped).
+// (fused multiply-adds explicit, as in cmul)
 __device__ __forceinline__ double2 r2c_one(double2 a, double2 b, double2 w) {
-#pragma clang fp contract(fast)
   const double2 bc = cconj(b);
   const double2 e = make_double2(0.5 * (a.x + bc.x), 0.5 * (a.y + bc.y));
   const double2 d = make_double2(0.5 * (a.x - bc.x), 0.5 * (a.y - bc.y));
-  const double pr = w.x * d.x - w.y * d.y, pi = w.x * d.y + w.y * d.x;
+  const double pr = fma(w.x, d.x, -(w.y * d.y)), pi = fma(w.x, d.y, w.y * d.x);
   return make_double2(e.x + pi, e.y - pr);
 }
 
 // Inverse (unnormalised c2r, X[H] = 0, Im X[0] ignored):
 // Z_k = (X_k + conj X_{H-k}) + i e^{+2 pi i k/nx} (X_k - conj X_{H-k}).
 __device__ __forceinline__ double2 c2r_one(double2 a, double2 b, double2 wconj) {
-#pragma clang fp contract(fast)
   const double2 bc = cconj(b);
   const double2 e = make_double2(a.x + bc.x, a.y + bc.y);
   const double2 d = make_double2(a.x - bc.x, a.y - bc.y);
-  const double pr = wconj.x * d.x - wconj.y * d.y, pi = wconj.x * d.y + wconj.y * d.x;
+  const double pr = fma(wconj.x, d.x, -(wconj.y * d.y)), pi = fma(wconj.x, d.y, wconj.y * d.x);
   return make_double2(e.x - pi, e.y + pr);
 }
 
