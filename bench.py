@@ -293,13 +293,30 @@ def batch_bench(nsims: int, steps: int, warmup: int = 3):
         sp = Spectral(p.nx, p.ny, p.xl, p.yl, p.nu)
         vks.append(band_limited_noise(sp, sp.dealiase_mask(), seed=2 + i, amp=1e-3, kmax2=64))
         sp.close()
-    b = Batch(ps, vks)
-    b.run(warmup)
-    hip_sync()
     t0 = time.perf_counter()
-    b.run(steps)
-    hip_sync()
-    t_b = time.perf_counter() - t0
+    b = Batch(ps, vks)  # measures the column stages' launch choices
+    t_create = time.perf_counter() - t0
+
+    def timed(nsteps):
+        b.run(warmup)
+        hip_sync()
+        t0 = time.perf_counter()
+        b.run(nsteps)
+        hip_sync()
+        return time.perf_counter() - t0
+
+    # captured graphs with tuned tiles (the default) and eager launches with
+    # the same tiles, alternated twice (best of each), then eager launches
+    # with the single path's tiles
+    t_b = t_eager = float("inf")
+    for _ in range(2):
+        b.configure(graph=True)
+        t_b = min(t_b, timed(steps))
+        b.configure(graph=False)
+        t_eager = min(t_eager, timed(steps))
+    desc = b.describe()
+    b.configure(autotune=False)
+    t_static = timed(steps)
     b.close()
     t_s = 0.0
     for p, vk in zip(ps, vks):
@@ -319,7 +336,15 @@ def batch_bench(nsims: int, steps: int, warmup: int = 3):
     return {"metric": f"sims*steps/s, {nsims} FSI simulations with uneven grids and parameters",
             "value": round(v_b, 1), "unit": "sims*steps/s", "sequential_gpu_value": round(v_s, 1),
             "speedup_vs_sequential": round(v_b / v_s, 3), "sims": nsims, "steps": steps, "warmup": warmup,
-            "grids": {str(k): v for k, v in sorted(grids.items())}, "dtype": "f64", "data": "synthetic"}
+            "grids": {str(k): v for k, v in sorted(grids.items())}, "dtype": "f64", "data": "synthetic",
+            "step_mode": "two captured HIP graphs per step, column tiles from the autotuner",
+            "timing": "best of two alternated runs per step mode",
+            "eager_tuned_value": round(nsims * steps / t_eager, 1),
+            "eager_static_tiles_value": round(nsims * steps / t_static, 1),
+            "create_and_tune_s": round(t_create, 3), "graph_captures": desc["captures"],
+            "tiles": {str(g["nx"]): {k: f"{'ct' if v['plan'] == 'compile-time' else 'rt'}x{v['tx']}"
+                                     f" {v['us']:.1f}us (static {v['static_us']:.1f}us)"
+                                     for k, v in g["column_stages"].items()} for g in desc["groups"]}}
 
 
 def adjoint_bench(steps: int = 1000, chk: int = 50, smooth: bool = True):

@@ -84,6 +84,7 @@ EXPORTS = (
     "fsibe_integrate_position", "fsibe_run", "fsibe_get_time", "fsibe_get_beam",
     "fsibe_sim_stream", "fsibe_profile_enable", "fsibe_profile_read", "fsibe_cofts", "fsibe_cofits",
     "fsibe_batch_create", "fsibe_batch_destroy", "fsibe_batch_run", "fsibe_batch_size", "fsibe_batch_member",
+    "fsibe_batch_configure", "fsibe_batch_describe", "fsibe_batch_set_tiles",
     "fsibe_solid_create", "fsibe_solid_destroy", "fsibe_solid_get", "fsibe_solid_set", "fsibe_solid_get_info",
     "fsibe_solid_set_info", "fsibe_solid_step", "fsibe_solid_guess", "fsibe_solid_residual",
     "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid", "fsibe_solid_wrapper", "fsibe_gravity_impulse",
@@ -148,6 +149,9 @@ def lib():
     L.fsibe_batch_size.argtypes = [vp]
     L.fsibe_batch_member.restype = vp
     L.fsibe_batch_member.argtypes = [vp, C.c_int]
+    L.fsibe_batch_configure.argtypes = [vp, C.c_int, C.c_int]
+    L.fsibe_batch_describe.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.fsibe_batch_set_tiles.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.fsibe_cofts.argtypes = [dp, dp, C.c_int, C.c_int]
     L.fsibe_cofits.argtypes = [dp, dp, C.c_int, C.c_int]
     L.fsibe_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]

@@ -201,6 +201,60 @@ static int with_cols(const FftPlan &py, int H, int rt_tx, Fn &&fn) {
   }
 }
 
+// A launch choice of a column stage: the compile-time plan (ct = 1) or the
+// runtime plan (ct = 0), with tx columns per tile.  Every choice runs the
+// same radix passes with the same twiddles on each column, so the choice
+// changes the speed, never the result.
+struct ColSel {
+  int ct, tx;
+};
+
+// compile-time instances with other tile widths than ct_tx: the grid sizes
+// of uneven batches, where the best width depends on how many simulations
+// share a launch
+constexpr bool ct_extra(int ny, int tx) {
+  return ((ny == 256 || ny == 384 || ny == 512 || ny == 768) && (tx == 2 || tx == 4)) ||
+         (ny == 1024 && (tx == 1 || tx == 4)) || (ny == 2048 && tx == 1);
+}
+
+static bool col_sel_valid(int ny, int H, int bm, ColSel c) {
+  if (c.tx < 1 || H % c.tx) return false;
+  if ((size_t)c.tx * bm * padlen(ny) * sizeof(double2) > (size_t)lds_limit()) return false;
+  if (c.ct) {
+    switch (ny) {
+      case 128: case 256: case 384: case 512: case 768: case 1024: case 2048: case 4096: break;
+      default: return false;
+    }
+    return c.tx == ct_tx(ny) || ct_extra(ny, c.tx);
+  }
+  return shape_of((long long)bm * c.tx * ny).emax > 0;
+}
+
+// fn(policy, nthreads, TX) for the column stage launch choice `sel`
+// (col_sel_valid must hold)
+template <int BM, class Fn>
+static int with_cols_sel(const FftPlan &py, int H, ColSel sel, Fn &&fn) {
+  if (sel.ct && sel.tx == ct_tx(py.L) && H % sel.tx == 0) return with_cols<BM>(py, H, sel.tx, fn);
+  if (sel.ct) {
+    switch (py.L * 8 + sel.tx) {
+#define COLSEL(NN, TT) \
+      case NN * 8 + TT: return fn(ColCT<NN, TT, ct_nt(NN * TT * BM / 8, 1024)>{py, TT}, ct_nt(NN * TT * BM / 8, 1024), TT);
+      COLSEL(256, 2) COLSEL(256, 4) COLSEL(384, 2) COLSEL(384, 4) COLSEL(512, 2) COLSEL(512, 4) COLSEL(768, 2)
+      COLSEL(768, 4) COLSEL(1024, 1) COLSEL(1024, 4) COLSEL(2048, 1)
+#undef COLSEL
+      default: return set_err(FSIBE_ESIZE, "no compile-time column plan for %d x %d", py.L, sel.tx);
+    }
+  }
+  const Shape sh = shape_of((long long)BM * sel.tx * py.L);
+  switch (sh.emax) {
+    case 4: return fn(ColRT<4>{py, sel.tx}, sh.nt, sel.tx);
+    case 8: return fn(ColRT<8>{py, sel.tx}, sh.nt, sel.tx);
+    case 16: return fn(ColRT<16>{py, sel.tx}, sh.nt, sel.tx);
+    case 32: return fn(ColRT<32>{py, sel.tx}, sh.nt, sel.tx);
+    default: return set_err(FSIBE_ESIZE, "column tile of %d x %d too large", BM * sel.tx, py.L);
+  }
+}
+
 #define KLAUNCH(KT, grid, nt, lds, st, ...)                         \
   do {                                                              \
     CHK(prep_lds(KT, lds));                                         \
@@ -2300,7 +2354,14 @@ struct BatchGroup {
   hipEvent_t ev_dt = nullptr;    // after the group's nonlinear-term stage
   size_t off[B_NST];  // argument arrays in the batch's argument buffer
   int tx[B_NST];      // columns per tile of the column stages
-  int nt[B_NST];      // threads per block
+  ColSel sel[B_NST];  // launch choice of the column stages
+  float tune_us[B_NST], base_us[B_NST];  // measured time of the choice and of the static one (0: not tuned)
+  int ncand[B_NST];   // choices measured
+  int cap[B_NST];     // grid.x of the stage in the captured step (0: stage left out)
+  // the group's stages as two captured graphs (up to the nonlinear term, and
+  // the rest), replayed on the group's stream
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
 };
 
 struct fsibe_batch {
@@ -2313,7 +2374,22 @@ struct fsibe_batch {
   bool res_pending = false;
   bool fsi = false;  // some member's beam is solved (iFLUSI=1): every step waits for its loads
   HostPool *pool = nullptr;
+  // each group's stages replayed from captured graphs while every stage
+  // fits the captured grids
+  bool use_graph = true;
+  int captures = 0, replays = 0;
+  bool tuned = false;
 };
+
+static void drop_graphs(fsibe_batch *b) {
+  for (auto &g : b->groups)
+    for (int i = 0; i < 2; ++i) {
+      if (g.gexec[i]) (void)hipGraphExecDestroy(g.gexec[i]);
+      if (g.graph[i]) (void)hipGraphDestroy(g.graph[i]);
+      g.gexec[i] = nullptr;
+      g.graph[i] = nullptr;
+    }
+}
 
 extern "C" int fsibe_batch_destroy(fsibe_batch *b) {
   if (!b) return FSIBE_OK;
@@ -2324,6 +2400,7 @@ extern "C" int fsibe_batch_destroy(fsibe_batch *b) {
     if (g.ev_dt) (void)hipEventDestroy(g.ev_dt);
   }
   if (b->ev_args) (void)hipEventDestroy(b->ev_args);
+  drop_graphs(b);
   for (fsibe_sim *s : b->m) fsibe_sim_destroy(s);
   if (b->h_args) (void)hipHostFree(b->h_args);
   if (b->d_args) (void)hipFree(b->d_args);
@@ -2332,6 +2409,162 @@ extern "C" int fsibe_batch_destroy(fsibe_batch *b) {
   delete b->pool;
   delete b;
   return FSIBE_OK;
+}
+
+// the single path's launch choice of a column stage holding `bm` fields per
+// tile, of a kernel over `split` fields
+static ColSel static_sel(const fsibe_sim *s, int bm, int split) {
+  const ColSel c{1, ct_tx(s->ny)};
+  if (col_sel_valid(s->ny, s->H, bm, c)) return c;
+  return ColSel{0, pick_tx(s->H, s->ny, bm, split)};
+}
+
+static const int kColStages[4] = {B_K1, B_K4, B_K5, B_K7};
+
+// Argument records of column stage k for every member of g with tile width
+// tx, for the autotuner: the inputs are the members' own fields, and every
+// output goes to a per-step temporary (T), which holds nothing between steps
+// (fsibe_run and batch steps recompute all of them), so timing a stage
+// leaves every simulation as it was.
+static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int tx, char *h, int *nblk_max) {
+  const int nm = (int)g.mem.size();
+  *nblk_max = 0;
+  for (int j = 0; j < nm; ++j) {
+    const fsibe_sim *s = b->m[g.mem[j]];
+    int nb = 0;
+    if (k == B_K1 || k == B_K5) {
+      ColsInvArgs &a = reinterpret_cast<ColsInvArgs *>(h)[j];
+      a = ColsInvArgs{};
+      const int nf = k == B_K1 ? 2 : g.nf5;
+      a.W = s->W[k == B_K1 ? s->n0 : s->n1];
+      for (int q = 0; q < 4; ++q) {
+        a.o[q] = q < nf ? s->T[q] : nullptr;
+        a.fsel[q] = k == B_K1 ? (q == 1 ? 4 : 0) : (q < nf ? q : 0);
+      }
+      a.nfield = nf;
+      a.nblk = nb = (s->H / tx) * nf;
+      a.c = s->sc;
+    } else if (k == B_K4) {
+      ColsAb2Args &a = reinterpret_cast<ColsAb2Args *>(h)[j];
+      a = ColsAb2Args{};
+      a.Rc = s->T[2]; a.Rpx = s->T[3]; a.W0 = s->W[s->n0]; a.NL1 = s->NL[s->n1];
+      a.NL0 = s->T[4]; a.W1 = s->T[5];
+      a.s = s->dscal; a.explicit_step = 0; a.nblk = nb = s->H / tx; a.c = s->sc;
+    } else {
+      ColsPressArgs &a = reinterpret_cast<ColsPressArgs *>(h)[j];
+      a = ColsPressArgs{};
+      a.Rq = s->T[4]; a.Rqy = s->T[5]; a.Cq = s->T[6]; a.r0 = 0; a.r1 = s->ny - 1;
+      a.nblk = nb = s->H / tx; a.c = s->sc;
+    }
+    *nblk_max = std::max(*nblk_max, nb);
+  }
+  const size_t rec = (k == B_K1 || k == B_K5) ? sizeof(ColsInvArgs) : k == B_K4 ? sizeof(ColsAb2Args) : sizeof(ColsPressArgs);
+  return rec * nm;
+}
+
+// one launch of column stage k of group g with launch choice c over the
+// records at d (reps times)
+static int tune_launch(const fsibe_batch *b, const BatchGroup &g, int k, ColSel c, const char *d, int gx, int reps) {
+  const fsibe_sim *s0 = b->m[g.mem[0]];
+  const int nm = (int)g.mem.size();
+  if (k == B_K1 || k == B_K5)
+    return with_cols_sel<1>(s0->py, s0->H, c, [&](auto F, int nt, int TX) -> int {
+      const size_t lds = sizeof(double2) * TX * padlen(s0->ny);
+      CHK(prep_lds(kb_cols_inv_fields<decltype(F)>, lds));
+      for (int r = 0; r < reps; ++r)
+        kb_cols_inv_fields<decltype(F)><<<dim3(gx, nm), nt, lds, g.stream>>>(reinterpret_cast<const ColsInvArgs *>(d), F, s0->twy);
+      HIPCHK(hipGetLastError());
+      return FSIBE_OK;
+    });
+  if (k == B_K4)
+    return with_cols_sel<2>(s0->py, s0->H, c, [&](auto F, int nt, int TX) -> int {
+      const size_t lds = sizeof(double2) * 2 * TX * padlen(s0->ny);
+      CHK(prep_lds(kb_cols_ab2<decltype(F)>, lds));
+      for (int r = 0; r < reps; ++r)
+        kb_cols_ab2<decltype(F)><<<dim3(gx, nm), nt, lds, g.stream>>>(reinterpret_cast<const ColsAb2Args *>(d), F, s0->twy);
+      HIPCHK(hipGetLastError());
+      return FSIBE_OK;
+    });
+  return with_cols_sel<2>(s0->py, s0->H, c, [&](auto F, int nt, int TX) -> int {
+    const size_t lds = sizeof(double2) * 2 * TX * padlen(s0->ny);
+    CHK(prep_lds(kb_cols_press<decltype(F)>, lds));
+    for (int r = 0; r < reps; ++r)
+      kb_cols_press<decltype(F)><<<dim3(gx, nm), nt, lds, g.stream>>>(reinterpret_cast<const ColsPressArgs *>(d), F, s0->twy);
+    HIPCHK(hipGetLastError());
+    return FSIBE_OK;
+  });
+}
+
+// Autotuner of a batch's column stages (the FFT passes along y, fused with
+// their spectral operators).  For each size group and stage, every valid
+// launch choice (compile-time or runtime plan, 1 .. 16 columns per tile) is
+// timed on the group's own members with HIP events, two rounds of 4
+// launches each after a warm-up launch, and the fastest is kept; the single
+// path's choice stays unless another is more than 2 % faster.  No choice
+// changes a result (ColSel), so the members stay bit-identical to their own
+// runs whatever the measurements say.
+static int tune_batch(fsibe_batch *b) {
+  for (fsibe_sim *s : b->m) HIPCHK(hipStreamSynchronize(s->stream));
+  size_t bytes = 0;
+  for (auto &g : b->groups)
+    bytes = std::max(bytes, g.mem.size() * std::max(sizeof(ColsInvArgs), std::max(sizeof(ColsAb2Args), sizeof(ColsPressArgs))));
+  std::vector<char> h(bytes);
+  char *d = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FSIBE_OK;
+  if (hipMalloc(&d, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = set_err(FSIBE_EHIP, "batch autotuner: allocation failed");
+  for (size_t gi = 0; rc == FSIBE_OK && gi < b->groups.size(); ++gi) {
+    BatchGroup &g = b->groups[gi];
+    const fsibe_sim *s0 = b->m[g.mem[0]];
+    for (int k : kColStages) {
+      const int bm = (k == B_K4 || k == B_K7) ? 2 : 1;
+      std::vector<ColSel> cand;
+      for (int ct = 1; ct >= 0; --ct)
+        for (int tx = 1; tx <= 16; tx *= 2)
+          if (col_sel_valid(s0->ny, s0->H, bm, ColSel{ct, tx})) cand.push_back(ColSel{ct, tx});
+      std::vector<float> best(cand.size(), 1e30f);
+      for (int round = 0; rc == FSIBE_OK && round < 2; ++round)
+        for (size_t ci = 0; rc == FSIBE_OK && ci < cand.size(); ++ci) {
+          int gx = 0;
+          const size_t nb = tune_records(b, g, k, cand[ci].tx, h.data(), &gx);
+          if (hipStreamSynchronize(g.stream) != hipSuccess || hipMemcpy(d, h.data(), nb, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = set_err(FSIBE_EHIP, "batch autotuner: upload failed");
+            break;
+          }
+          rc = tune_launch(b, g, k, cand[ci], d, gx, 1);
+          if (rc != FSIBE_OK) break;
+          const int reps = 4;
+          float ms = 0.0f;
+          if (hipEventRecord(e0, g.stream) != hipSuccess) { rc = set_err(FSIBE_EHIP, "batch autotuner: event"); break; }
+          rc = tune_launch(b, g, k, cand[ci], d, gx, reps);
+          if (rc != FSIBE_OK) break;
+          if (hipEventRecord(e1, g.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = set_err(FSIBE_EHIP, "batch autotuner: timing failed");
+            break;
+          }
+          best[ci] = std::min(best[ci], 1000.0f * ms / reps);
+        }
+      if (rc != FSIBE_OK) break;
+      size_t bi = 0, si = 0;
+      for (size_t ci = 0; ci < cand.size(); ++ci) {
+        if (best[ci] < best[bi]) bi = ci;
+        if (cand[ci].ct == g.sel[k].ct && cand[ci].tx == g.sel[k].tx) si = ci;
+      }
+      if (!(best[bi] < 0.98f * best[si])) bi = si;
+      g.sel[k] = cand[bi];
+      g.tx[k] = cand[bi].tx;
+      g.tune_us[k] = best[bi];
+      g.base_us[k] = best[si];
+      g.ncand[k] = (int)cand.size();
+    }
+  }
+  if (d) (void)hipFree(d);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc == FSIBE_OK) b->tuned = true;
+  return rc;
 }
 
 extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const double *const *vortk,
@@ -2399,20 +2632,23 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
     nt = std::max(0, std::min(nt, n - 1));
     b->pool = new HostPool(nt);
   }
-  // launch shapes of the column stages (the same choices the single path makes)
+  // launch choices of the column stages: the single path's, then the
+  // autotuner's measurements
   for (auto &g : b->groups) {
     const fsibe_sim *s = b->m[g.mem[0]];
-    auto col = [&](int k, int BM, int rt_tx) {
-      int tx = 1, nt = 0;
-      int rc = BM == 1 ? with_cols<1>(s->py, s->H, rt_tx, [&](auto, int nt_, int TX) -> int { tx = TX; nt = nt_; return FSIBE_OK; })
-                       : with_cols<2>(s->py, s->H, rt_tx, [&](auto, int nt_, int TX) -> int { tx = TX; nt = nt_; return FSIBE_OK; });
-      g.tx[k] = tx; g.nt[k] = nt;
-      return rc;
-    };
-    int rc = col(B_K1, 1, pick_tx(s->H, s->ny, 1, 2));
-    if (rc == FSIBE_OK) rc = col(B_K4, 2, pick_tx(s->H, s->ny, 2, 1));
-    if (rc == FSIBE_OK) rc = col(B_K5, 1, pick_tx(s->H, s->ny, 1, g.nf5));
-    if (rc == FSIBE_OK) rc = col(B_K7, 2, pick_tx(s->H, s->ny, 2, 1));
+    g.sel[B_K1] = static_sel(s, 1, 2);
+    g.sel[B_K4] = static_sel(s, 2, 1);
+    g.sel[B_K5] = static_sel(s, 1, g.nf5);
+    g.sel[B_K7] = static_sel(s, 2, 1);
+    for (int k = 0; k < B_NST; ++k) {
+      g.tx[k] = g.sel[k].tx;
+      g.tune_us[k] = g.base_us[k] = 0.0f;
+      g.ncand[k] = 0;
+      g.cap[k] = 0;
+    }
+  }
+  {
+    const int rc = tune_batch(b);
     if (rc != FSIBE_OK) return fail(rc);
   }
   *out = b;
@@ -2521,14 +2757,170 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
   }
 }
 
-#define BLAUNCH(KT, gx, gy, nt, lds, st, ...)                      \
+// how batch_stages launches: eager (set the LDS limit, then launch), only
+// set the LDS limits (before a capture), or only launch (under capture)
+enum StageLaunch { SL_EAGER, SL_PREP, SL_CAPTURE };
+
+#define SLAUNCH(KT, gx, gy, nt, lds, st, mode, ...)                \
   do {                                                              \
     if ((gx) > 0) {                                                 \
-      CHK(prep_lds(KT, lds));                                       \
-      KT<<<dim3((gx), (gy)), nt, lds, st>>>(__VA_ARGS__);           \
-      HIPCHK(hipGetLastError());                                    \
+      if ((mode) != SL_CAPTURE) CHK(prep_lds(KT, lds));             \
+      if ((mode) != SL_PREP) {                                      \
+        KT<<<dim3((gx), (gy)), nt, lds, st>>>(__VA_ARGS__);         \
+        HIPCHK(hipGetLastError());                                  \
+      }                                                             \
     }                                                               \
   } while (0)
+
+// stages [k0, k1) of every group, stage by stage, each group on its own
+// stream, with grid.x = grids[group][stage] (0: the stage is skipped)
+static int batch_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>> &grids, int k0, int k1,
+                        StageLaunch mode, int only = -1) {
+  auto arr = [&](const BatchGroup &g, int k) { return b->d_args + g.off[k]; };
+  for (int k = k0; k < k1; ++k)
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+      if (only >= 0 && (int)gi != only) continue;
+      const BatchGroup &g = b->groups[gi];
+      const fsibe_sim *s0 = b->m[g.mem[0]];
+      const int nm = (int)g.mem.size();
+      const int gx = grids[gi][k];
+      if (gx <= 0) continue;
+      switch (k) {
+        case B_MASK:
+          SLAUNCH(kb_mask, gx, nm, kMaskThreads, 0, g.stream, mode, reinterpret_cast<const MaskArgs *>(arr(g, B_MASK)));
+          break;
+        case B_K1:
+        case B_K5:
+          CHK(with_cols_sel<1>(s0->py, s0->H, g.sel[k], [&](auto F, int nt, int TX) -> int {
+            SLAUNCH(kb_cols_inv_fields<decltype(F)>, gx, nm, nt, sizeof(double2) * TX * padlen(s0->ny), g.stream, mode,
+                    reinterpret_cast<const ColsInvArgs *>(arr(g, k)), F, s0->twy);
+            return FSIBE_OK;
+          }));
+          break;
+        case B_K2:
+          CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
+            SLAUNCH(kb_rows_nl<decltype(F)>, gx, nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream, mode,
+                    reinterpret_cast<const RowsNlArgs *>(arr(g, B_K2)), F, s0->twx);
+            return FSIBE_OK;
+          }));
+          break;
+        case B_K4:
+          CHK(with_cols_sel<2>(s0->py, s0->H, g.sel[B_K4], [&](auto F, int nt, int TX) -> int {
+            SLAUNCH(kb_cols_ab2<decltype(F)>, gx, nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny), g.stream, mode,
+                    reinterpret_cast<const ColsAb2Args *>(arr(g, B_K4)), F, s0->twy);
+            return FSIBE_OK;
+          }));
+          break;
+        case B_K6:
+          CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
+            SLAUNCH(kb_rows_press<decltype(F)>, gx, nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream, mode,
+                    reinterpret_cast<const RowsPressArgs *>(arr(g, B_K6)), F, s0->twx);
+            return FSIBE_OK;
+          }));
+          break;
+        case B_K7:
+          CHK(with_cols_sel<2>(s0->py, s0->H, g.sel[B_K7], [&](auto F, int nt, int TX) -> int {
+            SLAUNCH(kb_cols_press<decltype(F)>, gx, nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny), g.stream, mode,
+                    reinterpret_cast<const ColsPressArgs *>(arr(g, B_K7)), F, s0->twy);
+            return FSIBE_OK;
+          }));
+          break;
+        case B_K8:
+          CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
+            SLAUNCH(kb_rows_box<decltype(F)>, gx, nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream, mode,
+                    reinterpret_cast<const RowsBoxArgs *>(arr(g, B_K8)), F, s0->twx);
+            return FSIBE_OK;
+          }));
+          break;
+        case B_K10:
+          SLAUNCH(kb_forces, gx, nm, kForceThreads, 0, g.stream, mode, reinterpret_cast<const ForcesArgs *>(arr(g, B_K10)));
+          break;
+        default:
+          break;
+      }
+    }
+  return FSIBE_OK;
+}
+
+// the argument records and every member's geometry, copied by kernels on
+// the upload stream (no copy-engine hand-off before the first stage)
+static int batch_gather(fsibe_batch *b) {
+  std::vector<StageSeg> segs;
+  segs.push_back(StageSeg{reinterpret_cast<const double2 *>(b->m_args), reinterpret_cast<double2 *>(b->d_args),
+                          (long long)(b->args_bytes / sizeof(double2))});
+  for (fsibe_sim *s : b->m)
+    segs.push_back(StageSeg{reinterpret_cast<const double2 *>(s->m_stage), reinterpret_cast<double2 *>(s->d_stage),
+                            (long long)(s->off_val / sizeof(double2))});
+  for (size_t k0 = 0; k0 < segs.size(); k0 += kMaxStageSegs) {
+    StageSegs ss{};
+    const int ns = (int)std::min(segs.size() - k0, (size_t)kMaxStageSegs);
+    for (int k = 0; k < ns; ++k) ss.s[k] = segs[k0 + k];
+    k_stage_gather<<<dim3(8, ns), 256, 0, b->stream>>>(ss);
+    HIPCHK(hipGetLastError());
+  }
+  return FSIBE_OK;
+}
+
+// Captures part 0 (mask, K1, K2) or part 1 (K4 .. K10) of one group's
+// step into a graph with the grids g.cap.  Every stage reads its argument
+// records from fixed device addresses (refreshed by the gather kernel each
+// step), so one capture serves every step whose blocks fit the captured
+// grids.  Kernel nodes only, in one chain on the group's stream.
+static int capture_part(fsibe_batch *b, int gi, int part) {
+  BatchGroup &g = b->groups[gi];
+  const int k0 = part == 0 ? B_MASK : B_K4, k1 = part == 0 ? B_K4 : B_NST;
+  std::vector<std::array<int, B_NST>> caps(b->groups.size());
+  for (int k = 0; k < B_NST; ++k) caps[gi][k] = g.cap[k];
+  CHK(batch_stages(b, caps, k0, k1, SL_PREP, gi));
+  HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeRelaxed));
+  const int rc = batch_stages(b, caps, k0, k1, SL_CAPTURE, gi);
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(g.stream, &gr);
+  if (rc != FSIBE_OK || e != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc != FSIBE_OK ? rc : set_err(FSIBE_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  if (e2 != hipSuccess) {
+    (void)hipGraphDestroy(gr);
+    return set_err(FSIBE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+  }
+  if (g.gexec[part]) (void)hipGraphExecDestroy(g.gexec[part]);
+  if (g.graph[part]) (void)hipGraphDestroy(g.graph[part]);
+  g.graph[part] = gr;
+  g.gexec[part] = ex;
+  b->captures++;
+  return FSIBE_OK;
+}
+
+// Launches a step's stages from the groups' graphs (after the gather, which
+// the caller has launched).  A stage whose blocks no longer fit its captured
+// grid (the mask box, the force window and the beam's force units grow and
+// shrink as the beam moves) is captured again with a quarter more room;
+// blocks beyond a member's own count return at once.
+static int batch_graph_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>> &need) {
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    BatchGroup &g = b->groups[gi];
+    bool fits[2] = {g.gexec[0] != nullptr, g.gexec[1] != nullptr};
+    for (int k = 0; k < B_NST; ++k)
+      if (need[gi][k] > g.cap[k]) {
+        const bool grows = k == B_MASK || k == B_K8 || k == B_K10;
+        g.cap[k] = grows ? need[gi][k] + need[gi][k] / 4 : need[gi][k];
+        fits[k < B_K4 ? 0 : 1] = false;
+      }
+    for (int part = 0; part < 2; ++part)
+      if (!fits[part]) CHK(capture_part(b, (int)gi, part));
+  }
+  for (auto &g : b->groups) {
+    HIPCHK(hipGraphLaunch(g.gexec[0], g.stream));
+    if (!b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+    HIPCHK(hipGraphLaunch(g.gexec[1], g.stream));
+    if (b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+  }
+  b->replays++;
+  return FSIBE_OK;
+}
 
 // one AB2 step of every member
 static int batch_step(fsibe_batch *b) {
@@ -2558,114 +2950,29 @@ static int batch_step(fsibe_batch *b) {
     for (size_t j = 0; j < g.mem.size(); ++j) fill_batch_args(b, g, (int)j, b->m[g.mem[j]], mx);
     for (int k = 0; k < B_NST; ++k) maxblk[gi][k] = mx[k];
   }
-  // the argument records and every member's geometry, copied by kernels on
-  // the upload stream (no copy-engine hand-off before the first stage)
-  {
-    std::vector<StageSeg> segs;
-    segs.push_back(StageSeg{reinterpret_cast<const double2 *>(b->m_args), reinterpret_cast<double2 *>(b->d_args),
-                            (long long)(b->args_bytes / sizeof(double2))});
-    for (fsibe_sim *s : b->m)
-      segs.push_back(StageSeg{reinterpret_cast<const double2 *>(s->m_stage), reinterpret_cast<double2 *>(s->d_stage),
-                              (long long)(s->off_val / sizeof(double2))});
-    for (size_t k0 = 0; k0 < segs.size(); k0 += kMaxStageSegs) {
-      StageSegs ss{};
-      const int ns = (int)std::min(segs.size() - k0, (size_t)kMaxStageSegs);
-      for (int k = 0; k < ns; ++k) ss.s[k] = segs[k0 + k];
-      k_stage_gather<<<dim3(8, ns), 256, 0, b->stream>>>(ss);
-      HIPCHK(hipGetLastError());
-    }
-  }
+  CHK(batch_gather(b));
   HIPCHK(hipEventRecord(b->ev_args, b->stream));
   for (auto &g : b->groups) HIPCHK(hipStreamWaitEvent(g.stream, b->ev_args, 0));
-  auto arr = [&](const BatchGroup &g, int k) { return b->d_args + g.off[k]; };
-  // stage by stage, each stage for every group
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    BLAUNCH(kb_mask, maxblk[gi][B_MASK], (int)g.mem.size(), kMaskThreads, 0, g.stream,
-            reinterpret_cast<const MaskArgs *>(arr(g, B_MASK)));
+  if (b->use_graph) {
+    CHK(batch_graph_stages(b, maxblk));
+  } else {
+    CHK(batch_stages(b, maxblk, B_MASK, B_K4, SL_EAGER));
+    if (!b->fsi)
+      for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+    CHK(batch_stages(b, maxblk, B_K4, B_NST, SL_EAGER));
+    if (b->fsi)
+      for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
   }
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    const fsibe_sim *s0 = b->m[g.mem[0]];
-    const int nm = (int)g.mem.size();
-    CHK(with_cols<1>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 1, 2), [&](auto F, int nt, int TX) -> int {
-      BLAUNCH(kb_cols_inv_fields<decltype(F)>, maxblk[gi][B_K1], nm, nt, sizeof(double2) * TX * padlen(s0->ny),
-              g.stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K1)), F, s0->twy);
-      return FSIBE_OK;
-    }));
-  }
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    const fsibe_sim *s0 = b->m[g.mem[0]];
-    const int nm = (int)g.mem.size();
-    CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
-      BLAUNCH(kb_rows_nl<decltype(F)>, maxblk[gi][B_K2], nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream,
-              reinterpret_cast<const RowsNlArgs *>(arr(g, B_K2)), F, s0->twx);
-      return FSIBE_OK;
-    }));
-  }
-  if (!b->fsi)
-    for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    const fsibe_sim *s0 = b->m[g.mem[0]];
-    const int nm = (int)g.mem.size();
-    CHK(with_cols<2>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
-      BLAUNCH(kb_cols_ab2<decltype(F)>, maxblk[gi][B_K4], nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny),
-              g.stream, reinterpret_cast<const ColsAb2Args *>(arr(g, B_K4)), F, s0->twy);
-      return FSIBE_OK;
-    }));
-  }
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    const fsibe_sim *s0 = b->m[g.mem[0]];
-    const int nm = (int)g.mem.size();
-    CHK(with_cols<1>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 1, g.nf5), [&](auto F, int nt, int TX) -> int {
-      BLAUNCH(kb_cols_inv_fields<decltype(F)>, maxblk[gi][B_K5], nm, nt, sizeof(double2) * TX * padlen(s0->ny),
-              g.stream, reinterpret_cast<const ColsInvArgs *>(arr(g, B_K5)), F, s0->twy);
-      return FSIBE_OK;
-    }));
-  }
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    const fsibe_sim *s0 = b->m[g.mem[0]];
-    const int nm = (int)g.mem.size();
-    CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
-      BLAUNCH(kb_rows_press<decltype(F)>, maxblk[gi][B_K6], nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream,
-              reinterpret_cast<const RowsPressArgs *>(arr(g, B_K6)), F, s0->twx);
-      return FSIBE_OK;
-    }));
-  }
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    const fsibe_sim *s0 = b->m[g.mem[0]];
-    const int nm = (int)g.mem.size();
-    CHK(with_cols<2>(s0->py, s0->H, pick_tx(s0->H, s0->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
-      BLAUNCH(kb_cols_press<decltype(F)>, maxblk[gi][B_K7], nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny),
-              g.stream, reinterpret_cast<const ColsPressArgs *>(arr(g, B_K7)), F, s0->twy);
-      return FSIBE_OK;
-    }));
-  }
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    const fsibe_sim *s0 = b->m[g.mem[0]];
-    const int nm = (int)g.mem.size();
-    CHK(with_rows<3>(s0->px, [&](auto F, int nt) -> int {
-      BLAUNCH(kb_rows_box<decltype(F)>, maxblk[gi][B_K8], nm, nt, sizeof(double2) * 3 * padlen(s0->H), g.stream,
-              reinterpret_cast<const RowsBoxArgs *>(arr(g, B_K8)), F, s0->twx);
-      return FSIBE_OK;
-    }));
-  }
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    const BatchGroup &g = b->groups[gi];
-    BLAUNCH(kb_forces, maxblk[gi][B_K10], (int)g.mem.size(), kForceThreads, 0, g.stream,
-            reinterpret_cast<const ForcesArgs *>(arr(g, B_K10)));
-  }
+  // the host waits for every group's nonlinear term (prescribed motion) or
+  // for the whole step (solved beams)
+  auto wait_groups = [&]() -> int {
+    for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
+    return FSIBE_OK;
+  };
   if (b->fsi) {
     // the beams of step n+1 need the loads of step n: wait for every group,
     // then each member reads its results, and its beam is solved (or moved)
-    for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-    for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
+    CHK(wait_groups());
     b->pool->run((int)b->m.size(), [&](int i) {
       fsibe_sim *s = b->m[i];
       s->upload_pending = false;
@@ -2690,7 +2997,7 @@ static int batch_step(fsibe_batch *b) {
     return FSIBE_OK;
   }
   // dt1 of every member, then the previous step's results
-  for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
+  CHK(wait_groups());
   const bool pending = b->res_pending;
   b->pool->run((int)b->m.size(), [&](int i) {
     fsibe_sim *s = b->m[i];
@@ -2748,6 +3055,100 @@ extern "C" int fsibe_batch_size(fsibe_batch *b) { return b ? (int)b->m.size() : 
 extern "C" fsibe_sim *fsibe_batch_member(fsibe_batch *b, int i) {
   if (!b || i < 0 || i >= (int)b->m.size()) return nullptr;
   return b->m[i];
+}
+
+// Step mode and autotuning of a batch.  use_graph: 1 replays captured
+// graphs, 0 launches every stage eagerly, < 0 keeps the mode.  autotune: 1
+// times the column stages' launch choices again and keeps the fastest, 0
+// returns to the single path's choices, < 0 keeps them.  Neither changes a
+// result.
+extern "C" int fsibe_batch_configure(fsibe_batch *b, int use_graph, int autotune) {
+  if (!b) return set_err(FSIBE_EARG, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (auto &g : b->groups) HIPCHK(hipStreamSynchronize(g.stream));
+  if (autotune >= 0) {
+    drop_graphs(b);
+    for (auto &g : b->groups) {
+      const fsibe_sim *s = b->m[g.mem[0]];
+      g.sel[B_K1] = static_sel(s, 1, 2);
+      g.sel[B_K4] = static_sel(s, 2, 1);
+      g.sel[B_K5] = static_sel(s, 1, g.nf5);
+      g.sel[B_K7] = static_sel(s, 2, 1);
+      for (int k = 0; k < B_NST; ++k) {
+        g.tx[k] = g.sel[k].tx;
+        g.tune_us[k] = g.base_us[k] = 0.0f;
+        g.ncand[k] = 0;
+        g.cap[k] = 0;
+      }
+    }
+    b->tuned = false;
+    if (autotune > 0) CHK(tune_batch(b));
+  }
+  if (use_graph >= 0) b->use_graph = use_graph != 0;
+  return FSIBE_OK;
+}
+
+// Pins the launch choice of one column stage of one size group (stage 1,
+// 4, 5 or 7 for K1, K4, K5, K7; groups in the order fsibe_batch_describe
+// lists them): a compile-time plan (ct = 1) or the runtime plan (ct = 0)
+// with tx columns per tile.  For reusing measured choices without
+// measuring again; no choice changes a result.
+extern "C" int fsibe_batch_set_tiles(fsibe_batch *b, int group, int stage, int ct, int tx) {
+  if (!b || group < 0 || group >= (int)b->groups.size()) return set_err(FSIBE_EARG, "bad batch or group");
+  const int k = stage == 1 ? B_K1 : stage == 4 ? B_K4 : stage == 5 ? B_K5 : stage == 7 ? B_K7 : -1;
+  if (k < 0) return set_err(FSIBE_EARG, "stage must be 1, 4, 5 or 7");
+  BatchGroup &g = b->groups[group];
+  const int bm = (k == B_K4 || k == B_K7) ? 2 : 1;
+  const ColSel c{ct ? 1 : 0, tx};
+  if (!col_sel_valid(g.ny, g.H, bm, c))
+    return set_err(FSIBE_EARG, "%s plan with %d columns per tile is not a launch choice for %d x %d",
+                   c.ct ? "compile-time" : "runtime", tx, g.nx, g.ny);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (auto &q : b->groups) HIPCHK(hipStreamSynchronize(q.stream));
+  drop_graphs(b);
+  for (auto &q : b->groups)
+    for (int j = 0; j < B_NST; ++j) q.cap[j] = 0;
+  g.sel[k] = c;
+  g.tx[k] = tx;
+  g.tune_us[k] = g.base_us[k] = 0.0f;
+  return FSIBE_OK;
+}
+
+// The batch's groups, launch choices and step mode as one JSON object.
+extern "C" int fsibe_batch_describe(fsibe_batch *b, char *buf, size_t len) {
+  if (!b || !buf || len == 0) return set_err(FSIBE_EARG, "bad argument");
+  static const char *kName[B_NST] = {"mask", "K1", "K2", "K4", "K5", "K6", "K7", "K8", "K10"};
+  std::string o;
+  char t[512];
+  snprintf(t, sizeof t, "{\"graph\": %d, \"tuned\": %d, \"captures\": %d, \"graph_steps\": %d, \"groups\": [",
+           b->use_graph ? 1 : 0, b->tuned ? 1 : 0, b->captures, b->replays);
+  o += t;
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    const BatchGroup &g = b->groups[gi];
+    snprintf(t, sizeof t, "%s{\"nx\": %d, \"ny\": %d, \"members\": %d, \"column_stages\": {", gi ? ", " : "", g.nx,
+             g.ny, (int)g.mem.size());
+    o += t;
+    for (int q = 0; q < 4; ++q) {
+      const int k = kColStages[q];
+      snprintf(t, sizeof t,
+               "%s\"%s\": {\"plan\": \"%s\", \"tx\": %d, \"us\": %.2f, \"static_us\": %.2f, \"candidates\": %d}",
+               q ? ", " : "", kName[k], g.sel[k].ct ? "compile-time" : "runtime", g.sel[k].tx, g.tune_us[k],
+               g.base_us[k], g.ncand[k]);
+      o += t;
+    }
+    o += "}, \"graph_grids\": {";
+    for (int k = 0; k < B_NST; ++k) {
+      snprintf(t, sizeof t, "%s\"%s\": %d", k ? ", " : "", kName[k], g.cap[k]);
+      o += t;
+    }
+    o += "}}";
+  }
+  o += "]}";
+  const size_t n = std::min(o.size(), len - 1);
+  memcpy(buf, o.data(), n);
+  buf[n] = '\0';
+  if (o.size() >= len) return set_err(FSIBE_ESIZE, "batch description needs %zu bytes", o.size() + 1);
+  return FSIBE_OK;
 }
 
 // ---- discrete adjoint (fsibe_adjoint_*) -------------------------------------

@@ -176,6 +176,15 @@ class FluidSim:
     def stream(self):
         return lib().fsibe_sim_stream(self.h)
 
+    @property
+    def solid(self):
+        """the solid solver fsibe_run uses for iFLUSI=1 (fsibe_sim_solid)"""
+        from .solid import Solid
+
+        h = C.c_void_p()
+        check(lib().fsibe_sim_solid(self.h, C.byref(h)))
+        return Solid(self.p, handle=h.value, owner=self)
+
 
 def save_field(path, field, nsave: int = 1, xl: float = 4.0, yl: float = 4.0, fieldtype: str = "vorticity",
                header: str | None = None):
@@ -185,15 +194,6 @@ def save_field(path, field, nsave: int = 1, xl: float = 4.0, yl: float = 4.0, fi
     ny, nx = f.shape
     check(lib().fsibe_save_field(os.fsencode(path), f, nx, ny, int(nsave), float(xl), float(yl),
                                  fieldtype.encode(), header.encode() if header is not None else None))
-
-    @property
-    def solid(self):
-        """the solid solver fsibe_run uses for iFLUSI=1 (fsibe_sim_solid)"""
-        from .solid import Solid
-
-        h = C.c_void_p()
-        check(lib().fsibe_sim_solid(self.h, C.byref(h)))
-        return Solid(self.p, handle=h.value, owner=self)
 
 
 class _Member(FluidSim):
@@ -211,9 +211,14 @@ class _Member(FluidSim):
 class Batch:
     """Independent simulations advanced together: every AB2 step is one
     launch per stage and grid size (fsibe_batch_*).  Member i behaves like a
-    FluidSim with params[i] run alone."""
+    FluidSim with params[i] run alone.
 
-    def __init__(self, params, vortk=None):
+    graph: replay captured HIP graphs (True, the library's default) or
+    launch every stage (False); autotune: measure the column stages' launch
+    choices (True, done at creation by default) or use the single path's
+    (False).  Neither changes a result."""
+
+    def __init__(self, params, vortk=None, graph=None, autotune=None):
         n = len(params)
         self.params = list(params)
         arr = (FsibeParams * n)(*[p.to_c() for p in params])
@@ -230,9 +235,32 @@ class Batch:
         self.h = h
         self._keep = []
         self.members = [_Member(params[i], lib().fsibe_batch_member(h, i)) for i in range(n)]
+        if graph is not None or autotune is not None:
+            self.configure(graph, autotune)
 
     def run(self, nsteps: int):
         check(lib().fsibe_batch_run(self.h, int(nsteps)))
+
+    def configure(self, graph=None, autotune=None):
+        """graph / autotune: True, False or None (unchanged); autotune=True
+        measures again"""
+        g = -1 if graph is None else int(bool(graph))
+        a = -1 if autotune is None else int(bool(autotune))
+        check(lib().fsibe_batch_configure(self.h, g, a))
+
+    def set_tiles(self, group: int, stage: int, ct: bool, tx: int):
+        """pin column stage `stage` (1, 4, 5, 7) of size group `group` to the
+        compile-time (ct) or runtime FFT plan with tx columns per tile"""
+        check(lib().fsibe_batch_set_tiles(self.h, int(group), int(stage), int(bool(ct)), int(tx)))
+
+    def describe(self) -> dict:
+        """groups, launch choices (with the autotuner's times in us) and the
+        step mode"""
+        import json
+
+        buf = C.create_string_buffer(1 << 16)
+        check(lib().fsibe_batch_describe(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
 
     def close(self):
         if getattr(self, "h", None):

@@ -316,6 +316,24 @@ int fsibe_batch_destroy(fsibe_batch *batch);
 int fsibe_batch_run(fsibe_batch *batch, int nsteps);
 int fsibe_batch_size(fsibe_batch *batch);
 fsibe_sim *fsibe_batch_member(fsibe_batch *batch, int i);
+/* Step mode and tile autotuning (no reference counterpart).  At creation a
+ * batch times every valid launch choice of its column stages (compile-time
+ * or runtime FFT plan, 1..16 columns per tile) per grid size on its own
+ * members and keeps the fastest, and each size group advances by replaying
+ * two captured HIP graphs per step on its own stream (the stages up to the
+ * nonlinear term, then the rest).
+ * use_graph: 1 graphs, 0 eager launches, < 0 unchanged; autotune: 1 measure
+ * again, 0 the single-simulation path's choices, < 0 unchanged.  No setting
+ * changes a result: every choice runs the same passes on every column. */
+int fsibe_batch_configure(fsibe_batch *batch, int use_graph, int autotune);
+/* pin the launch choice of column stage `stage` (1, 4, 5 or 7: K1, K4, K5,
+ * K7) of size group `group` (the order of fsibe_batch_describe): the
+ * compile-time FFT plan (ct = 1) or the runtime plan (ct = 0) with tx
+ * columns per tile; FSIBE_EARG for a choice that size cannot launch */
+int fsibe_batch_set_tiles(fsibe_batch *batch, int group, int stage, int ct, int tx);
+/* the groups, the chosen launch shapes with their measured times, and the
+ * step mode, as a NUL-terminated JSON object (FSIBE_ESIZE if len is short) */
+int fsibe_batch_describe(fsibe_batch *batch, char *buf, size_t len);
 
 /* ---- discrete adjoint ---------------------------------------------------
  * No reference counterpart (the reference has no adjoint).  For a

@@ -130,3 +130,102 @@ def test_batch_with_solved_beams(oracle_mod):
             assert np.array_equal(m.solid.get("theta_dot"), g.solid.get("theta_dot"))
         g.close()
     b.close()
+
+
+@pytest.mark.parametrize("coupled", [False, True])
+def test_graph_replays_match_eager_launches(coupled, oracle_mod):
+    """the batch step replayed from captured HIP graphs against the same
+    batch launching every stage, step by step: after every replay (with the
+    host's reads and synchronisations between replays) each member is
+    bit-identical, and a later call of several steps too"""
+    from fsibe_2d_amd import batch_configs
+    from fsibe_2d_amd.solver import Batch
+
+    ps = batch_configs(6, coupled=coupled)
+    vks = [_noise(oracle_mod, p, 40 + i) for i, p in enumerate(ps)]
+    a = Batch(ps, vks)
+    e = Batch(ps, vks, graph=False)
+    assert a.describe()["graph"] == 1 and e.describe()["graph"] == 0
+    for call in range(7):
+        n = 1 if call < 5 else 3
+        a.run(n)
+        e.run(n)
+        for i in range(len(ps)):
+            ma, me = a.members[i], e.members[i]
+            assert ma.time_state() == me.time_state(), (call, i)
+            assert np.array_equal(ma.field("vortk"), me.field("vortk")), (call, i)
+            so, eo = ma.step_out(), me.step_out()
+            for k in ("dt1", "Force", "Force_unst", "Force_press"):
+                assert so[k] == eo[k], (call, i, k)
+            assert np.array_equal(ma.beam()["theta"], me.beam()["theta"]), (call, i)
+    d = a.describe()
+    print(f"[batch graph coupled={coupled}] captures {d['captures']} graph steps {d['graph_steps']}", flush=True)
+    assert d["graph_steps"] == 4 + 2 * 3
+    # two graphs per size group, captured again only when a stage outgrows them
+    assert 2 * len(d["groups"]) <= d["captures"] <= 8 * len(d["groups"])
+    a.close()
+    e.close()
+
+
+def test_every_column_tile_choice_gives_identical_results(oracle_mod):
+    """every launch choice the autotuner may pick (compile-time or runtime
+    FFT plan, 1 .. 16 columns per tile) for every column stage of the four
+    grid sizes of the batched workload: the members stay bit-identical to
+    the single path's choices"""
+    from fsibe_2d_amd import FsibeError, Params
+    from fsibe_2d_amd.solver import Batch
+
+    ps = [Params(nx=n, ny=n, ns=48, dt_fixed=2e-4 * 256 / n, nu=2e-3) for n in (256, 384, 512, 768)]
+    vks = [_noise(oracle_mod, p, 60 + i) for i, p in enumerate(ps)]
+    steps = 3
+    ref = Batch(ps, vks, autotune=False)
+    ref.run(steps)
+    want = [m.field("vortk") for m in ref.members]
+    f_want = [m.step_out()["Force"] for m in ref.members]
+    groups = ref.describe()["groups"]
+    ref.close()
+    assert [g["nx"] for g in groups] == [256, 384, 512, 768]
+    tried = 0
+    for ct in (True, False):
+        for tx in (1, 2, 4, 8, 16):
+            b = Batch(ps, vks)
+            pinned = 0
+            for gi in range(len(groups)):
+                for stage in (1, 4, 5, 7):
+                    try:
+                        b.set_tiles(gi, stage, ct, tx)
+                        pinned += 1
+                    except FsibeError:
+                        pass
+            if pinned:
+                b.run(steps)
+                for i, m in enumerate(b.members):
+                    assert np.array_equal(m.field("vortk"), want[i]), (ct, tx, i)
+                    assert m.step_out()["Force"] == f_want[i], (ct, tx, i)
+                tried += pinned
+            print(f"[tiles ct={ct} tx={tx}] {pinned} stages pinned", flush=True)
+            b.close()
+    assert tried >= 40
+
+
+def test_autotuner_report_and_refusals(oracle_mod):
+    from fsibe_2d_amd import FsibeError, batch_configs
+    from fsibe_2d_amd.solver import Batch
+
+    ps = batch_configs(8)
+    b = Batch(ps, [None] * len(ps))
+    d = b.describe()
+    assert d["tuned"] == 1
+    for g in d["groups"]:
+        for k in ("K1", "K4", "K5", "K7"):
+            st = g["column_stages"][k]
+            assert st["candidates"] >= 3 and st["us"] > 0 and st["us"] <= st["static_us"], (g["nx"], k, st)
+    with pytest.raises(FsibeError):
+        b.set_tiles(0, 3, True, 1)  # not a column stage
+    with pytest.raises(FsibeError):
+        b.set_tiles(len(d["groups"]), 1, True, 1)
+    with pytest.raises(FsibeError):
+        b.set_tiles(0, 1, True, 8)  # no compile-time plan of that width
+    b.configure(autotune=False)
+    assert b.describe()["tuned"] == 0
+    b.close()
