@@ -95,6 +95,63 @@ __device__ void rows_c2r_pre(double2 *buf, int nrows, int H, int PL, const doubl
   __syncthreads();
 }
 
+// rows_c2r_pre fused into the staging of the rows: each thread takes the
+// pair (k, H-k) of every row, reads both half-spectrum entries through
+// val(f, k) (global loads and the field's own spectral factor) and writes
+// the pair's pre-processed values, as rows_c2r_pre computes them, to row f
+// at buf + f*PL.  One LDS write per element instead of a write, a read and
+// a write, and one barrier fewer.
+template <int NFMAX, class Val>
+__device__ __forceinline__ void stage_rows_c2r(double2 *buf, int nf, int H, int PL, const double2 *__restrict__ twx,
+                                               int tid, int nthr, Val val) {
+  const int hp = (H >> 1) + 1;
+  for (int k = tid; k < hp; k += nthr) {
+    if (k == 0) {
+#pragma unroll
+      for (int f = 0; f < NFMAX; ++f)
+        if (f < nf) {
+          const double a = val(f, 0).x;  // x-DC: real part only, x-Nyquist is zero
+          buf[f * PL] = make_double2(a, a);
+        }
+      continue;
+    }
+    const int q = H - k;
+    const double2 wk = cconj(twx[k]), wq = cconj(twx[q]);
+#pragma unroll
+    for (int f = 0; f < NFMAX; ++f)
+      if (f < nf) {
+        const double2 a = val(f, k), b = q != k ? val(f, q) : a;
+        double2 *X = buf + f * PL;
+        if (q != k) X[pidx(q)] = c2r_one(b, a, wq);
+        X[pidx(k)] = c2r_one(a, b, wk);
+      }
+  }
+  __syncthreads();
+}
+
+// rows_r2c_post fused into the final stores: each thread takes the pair
+// (k, H-k) of the NF rows, forms their spectra as rows_r2c_post does and
+// hands them to out(k, x[NF]) and out(H-k, x[NF]) (once for k = 0 and
+// k = H/2).  No LDS writes and no barrier after the last pass.
+template <int NF, class Out>
+__device__ __forceinline__ void rows_r2c_store(const double2 *buf, int H, int PL, const double2 *__restrict__ twx,
+                                               int tid, int nthr, Out out) {
+  const int hp = (H >> 1) + 1;
+  for (int k = tid; k < hp; k += nthr) {
+    const int q = (k == 0) ? 0 : H - k;
+    const double2 wk = twx[k], wq = twx[q];
+    double2 xk[NF], xq[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const double2 a = buf[f * PL + pidx(k)], b = buf[f * PL + pidx(q)];
+      xk[f] = r2c_one(a, b, wk);
+      xq[f] = r2c_one(b, a, wq);
+    }
+    out(k, xk);
+    if (q != k) out(q, xq);
+  }
+}
+
 // ============================================================================
 // Tier A kernels (packed layout at the boundary)
 // ============================================================================

@@ -318,16 +318,14 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const bool prow = penal_on && mb.x1 >= mb.x0 && y >= mb.y0 && y <= mb.y1;
   double ps1 = 0.0, ps2 = 0.0;  // the row's sums of the penalisation term (rowpen)
-  for (int k = tid; k < H; k += nt) {
+  // rows: w, dw/dy, dw/dx = i kx w
+  stage_rows_c2r<3>(lds, 3, H, PL, twx, tid, nt, [&](int f, int k) -> double2 {
+    if (f == 1) return B[(size_t)y * H + k];
     const double2 a = A[(size_t)y * H + k];
+    if (f == 0) return a;
     const double kf = (double)k * c.sx;
-    const int q = pidx(k);
-    f0[q] = a;
-    f1[q] = B[(size_t)y * H + k];
-    f2[q] = make_double2(-kf * a.y, kf * a.x);
-  }
-  __syncthreads();
-  rows_c2r_pre(lds, 3, H, PL, twx, tid, nt);
+    return make_double2(-kf * a.y, kf * a.x);
+  });
   F.template fft<true, 3>(lds, twx, tid, nt);
   for (int n = tid; n < H; n += nt) {
     const int q = pidx(n);
@@ -374,14 +372,12 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
   }
   __syncthreads();
   F.template fft<false, 3>(lds, twx, tid, nt);
-  rows_r2c_post(lds, 3, H, PL, twx, tid, nt);
-  for (int k = tid; k < H; k += nt) {
-    const int q = pidx(k);
-    const double2 xa = f0[q], xpx = f1[q], xpy = f2[q];
+  rows_r2c_store<3>(lds, H, PL, twx, tid, nt, [&](int k, const double2 *x) {
+    const double2 xa = x[0], xpx = x[1], xpy = x[2];
     const double kf = (double)k * c.sx;
     Rc[(size_t)y * H + k] = make_double2(xa.x - kf * xpy.y, xa.y + kf * xpy.x);
     Rpx[(size_t)y * H + k] = xpx;
-  }
+  });
 }
 
 // ============================================================================
@@ -469,16 +465,15 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const int nf = mode ? 3 : 2;
   double2 *g1 = mode ? f1 : f0, *g2 = mode ? f2 : f1;  // psi_x, psi_y slots
-  for (int k = tid; k < H; k += nt) {
+  // rows: (w,) dpsi/dx = i kx psi, dpsi/dy
+  stage_rows_c2r<3>(lds, nf, H, PL, twx, tid, nt, [&](int f, int k) -> double2 {
+    const int g = mode ? f : f + 1;
+    if (g == 0) return Cw[(size_t)y * H + k];
+    if (g == 2) return Cpsiy[(size_t)y * H + k];
     const double kf = (double)k * c.sx;
     const double2 p = Cpsi[(size_t)y * H + k];
-    const int q = pidx(k);
-    if (mode) f0[q] = Cw[(size_t)y * H + k];
-    g1[q] = make_double2(-kf * p.y, kf * p.x);
-    g2[q] = Cpsiy[(size_t)y * H + k];
-  }
-  __syncthreads();
-  rows_c2r_pre(lds, nf, H, PL, twx, tid, nt);
+    return make_double2(-kf * p.y, kf * p.x);
+  });
   if (mode) F.template fft<true, 3>(lds, twx, tid, nt);
   else F.template fft<true, 2>(lds, twx, tid, nt);
   double UMx = UMx0, UMy = UMy0, Mx = 0.0, My = 0.0, time = 0.0, dt1 = 0.0;
@@ -565,14 +560,11 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
     for (int i = 0; i < kNSUM; ++i) sums[(size_t)y * kNSUM + i] = acc[i];
   if (!pres) return;
   F.template fft<false, 2>(lds, twx, tid, nt);
-  rows_r2c_post(lds, 2, H, PL, twx, tid, nt);
-  for (int k = tid; k < H; k += nt) {
-    const int q = pidx(k);
-    const double2 xq = f0[q];
+  rows_r2c_store<2>(lds, H, PL, twx, tid, nt, [&](int k, const double2 *x) {
     const double kf = (double)k * c.sx;
-    Rq[(size_t)y * H + k] = make_double2(-kf * xq.y, kf * xq.x);
-    Rqy[(size_t)y * H + k] = f1[q];
-  }
+    Rq[(size_t)y * H + k] = make_double2(-kf * x[0].y, kf * x[0].x);
+    Rqy[(size_t)y * H + k] = x[1];
+  });
 }
 
 
@@ -639,19 +631,18 @@ __device__ __forceinline__ void rows_box_body(const double2 *__restrict__ Cq, co
   const int y = r0 + bid;
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
   const int nf = visc ? 3 : 1;
-  for (int k = tid; k < H; k += nt) {
-    const int q = pidx(k);
-    f0[q] = Cq[(size_t)y * H + k];
-    if (visc) {
-      const double kf = (double)k * c.sx;
+  // rows: q, and with viscosity d2psi/dxdy = i kx dpsi/dy and
+  // d2psi/dy2 - d2psi/dx2 = kx^2 psi + d2psi/dy2
+  stage_rows_c2r<3>(lds, nf, H, PL, twx, tid, nt, [&](int f, int k) -> double2 {
+    if (f == 0) return Cq[(size_t)y * H + k];
+    const double kf = (double)k * c.sx;
+    if (f == 1) {
       const double2 py = Cpsiy[(size_t)y * H + k];
-      const double2 p = Cpsi[(size_t)y * H + k], pyy = Cpsiyy[(size_t)y * H + k];
-      f1[q] = make_double2(-kf * py.y, kf * py.x);
-      f2[q] = make_double2((kf * kf) * p.x + pyy.x, (kf * kf) * p.y + pyy.y);
+      return make_double2(-kf * py.y, kf * py.x);
     }
-  }
-  __syncthreads();
-  rows_c2r_pre(lds, nf, H, PL, twx, tid, nt);
+    const double2 p = Cpsi[(size_t)y * H + k], pyy = Cpsiyy[(size_t)y * H + k];
+    return make_double2((kf * kf) * p.x + pyy.x, (kf * kf) * p.y + pyy.y);
+  });
   if (visc) F.template fft<true, 3>(lds, twx, tid, nt);
   else F.template fft<true, 1>(lds, twx, tid, nt);
   const size_t row = (size_t)bid * bw;
