@@ -2894,35 +2894,39 @@ static int capture_part(fsibe_batch *b, int gi, int part) {
   return FSIBE_OK;
 }
 
-// Launches a step's stages from the groups' graphs (after the gather, which
-// the caller has launched).  A stage whose blocks no longer fit its captured
-// grid (the mask box, the force window and the beam's force units grow and
-// shrink as the beam moves) is captured again with a quarter more room;
-// blocks beyond a member's own count return at once.
+// Launches group gi's stages of a step from its graphs (after the gather).
+// A stage whose blocks no longer fit its captured grid (the mask box, the
+// force window and the beam's force units grow and shrink as the beam
+// moves) is captured again with a quarter more room; blocks beyond a
+// member's own count return at once.
+static int group_graph_stages(fsibe_batch *b, int gi, const std::array<int, B_NST> &need) {
+  BatchGroup &g = b->groups[gi];
+  bool fits[2] = {g.gexec[0] != nullptr, g.gexec[1] != nullptr};
+  for (int k = 0; k < B_NST; ++k)
+    if (need[k] > g.cap[k]) {
+      const bool grows = k == B_MASK || k == B_K8 || k == B_K10;
+      g.cap[k] = grows ? need[k] + need[k] / 4 : need[k];
+      fits[k < B_K4 ? 0 : 1] = false;
+    }
+  for (int part = 0; part < 2; ++part)
+    if (!fits[part]) CHK(capture_part(b, gi, part));
+  HIPCHK(hipGraphLaunch(g.gexec[0], g.stream));
+  if (!b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+  HIPCHK(hipGraphLaunch(g.gexec[1], g.stream));
+  if (b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+  return FSIBE_OK;
+}
+
+// a step's stages of every group from the groups' graphs
 static int batch_graph_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>> &need) {
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    BatchGroup &g = b->groups[gi];
-    bool fits[2] = {g.gexec[0] != nullptr, g.gexec[1] != nullptr};
-    for (int k = 0; k < B_NST; ++k)
-      if (need[gi][k] > g.cap[k]) {
-        const bool grows = k == B_MASK || k == B_K8 || k == B_K10;
-        g.cap[k] = grows ? need[gi][k] + need[gi][k] / 4 : need[gi][k];
-        fits[k < B_K4 ? 0 : 1] = false;
-      }
-    for (int part = 0; part < 2; ++part)
-      if (!fits[part]) CHK(capture_part(b, (int)gi, part));
-  }
-  for (auto &g : b->groups) {
-    HIPCHK(hipGraphLaunch(g.gexec[0], g.stream));
-    if (!b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-    HIPCHK(hipGraphLaunch(g.gexec[1], g.stream));
-    if (b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-  }
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) CHK(group_graph_stages(b, (int)gi, need[gi]));
   b->replays++;
   return FSIBE_OK;
 }
 
-// one AB2 step of every member
+// one AB2 step of every member of a batch without solved beams (prescribed
+// motion): the host waits only for every group's nonlinear term, then
+// reads dt1 and the previous step's results while the rest runs
 static int batch_step(fsibe_batch *b) {
   for (fsibe_sim *s : b->m)
     if (!(s->time < s->p.Time_end)) return set_err(FSIBE_ESTATE, "batch: a member reached Time_end");
@@ -2957,47 +2961,11 @@ static int batch_step(fsibe_batch *b) {
     CHK(batch_graph_stages(b, maxblk));
   } else {
     CHK(batch_stages(b, maxblk, B_MASK, B_K4, SL_EAGER));
-    if (!b->fsi)
-      for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+    for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
     CHK(batch_stages(b, maxblk, B_K4, B_NST, SL_EAGER));
-    if (b->fsi)
-      for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-  }
-  // the host waits for every group's nonlinear term (prescribed motion) or
-  // for the whole step (solved beams)
-  auto wait_groups = [&]() -> int {
-    for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
-    return FSIBE_OK;
-  };
-  if (b->fsi) {
-    // the beams of step n+1 need the loads of step n: wait for every group,
-    // then each member reads its results, and its beam is solved (or moved)
-    CHK(wait_groups());
-    b->pool->run((int)b->m.size(), [&](int i) {
-      fsibe_sim *s = b->m[i];
-      s->upload_pending = false;
-      parse_results(s, s->slot);
-      s->dt1 = s->last.dt1;
-      s->slot ^= 1;
-      const fsibe_params &p = s->p;
-      if (p.iFLUSI == 1 && p.iBeam > 0) {
-        if (solid_step(s) != FSIBE_OK) {
-          errs[i] = fsibe_last_error();
-          bad.store(i);
-        }
-      } else if (p.iMotion > 0 && p.iBeam > 0) {
-        integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
-                                s->by.data(), s->bvx.data(), s->bvy.data());
-      }
-      std::swap(s->n0, s->n1);
-      s->time = s->time + s->dt1;
-      s->it++;
-    });
-    if (bad.load() >= 0) return set_err(FSIBE_EDOMAIN, "batch member %d: %s", bad.load(), errs[bad.load()].c_str());
-    return FSIBE_OK;
   }
   // dt1 of every member, then the previous step's results
-  CHK(wait_groups());
+  for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
   const bool pending = b->res_pending;
   b->pool->run((int)b->m.size(), [&](int i) {
     fsibe_sim *s = b->m[i];
@@ -3016,6 +2984,146 @@ static int batch_step(fsibe_batch *b) {
   return FSIBE_OK;
 }
 
+// ---- coupled batches: one chain of steps per size group -------------------
+// The beams of step n+1 need the loads of step n, so a coupled member's host
+// work (reading its results, the beam solve, the next geometry) sits between
+// two of its steps.  Each size group runs its own chain: when a group's step
+// ends, the pool does its members' host work and the group's next step is
+// launched while the other groups' steps keep the GPU busy.  Members never
+// wait for another group, and each stays bit-identical to its own run.
+
+// geometry and argument records of group gi's next step; need = its grids
+static int group_prepare(fsibe_batch *b, int gi, std::array<int, B_NST> &need) {
+  BatchGroup &g = b->groups[gi];
+  for (int i : g.mem)
+    if (!(b->m[i]->time < b->m[i]->p.Time_end)) return set_err(FSIBE_ESTATE, "batch: a member reached Time_end");
+  std::atomic<int> bad{-1};
+  std::vector<std::string> errs(g.mem.size());
+  b->pool->run((int)g.mem.size(), [&](int j) {
+    fsibe_sim *s = b->m[g.mem[j]];
+    s->dt0 = s->dt1;
+    if (stage_geometry(s, s->time) != FSIBE_OK) {
+      errs[j] = fsibe_last_error();
+      bad.store(j);
+    }
+  });
+  if (bad.load() >= 0)
+    return set_err(FSIBE_EDOMAIN, "batch member %d: %s", g.mem[bad.load()], errs[bad.load()].c_str());
+  int mx[B_NST] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t j = 0; j < g.mem.size(); ++j) fill_batch_args(b, g, (int)j, b->m[g.mem[j]], mx);
+  for (int k = 0; k < B_NST; ++k) need[k] = mx[k];
+  return FSIBE_OK;
+}
+
+// group gi's argument records and its members' geometry, copied on its stream
+static int group_gather(fsibe_batch *b, int gi) {
+  const BatchGroup &g = b->groups[gi];
+  const size_t nm = g.mem.size();
+  const size_t a0 = g.off[0], a1 = g.off[B_NST - 1] + ((kBatchArgSize[B_NST - 1] * nm + 255) & ~(size_t)255);
+  std::vector<StageSeg> segs;
+  segs.push_back(StageSeg{reinterpret_cast<const double2 *>(b->m_args + a0), reinterpret_cast<double2 *>(b->d_args + a0),
+                          (long long)((a1 - a0) / sizeof(double2))});
+  for (int i : g.mem) {
+    const fsibe_sim *s = b->m[i];
+    segs.push_back(StageSeg{reinterpret_cast<const double2 *>(s->m_stage), reinterpret_cast<double2 *>(s->d_stage),
+                            (long long)(s->off_val / sizeof(double2))});
+  }
+  for (size_t k0 = 0; k0 < segs.size(); k0 += kMaxStageSegs) {
+    StageSegs ss{};
+    const int ns = (int)std::min(segs.size() - k0, (size_t)kMaxStageSegs);
+    for (int k = 0; k < ns; ++k) ss.s[k] = segs[k0 + k];
+    k_stage_gather<<<dim3(8, ns), 256, 0, g.stream>>>(ss);
+    HIPCHK(hipGetLastError());
+  }
+  return FSIBE_OK;
+}
+
+// prepare and launch group gi's next step; g.ev_dt marks its end
+static int group_start(fsibe_batch *b, int gi) {
+  std::array<int, B_NST> need{};
+  CHK(group_prepare(b, gi, need));
+  CHK(group_gather(b, gi));
+  if (b->use_graph) {
+    CHK(group_graph_stages(b, gi, need));
+  } else {
+    std::vector<std::array<int, B_NST>> grids(b->groups.size());
+    grids[gi] = need;
+    CHK(batch_stages(b, grids, B_MASK, B_NST, SL_EAGER, gi));
+    HIPCHK(hipEventRecord(b->groups[gi].ev_dt, b->groups[gi].stream));
+  }
+  return FSIBE_OK;
+}
+
+// after group gi's step: each member reads its results, and its beam is
+// solved (or moved); time and levels advance
+static int group_finish(fsibe_batch *b, int gi) {
+  BatchGroup &g = b->groups[gi];
+  std::atomic<int> bad{-1};
+  std::vector<std::string> errs(g.mem.size());
+  b->pool->run((int)g.mem.size(), [&](int j) {
+    fsibe_sim *s = b->m[g.mem[j]];
+    s->upload_pending = false;
+    parse_results(s, s->slot);
+    s->dt1 = s->last.dt1;
+    s->slot ^= 1;
+    const fsibe_params &p = s->p;
+    if (p.iFLUSI == 1 && p.iBeam > 0) {
+      if (solid_step(s) != FSIBE_OK) {
+        errs[j] = fsibe_last_error();
+        bad.store(j);
+      }
+    } else if (p.iMotion > 0 && p.iBeam > 0) {
+      integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
+                              s->by.data(), s->bvx.data(), s->bvy.data());
+    }
+    std::swap(s->n0, s->n1);
+    s->time = s->time + s->dt1;
+    s->it++;
+  });
+  if (bad.load() >= 0)
+    return set_err(FSIBE_EDOMAIN, "batch member %d: %s", g.mem[bad.load()], errs[bad.load()].c_str());
+  return FSIBE_OK;
+}
+
+// nsteps AB2 steps of every group of a coupled batch.  A member whose
+// emergency brake trips stops its group after that step; the other groups
+// finish the step they are running and take no further one.
+static int batch_run_groups(fsibe_batch *b, int nsteps) {
+  const int ng = (int)b->groups.size();
+  std::vector<int> left(ng, nsteps);
+  std::vector<char> busy(ng, 0);
+  int rc = FSIBE_OK;
+  bool stop = false;
+  for (int gi = 0; gi < ng && rc == FSIBE_OK; ++gi) {
+    rc = group_start(b, gi);
+    if (rc == FSIBE_OK) busy[gi] = 1;
+  }
+  for (;;) {
+    int gi = -1;
+    bool any = false;
+    for (int k = 0; k < ng && gi < 0; ++k) {
+      if (!busy[k]) continue;
+      any = true;
+      const hipError_t e = hipEventQuery(b->groups[k].ev_dt);
+      if (e == hipSuccess) gi = k;
+      else if (e != hipErrorNotReady) return set_err(FSIBE_EHIP, "batch: %s", hipGetErrorString(e));
+    }
+    if (!any) break;
+    if (gi < 0) continue;
+    busy[gi] = 0;
+    --left[gi];
+    const int rf = group_finish(b, gi);
+    if (rf != FSIBE_OK && rc == FSIBE_OK) rc = rf;
+    for (int i : b->groups[gi].mem) stop = stop || b->m[i]->last.status == 1;
+    if (rc == FSIBE_OK && !stop && left[gi] > 0) {
+      rc = group_start(b, gi);
+      if (rc == FSIBE_OK) busy[gi] = 1;
+    }
+  }
+  if (b->use_graph && ng > 0) b->replays += nsteps - left[0];  // steps of the first group
+  return rc;
+}
+
 extern "C" int fsibe_batch_run(fsibe_batch *b, int nsteps) {
   if (!b) return set_err(FSIBE_EARG, "null batch");
   // the startup (Euler) step of members that have not taken it, on their
@@ -3032,6 +3140,10 @@ extern "C" int fsibe_batch_run(fsibe_batch *b, int nsteps) {
   }
   for (fsibe_sim *s : b->m) HIPCHK(hipStreamSynchronize(s->stream));
   int rc = FSIBE_OK;
+  if (b->fsi) {
+    if (nsteps > done) rc = batch_run_groups(b, nsteps - done);
+    return rc;
+  }
   for (int k = done; k < nsteps; ++k) {
     rc = batch_step(b);
     if (rc != FSIBE_OK) break;
