@@ -16,6 +16,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <cstdarg>
@@ -788,6 +789,10 @@ struct fsibe_sim {
   bool k1_ready = false;
   // K2's rows outside k2_b0 .. k2_b1 launched ahead with K1 (iFLUSI=1 loop)
   bool k2_ready = false;
+  // launch choices of the column stages K1, K4, K5, K7 (the autotuner's,
+  // once fsibe_run has measured them for this grid; else the static ones)
+  ColSel csel[4];
+  bool csel_ready = false;
   int k2_b0 = 0, k2_b1 = -1;
   int k1_level = -1;
   // what each slot holds: the fluid step's row sums, the force terms
@@ -1614,10 +1619,21 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
 }
 
 // ---- fluid step ----------------------------------------------------------
-static int cols_inv_fields(fsibe_sim *s, const double2 *W, double2 *const *outs, const int *fsel, int nf) {
+static ColSel static_sel(const fsibe_sim *s, int bm, int split);
+
+// launch choice of column stage `which` (0: K1 and the other inverse
+// transforms of one level, 1: K4, 2: K5, 3: K7) of a kernel over `split`
+// fields
+static ColSel sim_sel(const fsibe_sim *s, int which, int split) {
+  if (s->csel_ready) return s->csel[which];
+  return static_sel(s, (which == 1 || which == 3) ? 2 : 1, split);
+}
+
+static int cols_inv_fields(fsibe_sim *s, const double2 *W, double2 *const *outs, const int *fsel, int nf,
+                           int which = 0) {
   double2 *o[4] = {outs[0], nf > 1 ? outs[1] : nullptr, nf > 2 ? outs[2] : nullptr, nf > 3 ? outs[3] : nullptr};
   int f[4] = {fsel[0], nf > 1 ? fsel[1] : 0, nf > 2 ? fsel[2] : 0, nf > 3 ? fsel[3] : 0};
-  return with_cols<1>(s->py, s->H, pick_tx(s->H, s->ny, 1, nf), [&](auto F, int nt, int TX) -> int {
+  return with_cols_sel<1>(s->py, s->H, sim_sel(s, which, nf), [&](auto F, int nt, int TX) -> int {
     const size_t lds = sizeof(double2) * TX * padlen(s->ny);
     const int grid = (s->H / TX) * nf;
     KLAUNCH(k_cols_inv_fields<decltype(F)>, grid, nt, lds, s->stream, W, o[0], o[1], o[2], o[3], f[0], f[1], f[2],
@@ -1760,7 +1776,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
   // K4
   {
     prof_begin(s, P_COLS_AB2);
-    CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+    CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 1, 1), [&](auto F, int nt, int TX) -> int {
       const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
       KLAUNCH(k_cols_ab2<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[2], s->T[3], s->W[n0], s->NL[n1],
               s->NL[n0], s->W[n1], s->dscal, (int)explicit_step, F, s->twy, s->sc);
@@ -1774,7 +1790,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     double2 *outs[4] = {s->T[0], s->T[1], s->T[2], s->T[3]};
     const int fsel[4] = {0, 1, 2, 3};
     prof_begin(s, P_COLS_W1);
-    CHK(cols_inv_fields(s, s->W[n1], outs, fsel, st ? 4 : 3));
+    CHK(cols_inv_fields(s, s->W[n1], outs, fsel, st ? 4 : 3, 2));
     prof_end(s, P_COLS_W1);
   }
   // K6
@@ -1801,7 +1817,7 @@ static int evolve_pressure_box(fsibe_sim *s) {
     const Box &fb = s->fbox;
     {
       prof_begin(s, P_COLS_PRESS);
-      CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+      CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 3, 1), [&](auto F, int nt, int TX) -> int {
         const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
         KLAUNCH(k_cols_press<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[4], s->T[5], s->T[6], fb.y0, fb.y1,
                 F, s->twy, s->sc);
@@ -2088,8 +2104,11 @@ static void init_beam_host(fsibe_sim *s) {
   s->beam_time = 0.0;
 }
 
+static int sim_tune_cols(fsibe_sim *s);
+
 extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
+  if (nsteps > 0) CHK(sim_tune_cols(s));
   const bool fsi = s->p.iFLUSI == 1 && s->p.iBeam > 0;
   if (fsi) {
     CHK(ensure_solid(s));
@@ -2199,7 +2218,7 @@ extern "C" int fsibe_get_field(fsibe_sim *s, int which, double *out) {
     case FSIBE_FIELD_PRESS: {
       if (!s->pressure_on) { memset(out, 0, sizeof(double) * N); break; }
       // full-grid pressure from the last step's right-hand side (T4, T5)
-      CHK(with_cols<2>(s->py, s->H, pick_tx(s->H, s->ny, 2, 1), [&](auto F, int nt, int TX) -> int {
+      CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 3, 1), [&](auto F, int nt, int TX) -> int {
         const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
         KLAUNCH(k_cols_press<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[4], s->T[5], s->T[6], 0, s->ny - 1,
                 F, s->twy, s->sc);
@@ -2422,15 +2441,18 @@ static ColSel static_sel(const fsibe_sim *s, int bm, int split) {
 static const int kColStages[4] = {B_K1, B_K4, B_K5, B_K7};
 
 // Argument records of column stage k for every member of g with tile width
-// tx, for the autotuner: the inputs are the members' own fields, and every
-// output goes to a per-step temporary (T), which holds nothing between steps
-// (fsibe_run and batch steps recompute all of them), so timing a stage
-// leaves every simulation as it was.
-static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int tx, char *h, int *nblk_max) {
+// tx, for the autotuner: the inputs are the members' own fields (read
+// only), and every output goes to the tuner's scratch, four fields of
+// H ny per member from scr, so timing a stage leaves every simulation as it
+// was (the temporaries T also hold the last step's pressure inputs).
+static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int tx, double2 *scr, char *h,
+                           int *nblk_max) {
   const int nm = (int)g.mem.size();
   *nblk_max = 0;
   for (int j = 0; j < nm; ++j) {
     const fsibe_sim *s = b->m[g.mem[j]];
+    const size_t NH = (size_t)s->H * s->ny;
+    double2 *o = scr + (size_t)j * 4 * NH;
     int nb = 0;
     if (k == B_K1 || k == B_K5) {
       ColsInvArgs &a = reinterpret_cast<ColsInvArgs *>(h)[j];
@@ -2438,7 +2460,7 @@ static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int
       const int nf = k == B_K1 ? 2 : g.nf5;
       a.W = s->W[k == B_K1 ? s->n0 : s->n1];
       for (int q = 0; q < 4; ++q) {
-        a.o[q] = q < nf ? s->T[q] : nullptr;
+        a.o[q] = q < nf ? o + q * NH : nullptr;
         a.fsel[q] = k == B_K1 ? (q == 1 ? 4 : 0) : (q < nf ? q : 0);
       }
       a.nfield = nf;
@@ -2448,12 +2470,12 @@ static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int
       ColsAb2Args &a = reinterpret_cast<ColsAb2Args *>(h)[j];
       a = ColsAb2Args{};
       a.Rc = s->T[2]; a.Rpx = s->T[3]; a.W0 = s->W[s->n0]; a.NL1 = s->NL[s->n1];
-      a.NL0 = s->T[4]; a.W1 = s->T[5];
+      a.NL0 = o; a.W1 = o + NH;
       a.s = s->dscal; a.explicit_step = 0; a.nblk = nb = s->H / tx; a.c = s->sc;
     } else {
       ColsPressArgs &a = reinterpret_cast<ColsPressArgs *>(h)[j];
       a = ColsPressArgs{};
-      a.Rq = s->T[4]; a.Rqy = s->T[5]; a.Cq = s->T[6]; a.r0 = 0; a.r1 = s->ny - 1;
+      a.Rq = s->T[4]; a.Rqy = s->T[5]; a.Cq = o; a.r0 = 0; a.r1 = s->ny - 1;
       a.nblk = nb = s->H / tx; a.c = s->sc;
     }
     *nblk_max = std::max(*nblk_max, nb);
@@ -2517,6 +2539,11 @@ static int tune_batch(fsibe_batch *b) {
   for (size_t gi = 0; rc == FSIBE_OK && gi < b->groups.size(); ++gi) {
     BatchGroup &g = b->groups[gi];
     const fsibe_sim *s0 = b->m[g.mem[0]];
+    double2 *scr = nullptr;  // the stages' outputs while they are timed
+    if (hipMalloc(&scr, sizeof(double2) * 4 * (size_t)s0->H * s0->ny * g.mem.size()) != hipSuccess) {
+      rc = set_err(FSIBE_EHIP, "batch autotuner: allocation failed");
+      break;
+    }
     for (int k : kColStages) {
       const int bm = (k == B_K4 || k == B_K7) ? 2 : 1;
       std::vector<ColSel> cand;
@@ -2527,7 +2554,7 @@ static int tune_batch(fsibe_batch *b) {
       for (int round = 0; rc == FSIBE_OK && round < 2; ++round)
         for (size_t ci = 0; rc == FSIBE_OK && ci < cand.size(); ++ci) {
           int gx = 0;
-          const size_t nb = tune_records(b, g, k, cand[ci].tx, h.data(), &gx);
+          const size_t nb = tune_records(b, g, k, cand[ci].tx, scr, h.data(), &gx);
           if (hipStreamSynchronize(g.stream) != hipSuccess || hipMemcpy(d, h.data(), nb, hipMemcpyHostToDevice) != hipSuccess) {
             rc = set_err(FSIBE_EHIP, "batch autotuner: upload failed");
             break;
@@ -2559,12 +2586,53 @@ static int tune_batch(fsibe_batch *b) {
       g.base_us[k] = best[si];
       g.ncand[k] = (int)cand.size();
     }
+    (void)hipStreamSynchronize(g.stream);
+    (void)hipFree(scr);
   }
   if (d) (void)hipFree(d);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (rc == FSIBE_OK) b->tuned = true;
   return rc;
+}
+
+// The single path's column stages take the autotuner's choices too:
+// measured once per process and grid (nx, ny, fields of K5) on a batch of
+// that one simulation at its first fsibe_run, and kept for every
+// simulation of that grid.
+static std::mutex g_csel_mu;
+static std::map<std::array<int, 3>, std::array<ColSel, 4>> g_csel;
+
+static int sim_tune_cols(fsibe_sim *s) {
+  if (s->csel_ready) return FSIBE_OK;
+  const int nf5 = (s->forces_on && s->visc) ? 4 : 3;
+  const std::array<int, 3> key{s->nx, s->ny, nf5};
+  std::lock_guard<std::mutex> lock(g_csel_mu);
+  auto it = g_csel.find(key);
+  if (it == g_csel.end()) {
+    fsibe_batch t;  // borrows the simulation; releases nothing of it
+    t.m.push_back(s);
+    BatchGroup g;
+    g.nx = s->nx; g.ny = s->ny; g.H = s->H; g.nf5 = nf5; g.pres = s->pressure_on;
+    g.mem.push_back(0);
+    g.stream = s->stream;
+    g.sel[B_K1] = static_sel(s, 1, 2);
+    g.sel[B_K4] = static_sel(s, 2, 1);
+    g.sel[B_K5] = static_sel(s, 1, nf5);
+    g.sel[B_K7] = static_sel(s, 2, 1);
+    t.groups.push_back(g);
+    const int rc = tune_batch(&t);
+    const std::array<ColSel, 4> sel{t.groups[0].sel[B_K1], t.groups[0].sel[B_K4], t.groups[0].sel[B_K5],
+                                    t.groups[0].sel[B_K7]};
+    t.groups[0].stream = nullptr;
+    t.m.clear();
+    t.groups.clear();
+    if (rc != FSIBE_OK) return rc;
+    it = g_csel.emplace(key, sel).first;
+  }
+  for (int q = 0; q < 4; ++q) s->csel[q] = it->second[q];
+  s->csel_ready = true;
+  return FSIBE_OK;
 }
 
 extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const double *const *vortk,
