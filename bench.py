@@ -424,6 +424,8 @@ def main(argv=None) -> int:
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="print HBM bytes per dispatch from four rocprofv3 --pmc csv directories")
     ap.add_argument("--copy-mib", type=int, default=64)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched workload (--batch-sims, --batch-steps); prints its JSON")
     args = ap.parse_args(argv)
     if args.summarize_trace:
         print(summarize_trace(args.summarize_trace))
@@ -441,6 +443,9 @@ def main(argv=None) -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 20}))
+        return 0
+    if args.batch_only:
+        print(json.dumps(batch_bench(args.batch_sims, args.batch_steps)))
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

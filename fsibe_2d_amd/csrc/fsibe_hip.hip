@@ -812,6 +812,7 @@ struct fsibe_sim {
   Window win;
   int nprim = 0, ntg = 0;
   size_t off_prim = 0, off_tg = 0, off_nd = 0, off_cd = 0, off_val = 0;
+  int nprim_cap = 0;  // room for mask primitives in the staging buffer (and the mask kernel's LDS)
   ForceGeom fg;
   bool forces_ready = false;
   bool press_box_valid = false;
@@ -1192,6 +1193,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     auto al = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
     const int nprim_max = 2 * nsb + 8;
     const int ntg_max = 6 * nsb + 6 * kNC;
+    s->nprim_cap = nprim_max;
     s->off_prim = al(sizeof(Prim) * nprim_max);
     s->off_tg = al(sizeof(double) * 3 * ntg_max);
     s->off_nd = al(sizeof(double) * 4 * nsb);
@@ -1257,7 +1259,8 @@ extern "C" int fsibe_swap_levels(fsibe_sim *s) {
 // ---- geometry of one step (host) ------------------------------------------
 static void add_prim(fsibe_sim *s, Prim &P, int &np) {
   Prim *dst = reinterpret_cast<Prim *>(s->h_stage + s->off_prim);
-  dst[np++] = P;
+  if (np < s->nprim_cap) dst[np] = P;  // the caller checks the count
+  ++np;
 }
 
 // the mask's primitives and box, and the cal_drag window (create_mask.f90)
@@ -1359,6 +1362,7 @@ static int build_mask_geometry(fsibe_sim *s) {
     dot(X[0], Y[0], VX[0], VY[0], t);
     if (s->p.iSharpTrailing == 0) dot(X[ns - 1], Y[ns - 1], VX[ns - 1], VY[ns - 1], t);
   }
+  if (np > s->nprim_cap) return set_err(FSIBE_ESIZE, "mask: %d primitives, room for %d", np, s->nprim_cap);
   s->nprim = np;
   s->mbox = mb;
 
@@ -1607,7 +1611,7 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
     dim3 blk(kMaskThreads);
     dim3 grd((s->mbox.w() + kMaskTile - 1) / kMaskTile, (s->mbox.h() + kMaskTile - 1) / kMaskTile);
     prof_begin(s, P_MASK);
-    k_mask<<<grd, blk, 0, s->stream>>>(reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim,
+    k_mask<<<grd, blk, sizeof(int4) * std::max(s->nprim, 1), s->stream>>>(reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim,
                                        s->wallrow ? s->wallrow + 0 : nullptr, mc, s->mbox.x0, s->mbox.y0,
                                        s->mbox.w(), s->mbox.h(), s->mask_m, s->mask_vx, s->mask_vy, s->mask_win);
     HIPCHK(hipGetLastError());
@@ -2377,6 +2381,7 @@ struct BatchGroup {
   float tune_us[B_NST], base_us[B_NST];  // measured time of the choice and of the static one (0: not tuned)
   int ncand[B_NST];   // choices measured
   int cap[B_NST];     // grid.x of the stage in the captured step (0: stage left out)
+  int nprim_cap = 1;  // the members' largest room for mask primitives (the mask stage's LDS)
   // the group's stages as two captured graphs (up to the nonlinear term, and
   // the rest), replayed on the group's stream
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -2605,6 +2610,7 @@ static std::map<std::array<int, 3>, std::array<ColSel, 4>> g_csel;
 
 static int sim_tune_cols(fsibe_sim *s) {
   if (s->csel_ready) return FSIBE_OK;
+  if (env_int("FSIBE_AUTOTUNE", 1) == 0) return FSIBE_OK;  // the static choices (reproducible profiles)
   const int nf5 = (s->forces_on && s->visc) ? 4 : 3;
   const std::array<int, 3> key{s->nx, s->ny, nf5};
   std::lock_guard<std::mutex> lock(g_csel_mu);
@@ -2675,6 +2681,7 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
       g->nx = s->nx; g->ny = s->ny; g->H = s->H; g->nf5 = nf5; g->pres = s->pressure_on;
     }
     g->mem.push_back(i);
+    g->nprim_cap = std::max(g->nprim_cap, s->nprim_cap);
   }
   size_t off = 0;
   for (auto &g : b->groups)
@@ -2715,7 +2722,7 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
       g.cap[k] = 0;
     }
   }
-  {
+  if (env_int("FSIBE_AUTOTUNE", 1) != 0) {
     const int rc = tune_batch(b);
     if (rc != FSIBE_OK) return fail(rc);
   }
@@ -2855,7 +2862,8 @@ static int batch_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>
       if (gx <= 0) continue;
       switch (k) {
         case B_MASK:
-          SLAUNCH(kb_mask, gx, nm, kMaskThreads, 0, g.stream, mode, reinterpret_cast<const MaskArgs *>(arr(g, B_MASK)));
+          SLAUNCH(kb_mask, gx, nm, kMaskThreads, sizeof(int4) * g.nprim_cap, g.stream, mode,
+                  reinterpret_cast<const MaskArgs *>(arr(g, B_MASK)));
           break;
         case B_K1:
         case B_K5:

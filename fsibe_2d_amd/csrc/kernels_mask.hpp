@@ -141,7 +141,7 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
                                           double *__restrict__ om, double *__restrict__ ovx,
                                           double *__restrict__ ovy, int *__restrict__ owin, int tbx, int tby) {
 #pragma clang fp contract(off)
-  __shared__ int4 bnd[kMaxTilePrims];  // ix0, ix1, iy0, iy1
+  extern __shared__ int4 bnd[];  // ix0, ix1, iy0, iy1 of the np primitives (dynamic LDS, np entries)
   __shared__ Prim sp[kMaskStage];
   __shared__ int list[kMaskStage];
   __shared__ double val[kMaskStage][kMaskPix];

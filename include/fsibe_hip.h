@@ -326,7 +326,11 @@ fsibe_sim *fsibe_batch_member(fsibe_batch *batch, int i);
  * nonlinear term, then the rest).
  * use_graph: 1 graphs, 0 eager launches, < 0 unchanged; autotune: 1 measure
  * again, 0 the single-simulation path's choices, < 0 unchanged.  No setting
- * changes a result: every choice runs the same passes on every column. */
+ * changes a result: every choice runs the same passes on every column.
+ * Single simulations take the tuner's choices too (measured at the first
+ * fsibe_run of a grid in the process).  The environment variable
+ * FSIBE_AUTOTUNE=0 keeps the static choices everywhere, for profiles that
+ * must launch the same kernels from run to run. */
 int fsibe_batch_configure(fsibe_batch *batch, int use_graph, int autotune);
 /* pin the launch choice of column stage `stage` (1, 4, 5 or 7: K1, K4, K5,
  * K7) of size group `group` (the order of fsibe_batch_describe): the
