@@ -2295,9 +2295,8 @@ extern "C" int fsibe_profile_read(fsibe_sim *s, double *total_ms, long long *cou
 // simulations grouped by grid size.  Each member takes its startup step on
 // its own; every AB2 step after that is one launch per stage and size group,
 // with one argument record per member at blockIdx.y (kb_* kernels).  Members
-// keep their own time, dt, beam and parameters.  The loop is pipelined like
-// fsibe_run: the host waits only for the nonlinear-term stage, reads every
-// member's dt1, and prepares the next step's geometry while the rest runs.
+// keep their own time, dt, beam and parameters.  Each size group runs its
+// own chain of steps on its own stream (batch_run_groups).
 // ===========================================================================
 // A few host threads for the per-member host work of a batch step (beam
 // geometry, reading results): for(i = 0 .. n-1) fn(i), spread over the pool
@@ -2391,11 +2390,8 @@ struct BatchGroup {
 struct fsibe_batch {
   std::vector<fsibe_sim *> m;
   std::vector<BatchGroup> groups;
-  hipStream_t stream = nullptr;  // argument upload
-  hipEvent_t ev_dt = nullptr, ev_args = nullptr;
   char *h_args = nullptr, *d_args = nullptr, *m_args = nullptr;  // m_args: device address of h_args
   size_t args_bytes = 0;
-  bool res_pending = false;
   bool fsi = false;  // some member's beam is solved (iFLUSI=1): every step waits for its loads
   HostPool *pool = nullptr;
   // each group's stages replayed from captured graphs while every stage
@@ -2417,19 +2413,15 @@ static void drop_graphs(fsibe_batch *b) {
 
 extern "C" int fsibe_batch_destroy(fsibe_batch *b) {
   if (!b) return FSIBE_OK;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto &g : b->groups) {
     if (g.stream) (void)hipStreamSynchronize(g.stream);
     if (g.stream) (void)hipStreamDestroy(g.stream);
     if (g.ev_dt) (void)hipEventDestroy(g.ev_dt);
   }
-  if (b->ev_args) (void)hipEventDestroy(b->ev_args);
   drop_graphs(b);
   for (fsibe_sim *s : b->m) fsibe_sim_destroy(s);
   if (b->h_args) (void)hipHostFree(b->h_args);
   if (b->d_args) (void)hipFree(b->d_args);
-  if (b->ev_dt) (void)hipEventDestroy(b->ev_dt);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b->pool;
   delete b;
   return FSIBE_OK;
@@ -2693,9 +2685,7 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
   for (auto &g : b->groups)
     if (hipStreamCreate(&g.stream) != hipSuccess || hipEventCreateWithFlags(&g.ev_dt, hipEventDisableTiming) != hipSuccess)
       return fail(set_err(FSIBE_EHIP, "batch: stream creation failed"));
-  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->ev_dt, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_args, hipEventDisableTiming) != hipSuccess ||
-      hipHostMalloc((void **)&b->h_args, off, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+  if (hipHostMalloc((void **)&b->h_args, off, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer((void **)&b->m_args, b->h_args, 0) != hipSuccess ||
       hipMalloc(&b->d_args, off) != hipSuccess)
     return fail(set_err(FSIBE_EHIP, "batch: allocation failed"));
@@ -2918,25 +2908,6 @@ static int batch_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>
   return FSIBE_OK;
 }
 
-// the argument records and every member's geometry, copied by kernels on
-// the upload stream (no copy-engine hand-off before the first stage)
-static int batch_gather(fsibe_batch *b) {
-  std::vector<StageSeg> segs;
-  segs.push_back(StageSeg{reinterpret_cast<const double2 *>(b->m_args), reinterpret_cast<double2 *>(b->d_args),
-                          (long long)(b->args_bytes / sizeof(double2))});
-  for (fsibe_sim *s : b->m)
-    segs.push_back(StageSeg{reinterpret_cast<const double2 *>(s->m_stage), reinterpret_cast<double2 *>(s->d_stage),
-                            (long long)(s->off_val / sizeof(double2))});
-  for (size_t k0 = 0; k0 < segs.size(); k0 += kMaxStageSegs) {
-    StageSegs ss{};
-    const int ns = (int)std::min(segs.size() - k0, (size_t)kMaxStageSegs);
-    for (int k = 0; k < ns; ++k) ss.s[k] = segs[k0 + k];
-    k_stage_gather<<<dim3(8, ns), 256, 0, b->stream>>>(ss);
-    HIPCHK(hipGetLastError());
-  }
-  return FSIBE_OK;
-}
-
 // Captures part 0 (mask, K1, K2) or part 1 (K4 .. K10) of one group's
 // step into a graph with the grids g.cap.  Every stage reads its argument
 // records from fixed device addresses (refreshed by the gather kernel each
@@ -2987,86 +2958,19 @@ static int group_graph_stages(fsibe_batch *b, int gi, const std::array<int, B_NS
   for (int part = 0; part < 2; ++part)
     if (!fits[part]) CHK(capture_part(b, gi, part));
   HIPCHK(hipGraphLaunch(g.gexec[0], g.stream));
-  if (!b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
   HIPCHK(hipGraphLaunch(g.gexec[1], g.stream));
-  if (b->fsi) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
+  HIPCHK(hipEventRecord(g.ev_dt, g.stream));
   return FSIBE_OK;
 }
 
-// a step's stages of every group from the groups' graphs
-static int batch_graph_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>> &need) {
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) CHK(group_graph_stages(b, (int)gi, need[gi]));
-  b->replays++;
-  return FSIBE_OK;
-}
-
-// one AB2 step of every member of a batch without solved beams (prescribed
-// motion): the host waits only for every group's nonlinear term, then
-// reads dt1 and the previous step's results while the rest runs
-static int batch_step(fsibe_batch *b) {
-  for (fsibe_sim *s : b->m)
-    if (!(s->time < s->p.Time_end)) return set_err(FSIBE_ESTATE, "batch: a member reached Time_end");
-  std::atomic<int> bad{-1};
-  std::vector<std::string> errs(b->m.size());
-  b->pool->run((int)b->m.size(), [&](int i) {
-    fsibe_sim *s = b->m[i];
-    s->dt0 = s->dt1;
-    if (stage_geometry(s, s->time) != FSIBE_OK) {
-      errs[i] = fsibe_last_error();
-      bad.store(i);
-    }
-  });
-  if (bad.load() >= 0) return set_err(FSIBE_EDOMAIN, "batch member %d: %s", bad.load(), errs[bad.load()].c_str());
-  // the previous step must be done with the argument buffer before it is
-  // overwritten: the upload stream waits for every group's last stage
-  for (auto &g : b->groups) {
-    HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-    HIPCHK(hipStreamWaitEvent(b->stream, g.ev_dt, 0));
-  }
-  std::vector<std::array<int, B_NST>> maxblk(b->groups.size());
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    BatchGroup &g = b->groups[gi];
-    int mx[B_NST] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (size_t j = 0; j < g.mem.size(); ++j) fill_batch_args(b, g, (int)j, b->m[g.mem[j]], mx);
-    for (int k = 0; k < B_NST; ++k) maxblk[gi][k] = mx[k];
-  }
-  CHK(batch_gather(b));
-  HIPCHK(hipEventRecord(b->ev_args, b->stream));
-  for (auto &g : b->groups) HIPCHK(hipStreamWaitEvent(g.stream, b->ev_args, 0));
-  if (b->use_graph) {
-    CHK(batch_graph_stages(b, maxblk));
-  } else {
-    CHK(batch_stages(b, maxblk, B_MASK, B_K4, SL_EAGER));
-    for (auto &g : b->groups) HIPCHK(hipEventRecord(g.ev_dt, g.stream));
-    CHK(batch_stages(b, maxblk, B_K4, B_NST, SL_EAGER));
-  }
-  // dt1 of every member, then the previous step's results
-  for (auto &g : b->groups) HIPCHK(wait_event(g.ev_dt));
-  const bool pending = b->res_pending;
-  b->pool->run((int)b->m.size(), [&](int i) {
-    fsibe_sim *s = b->m[i];
-    s->dt1 = *reinterpret_cast<volatile const double *>(s->h_res + (size_t)s->slot * s->res_stride);
-    if (pending) parse_results(s, s->slot ^ 1);
-    s->slot ^= 1;
-    const fsibe_params &p = s->p;
-    if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0)
-      integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
-                              s->by.data(), s->bvx.data(), s->bvy.data());
-    std::swap(s->n0, s->n1);
-    s->time = s->time + s->dt1;
-    s->it++;
-  });
-  b->res_pending = true;
-  return FSIBE_OK;
-}
-
-// ---- coupled batches: one chain of steps per size group -------------------
-// The beams of step n+1 need the loads of step n, so a coupled member's host
-// work (reading its results, the beam solve, the next geometry) sits between
-// two of its steps.  Each size group runs its own chain: when a group's step
-// ends, the pool does its members' host work and the group's next step is
-// launched while the other groups' steps keep the GPU busy.  Members never
-// wait for another group, and each stays bit-identical to its own run.
+// ---- one chain of steps per size group --------------------------------------
+// A member's host work (reading its results, the beam solve of a coupled
+// member or the leading edge's motion of a prescribed one, the next
+// geometry) sits between two of its steps.  Each size group runs its own
+// chain: when a group's step ends, the pool does its members' host work and
+// the group's next step is launched while the other groups' steps keep the
+// GPU busy.  Members never wait for another group, and each stays
+// bit-identical to its own run.
 
 // geometry and argument records of group gi's next step; need = its grids
 static int group_prepare(fsibe_batch *b, int gi, std::array<int, B_NST> &need) {
@@ -3161,7 +3065,7 @@ static int group_finish(fsibe_batch *b, int gi) {
   return FSIBE_OK;
 }
 
-// nsteps AB2 steps of every group of a coupled batch.  A member whose
+// nsteps AB2 steps of every group of a batch.  A member whose
 // emergency brake trips stops its group after that step; the other groups
 // finish the step they are running and take no further one.
 static int batch_run_groups(fsibe_batch *b, int nsteps) {
@@ -3215,27 +3119,7 @@ extern "C" int fsibe_batch_run(fsibe_batch *b, int nsteps) {
     done = any ? 1 : 0;
   }
   for (fsibe_sim *s : b->m) HIPCHK(hipStreamSynchronize(s->stream));
-  int rc = FSIBE_OK;
-  if (b->fsi) {
-    if (nsteps > done) rc = batch_run_groups(b, nsteps - done);
-    return rc;
-  }
-  for (int k = done; k < nsteps; ++k) {
-    rc = batch_step(b);
-    if (rc != FSIBE_OK) break;
-    // a member's solid solver tripped its emergency brake: the batch stops
-    // after this step, as that member's own loop would
-    bool brake = false;
-    for (fsibe_sim *s : b->m) brake = brake || s->last.status == 1;
-    if (brake) break;
-  }
-  if (b->res_pending) {
-    HIPCHK(hipStreamSynchronize(b->stream));
-    for (auto &g : b->groups) HIPCHK(hipStreamSynchronize(g.stream));
-    for (fsibe_sim *s : b->m) parse_results(s, s->slot ^ 1);
-    b->res_pending = false;
-  }
-  return rc;
+  return nsteps > done ? batch_run_groups(b, nsteps - done) : FSIBE_OK;
 }
 
 extern "C" int fsibe_batch_size(fsibe_batch *b) { return b ? (int)b->m.size() : 0; }
@@ -3252,7 +3136,6 @@ extern "C" fsibe_sim *fsibe_batch_member(fsibe_batch *b, int i) {
 // result.
 extern "C" int fsibe_batch_configure(fsibe_batch *b, int use_graph, int autotune) {
   if (!b) return set_err(FSIBE_EARG, "null batch");
-  HIPCHK(hipStreamSynchronize(b->stream));
   for (auto &g : b->groups) HIPCHK(hipStreamSynchronize(g.stream));
   if (autotune >= 0) {
     drop_graphs(b);
@@ -3291,7 +3174,6 @@ extern "C" int fsibe_batch_set_tiles(fsibe_batch *b, int group, int stage, int c
   if (!col_sel_valid(g.ny, g.H, bm, c))
     return set_err(FSIBE_EARG, "%s plan with %d columns per tile is not a launch choice for %d x %d",
                    c.ct ? "compile-time" : "runtime", tx, g.nx, g.ny);
-  HIPCHK(hipStreamSynchronize(b->stream));
   for (auto &q : b->groups) HIPCHK(hipStreamSynchronize(q.stream));
   drop_graphs(b);
   for (auto &q : b->groups)

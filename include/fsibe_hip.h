@@ -299,9 +299,9 @@ int fsibe_get_beam(fsibe_sim *sim, double *x, double *y, double *vx, double *vy,
  * No reference counterpart (the reference runs one simulation per process;
  * sequential runs at several resolutions, dns.f90:70-101).  n members with
  * their own parameters (grid sizes may differ; iSponge!=10; a member with
- * iFLUSI=1 has its beam solved by the library's solid solver, and then
- * each size group runs its own chain of steps, its members' beam solves
- * overlapping the other groups' steps),
+ * iFLUSI=1 has its beam solved by the library's solid solver; each size
+ * group runs its own chain of steps, its members' host work overlapping the
+ * other groups' steps),
  * vortk[i] the packed initial vorticity of member i (or NULL).  Every AB2
  * step of the batch is one launch per stage and grid size; each member's
  * results are those of fsibe_run on that member alone.  The members are
@@ -313,8 +313,8 @@ int fsibe_batch_create(const fsibe_params *params, int n, const double *const *v
 int fsibe_batch_destroy(fsibe_batch *batch);
 /* advance every member nsteps steps (the first of a member is its startup
  * step); stops early after a step in which a member's solid solver tripped
- * its emergency brake (that member's step_out status = 1; with solved
- * beams, members of other size groups may be one step further) */
+ * its emergency brake (that member's step_out status = 1; members of
+ * other size groups may be one step further) */
 int fsibe_batch_run(fsibe_batch *batch, int nsteps);
 int fsibe_batch_size(fsibe_batch *batch);
 fsibe_sim *fsibe_batch_member(fsibe_batch *batch, int i);

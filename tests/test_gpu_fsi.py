@@ -130,3 +130,28 @@ def test_full_size_coupled(n, steps, oracle_mod):
     g.run(steps)
     _compare(g, o, so, t_o, f"fsi coupled {n}", unst_rel=1e-5)
     g.close()
+
+
+def test_autotuned_tiles_do_not_change_results(oracle_mod, monkeypatch):
+    """fsibe_run measures the column stages' launch choices at the first run
+    of a grid; FSIBE_AUTOTUNE=0 keeps the static ones.  A coupled run with
+    each gives bit-identical fields, loads and beam (the choices differ only
+    in tiles per block, never in the arithmetic)"""
+    from fsibe_2d_amd import Params
+    from fsibe_2d_amd.solver import FluidSim
+
+    p = Params(nx=384, ny=384, ns=48, iFLUSI=1, mue=0.7, eta=0.08, dt_fixed=1e-4, nu=2e-3)
+    o = oracle_mod.Oracle(nx=p.nx, ny=p.ny, iBeam=0, iSponge=0, xl=p.xl, yl=p.yl)
+    vk = o.init_noise(seed=3, amp=1e-3, kmax2=64)
+    res = []
+    for tune in ("0", "1"):
+        monkeypatch.setenv("FSIBE_AUTOTUNE", tune)
+        g = FluidSim(p)
+        g.set_vortk(vk)
+        out = g.run(6)
+        res.append((g.field("vortk"), out["Force"], out["Force_press"], g.beam()["theta"]))
+        g.close()
+    a, b = res
+    assert np.array_equal(a[0], b[0])
+    assert a[1] == b[1] and a[2] == b[2]
+    assert np.array_equal(a[3], b[3])
