@@ -372,8 +372,10 @@ extern "C" int fsibe_adjoint_gradient(fsibe_adjoint *a, const double coef[8], do
     t.n = 0;
     s->tape = &t;
     int rc = FSIBE_OK;
+    // pipelined as fsibe_run's prescribed-motion loop: the host prepares
+    // step j+1 while step j runs and reads each step's results one step late
     for (int j = 0; j < len && rc == FSIBE_OK; ++j) {
-      rc = run_one(s, false, false);
+      rc = run_one(s, true, false);
       if (rc == FSIBE_OK) {
         t.rec[j].dt1 = s->dt1;
         if (t.n != j + 1) rc = set_err(FSIBE_ESTATE, "adjoint: step %d was not recorded", k * a->chk + j);
@@ -383,6 +385,12 @@ extern "C" int fsibe_adjoint_gradient(fsibe_adjoint *a, const double coef[8], do
       }
     }
     s->tape = nullptr;
+    s->pipe = false;
+    if (s->res_pending) {  // the last step's results (the reverse sweep needs none of them)
+      if (hipStreamSynchronize(s->stream) != hipSuccess && rc == FSIBE_OK) rc = set_err(FSIBE_EHIP, "adjoint: sync failed");
+      parse_results(s, s->slot ^ 1);
+      s->res_pending = false;
+    }
     CHK(rc);
     for (int j = len - 1; j >= 0; --j) {
       const double *u1n = j == len - 1 ? s->u1 : t.u1 + (size_t)(j + 1) * t.N;
