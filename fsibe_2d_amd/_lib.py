@@ -90,7 +90,7 @@ EXPORTS = (
     "fsibe_solid_jacobian", "fsibe_solid_rhs", "fsibe_sim_solid", "fsibe_solid_wrapper", "fsibe_gravity_impulse",
     "fsibe_adjoint_create", "fsibe_adjoint_destroy", "fsibe_adjoint_objective", "fsibe_adjoint_gradient",
     "fsibe_adjoint_timings", "fsibe_backup_write", "fsibe_backup_read", "fsibe_inicond_write",
-    "fsibe_inicond_read", "fsibe_pressure_snapshot", "fsibe_save_field",
+    "fsibe_inicond_read", "fsibe_pressure_snapshot", "fsibe_save_field", "fsibe_save_gif",
 )
 
 _lib = None
@@ -182,6 +182,7 @@ def lib():
     L.fsibe_pressure_snapshot.argtypes = [vp, C.c_double, C.c_int, dp]
     L.fsibe_save_field.argtypes = [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_char_p,
                                    C.c_char_p]
+    L.fsibe_save_gif.argtypes = [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
     _lib = L
     return L
 

@@ -3,6 +3,8 @@
 //
 //   fsibe_pressure_snapshot  ComputePressureSnapshot (NavierStokes.f90:598-665)
 //   fsibe_save_field         SaveField (FieldExport.f90:612-648)
+//   fsibe_save_gif           SaveGIF (FieldExport.f90:305-384) with the GIF89a
+//                            writer of gif_util.f90
 
 // ---- ComputePressureSnapshot -------------------------------------------------
 // Not part of the time step: the static pressure of a stored vorticity level
@@ -188,5 +190,199 @@ extern "C" int fsibe_save_field(const char *path, const double *field, int nx, i
   }
   const bool ok = fclose(f) == 0;
   if (!ok) return set_err(FSIBE_EARG, "SaveField: writing %s failed", path);
+  return FSIBE_OK;
+}
+
+// ---- SaveGIF -----------------------------------------------------------------
+// A field as a 256-colour GIF: each value is mapped to a palette index,
+// 1 + nint(255 (f - min) / (max - min)) clamped to 1 .. 256 (minus one),
+// row ny-1 of the field at the top of the image (SaveGIF, FieldExport.f90:
+// 336-380), and the index raster is LZW-coded as gif_util.f90 does it.
+
+// Fortran nint: the nearest integer, halves away from zero
+static long fnint_l(double v) { return std::lround(v); }
+
+// the computed palettes of FargePalette (FieldExport.f90:1653-1711): 11
+// (Matlab "spring"), 13 (Matlab's standard map, SaveGIF's default) and 14
+// (13 in 16 flat bands); rgb[3][256].  The tabulated palettes 1-10, and 12
+// (whose first and last entries FargePalette leaves unset), are not built.
+static bool gif_palette(int ip, int rgb[3][256]) {
+  int p13[3][256];
+  for (int i = 1; i <= 256; ++i) {
+    const double t = (double)(i - 1) / 255.0;
+    int r, g, b;
+    if (t <= 0.20) {
+      r = 0; g = 0; b = 143 + (int)fnint_l(t * (255.0 - 143.0) * 5.0);
+    } else if (t > 0.20 && t <= 0.40) {
+      r = 0; g = (int)fnint_l((t - 0.20) * 255.0 * 5.0); b = 255;
+    } else if (t > 0.40 && t <= 0.60) {
+      r = (int)fnint_l((t - 0.40) * 255.0 * 5.0); g = 255; b = 255 - (int)fnint_l((t - 0.40) * 255.0 * 5.0);
+    } else if (t > 0.60 && t <= 0.80) {
+      r = 255; g = 255 - (int)fnint_l((t - 0.60) * 255.0 * 5.0); b = 0;
+    } else {
+      r = 255 - (int)fnint_l((t - 0.80) * (255.0 - 128.0) * 5.0); g = 0; b = 0;
+    }
+    p13[0][i - 1] = r; p13[1][i - 1] = g; p13[2][i - 1] = b;
+    if (ip == 11) {
+      rgb[0][i - 1] = (int)fnint_l(t * 255.0);
+      rgb[1][i - 1] = (int)fnint_l(255.0 - t * 255.0);
+      rgb[2][i - 1] = 255;
+    }
+  }
+  if (ip == 11) return true;
+  if (ip == 13) {
+    memcpy(rgb, p13, sizeof p13);
+    return true;
+  }
+  if (ip == 14) {  // 16 bands of nint(255/16) = 16 entries, each the colour of its first
+    const int n = 16, sector = (int)fnint_l(255.0 / (double)n);
+    for (int j = 0; j < n; ++j)
+      for (int i = j * sector + 1; i <= (j + 1) * sector; ++i)
+        for (int c = 0; c < 3; ++c) rgb[c][i - 1] = p13[c][j * sector];
+    return true;
+  }
+  return false;
+}
+
+// The LZW raster of gif_util.f90 (giflzw, slicewrite, flushbuffer): 8-bit
+// pixels, codes from 9 bits up; a code's width grows when the next code to
+// be assigned passes the current maximum; at 4096 codes a clear code
+// restarts the table; data sub-blocks of at most 255 bytes.
+class GifLzw {
+ public:
+  explicit GifLzw(std::vector<unsigned char> &out) : out_(out) {}
+  void code(int c) {
+    accum_ += (unsigned long long)c << skip_;
+    skip_ += slen_;
+    buf_[ibuf_] = (unsigned char)(accum_ & 255u);
+    while (skip_ >= 8) {
+      ++ibuf_;
+      accum_ >>= 8;
+      skip_ -= 8;
+      buf_[ibuf_] = (unsigned char)(accum_ & 255u);
+    }
+    if (ibuf_ > 255) flush();
+  }
+  // writes up to 255 complete bytes, or at the end the partial last byte too
+  void flush() {
+    int bl;
+    if (ibuf_ > 255) bl = 255;
+    else if (skip_ != 0) bl = ibuf_;
+    else if (ibuf_ > 1) bl = ibuf_ - 1;
+    else return;
+    out_.push_back((unsigned char)bl);
+    out_.insert(out_.end(), buf_ + 1, buf_ + 1 + bl);
+    memmove(buf_ + 1, buf_ + 1 + bl, (size_t)(ibuf_ - bl));
+    ibuf_ -= bl;
+  }
+  int slen_ = 9;
+
+ private:
+  std::vector<unsigned char> &out_;
+  unsigned char buf_[260] = {0};  // buf_[1 .. ibuf_], buf_[ibuf_] the partial byte
+  int ibuf_ = 1, skip_ = 0;
+  unsigned long long accum_ = 0;
+};
+
+static void gif_raster(std::vector<unsigned char> &out, const std::vector<unsigned char> &px) {
+  const int blen = 8, maxbase = 255, CC = 256, EOI = 257, maxcode = 4095;
+  out.push_back((unsigned char)blen);
+  GifLzw w(out);
+  // the table: code of the sequence [P]K, looked up by (P << 8) | K
+  std::vector<int> table((size_t)(maxcode + 1) << 8, -1);
+  std::vector<int> used;  // entries to clear on a reset
+  int ncod = 0, curmaxcode = 0;
+  auto init = [&]() {
+    for (int e : used) table[(size_t)e] = -1;
+    used.clear();
+    ncod = CC + 2;
+    w.slen_ = blen + 1;
+    curmaxcode = (1 << w.slen_) - 1;
+  };
+  init();
+  w.code(CC);
+  int P = -1;
+  for (size_t q = 0; q < px.size(); ++q) {
+    const int K = px[q] & maxbase;
+    if (q == 0) {
+      P = K;
+      continue;
+    }
+    const int e = (P << 8) | K;
+    if (table[(size_t)e] >= 0) {
+      P = table[(size_t)e];
+      continue;
+    }
+    w.code(P);
+    if (ncod > maxcode) {
+      w.code(CC);
+      init();
+    } else {
+      if (ncod > curmaxcode) {
+        w.slen_ += 1;
+        curmaxcode = curmaxcode * 2 + 1;
+      }
+      table[(size_t)e] = ncod;
+      used.push_back(e);
+      ++ncod;
+    }
+    P = K;
+  }
+  w.code(P);
+  w.code(EOI);
+  w.flush();
+}
+
+extern "C" int fsibe_save_gif(const char *path, const double *field, int nx, int ny, int palette, int have_range,
+                              double color_min, double color_max) {
+  if (!path || !field || nx < 1 || ny < 1 || nx > 65535 || ny > 65535) return set_err(FSIBE_EARG, "bad argument");
+  int rgb[3][256];
+  if (!gif_palette(palette, rgb))
+    return set_err(FSIBE_EUNSUPPORTED, "SaveGIF: palette %d (only the computed palettes 11, 13 and 14)", palette);
+  double minf = color_min, maxf = color_max;
+  if (!have_range) {  // minval / maxval of the field
+    minf = maxf = field[0];
+    for (size_t q = 1; q < (size_t)nx * ny; ++q) {
+      minf = std::min(minf, field[q]);
+      maxf = std::max(maxf, field[q]);
+    }
+  }
+  // Pixel(i, j) for field(i, ny+1-j): raster rows from the field's last row
+  std::vector<unsigned char> px((size_t)nx * ny);
+  for (int j = 0; j < ny; ++j) {
+    const double *row = field + (size_t)(ny - 1 - j) * nx;
+    for (int i = 0; i < nx; ++i) {
+      const double v = 255.0 * (row[i] - minf) / (maxf - minf);
+      long it = std::isnan(v) ? 1 : 1 + fnint_l(v);
+      if (it < 1) it = 1;
+      if (it > 256) it = 256;
+      px[(size_t)j * nx + i] = (unsigned char)(it - 1);
+    }
+  }
+  std::vector<unsigned char> g;
+  const char hdr[] = "GIF89a";
+  g.insert(g.end(), hdr, hdr + 6);
+  auto u16 = [&](int v) { g.push_back((unsigned char)(v & 255)); g.push_back((unsigned char)((v >> 8) & 255)); };
+  u16(nx);
+  u16(ny);
+  g.push_back((unsigned char)(128 + 7 * 16 + 7));  // global colour table of 2^8 entries
+  g.push_back(0);                                  // background
+  g.push_back(0);                                  // aspect
+  for (int i = 0; i < 256; ++i)
+    for (int c = 0; c < 3; ++c) g.push_back((unsigned char)rgb[c][i]);
+  g.push_back(',');
+  u16(0);
+  u16(0);
+  u16(nx);
+  u16(ny);
+  g.push_back(7);  // no local table, not interlaced, 8-bit pixels
+  gif_raster(g, px);
+  g.push_back(0);
+  g.push_back(';');
+  std::string name = std::string(path) + ".gif";  // SaveGIF appends the extension
+  FILE *f = fopen(name.c_str(), "wb");
+  if (!f) return set_err(FSIBE_EARG, "SaveGIF: cannot open %s", name.c_str());
+  const bool ok = fwrite(g.data(), 1, g.size(), f) == g.size();
+  if (fclose(f) != 0 || !ok) return set_err(FSIBE_EARG, "SaveGIF: writing %s failed", name.c_str());
   return FSIBE_OK;
 }

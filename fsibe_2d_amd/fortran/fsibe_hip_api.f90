@@ -360,6 +360,15 @@ module fsibe_hip_api
       real(c_double), value :: xl, yl
       integer(c_int) :: rc
     end function
+    function fsibe_save_gif(path, field, nx, ny, palette, have_range, color_min, color_max) &
+        bind(C, name="fsibe_save_gif") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), intent(in) :: field(*)
+      integer(c_int), value :: nx, ny, palette, have_range
+      real(c_double), value :: color_min, color_max
+      integer(c_int) :: rc
+    end function
     function fsibe_adjoint_gradient(adj, coef, J, grad) bind(C, name="fsibe_adjoint_gradient") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: adj

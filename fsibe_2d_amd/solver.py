@@ -196,6 +196,17 @@ def save_field(path, field, nsave: int = 1, xl: float = 4.0, yl: float = 4.0, fi
                                  fieldtype.encode(), header.encode() if header is not None else None))
 
 
+def save_gif(path, field, palette: int = 13, color_min: float | None = None, color_max: float | None = None):
+    """SaveGIF (FieldExport.f90:305-384): path + ".gif" from a host field
+    [ny][nx], coloured by FargePalette's palette 11, 13 or 14 between
+    color_min and color_max (default: the field's own range)"""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    ny, nx = f.shape
+    rng = color_min is not None and color_max is not None
+    check(lib().fsibe_save_gif(os.fsencode(path), f, nx, ny, int(palette), int(rng),
+                               float(color_min) if rng else 0.0, float(color_max) if rng else 0.0))
+
+
 class _Member(FluidSim):
     """a member of a Batch (owned by the batch, not destroyed on its own)"""
 

@@ -400,6 +400,15 @@ int fsibe_pressure_snapshot(fsibe_sim *sim, double time, int level, double *pres
  * written blank-padded to 256 characters (NULL: none).  No GPU involved. */
 int fsibe_save_field(const char *path, const double *field, int nx, int ny, int nsave, double xl, double yl,
                      const char *fieldtype, const char *header);
+/* SaveGIF(field, filename, iColorTable, color_min, color_max)
+ * (FieldExport.f90:305-384, written by gif_util.f90's writegif) for a host
+ * field [ny][nx]: writes path + ".gif", a GIF89a of 256 colours with the
+ * field's last row at the top.  palette: FargePalette's computed palettes
+ * 11, 13 (SaveGIF's default) or 14 (FSIBE_EUNSUPPORTED for the tabulated
+ * ones); have_range = 0 takes minval/maxval of the field as SaveGIF does
+ * when color_min/color_max are absent.  No GPU involved. */
+int fsibe_save_gif(const char *path, const double *field, int nx, int ny, int palette, int have_range,
+                   double color_min, double color_max);
 
 /* HIP stream of the simulation (hipStream_t) for event timing */
 void *fsibe_sim_stream(fsibe_sim *sim);

@@ -6,6 +6,13 @@
 # logical ==) and Interpolation.f90 (needs nrutil, which flang rejects) are
 # unbuildable here and are not stubbed.
 #
+# oracle/_ref/save_gif_ref: the reference's SaveGIF (FieldExport.f90) and
+# GIF writer (gif_util.f90), compiled where they lie, driven by
+# oracle/ref_gif_driver.f90.  FieldExport.f90 also holds SaveField, which
+# calls BeamIO's WriteHeaderToFile; BeamIO needs the MKL solid solver and is
+# not built, and the driver never calls SaveField, so the link leaves that
+# one symbol unresolved instead of supplying it.
+#
 # Usage: make -f oracle/ref.mk   (from the repository root)
 REF ?= /root/reference
 FLANG ?= /opt/rocm/bin/amdflang
@@ -15,7 +22,7 @@ SRCS := share_vars cofdx cofdy cofdxdx cofdydy cofdxdy poisson cal_vis \
         dealiase_mask mean_velocity mouvement integrate_position
 OBJS := $(addprefix $(OUT)/,$(addsuffix .o,$(SRCS)))
 
-all: $(OUT)/libfsibe_ref.so
+all: $(OUT)/libfsibe_ref.so $(OUT)/save_gif_ref
 
 $(OUT)/%.o: $(REF)/%.f90 | $(OUT)
 	cd $(OUT) && $(FLANG) $(FFLAGS) -c $< -o $(notdir $@)
@@ -30,6 +37,15 @@ $(OUT)/ref_harness.o: oracle/ref_harness.f90 $(OUT)/mouvement.o | $(OUT)
 
 $(OUT)/libfsibe_ref.so: $(OBJS) $(OUT)/ref_harness.o
 	$(FLANG) -shared -o $@ $^
+
+$(OUT)/gif_util.o: $(REF)/gif_util.f90 | $(OUT)
+	cd $(OUT) && $(FLANG) $(FFLAGS) -c $< -o gif_util.o
+$(OUT)/FieldExport.o: $(REF)/FieldExport.f90 $(OUT)/share_vars.o $(OUT)/gif_util.o | $(OUT)
+	cd $(OUT) && $(FLANG) $(FFLAGS) -c $< -o FieldExport.o
+
+$(OUT)/save_gif_ref: oracle/ref_gif_driver.f90 $(OUT)/FieldExport.o $(OUT)/gif_util.o $(OUT)/share_vars.o
+	cd $(OUT) && $(FLANG) $(FFLAGS) -o save_gif_ref $(abspath oracle/ref_gif_driver.f90) FieldExport.o \
+	    gif_util.o share_vars.o -Wl,--unresolved-symbols=ignore-in-object-files
 
 $(OUT):
 	mkdir -p $(OUT)

@@ -55,3 +55,73 @@ def test_pressure_snapshot_of_taylor_green(oracle_mod):
     pe = (np.cos(2 * X) + np.cos(2 * Y)) / 4
     q = pe + 0.5 * u2
     np.testing.assert_allclose(p, pe - q.mean(), rtol=0, atol=1e-13)
+
+
+# ---- SaveGIF -------------------------------------------------------------------
+GIF_REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "save_gif_ref")
+
+
+def _gif_fields():
+    rng = np.random.default_rng(7)
+    y, x = np.mgrid[0:64, 0:96]
+    smooth = np.sin(x / 9.0) * np.cos(y / 7.0) + 0.3 * x / 96.0
+    noise = rng.standard_normal((150, 200))  # enough distinct runs to fill and clear the 4096-code table
+    return [("smooth", smooth, 0, None), ("noise", noise, 0, None), ("band14", smooth, 14, (-0.5, 0.9)),
+            ("spring11", noise[:40, :70], 11, None), ("clamped13", 3.0 * smooth, 13, (-1.0, 1.0))]
+
+
+def _save(path, f, pal, rng):
+    from fsibe_2d_amd.solver import save_gif
+
+    if rng is None:
+        save_gif(path, f, palette=pal or 13)
+    else:
+        save_gif(path, f, palette=pal, color_min=rng[0], color_max=rng[1])
+
+
+@pytest.mark.skipif(not os.path.exists(GIF_REF), reason="oracle/_ref/save_gif_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("case", range(5))
+def test_save_gif_matches_reference_bytes(tmp_path, case):
+    """the GIF of fsibe_save_gif against the one the reference's SaveGIF
+    (FieldExport.f90 with gif_util.f90, built from the reference's sources)
+    writes for the same field: identical bytes (palette, index mapping,
+    orientation, LZW codes, sub-blocks)"""
+    name, f, pal, rng = _gif_fields()[case]
+    ny, nx = f.shape
+    raw = tmp_path / "in.bin"
+    with open(raw, "wb") as h:
+        h.write(np.array([nx, ny, pal, 0 if rng is None else 1], dtype="<i4").tobytes())
+        h.write(np.array(rng if rng is not None else (0.0, 0.0), dtype="<f8").tobytes())
+        h.write(np.ascontiguousarray(f, dtype="<f8").tobytes())
+    subprocess.run([GIF_REF, str(raw), str(tmp_path / "ref")], check=True, capture_output=True)
+    _save(str(tmp_path / "ours"), f, pal, rng)
+    a = (tmp_path / "ours.gif").read_bytes()
+    b = (tmp_path / "ref.gif").read_bytes()
+    print(f"[gif {name}] {len(a)} bytes", flush=True)
+    assert a == b
+
+
+def test_save_gif_decodes_to_the_index_raster(tmp_path):
+    """an independent GIF decoder (Pillow) reads back the raster: size, the
+    field's last row on top, and the palette's ends (palette 13 runs from
+    (0, 0, 143) to (128, 0, 0))"""
+    from PIL import Image
+
+    y, x = np.mgrid[0:50, 0:300]
+    f = x + 0.001 * y  # every column its own colour band, rows distinguishable
+    _save(str(tmp_path / "g"), f, 13, None)
+    im = Image.open(tmp_path / "g.gif")
+    assert im.size == (300, 50) and im.mode == "P"
+    idx = np.array(im)
+    lo, hi = f.min(), f.max()
+    want = np.clip(1 + np.floor(255.0 * (f - lo) / (hi - lo) + 0.5).astype(int), 1, 256) - 1
+    assert np.array_equal(idx, want[::-1])
+    pal = np.array(im.getpalette()[:768]).reshape(256, 3)
+    assert tuple(pal[0]) == (0, 0, 143) and tuple(pal[255]) == (128, 0, 0)
+
+
+def test_save_gif_refuses_tabulated_palettes(tmp_path):
+    from fsibe_2d_amd import FsibeError
+
+    with pytest.raises(FsibeError):
+        _save(str(tmp_path / "g"), np.zeros((8, 8)), 5, None)
