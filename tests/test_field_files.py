@@ -67,7 +67,8 @@ def _gif_fields():
     smooth = np.sin(x / 9.0) * np.cos(y / 7.0) + 0.3 * x / 96.0
     noise = rng.standard_normal((150, 200))  # enough distinct runs to fill and clear the 4096-code table
     return [("smooth", smooth, 0, None), ("noise", noise, 0, None), ("band14", smooth, 14, (-0.5, 0.9)),
-            ("spring11", noise[:40, :70], 11, None), ("clamped13", 3.0 * smooth, 13, (-1.0, 1.0))]
+            ("spring11", noise[:40, :70], 11, None), ("clamped13", 3.0 * smooth, 13, (-1.0, 1.0)),
+            ("constant", np.full((16, 24), 0.25), 0, None)]  # 0/0: every pixel gets colour 0
 
 
 def _save(path, f, pal, rng):
@@ -80,7 +81,7 @@ def _save(path, f, pal, rng):
 
 
 @pytest.mark.skipif(not os.path.exists(GIF_REF), reason="oracle/_ref/save_gif_ref not built (needs the reference sources)")
-@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("case", range(6))
 def test_save_gif_matches_reference_bytes(tmp_path, case):
     """the GIF of fsibe_save_gif against the one the reference's SaveGIF
     (FieldExport.f90 with gif_util.f90, built from the reference's sources)
