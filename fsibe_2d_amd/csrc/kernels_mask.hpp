@@ -67,7 +67,7 @@ __device__ __forceinline__ double angle_deg(double g, double pi) {
 // sharp end marks "not touched" with -2 and otherwise overwrites.
 constexpr int kMaskTile = 8;
 constexpr int kMaskPix = kMaskTile * kMaskTile;  // pixels per tile = lanes of a wave
-constexpr int kMaskWaves = 4;
+constexpr int kMaskWaves = 8;
 constexpr int kMaskThreads = kMaskPix * kMaskWaves;
 constexpr int kMaxTilePrims = 1024;  // >= number of primitives (host checks)
 constexpr int kMaskStage = 64;       // primitives staged per pass (one pass for a tile of the beam)
