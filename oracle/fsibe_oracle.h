@@ -22,13 +22,14 @@
  * and its golden vectors; cal_velocity, cal_nlk (nonlinear, penalisation and
  * sponge terms), the CFL step, cal_energy, time_avg_vort and the integrator
  * by closed forms (Taylor-Green decay, AB2 order); SmoothStep, DrawADot, a
- * straight beam and the type-3 sponge by erf closed forms and symmetry;
+ * straight beam, the channel walls and the sponge masks of types 1, 2, 3
+ * and 10 by erf closed forms and symmetry;
  * phi*, DeltaInterpolation and GetForcesInterp by moment conditions and
  * linear / sinusoidal fields.
  * Parity unpinned (no independent check; GPU parity against this code only):
  * cal_drag's Force / Force_unst window sums, BeamSurfaceIntegral, the bent
- * beam parts of create_mask (DrawHinge sectors, DrawSharpEnd), walls, and the
- * sponge types 1, 2 and 10.
+ * beam parts of create_mask (DrawHinge sectors, DrawSharpEnd), and the
+ * iSponge = 10 term of cal_nlk.
  */
 #ifndef FSIBE_ORACLE_H
 #define FSIBE_ORACLE_H

@@ -1,7 +1,8 @@
 """create_mask / create_sponge_mask of the oracle pinned by closed forms
 (create_mask.f90): the erf step (SmoothStep :475-520), a cylinder drawn as
 one dot (:67-69, DrawADot :660-701), up-down symmetry and rigid velocity of a
-straight heaving beam, and the type-3 sponge (:315-330)."""
+straight heaving beam, the channel walls (:34-53), and the sponge masks of
+types 1, 2, 3 and 10 (:237-330)."""
 import numpy as np
 import pytest
 from scipy.special import erf
@@ -196,3 +197,62 @@ def test_zero_mean_pressure_forcing(oracle_mod):
     b10 = dt1 / dt0 * (0.5 * dt1 + dt0)
     b11 = -(0.5 * dt1 * dt1) / dt0
     np.testing.assert_allclose(np.array(d2.u_mean), um1 + b10 * f2 + b11 * f1, rtol=1e-12, atol=1e-18)
+
+
+def test_channel_walls(oracle_mod):
+    """iWalls = 1 (create_mask.f90:34-53): the bottom wall is the erf step
+    of the height y about h_channel (width N dy), the top wall the same step
+    of the distance yl - y to the top; 1 deep in the walls, 1/2 at y =
+    h_channel, 0 in the channel; every column alike"""
+    n, h, N, eps = 64, 0.5, 2.0, 1e-3
+    o = oracle_mod.Oracle(nx=n, ny=n, iBeam=0, iSponge=0, iWalls=1, h_channel=h, N_smooth=N, eps=eps)
+    o.create_mask(0.0)
+    m = o.field("mask") * eps
+    dy = 4.0 / n
+    y = np.arange(n) * dy
+    assert np.array_equal(m, np.repeat(m[:, :1], n, axis=1))
+    jb = int(np.floor((h + N * dy) / dy + 0.5))  # rows the bottom wall is drawn on
+    np.testing.assert_allclose(m[: jb + 1, 0], smooth(y[: jb + 1], h, N * dy), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(m[n - jb:, 0], smooth(4.0 - y[n - jb:], h, N * dy), rtol=0, atol=1e-15)
+    assert m[0, 0] == pytest.approx(1.0, abs=1e-15) and m[n // 2, 0] == 0.0
+    assert m[int(round(h / dy)), 0] == pytest.approx(0.5, abs=1e-15)
+    np.testing.assert_allclose(m[1: jb + 1, 0], m[n - 1: n - jb - 1: -1, 0], rtol=0, atol=1e-15)
+
+
+def test_outflow_inflow_and_sharp_sponges(oracle_mod):
+    """create_sponge_mask types 10, 1 and 2 (create_mask.f90:237-306):
+    10 is 1/eps_sponge on the columns from xl - S on; 1 (outflow) rises
+    from ~0 at xl - S towards 1 at the sponge's middle xl - S/2 as the erf
+    step of the distance to the middle about S/2 - N dx, and from the middle
+    on follows the step of its right half; 2 (inflow) is the same
+    step on both sides of its middle S/2, symmetric about it, ~0 at x = 0
+    and x = S and 0 beyond; all uniform in y"""
+    n, S, N, es = 64, 1.0, 2.0, 5e-3
+    dx = 4.0 / n
+    x = np.arange(n) * dx
+    ms = {}
+    for t in (10, 1, 2):
+        o = oracle_mod.Oracle(nx=n, ny=n, iBeam=0, iSponge=t, SpongeSize=S, N_smooth=N, eps_sponge=es)
+        o.create_mask(0.0)
+        o.create_sponge_mask()
+        ms[t] = o.field("mask_sponge") * es
+        assert np.array_equal(ms[t], np.repeat(ms[t][:1], n, axis=0)), t
+    r10 = ms[10][0]
+    i0 = int(np.floor((4.0 - S) / dx + 0.5))
+    assert np.all(r10[i0:] == 1.0) and np.all(r10[:i0] == 0.0)
+    r1 = ms[1][0]
+    imid = int(np.floor((4.0 - 0.5 * S) / dx + 0.5))
+    th = 0.5 * (S - 2.0 * N * dx)
+    np.testing.assert_allclose(r1[i0:imid], smooth(4.0 - 0.5 * S - x[i0:imid], th, N * dx), rtol=0, atol=1e-15)
+    # from the middle column on, the right half (drawn second, so it owns the
+    # middle column): the step of x - (xl - S/2) - dx about 0.4 (S - 2 N dx)
+    np.testing.assert_allclose(r1[imid:], smooth(x[imid:] - 4.0 + 0.5 * S - dx, 0.8 * th, N * dx), rtol=0,
+                               atol=1e-15)
+    assert r1[i0] < 0.01 and r1[imid - 1] == pytest.approx(1.0, abs=1e-6)  # the smoothing layer's tail
+    assert np.all(np.diff(r1[i0:imid]) >= 0.0) and np.all(r1[:i0] == 0.0)
+    r2 = ms[2][0]
+    m2 = int(np.floor(0.5 * S / dx + 0.5))
+    np.testing.assert_allclose(r2[: m2 + 1], smooth(0.5 * S - x[: m2 + 1], th, N * dx), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(r2[: m2 + 1], r2[2 * m2: m2 - 1: -1], rtol=0, atol=1e-15)
+    assert r2[0] < 0.01 and r2[m2] == pytest.approx(1.0, abs=1e-6)
+    assert np.all(r2[2 * m2 + 1:] == 0.0)
