@@ -26,10 +26,13 @@
  * and 10 by erf closed forms and symmetry;
  * phi*, DeltaInterpolation and GetForcesInterp by moment conditions and
  * linear / sinusoidal fields.
+ * cal_drag's Force by the integral of chi (u - u_s) over the grid, which
+ * it equals with the window grown to the whole mask.
  * Parity unpinned (no independent check; GPU parity against this code only):
- * cal_drag's Force / Force_unst window sums, BeamSurfaceIntegral, the bent
- * beam parts of create_mask (DrawHinge sectors, DrawSharpEnd), and the
- * iSponge = 10 term of cal_nlk.
+ * the exact bounds of cal_drag's reference window and Force_unst, the
+ * stress and pressure parts of BeamSurfaceIntegral, the bent beam parts of
+ * create_mask (DrawHinge sectors, DrawSharpEnd), and the iSponge = 10 term
+ * of cal_nlk.
  */
 #ifndef FSIBE_ORACLE_H
 #define FSIBE_ORACLE_H

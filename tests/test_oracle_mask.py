@@ -1,8 +1,9 @@
 """create_mask / create_sponge_mask of the oracle pinned by closed forms
 (create_mask.f90): the erf step (SmoothStep :475-520), a cylinder drawn as
 one dot (:67-69, DrawADot :660-701), up-down symmetry and rigid velocity of a
-straight heaving beam, the channel walls (:34-53), and the sponge masks of
-types 1, 2, 3 and 10 (:237-330)."""
+straight heaving beam, the channel walls (:34-53), the sponge masks of
+types 1, 2, 3 and 10 (:237-330), and cal_drag's window sums against the
+integral of the penalisation (NavierStokes.f90:512-593)."""
 import numpy as np
 import pytest
 from scipy.special import erf
@@ -256,3 +257,30 @@ def test_outflow_inflow_and_sharp_sponges(oracle_mod):
     np.testing.assert_allclose(r2[: m2 + 1], r2[2 * m2: m2 - 1: -1], rtol=0, atol=1e-15)
     assert r2[0] < 0.01 and r2[m2] == pytest.approx(1.0, abs=1e-6)
     assert np.all(r2[2 * m2 + 1:] == 0.0)
+
+
+def test_drag_window_sums(oracle_mod):
+    """cal_drag's Force (NavierStokes.f90:512-593) is dx dy times the sum of
+    the penalisation chi (u - u_s) over its window; BeamSurfaceIntegral's
+    first two entries are the same sum over the whole grid.  With the
+    window grown to the whole mask (iSmoothCutoff = 1) the two are equal,
+    and both equal the integral of chi (u - u_s) computed here from the
+    fields; with the reference's window (1.5 t_beam around the beam) the
+    smooth tails outside it are left out, so the window sum is smaller"""
+    res = {}
+    for smooth in (1, 0):
+        o = oracle_mod.Oracle(oracle_mod.make_params(nx=64, ny=64, ns=24, iSmoothCutoff=smooth, dt_fixed=2e-4))
+        o.set_vortk(o.init_noise(seed=0, amp=1e-3, kmax2=64))
+        o.run(5)
+        F = np.array(o.beam_arrays()["Force"])
+        bsi = np.array(list(o.diag().bsi)[:2])
+        m, vx, vy = o.field("mask"), o.field("maskvx"), o.field("maskvy")
+        u1, u2 = o.field("u1"), o.field("u2")
+        dxdy = (4.0 / 64) ** 2
+        full = np.array([np.sum(m * (u1 - vx)), np.sum(m * (u2 - vy))]) * dxdy
+        np.testing.assert_allclose(bsi, full, rtol=1e-12)
+        res[smooth] = (F, bsi)
+    F1, b1 = res[1]
+    assert np.array_equal(F1, b1)
+    F0, b0 = res[0]
+    assert np.all(np.abs(F0) < np.abs(b0))
