@@ -27,10 +27,11 @@
  * phi*, DeltaInterpolation and GetForcesInterp by moment conditions and
  * linear / sinusoidal fields.
  * cal_drag's Force by the integral of chi (u - u_s) over the grid, which
- * it equals with the window grown to the whole mask.
+ * it equals with the window grown to the whole mask; the pressure part of
+ * BeamSurfaceIntegral by a linear pressure field on a straight beam.
  * Parity unpinned (no independent check; GPU parity against this code only):
  * the exact bounds of cal_drag's reference window and Force_unst, the
- * stress and pressure parts of BeamSurfaceIntegral, the bent beam parts of
+ * stress part of BeamSurfaceIntegral, the bent beam parts of
  * create_mask (DrawHinge sectors, DrawSharpEnd), and the iSponge = 10 term
  * of cal_nlk.
  */

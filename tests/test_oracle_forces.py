@@ -108,3 +108,27 @@ def test_shear_stress_of_sinusoidal_flow(oracle_mod):
     assert np.abs(r["tau"] - tau_d).max() < 1e-12 * abs(tau_d)
     # sigma_n = -A = 0 on a horizontal beam, so the pressure load is 0
     assert np.abs(r["pressure"]).max() < 1e-15
+
+
+def test_surface_integral_pressure_of_a_linear_field(oracle_mod):
+    """the pressure part of BeamSurfaceIntegral (BeamForces.f90:306-489) on a
+    straight horizontal beam and p = a (y - y_beam), which the Delta
+    interpolation reproduces exactly: the two lines at +-t_beam give
+    -2 a t_beam ds per node, and the two end semicircles of radius
+    rad = t_beam + N dx (128 points each over pi, both ends included) give
+    -a rad^2 pi 129/127, the trapezoid sum of the circle's area; no
+    x-component"""
+    p = oracle_mod.make_params(nx=128, ny=128, ns=24)
+    o = oracle_mod.Oracle(p)
+    o.create_mask(0.0)
+    b = o.beam_arrays()
+    assert np.all(b["y"] == 2.0) and np.all(b["theta"] == 0.0)
+    n, dx, a = 128, 4.0 / 128, 0.7
+    Y = np.repeat((np.arange(n) * dx)[:, None], n, axis=1)
+    o.get_forces(0.0, a * (Y - 2.0), np.zeros((2, n, n)))
+    bsi = np.array(list(o.diag().bsi))
+    rad = p.t_beam + p.N_smooth * dx
+    ds = p.length / (p.ns - 1)
+    want = -a * (2.0 * p.t_beam * ds * p.ns + np.pi * rad ** 2 * 129.0 / 127.0)
+    assert abs(bsi[6]) < 1e-15 and bsi[9] == pytest.approx(want, rel=1e-13)
+    assert np.all(bsi[[4, 5, 7, 8]] == 0.0)  # no velocity: no stresses
