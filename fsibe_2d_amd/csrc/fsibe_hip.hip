@@ -2852,7 +2852,7 @@ static int batch_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>
       if (gx <= 0) continue;
       switch (k) {
         case B_MASK:
-          SLAUNCH(kb_mask, gx, nm, kMaskThreads, sizeof(int4) * g.nprim_cap, g.stream, mode,
+          SLAUNCH(kb_mask, gx, nm, kMaskThreadsBatch, sizeof(int4) * g.nprim_cap, g.stream, mode,
                   reinterpret_cast<const MaskArgs *>(arr(g, B_MASK)));
           break;
         case B_K1:

@@ -67,8 +67,11 @@ __device__ __forceinline__ double angle_deg(double g, double pi) {
 // sharp end marks "not touched" with -2 and otherwise overwrites.
 constexpr int kMaskTile = 8;
 constexpr int kMaskPix = kMaskTile * kMaskTile;  // pixels per tile = lanes of a wave
-constexpr int kMaskWaves = 8;
-constexpr int kMaskThreads = kMaskPix * kMaskWaves;
+// waves per tile: 8 for one simulation's mask (its ~200 tiles leave most
+// CUs free, so more waves per tile shorten the kernel), 4 for a batch's
+// (thousands of tiles fill the GPU, and smaller workgroups pack better)
+constexpr int kMaskWaves = 8, kMaskWavesBatch = 4;
+constexpr int kMaskThreads = kMaskPix * kMaskWaves, kMaskThreadsBatch = kMaskPix * kMaskWavesBatch;
 constexpr int kMaxTilePrims = 1024;  // >= number of primitives (host checks)
 constexpr int kMaskStage = 64;       // primitives staged per pass (one pass for a tile of the beam)
 
@@ -136,6 +139,7 @@ __device__ __forceinline__ double prim_value(const Prim &P, const MaskConsts &c,
 
 // owin (may be null): index of the primitive that set each pixel's value
 // (-1: none, the wall or zero), read by the adjoint of the mask
+template <int WAVES>
 __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np, const double *__restrict__ wallrow,
                                           const MaskConsts &c, int bx0, int by0, int bw, int bh,
                                           double *__restrict__ om, double *__restrict__ ovx,
@@ -149,7 +153,8 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
   const int tid = threadIdx.x, lane = tid & (kMaskPix - 1), wv = tid / kMaskPix;
   const int tx0 = bx0 + tbx * kMaskTile, ty0 = by0 + tby * kMaskTile;
   const int tx1 = tx0 + kMaskTile - 1, ty1 = ty0 + kMaskTile - 1;
-  for (int p = tid; p < np; p += kMaskThreads) {
+  constexpr int NTHR = kMaskPix * WAVES;
+  for (int p = tid; p < np; p += NTHR) {
     const int4 h = reinterpret_cast<const int4 *>(prims + p)[0];  // type ix0 ix1 iy0
     bnd[p] = make_int4(h.y, h.z, h.w, prims[p].iy1);
   }
@@ -203,12 +208,12 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
     // read once per pass: wave 0 rewrites both at the start of the next pass
     const int n = nstage, pn = pnext;
     if (n == 0) break;
-    for (int e = tid; e < n * (int)(sizeof(Prim) / 8); e += kMaskThreads) {
+    for (int e = tid; e < n * (int)(sizeof(Prim) / 8); e += NTHR) {
       const int q = e / (int)(sizeof(Prim) / 8), w = e - q * (int)(sizeof(Prim) / 8);
       reinterpret_cast<double *>(sp + q)[w] = reinterpret_cast<const double *>(prims + list[q])[w];
     }
     __syncthreads();
-    for (int q = wv; q < n; q += kMaskWaves) val[q][lane] = prim_value(sp[q], c, i, j, xi, yj);
+    for (int q = wv; q < n; q += WAVES) val[q][lane] = prim_value(sp[q], c, i, j, xi, yj);
     __syncthreads();
     if (wv == 0 && valid) {
       for (int q = 0; q < n; ++q) {
@@ -279,7 +284,7 @@ __global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ 
                                               const double *__restrict__ wallrow, MaskConsts c, int bx0, int by0,
                                               int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
                                               double *__restrict__ ovy, int *__restrict__ owin) {
-  mask_body(prims, np, wallrow, c, bx0, by0, bw, bh, om, ovx, ovy, owin, blockIdx.x, blockIdx.y);
+  mask_body<kMaskWaves>(prims, np, wallrow, c, bx0, by0, bw, bh, om, ovx, ovy, owin, blockIdx.x, blockIdx.y);
 }
 
 // batched: one record per simulation at blockIdx.y, tiles flattened on x
@@ -290,11 +295,12 @@ struct MaskArgs {
   MaskConsts c;
   int np, bx0, by0, bw, bh, tiles_x, tiles, pad;
 };
-__global__ void __launch_bounds__(kMaskThreads) kb_mask(const MaskArgs *__restrict__ arr) {
+__global__ void __launch_bounds__(kMaskThreadsBatch) kb_mask(const MaskArgs *__restrict__ arr) {
   const MaskArgs &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.tiles) return;
   const int tby = blockIdx.x / a.tiles_x, tbx = blockIdx.x - tby * a.tiles_x;
-  mask_body(a.prims, a.np, a.wallrow, a.c, a.bx0, a.by0, a.bw, a.bh, a.om, a.ovx, a.ovy, nullptr, tbx, tby);
+  mask_body<kMaskWavesBatch>(a.prims, a.np, a.wallrow, a.c, a.bx0, a.by0, a.bw, a.bh, a.om, a.ovx, a.ovy, nullptr, tbx,
+                             tby);
 }
 
 }  // namespace fsibe
