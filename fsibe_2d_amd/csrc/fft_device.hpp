@@ -8,7 +8,7 @@
 // the workgroup synchronises, and every thread writes the results back to the
 // autosorted positions; the result is in natural order after the last pass.
 //
-// Radices 8, 4, 2 and 3 are supported; the host picks the pass sequence
+// Radices 16, 8, 4, 2 and 3 are supported; the host picks the pass sequence
 // (FftPlan).  Twiddles: w = exp(-2 pi i k/(Ns R)) is read once per butterfly
 // from the table tw[m] = exp(-2 pi i m / Ltab) (stride Ltab/L) and its powers
 // w^2 .. w^(R-1) are formed by complex multiplication.
@@ -82,6 +82,38 @@ template <bool INV> struct Dft<8, INV> {
     v[2] = cadd(a[2], t2);   v[6] = csub(a[2], t2);
     v[3] = make_double2(fma(h, s3.x, a[3].x), fma(h, s3.y, a[3].y));
     v[7] = make_double2(fma(-h, s3.x, a[3].x), fma(-h, s3.y, a[3].y));
+  }
+};
+
+// 16 = 4 x 4: radix-4 DFTs of the four stride-4 subsequences, the twiddles
+// W16^(q k1), then radix-4 DFTs across them; output k1 + 4 k2
+template <bool INV> struct Dft<16, INV> {
+  __device__ __forceinline__ static void run(double2 *v) {
+    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;  // cos, sin (pi/8)
+    const double h = 0.70710678118654752440;                                // sqrt(1/2)
+    double2 a[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[q][r] = v[q + 4 * r];
+      Dft<4, INV>::run(a[q]);
+    }
+    // W16^m = exp(-+ 2 pi i m / 16) for m = 1, 2, 3, 6, 9 (m = 4 is -+i)
+    const double2 w1 = INV ? make_double2(c1, s1) : make_double2(c1, -s1);
+    const double2 w2 = INV ? make_double2(h, h) : make_double2(h, -h);
+    const double2 w3 = INV ? make_double2(s1, c1) : make_double2(s1, -c1);
+    const double2 w6 = INV ? make_double2(-h, h) : make_double2(-h, -h);
+    const double2 w9 = INV ? make_double2(-c1, -s1) : make_double2(-c1, s1);
+    a[1][1] = cmul(a[1][1], w1); a[1][2] = cmul(a[1][2], w2); a[1][3] = cmul(a[1][3], w3);
+    a[2][1] = cmul(a[2][1], w2); a[2][2] = mul_mi<INV>(a[2][2]); a[2][3] = cmul(a[2][3], w6);
+    a[3][1] = cmul(a[3][1], w3); a[3][2] = cmul(a[3][2], w6); a[3][3] = cmul(a[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      double2 b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Dft<4, INV>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
   }
 };
 
@@ -161,6 +193,7 @@ __device__ void lds_fft(double2 *__restrict__ buf, const FftPlan &plan, int batc
   for (int p = 0; p < plan.np; ++p) {
     const int R = plan.r[p];
     switch (R) {
+      case 16: stockham_pass<16, (EMAX + 15) / 16, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
       case 8: stockham_pass<8, (EMAX + 7) / 8, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
       case 4: stockham_pass<4, (EMAX + 3) / 4, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
       case 2: stockham_pass<2, (EMAX + 1) / 2, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
@@ -173,7 +206,7 @@ __device__ void lds_fft(double2 *__restrict__ buf, const FftPlan &plan, int batc
 
 // ---- compile-time plans ------------------------------------------------------------
 // The same pass sequence as the host's runtime planner: factors 3 first, then
-// (4,4) or 2 or 4 for the power of two left after the eights, then 8s.
+// 16 or 2 or 4 for the power of two left after the eights, then 8s.
 __host__ __device__ constexpr int cp_n3(int L) {
   int c = 0;
   while (L > 0 && L % 3 == 0) { L /= 3; ++c; }
@@ -187,13 +220,13 @@ __host__ __device__ constexpr int cp_e2(int L) {
 }
 __host__ __device__ constexpr int cp_np(int L) {
   const int n3 = cp_n3(L), e2 = cp_e2(L), a = e2 / 3, b = e2 % 3;
-  return n3 + a + ((b == 1 && a >= 1) ? 1 : (b != 0 ? 1 : 0));
+  return n3 + a + ((b == 1 && a >= 1) ? 0 : (b != 0 ? 1 : 0));
 }
 __host__ __device__ constexpr int cp_radix(int L, int p) {
   const int n3 = cp_n3(L), e2 = cp_e2(L), a = e2 / 3, b = e2 % 3;
   if (p < n3) return 3;
   p -= n3;
-  if (b == 1 && a >= 1) return p < 2 ? 4 : 8;
+  if (b == 1 && a >= 1) return p < 1 ? 16 : 8;
   if (b == 1) return p < 1 ? 2 : 8;
   if (b == 2) return p < 1 ? 4 : 8;
   return 8;

@@ -70,7 +70,8 @@ extern "C" const char *fsibe_last_error(void) { return g_err.c_str(); }
 // ---------------------------------------------------------------------------
 // planning
 // ---------------------------------------------------------------------------
-// radices for length L (8 first, then 4s, a 2, 3s); 0 if unsupported
+// radices for length L: 3s, then a 16, 2 or 4 for the power of two left
+// after the eights, then 8s (cp_radix); false if unsupported
 static bool make_plan(int L, FftPlan &p) {
   p = FftPlan{};
   p.L = L;
@@ -82,7 +83,7 @@ static bool make_plan(int L, FftPlan &p) {
   std::vector<int> r;
   for (int i = 0; i < n3; ++i) r.push_back(3);
   int a = e2 / 3, b = e2 % 3;
-  if (b == 1 && a >= 1) { a -= 1; r.push_back(4); r.push_back(4); }
+  if (b == 1 && a >= 1) { a -= 1; r.push_back(16); }
   else if (b == 1) r.push_back(2);
   else if (b == 2) r.push_back(4);
   for (int i = 0; i < a; ++i) r.push_back(8);
