@@ -254,6 +254,58 @@ def cpu_baseline(p, seconds: float, threads: int):
     return steps / el, steps, el
 
 
+def host_cpus():
+    """CPU model, os.cpu_count() and the CPUs this process may run on"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "usable_cpus": usable}
+
+
+def turbulence_bench(n: int, steps: int, warmup: int = 5):
+    """The solver-only workload: n^2 decaying 2-D turbulence (no mask, no
+    sponge, fluid at rest, nu = 1e-5, dt = 1e-4, domain 2 pi) from the
+    random-phase start (k0 = 12, unit energy, seed 1); steps/s with the
+    initial field resident in HBM, and the fraction of 8 TB/s on the
+    workload's algorithmic bytes 272 n^2 per step (6 transforms of 4 passes
+    of 8 n^2 bytes + 10 auxiliary field passes)."""
+    from fsibe_2d_amd import turbulence_4096
+    from fsibe_2d_amd.inputs import decaying_turbulence
+    from fsibe_2d_amd.solver import FluidSim
+
+    p = turbulence_4096(n)
+    sim = FluidSim(p)
+    sim.set_vortk(decaying_turbulence(n, seed=1))
+    sim.run(warmup)
+    hip_sync()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    hip_sync()
+    el = time.perf_counter() - t0
+    out = sim.step_out()
+    sim.close()
+    bytes_step = 272.0 * n * n
+    rate = steps / el
+    return {"workload": f"turbulence_{n}: {n}^2 solver-only decaying 2-D turbulence, domain 2 pi, nu=1e-5, "
+                        f"dt=1e-4, E(k) ~ k^4 exp(-2 (k/12)^2) unit energy, seed 1",
+            "value": round(rate, 3), "unit": "steps/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(1e3 * el / steps, 5),
+            "alg_bytes_per_step": bytes_step,
+            "step_alg_GBps": round(bytes_step * rate / 1e9, 1),
+            "step_alg_frac": round(bytes_step * rate / 1e9 / HBM_PEAK_GBS, 4),
+            "kinetic_energy_last": out["energies"][1]}
+
+
 def solid_step_us(p, steps: int = 2000):
     """host time of one solid-solver step (GravityImpulse + SolidSolverWrapper)
     of the workload's beam under a steady load, on this host"""
@@ -417,6 +469,8 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-sims", type=int, default=64, help="members of the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--adjoint-steps", type=int, default=1000, help="steps of the adjoint leg (0: skip)")
+    ap.add_argument("--turb-n", type=int, default=4096, help="grid of the solver-only turbulence leg")
+    ap.add_argument("--turb-steps", type=int, default=200, help="steps of the turbulence leg (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 20 device-to-device copies of MIB MiB (PMC traffic calibration)")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -538,11 +592,16 @@ def main(argv=None) -> int:
     cpu = None
     anchor = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+        hc = host_cpus()
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        # OMP_NUM_THREADS when the environment sets it (the host's CPU share
+        # for this process), else every CPU the process may run on
+        threads = omp if omp > 0 else hc["usable_cpus"]
         rate, k, sec = cpu_baseline(p, args.cpu_seconds, threads)
         anchor = fft_anchor(n, threads)
         ratio = anchor["oracle_over_scipy"]
         cpu = {"value": round(rate, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+               "threads_from": "OMP_NUM_THREADS" if omp > 0 else "sched_getaffinity", **hc,
                "sample": f"{k} coupled AB2 steps of the same {n}^2 FSI workload after startup + 1 step, "
                          f"{sec:.1f} s, C/OpenMP fluid oracle with {threads} threads, numpy solid oracle"
                          + (f"; one {n}^2 r2c FFT: oracle {anchor['oracle_rfft2_ms']} ms, "
@@ -554,6 +613,8 @@ def main(argv=None) -> int:
     if args.adjoint_steps > 0:
         adjoint = adjoint_bench(args.adjoint_steps, smooth=True)
         adjoint_ref = adjoint_bench(args.adjoint_steps, smooth=False)
+
+    turb = turbulence_bench(args.turb_n, args.turb_steps) if args.turb_steps > 0 else None
 
     step_bytes = 848.0 * n * n  # reference's per-step algorithmic bytes (19 transforms + 30 passes)
     rec = {
@@ -584,6 +645,7 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "cpu_fft_anchor": anchor,
         "batched": batched,
+        "turbulence": turb,
         "adjoint": adjoint,
         "adjoint_reference_cutoffs": adjoint_ref,
         "step_alg_GBps": round(step_bytes * steps_per_s / 1e9, 1),

@@ -6,6 +6,9 @@ the logarithm is finite.  The band-limited noise start replaces the
 reference's inicond cases (init_fields.f90:38-290), which have no random
 start: vort = amp * xi, vortk = coftxy(vort) with every (kx, ky) block with
 kx^2 + ky^2 > kmax2 zeroed, then multiplied by the dealiasing mask.
+
+decaying_turbulence() is the random-phase start of the solver-only 4096^2
+workload, defined directly in Fourier space (no GPU involved).
 """
 from __future__ import annotations
 
@@ -51,3 +54,90 @@ def band_limited_noise(spectral, dealiase, seed: int = 0, amp: float = 1e-3,
     blk = np.repeat(np.repeat(drop, 2, axis=0), 2, axis=1)
     vk[blk] = 0.0
     return vk * dealiase
+
+
+def uniforms(seed: int, n: int) -> np.ndarray:
+    """n uniforms u = (x >> 11) 2^-53 in [0, 1) of SplitMix64(seed)"""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def half_to_packed(z: np.ndarray) -> np.ndarray:
+    """Packed real layout of coftxy (cof_fftw33.f90:167-189) from the Fourier
+    coefficients c(kx, ky) of f = sum c e^{i(kx x + ky y)}: z[j, kx] for
+    kx = 0 .. nx/2-1 and j = 0 .. ny-1 in FFT order (ky = j, or j - ny for
+    j > ny/2).  Both Nyquist modes are dropped, as coftxy drops them.  Per
+    (kx, ky) block with p + iq = c(kx, ky), r + is = c(kx, -ky):
+
+        P[2ky, 2kx] = (p + r)/2      P[2ky, 2kx+1]   = (q + s)/2
+        P[2ky+1, 2kx] = (q - s)/2    P[2ky+1, 2kx+1] = (r - p)/2
+
+    and the row ky = 0 holds p, q at (0, 2kx), (0, 2kx+1) with row 1 zero."""
+    ny, H = z.shape
+    nx = 2 * H
+    P = np.zeros((ny, nx))
+    zp = z[1:ny // 2]                 # ky = 1 .. ny/2-1
+    zm = z[ny - 1:ny // 2:-1]         # ky = -1 .. -(ny/2-1)
+    p, q, r, s = zp.real, zp.imag, zm.real, zm.imag
+    P[2:ny:2, 0::2] = 0.5 * (p + r)
+    P[2:ny:2, 1::2] = 0.5 * (q + s)
+    P[3:ny:2, 0::2] = 0.5 * (q - s)
+    P[3:ny:2, 1::2] = 0.5 * (r - p)
+    P[0, 0::2] = z[0].real
+    P[0, 1::2] = z[0].imag
+    return P
+
+
+def turbulence_spectrum(k, k0: float = 12.0):
+    """E(k) = k^4 exp(-2 (k/k0)^2), unnormalised"""
+    k = np.asarray(k, dtype=np.float64)
+    return k ** 4 * np.exp(-2.0 * (k / k0) ** 2)
+
+
+def decaying_turbulence(n: int, seed: int = 1, k0: float = 12.0, energy: float = 1.0,
+                        packed: bool = True):
+    """Vorticity of random-phase 2-D turbulence on the 2 pi x 2 pi periodic
+    n x n grid, with the energy spectrum E(k) ~ k^4 exp(-2 (k/k0)^2) and
+    (1/2) <|u|^2> = energy.
+
+    Modes: integer wavenumbers (kx, ky) != 0 with |kx|, |ky| < n/2 (the
+    Nyquist modes are absent, as in the solver's fields), grouped into shells
+    s = floor(|k| + 1/2).  Every mode of shell s carries kinetic energy
+    (1/2)|u_k|^2 = C E(s) / n_s, where n_s counts the shell's modes over the
+    full plane and C = energy / sum_s E(s); so the shell-summed spectrum is
+    exactly C E(s) and the mean energy exactly `energy`.  With u = curl psi,
+    |u_k|^2 = |k|^2 |psi_k|^2 and omega_k = |k|^2 psi_k, hence
+    |omega_k| = |k| sqrt(2 C E(s) / n_s).
+
+    Phases: phi = 2 pi u with u the SplitMix64(seed) uniforms drawn one per
+    entry of the half spectrum z[j, kx] (kx = 0 .. n/2-1 fastest, j = 0 .. n-1
+    in FFT order); omega_k = |omega_k| e^{i phi}.  The column kx = 0 is made
+    Hermitian, c(0, -ky) = conj(c(0, ky)), so the field is real.
+
+    Returns the packed layout of coftxy (packed=True) or the half spectrum z."""
+    if n % 2 or n < 4:
+        raise ValueError("n must be even and >= 4")
+    H = n // 2
+    j = np.arange(n)
+    ky = np.where(j < H, j, j - n).astype(np.float64)
+    kx = np.arange(H, dtype=np.float64)
+    kk = np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2)
+    present = np.ones((n, H), dtype=bool)
+    present[H, :] = False              # ky = -n/2 (Nyquist row)
+    present[0, 0] = False              # mean
+    shell = np.floor(kk + 0.5).astype(np.int64)
+    # shell counts over the full plane: a mode with kx > 0 stands for itself
+    # and its conjugate (-kx, -ky); on kx = 0, ky and -ky are both entries
+    weight = np.where(kx[None, :] > 0, 2, 1) * present
+    ns = np.bincount(shell.ravel(), weights=weight.ravel().astype(np.float64))
+    smax = ns.size
+    s_all = np.arange(smax, dtype=np.float64)
+    E = turbulence_spectrum(s_all, k0) * (ns > 0)
+    C = energy / E.sum()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        per_mode = np.where(ns > 0, C * E / ns, 0.0)
+    amp = kk * np.sqrt(2.0 * per_mode[shell]) * present
+    phi = 2.0 * np.pi * uniforms(seed, n * H).reshape(n, H)
+    z = amp * np.exp(1j * phi)
+    # Hermitian column kx = 0: c(0, -ky) = conj(c(0, ky))
+    z[H + 1:, 0] = np.conj(z[H - 1:0:-1, 0])
+    return half_to_packed(z) if packed else z

@@ -17,7 +17,9 @@ PKEYS = ("nx", "ny", "ns", "iMeanVelocity", "iSponge", "iBeam", "iWalls", "iCyli
          "heave_freq", "motion_coef", "iSmoothCutoff")
 
 
-def oracle_fsi(oracle_mod, p, vk, nsteps):
+def oracle_fsi(oracle_mod, p, vk, nsteps, history=None):
+    """the coupled loop on the oracle; history (a list) receives per step
+    (time, tip y, Newton iterations, Force)"""
     from oracle.solid import SolidOracle
 
     o = oracle_mod.Oracle(oracle_mod.make_params(**{k: getattr(p, k) for k in PKEYS}))
@@ -35,6 +37,8 @@ def oracle_fsi(oracle_mod, p, vk, nsteps):
         so.step(time, dt1)
         o.host_swap()
         time, dt0 = time + dt1, dt1
+        if history is not None:
+            history.append((time, so.y[-1], so.iter, tuple(o.beam_arrays()["Force"])))
     return o, so, time
 
 
@@ -155,3 +159,43 @@ def test_autotuned_tiles_do_not_change_results(oracle_mod, monkeypatch):
     assert np.array_equal(a[0], b[0])
     assert a[1] == b[1] and a[2] == b[2]
     assert np.array_equal(a[3], b[3])
+
+
+def test_workload_256_coupled_100_steps(oracle_mod):
+    """fsi_256(): 256^2, Re 200, the flexible beam of 64 nodes (mass ratio 1,
+    stiffness 0.1, BDF2) clamped at (1, 2) and heaving 0.1 sin(2 pi t),
+    band-limited noise start (seed 0), 100 coupled steps.  Every step: the
+    tip's y and cal_drag's Force within 1e-9 (relative, floor 1e-6 of the
+    run's maximum) and equal Newton iteration counts; after the last step
+    the vorticity within 1e-10 and the full comparison."""
+    from fsibe_2d_amd import fsi_256
+    from fsibe_2d_amd.solver import FluidSim
+
+    p = fsi_256()
+    o0 = oracle_mod.Oracle(nx=p.nx, ny=p.ny, iBeam=0, iSponge=0)
+    vk = o0.init_noise(seed=0, amp=1e-3, kmax2=64)
+    nsteps = 100
+    hist = []
+    o, so, t_o = oracle_fsi(oracle_mod, p, vk, nsteps, history=hist)
+    g = FluidSim(p)
+    g.set_vortk(vk)
+    tips, forces = [], []
+    for step in range(nsteps):
+        out = g.run(1)
+        t_g = g.time_state()[0]
+        assert t_g == pytest.approx(hist[step][0], rel=1e-14)
+        assert out["ibes_iter"] == hist[step][2], (step, out["ibes_iter"], hist[step][2])
+        tips.append(g.beam()["y"][-1])
+        forces.append(out["Force"])
+    tip_o = np.array([h[1] for h in hist]) - p.y0
+    tip_g = np.array(tips) - p.y0
+    assert _floor_close(tip_g, tip_o, 1e-9), np.abs(tip_g - tip_o).max()
+    f_o = np.array([h[3] for h in hist])
+    f_g = np.array(forces)
+    for c in range(2):
+        assert _floor_close(f_g[:, c], f_o[:, c], 1e-9), (c, np.abs(f_g[:, c] - f_o[:, c]).max())
+    print(f"[fsi_256 x100] tip deflection {tip_o[-1]:.4e} max tip err {np.abs(tip_g - tip_o).max():.1e}",
+          flush=True)
+    assert abs(so.y[-1] - so.y[0]) > 1e-5  # the beam bends: its tip lags the clamped end
+    _compare(g, o, so, t_o, "fsi_256 coupled x100")
+    g.close()
