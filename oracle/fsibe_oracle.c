@@ -1102,6 +1102,30 @@ void o_cal_velocity(o_sim *s, double time, const double *vortk, double *u,
 }
 
 /* cal_nlk: NavierStokes.f90:9-177 */
+/* the iSponge = 10 part of the x-penalisation (NavierStokes.f90:103-121):
+ * w = -mask_sponge * du1/dx (spectral derivative), and for ixmin <= ix <=
+ * ixmax (ixmin = nint((xl - SpongeSize)/dx), ixmax = nx - 16) the trapezoid
+ * integral of w from ixmin to ix along each row is added to pen1 */
+void o_sponge10_term(o_sim *s, const double *u1, double *pen1) {
+  const o_params *p = &s->p;
+  const int nx = s->nx, ny = s->ny;
+  const size_t N = (size_t)nx * ny;
+  const double dx = s->dx;
+  double *work2 = dalloc(N), *dk = dalloc(N);
+  o_coftxy(s, u1, work2); o_cofdx(s, work2, dk); o_cofitxy(s, dk, work2);
+  for (size_t k = 0; k < N; k++) work2[k] = -(s->mask_sponge[k] * work2[k]);
+  const int ixmin = fnint((p->xl - p->SpongeSize) / dx), ixmax = nx - 1 - 15;
+  for (int iy = 0; iy < ny; iy++) {
+    const double *w = work2 + (size_t)iy * nx;
+    for (int ix = ixmin; ix <= ixmax; ix++) {
+      double summe = 0.0;
+      for (int is = ixmin; is <= ix - 1; is++) summe = summe + 0.5 * (w[is] + w[is + 1]) * dx;
+      pen1[(size_t)iy * nx + ix] = pen1[(size_t)iy * nx + ix] + summe;
+    }
+  }
+  free(work2); free(dk);
+}
+
 void o_cal_nlk(o_sim *s, double time, double *dt1, double *nlk,
                const double *vortk, const double *u) {
   const o_params *p = &s->p;
@@ -1140,20 +1164,7 @@ void o_cal_nlk(o_sim *s, double time, double *dt1, double *nlk,
       s->fm_new[1] = c / (double)N;
       s->fm_set = 1;
     }
-    if (p->iSponge == 10) { /* :103-121: trapezoid integral of -chi_s du1/dx along x */
-      const int nx = s->nx, ny = s->ny;
-      o_coftxy(s, u1, work2); o_cofdx(s, work2, penvortk); o_cofitxy(s, penvortk, work2);
-      for (size_t k = 0; k < N; k++) work2[k] = -(s->mask_sponge[k] * work2[k]);
-      const int ixmin = fnint((p->xl - p->SpongeSize) / dx), ixmax = nx - 1 - 15;
-      for (int iy = 0; iy < ny; iy++) {
-        const double *w = work2 + (size_t)iy * nx;
-        for (int ix = ixmin; ix <= ixmax; ix++) {
-          double summe = 0.0;
-          for (int is = ixmin; is <= ix - 1; is++) summe = summe + 0.5 * (w[is] + w[is + 1]) * dx;
-          pen1[(size_t)iy * nx + ix] = pen1[(size_t)iy * nx + ix] + summe;
-        }
-      }
-    }
+    if (p->iSponge == 10) o_sponge10_term(s, u1, pen1); /* :103-121 */
     o_coftxy(s, pen2, work2); o_cofdx(s, work2, pen2);
     o_coftxy(s, pen1, work2); o_cofdy(s, work2, pen1);
   } else {
@@ -1478,6 +1489,14 @@ static void beam_surface_integral(o_sim *s, double time, o_beam *b,
   s->bsi[4] = Ft[0]; s->bsi[5] = Fn[0]; s->bsi[6] = Fp[0];
   s->bsi[7] = Ft[1]; s->bsi[8] = Fn[1]; s->bsi[9] = Fp[1];
   if (err) fprintf(stderr, "oracle: DeltaInterpolation target outside field\n");
+}
+
+/* BeamSurfaceIntegral of the held beam on the given stress and pressure
+ * fields (instead of the stresses GetForces derives from u); the 10 results
+ * land in the diagnostics (o_get_diag bsi) */
+void o_surface_integral(o_sim *s, double time, const double *u, const double *sa, const double *sb,
+                        const double *press) {
+  beam_surface_integral(s, time, &s->beam, u, sa, sb, press);
 }
 
 /* GetForces -> GetForcesInterp with SPL=3: BeamForces.f90:6-51, 63-288 */

@@ -29,11 +29,18 @@
  * cal_drag's Force by the integral of chi (u - u_s) over the grid, which
  * it equals with the window grown to the whole mask; the pressure part of
  * BeamSurfaceIntegral by a linear pressure field on a straight beam.
- * Parity unpinned (no independent check; GPU parity against this code only):
- * the exact bounds of cal_drag's reference window and Force_unst, the
- * stress part of BeamSurfaceIntegral, the bent beam parts of
- * create_mask (DrawHinge sectors, DrawSharpEnd), and the iSponge = 10 term
- * of cal_nlk.
+ * tests/test_oracle_pins.py pins the rest: DrawHinge's sectors (every
+ * branch pair of beta1/beta2, both turn directions) by the distance to a
+ * bent polyline computed by brute force; DrawSharpEnd by the product of its
+ * two erf steps on a straight beam; the iSponge = 10 term by the closed form
+ * of its trapezoid sum and its second-order convergence; cal_drag's window
+ * by a sharp mask it must hold and a one-column shift of its truncated
+ * bound, and Force_unst by a rigid heave's volume times acceleration; the
+ * stress part of BeamSurfaceIntegral by the tractions S.n of linear stress
+ * fields.  Each of these tests fails on a planted mistake in the routine it
+ * checks (a wrong branch, half weight, off-by-one window, nint for int,
+ * swapped sign or node, max instead of overwrite).
+ * Parity unpinned: none of the fluid step's routines.
  */
 #ifndef FSIBE_ORACLE_H
 #define FSIBE_ORACLE_H
@@ -124,6 +131,7 @@ void o_pressure_snapshot(o_sim *s, double time, int level, double *press);
 void o_cal_velocity(o_sim *s, double time, const double *vortk, double *u,
                     double *stream);
 /* u is (nx*ny*2): u(:,:,1) then u(:,:,2) */
+void o_sponge10_term(o_sim *s, const double *u1, double *pen1);
 void o_cal_nlk(o_sim *s, double time, double *dt1, double *nlk,
                const double *vortk, const double *u);
 void o_pressure(o_sim *s, double time, double dt1, const double *vortk,
@@ -139,6 +147,8 @@ void o_cal_drag(o_sim *s, double time, double dt1, const double *penal,
 double o_delta_phi4star(double x, double dx, double pi);
 double o_delta_interp(o_sim *s, double x_target, double y_target,
                       const double *field, int *err);
+void o_surface_integral(o_sim *s, double time, const double *u, const double *sa, const double *sb,
+                        const double *press);
 void o_get_forces(o_sim *s, double time, o_beam *b, const double *press,
                   const double *u);
 

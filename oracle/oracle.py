@@ -123,6 +123,9 @@ def lib():
         L.o_delta_interp.restype = C.c_double
         L.o_delta_interp.argtypes = [vp, C.c_double, C.c_double, dp, C.POINTER(C.c_int)]
         L.o_get_forces.argtypes = [vp, C.c_double, C.POINTER(OBeam), dp, dp]
+        L.o_surface_integral.argtypes = [vp, C.c_double, dp, dp, dp, dp]
+        L.o_sponge10_term.argtypes = [vp, dp, dp]
+        L.o_cal_drag.argtypes = [vp, C.c_double, C.c_double, dp, C.POINTER(OBeam)]
         L.o_set_vortk.argtypes = [vp, dp]
         L.o_host_create_mask.argtypes = [vp, C.c_double]
         L.o_host_evolve.restype = C.c_double
@@ -302,6 +305,27 @@ class Oracle:
         (u: (2, ny, nx)); results land in the beam (pressure, tau, Force_press)."""
         self.L.o_get_forces(self.h, time, self.L.o_get_beam(self.h), self._as(press),
                             np.ascontiguousarray(u, dtype=np.float64))
+
+    def surface_integral(self, time, u, stress_a, stress_b, press):
+        """BeamSurfaceIntegral (BeamForces.f90:306-509) of the held beam on
+        the given fields; returns the 10 values of diag().bsi"""
+        self.L.o_surface_integral(self.h, time, np.ascontiguousarray(u, dtype=np.float64), self._as(stress_a),
+                                  self._as(stress_b), self._as(press))
+        return np.array(list(self.diag().bsi)[:10])
+
+    def sponge10_term(self, u1):
+        """the iSponge = 10 increment of the x-penalisation
+        (NavierStokes.f90:103-121) for the velocity u1 and the held
+        mask_sponge"""
+        out = np.zeros((self.ny, self.nx))
+        self.L.o_sponge10_term(self.h, self._as(u1), out)
+        return out
+
+    def cal_drag(self, time, dt1, penal):
+        """cal_drag (NavierStokes.f90:512-593) of the held beam and masks on
+        penal (2, ny, nx); Force / Force_unst land in the beam"""
+        self.L.o_cal_drag(self.h, time, dt1, np.ascontiguousarray(penal, dtype=np.float64),
+                          self.L.o_get_beam(self.h))
 
     # per-step calls for a host that moves the beam itself (iFLUSI=1)
     def host_create_mask(self, time):
