@@ -119,25 +119,38 @@ extern "C" int fsibe_backup_write(fsibe_sim *s, const char *path, int ivideo, do
   const size_t N = (size_t)s->nx * s->ny;
   const int n0 = s->n0, n1 = s->n1;
   std::vector<double> vk(2 * N), nl(2 * N), wv(2 * N, 0.0), mk(3 * N), u(2 * N), vm(N);
+  const bool restored = s->restored.size() == 9 * N;
+  if (restored) {
+    // no step since fsibe_backup_read: the file's own arrays
+    const double *r = s->restored.data();
+    std::copy(r, r + 2 * N, vk.begin());
+    std::copy(r + 2 * N, r + 4 * N, nl.begin());
+    std::copy(r + 4 * N, r + 6 * N, wv.begin());
+    std::copy(r + 6 * N, r + 9 * N, mk.begin());
+  }
   // vortk and nlk at the Fortran level indices 0, 1 the library also uses
-  CHK(download_packed(s, s->W[n0], vk.data() + n0 * N));
-  CHK(download_packed(s, s->W[n1], vk.data() + n1 * N));
-  CHK(download_packed(s, s->NL[n1], nl.data() + n1 * N));
-  CHK(download_packed(s, s->NL[n0], nl.data() + n0 * N));
+  if (!restored) CHK(download_packed(s, s->W[n0], vk.data() + n0 * N));
+  if (!restored) {
+    CHK(download_packed(s, s->W[n1], vk.data() + n1 * N));
+    CHK(download_packed(s, s->NL[n1], nl.data() + n1 * N));
+    CHK(download_packed(s, s->NL[n0], nl.data() + n0 * N));
+  }
   // workvis(n0) = vis(dt1) of the last step, workvis(n1) = vis(dt0) of the
   // step before (never set after the startup step alone); the reference's
   // nlk(n0) holds the previous nonlinear term times workvis of its step
   // (NavierStokes.f90:478-481), which the library applies on the fly
-  vis_packed_host(s, s->dt1, wv.data() + n0 * N);
-  if (s->it >= 1) {
-    vis_packed_host(s, s->dt0, wv.data() + n1 * N);
-    double *a = nl.data() + n0 * N;
-    const double *w = wv.data() + n1 * N;
-    for (size_t i = 0; i < N; ++i) a[i] = a[i] * w[i];
+  if (!restored) {
+    vis_packed_host(s, s->dt1, wv.data() + n0 * N);
+    if (s->it >= 1) {
+      vis_packed_host(s, s->dt0, wv.data() + n1 * N);
+      double *a = nl.data() + n0 * N;
+      const double *w = wv.data() + n1 * N;
+      for (size_t i = 0; i < N; ++i) a[i] = a[i] * w[i];
+    }
+    CHK(box_to_full(s, s->mask_m, s->mbox, mk.data()));
+    CHK(box_to_full(s, s->mask_vx, s->mbox, mk.data() + N));
+    CHK(box_to_full(s, s->mask_vy, s->mbox, mk.data() + 2 * N));
   }
-  CHK(box_to_full(s, s->mask_m, s->mbox, mk.data()));
-  CHK(box_to_full(s, s->mask_vx, s->mbox, mk.data() + N));
-  CHK(box_to_full(s, s->mask_vy, s->mbox, mk.data() + 2 * N));
   HIPCHK(hipMemcpyAsync(u.data(), s->u1, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipMemcpyAsync(u.data() + N, s->u2, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipMemcpyAsync(vm.data(), s->vor_mean, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
@@ -275,7 +288,7 @@ extern "C" int fsibe_backup_read(fsibe_sim *s, const char *path, int *ivideo, do
   s->time = t;
   s->dt1 = dt1;
   s->dt0 = dt1;
-  s->it = 0;
+  s->it = 0;  // the loop counter restarts at 0 after a restart, as time_step.f90:174 sets it
   s->startup_done = true;
   s->have_beam = false;  // create_mask runs again from the beam (time_step.f90:84)
   s->res_pending = false;
@@ -321,6 +334,12 @@ extern "C" int fsibe_backup_read(fsibe_sim *s, const char *path, int *ivideo, do
       s->solid->sv.pre_ok_ = false;
     }
   }
+  // the record's spectra and masks, for a backup written before the next step
+  s->restored.resize(9 * N);
+  std::copy(vk.begin(), vk.end(), s->restored.begin());
+  std::copy(nl.begin(), nl.end(), s->restored.begin() + 2 * N);
+  std::copy(wv.begin(), wv.end(), s->restored.begin() + 4 * N);
+  std::copy(mk.begin(), mk.end(), s->restored.begin() + 6 * N);
   if (ivideo) *ivideo = iv;
   if (colorscale) *colorscale = cs;
   if (colorscale_done) *colorscale_done = (cd != 0) ? 1 : 0;

@@ -33,6 +33,7 @@
 #include "kernels_adjoint.hpp"
 #include "kinematics.hpp"
 #include "solid.hpp"
+#include "wait.hpp"
 
 using namespace fsibe;
 
@@ -823,6 +824,14 @@ struct fsibe_sim {
   bool startup_done = false;
   fsibe_step_out last;
   std::vector<double> pbeam, tbeam;
+  // the spectral and mask arrays of the field record of the last
+  // fsibe_backup_read (vortk, nlk, workvis at both levels, mask, maskvx,
+  // maskvy; packed layout), kept until the state changes (a step, a new
+  // mask, a level swap, new vorticity): the device holds the spectra as half
+  // spectra, whose conversion back is exact only to rounding, and neither
+  // the scaled nlk(n0) nor workvis(n1) of the AB2 history, so a backup
+  // written before the next step writes these verbatim
+  std::vector<double> restored;
   fsibe_solid *solid = nullptr;  // iFLUSI=1: the beam solved by the library (fsibe_run)
   // time of the kinematics in the beam arrays (integrate_position's time)
   double beam_time = 0.0;
@@ -835,15 +844,24 @@ struct fsibe_sim {
 static void tape_mask(fsibe_sim *s);
 static int tape_fluid(fsibe_sim *s, bool explicit_step);
 
+static double now_ms();
+
 // host phases timed with the steady clock into the otherwise unused classes
 // wait for an event by polling: the waiting thread stays on its core, so the
 // host work that follows (the beam solve, the next geometry) starts at once
-// and with warm caches, where a blocking wait may let the core sleep
-static hipError_t wait_event(hipEvent_t ev) {
-  for (;;) {
-    const hipError_t e = hipEventQuery(ev);
-    if (e != hipErrorNotReady) return e;
-  }
+// and with warm caches, where a blocking wait may let the core sleep.  The
+// poll has a deadline (wait.hpp): FSIBE_EHIP naming `stage` if the event has
+// not completed by then.
+static int wait_event(hipEvent_t ev, const char *stage) {
+  hipError_t err = hipSuccess;
+  std::string msg;
+  const int rc = bounded_wait(
+      [&] {
+        err = hipEventQuery(ev);
+        return err == hipSuccess ? Poll::Done : err == hipErrorNotReady ? Poll::Busy : Poll::Failed;
+      },
+      now_ms, [&] { return hipGetErrorString(err); }, wait_timeout_ms(), stage, msg);
+  return rc == FSIBE_OK ? FSIBE_OK : set_err(rc, "%s", msg.c_str());
 }
 
 static double now_ms() {
@@ -1248,12 +1266,14 @@ extern "C" int fsibe_set_vortk(fsibe_sim *s, const double *vk) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   s->press_box_valid = false;
+  std::vector<double>().swap(s->restored);
   return FSIBE_OK;
 }
 
 extern "C" int fsibe_swap_levels(fsibe_sim *s) {
   if (!s) return set_err(FSIBE_EARG, "null argument");
   std::swap(s->n0, s->n1);
+  std::vector<double>().swap(s->restored);
   return FSIBE_OK;
 }
 
@@ -1562,6 +1582,7 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
                             bool with_forces) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
   if (s->ns > 0 && (!x || !y || !vx || !vy || !theta)) return set_err(FSIBE_EARG, "null beam array");
+  std::vector<double>().swap(s->restored);
   if (s->upload_pending) {
     HIPCHK(hipEventSynchronize(s->ev_upload));
     s->upload_pending = false;
@@ -1717,6 +1738,7 @@ static int launch_k2_ahead(fsibe_sim *s) {
 // sides and the row sums
 static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_step) {
   if (s->p.iBeam > 0 && !s->have_beam) return set_err(FSIBE_ESTATE, "fsibe_create_mask must be called first");
+  std::vector<double>().swap(s->restored);
   if (!s->have_beam)  // no beam: walls / cylinder only (create_mask.f90:34-69)
     CHK(fsibe_create_mask(s, time, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr));
   const int n0 = s->n0, n1 = s->n1;
@@ -2061,7 +2083,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     // the step's fluid kernels finish
     if (p.iFLUSI == 1 && p.iBeam > 0 && s->solid && s->time > p.T_release)
       (void)s->solid->sv.predict(s->time, s->solid->b);
-    HIPCHK(wait_event(s->ev_step));
+    CHK(wait_event(s->ev_step, "the step's results (pressure and force kernels)"));
     s->upload_pending = false;
     parse_results(s, s->slot);
     s->dt1 = s->last.dt1;
@@ -2071,7 +2093,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     prof_host(s, P_FORCES, th0);
     s->dt1 = s->last.dt1;
   } else {
-    HIPCHK(wait_event(s->ev_dt));
+    CHK(wait_event(s->ev_dt, "the step's time step dt1 (nonlinear-term kernel)"));
     s->upload_pending = false;  // the geometry upload precedes the nonlinear-term kernel
     s->dt1 = *reinterpret_cast<volatile const double *>(s->h_res + (size_t)s->slot * s->res_stride);
     if (s->res_pending) parse_results(s, s->slot ^ 1);  // finished before this step started
@@ -3079,18 +3101,31 @@ static int batch_run_groups(fsibe_batch *b, int nsteps) {
     rc = group_start(b, gi);
     if (rc == FSIBE_OK) busy[gi] = 1;
   }
+  const double timeout = wait_timeout_ms();
   for (;;) {
+    // the next group whose step has written its dt1, polled with the
+    // deadline of wait.hpp (a group that never completes is an error)
     int gi = -1;
     bool any = false;
-    for (int k = 0; k < ng && gi < 0; ++k) {
-      if (!busy[k]) continue;
-      any = true;
-      const hipError_t e = hipEventQuery(b->groups[k].ev_dt);
-      if (e == hipSuccess) gi = k;
-      else if (e != hipErrorNotReady) return set_err(FSIBE_EHIP, "batch: %s", hipGetErrorString(e));
-    }
+    for (int k = 0; k < ng; ++k) any = any || busy[k];
     if (!any) break;
-    if (gi < 0) continue;
+    hipError_t herr = hipSuccess;
+    std::string msg;
+    const int rw = bounded_wait(
+        [&] {
+          for (int k = 0; k < ng; ++k) {
+            if (!busy[k]) continue;
+            herr = hipEventQuery(b->groups[k].ev_dt);
+            if (herr == hipSuccess) {
+              gi = k;
+              return Poll::Done;
+            }
+            if (herr != hipErrorNotReady) return Poll::Failed;
+          }
+          return Poll::Busy;
+        },
+        now_ms, [&] { return hipGetErrorString(herr); }, timeout, "a batch step (nonlinear-term kernel)", msg);
+    if (rw != FSIBE_OK) return set_err(rw, "batch: %s", msg.c_str());
     busy[gi] = 0;
     --left[gi];
     const int rf = group_finish(b, gi);
