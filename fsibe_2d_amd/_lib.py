@@ -76,7 +76,7 @@ SOLID_ARRAYS = ("x", "y", "vx", "vy", "theta", "theta_dot", "ax", "ay", "pressur
 EXPORTS = (
     "fsibe_last_error", "fsibe_spectral_create", "fsibe_spectral_destroy", "fsibe_coftxy",
     "fsibe_cofitxy", "fsibe_cofdx", "fsibe_cofdy", "fsibe_cofdxdx", "fsibe_cofdydy",
-    "fsibe_cofdxdy", "fsibe_poisson", "fsibe_vis", "fsibe_dealiase_mask", "fsibe_dev_coftxy",
+    "fsibe_cofdxdy", "fsibe_poisson", "fsibe_vis", "fsibe_dealiase_mask", "fsibe_dev_coftxy", "fsibe_dev_coftxy_noalloc",
     "fsibe_dev_cofitxy", "fsibe_dev_spectral_op", "fsibe_dev_vis", "fsibe_spectral_sync",
     "fsibe_sim_create", "fsibe_sim_destroy", "fsibe_set_vortk", "fsibe_create_mask",
     "fsibe_evolve_explicit", "fsibe_evolve_ab2", "fsibe_get_forces", "fsibe_get_step_out",
@@ -117,6 +117,7 @@ def lib():
     L.fsibe_dealiase_mask.argtypes = [vp, dp]
     L.fsibe_dev_coftxy.argtypes = [vp, vp, vp, vp]
     L.fsibe_dev_cofitxy.argtypes = [vp, vp, vp, vp]
+    L.fsibe_dev_coftxy_noalloc.argtypes = [C.c_int, C.c_int, vp, vp, vp]
     L.fsibe_dev_spectral_op.argtypes = [vp, C.c_int, vp, vp, vp]
     L.fsibe_dev_vis.argtypes = [vp, C.c_double, vp, vp]
     L.fsibe_spectral_sync.argtypes = [vp]

@@ -438,6 +438,85 @@ extern "C" int fsibe_spectral_sync(fsibe_spectral *c) {
   return FSIBE_OK;
 }
 
+// ---- coftxy without allocation ----------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops, harnesses that forbid hipMalloc):
+// fsibe_dev_coftxy_noalloc runs the kernels of fsibe_dev_coftxy with the
+// intermediate half spectrum and the two twiddle tables in `scratch`, a
+// static device array of the library (each GPU holds its own copy until the
+// process ends).  Layout in double2 elements: the half spectrum
+// [0, kNaMaxPoints/2), then the x table (nx values), then the y table (ny
+// values).  Every element the kernels read is written first by the same
+// call, so nothing needs clearing.  Calls that overlap on one GPU race on it.
+static constexpr long long kNaMaxPoints = 2048LL * 2048LL;  // nx * ny
+static constexpr int kNaMaxLen = 4096;                       // nx and ny each
+static constexpr size_t kNaElems = (size_t)(kNaMaxPoints / 2) + 2 * (size_t)kNaMaxLen;
+static __device__ __align__(256) double2 scratch[kNaElems];
+
+// the device address of `scratch` per device (hipGetSymbolAddress once per
+// device, with that device current); the array's name is only a host-side
+// stand-in and is never passed to a kernel or copy
+static std::mutex g_na_mu;
+static std::map<int, double2 *> g_na_ptr;
+// the sizes whose twiddle tables a device's scratch holds (so repeated calls
+// of one size upload them once)
+static std::map<int, std::pair<int, int>> g_na_tw;
+
+static int noalloc_scratch(int dev, double2 **out) {
+  std::lock_guard<std::mutex> g(g_na_mu);
+  auto it = g_na_ptr.find(dev);
+  if (it != g_na_ptr.end()) {
+    *out = it->second;
+    return FSIBE_OK;
+  }
+  double2 *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&ptr), HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_err(FSIBE_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+  g_na_ptr[dev] = ptr;
+  *out = ptr;
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_dev_coftxy_noalloc(int nx, int ny, const double *d_f, double *d_fk, void *s) {
+  if (!d_f || !d_fk) return set_err(FSIBE_EARG, "null argument");
+  if (!size_ok(nx, ny))
+    return set_err(FSIBE_ESIZE, "grid %dx%d not supported (even sizes, nx/2 and ny = 2^a 3^b)", nx, ny);
+  if ((long long)nx * ny > kNaMaxPoints || nx > kNaMaxLen || ny > kNaMaxLen)
+    return set_err(FSIBE_ESIZE, "grid %dx%d exceeds the static buffer of fsibe_dev_coftxy_noalloc "
+                   "(nx*ny <= %lld, nx, ny <= %d)", nx, ny, kNaMaxPoints, kNaMaxLen);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  double2 *ptr = nullptr;
+  CHK(noalloc_scratch(dev, &ptr));
+  const hipStream_t st = (hipStream_t)s;
+  double2 *tmp = ptr, *twx = ptr + kNaMaxPoints / 2, *twy = twx + kNaMaxLen;
+  FftPlan px, py;
+  make_plan(nx / 2, px);
+  make_plan(ny, py);
+  {
+    std::lock_guard<std::mutex> g(g_na_mu);
+    auto it = g_na_tw.find(dev);
+    if (it == g_na_tw.end() || it->second != std::make_pair(nx, ny)) {
+      // the tables of fsibe_spectral_create, so the results are those of
+      // fsibe_dev_coftxy bit for bit; copied synchronously from host memory
+      const auto tx = twiddles(nx), ty = twiddles(ny);
+      HIPCHK(hipMemcpyAsync(twx, tx.data(), sizeof(double2) * nx, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(twy, ty.data(), sizeof(double2) * ny, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      g_na_tw[dev] = std::make_pair(nx, ny);
+    }
+  }
+  CHK(rows_r2c(nx, ny, px, twx, d_f, tmp, st));
+  const int H = nx / 2;
+  const double norm = 1.0 / ((double)nx * (double)ny);
+  return with_cols<1>(py, H, pick_tx(H, ny, 1, 1), [&](auto F, int nt, int TX) -> int {
+    const size_t lds = sizeof(double2) * TX * padlen(ny);
+    KLAUNCH(k_cols_fwd_pack<decltype(F)>, H / TX, nt, lds, st, tmp, d_fk, F, twy, H, norm);
+    return FSIBE_OK;
+  });
+}
+
 // host-pointer wrappers: copy in, run, copy out
 static int host_call(fsibe_spectral *c, const double *in, double *out, int kind, int which, double dt) {
   if (!c || !out || (kind != 3 && !in)) return set_err(FSIBE_EARG, "null argument");

@@ -92,6 +92,20 @@ int fsibe_dev_spectral_op(fsibe_spectral *ctx, int which, const double *d_in, do
 int fsibe_dev_vis(fsibe_spectral *ctx, double dt, double *d_work, void *stream);
 int fsibe_spectral_sync(fsibe_spectral *ctx);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): coftxy (cof_fftw33.f90:142-193) of the device field d_f
+ * (nx*ny reals) into d_fk (packed layout), bit-identical to
+ * fsibe_dev_coftxy, with no context and no allocation.  The intermediate half
+ * spectrum and the twiddle tables live in a static device buffer of the
+ * library (32 MiB + 128 KiB on each GPU it runs on, held until the process
+ * ends): nx*ny <= 2048*2048 and nx, ny <= 4096, else FSIBE_ESIZE.  The first
+ * call of a grid size on a device (and the first after a call of another
+ * size) uploads the twiddle tables and synchronises `stream` (NULL: the
+ * default stream); later calls only launch the two kernels, asynchronously.
+ * Calls that overlap on one GPU share the buffer and race. */
+int fsibe_dev_coftxy_noalloc(int nx, int ny, const double *d_f, double *d_fk, void *stream);
+
 /* ========================================================================= */
 /* Tier B -- device-resident fluid step (the performance path).               */
 /* Mirrors the reference's per-step calls (time_step.f90:187,193,199):       */
