@@ -61,18 +61,13 @@ def alg_bytes(name: str, nx: int, ny: int, visc: bool, box_rows: int, box_cols: 
 
 
 def hip_sync():
-    try:
-        import torch
+    """hipDeviceSynchronize of the HIP runtime the library links
+    (libamdhip64.so.7); torch bundles a second runtime, which is not used
+    here"""
+    from fsibe_2d_amd._lib import lib
 
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-            return
-    except Exception:
-        pass
-    try:
-        C.CDLL("libamdhip64.so").hipDeviceSynchronize()
-    except OSError:
-        pass
+    lib()
+    C.CDLL("libamdhip64.so.7").hipDeviceSynchronize()
 
 
 # kernel names in rocprofv3 output for each class of fsibe_profile_read
@@ -480,6 +475,8 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-mib", type=int, default=64)
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched workload (--batch-sims, --batch-steps); prints its JSON")
+    ap.add_argument("--turb-only", action="store_true",
+                    help="only the solver-only turbulence workload (--turb-n, --turb-steps); prints its JSON")
     args = ap.parse_args(argv)
     if args.summarize_trace:
         print(summarize_trace(args.summarize_trace))
@@ -500,6 +497,9 @@ def main(argv=None) -> int:
         return 0
     if args.batch_only:
         print(json.dumps(batch_bench(args.batch_sims, args.batch_steps)))
+        return 0
+    if args.turb_only:
+        print(json.dumps(turbulence_bench(args.turb_n, args.turb_steps)))
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

@@ -160,12 +160,18 @@ constexpr int ct_nt(int butterflies, int cap) {
 }
 constexpr int ct_tx(int ny) { return (ny == 1024 || ny == 2048) ? 2 : 1; }
 
+// threads of a row workgroup: one per radix-8 butterfly, at most 256, or
+// at most 1024 for rows of 1024 complex values and more, whose LDS
+// footprint (three sequences: 55-110 KB) leaves room for one or two
+// workgroups per CU, which then need enough waves of their own
+constexpr int row_nt(int H, int nseq) { return ct_nt(H * nseq / 8, H >= 1024 ? 1024 : 256); }
+
 // fn(policy, nthreads) for row transforms of length H holding NSEQ sequences
 template <int NSEQ, class Fn>
 static int with_rows(const FftPlan &px, Fn &&fn) {
   switch (px.L) {
 #define ROWCASE(HH) \
-    case HH: return fn(RowCT<HH, ct_nt(HH * NSEQ / 8, 256)>{px}, ct_nt(HH * NSEQ / 8, 256));
+    case HH: return fn(RowCT<HH, row_nt(HH, NSEQ)>{px}, row_nt(HH, NSEQ));
     ROWCASE(64) ROWCASE(128) ROWCASE(192) ROWCASE(256) ROWCASE(384) ROWCASE(512) ROWCASE(1024) ROWCASE(2048)
 #undef ROWCASE
     default: break;
@@ -1802,7 +1808,8 @@ static int launch_k2_ahead(fsibe_sim *s) {
   const size_t lds = sizeof(double2) * 3 * padlen(s->H);
   CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
     KLAUNCH(k_rows_nl<decltype(F)>, na + nb, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
-            s->sponge_on ? s->ms : nullptr, none, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
+            s->sponge_on ? s->ms : nullptr, none, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
+            F, s->twx, s->sc,
             s->umax_part, s->ny, s->dscal, s->fp, 0.0, 0.0, 0, res_dev(s, s->slot), 0, na, b1 + 1, 0,
             s->penal_on ? s->rowpen : nullptr);
     return FSIBE_OK;
@@ -1851,7 +1858,8 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     prof_begin(s, P_ROWS_NL);
     CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
       KLAUNCH(k_rows_nl<decltype(F)>, na, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
-              s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->T[3], F, s->twx, s->sc,
+              s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
+              F, s->twx, s->sc,
               s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), ra, na,
               0, 1, s->penal_on ? s->rowpen : nullptr);
       return FSIBE_OK;
@@ -1884,7 +1892,9 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     prof_begin(s, P_COLS_AB2);
     CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 1, 1), [&](auto F, int nt, int TX) -> int {
       const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
-      KLAUNCH(k_cols_ab2<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[2], s->T[3], s->W[n0], s->NL[n1],
+      // without penalisation K2 leaves Rpx unwritten (it is 0) and K4 skips it
+      KLAUNCH(k_cols_ab2<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[2], s->penal_on ? s->T[3] : nullptr,
+              s->W[n0], s->NL[n1],
               s->NL[n0], s->W[n1], s->dscal, (int)explicit_step, F, s->twy, s->sc);
       return FSIBE_OK;
     }));

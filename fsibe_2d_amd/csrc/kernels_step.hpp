@@ -371,13 +371,26 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
     }
   }
   __syncthreads();
-  F.template fft<false, 3>(lds, twx, tid, nt);
-  rows_r2c_store<3>(lds, H, PL, twx, tid, nt, [&](int k, const double2 *x) {
-    const double2 xa = x[0], xpx = x[1], xpy = x[2];
-    const double kf = (double)k * c.sx;
-    Rc[(size_t)y * H + k] = make_double2(xa.x - kf * xpy.y, xa.y + kf * xpy.x);
-    Rpx[(size_t)y * H + k] = xpx;
-  });
+  if (prow) {
+    F.template fft<false, 3>(lds, twx, tid, nt);
+    rows_r2c_store<3>(lds, H, PL, twx, tid, nt, [&](int k, const double2 *x) {
+      const double2 xa = x[0], xpx = x[1], xpy = x[2];
+      const double kf = (double)k * c.sx;
+      Rc[(size_t)y * H + k] = make_double2(xa.x - kf * xpy.y, xa.y + kf * xpy.x);
+      Rpx[(size_t)y * H + k] = xpx;
+    });
+  } else {
+    // no mask on this row: px = py = 0, whose transforms are 0, so only the
+    // first field is transformed (a sequence rounds alike in any batch) and
+    // Rc takes the same arithmetic with the zero spectra
+    F.template fft<false, 1>(lds, twx, tid, nt);
+    rows_r2c_store<1>(lds, H, PL, twx, tid, nt, [&](int k, const double2 *x) {
+      const double2 xa = x[0];
+      const double kf = (double)k * c.sx;
+      Rc[(size_t)y * H + k] = make_double2(xa.x - kf * 0.0, xa.y + kf * 0.0);
+      if (Rpx) Rpx[(size_t)y * H + k] = make_double2(0.0, 0.0);  // null: no penalisation anywhere
+    });
+  }
 }
 
 // ============================================================================
@@ -394,13 +407,16 @@ __device__ __forceinline__ void cols_ab2_body(const double2 *__restrict__ Rc, co
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
   const int kx0 = xcd_remap(bid, nblk) * TX;
   const int TT = 2 * TX;
+  // Rpx null: no penalisation anywhere in the simulation, its spectrum is 0
+  const bool px_on = Rpx != nullptr;
   for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
     lds[pidx(y) * TT + cc] = Rc[(size_t)y * H + kx0 + cc];
-    lds[pidx(y) * TT + TX + cc] = Rpx[(size_t)y * H + kx0 + cc];
+    if (px_on) lds[pidx(y) * TT + TX + cc] = Rpx[(size_t)y * H + kx0 + cc];
   }
   __syncthreads();
-  F.template fft<false, 2, 2>(lds, twy, tid, nt);
+  if (px_on) F.template fft<false, 2, 2>(lds, twy, tid, nt);
+  else F.template fft<false, 1, 2>(lds, twy, tid, nt);
   const double b10 = s->b10, b11 = s->b11, cx1 = s->cx1, cy1 = s->cy1, cx0 = s->cx0, cy0 = s->cy0;
   const int hy = ny >> 1;
   for (int e = tid; e < ny * TX; e += nt) {
@@ -411,7 +427,8 @@ __device__ __forceinline__ void cols_ab2_body(const double2 *__restrict__ Rc, co
       const int ky = ky_signed(idx, ny);
       const int aky = ky < 0 ? -ky : ky;
       const double kyf = (double)ky * c.sy;
-      const double2 fc = lds[pidx(idx) * TT + cc], fp = lds[pidx(idx) * TT + TX + cc];
+      const double2 fc = lds[pidx(idx) * TT + cc];
+      const double2 fp = px_on ? lds[pidx(idx) * TT + TX + cc] : make_double2(0.0, 0.0);
       const double2 Fc = make_double2(fc.x * c.norm, fc.y * c.norm);
       const double2 Fp = make_double2(fp.x * c.norm, fp.y * c.norm);
       nl = make_double2(Fc.x + kyf * Fp.y, Fc.y - kyf * Fp.x);
@@ -555,7 +572,16 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
     __syncthreads();
   }
   if (!mode) return;
-  block_sum<kNSUM>(acc, red);
+  if (mrow) {
+    block_sum<kNSUM>(acc, red);
+  } else {
+    // a row outside the mask box: the mask sums 2..10 are exactly 0, and
+    // the wave reductions (two LDS permutes per step and value) are skipped
+    double a2[2] = {acc[0], acc[1]};
+    block_sum<2>(a2, red);
+    acc[0] = a2[0];
+    acc[1] = a2[1];
+  }
   if (tid == 0)
     for (int i = 0; i < kNSUM; ++i) sums[(size_t)y * kNSUM + i] = acc[i];
   if (!pres) return;
