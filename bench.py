@@ -178,6 +178,34 @@ def summarize_pmc(fetch_dir: str, write_dir: str, cfetch_dir: str, cwrite_dir: s
     return "\n".join(out)
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_WAIT_INST_LDS",
+               "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES")
+
+
+def summarize_sq(d: str) -> str:
+    """per kernel (name and grid) of a rocprofv3 --pmc run of the SQ
+    counters: dispatches, waves per dispatch, and per wave the VALU and LDS
+    instructions, LDS bank-conflict cycles, cycles waiting to issue an LDS
+    instruction and cycles issuing VALU instructions; SQ_BUSY_CYCLES per
+    dispatch"""
+    acc, disp = {}, {}
+    for r in _csv_rows(_find(d, "counter_collection.csv")):
+        k = f"{_short(r['Kernel_Name'])} grid={r['Grid_Size']}"
+        acc.setdefault(k, {}).setdefault(r["Counter_Name"], 0.0)
+        acc[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp.setdefault(k, set()).add(r["Dispatch_Id"])
+    out = [" disp  waves/disp  valu/wave  lds/wave  bankconf/wave  waitlds/wave  activevalu/wave  busy/disp  kernel"]
+    for k in sorted(acc, key=lambda k: -acc[k].get("SQ_BUSY_CYCLES", 0.0)):
+        c, n = acc[k], len(disp[k])
+        w = c.get("SQ_WAVES", 0.0)
+        if w <= 0:
+            continue
+        out.append(f"{n:5d} {w / n:11.0f} {c.get('SQ_INSTS_VALU', 0) / w:10.0f} {c.get('SQ_INSTS_LDS', 0) / w:9.0f} "
+                   f"{c.get('SQ_LDS_BANK_CONFLICT', 0) / w:14.0f} {c.get('SQ_WAIT_INST_LDS', 0) / w:13.0f} "
+                   f"{c.get('SQ_ACTIVE_INST_VALU', 0) / w:16.0f} {c.get('SQ_BUSY_CYCLES', 0) / n:10.0f}  {k}")
+    return "\n".join(out)
+
+
 def fft_anchor(n: int, threads: int, reps: int = 10):
     """ms per 2-D r2c transform of an n x n FP64 field: the oracle's FFT and
     scipy.fft.rfft2 with the same number of workers, on this host."""
@@ -472,6 +500,8 @@ def main(argv=None) -> int:
                     help="print the per-kernel table of a rocprofv3 --kernel-trace --stats csv directory")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="print HBM bytes per dispatch from four rocprofv3 --pmc csv directories")
+    ap.add_argument("--summarize-sq", metavar="DIR",
+                    help="print per-kernel SQ counter figures of a rocprofv3 --pmc csv directory")
     ap.add_argument("--copy-mib", type=int, default=64)
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched workload (--batch-sims, --batch-steps); prints its JSON")
@@ -480,6 +510,9 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.summarize_trace:
         print(summarize_trace(args.summarize_trace))
+        return 0
+    if args.summarize_sq:
+        print(summarize_sq(args.summarize_sq))
         return 0
     if args.summarize_pmc:
         print(summarize_pmc(*args.summarize_pmc, args.copy_mib))

@@ -11,7 +11,7 @@
 // Radices 16, 8, 4, 2 and 3 are supported; the host picks the pass sequence
 // (FftPlan).  Twiddles: w = exp(-2 pi i k/(Ns R)) is read once per butterfly
 // from the table tw[m] = exp(-2 pi i m / Ltab) (stride Ltab/L) and its powers
-// w^2 .. w^(R-1) are formed by complex multiplication.
+// w^2 .. w^(R-1) are formed by complex multiplication (twiddle_row).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,7 +21,16 @@ struct FftPlan {
   int L;        // transform length
   int np;       // number of passes
   int r[16];    // radices, pass order
+  // the radices again, 4 bits per pass (radix_code), for the device: read
+  // with shifts, where an index into r[] at run time would place the plan
+  // in per-lane scratch memory
+  unsigned long long code;
 };
+
+// 4-bit code of a radix: 2 -> 0, 3 -> 1, 4 -> 2, 8 -> 3, 16 -> 4
+__host__ __device__ __forceinline__ constexpr int radix_code(int R) {
+  return R == 2 ? 0 : R == 3 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+}
 
 // padded position of element e and padded length of a sequence of L elements
 __host__ __device__ __forceinline__ constexpr int pidx(int e) { return e + (e >> 3); }
@@ -130,6 +139,21 @@ template <bool INV> struct Dft<3, INV> {
   }
 };
 
+// v[r] *= w1^r for r = 1 .. R-1.  The powers come by binary splitting,
+// w^r = w^(r/2) w^(r - r/2), so each is at most ceil(log2 r) products from
+// w1 (a chain w^r = w^(r-1) w1 would be r-1 products deep, with that many
+// roundings and a serial dependency); every pass kind calls this, so every
+// instance of a transform rounds alike.
+template <int R>
+__device__ __forceinline__ void twiddle_row(double2 *v, double2 w1) {
+  double2 w[R];
+  w[1] = w1;
+#pragma unroll
+  for (int r = 2; r < R; ++r) w[r] = cmul(w[r / 2], w[r - r / 2]);
+#pragma unroll
+  for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
+}
+
 // ---- one in-place Stockham pass ------------------------------------------------
 // Work item w in [0, L/R * batch) is decoded as (b, j) with b fastest when
 // BFAST (column tiles: b is the contiguous index) and j fastest otherwise.
@@ -156,13 +180,7 @@ __device__ __forceinline__ void stockham_pass(double2 *__restrict__ buf, int L, 
       if (Ns > 1) {
         double2 w1 = tw[k * tstep];
         if (INV) w1.y = -w1.y;
-        double2 wr = w1;
-        v[it][1] = cmul(v[it][1], wr);
-#pragma unroll
-        for (int r = 2; r < R; ++r) {
-          wr = cmul(wr, w1);
-          v[it][r] = cmul(v[it][r], wr);
-        }
+        twiddle_row<R>(v[it], w1);
       }
       Dft<R, INV>::run(v[it]);
     }
@@ -184,23 +202,28 @@ __device__ __forceinline__ void stockham_pass(double2 *__restrict__ buf, int L, 
 }
 
 // Runs every pass of `plan`.  EMAX bounds L*batch/nthr (elements per thread);
-// the host guarantees it.  Caller must __syncthreads() before (data in LDS);
-// the function ends with a barrier.
+// the host guarantees it, and gives plans with a radix-16 pass only to
+// instances with EMAX >= 16 (the others leave the 16-point butterfly out:
+// it would set their register budget).  Caller must __syncthreads() before
+// (data in LDS); the function ends with a barrier.
 template <int EMAX, bool INV, bool BFAST>
 __device__ void lds_fft(double2 *__restrict__ buf, const FftPlan &plan, int batch, int bs, int es,
                         const double2 *__restrict__ tw, int twmul, int tid, int nthr) {
   int Ns = 1;
+  const unsigned long long code = plan.code;
   for (int p = 0; p < plan.np; ++p) {
-    const int R = plan.r[p];
-    switch (R) {
-      case 16: stockham_pass<16, (EMAX + 15) / 16, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
-      case 8: stockham_pass<8, (EMAX + 7) / 8, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
-      case 4: stockham_pass<4, (EMAX + 3) / 4, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
-      case 2: stockham_pass<2, (EMAX + 1) / 2, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
-      case 3: stockham_pass<3, (EMAX + 2) / 3, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); break;
+    switch ((int)((code >> (4 * p)) & 15ull)) {
+      case 4:
+        if constexpr (EMAX >= 16)
+          stockham_pass<16, (EMAX + 15) / 16, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr);
+        Ns *= 16;
+        break;
+      case 3: stockham_pass<8, (EMAX + 7) / 8, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); Ns *= 8; break;
+      case 2: stockham_pass<4, (EMAX + 3) / 4, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); Ns *= 4; break;
+      case 0: stockham_pass<2, (EMAX + 1) / 2, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); Ns *= 2; break;
+      case 1: stockham_pass<3, (EMAX + 2) / 3, INV, BFAST>(buf, plan.L, batch, bs, es, Ns, tw, twmul, tid, nthr); Ns *= 3; break;
       default: break;
     }
-    Ns *= R;
   }
 }
 
@@ -258,13 +281,7 @@ __device__ __forceinline__ void fft_ct(double2 *__restrict__ buf, const double2 
         if constexpr (NS > 1) {
           double2 w1 = tw[(j % NS) * TSTEP];
           if (INV) w1.y = -w1.y;
-          double2 wr = w1;
-          v[it][1] = cmul(v[it][1], wr);
-#pragma unroll
-          for (int r = 2; r < R; ++r) {
-            wr = cmul(wr, w1);
-            v[it][r] = cmul(v[it][r], wr);
-          }
+          twiddle_row<R>(v[it], w1);
         }
         Dft<R, INV>::run(v[it]);
       }

@@ -91,6 +91,8 @@ static bool make_plan(int L, FftPlan &p) {
   if ((int)r.size() > 16) return false;
   p.np = (int)r.size();
   for (int i = 0; i < p.np; ++i) p.r[i] = r[i];
+  p.code = 0;
+  for (int i = 0; i < p.np; ++i) p.code |= (unsigned long long)radix_code(r[i]) << (4 * i);
   return true;
 }
 
@@ -112,8 +114,20 @@ struct Shape {
   int nt, emax;
 };
 
-static Shape shape_of(long long elems) {
-  // one radix-8 butterfly per thread: nt ~ elems / 8, between 256 and 1024
+static bool has_radix16(const FftPlan &p) {
+  for (int i = 0; i < p.np; ++i)
+    if (p.r[i] == 16) return true;
+  return false;
+}
+
+static Shape shape_of(long long elems, bool r16) {
+  // one radix-8 butterfly per thread: nt ~ elems / 8, between 256 and 1024;
+  // a plan with a radix-16 pass runs on the 256-thread instances (EMAX 16
+  // or 32), the only ones whose passes include the 16-point butterfly
+  if (r16) {
+    const long long e = (elems + kNT - 1) / kNT;
+    return Shape{kNT, e <= 16 ? 16 : e <= 32 ? 32 : -1};
+  }
   int nt = kNT;
   while (nt < 1024 && elems >= 16LL * nt) nt *= 2;
   long long e = (elems + nt - 1) / nt;
@@ -176,7 +190,7 @@ static int with_rows(const FftPlan &px, Fn &&fn) {
 #undef ROWCASE
     default: break;
   }
-  const Shape sh = shape_of((long long)NSEQ * px.L);
+  const Shape sh = shape_of((long long)NSEQ * px.L, has_radix16(px));
   switch (sh.emax) {
     case 4: return fn(RowRT<4>{px}, sh.nt);
     case 8: return fn(RowRT<8>{px}, sh.nt);
@@ -200,7 +214,7 @@ static int with_cols(const FftPlan &py, int H, int rt_tx, Fn &&fn) {
 #undef COLCASE
     default: break;
   }
-  const Shape sh = shape_of((long long)BM * rt_tx * py.L);
+  const Shape sh = shape_of((long long)BM * rt_tx * py.L, has_radix16(py));
   switch (sh.emax) {
     case 4: return fn(ColRT<4>{py, rt_tx}, sh.nt, rt_tx);
     case 8: return fn(ColRT<8>{py, rt_tx}, sh.nt, rt_tx);
@@ -236,7 +250,9 @@ static bool col_sel_valid(int ny, int H, int bm, ColSel c) {
     }
     return c.tx == ct_tx(ny) || ct_extra(ny, c.tx);
   }
-  return shape_of((long long)bm * c.tx * ny).emax > 0;
+  FftPlan pl;
+  make_plan(ny, pl);
+  return shape_of((long long)bm * c.tx * ny, has_radix16(pl)).emax > 0;
 }
 
 // fn(policy, nthreads, TX) for the column stage launch choice `sel`
@@ -254,7 +270,7 @@ static int with_cols_sel(const FftPlan &py, int H, ColSel sel, Fn &&fn) {
       default: return set_err(FSIBE_ESIZE, "no compile-time column plan for %d x %d", py.L, sel.tx);
     }
   }
-  const Shape sh = shape_of((long long)BM * sel.tx * py.L);
+  const Shape sh = shape_of((long long)BM * sel.tx * py.L, has_radix16(py));
   switch (sh.emax) {
     case 4: return fn(ColRT<4>{py, sel.tx}, sh.nt, sel.tx);
     case 8: return fn(ColRT<8>{py, sel.tx}, sh.nt, sel.tx);
@@ -2542,6 +2558,15 @@ extern "C" int fsibe_batch_destroy(fsibe_batch *b) {
 // the single path's launch choice of a column stage holding `bm` fields per
 // tile, of a kernel over `split` fields
 static ColSel static_sel(const fsibe_sim *s, int bm, int split) {
+  // the first column stage (two fields of one spectrum) of 1024-point
+  // columns: 4 columns per tile as the static choice (its LDS accesses see
+  // 64 bank-conflict cycles per wave against 192 with 2); the two are
+  // within a few percent in time and the tuner keeps whichever it measures
+  // 2% faster
+  if (bm == 1 && split == 2 && s->ny == 1024) {
+    const ColSel c4{1, 4};
+    if (col_sel_valid(s->ny, s->H, bm, c4)) return c4;
+  }
   const ColSel c{1, ct_tx(s->ny)};
   if (col_sel_valid(s->ny, s->H, bm, c)) return c;
   return ColSel{0, pick_tx(s->H, s->ny, bm, split)};
