@@ -166,10 +166,12 @@ struct o_sim {
   double *vortk[2], *nlk[2], *workvis[2];
   double *u;
   double *press, *stress_a, *stress_b, *vort;
-  o_beam beam;
+  o_beam beams[2]; /* beams(1:iBeam), iBeam <= 2 (init_beam.f90:48-76) */
+  int nb;
   double time, dt0, dt1, u_max;
   int n0, n1, it, started, startup_done;
   double energies[6], bsi[11], fx_int, fy_int;
+  double bsi_b[2][11], fx_int_b[2], fy_int_b[2]; /* per beam; bsi, fx_int, fy_int: the first beam's */
   double u_mean[2];
   double fmean[2][2];
   double fm_new[2]; /* f_mean of the last cal_nlk (NavierStokes.f90:97-100) */
@@ -229,7 +231,9 @@ o_sim *o_sim_create(const o_params *p) {
   if (p->idealis == 1) o_dealiase_mask(s, s->dealiase);
   /* time_step.f90:11,43,79 */
   s->n1 = 1; s->n0 = 0; s->time = 0.0; s->dt1 = 0.0; s->dt0 = 0.0;
-  if (p->iBeam > 0) o_init_beam(s, &s->beam);
+  s->nb = p->iBeam > 2 ? 2 : (p->iBeam > 0 ? p->iBeam : 0);
+  if (p->iBeam > 2) { fprintf(stderr, "oracle: at most two beams\n"); abort(); }
+  for (int ib = 0; ib < s->nb; ib++) o_init_beam_k(s, &s->beams[ib], ib);
   return s;
 }
 
@@ -241,7 +245,7 @@ void o_sim_destroy(o_sim *s) {
   free(s->mask_sponge); free(s->mask_frame); free(s->vor_mean);
   for (int i = 0; i < 2; i++) { free(s->vortk[i]); free(s->nlk[i]); free(s->workvis[i]); }
   free(s->u); free(s->press); free(s->stress_a); free(s->stress_b); free(s->vort);
-  if (s->p.iBeam > 0) o_free_beam(&s->beam);
+  for (int ib = 0; ib < s->nb; ib++) o_free_beam(&s->beams[ib]);
   free(s);
 }
 
@@ -694,8 +698,10 @@ void o_integrate_position(o_sim *s, double time, o_beam *b) {
   }
 }
 
-/* init_beam.f90:48-169 (one beam) */
-void o_init_beam(o_sim *s, o_beam *b) {
+/* init_beam.f90:48-169 for beam ib (0-based): the first at (x0, y0) with
+ * phase 0; the second ("dragonfly", :50-76) at (x0 + 1.25, y0), the same
+ * motion, which must be one of 300-307, with phase (iMotion - 300) pi / 4 */
+void o_init_beam_k(o_sim *s, o_beam *b, int ib) {
   const int ns = s->ns;
   memset(b, 0, sizeof(*b));
   b->x = dalloc(ns); b->y = dalloc(ns); b->vx = dalloc(ns); b->vy = dalloc(ns);
@@ -703,6 +709,20 @@ void o_init_beam(o_sim *s, o_beam *b) {
   b->pressure_new = dalloc(ns); b->tau_new = dalloc(ns);
   b->x0 = s->p.x0; b->y0 = s->p.y0; b->iMouvement = s->p.iMotion;
   b->AngleBeam = s->p.AngleBeam; b->phase = 0.0;
+  if (ib == 1) {
+    b->x0 = s->p.x0 + 1.25;
+    switch (b->iMouvement) {
+      case 300: b->phase = 0.00 * s->pi; break;
+      case 301: b->phase = 0.25 * s->pi; break;
+      case 302: b->phase = 0.50 * s->pi; break;
+      case 303: b->phase = 0.75 * s->pi; break;
+      case 304: b->phase = 1.00 * s->pi; break;
+      case 305: b->phase = 1.25 * s->pi; break;
+      case 306: b->phase = 1.50 * s->pi; break;
+      case 307: b->phase = 1.75 * s->pi; break;
+      default: fprintf(stderr, "oracle: error...iMotion bad bad (init_beam.f90:72-73)\n"); abort();
+    }
+  }
   double alpha, at, att, LE[6];
   o_mouvement(s, 0.0, b, &alpha, &at, &att, LE);
   for (int n = 0; n < ns; n++) {
@@ -712,6 +732,8 @@ void o_init_beam(o_sim *s, o_beam *b) {
   b->UnsteadyCorrectionsReady = 0;
   o_integrate_position(s, 0.0, b);
 }
+
+void o_init_beam(o_sim *s, o_beam *b) { o_init_beam_k(s, b, 0); }
 
 void o_free_beam(o_beam *b) {
   free(b->x); free(b->y); free(b->vx); free(b->vy); free(b->theta);
@@ -955,7 +977,7 @@ void o_create_mask(o_sim *s, double time, o_beam *b) {
   }
   if (s->p.iCylinder == 1) /* :67-69 */
     draw_a_dot(s, s->p.x0 - s->p.R_cylinder, s->p.y0, 0.0, 0.0, s->p.R_cylinder, N);
-  if (s->p.iBeam > 0) { /* :76-119 */
+  for (int ib = 0; ib < s->nb; ib++, b++) { /* :74-119, do ib = 1, iBeam */
     double alpha, at, att, LE[6];
     o_mouvement(s, time, b, &alpha, &at, &att, LE);
     const double t = s->p.t_beam;
@@ -1182,9 +1204,9 @@ void o_cal_nlk(o_sim *s, double time, double *dt1, double *nlk,
   free(work2); free(vdx); free(vdy); free(pen1); free(pen2); free(penvortk);
 }
 
-/* cal_drag: NavierStokes.f90:512-593 (one beam) */
-void o_cal_drag(o_sim *s, double time, double dt1, const double *penal,
-                o_beam *b) {
+/* cal_drag of one beam: NavierStokes.f90:523-589 (the loop body) */
+static void cal_drag_one(o_sim *s, double time, double dt1, const double *penal,
+                         o_beam *b) {
   const int nx = s->nx, ny = s->ny, ns = s->ns;
   const double dx = s->dx, dy = s->dy, norm = dx * dy;
   const o_params *p = &s->p;
@@ -1247,6 +1269,12 @@ void o_cal_drag(o_sim *s, double time, double dt1, const double *penal,
   b->drag_unst_old = b->drag_unst_new;
   b->lift_unst_old = b->lift_unst_new;
   b->UnsteadyCorrectionsReady = 1;
+}
+
+/* cal_drag: NavierStokes.f90:512-593; b points to beams(1:iBeam) */
+void o_cal_drag(o_sim *s, double time, double dt1, const double *penal,
+                o_beam *b) {
+  for (int ib = 0; ib < s->nb; ib++) cal_drag_one(s, time, dt1, penal, &b[ib]); /* :527-591 */
 }
 
 /* cal_energy: NavierStokes.f90:222-263 (values kept instead of appended) */
@@ -1415,7 +1443,7 @@ double o_delta_interp(o_sim *s, double xt, double yt, const double *field,
 /* BeamSurfaceIntegral: BeamForces.f90:306-509 */
 static void beam_surface_integral(o_sim *s, double time, o_beam *b,
                                   const double *u, const double *sa,
-                                  const double *sb, const double *press) {
+                                  const double *sb, const double *press, double *bsi) {
   const int nc = 128, ns = s->ns;
   const double dx = s->dx, dy = s->dy, pi = s->pi, ds = s->ds;
   const double mdx = fmax(dx, dy), rad = s->p.t_beam + s->p.N_smooth * mdx;
@@ -1482,12 +1510,12 @@ static void beam_surface_integral(o_sim *s, double time, o_beam *b,
   double sd = 0.0, sl = 0.0;
   for (size_t k = 0; k < N; k++) sd += s->mask[k] * (u[k] - s->maskvx[k]);
   for (size_t k = 0; k < N; k++) sl += s->mask[k] * (u[N + k] - s->maskvy[k]);
-  s->bsi[0] = sd * dx * dy;
-  s->bsi[1] = sl * dx * dy;
-  s->bsi[2] = Ft[0] + Fn[0] + Fp[0];
-  s->bsi[3] = Ft[1] + Fn[1] + Fp[1];
-  s->bsi[4] = Ft[0]; s->bsi[5] = Fn[0]; s->bsi[6] = Fp[0];
-  s->bsi[7] = Ft[1]; s->bsi[8] = Fn[1]; s->bsi[9] = Fp[1];
+  bsi[0] = sd * dx * dy;
+  bsi[1] = sl * dx * dy;
+  bsi[2] = Ft[0] + Fn[0] + Fp[0];
+  bsi[3] = Ft[1] + Fn[1] + Fp[1];
+  bsi[4] = Ft[0]; bsi[5] = Fn[0]; bsi[6] = Fp[0];
+  bsi[7] = Ft[1]; bsi[8] = Fn[1]; bsi[9] = Fp[1];
   if (err) fprintf(stderr, "oracle: DeltaInterpolation target outside field\n");
 }
 
@@ -1496,23 +1524,21 @@ static void beam_surface_integral(o_sim *s, double time, o_beam *b,
  * land in the diagnostics (o_get_diag bsi) */
 void o_surface_integral(o_sim *s, double time, const double *u, const double *sa, const double *sb,
                         const double *press) {
-  beam_surface_integral(s, time, &s->beam, u, sa, sb, press);
+  beam_surface_integral(s, time, &s->beams[0], u, sa, sb, press, s->bsi);
 }
 
 /* GetForces -> GetForcesInterp with SPL=3: BeamForces.f90:6-51, 63-288 */
+static void get_forces_one(o_sim *s, double time, o_beam *b, const double *press,
+                           const double *u, int ib);
+
+/* GetForces: BeamForces.f90:6-51, do i = 1, iBeam (b points to
+ * beams(1:iBeam)); the stresses (:105-124) are the same for every beam and
+ * computed once */
 void o_get_forces(o_sim *s, double time, o_beam *b, const double *press,
                   const double *u) {
-  const int ns = s->ns;
-  size_t N = (size_t)s->nx * s->ny;
-  const double dx = s->dx, dy = s->dy, ds = s->ds, nu = s->p.nu;
-  int visc = (s->p.iViscous == 1);
-  double alpha, at, att, LE[6];
-  int err = 0;
-  o_mouvement(s, time, b, &alpha, &at, &att, LE);
-  double *tau_beam = b->tau_new, *pb = b->pressure_new;
-  double *sigma_beam = dalloc(ns), *theta = dalloc(ns);
-  for (int n = 0; n < ns; n++) { tau_beam[n] = 0.0; pb[n] = 0.0; }
-  if (visc) { /* :105-124 */
+  const size_t N = (size_t)s->nx * s->ny;
+  const double nu = s->p.nu;
+  if (s->p.iViscous == 1) { /* :105-124 */
     double *w1 = dalloc(N), *w2 = dalloc(N), *w3 = dalloc(N);
     o_coftxy(s, u, w1); o_cofdx(s, w1, w2); o_cofitxy(s, w2, s->stress_a);
     for (size_t k = 0; k < N; k++) s->stress_a[k] = 2.0 * nu * s->stress_a[k];
@@ -1520,8 +1546,26 @@ void o_get_forces(o_sim *s, double time, o_beam *b, const double *press,
     o_coftxy(s, u + N, w1); o_cofdx(s, w1, w2); o_cofitxy(s, w2, s->stress_b);
     for (size_t k = 0; k < N; k++) s->stress_b[k] = nu * (w3[k] + s->stress_b[k]);
     free(w1); free(w2); free(w3);
-    beam_surface_integral(s, time, b, u, s->stress_a, s->stress_b, press); /* :129 */
   }
+  for (int ib = 0; ib < s->nb; ib++) get_forces_one(s, time, &b[ib], press, u, ib);
+  memcpy(s->bsi, s->bsi_b[0], sizeof(s->bsi));
+  s->fx_int = s->fx_int_b[0];
+  s->fy_int = s->fy_int_b[0];
+}
+
+/* GetForcesInterp with SPL=3 for beam ib: BeamForces.f90:63-288 */
+static void get_forces_one(o_sim *s, double time, o_beam *b, const double *press,
+                           const double *u, int ib) {
+  const int ns = s->ns;
+  const double dx = s->dx, dy = s->dy, ds = s->ds;
+  int visc = (s->p.iViscous == 1);
+  double alpha, at, att, LE[6];
+  int err = 0;
+  o_mouvement(s, time, b, &alpha, &at, &att, LE);
+  double *tau_beam = b->tau_new, *pb = b->pressure_new;
+  double *sigma_beam = dalloc(ns), *theta = dalloc(ns);
+  for (int n = 0; n < ns; n++) { tau_beam[n] = 0.0; pb[n] = 0.0; }
+  if (visc) beam_surface_integral(s, time, b, u, s->stress_a, s->stress_b, press, s->bsi_b[ib]); /* :129 */
   for (int n = 0; n < ns; n++) theta[n] = b->theta[n] + alpha; /* :150 */
   const double mdx = fmax(dy, dx);
   for (int n = 0; n < ns; n++) { /* :153-216, SPL=3 */
@@ -1564,7 +1608,7 @@ void o_get_forces(o_sim *s, double time, o_beam *b, const double *press,
     fy = fy + (cos(theta[i]) * pb[i] + sin(theta[i]) * tau_beam[i]) * ds;
   }
   b->Force_press[0] = fx; b->Force_press[1] = fy;
-  s->fx_int = fx; s->fy_int = fy;
+  s->fx_int_b[ib] = fx; s->fy_int_b[ib] = fy;
   if (err) fprintf(stderr, "oracle: DeltaInterpolation target outside field\n");
   free(sigma_beam); free(theta);
 }
@@ -1586,7 +1630,7 @@ static void require_no_fsi(o_sim *s) {
 }
 
 void o_startup_step(o_sim *s) {
-  o_beam *b = &s->beam;
+  o_beam *b = s->beams;
   require_no_fsi(s);
   if (!s->started) { /* time_step.f90:84,90 */
     o_create_mask(s, s->time, b);
@@ -1598,13 +1642,14 @@ void o_startup_step(o_sim *s) {
   o_evolve_explicit(s, s->time, &s->dt1, s->n0, s->n1, b);
   if ((s->p.iFLUSI == 1 || s->p.iSaveBeam > 0) && s->p.iBeam > 0)
     o_get_forces(s, s->time, b, s->press, s->u);
-  if (s->p.iFLUSI == 0 && s->p.iMotion > 0 && s->p.iBeam > 0) o_integrate_position(s, s->time, b);
+  if (s->p.iFLUSI == 0 && s->p.iMotion > 0) /* time_step.f90:215-218 */
+    for (int ib = 0; ib < s->nb; ib++) o_integrate_position(s, s->time, &b[ib]);
   int t = s->n1; s->n1 = s->n0; s->n0 = t;
   s->time = s->time + s->dt1;
 }
 
 void o_ab2_step(o_sim *s) {
-  o_beam *b = &s->beam;
+  o_beam *b = s->beams;
   require_no_fsi(s);
   s->dt0 = s->dt1;
   if ((s->p.iSponge >= 4 && s->time < s->p.T_fullspeed) || s->p.iFLUSI == 1 || s->p.iMotion > 0)
@@ -1612,7 +1657,8 @@ void o_ab2_step(o_sim *s) {
   o_evolve_ab2(s, s->time, s->dt0, &s->dt1, s->n0, s->n1, b);
   if ((s->p.iFLUSI == 1 || s->p.iSaveBeam > 0) && s->p.iBeam > 0)
     o_get_forces(s, s->time, b, s->press, s->u);
-  if (s->p.iFLUSI == 0 && s->p.iMotion > 0 && s->p.iBeam > 0) o_integrate_position(s, s->time, b);
+  if (s->p.iFLUSI == 0 && s->p.iMotion > 0) /* time_step.f90:215-218 */
+    for (int ib = 0; ib < s->nb; ib++) o_integrate_position(s, s->time, &b[ib]);
   int t = s->n1; s->n1 = s->n0; s->n0 = t;
   s->time = s->time + s->dt1;
   s->it++;
@@ -1621,7 +1667,7 @@ void o_ab2_step(o_sim *s) {
 /* The per-step calls of time_step.f90:187-199 for a host that moves the beam
  * itself (iFLUSI=1: its solid solver writes the beam arrays between steps). */
 void o_host_create_mask(o_sim *s, double time) {
-  o_create_mask(s, time, &s->beam);
+  o_create_mask(s, time, s->beams);
   if (!s->started) { /* after the startup mask (time_step.f90:84,90): sponge 22 reads it */
     o_create_sponge_mask(s);
     s->started = 1;
@@ -1630,12 +1676,12 @@ void o_host_create_mask(o_sim *s, double time) {
 double o_host_evolve(o_sim *s, double time, double dt0, int explicit_step) {
   double dt1 = 0.0;
   if (explicit_step)
-    o_evolve_explicit(s, time, &dt1, s->n0, s->n1, &s->beam);
+    o_evolve_explicit(s, time, &dt1, s->n0, s->n1, s->beams);
   else
-    o_evolve_ab2(s, time, dt0, &dt1, s->n0, s->n1, &s->beam);
+    o_evolve_ab2(s, time, dt0, &dt1, s->n0, s->n1, s->beams);
   return dt1;
 }
-void o_host_get_forces(o_sim *s, double time) { o_get_forces(s, time, &s->beam, s->press, s->u); }
+void o_host_get_forces(o_sim *s, double time) { o_get_forces(s, time, s->beams, s->press, s->u); }
 void o_host_swap(o_sim *s) {
   int t = s->n1; s->n1 = s->n0; s->n0 = t;
 }
@@ -1684,7 +1730,16 @@ void o_set_field(o_sim *s, int which, const double *in) {
   if (p) memcpy(p, in, sizeof(double) * (size_t)s->nx * s->ny);
 }
 
-o_beam *o_get_beam(o_sim *s) { return &s->beam; }
+o_beam *o_get_beam(o_sim *s) { return &s->beams[0]; }
+o_beam *o_get_beam_k(o_sim *s, int ib) { return (ib >= 0 && ib < s->nb) ? &s->beams[ib] : NULL; }
+int o_nbeams(const o_sim *s) { return s->nb; }
+/* BeamSurfaceIntegral and GetForcesInterp's force sums of beam ib from the
+ * last o_get_forces: bsi[0..9], then fx_int, fy_int */
+void o_get_beam_bsi(o_sim *s, int ib, double *out) {
+  for (int i = 0; i < 10; i++) out[i] = s->bsi_b[ib][i];
+  out[10] = s->fx_int_b[ib];
+  out[11] = s->fy_int_b[ib];
+}
 
 void o_get_diag(o_sim *s, o_diag *d) {
   memset(d, 0, sizeof(*d));

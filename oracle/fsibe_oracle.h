@@ -116,12 +116,14 @@ void o_mouvement(o_sim *s, double time, const o_beam *b, double *alpha,
                  double *alpha_t, double *alpha_tt, double LeadingEdge[6]);
 void o_integrate_position(o_sim *s, double time, o_beam *b);
 void o_init_beam(o_sim *s, o_beam *b); /* allocates arrays of length ns */
+void o_init_beam_k(o_sim *s, o_beam *b, int ib); /* beam ib of init_beam.f90:48-76 */
 void o_free_beam(o_beam *b);
 double o_mean_ux(o_sim *s, double time);
 double o_mean_uy(o_sim *s, double time);
 
 /* ---- geometry (create_mask.f90) -----------------------------------------*/
 void o_smoothstep(o_sim *s, double *f, double x, double t, double h);
+/* the beam routines take b = beams(1:iBeam), o_nbeams(s) consecutive beams */
 void o_create_mask(o_sim *s, double time, o_beam *b);
 void o_create_sponge_mask(o_sim *s);
 void o_frame_mask(o_sim *s, double *mask1);
@@ -172,7 +174,10 @@ enum {
 };
 void o_get_field(o_sim *s, int which, double *out); /* current n0 slab */
 void o_set_field(o_sim *s, int which, const double *in);
-o_beam *o_get_beam(o_sim *s);
+o_beam *o_get_beam(o_sim *s);           /* the first beam (of o_nbeams consecutive ones) */
+o_beam *o_get_beam_k(o_sim *s, int ib);
+int o_nbeams(const o_sim *s);
+void o_get_beam_bsi(o_sim *s, int ib, double *out);
 /* scalars: time, dt0, dt1, u_max, energies[6], bsi[11], n0, it */
 typedef struct o_diag {
   double time, dt0, dt1, u_max;

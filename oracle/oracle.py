@@ -110,6 +110,11 @@ def lib():
         L.o_cal_nlk.argtypes = [vp, C.c_double, C.POINTER(C.c_double), dp, dp, dp]
         L.o_get_beam.restype = C.POINTER(OBeam)
         L.o_get_beam.argtypes = [vp]
+        L.o_get_beam_k.restype = C.POINTER(OBeam)
+        L.o_get_beam_k.argtypes = [vp, C.c_int]
+        L.o_nbeams.restype = C.c_int
+        L.o_nbeams.argtypes = [vp]
+        L.o_get_beam_bsi.argtypes = [vp, C.c_int, dp]
         L.o_mouvement.argtypes = [vp, C.c_double, C.POINTER(OBeam), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), dp]
         L.o_integrate_position.argtypes = [vp, C.c_double, C.POINTER(OBeam)]
@@ -256,31 +261,52 @@ class Oracle:
     def beam(self):
         return self.L.o_get_beam(self.h).contents
 
-    def beam_arrays(self):
-        b = self.beam
+    @property
+    def nbeams(self):
+        return self.L.o_nbeams(self.h)
+
+    def _beam_ptr(self, ib):
+        if not 0 <= ib < max(self.nbeams, 1):
+            raise IndexError(f"beam {ib} of {self.nbeams}")
+        return self.L.o_get_beam_k(self.h, ib) if self.nbeams else self.L.o_get_beam(self.h)
+
+    def beam_arrays(self, ib=0):
+        b = self._beam_ptr(ib).contents
         ns = self.p.ns
         g = lambda ptr: np.ctypeslib.as_array(ptr, shape=(ns,)).copy()
         return dict(x=g(b.x), y=g(b.y), vx=g(b.vx), vy=g(b.vy), theta=g(b.theta),
                     theta_dot=g(b.theta_dot), pressure=g(b.pressure_new), tau=g(b.tau_new),
                     Force=tuple(b.Force), Force_unst=tuple(b.Force_unst),
-                    Force_press=tuple(b.Force_press))
+                    Force_press=tuple(b.Force_press), x0=b.x0, y0=b.y0, phase=b.phase)
 
-    def set_beam_arrays(self, **arrs):
-        b = self.beam
+    def set_beam_arrays(self, ib=0, **arrs):
+        b = self._beam_ptr(ib).contents
         ns = self.p.ns
         for k, v in arrs.items():
             dst = np.ctypeslib.as_array(getattr(b, k), shape=(ns,))
             dst[:] = np.asarray(v, dtype=np.float64)
 
-    def mouvement(self, time):
+    def beam_bsi(self, ib=0):
+        """BeamSurfaceIntegral's 10 values of beam ib from the last
+        get_forces, then GetForcesInterp's fx_int, fy_int"""
+        out = np.zeros(12)
+        self.L.o_get_beam_bsi(self.h, ib, out)
+        return out
+
+    def mouvement(self, time, ib=0):
         a, at, att = C.c_double(), C.c_double(), C.c_double()
         le = np.zeros(6)
-        self.L.o_mouvement(self.h, time, self.L.o_get_beam(self.h), C.byref(a), C.byref(at),
-                           C.byref(att), le)
+        self.L.o_mouvement(self.h, time, self._beam_ptr(ib), C.byref(a), C.byref(at), C.byref(att), le)
         return a.value, at.value, att.value, le
 
-    def integrate_position(self, time):
-        self.L.o_integrate_position(self.h, time, self.L.o_get_beam(self.h))
+    def integrate_position(self, time, ib=None):
+        """integrate_position of beam ib, or of every beam (ib None)"""
+        for k in (range(max(self.nbeams, 1)) if ib is None else (ib,)):
+            self.L.o_integrate_position(self.h, time, self._beam_ptr(k))
+
+    def beam_view(self, ib):
+        """the kinematics interface of one beam (for SolidOracle)"""
+        return BeamView(self, ib)
 
     def create_mask(self, time):
         self.L.o_create_mask(self.h, time, self.L.o_get_beam(self.h))
@@ -382,6 +408,26 @@ class Oracle:
         self.L.o_cal_nlk(self.h, time, C.byref(dt1), nlk, self._as(vk),
                          np.ascontiguousarray(u, dtype=np.float64))
         return nlk, dt1.value
+
+
+class BeamView:
+    """One beam of an Oracle, with the kinematics interface SolidOracle
+    reads (p, beam_arrays, set_beam_arrays, mouvement, integrate_position)."""
+
+    def __init__(self, o, ib):
+        self.o, self.ib, self.p = o, ib, o.p
+
+    def beam_arrays(self):
+        return self.o.beam_arrays(self.ib)
+
+    def set_beam_arrays(self, **arrs):
+        self.o.set_beam_arrays(self.ib, **arrs)
+
+    def mouvement(self, time):
+        return self.o.mouvement(time, self.ib)
+
+    def integrate_position(self, time):
+        self.o.integrate_position(time, self.ib)
 
 
 def normals(seed: int, n: int) -> np.ndarray:
