@@ -91,6 +91,8 @@ EXPORTS = (
     "fsibe_adjoint_create", "fsibe_adjoint_destroy", "fsibe_adjoint_objective", "fsibe_adjoint_gradient",
     "fsibe_adjoint_timings", "fsibe_backup_write", "fsibe_backup_read", "fsibe_inicond_write",
     "fsibe_inicond_read", "fsibe_pressure_snapshot", "fsibe_save_field", "fsibe_save_gif",
+    "fsibe_create_mask_beams", "fsibe_get_step_out_beam", "fsibe_beam_placement", "fsibe_integrate_position_beam",
+    "fsibe_solid_create_beam", "fsibe_sim_solid_beam",
 )
 
 _lib = None
@@ -184,6 +186,14 @@ def lib():
     L.fsibe_save_field.argtypes = [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_char_p,
                                    C.c_char_p]
     L.fsibe_save_gif.argtypes = [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.fsibe_create_mask_beams.argtypes = [vp, C.c_double, dp, dp, dp, dp, dp, dp, dp]
+    L.fsibe_get_step_out_beam.argtypes = [vp, C.c_int, C.POINTER(FsibeStepOut)]
+    L.fsibe_beam_placement.argtypes = [C.POINTER(FsibeParams), C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.fsibe_integrate_position_beam.argtypes = [C.POINTER(FsibeParams), C.c_int, C.c_double, dp, dp,
+                                                dp, dp, dp, dp]
+    L.fsibe_solid_create_beam.argtypes = [C.POINTER(FsibeParams), C.c_int, pvp]
+    L.fsibe_sim_solid_beam.argtypes = [vp, C.c_int, pvp]
     _lib = L
     return L
 

@@ -52,7 +52,7 @@ static void tape_mask(fsibe_sim *s) {
   const int j = t.n;
   AdjTape::Rec &r = t.rec[j];
   r.mbox = s->mbox;
-  r.win = s->win;
+  r.win = s->win[0];  // one beam: the adjoint runs iBeam = 1 only
   r.tau = s->beam_time;
   const size_t nb = s->mbox.empty() ? 0 : (size_t)s->mbox.w() * s->mbox.h();
   adj_copy_d(s->stream, s->mask_m, t.m + j * t.N, nb);
@@ -114,9 +114,10 @@ static int snap_save(fsibe_sim *s, SimSnap &q) {
   snap_dev(s, q, true);
   HIPCHK(hipGetLastError());
   q.time = s->time; q.dt0 = s->dt0; q.dt1 = s->dt1; q.beam_time = s->beam_time;
-  q.jsum_f = s->jsum_f; q.jsum_t = s->jsum_t; q.drag_unst_old = s->drag_unst_old; q.lift_unst_old = s->lift_unst_old;
+  q.jsum_f = s->jsum_f; q.jsum_t = s->jsum_t; q.drag_unst_old = s->drag_unst_old[0];
+  q.lift_unst_old = s->lift_unst_old[0];
   q.it = s->it; q.n0 = s->n0; q.n1 = s->n1; q.slot = s->slot;
-  q.startup_done = s->startup_done; q.have_beam = s->have_beam; q.unst_ready = s->unst_ready;
+  q.startup_done = s->startup_done; q.have_beam = s->have_beam; q.unst_ready = s->unst_ready[0];
   q.bx = s->bx; q.by = s->by; q.bvx = s->bvx; q.bvy = s->bvy; q.btheta = s->btheta; q.btheta_dot = s->btheta_dot;
   return FSIBE_OK;
 }
@@ -126,9 +127,10 @@ static int snap_restore(fsibe_sim *s, SimSnap &q) {
   snap_dev(s, q, false);
   HIPCHK(hipGetLastError());
   s->time = q.time; s->dt0 = q.dt0; s->dt1 = q.dt1; s->beam_time = q.beam_time;
-  s->jsum_f = q.jsum_f; s->jsum_t = q.jsum_t; s->drag_unst_old = q.drag_unst_old; s->lift_unst_old = q.lift_unst_old;
+  s->jsum_f = q.jsum_f; s->jsum_t = q.jsum_t; s->drag_unst_old[0] = q.drag_unst_old;
+  s->lift_unst_old[0] = q.lift_unst_old;
   s->it = q.it; s->n0 = q.n0; s->n1 = q.n1; s->slot = q.slot;
-  s->startup_done = q.startup_done; s->have_beam = q.have_beam; s->unst_ready = q.unst_ready;
+  s->startup_done = q.startup_done; s->have_beam = q.have_beam; s->unst_ready[0] = q.unst_ready;
   s->bx = q.bx; s->by = q.by; s->bvx = q.bvx; s->bvy = q.bvy; s->btheta = q.btheta; s->btheta_dot = q.btheta_dot;
   s->res_pending = false;
   s->slot_fluid[0] = s->slot_fluid[1] = false;

@@ -114,7 +114,6 @@ static int upload_packed(fsibe_sim *s, const double *host, double2 *dst) {
 extern "C" int fsibe_backup_write(fsibe_sim *s, const char *path, int ivideo, double colorscale,
                                   int colorscale_done) {
   if (!s || !path) return set_err(FSIBE_EARG, "null argument");
-  if (s->p.iBeam > 1) return set_err(FSIBE_EUNSUPPORTED, "backup: one beam at most");
   HIPCHK(hipStreamSynchronize(s->stream));
   const size_t N = (size_t)s->nx * s->ny;
   const int n0 = s->n0, n1 = s->n1;
@@ -165,23 +164,27 @@ extern "C" int fsibe_backup_write(fsibe_sim *s, const char *path, int ivideo, do
                            {mk.data(), 8 * N}, {mk.data() + N, 8 * N}, {mk.data() + 2 * N, 8 * N}, {&iv, 4},
                            {u.data(), 16 * N}}) &&
             frec_write(f, {{&ns, 4}}) && frec_write(f, {{&colorscale, 8}, {&cd, 4}});
-  if (ok && s->p.iBeam == 1) {
-    const size_t B = 8 * (size_t)ns;
-    const bool solid = s->p.iFLUSI == 1 && s->solid;
+  // six records per beam (save_fields.f90:132-139, do i = 1, iBeam)
+  for (int ib = 0; ok && ib < s->nb; ++ib) {
+    const size_t B = 8 * (size_t)ns, o = (size_t)ib * ns;
+    const bool solid = s->p.iFLUSI == 1 && s->solid[ib];
     std::vector<double> zero(6 * (size_t)ns, 0.0);
-    const ibes::Beam *b = solid ? &s->solid->b : nullptr;
-    const double *pn = solid ? b->pressure_new.data() : s->pbeam.data();
-    const double *po = solid ? b->pressure_old.data() : s->pbeam.data();
-    const double *tn = solid ? b->tau_new.data() : s->tbeam.data();
-    const double *to = solid ? b->tau_old.data() : s->tbeam.data();
+    const ibes::Beam *b = solid ? &s->solid[ib]->b : nullptr;
+    const double *pn = solid ? b->pressure_new.data() : s->pbeam.data() + o;
+    const double *po = solid ? b->pressure_old.data() : s->pbeam.data() + o;
+    const double *tn = solid ? b->tau_new.data() : s->tbeam.data() + o;
+    const double *to = solid ? b->tau_old.data() : s->tbeam.data() + o;
     const double Ek = solid ? b->E_kinetic : 0.0, Ep = solid ? b->E_pot : 0.0, Ee = solid ? b->E_elastic : 0.0;
-    const double x0 = s->p.x0, y0 = s->p.y0, ab = s->p.AngleBeam, dto = solid ? b->dt_old : 0.0;
-    const double dun = s->drag_unst_old, lun = s->lift_unst_old;
-    const int32_t imv = s->p.iMotion, ucr = s->unst_ready ? kFTrue : 0;
-    const double *F = s->last.Force, *Fu = s->last.Force_unst, *Fp = s->last.Force_press;
+    double x0 = 0.0, y0 = 0.0, ph = 0.0;
+    beam_placement(s->p, ib, x0, y0, ph);
+    const double ab = s->p.AngleBeam, dto = solid ? b->dt_old : 0.0;
+    const double dun = s->drag_unst_old[ib], lun = s->lift_unst_old[ib];
+    const int32_t imv = s->p.iMotion, ucr = s->unst_ready[ib] ? kFTrue : 0;
+    const fsibe_step_out &lb = s->last_b[ib];
+    const double *F = lb.Force, *Fu = lb.Force_unst, *Fp = lb.Force_press;
     const double *IF = solid ? b->Inertial_Force : zero.data();
-    ok = frec_write(f, {{s->bx.data(), B}, {s->by.data(), B}, {s->bvx.data(), B}, {s->bvy.data(), B},
-                        {s->btheta.data(), B}, {s->btheta_dot.data(), B}}) &&
+    ok = frec_write(f, {{s->bx.data() + o, B}, {s->by.data() + o, B}, {s->bvx.data() + o, B},
+                        {s->bvy.data() + o, B}, {s->btheta.data() + o, B}, {s->btheta_dot.data() + o, B}}) &&
          frec_write(f, {{pn, B}, {po, B}, {tn, B}, {to, B}}) &&
          frec_write(f, {{F, 16}, {Fu, 16}, {Fp, 16}, {&Ek, 8}, {&Ep, 8}, {&Ee, 8}, {&x0, 8}, {&y0, 8}}) &&
          frec_write(f, {{&ab, 8}, {&imv, 4}, {&dun, 8}, {&dun, 8}, {&lun, 8}, {&lun, 8}}) &&
@@ -198,7 +201,6 @@ extern "C" int fsibe_backup_write(fsibe_sim *s, const char *path, int ivideo, do
 extern "C" int fsibe_backup_read(fsibe_sim *s, const char *path, int *ivideo, double *colorscale,
                                  int *colorscale_done) {
   if (!s || !path) return set_err(FSIBE_EARG, "null argument");
-  if (s->p.iBeam > 1) return set_err(FSIBE_EUNSUPPORTED, "backup: one beam at most");
   const size_t N = (size_t)s->nx * s->ny;
   const int ns = s->ns;
   FILE *f = fopen(path, "rb");
@@ -245,22 +247,29 @@ extern "C" int fsibe_backup_read(fsibe_sim *s, const char *path, int *ivideo, do
     if (!c.done()) return fail("colorscale record");
   }
   const size_t B = 8 * (size_t)ns;
-  std::vector<double> arr(6 * (size_t)ns), ld(4 * (size_t)ns), sc(11), oo(6 * (size_t)ns), acc(2 * (size_t)ns), inf(2);
-  double ab = 0.0, un[4] = {0, 0, 0, 0}, dto = 0.0;
-  int32_t imv = 0, ucr = 0;
-  if (s->p.iBeam == 1) {
+  // the six records of each beam (init_fields.f90:72-79)
+  struct BeamRec {
+    std::vector<double> arr, ld, sc, oo, acc, inf;
+    double ab = 0.0, un[4] = {0, 0, 0, 0}, dto = 0.0;
+    int32_t imv = 0, ucr = 0;
+  };
+  BeamRec br[2];
+  for (int ib = 0; ib < s->nb; ++ib) {
+    BeamRec &q = br[ib];
+    q.arr.resize(6 * (size_t)ns); q.ld.resize(4 * (size_t)ns); q.sc.resize(11); q.oo.resize(6 * (size_t)ns);
+    q.acc.resize(2 * (size_t)ns); q.inf.resize(2);
     bool ok = frec_read(f, r);
-    if (ok) { FCursor c(r); c.get(arr.data(), 6 * B); ok = c.done(); }
+    if (ok) { FCursor c(r); c.get(q.arr.data(), 6 * B); ok = c.done(); }
     ok = ok && frec_read(f, r);
-    if (ok) { FCursor c(r); c.get(ld.data(), 4 * B); ok = c.done(); }
+    if (ok) { FCursor c(r); c.get(q.ld.data(), 4 * B); ok = c.done(); }
     ok = ok && frec_read(f, r);
-    if (ok) { FCursor c(r); c.get(sc.data(), 88); ok = c.done(); }
+    if (ok) { FCursor c(r); c.get(q.sc.data(), 88); ok = c.done(); }
     ok = ok && frec_read(f, r);
-    if (ok) { FCursor c(r); c.get(&ab, 8); c.get(&imv, 4); c.get(un, 32); ok = c.done(); }
+    if (ok) { FCursor c(r); c.get(&q.ab, 8); c.get(&q.imv, 4); c.get(q.un, 32); ok = c.done(); }
     ok = ok && frec_read(f, r);
-    if (ok) { FCursor c(r); c.get(&ucr, 4); c.get(&dto, 8); c.get(oo.data(), 6 * B); ok = c.done(); }
+    if (ok) { FCursor c(r); c.get(&q.ucr, 4); c.get(&q.dto, 8); c.get(q.oo.data(), 6 * B); ok = c.done(); }
     ok = ok && frec_read(f, r);
-    if (ok) { FCursor c(r); c.get(acc.data(), 2 * B); c.get(inf.data(), 16); ok = c.done(); }
+    if (ok) { FCursor c(r); c.get(q.acc.data(), 2 * B); c.get(q.inf.data(), 16); ok = c.done(); }
     if (!ok) return fail("beam records");
   }
   {
@@ -297,43 +306,47 @@ extern "C" int fsibe_backup_read(fsibe_sim *s, const char *path, int *ivideo, do
   s->k1_ready = false;
   s->k2_ready = false;
   s->press_box_valid = false;
-  if (s->p.iBeam == 1) {
-    double *dst[6] = {s->bx.data(), s->by.data(), s->bvx.data(), s->bvy.data(), s->btheta.data(),
-                      s->btheta_dot.data()};
-    for (int k = 0; k < 6; ++k) memcpy(dst[k], arr.data() + (size_t)k * ns, B);
-    memcpy(s->pbeam.data(), ld.data(), B);
-    memcpy(s->tbeam.data(), ld.data() + 2 * (size_t)ns, B);
+  s->beam_time = t - dt1;
+  if (s->p.iFLUSI == 1 && s->nb > 0) CHK(ensure_solid(s));
+  for (int ib = 0; ib < s->nb; ++ib) {
+    const BeamRec &q = br[ib];
+    const size_t o = (size_t)ib * ns;
+    double *dst[6] = {s->bx.data() + o, s->by.data() + o, s->bvx.data() + o, s->bvy.data() + o,
+                      s->btheta.data() + o, s->btheta_dot.data() + o};
+    for (int k = 0; k < 6; ++k) memcpy(dst[k], q.arr.data() + (size_t)k * ns, B);
+    memcpy(s->pbeam.data() + o, q.ld.data(), B);
+    memcpy(s->tbeam.data() + o, q.ld.data() + 2 * (size_t)ns, B);
+    fsibe_step_out &lb = s->last_b[ib];
     for (int k = 0; k < 2; ++k) {
-      s->last.Force[k] = sc[k];
-      s->last.Force_unst[k] = sc[2 + k];
-      s->last.Force_press[k] = sc[4 + k];
+      lb.Force[k] = q.sc[k];
+      lb.Force_unst[k] = q.sc[2 + k];
+      lb.Force_press[k] = q.sc[4 + k];
     }
-    s->drag_unst_old = un[1];
-    s->lift_unst_old = un[3];
-    s->unst_ready = (ucr & 1) != 0 || ucr != 0;
-    s->beam_time = t - dt1;
+    s->drag_unst_old[ib] = q.un[1];
+    s->lift_unst_old[ib] = q.un[3];
+    s->unst_ready[ib] = q.ucr != 0;
     if (s->p.iFLUSI == 1) {
-      CHK(ensure_solid(s));
-      ibes::Beam &b = s->solid->b;
+      ibes::Beam &b = s->solid[ib]->b;
       std::vector<double> *ba[6] = {&b.x, &b.y, &b.vx, &b.vy, &b.theta, &b.theta_dot};
-      for (int k = 0; k < 6; ++k) memcpy(ba[k]->data(), arr.data() + (size_t)k * ns, B);
-      memcpy(b.pressure_new.data(), ld.data(), B);
-      memcpy(b.pressure_old.data(), ld.data() + (size_t)ns, B);
-      memcpy(b.tau_new.data(), ld.data() + 2 * (size_t)ns, B);
-      memcpy(b.tau_old.data(), ld.data() + 3 * (size_t)ns, B);
-      b.E_kinetic = sc[6]; b.E_pot = sc[7]; b.E_elastic = sc[8];
-      b.dt_old = dto;
-      memcpy(b.oldold.data(), oo.data(), 6 * B);
-      memcpy(b.ax.data(), acc.data(), B);
-      memcpy(b.ay.data(), acc.data() + ns, B);
-      b.Inertial_Force[0] = inf[0];
-      b.Inertial_Force[1] = inf[1];
+      for (int k = 0; k < 6; ++k) memcpy(ba[k]->data(), q.arr.data() + (size_t)k * ns, B);
+      memcpy(b.pressure_new.data(), q.ld.data(), B);
+      memcpy(b.pressure_old.data(), q.ld.data() + (size_t)ns, B);
+      memcpy(b.tau_new.data(), q.ld.data() + 2 * (size_t)ns, B);
+      memcpy(b.tau_old.data(), q.ld.data() + 3 * (size_t)ns, B);
+      b.E_kinetic = q.sc[6]; b.E_pot = q.sc[7]; b.E_elastic = q.sc[8];
+      b.dt_old = q.dto;
+      memcpy(b.oldold.data(), q.oo.data(), 6 * B);
+      memcpy(b.ax.data(), q.acc.data(), B);
+      memcpy(b.ay.data(), q.acc.data() + ns, B);
+      b.Inertial_Force[0] = q.inf[0];
+      b.Inertial_Force[1] = q.inf[1];
       // not in the file: init_beam's .true. holds (init_beam.f90:107), so a
       // BDF2 run restarts with a Crank-Nicolson step, as the reference does
       b.StartupStep = true;
-      s->solid->sv.pre_ok_ = false;
+      s->solid[ib]->sv.pre_ok_ = false;
     }
   }
+  sync_last(s);
   // the record's spectra and masks, for a backup written before the next step
   s->restored.resize(9 * N);
   std::copy(vk.begin(), vk.end(), s->restored.begin());

@@ -668,6 +668,25 @@ extern "C" int fsibe_integrate_position(const fsibe_params *p, int im, double ti
   return FSIBE_OK;
 }
 
+extern "C" int fsibe_beam_placement(const fsibe_params *p, int ib, double *x0, double *y0, double *phase) {
+  if (!p || !x0 || !y0 || !phase) return set_err(FSIBE_EARG, "null argument");
+  if (ib < 0 || ib > 1 || !beam_placement(*p, ib, *x0, *y0, *phase))
+    return set_err(FSIBE_EARG, "error...iMotion bad bad: beam %d needs iMotion 300..307 (init_beam.f90:55-74)",
+                   ib + 1);
+  return FSIBE_OK;
+}
+
+extern "C" int fsibe_integrate_position_beam(const fsibe_params *p, int ib, double time, const double *theta,
+                                             const double *theta_dot, double *x, double *y, double *vx,
+                                             double *vy) {
+  if (!p || !theta || !theta_dot || !x || !y || !vx || !vy) return set_err(FSIBE_EARG, "null argument");
+  if (p->ns < 2) return set_err(FSIBE_EARG, "ns must be >= 2");
+  double x0 = 0.0, y0 = 0.0, ph = 0.0;
+  CHK(fsibe_beam_placement(p, ib, &x0, &y0, &ph));
+  integrate_position_host(*p, p->iMotion, x0, y0, ph, time, theta, theta_dot, x, y, vx, vy);
+  return FSIBE_OK;
+}
+
 // ---------------------------------------------------------------------------
 // host solid solver (solid.hpp)
 // ---------------------------------------------------------------------------
@@ -677,17 +696,26 @@ struct fsibe_solid {
 };
 
 extern "C" int fsibe_solid_create(const fsibe_params *p, fsibe_solid **out) {
+  return fsibe_solid_create_beam(p, 0, out);
+}
+
+extern "C" int fsibe_solid_create_beam(const fsibe_params *p, int ib, fsibe_solid **out) {
   if (!p || !out) return set_err(FSIBE_EARG, "null argument");
   *out = nullptr;
   if (p->iBeam < 1) return set_err(FSIBE_EARG, "the solid solver needs a beam (iBeam >= 1)");
   if (!(p->length > 0.0)) return set_err(FSIBE_EARG, "beam length must be > 0");
+  double x0 = 0.0, y0 = 0.0, ph = 0.0;
+  if (ib < 0 || ib >= p->iBeam || !beam_placement(*p, ib, x0, y0, ph))
+    return set_err(FSIBE_EARG, "beam %d of %d: the second beam needs iMotion 300..307 (init_beam.f90:55-74)",
+                   ib + 1, p->iBeam);
   fsibe_solid *b = new fsibe_solid();
   if (!b->sv.init(*p)) {
     const std::string e = b->sv.err;
     delete b;
     return set_err(FSIBE_EARG, "%s", e.c_str());
   }
-  ibes::init_beam(*p, b->b);
+  b->sv.x0 = x0; b->sv.y0 = y0; b->sv.phase = ph;
+  ibes::init_beam(*p, b->b, x0, y0, ph);
   *out = b;
   return FSIBE_OK;
 }
@@ -853,6 +881,11 @@ struct fsibe_sim {
   Prof prof;
   fsibe_params p;
   int nx, ny, H, ns;
+  // beams (iBeam <= 2, init_beam.f90:48-76): their count, and per beam the
+  // leading edge (x0, y0) and the phase of mouvement; the beam arrays below
+  // hold nb * ns values, beam ib at offset ib * ns
+  int nb = 0;
+  double bx0[2] = {0, 0}, by0[2] = {0, 0}, bphase[2] = {0, 0};
   double pi, dx, dy, ds, scalex, scaley, gerf;
   bool sharp, penal_on, sponge_on, pressure_on, forces_on, visc;
   StepConsts sc;
@@ -901,18 +934,20 @@ struct fsibe_sim {
   // what each slot holds: the fluid step's row sums, the force terms
   bool slot_fluid[2] = {false, false}, slot_forces[2] = {false, false};
   int slot_units[2] = {0, 0};
-  // cal_drag's previous penalisation sums (type Solid drag_unst_old,
-  // lift_unst_old, UnsteadyCorrectionsReady; NavierStokes.f90:579-589)
-  double drag_unst_old = 0.0, lift_unst_old = 0.0;
-  bool unst_ready = false;
+  // cal_drag's previous penalisation sums per beam (type Solid
+  // drag_unst_old, lift_unst_old, UnsteadyCorrectionsReady;
+  // NavierStokes.f90:579-589)
+  double drag_unst_old[2] = {0, 0}, lift_unst_old[2] = {0, 0};
+  bool unst_ready[2] = {false, false};
   int n0 = 0, n1 = 1;
   // geometry of the current step (set by create_mask)
   bool have_beam = false, sponge_done = false;
-  double time_mask = 0.0, alpha = 0.0, LE[6] = {0, 0, 0, 0, 0, 0};
+  // mouvement(time_mask) per beam: alpha, LeadingEdge(1:6)
+  double time_mask = 0.0, alpha[2] = {0, 0}, LE[2][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};
   std::vector<double> bx, by, bvx, bvy, btheta, btheta_dot;
   std::vector<double> seg_ang, seg_c, seg_s;  // mask geometry scratch, one per segment
   Box mbox, fbox;
-  Window win;
+  Window win[2];  // cal_drag's window per beam
   int nprim = 0, ntg = 0;
   size_t off_prim = 0, off_tg = 0, off_nd = 0, off_cd = 0, off_val = 0;
   int nprim_cap = 0;  // room for mask primitives in the staging buffer (and the mask kernel's LDS)
@@ -923,7 +958,8 @@ struct fsibe_sim {
   double time = 0.0, dt0 = 0.0, dt1 = 0.0;
   int it = 0;
   bool startup_done = false;
-  fsibe_step_out last;
+  fsibe_step_out last;       // the last step's results; the per-beam fields of the first beam
+  fsibe_step_out last_b[2];  // the per-beam fields (Force ... fx_int, ibes_iter) of each beam
   std::vector<double> pbeam, tbeam;
   // the spectral and mask arrays of the field record of the last
   // fsibe_backup_read (vortk, nlk, workvis at both levels, mask, maskvx,
@@ -933,7 +969,7 @@ struct fsibe_sim {
   // the scaled nlk(n0) nor workvis(n1) of the AB2 history, so a backup
   // written before the next step writes these verbatim
   std::vector<double> restored;
-  fsibe_solid *solid = nullptr;  // iFLUSI=1: the beam solved by the library (fsibe_run)
+  fsibe_solid *solid[2] = {nullptr, nullptr};  // iFLUSI=1: the beams solved by the library (fsibe_run)
   // time of the kinematics in the beam arrays (integrate_position's time)
   double beam_time = 0.0;
   // sums of -Force(1) dt1 and dt1 over the steps read so far (the objective
@@ -1020,7 +1056,9 @@ static StepOutDev *res_dev(const fsibe_sim *s, int k) {
 static double *pbeam_dev(const fsibe_sim *s, int k) {
   return reinterpret_cast<double *>(s->d_res + (size_t)k * s->res_stride + sizeof(StepOutDev));
 }
-static size_t rows_off(const fsibe_sim *s) { return sizeof(StepOutDev) + sizeof(double) * 2 * std::max(s->ns, 1); }
+// beam nodes of all beams (at least 1: the size of the per-node arrays)
+static int nodes_all(const fsibe_sim *s) { return std::max(s->nb * s->ns, 1); }
+static size_t rows_off(const fsibe_sim *s) { return sizeof(StepOutDev) + sizeof(double) * 2 * nodes_all(s); }
 static size_t parts_off(const fsibe_sim *s) { return rows_off(s) + sizeof(double) * (size_t)s->ny * kNSUM; }
 static double *rows_dev(const fsibe_sim *s, int k) {
   return reinterpret_cast<double *>(s->d_res + (size_t)k * s->res_stride + rows_off(s));
@@ -1193,7 +1231,8 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
     for (int j = 0; j < 2; ++j)
       if (s->prof.ev[k][j]) (void)hipEventDestroy(s->prof.ev[k][j]);
   (void)hipStreamDestroy(s->stream);
-  delete s->solid;
+  delete s->solid[0];
+  delete s->solid[1];
   if (s->mask_win) (void)hipFree(s->mask_win);
   delete s;
   return FSIBE_OK;
@@ -1207,16 +1246,22 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     return set_err(FSIBE_ESIZE, "grid %dx%d not supported (even sizes, nx/2 and ny = 2^a 3^b)", p.nx, p.ny);
   if (!(p.iSponge >= 0 && p.iSponge <= 4) && p.iSponge != 10 && p.iSponge != 22)
     return set_err(FSIBE_EUNSUPPORTED, "iSponge=%d is not a sponge type of create_mask.f90", p.iSponge);
-  if (p.iBeam > 1) return set_err(FSIBE_EUNSUPPORTED, "iBeam=%d: one beam is supported", p.iBeam);
-  if (p.iBeam == 1 && p.ns < 3) return set_err(FSIBE_EARG, "ns must be >= 3");
-  if (p.iBeam == 1 && 2 * p.ns + 8 > kMaxTilePrims)
-    return set_err(FSIBE_EUNSUPPORTED, "ns=%d: at most %d beam nodes", p.ns, (kMaxTilePrims - 8) / 2);
+  if (p.iBeam < 0 || p.iBeam > 2) return set_err(FSIBE_EUNSUPPORTED, "iBeam=%d: at most two beams", p.iBeam);
+  // the second beam's phase exists for the dragonfly motions only; the
+  // reference stops otherwise (init_beam.f90:72-73)
+  if (p.iBeam == 2 && !(p.iMotion >= 300 && p.iMotion <= 307))
+    return set_err(FSIBE_EARG, "error...iMotion bad bad: two beams need iMotion 300..307 (init_beam.f90:55-74)");
+  if (p.iBeam > 0 && p.ns < 3) return set_err(FSIBE_EARG, "ns must be >= 3");
+  if (p.iBeam > 0 && 2 * p.iBeam * p.ns + 8 > kMaxTilePrims)
+    return set_err(FSIBE_EUNSUPPORTED, "ns=%d: at most %d beam nodes per beam", p.ns,
+                   (kMaxTilePrims - 8) / (2 * p.iBeam));
   if (!(p.eps > 0.0) || !(p.xl > 0.0) || !(p.yl > 0.0)) return set_err(FSIBE_EARG, "eps, xl, yl must be > 0");
   if (((p.iSponge > 0 && p.iSponge < 4) || p.iSponge == 10 || p.iSponge == 22) && !(p.eps_sponge > 0.0))
     return set_err(FSIBE_EARG, "vorticity sponge needs eps_sponge > 0 (params.f90:185-188)");
   fsibe_sim *s = new fsibe_sim();
   s->p = p;
   s->nx = p.nx; s->ny = p.ny; s->H = p.nx / 2; s->ns = p.iBeam ? p.ns : 0;
+  s->nb = p.iBeam;
   s->pi = 4.0 * atan(1.0);
   s->dx = p.xl / (double)p.nx;
   s->dy = p.yl / (double)p.ny;
@@ -1292,10 +1337,11 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipMemset(s->umax_part, 0, sizeof(double) * s->ny));
   SIMCHK(hipMalloc(&s->dscal, sizeof(DevScalars)));
   SIMCHK(hipMemset(s->dscal, 0, sizeof(DevScalars)));
-  const int nsb0 = std::max(s->ns, 1);
-  // slot: StepOutDev | p_beam[nsb] | tau_beam[nsb] | row sums[ny][kNSUM] | force terms[8][nsb + kNC]
+  const int nsb0 = std::max(s->nb * s->ns, 1), nb1 = std::max(s->nb, 1);
+  // slot: StepOutDev | p_beam[nsb] | tau_beam[nsb] | row sums[ny][kNSUM] |
+  // force terms[8][nsb + nb kNC] (nsb = nb ns: every beam's nodes)
   const size_t res_bytes = sizeof(StepOutDev) + sizeof(double) * (2 * nsb0 + (size_t)s->ny * kNSUM +
-                                                                   8 * (size_t)(nsb0 + kNC));
+                                                                   8 * (size_t)(nsb0 + nb1 * kNC));
   s->res_stride = (res_bytes + 255) & ~(size_t)255;
   SIMCHK(hipHostMalloc((void **)&s->h_res, 2 * s->res_stride, hipHostMallocMapped | hipHostMallocCoherent));
   memset(s->h_res, 0, 2 * s->res_stride);
@@ -1306,18 +1352,18 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   SIMCHK(hipMalloc(&s->fb_p, sizeof(double) * N));
   SIMCHK(hipMalloc(&s->fb_sa, sizeof(double) * N));
   SIMCHK(hipMalloc(&s->fb_sb, sizeof(double) * N));
-  const int nsb = std::max(s->ns, 1);
+  const int nsb = std::max(s->nb * s->ns, 1);
   // staging layout
   {
     size_t off = 0;
     auto al = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
     const int nprim_max = 2 * nsb + 8;
-    const int ntg_max = 6 * nsb + 6 * kNC;
+    const int ntg_max = 6 * nsb + 6 * kNC * nb1;
     s->nprim_cap = nprim_max;
     s->off_prim = al(sizeof(Prim) * nprim_max);
     s->off_tg = al(sizeof(double) * 3 * ntg_max);
     s->off_nd = al(sizeof(double) * 4 * nsb);
-    s->off_cd = al(sizeof(double) * 4 * 2 * kNC);
+    s->off_cd = al(sizeof(double) * 4 * 2 * kNC * nb1);
     s->off_val = al(sizeof(double) * ntg_max);
     s->stage_bytes = off;
   }
@@ -1353,7 +1399,15 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   s->seg_ang.assign(nsb, 0.0); s->seg_c.assign(nsb, 0.0); s->seg_s.assign(nsb, 0.0);
   s->pbeam.assign(nsb, 0.0); s->tbeam.assign(nsb, 0.0);
   memset(&s->last, 0, sizeof(s->last));
-  s->win = Window{0, -1, 0, -1};
+  memset(s->last_b, 0, sizeof(s->last_b));
+  s->win[0] = s->win[1] = Window{0, -1, 0, -1};
+  // init_beam.f90:48-76: the first beam at (x0, y0), phase 0; the second at
+  // (x0 + 1.25, y0) with the dragonfly phase (iMotion - 300) pi / 4
+  for (int ib = 0; ib < s->nb; ++ib) {
+    s->bx0[ib] = p.x0 + (ib == 1 ? 1.25 : 0.0);
+    s->by0[ib] = p.y0;
+    s->bphase[ib] = ib == 1 ? 0.25 * (double)(p.iMotion - 300) * s->pi : 0.0;
+  }
 #undef SIMCHK
   *out = s;
   return FSIBE_OK;
@@ -1418,12 +1472,16 @@ static int build_mask_geometry(fsibe_sim *s) {
     mb.add(P.ix0, P.ix1, P.iy0, P.iy1);
   };
   if (s->p.iCylinder == 1) dot(s->p.x0 - s->p.R_cylinder, s->p.y0, 0.0, 0.0, s->p.R_cylinder);
-  if (s->p.iBeam > 0) {
-    const double *X = s->bx.data(), *Y = s->by.data(), *VX = s->bvx.data(), *VY = s->bvy.data();
+  // beam after beam (create_mask.f90:74-119): each primitive overwrites
+  // where its value is strictly larger, so the list order is the drawing order
+  for (int ib = 0; ib < s->nb; ++ib) {
+    const int o = ib * ns;
+    const double *X = s->bx.data() + o, *Y = s->by.data() + o, *VX = s->bvx.data() + o, *VY = s->bvy.data() + o;
+    const double *TH = s->btheta.data() + o;
     const double t_star = t_cut;
     // the segments' directions atan2(dy, dx), shared by the hinges (DrawHinge
     // takes the same atan2 of the same differences, create_mask.f90:104-105)
-    double *seg_ang = s->seg_ang.data(), *seg_c = s->seg_c.data(), *seg_s = s->seg_s.data();
+    double *seg_ang = s->seg_ang.data() + o, *seg_c = s->seg_c.data() + o, *seg_s = s->seg_s.data() + o;
     for (int i = 0; i <= ns - 2; ++i) {  // DrawBeamSegment :740-761
       Prim P{};
       P.type = PRIM_SEG;
@@ -1447,7 +1505,7 @@ static int build_mask_geometry(fsibe_sim *s) {
     if (s->p.iSharpTrailing == 1) {  // DrawSharpEnd :796-822
       Prim P{};
       P.type = PRIM_SHARP;
-      const double al = s->btheta[ns - 1] + s->alpha;
+      const double al = TH[ns - 1] + s->alpha[ib];
       const double pex = X[ns - 1], pey = Y[ns - 1];
       const double p1x = pex - 2.0 * mdx * cos(al), p1y = pey - 2.0 * mdx * sin(al);
       const double p2x = pex + 4.0 * mdx * cos(al), p2y = pey + 4.0 * mdx * sin(al);
@@ -1488,10 +1546,12 @@ static int build_mask_geometry(fsibe_sim *s) {
   s->nprim = np;
   s->mbox = mb;
 
-  // cal_drag window (NavierStokes.f90:538-564)
-  {
+  // cal_drag window of each beam (NavierStokes.f90:528-564); without a beam
+  // the whole grid
+  for (int ib = 0; ib < std::max(s->nb, 1); ++ib) {
     int ixmin, ixmax, iymin, iymax;
-    const double *X = s->bx.data(), *Y = s->by.data();
+    const double *X = s->bx.data() + ib * ns, *Y = s->by.data() + ib * ns;
+    const double *LE = s->LE[ib];
     if (s->p.iBeam > 0) {
       double ymax = Y[0], ymin = Y[0], xmin = X[0], xmax = X[0];
       for (int i = 1; i < ns; ++i) {
@@ -1499,9 +1559,9 @@ static int build_mask_geometry(fsibe_sim *s) {
         xmin = std::min(xmin, X[i]); xmax = std::max(xmax, X[i]);
       }
       const double tb = s->p.t_beam, rc = s->p.R_cylinder;
-      iymax = (int)((std::max(ymax + 1.5 * tb, s->LE[1] + 2.0 * rc)) / dy);
-      iymin = (int)((std::min(ymin - 1.5 * tb, s->LE[1] - 2.0 * rc)) / dy);
-      ixmin = (int)(std::min((s->LE[0] - std::max(2.5 * rc, 1.5 * tb)), (xmin - 1.5 * tb)) / dx);
+      iymax = (int)((std::max(ymax + 1.5 * tb, LE[1] + 2.0 * rc)) / dy);
+      iymin = (int)((std::min(ymin - 1.5 * tb, LE[1] - 2.0 * rc)) / dy);
+      ixmin = (int)(std::min((LE[0] - std::max(2.5 * rc, 1.5 * tb)), (xmin - 1.5 * tb)) / dx);
       ixmax = (int)((xmax + 1.5 * tb) / dx);
     } else {
       iymax = ny - 1; iymin = 0; ixmax = nx - 1; ixmin = 0;
@@ -1528,8 +1588,9 @@ static int build_mask_geometry(fsibe_sim *s) {
     }
     ixmin = std::max(ixmin, 0); ixmax = std::min(ixmax, nx - 1);
     iymin = std::max(iymin, 0); iymax = std::min(iymax, ny - 1);
-    s->win = Window{ixmin, ixmax, iymin, iymax};
+    s->win[ib] = Window{ixmin, ixmax, iymin, iymax};
   }
+  if (s->nb < 2) s->win[1] = Window{0, -1, 0, -1};
   return FSIBE_OK;
 }
 
@@ -1557,43 +1618,47 @@ static int build_force_geometry(fsibe_sim *s, double time) {
       tg[3 * nt] = xt; tg[3 * nt + 1] = yt; tg[3 * nt + 2] = (double)fid;
       ++nt;
     };
-    const double *X = s->bx.data(), *Y = s->by.data();
     const double rr = s->p.t_beam + s->p.N_smooth * std::max(dy, dx);
     const double r1 = s->p.t_beam - 0.0 * s->p.N_smooth * std::max(dy, dx);
-    for (int n = 0; n < ns; ++n) {
-      const double th = s->btheta[n] + s->alpha;
-      const double st = sin(th), ct = cos(th);
-      nd[4 * n] = st; nd[4 * n + 1] = ct; nd[4 * n + 2] = sin(2.0 * th); nd[4 * n + 3] = cos(2.0 * th);
-      const double xu = X[n] - rr * st, yu = Y[n] + rr * ct, xb = X[n] + rr * st, yb = Y[n] - rr * ct;
-      const double xu1 = X[n] - r1 * st, yu1 = Y[n] + r1 * ct, xb1 = X[n] + r1 * st, yb1 = Y[n] - r1 * ct;
-      target(xu1, yu1, 0);
-      target(xb1, yb1, 0);
-      target(xu, yu, 1);
-      target(xu, yu, 2);
-      target(xb, yb, 1);
-      target(xb, yb, 2);
-    }
     const bool bsi = s->visc && s->p.iCylinder == 0;
-    if (bsi) {
-      const double dphi = s->pi / (double)(kNC - 1);
-      double beta = s->btheta[ns - 1] + s->alpha;
-      for (int i = 0; i < kNC; ++i) {
-        const double phi = -(0.5 * s->pi - beta) + (double)i * dphi;
-        const double xc = X[ns - 1] + rr * cos(phi), yc = Y[ns - 1] + rr * sin(phi);
-        cd[4 * i] = cos(phi); cd[4 * i + 1] = sin(phi); cd[4 * i + 2] = cos(2.0 * phi); cd[4 * i + 3] = sin(2.0 * phi);
-        target(xc, yc, 1);
-        target(xc, yc, 2);
-        target(xc, yc, 0);
+    // GetForces loops over the beams (BeamForces.f90:24-47): beam after beam
+    for (int ib = 0; ib < s->nb; ++ib, nd += 4 * ns, cd += 8 * kNC) {
+      const double *X = s->bx.data() + ib * ns, *Y = s->by.data() + ib * ns, *TH = s->btheta.data() + ib * ns;
+      const double alpha = s->alpha[ib];
+      for (int n = 0; n < ns; ++n) {
+        const double th = TH[n] + alpha;
+        const double st = sin(th), ct = cos(th);
+        nd[4 * n] = st; nd[4 * n + 1] = ct; nd[4 * n + 2] = sin(2.0 * th); nd[4 * n + 3] = cos(2.0 * th);
+        const double xu = X[n] - rr * st, yu = Y[n] + rr * ct, xb = X[n] + rr * st, yb = Y[n] - rr * ct;
+        const double xu1 = X[n] - r1 * st, yu1 = Y[n] + r1 * ct, xb1 = X[n] + r1 * st, yb1 = Y[n] - r1 * ct;
+        target(xu1, yu1, 0);
+        target(xb1, yb1, 0);
+        target(xu, yu, 1);
+        target(xu, yu, 2);
+        target(xb, yb, 1);
+        target(xb, yb, 2);
       }
-      beta = s->btheta[1] + s->alpha;  // BeamForces.f90:390 takes node 1
-      for (int i = 0; i < kNC; ++i) {
-        const double phi = (0.5 * s->pi + beta) + (double)i * dphi;
-        const double xc = X[0] + rr * cos(phi), yc = Y[0] + rr * sin(phi);
-        const int q = kNC + i;
-        cd[4 * q] = cos(phi); cd[4 * q + 1] = sin(phi); cd[4 * q + 2] = cos(2.0 * phi); cd[4 * q + 3] = sin(2.0 * phi);
-        target(xc, yc, 1);
-        target(xc, yc, 2);
-        target(xc, yc, 0);
+      if (bsi) {
+        const double dphi = s->pi / (double)(kNC - 1);
+        double beta = TH[ns - 1] + alpha;
+        for (int i = 0; i < kNC; ++i) {
+          const double phi = -(0.5 * s->pi - beta) + (double)i * dphi;
+          const double xc = X[ns - 1] + rr * cos(phi), yc = Y[ns - 1] + rr * sin(phi);
+          cd[4 * i] = cos(phi); cd[4 * i + 1] = sin(phi); cd[4 * i + 2] = cos(2.0 * phi); cd[4 * i + 3] = sin(2.0 * phi);
+          target(xc, yc, 1);
+          target(xc, yc, 2);
+          target(xc, yc, 0);
+        }
+        beta = TH[1] + alpha;  // BeamForces.f90:390 takes node 1
+        for (int i = 0; i < kNC; ++i) {
+          const double phi = (0.5 * s->pi + beta) + (double)i * dphi;
+          const double xc = X[0] + rr * cos(phi), yc = Y[0] + rr * sin(phi);
+          const int q = kNC + i;
+          cd[4 * q] = cos(phi); cd[4 * q + 1] = sin(phi); cd[4 * q + 2] = cos(2.0 * phi); cd[4 * q + 3] = sin(2.0 * phi);
+          target(xc, yc, 1);
+          target(xc, yc, 2);
+          target(xc, yc, 0);
+        }
       }
     }
     if (bad) return set_err(FSIBE_EDOMAIN, "DeltaInterpolation: target coordinates not in the field "
@@ -1661,12 +1726,23 @@ static int ensure_sponge(fsibe_sim *s) {
 // with_forces = false: only the mask's part of the geometry is built and
 // uploaded; the caller builds and uploads the force part (force_geometry)
 // before the pressure and force kernels
+// beam arrays of nb ns values (beam ib's nodes from ib ns), alpha[nb],
+// LE[6 nb] (NULL: zeros)
 static int create_mask_impl(fsibe_sim *s, double time, const double *x, const double *y, const double *vx,
-                            const double *vy, const double *theta, double alpha, const double LE[6],
+                            const double *vy, const double *theta, const double *alpha, const double *LE,
                             bool with_forces);
 
 extern "C" int fsibe_create_mask(fsibe_sim *s, double time, const double *x, const double *y, const double *vx,
                                  const double *vy, const double *theta, double alpha, const double LE[6]) {
+  if (s && s->nb > 1)
+    return set_err(FSIBE_EARG, "%d beams: fsibe_create_mask_beams takes every beam's kinematics", s->nb);
+  return create_mask_impl(s, time, x, y, vx, vy, theta, &alpha, LE, true);
+}
+
+extern "C" int fsibe_create_mask_beams(fsibe_sim *s, double time, const double *x, const double *y,
+                                       const double *vx, const double *vy, const double *theta,
+                                       const double *alpha, const double *LE) {
+  if (s && s->nb > 0 && !alpha) return set_err(FSIBE_EARG, "null alpha");
   return create_mask_impl(s, time, x, y, vx, vy, theta, alpha, LE, true);
 }
 
@@ -1679,7 +1755,7 @@ static int force_geometry(fsibe_sim *s, double time) {
 }
 
 static int create_mask_impl(fsibe_sim *s, double time, const double *x, const double *y, const double *vx,
-                            const double *vy, const double *theta, double alpha, const double LE[6],
+                            const double *vy, const double *theta, const double *alpha, const double *LE,
                             bool with_forces) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
   if (s->ns > 0 && (!x || !y || !vx || !vy || !theta)) return set_err(FSIBE_EARG, "null beam array");
@@ -1688,11 +1764,14 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
     HIPCHK(hipEventSynchronize(s->ev_upload));
     s->upload_pending = false;
   }
-  for (int i = 0; i < s->ns; ++i) {
+  for (int i = 0; i < s->nb * s->ns; ++i) {
     s->bx[i] = x[i]; s->by[i] = y[i]; s->bvx[i] = vx[i]; s->bvy[i] = vy[i]; s->btheta[i] = theta[i];
   }
-  s->alpha = alpha;
-  for (int i = 0; i < 6; ++i) s->LE[i] = LE ? LE[i] : 0.0;
+  for (int ib = 0; ib < 2; ++ib) {
+    const bool have = ib < s->nb || (ib == 0 && alpha);
+    s->alpha[ib] = have && alpha ? alpha[ib] : 0.0;
+    for (int i = 0; i < 6; ++i) s->LE[ib][i] = have && LE ? LE[6 * ib + i] : 0.0;
+  }
   s->time_mask = time;
   s->have_beam = true;
   const double th0 = now_ms();
@@ -1785,7 +1864,8 @@ static int velocity_only(fsibe_sim *s, double time) {
   MaskBox mb = mask_box(s);
   return with_rows<3>(s->px, [&](auto F, int nt) -> int {
     KLAUNCH(k_rows_press<decltype(F)>, s->ny, nt, lds, s->stream, nullptr, s->T[1], s->T[2], s->u1, s->u2,
-            s->vor_mean, mb, s->win, rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal, UMx, UMy, 0, 0,
+            s->vor_mean, mb, s->win[0], s->win[1], rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal, UMx,
+            UMy, 0, 0,
             F, s->twx,
             s->sc);
     return FSIBE_OK;
@@ -1931,7 +2011,8 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     prof_begin(s, P_ROWS_PRESS);
     CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
       KLAUNCH(k_rows_press<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->T[2], s->u1, s->u2,
-              s->vor_mean, mb, s->win, rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal, 0.0, 0.0, 1,
+              s->vor_mean, mb, s->win[0], s->win[1], rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal,
+              0.0, 0.0, 1,
               (int)s->pressure_on, F, s->twx, s->sc);
       return FSIBE_OK;
     }));
@@ -1982,20 +2063,32 @@ static int launch_forces(fsibe_sim *s) {
   const double *tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
   double *val = reinterpret_cast<double *>(s->d_stage + s->off_val);
   if (s->ntg <= 0) return set_err(FSIBE_ESTATE, "no interpolation targets");
-  // one block per beam node and per pair of circle points
-  const int units = s->fg.ns + s->fg.nc;
-  if (s->ntg != 6 * s->fg.ns + 3 * 2 * s->fg.nc || s->fg.nc > kNC)
+  // one block per beam node and per pair of circle points, beam after beam
+  const int units = s->nb * (s->fg.ns + s->fg.nc);
+  if (s->ntg != s->nb * (6 * s->fg.ns + 3 * 2 * s->fg.nc) || s->fg.nc > kNC)
     return set_err(FSIBE_ESTATE, "interpolation targets do not match the beam");
   prof_begin(s, P_INTERP);
   k_forces<<<units, kForceThreads, 0, s->stream>>>(
       tg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
       reinterpret_cast<const double *>(s->d_stage + s->off_cd), pbeam_dev(s, s->slot),
-      pbeam_dev(s, s->slot) + std::max(s->ns, 1), parts_dev(s, s->slot));
+      pbeam_dev(s, s->slot) + nodes_all(s), parts_dev(s, s->slot));
   HIPCHK(hipGetLastError());
   prof_end(s, P_INTERP);
   s->slot_forces[s->slot] = true;
   s->slot_units[s->slot] = units;
   return FSIBE_OK;
+}
+
+// the first beam's per-beam fields of last_b into last
+static void sync_last(fsibe_sim *s) {
+  const fsibe_step_out &b = s->last_b[0];
+  fsibe_step_out &r = s->last;
+  for (int i = 0; i < 2; ++i) {
+    r.Force[i] = b.Force[i]; r.Force_unst[i] = b.Force_unst[i]; r.Force_press[i] = b.Force_press[i];
+  }
+  for (int i = 2; i < 10; ++i) r.bsi[i] = b.bsi[i];
+  r.fx_int = b.fx_int; r.fy_int = b.fy_int;
+  r.ibes_iter = b.ibes_iter;
 }
 
 // Read result slot k (its kernels have finished) into s->last: dt1, u_max;
@@ -2005,7 +2098,7 @@ static int launch_forces(fsibe_sim *s) {
 // k_forces (Force_press :261-267, surface integrals :485-496) added over
 // units in order.  Each part is read once per step.
 static void parse_results(fsibe_sim *s, int k) {
-  const int nsb = std::max(s->ns, 1);
+  const int nsb = nodes_all(s), nb = s->nb, nr = std::max(nb, 1);
   const char *base = s->h_res + (size_t)k * s->res_stride;
   const StepOutDev *o = reinterpret_cast<const StepOutDev *>(base);
   fsibe_step_out &r = s->last;
@@ -2027,45 +2120,65 @@ static void parse_results(fsibe_sim *s, int k) {
     r.energies[3] = dxdy * tot[2];
     r.energies[4] = dxdy * tot[3];
     r.energies[5] = dxdy * s->p.U_mean_true * tot[4];
+    // BeamSurfaceIntegral's drag and lift are whole-grid sums, the same for
+    // every beam (BeamForces.f90:483-486)
     r.bsi[0] = tot[5] * dxdy;
     r.bsi[1] = tot[6] * dxdy;
-    if (s->p.iBeam > 0) {
-      r.Force[0] = tot[7] * dxdy;
-      r.Force[1] = tot[8] * dxdy;
-      s->jsum_f += -r.Force[0] * o->dt1;
-      s->jsum_t += o->dt1;
-      const double dn = tot[9] * dxdy * s->p.eps, ln = tot[10] * dxdy * s->p.eps;
-      if (s->unst_ready) {
-        r.Force_unst[0] = (dn - s->drag_unst_old) / o->dt1;
-        r.Force_unst[1] = (ln - s->lift_unst_old) / o->dt1;
+    // cal_drag per beam over its own window (NavierStokes.f90:528-589)
+    for (int ib = 0; ib < nb; ++ib) {
+      fsibe_step_out &rb = s->last_b[ib];
+      const double *t = tot + 7 + 4 * ib;
+      rb.Force[0] = t[0] * dxdy;
+      rb.Force[1] = t[1] * dxdy;
+      const double dn = t[2] * dxdy * s->p.eps, ln = t[3] * dxdy * s->p.eps;
+      if (s->unst_ready[ib]) {
+        rb.Force_unst[0] = (dn - s->drag_unst_old[ib]) / o->dt1;
+        rb.Force_unst[1] = (ln - s->lift_unst_old[ib]) / o->dt1;
       } else {
-        r.Force_unst[0] = 0.0;
-        r.Force_unst[1] = 0.0;
+        rb.Force_unst[0] = 0.0;
+        rb.Force_unst[1] = 0.0;
       }
-      s->drag_unst_old = dn;
-      s->lift_unst_old = ln;
-      s->unst_ready = true;
+      s->drag_unst_old[ib] = dn;
+      s->lift_unst_old[ib] = ln;
+      s->unst_ready[ib] = true;
+    }
+    if (nb > 0) {
+      s->jsum_f += -s->last_b[0].Force[0] * o->dt1;
+      s->jsum_t += o->dt1;
     }
   }
   if (s->slot_forces[k]) {
     s->slot_forces[k] = false;
     const double *part = reinterpret_cast<const double *>(base + parts_off(s));
-    double tot[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int u = 0; u < s->slot_units[k]; ++u)
-      for (int i = 0; i < 8; ++i) tot[i] += part[8 * u + i];
-    r.Force_press[0] = tot[0];
-    r.Force_press[1] = tot[1];
-    r.fx_int = tot[0];
-    r.fy_int = tot[1];
-    if (s->fg.bsi_on) {
-      r.bsi[2] = tot[2] + tot[4] + tot[6];
-      r.bsi[3] = tot[3] + tot[5] + tot[7];
-      r.bsi[4] = tot[2]; r.bsi[5] = tot[4]; r.bsi[6] = tot[6];
-      r.bsi[7] = tot[3]; r.bsi[8] = tot[5]; r.bsi[9] = tot[7];
+    const int per = s->slot_units[k] / nr;
+    for (int ib = 0; ib < nr; ++ib) {
+      fsibe_step_out &rb = s->last_b[ib];
+      double tot[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int u = ib * per; u < (ib + 1) * per; ++u)
+        for (int i = 0; i < 8; ++i) tot[i] += part[8 * u + i];
+      rb.Force_press[0] = tot[0];
+      rb.Force_press[1] = tot[1];
+      rb.fx_int = tot[0];
+      rb.fy_int = tot[1];
+      if (s->fg.bsi_on) {
+        rb.bsi[2] = tot[2] + tot[4] + tot[6];
+        rb.bsi[3] = tot[3] + tot[5] + tot[7];
+        rb.bsi[4] = tot[2]; rb.bsi[5] = tot[4]; rb.bsi[6] = tot[6];
+        rb.bsi[7] = tot[3]; rb.bsi[8] = tot[5]; rb.bsi[9] = tot[7];
+      }
     }
     const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
-    for (int i = 0; i < s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
+    for (int i = 0; i < s->nb * s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
   }
+  // each beam's record carries the step's common fields; s->last the first
+  // beam's per-beam fields
+  for (int ib = 0; ib < nr; ++ib) {
+    fsibe_step_out &rb = s->last_b[ib];
+    rb.dt1 = r.dt1; rb.u_max = r.u_max; rb.status = 0; rb.ibes_iter = 0;
+    for (int i = 0; i < 6; ++i) rb.energies[i] = r.energies[i];
+    rb.bsi[0] = r.bsi[0]; rb.bsi[1] = r.bsi[1];
+  }
+  sync_last(s);
 }
 
 static int fetch_results(fsibe_sim *s) {
@@ -2101,11 +2214,15 @@ extern "C" int fsibe_get_forces(fsibe_sim *s, double time, double *pressure_beam
   if (!s->press_box_valid) return set_err(FSIBE_ESTATE, "no pressure of the current step (evolve first)");
   CHK(launch_forces(s));
   CHK(fetch_results(s));
-  for (int i = 0; i < s->ns; ++i) {
+  for (int i = 0; i < s->nb * s->ns; ++i) {
     if (pressure_beam) pressure_beam[i] = s->pbeam[i];
     if (tau_beam) tau_beam[i] = s->tbeam[i];
   }
-  if (Force_press) { Force_press[0] = s->last.Force_press[0]; Force_press[1] = s->last.Force_press[1]; }
+  if (Force_press)
+    for (int ib = 0; ib < std::max(s->nb, 1); ++ib) {
+      Force_press[2 * ib] = s->last_b[ib].Force_press[0];
+      Force_press[2 * ib + 1] = s->last_b[ib].Force_press[1];
+    }
   return FSIBE_OK;
 }
 
@@ -2115,36 +2232,88 @@ extern "C" int fsibe_get_step_out(fsibe_sim *s, fsibe_step_out *out) {
   return FSIBE_OK;
 }
 
+extern "C" int fsibe_get_step_out_beam(fsibe_sim *s, int ib, fsibe_step_out *out) {
+  if (!s || !out) return set_err(FSIBE_EARG, "null argument");
+  if (ib < 0 || ib >= std::max(s->nb, 1)) return set_err(FSIBE_EARG, "beam %d of %d", ib, s->nb);
+  *out = s->last_b[ib];
+  out->status = s->last.status;
+  return FSIBE_OK;
+}
+
 // ---- time loop (time_step.f90:120-162, :178-341) ---------------------------
 // the beam of the library's solid solver, for iFLUSI=1
 static int ensure_solid(fsibe_sim *s) {
-  if (s->solid) return FSIBE_OK;
   if (s->p.iBeam < 1) return set_err(FSIBE_EARG, "iFLUSI=1 without a beam");
-  return fsibe_solid_create(&s->p, &s->solid);
+  for (int ib = 0; ib < s->nb; ++ib)
+    if (!s->solid[ib]) CHK(fsibe_solid_create_beam(&s->p, ib, &s->solid[ib]));
+  return FSIBE_OK;
 }
 
 static void beam_from_solid(fsibe_sim *s) {
-  const ibes::Beam &b = s->solid->b;
-  s->bx = b.x; s->by = b.y; s->bvx = b.vx; s->bvy = b.vy; s->btheta = b.theta; s->btheta_dot = b.theta_dot;
+  const int ns = s->ns;
+  for (int ib = 0; ib < s->nb; ++ib) {
+    const ibes::Beam &b = s->solid[ib]->b;
+    const size_t o = (size_t)ib * ns;
+    std::copy(b.x.begin(), b.x.end(), s->bx.begin() + o);
+    std::copy(b.y.begin(), b.y.end(), s->by.begin() + o);
+    std::copy(b.vx.begin(), b.vx.end(), s->bvx.begin() + o);
+    std::copy(b.vy.begin(), b.vy.end(), s->bvy.begin() + o);
+    std::copy(b.theta.begin(), b.theta.end(), s->btheta.begin() + o);
+    std::copy(b.theta_dot.begin(), b.theta_dot.end(), s->btheta_dot.begin() + o);
+  }
 }
 
-// GetForces' loads -> GravityImpulse + SolidSolverWrapper(time, dt1) ->
-// pressure_old = pressure_new (time_step.f90:199-228)
+// GetForces' loads -> GravityImpulse + SolidSolverWrapper(time, dt1), one
+// IBES solve per beam (SolidSolver.f90:25-27) -> pressure_old = pressure_new
+// (time_step.f90:199-228).  The emergency brake of any beam ends the run.
 static int solid_step(fsibe_sim *s) {
-  ibes::Beam &b = s->solid->b;
-  for (int i = 0; i < s->ns; ++i) { b.pressure_new[i] = s->pbeam[i]; b.tau_new[i] = s->tbeam[i]; }
-  if (!s->solid->sv.step(s->time, s->dt1, b)) return set_err(FSIBE_EDOMAIN, "%s", s->solid->sv.err.c_str());
+  int brake = 0;
+  for (int ib = 0; ib < s->nb; ++ib) {
+    fsibe_solid *sd = s->solid[ib];
+    ibes::Beam &b = sd->b;
+    const int o = ib * s->ns;
+    for (int i = 0; i < s->ns; ++i) { b.pressure_new[i] = s->pbeam[o + i]; b.tau_new[i] = s->tbeam[o + i]; }
+    if (!sd->sv.step(s->time, s->dt1, b))
+      return set_err(FSIBE_EDOMAIN, "beam %d: %s", ib + 1, sd->sv.err.c_str());
+    s->last_b[ib].ibes_iter = sd->sv.iter;
+    brake |= sd->sv.brake ? 1 : 0;
+  }
   beam_from_solid(s);
-  s->last.ibes_iter = s->solid->sv.iter;
-  s->last.status = s->solid->sv.brake ? 1 : 0;
+  s->last.ibes_iter = s->last_b[0].ibes_iter;
+  s->last.status = brake;
   return FSIBE_OK;
 }
 
 extern "C" int fsibe_sim_solid(fsibe_sim *s, fsibe_solid **out) {
+  return fsibe_sim_solid_beam(s, 0, out);
+}
+
+extern "C" int fsibe_sim_solid_beam(fsibe_sim *s, int ib, fsibe_solid **out) {
   if (!s || !out) return set_err(FSIBE_EARG, "null argument");
+  if (ib < 0 || ib >= std::max(s->nb, 1)) return set_err(FSIBE_EARG, "beam %d of %d", ib, s->nb);
   CHK(ensure_solid(s));
-  *out = s->solid;
+  *out = s->solid[ib];
   return FSIBE_OK;
+}
+
+// mouvement(time) of every beam (create_mask.f90:77): alpha[ib], LE[6 ib ..]
+static void motion_all(const fsibe_sim *s, double time, double *alpha, double *LE) {
+  for (int ib = 0; ib < s->nb; ++ib) {
+    double x0 = 0.0, y0 = 0.0, ph = 0.0, at, att;
+    beam_placement(s->p, ib, x0, y0, ph);
+    mouvement_host(s->p, s->p.iMotion, x0, y0, s->p.AngleBeam, ph, time, alpha[ib], at, att, LE + 6 * ib);
+  }
+}
+
+// integrate_position(time, beams(i)) of every beam (time_step.f90:215-217)
+static void integrate_all(fsibe_sim *s, double time) {
+  for (int ib = 0; ib < s->nb; ++ib) {
+    double x0 = 0.0, y0 = 0.0, ph = 0.0;
+    beam_placement(s->p, ib, x0, y0, ph);
+    const size_t o = (size_t)ib * s->ns;
+    integrate_position_host(s->p, s->p.iMotion, x0, y0, ph, time, s->btheta.data() + o, s->btheta_dot.data() + o,
+                            s->bx.data() + o, s->by.data() + o, s->bvx.data() + o, s->bvy.data() + o);
+  }
 }
 
 // One step of the loop.  pipe: do not wait for the step to finish; wait only
@@ -2159,8 +2328,8 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
   // (interpolation targets) while the GPU already runs the fluid kernels
   const bool split = !pipe && p.iFLUSI == 1 && p.iBeam > 0 && s->forces_on;
   if (p.iBeam > 0 && (moving || !s->have_beam)) {
-    double alpha, at, att, LE[6];
-    mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, s->time, alpha, at, att, LE);
+    double alpha[2], LE[12];
+    motion_all(s, s->time, alpha, LE);
     CHK(create_mask_impl(s, s->time, s->bx.data(), s->by.data(), s->bvx.data(), s->bvy.data(),
                          s->btheta.data(), alpha, LE, !split));
   } else if (p.iBeam > 0) {
@@ -2186,8 +2355,9 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     CHK(launch_k2_ahead(s));
     // the beam solver's predictor needs only the state at t_n: it runs while
     // the step's fluid kernels finish
-    if (p.iFLUSI == 1 && p.iBeam > 0 && s->solid && s->time > p.T_release)
-      (void)s->solid->sv.predict(s->time, s->solid->b);
+    if (p.iFLUSI == 1 && p.iBeam > 0 && s->time > p.T_release)
+      for (int ib = 0; ib < s->nb; ++ib)
+        if (s->solid[ib]) (void)s->solid[ib]->sv.predict(s->time, s->solid[ib]->b);
     CHK(wait_event(s->ev_step, "the step's results (pressure and force kernels)"));
     s->upload_pending = false;
     parse_results(s, s->slot);
@@ -2210,8 +2380,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     CHK(solid_step(s));
     prof_host(s, P_SCALARS, th0);
   } else if (p.iFLUSI == 0 && p.iMotion > 0 && p.iBeam > 0) {
-    integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
-                            s->by.data(), s->bvx.data(), s->bvy.data());
+    integrate_all(s, s->time);
     s->beam_time = s->time;
   }
   std::swap(s->n0, s->n1);
@@ -2222,17 +2391,17 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
 }
 
 static void init_beam_host(fsibe_sim *s) {
-  // init_beam.f90:130-167 for one beam
-  const fsibe_params &p = s->p;
-  double alpha, at, att, LE[6];
-  mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, 0.0, alpha, at, att, LE);
-  for (int n = 0; n < s->ns; ++n) {
-    s->bx[n] = LE[0] + (double)n * s->ds;
-    s->by[n] = LE[1];
-    s->bvx[n] = 0.0; s->bvy[n] = 0.0; s->btheta[n] = 0.0; s->btheta_dot[n] = 0.0;
-  }
-  integrate_position_host(p, p.iMotion, 0.0, s->btheta.data(), s->btheta_dot.data(), s->bx.data(), s->by.data(),
-                          s->bvx.data(), s->bvy.data());
+  // init_beam.f90:130-167 for every beam
+  double alpha[2], LE[12];
+  motion_all(s, 0.0, alpha, LE);
+  for (int ib = 0; ib < s->nb; ++ib)
+    for (int n = 0; n < s->ns; ++n) {
+      const int i = ib * s->ns + n;
+      s->bx[i] = LE[6 * ib] + (double)n * s->ds;
+      s->by[i] = LE[6 * ib + 1];
+      s->bvx[i] = 0.0; s->bvy[i] = 0.0; s->btheta[i] = 0.0; s->btheta_dot[i] = 0.0;
+    }
+  integrate_all(s, 0.0);
   s->beam_time = 0.0;
 }
 
@@ -2286,7 +2455,7 @@ extern "C" int fsibe_get_time(fsibe_sim *s, double *time, double *dt1, int *it) 
 extern "C" int fsibe_get_beam(fsibe_sim *s, double *x, double *y, double *vx, double *vy, double *theta,
                               double *pressure_new, double *tau_new) {
   if (!s) return set_err(FSIBE_EARG, "null sim");
-  for (int i = 0; i < s->ns; ++i) {
+  for (int i = 0; i < s->nb * s->ns; ++i) {
     if (x) x[i] = s->bx[i];
     if (y) y[i] = s->by[i];
     if (vx) vx[i] = s->bvx[i];
@@ -2862,8 +3031,7 @@ static int stage_geometry(fsibe_sim *s, double time) {
   const fsibe_params &p = s->p;
   const bool moving = (p.iSponge >= 4 && s->time < p.T_fullspeed) || p.iFLUSI == 1 || p.iMotion > 0;
   if (p.iBeam > 0 && moving) {
-    double at, att;
-    mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, time, s->alpha, at, att, s->LE);
+    motion_all(s, time, s->alpha, &s->LE[0][0]);
     s->time_mask = time;
   }
   return build_geometry(s, time);
@@ -2920,7 +3088,8 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
   {  // K6
     RowsPressArgs &a = reinterpret_cast<RowsPressArgs *>(A + g.off[B_K6])[j];
     a.Cw = s->T[0]; a.Cpsi = s->T[1]; a.Cpsiy = s->T[2]; a.u1 = s->u1; a.u2 = s->u2; a.vor_mean = s->vor_mean;
-    a.sums = rows_dev(s, s->slot); a.umax_part = s->umax_part; a.mb = mb; a.win = s->win;
+    a.sums = rows_dev(s, s->slot); a.umax_part = s->umax_part; a.mb = mb; a.win = s->win[0];
+    a.win2 = s->win[1];
     a.Rq = s->T[4]; a.Rqy = s->T[5]; a.s = s->dscal; a.UMx0 = 0.0; a.UMy0 = 0.0; a.mode = 1;
     a.pres = s->pressure_on; a.nblk = s->ny; a.c = s->sc;
     maxblk[B_K6] = std::max(maxblk[B_K6], a.nblk);
@@ -2944,15 +3113,15 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
   s->press_box_valid = fz;
   {  // K10
     ForcesArgs &a = reinterpret_cast<ForcesArgs *>(A + g.off[B_K10])[j];
-    const bool ok = fz && s->ntg > 0 && s->ntg == 6 * s->fg.ns + 6 * s->fg.nc;
+    const bool ok = fz && s->ntg > 0 && s->ntg == s->nb * (6 * s->fg.ns + 6 * s->fg.nc);
     a.tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
     a.press = s->fb_p; a.sa = s->fb_sa; a.sb = s->fb_sb;
     a.nd = reinterpret_cast<const double *>(s->d_stage + s->off_nd);
     a.cd = reinterpret_cast<const double *>(s->d_stage + s->off_cd);
     a.val = reinterpret_cast<double *>(s->d_stage + s->off_val);
-    a.pbeam = pbeam_dev(s, s->slot); a.tbeam = pbeam_dev(s, s->slot) + std::max(s->ns, 1);
+    a.pbeam = pbeam_dev(s, s->slot); a.tbeam = pbeam_dev(s, s->slot) + nodes_all(s);
     a.part = parts_dev(s, s->slot); a.g = s->fg;
-    a.nblk = ok ? s->fg.ns + s->fg.nc : 0;
+    a.nblk = ok ? s->nb * (s->fg.ns + s->fg.nc) : 0;
     maxblk[B_K10] = std::max(maxblk[B_K10], a.nblk);
     s->slot_forces[s->slot] = ok;
     s->slot_units[s->slot] = a.nblk;
@@ -3190,8 +3359,7 @@ static int group_finish(fsibe_batch *b, int gi) {
         bad.store(j);
       }
     } else if (p.iMotion > 0 && p.iBeam > 0) {
-      integrate_position_host(p, p.iMotion, s->time, s->btheta.data(), s->btheta_dot.data(), s->bx.data(),
-                              s->by.data(), s->bvx.data(), s->bvy.data());
+      integrate_all(s, s->time);
     }
     std::swap(s->n0, s->n1);
     s->time = s->time + s->dt1;

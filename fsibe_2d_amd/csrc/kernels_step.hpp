@@ -461,16 +461,19 @@ __device__ __forceinline__ void cols_ab2_body(const double2 *__restrict__ Rc, co
 // Partial sums per row (NSUM):
 //  0 sum w^2   1 sum |u-M|^2  2 porous  3 inflow_solid  4 inflow_mean
 //  5 sum p1    6 sum p2 (whole grid)    7 sum p1  8 sum p2  9 sum chi vx
-//  10 sum chi vy (cal_drag window)
+//  10 sum chi vy (the first beam's cal_drag window)   11..14 the same four
+//  sums over the second beam's window (win2, empty with one beam; its
+//  reductions are then skipped and the sums stay 0)
 // ============================================================================
-constexpr int kNSUM = 11;
+constexpr int kNSUM = 15;
+constexpr int kNSUM1 = 11;  // the sums of a run with at most one beam
 struct Window { int x0, x1, y0, y1; };
 
 template <class FX>
 __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
                                                     const double2 *__restrict__ Cpsiy, double *__restrict__ u1,
                                                     double *__restrict__ u2, double *__restrict__ vor_mean,
-                                                    MaskBox mb, Window win, double *__restrict__ sums,
+                                                    MaskBox mb, Window win, Window win2, double *__restrict__ sums,
                                                     double *__restrict__ umax_part, double2 *__restrict__ Rq,
                                                     double2 *__restrict__ Rqy, const DevScalars *__restrict__ s,
                                                     double UMx0, double UMy0, int mode, int pres, FX F,
@@ -499,6 +502,8 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
   }
   const bool mrow = mb.x1 >= mb.x0 && y >= mb.y0 && y <= mb.y1;
   const bool wrow = y >= win.y0 && y <= win.y1;
+  const bool two = win2.x1 >= win2.x0;
+  const bool wrow2 = two && y >= win2.y0 && y <= win2.y1;
   double acc[kNSUM];
 #pragma unroll
   for (int i = 0; i < kNSUM; ++i) acc[i] = 0.0;
@@ -549,6 +554,12 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
           acc[9] += vx * m;
           acc[10] += vy * m;
         }
+        if (wrow2 && x + h >= win2.x0 && x + h <= win2.x1) {
+          acc[11] += p1;
+          acc[12] += p2;
+          acc[13] += vx * m;
+          acc[14] += vy * m;
+        }
         if (h) { q1.y = p1; q2.y = p2; } else { q1.x = p1; q2.x = p2; }
       }
     }
@@ -572,8 +583,16 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
     __syncthreads();
   }
   if (!mode) return;
-  if (mrow) {
+  if (mrow && two) {
     block_sum<kNSUM>(acc, red);
+  } else if (mrow) {
+    // one window: the sums 11..14 are exactly 0
+    double a1[kNSUM1];
+#pragma unroll
+    for (int i = 0; i < kNSUM1; ++i) a1[i] = acc[i];
+    block_sum<kNSUM1>(a1, red);
+#pragma unroll
+    for (int i = 0; i < kNSUM1; ++i) acc[i] = a1[i];
   } else {
     // a row outside the mask box: the mask sums 2..10 are exactly 0, and
     // the wave reductions (two LDS permutes per step and value) are skipped
@@ -690,7 +709,7 @@ __device__ __forceinline__ void rows_box_body(const double2 *__restrict__ Cq, co
 // integral (:306-489).  Interpolation targets are prepared on the host.
 // ============================================================================
 struct ForceGeom {
-  int ns, nc, nt;           // beam nodes, circle points, targets
+  int ns, nc, nt;           // beam nodes, circle points (per beam), targets (all beams)
   int bx0, by0, bw, bh;     // box of the press/stress arrays
   double dxi, dyi, pi;      // DeltaInterpolation spacing (Interpolation.f90:15-16)
   double ds, dL, soft;
@@ -711,13 +730,15 @@ __device__ __forceinline__ double phi4star(double x, double dx, double pi) {
 }
 
 // targets: tg[3*t] = x, tg[3*t+1] = y, tg[3*t+2] = field id (0 press, 1 sa, 2 sb);
-// node n owns targets 6n .. 6n+5 (P(u1) P(b1) A(u) B(u) A(b) B(b)), circle
-// point i (trailing i < nc, leading nc <= i < 2nc) owns 6ns + 3i + {0 A, 1 B, 2 P}.
-// node data nd[4n + ...]: sin th, cos th, sin 2th, cos 2th; circle data
-// cd[4i + ...]: cos phi, sin phi, cos 2phi, sin 2phi.
+// beam b's targets start at b (6ns + 6nc): there node n owns targets
+// 6n .. 6n+5 (P(u1) P(b1) A(u) B(u) A(b) B(b)), circle point i (trailing
+// i < nc, leading nc <= i < 2nc) owns 6ns + 3i + {0 A, 1 B, 2 P}.
+// node data nd[4 (b ns + n) + ...]: sin th, cos th, sin 2th, cos 2th; circle
+// data cd[4 (2 b nc + i) + ...]: cos phi, sin phi, cos 2phi, sin 2phi;
+// loads pbeam/tbeam[b ns + n].
 //
 // One block of 64 threads per unit: a beam node (6 targets) or a pair of
-// circle points (6 targets).  Thread (t, a) evaluates the kernel weights
+// circle points (6 targets); beam b owns units b (ns + nc) .. (b+1)(ns + nc) - 1.  Thread (t, a) evaluates the kernel weights
 // wx[a], wy[a] and the 9 products of stencil column a (DeltaInterpolation,
 // Interpolation.f90:26-44); one thread per target adds the 81 products in
 // the reference's order (ix outer, iy inner).  Thread 0 then forms the
@@ -772,9 +793,13 @@ __device__ __forceinline__ void forces_body(const double *__restrict__ tg, const
                                                           double *__restrict__ part, int bid, int nblk) {
 #pragma clang fp contract(off)
   __shared__ double vals[6];
-  const int u = bid, ns = g.ns;
+  const int ns = g.ns, per = ns + g.nc, b = bid / per, u = bid - b * per;
   const bool node = u < ns;
-  const int t0 = node ? 6 * u : 6 * ns + 6 * (u - ns);
+  const int t0 = b * 6 * per + (node ? 6 * u : 6 * ns + 6 * (u - ns));
+  nd += (size_t)4 * b * ns;
+  cd += (size_t)8 * b * g.nc;
+  pbeam += (size_t)b * ns;
+  tbeam += (size_t)b * ns;
   interp6(tg, t0, 6, press, sa, sb, g, vals, val);
   if (threadIdx.x == 0) {
     // acc: 0 fx, 1 fy, 2 Ft0, 3 Ft1, 4 Fn0, 5 Fn1, 6 Fp0, 7 Fp1
@@ -824,7 +849,7 @@ __device__ __forceinline__ void forces_body(const double *__restrict__ tg, const
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) part[8 * u + k] = acc[k];
+    for (int k = 0; k < 8; ++k) part[8 * bid + k] = acc[k];
   }
 }
 
@@ -907,12 +932,13 @@ template <class FX>
 __global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
                                                        const double2 *__restrict__ Cpsiy, double *__restrict__ u1,
                                                        double *__restrict__ u2, double *__restrict__ vor_mean,
-                                                       MaskBox mb, Window win, double *__restrict__ sums,
+                                                       MaskBox mb, Window win, Window win2,
+                                                       double *__restrict__ sums,
                                                        double *__restrict__ umax_part, double2 *__restrict__ Rq,
                                                        double2 *__restrict__ Rqy, const DevScalars *__restrict__ s,
                                                        double UMx0, double UMy0, int mode, int pres, FX F,
                                                        const double2 *__restrict__ twx, StepConsts c) {
-  rows_press_body(Cw, Cpsi, Cpsiy, u1, u2, vor_mean, mb, win, sums, umax_part, Rq, Rqy, s, UMx0, UMy0, mode, pres,
+  rows_press_body(Cw, Cpsi, Cpsiy, u1, u2, vor_mean, mb, win, win2, sums, umax_part, Rq, Rqy, s, UMx0, UMy0, mode, pres,
                   F, twx, c, blockIdx.x, gridDim.x);
 }
 template <class FY>
@@ -972,7 +998,7 @@ struct RowsPressArgs {
   const double2 *Cw, *Cpsi, *Cpsiy;
   double *u1, *u2, *vor_mean, *sums, *umax_part;
   MaskBox mb;
-  Window win;
+  Window win, win2;
   double2 *Rq, *Rqy;
   const DevScalars *s;
   double UMx0, UMy0;
@@ -1027,7 +1053,7 @@ __global__ void __launch_bounds__(FX::LB) kb_rows_press(const RowsPressArgs *__r
                                                         const double2 *__restrict__ twx) {
   const RowsPressArgs &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
-  rows_press_body(a.Cw, a.Cpsi, a.Cpsiy, a.u1, a.u2, a.vor_mean, a.mb, a.win, a.sums, a.umax_part, a.Rq, a.Rqy, a.s,
+  rows_press_body(a.Cw, a.Cpsi, a.Cpsiy, a.u1, a.u2, a.vor_mean, a.mb, a.win, a.win2, a.sums, a.umax_part, a.Rq, a.Rqy, a.s,
                   a.UMx0, a.UMy0, a.mode, a.pres, F, twx, a.c, blockIdx.x, a.nblk);
 }
 template <class FY>

@@ -99,10 +99,13 @@ static void mouvement_host(const fsibe_params &p, int im, double x0, double y0, 
 // integrate_position.f90:1-47.  The reference recomputes each node's prefix
 // sums from i = 0; carrying them from node s to s+1 performs the same
 // additions in the same order, so the result is identical at O(ns) cost.
-static void integrate_position_host(const fsibe_params &p, int im, double time, const double *theta,
-                                    const double *theta_dot, double *x, double *y, double *vx, double *vy) {
+// The beam's placement (type Solid x0, y0, phase) is an argument: the
+// second beam of a tandem run sits at x0 + 1.25 with its own phase.
+static void integrate_position_host(const fsibe_params &p, int im, double x0, double y0, double phase, double time,
+                                    const double *theta, const double *theta_dot, double *x, double *y, double *vx,
+                                    double *vy) {
   double alpha, at, att, LE[6];
-  mouvement_host(p, im, p.x0, p.y0, p.AngleBeam, 0.0, time, alpha, at, att, LE);
+  mouvement_host(p, im, x0, y0, p.AngleBeam, phase, time, alpha, at, att, LE);
   const int ns = p.ns;
   const double ds = p.length / (double)(ns - 1);
   const double ca = cos(alpha), sa = sin(alpha);
@@ -123,3 +126,23 @@ static void integrate_position_host(const fsibe_params &p, int im, double time, 
   }
 }
 
+// the first beam at the parameters' leading edge
+static void integrate_position_host(const fsibe_params &p, int im, double time, const double *theta,
+                                    const double *theta_dot, double *x, double *y, double *vx, double *vy) {
+  integrate_position_host(p, im, p.x0, p.y0, 0.0, time, theta, theta_dot, x, y, vx, vy);
+}
+
+// placement of beam ib (init_beam.f90:44-76): the first at (x0, y0) with
+// phase 0; the second, for the dragonfly motions 300..307 only, at
+// (x0 + 1.25, y0) with phase (iMotion - 300) pi / 4.  false: no such beam.
+static bool beam_placement(const fsibe_params &p, int ib, double &x0, double &y0, double &phase) {
+  const double pi = 4.0 * atan(1.0);
+  if (ib == 0) {
+    x0 = p.x0; y0 = p.y0; phase = 0.0;
+    return true;
+  }
+  if (ib != 1 || p.iMotion < 300 || p.iMotion > 307) return false;
+  x0 = p.x0 + 1.25; y0 = p.y0;
+  phase = 0.25 * (double)(p.iMotion - 300) * pi;
+  return true;
+}

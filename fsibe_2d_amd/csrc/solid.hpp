@@ -259,6 +259,8 @@ struct Band {
 // The solver for one beam: parameters, scratch, and the step.
 struct Solver {
   fsibe_params p;
+  // the solved beam's placement (type Solid x0, y0, phase; init_beam.f90:44-76)
+  double x0 = 0.0, y0 = 0.0, phase = 0.0;
   int ns = 0, nx = 0;     // nx = 2 ns + 4 unknowns
   double ds = 0.0, grav = 0.0;
   int method = kBDF2;     // TimeMethodSolid (restored after a startup step)
@@ -288,6 +290,7 @@ struct Solver {
   bool init(const fsibe_params &pp) {
     p = pp;
     ns = p.ns;
+    x0 = p.x0; y0 = p.y0; phase = 0.0;
     if (ns < 8) { err = "the solid solver needs ns >= 8 (one-sided fourth-derivative stencils)"; return false; }
     if (!(p.TimeMethodSolid == kEulerImplicit || p.TimeMethodSolid == kCrankNicholson || p.TimeMethodSolid == kBDF2)) {
       err = "TimeMethodSolid must be 1 (implicit Euler), 2 (Crank-Nicolson) or 5 (BDF2)";
@@ -325,7 +328,7 @@ struct Solver {
   }
 
   void motion(double t, double &a, double &at, double &att, double LE[6]) const {
-    mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, t, a, at, att, LE);
+    mouvement_host(p, p.iMotion, x0, y0, p.AngleBeam, phase, t, a, at, att, LE);
   }
 
   // GravityImpulse (SolidSolver.f90:874-911): grav as a function of time
@@ -647,7 +650,7 @@ struct Solver {
     double sk = 0.0, sp = 0.0, se = 0.0, fx = 0.0, fy = 0.0;
     for (int i = 0; i < ns; ++i) {
       sk = sk + (b.vx[i] * b.vx[i] + b.vy[i] * b.vy[i]);
-      sp = sp + (b.y[i] - p.y0);
+      sp = sp + (b.y[i] - y0);
       se = se + th_s[i] * th_s[i];
       fx = fx + b.ax[i];
       fy = fy + b.ay[i];
@@ -660,8 +663,8 @@ struct Solver {
   }
 
   void integrate(double time, Beam &b) const {
-    integrate_position_host(p, p.iMotion, time, b.theta.data(), b.theta_dot.data(), b.x.data(), b.y.data(),
-                            b.vx.data(), b.vy.data());
+    integrate_position_host(p, p.iMotion, x0, y0, phase, time, b.theta.data(), b.theta_dot.data(), b.x.data(),
+                            b.y.data(), b.vx.data(), b.vy.data());
   }
 
   // the scheme of the next IBES step (a BDF2 run starts with Crank-Nicolson)
@@ -862,11 +865,12 @@ struct Solver {
   }
 };
 
-// init_beams for one beam (init_beam.f90:80-120) and InitializeSolidSolver
-inline void init_beam(const fsibe_params &p, Beam &b) {
+// init_beams for one beam (init_beam.f90:80-120) at placement (x0, y0,
+// phase) and InitializeSolidSolver
+inline void init_beam(const fsibe_params &p, Beam &b, double x0, double y0, double phase) {
   b.resize(p.ns);
   double alpha, at, att, LE[6];
-  mouvement_host(p, p.iMotion, p.x0, p.y0, p.AngleBeam, 0.0, 0.0, alpha, at, att, LE);
+  mouvement_host(p, p.iMotion, x0, y0, p.AngleBeam, phase, 0.0, alpha, at, att, LE);
   const double ds = p.length / (double)(p.ns - 1);
   for (int n = 0; n < p.ns; ++n) {
     b.x[n] = LE[0] + (double)n * ds;
@@ -874,8 +878,8 @@ inline void init_beam(const fsibe_params &p, Beam &b) {
   }
   b.dt_old = 0.0;
   b.StartupStep = true;
-  integrate_position_host(p, p.iMotion, 0.0, b.theta.data(), b.theta_dot.data(), b.x.data(), b.y.data(), b.vx.data(),
-                          b.vy.data());
+  integrate_position_host(p, p.iMotion, x0, y0, phase, 0.0, b.theta.data(), b.theta_dot.data(), b.x.data(),
+                          b.y.data(), b.vx.data(), b.vy.data());
 }
 
 }  // namespace ibes

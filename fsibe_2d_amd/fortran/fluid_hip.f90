@@ -6,7 +6,8 @@
 ! become hip_create_mask / hip_evolve_ab2 / hip_get_forces below, with the
 ! fluid state (vortk, nlk, u, masks, vor_mean, press) resident on the GPU.
 ! The beam (type Solid, share_vars.f90:50-63) stays on the host and only its
-! node arrays and the per-node loads cross the bus.  One beam (iBeam = 1).
+! node arrays and the per-node loads cross the bus.  Up to two beams (the
+! reference's loops over beams(1:iBeam)).
 module fluid_hip
   use iso_c_binding
   use share_vars
@@ -47,15 +48,31 @@ contains
   end subroutine hip_fluid_destroy
 
   ! create_mask(time, beams) (create_mask.f90:2-218): the leading-edge
-  ! kinematics come from the host's own mouvement (mouvement.f90:5-241)
+  ! kinematics come from the host's own mouvement (mouvement.f90:5-241) of
+  ! each beam (create_mask.f90:77); the beams' node arrays go side by side
   subroutine hip_create_mask(time, beams)
     use motion
     real(kind=pr), intent(in) :: time
     type(solid), dimension(1:iBeam), intent(inout) :: beams
-    real(kind=pr) :: alpha, alpha_t, alpha_tt, LeadingEdge(1:6)
-    call mouvement(time, alpha, alpha_t, alpha_tt, LeadingEdge, beams(1))
-    call fsibe_check(fsibe_create_mask(fsibe_sim, time, beams(1)%x, beams(1)%y, beams(1)%vx, &
-                                       beams(1)%vy, beams(1)%theta, alpha, LeadingEdge), "create_mask")
+    real(kind=pr) :: alpha(1:max(iBeam, 1)), alpha_t, alpha_tt, LeadingEdge(1:6, 1:max(iBeam, 1))
+    real(kind=pr), dimension(0:ns * max(iBeam, 1) - 1) :: bx, by, bvx, bvy, bth
+    integer :: i, o
+    if (iBeam < 1) then
+      alpha = 0.0_pr
+      LeadingEdge = 0.0_pr
+      bx = 0.0_pr; by = 0.0_pr; bvx = 0.0_pr; bvy = 0.0_pr; bth = 0.0_pr
+    end if
+    do i = 1, iBeam
+      call mouvement(time, alpha(i), alpha_t, alpha_tt, LeadingEdge(:, i), beams(i))
+      o = (i - 1) * ns
+      bx(o:o + ns - 1) = beams(i)%x
+      by(o:o + ns - 1) = beams(i)%y
+      bvx(o:o + ns - 1) = beams(i)%vx
+      bvy(o:o + ns - 1) = beams(i)%vy
+      bth(o:o + ns - 1) = beams(i)%theta
+    end do
+    call fsibe_check(fsibe_create_mask_beams(fsibe_sim, time, bx, by, bvx, bvy, bth, alpha, LeadingEdge), &
+                     "create_mask")
   end subroutine hip_create_mask
 
   ! EvolveFluidExplicit(time, dt1, ...) (NavierStokes.f90:422-459)
@@ -80,19 +97,28 @@ contains
   subroutine fetch_drag(beams)
     type(solid), dimension(1:iBeam), intent(inout) :: beams
     type(fsibe_step_out_t) :: o
-    call fsibe_check(fsibe_get_step_out(fsibe_sim, o), "fsibe_get_step_out")
-    if (iBeam > 0) then
-      beams(1)%Force = o%Force
-      beams(1)%Force_unst = o%Force_unst
-    end if
+    integer :: i
+    do i = 1, iBeam
+      call fsibe_check(fsibe_get_step_out_beam(fsibe_sim, int(i - 1, c_int), o), "fsibe_get_step_out_beam")
+      beams(i)%Force = o%Force
+      beams(i)%Force_unst = o%Force_unst
+    end do
   end subroutine fetch_drag
 
-  ! GetForces(time, beams, press, u, "new") (BeamForces.f90:6-51)
+  ! GetForces(time, beams, press, u, "new") (BeamForces.f90:6-51), every beam
   subroutine hip_get_forces(time, beams)
     real(kind=pr), intent(in) :: time
     type(solid), dimension(1:iBeam), intent(inout) :: beams
-    call fsibe_check(fsibe_get_forces(fsibe_sim, time, beams(1)%pressure_new, beams(1)%tau_new, &
-                                      beams(1)%Force_press), "GetForces")
+    real(kind=pr), dimension(0:ns * max(iBeam, 1) - 1) :: pb, tb
+    real(kind=pr) :: fp(1:2, 1:max(iBeam, 1))
+    integer :: i, o
+    call fsibe_check(fsibe_get_forces(fsibe_sim, time, pb, tb, fp), "GetForces")
+    do i = 1, iBeam
+      o = (i - 1) * ns
+      beams(i)%pressure_new = pb(o:o + ns - 1)
+      beams(i)%tau_new = tb(o:o + ns - 1)
+      beams(i)%Force_press = fp(:, i)
+    end do
   end subroutine hip_get_forces
 
   ! inter = n1; n1 = n0; n0 = inter (time_step.f90:222)

@@ -203,13 +203,28 @@ module fsibe_hip_api
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: sim
       real(c_double), value :: time
-      real(c_double), intent(out) :: pressure_beam(*), tau_beam(*), Force_press(2)
+      real(c_double), intent(out) :: pressure_beam(*), tau_beam(*), Force_press(*)
       integer(c_int) :: rc
     end function
     function fsibe_get_step_out(sim, out) bind(C, name="fsibe_get_step_out") result(rc)
       import :: c_int, c_ptr, fsibe_step_out_t
       type(c_ptr), value :: sim
       type(fsibe_step_out_t), intent(out) :: out
+      integer(c_int) :: rc
+    end function
+    function fsibe_get_step_out_beam(sim, ib, out) bind(C, name="fsibe_get_step_out_beam") result(rc)
+      import :: c_int, c_ptr, fsibe_step_out_t
+      type(c_ptr), value :: sim
+      integer(c_int), value :: ib
+      type(fsibe_step_out_t), intent(out) :: out
+      integer(c_int) :: rc
+    end function
+    function fsibe_create_mask_beams(sim, time, x, y, vx, vy, theta, alpha, LeadingEdge) &
+        bind(C, name="fsibe_create_mask_beams") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      real(c_double), value :: time
+      real(c_double), intent(in) :: x(*), y(*), vx(*), vy(*), theta(*), alpha(*), LeadingEdge(*)
       integer(c_int) :: rc
     end function
     function fsibe_swap_levels(sim) bind(C, name="fsibe_swap_levels") result(rc)
@@ -236,6 +251,13 @@ module fsibe_hip_api
     function fsibe_solid_create(p, b) bind(C, name="fsibe_solid_create") result(rc)
       import :: c_int, c_ptr, fsibe_params_t
       type(fsibe_params_t), intent(in) :: p
+      type(c_ptr), intent(out) :: b
+      integer(c_int) :: rc
+    end function
+    function fsibe_solid_create_beam(p, ib, b) bind(C, name="fsibe_solid_create_beam") result(rc)
+      import :: c_int, c_ptr, fsibe_params_t
+      type(fsibe_params_t), intent(in) :: p
+      integer(c_int), value :: ib
       type(c_ptr), intent(out) :: b
       integer(c_int) :: rc
     end function

@@ -7,13 +7,14 @@
 ! so that a build without MKL (which the reference's IBES_solver and
 ! Solve_LGS need) compiles this file in place of SolidSolver.f90.  The beam
 ! (type Solid, share_vars.f90:50-63) stays the caller's: every call copies
-! its arrays into the solver and back.  One beam (iBeam = 1).
+! its arrays into the solver and back.  One solver per beam, up to two beams
+! (init_beam.f90:44-76 places the second).
 module SolidSolver
   use iso_c_binding
   use share_vars
   use fsibe_hip_api
   implicit none
-  type(c_ptr), save :: fsibe_beam = c_null_ptr
+  type(c_ptr), save :: fsibe_beams(2) = c_null_ptr
 
 contains
 
@@ -44,15 +45,14 @@ contains
     type(solid), dimension(1:iBeam), intent(inout) :: beams
     type(fsibe_params_t) :: p
     integer :: i
-    if (iBeam > 1) then
-      write (*, '(A)') "solid_hip: one beam only (iBeam = 1)"
+    if (iBeam > 2) then
+      write (*, '(A)') "solid_hip: at most two beams"
       stop
     end if
-    if (.not. c_associated(fsibe_beam)) then
-      call solid_params(p)
-      call fsibe_check(fsibe_solid_create(p, fsibe_beam), "InitializeSolidSolver")
-    end if
+    call solid_params(p)
     do i = 1, iBeam
+      if (.not. c_associated(fsibe_beams(i))) &
+        call fsibe_check(fsibe_solid_create_beam(p, int(i - 1, c_int), fsibe_beams(i)), "InitializeSolidSolver")
       beams(i)%StartupStep = .true.
     end do
   end subroutine InitializeSolidSolver
@@ -65,59 +65,64 @@ contains
     call fsibe_check(fsibe_gravity_impulse(p, time, grav), "GravityImpulse")
   end subroutine GravityImpulse
 
-  ! SolidSolverWrapper (SolidSolver.f90:13-44): IBES step or the held
-  ! beam's leading-edge motion, then SolidEnergies
+  ! SolidSolverWrapper (SolidSolver.f90:13-44): per beam (:25-33) an IBES
+  ! step or the beam's leading-edge motion, then SolidEnergies
   subroutine SolidSolverWrapper(time, dt, beams)
     real(kind=pr), intent(in) :: time, dt
     type(solid), dimension(1:iBeam), intent(inout) :: beams
     type(fsibe_solid_info_t) :: info
-    if (.not. c_associated(fsibe_beam)) call InitializeSolidSolver(beams)
-    call put(FSIBE_SOLID_X, beams(1)%x)
-    call put(FSIBE_SOLID_Y, beams(1)%y)
-    call put(FSIBE_SOLID_VX, beams(1)%vx)
-    call put(FSIBE_SOLID_VY, beams(1)%vy)
-    call put(FSIBE_SOLID_THETA, beams(1)%theta)
-    call put(FSIBE_SOLID_THETA_DOT, beams(1)%theta_dot)
-    call put(FSIBE_SOLID_AX, beams(1)%ax)
-    call put(FSIBE_SOLID_AY, beams(1)%ay)
-    call put(FSIBE_SOLID_PRESSURE_OLD, beams(1)%pressure_old)
-    call put(FSIBE_SOLID_PRESSURE_NEW, beams(1)%pressure_new)
-    call put(FSIBE_SOLID_TAU_OLD, beams(1)%tau_old)
-    call put(FSIBE_SOLID_TAU_NEW, beams(1)%tau_new)
-    call fsibe_check(fsibe_solid_set(fsibe_beam, FSIBE_SOLID_OLDOLD, beams(1)%beam_oldold), "SolidSolverWrapper")
-    info%dt_old = beams(1)%dt_old
-    info%grav = grav
-    info%StartupStep = merge(1_c_int32_t, 0_c_int32_t, beams(1)%StartupStep)
-    call fsibe_check(fsibe_solid_set_info(fsibe_beam, info), "SolidSolverWrapper")
-    call fsibe_check(fsibe_solid_wrapper(fsibe_beam, time, dt), "SolidSolverWrapper")
-    call get(FSIBE_SOLID_X, beams(1)%x)
-    call get(FSIBE_SOLID_Y, beams(1)%y)
-    call get(FSIBE_SOLID_VX, beams(1)%vx)
-    call get(FSIBE_SOLID_VY, beams(1)%vy)
-    call get(FSIBE_SOLID_THETA, beams(1)%theta)
-    call get(FSIBE_SOLID_THETA_DOT, beams(1)%theta_dot)
-    call get(FSIBE_SOLID_AX, beams(1)%ax)
-    call get(FSIBE_SOLID_AY, beams(1)%ay)
-    call fsibe_check(fsibe_solid_get(fsibe_beam, FSIBE_SOLID_OLDOLD, beams(1)%beam_oldold), "SolidSolverWrapper")
-    call fsibe_check(fsibe_solid_get_info(fsibe_beam, info), "SolidSolverWrapper")
-    beams(1)%dt_old = info%dt_old
-    beams(1)%StartupStep = info%StartupStep /= 0
-    beams(1)%E_kinetic = info%E_kinetic
-    beams(1)%E_pot = info%E_pot
-    beams(1)%E_elastic = info%E_elastic
-    beams(1)%Inertial_Force = info%Inertial_Force
-    ! the emergency brake ends the loop after this step (SolidSolver.f90:370-380)
-    if (info%brake /= 0) continue_timestep = .false.
+    type(c_ptr) :: h
+    integer :: i
+    if (.not. c_associated(fsibe_beams(1))) call InitializeSolidSolver(beams)
+    do i = 1, iBeam
+      h = fsibe_beams(i)
+      call put(FSIBE_SOLID_X, beams(i)%x)
+      call put(FSIBE_SOLID_Y, beams(i)%y)
+      call put(FSIBE_SOLID_VX, beams(i)%vx)
+      call put(FSIBE_SOLID_VY, beams(i)%vy)
+      call put(FSIBE_SOLID_THETA, beams(i)%theta)
+      call put(FSIBE_SOLID_THETA_DOT, beams(i)%theta_dot)
+      call put(FSIBE_SOLID_AX, beams(i)%ax)
+      call put(FSIBE_SOLID_AY, beams(i)%ay)
+      call put(FSIBE_SOLID_PRESSURE_OLD, beams(i)%pressure_old)
+      call put(FSIBE_SOLID_PRESSURE_NEW, beams(i)%pressure_new)
+      call put(FSIBE_SOLID_TAU_OLD, beams(i)%tau_old)
+      call put(FSIBE_SOLID_TAU_NEW, beams(i)%tau_new)
+      call fsibe_check(fsibe_solid_set(h, FSIBE_SOLID_OLDOLD, beams(i)%beam_oldold), "SolidSolverWrapper")
+      info%dt_old = beams(i)%dt_old
+      info%grav = grav
+      info%StartupStep = merge(1_c_int32_t, 0_c_int32_t, beams(i)%StartupStep)
+      call fsibe_check(fsibe_solid_set_info(h, info), "SolidSolverWrapper")
+      call fsibe_check(fsibe_solid_wrapper(h, time, dt), "SolidSolverWrapper")
+      call get(FSIBE_SOLID_X, beams(i)%x)
+      call get(FSIBE_SOLID_Y, beams(i)%y)
+      call get(FSIBE_SOLID_VX, beams(i)%vx)
+      call get(FSIBE_SOLID_VY, beams(i)%vy)
+      call get(FSIBE_SOLID_THETA, beams(i)%theta)
+      call get(FSIBE_SOLID_THETA_DOT, beams(i)%theta_dot)
+      call get(FSIBE_SOLID_AX, beams(i)%ax)
+      call get(FSIBE_SOLID_AY, beams(i)%ay)
+      call fsibe_check(fsibe_solid_get(h, FSIBE_SOLID_OLDOLD, beams(i)%beam_oldold), "SolidSolverWrapper")
+      call fsibe_check(fsibe_solid_get_info(h, info), "SolidSolverWrapper")
+      beams(i)%dt_old = info%dt_old
+      beams(i)%StartupStep = info%StartupStep /= 0
+      beams(i)%E_kinetic = info%E_kinetic
+      beams(i)%E_pot = info%E_pot
+      beams(i)%E_elastic = info%E_elastic
+      beams(i)%Inertial_Force = info%Inertial_Force
+      ! the emergency brake ends the loop after this step (SolidSolver.f90:370-380)
+      if (info%brake /= 0) continue_timestep = .false.
+    end do
   contains
     subroutine put(which, a)
       integer(c_int), intent(in) :: which
       real(kind=pr), dimension(0:ns-1), intent(in) :: a
-      call fsibe_check(fsibe_solid_set(fsibe_beam, which, a), "SolidSolverWrapper")
+      call fsibe_check(fsibe_solid_set(h, which, a), "SolidSolverWrapper")
     end subroutine put
     subroutine get(which, a)
       integer(c_int), intent(in) :: which
       real(kind=pr), dimension(0:ns-1), intent(out) :: a
-      call fsibe_check(fsibe_solid_get(fsibe_beam, which, a), "SolidSolverWrapper")
+      call fsibe_check(fsibe_solid_get(h, which, a), "SolidSolverWrapper")
     end subroutine get
   end subroutine SolidSolverWrapper
 

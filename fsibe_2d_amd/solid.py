@@ -21,12 +21,14 @@ from .params import Params
 
 
 class Solid:
-    def __init__(self, params: Params | None = None, handle=None, owner=None):
+    def __init__(self, params: Params | None = None, handle=None, owner=None, beam: int = 0):
+        """params: a new solver for beam `beam` at its placement
+        (init_beam.f90:44-76); handle: a solver owned by `owner`"""
         if handle is None:
             self.p = params
             self._pc = params.to_c()
             h = C.c_void_p()
-            check(lib().fsibe_solid_create(C.byref(self._pc), C.byref(h)))
+            check(lib().fsibe_solid_create_beam(C.byref(self._pc), int(beam), C.byref(h)))
             self.h, self._owned = h, True
         else:
             self.p = params

@@ -46,21 +46,32 @@ def mouvement(p: Params, time: float, iMouvement: int | None = None, phase: floa
     return a.value, at.value, att.value, le
 
 
-def integrate_position(p: Params, time: float, theta, theta_dot):
-    """Beam positions and velocities from theta (integrate_position.f90)."""
+def integrate_position(p: Params, time: float, theta, theta_dot, beam: int = 0):
+    """Beam positions and velocities from theta (integrate_position.f90) for
+    beam `beam` at its placement (beam_placement)."""
     ns = p.ns
     th = np.ascontiguousarray(theta, dtype=np.float64)
     thd = np.ascontiguousarray(theta_dot, dtype=np.float64)
     x, y, vx, vy = (np.zeros(ns) for _ in range(4))
     pc = p.to_c()
-    check(lib().fsibe_integrate_position(C.byref(pc), p.iMotion, float(time), th, thd, x, y, vx, vy))
+    check(lib().fsibe_integrate_position_beam(C.byref(pc), int(beam), float(time), th, thd, x, y, vx, vy))
     return x, y, vx, vy
+
+
+def beam_placement(p: Params, beam: int):
+    """(x0, y0, phase) of beam `beam` (init_beam.f90:44-76): the second beam
+    of the dragonfly motions 300..307 at x0 + 1.25 with phase (iMotion-300) pi/4"""
+    x0, y0, ph = C.c_double(), C.c_double(), C.c_double()
+    pc = p.to_c()
+    check(lib().fsibe_beam_placement(C.byref(pc), int(beam), C.byref(x0), C.byref(y0), C.byref(ph)))
+    return x0.value, y0.value, ph.value
 
 
 class FluidSim:
     def __init__(self, params: Params):
         self.p = params
         self.nx, self.ny, self.ns = params.nx, params.ny, params.ns
+        self.nb = params.iBeam
         self._pc = params.to_c()
         h = C.c_void_p()
         check(lib().fsibe_sim_create(C.byref(self._pc), C.byref(h)))
@@ -101,9 +112,15 @@ class FluidSim:
 
     # ---- the reference's per-step calls ----------------------------------------
     def create_mask(self, time, x, y, vx, vy, theta, alpha, LeadingEdge):
-        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
-        check(lib().fsibe_create_mask(self.h, float(time), f(x), f(y), f(vx), f(vy), f(theta),
-                                      float(alpha), f(LeadingEdge)))
+        """one beam: arrays of ns values, alpha a number, LeadingEdge 6 values;
+        iBeam beams: arrays (iBeam, ns), alpha (iBeam,), LeadingEdge (iBeam, 6)"""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if self.nb > 1:
+            check(lib().fsibe_create_mask_beams(self.h, float(time), f(x), f(y), f(vx), f(vy), f(theta),
+                                                f(alpha), f(LeadingEdge)))
+        else:
+            check(lib().fsibe_create_mask(self.h, float(time), f(x), f(y), f(vx), f(vy), f(theta),
+                                          float(alpha), f(LeadingEdge)))
 
     def evolve_fluid_explicit(self, time) -> float:
         d = C.c_double()
@@ -116,13 +133,23 @@ class FluidSim:
         return d.value
 
     def get_forces(self, time):
-        pb, tb, fp = np.zeros(max(self.ns, 1)), np.zeros(max(self.ns, 1)), np.zeros(2)
+        """(pressure, tau, Force_press): ns, ns, 2 values for one beam; (iBeam,
+        ns), (iBeam, ns), (iBeam, 2) for several"""
+        nb = max(self.nb, 1)
+        pb, tb, fp = np.zeros(max(nb * self.ns, 1)), np.zeros(max(nb * self.ns, 1)), np.zeros(2 * nb)
         check(lib().fsibe_get_forces(self.h, float(time), pb, tb, fp))
+        if self.nb > 1:
+            return pb.reshape(nb, self.ns), tb.reshape(nb, self.ns), fp.reshape(nb, 2)
         return pb[: self.ns], tb[: self.ns], fp
 
-    def step_out(self) -> dict:
+    def step_out(self, beam: int | None = None) -> dict:
+        """the last step's results; beam: the per-beam fields of that beam
+        (default: the first)"""
         o = FsibeStepOut()
-        check(lib().fsibe_get_step_out(self.h, C.byref(o)))
+        if beam is None:
+            check(lib().fsibe_get_step_out(self.h, C.byref(o)))
+        else:
+            check(lib().fsibe_get_step_out_beam(self.h, int(beam), C.byref(o)))
         return step_out_dict(o)
 
     def swap_levels(self):
@@ -139,12 +166,17 @@ class FluidSim:
         check(lib().fsibe_get_time(self.h, C.byref(t), C.byref(d), C.byref(it)))
         return t.value, d.value, it.value
 
-    def beam(self) -> dict:
-        n = max(self.ns, 1)
+    def beam(self, beam: int = 0) -> dict:
+        """state of beam `beam` held by run()"""
+        nb = max(self.nb, 1)
+        if not 0 <= beam < nb:
+            raise IndexError(f"beam {beam} of {self.nb}")
+        n = max(nb * self.ns, 1)
         arrs = [np.zeros(n) for _ in range(7)]
         check(lib().fsibe_get_beam(self.h, *arrs))
         keys = ("x", "y", "vx", "vy", "theta", "pressure", "tau")
-        return {k: a[: self.ns] for k, a in zip(keys, arrs)}
+        o = beam * self.ns
+        return {k: a[o: o + self.ns] for k, a in zip(keys, arrs)}
 
     def backup_write(self, path, ivideo: int = 1, colorscale: float = 0.0, colorscale_done: bool = False):
         """MakeRuntimeBackup (save_fields.f90:109-152) into `path`"""
@@ -179,10 +211,14 @@ class FluidSim:
     @property
     def solid(self):
         """the solid solver fsibe_run uses for iFLUSI=1 (fsibe_sim_solid)"""
+        return self.solid_of(0)
+
+    def solid_of(self, beam: int):
+        """the solid solver of beam `beam` (fsibe_sim_solid_beam)"""
         from .solid import Solid
 
         h = C.c_void_p()
-        check(lib().fsibe_sim_solid(self.h, C.byref(h)))
+        check(lib().fsibe_sim_solid_beam(self.h, int(beam), C.byref(h)))
         return Solid(self.p, handle=h.value, owner=self)
 
 
@@ -213,6 +249,7 @@ class _Member(FluidSim):
     def __init__(self, params: Params, handle):
         self.p = params
         self.nx, self.ny, self.ns = params.nx, params.ny, params.ns
+        self.nb = params.iBeam
         self.h = C.c_void_p(handle)
 
     def close(self):

@@ -174,10 +174,20 @@ int fsibe_set_vortk(fsibe_sim *sim, const double *vortk_packed);
  * type Solid members x, y, vx, vy, theta) together with the leading-edge
  * kinematics mouvement(time) returns (alpha, LeadingEdge(1:6)); the masks are
  * rebuilt on the GPU.  The same beam state and kinematics are used by the
- * next fsibe_evolve_* (cal_drag) and fsibe_get_forces calls. */
+ * next fsibe_evolve_* (cal_drag) and fsibe_get_forces calls.  For iBeam <= 1;
+ * with two beams FSIBE_EARG (use fsibe_create_mask_beams). */
 int fsibe_create_mask(fsibe_sim *sim, double time, const double *x, const double *y,
                       const double *vx, const double *vy, const double *theta, double alpha,
                       const double LeadingEdge[6]);
+/* create_mask(time, beams) for iBeam beams (create_mask.f90:74-119, a loop
+ * over beams(ib) calling mouvement(time, ..., beams(ib)) for each): x .. theta
+ * hold iBeam*ns values, beam ib's nodes from ib*ns; alpha[iBeam] and
+ * LeadingEdge[6*iBeam] are each beam's mouvement(time).  The beams are drawn
+ * in order, each primitive where its value is strictly larger, as the
+ * reference draws them. */
+int fsibe_create_mask_beams(fsibe_sim *sim, double time, const double *x, const double *y,
+                            const double *vx, const double *vy, const double *theta,
+                            const double *alpha, const double *LeadingEdge);
 
 /* EvolveFluidExplicit(time, dt1, n0, n1, ...) -- NavierStokes.f90:422-459.
  * Returns dt1 chosen by the CFL logic of cal_nlk (NavierStokes.f90:59-78). */
@@ -185,11 +195,18 @@ int fsibe_evolve_explicit(fsibe_sim *sim, double time, double *dt1);
 /* EvolveFluidAB2(time, dt0, dt1, n0, n1, ...) -- NavierStokes.f90:463-508 */
 int fsibe_evolve_ab2(fsibe_sim *sim, double time, double dt0, double *dt1);
 /* GetForces(time, beams, press, u, "new") -- BeamForces.f90:6-288 with the
- * hard-wired SPL=3 Delta interpolation.  pressure_beam, tau_beam: ns values. */
+ * hard-wired SPL=3 Delta interpolation, every beam (:24-47).  pressure_beam,
+ * tau_beam: iBeam*ns values (beam ib from ib*ns); Force_press: 2*iBeam values
+ * (2 without a beam). */
 int fsibe_get_forces(fsibe_sim *sim, double time, double *pressure_beam, double *tau_beam,
                      double Force_press[2]);
-/* results of the last evolve / get_forces (cal_drag forces, energies, ...) */
+/* results of the last evolve / get_forces (cal_drag forces, energies, ...);
+ * the per-beam fields (Force, Force_unst, Force_press, bsi[2..9], fx_int,
+ * fy_int, ibes_iter) are the first beam's */
 int fsibe_get_step_out(fsibe_sim *sim, fsibe_step_out *out);
+/* the same with the per-beam fields of beam ib (0 <= ib < iBeam): beams(ib+1)
+ * of cal_drag (NavierStokes.f90:528-589) and GetForces (BeamForces.f90:24-47) */
+int fsibe_get_step_out_beam(fsibe_sim *sim, int ib, fsibe_step_out *out);
 /* swap time levels n0 <-> n1 (time_step.f90:152,222) */
 int fsibe_swap_levels(fsibe_sim *sim);
 
@@ -223,6 +240,14 @@ int fsibe_mouvement(const fsibe_params *p, int iMouvement, double x0, double y0,
 int fsibe_integrate_position(const fsibe_params *p, int iMouvement, double time,
                              const double *theta, const double *theta_dot, double *x, double *y,
                              double *vx, double *vy);
+/* placement of beam ib (init_beam.f90:44-76): ib 0 at (x0, y0), phase 0; ib 1
+ * (iMotion 300..307 only, else FSIBE_EARG as the reference stops) at
+ * (x0 + 1.25, y0), phase (iMotion - 300) pi / 4 */
+int fsibe_beam_placement(const fsibe_params *p, int ib, double *x0, double *y0, double *phase);
+/* integrate_position(time, beams(ib+1)) with iMotion and beam ib's placement */
+int fsibe_integrate_position_beam(const fsibe_params *p, int ib, double time,
+                                  const double *theta, const double *theta_dot, double *x,
+                                  double *y, double *vx, double *vy);
 
 /* ---- host solid solver (SolidSolver.f90) -----------------------------------
  * One beam: the Newton solver IBES_solver (SolidSolver.f90:99-414) for the
@@ -237,6 +262,8 @@ typedef struct fsibe_solid fsibe_solid;
  * edge of mouvement(0), integrate_position(0)) and InitializeSolidSolver
  * (SolidSolver.f90:54-67).  Needs ns >= 8. */
 int fsibe_solid_create(const fsibe_params *p, fsibe_solid **out);
+/* the same for beam ib at its placement (fsibe_beam_placement) */
+int fsibe_solid_create_beam(const fsibe_params *p, int ib, fsibe_solid **out);
 int fsibe_solid_destroy(fsibe_solid *b);
 /* arrays of type(solid) (share_vars.f90:50-63), ns values each; OLDOLD is
  * beam_oldold(0:ns-1, 1:6), 6*ns values, column-major (x y vx vy theta
@@ -290,8 +317,10 @@ int fsibe_solid_jacobian(fsibe_solid *b, double time, double dt, const double *x
 int fsibe_solid_rhs(fsibe_solid *b, double time, const double *theta, const double *theta_dot,
                     const double *pressure, const double *tau, double *theta_t, double *theta_tt,
                     double *T);
-/* the solver fsibe_run uses for iFLUSI=1 (created by its first call) */
+/* the solver fsibe_run uses for iFLUSI=1 (created by its first call); of the
+ * first beam, or of beam ib */
 int fsibe_sim_solid(fsibe_sim *sim, fsibe_solid **out);
+int fsibe_sim_solid_beam(fsibe_sim *sim, int ib, fsibe_solid **out);
 
 /* ---- the time loop of time_step.f90:120-341 --------------------------------
  * The startup Euler step if not yet taken, then AB2 steps, each create_mask
@@ -306,7 +335,8 @@ int fsibe_sim_solid(fsibe_sim *sim, fsibe_solid **out);
 int fsibe_run(fsibe_sim *sim, int nsteps, fsibe_step_out *out);
 /* current time, dt1 and number of AB2 steps taken by fsibe_run */
 int fsibe_get_time(fsibe_sim *sim, double *time, double *dt1, int *it);
-/* beam state held by fsibe_run (x, y, vx, vy, theta, pressure_new, tau_new) */
+/* beam state held by fsibe_run (x, y, vx, vy, theta, pressure_new, tau_new),
+ * iBeam*ns values each (beam ib from ib*ns) */
 int fsibe_get_beam(fsibe_sim *sim, double *x, double *y, double *vx, double *vy, double *theta,
                    double *pressure_new, double *tau_new);
 /* ---- batches of independent simulations --------------------------------

@@ -57,7 +57,9 @@ def test_argument_errors_before_any_device_call():
 
     assert create(iSponge=7) == 4 and "sponge type" in _err(L)
     assert create(iSponge=22, eps_sponge=0.0) == 1
-    assert create(iBeam=2) == 4
+    assert create(iBeam=3) == 4 and "at most two beams" in _err(L)
+    # a second beam exists for the dragonfly motions only (init_beam.f90:55-74)
+    assert create(iBeam=2, iMotion=2) == 1 and "iMotion bad" in _err(L)
     assert create(eps=0.0) == 1
     assert create(iSponge=3, eps_sponge=0.0) == 1
     assert isinstance(FsibeParams(), C.Structure)
@@ -165,3 +167,29 @@ def test_host_1d_transforms_against_numpy(L):
         k = (L - 1) // 2
         lost = -2 * c[:, k].imag[:, None] * np.sin(2 * np.pi * k * np.arange(L) / L)[None, :]
         assert np.abs(g - (f - lost)).max() < 1e-13
+
+
+@pytest.mark.parametrize("im", [300, 303, 307])
+def test_beam_placement_and_second_beam_kinematics(oracle_mod, im):
+    """init_beam.f90:44-76 as the oracle places the beams; each beam's
+    integrate_position with its own x0 and phase (integrate_position.f90:14)"""
+    from fsibe_2d_amd._lib import FsibeError
+    from fsibe_2d_amd.params import Params
+    from fsibe_2d_amd.solver import beam_placement, integrate_position
+
+    p = Params(nx=64, ny=64, ns=16, iBeam=2, iMotion=im, x0=0.9, y0=2.0)
+    o = oracle_mod.Oracle(nx=64, ny=64, ns=16, iBeam=2, iMotion=im, iSponge=0, x0=0.9, y0=2.0)
+    rng = np.random.default_rng(im)
+    for ib in range(2):
+        b = o.beam_arrays(ib)
+        assert beam_placement(p, ib) == (b["x0"], b["y0"], b["phase"])
+        o.set_beam_arrays(ib, theta=rng.normal(0, 0.1, 16), theta_dot=rng.normal(0, 0.5, 16))
+    for t in (0.0, 0.31, 0.77):
+        o.integrate_position(t)
+        for ib in range(2):
+            want = o.beam_arrays(ib)
+            got = integrate_position(p, t, want["theta"], want["theta_dot"], beam=ib)
+            for k, g in zip(("x", "y", "vx", "vy"), got):
+                np.testing.assert_array_equal(g, want[k], err_msg=f"{k} beam {ib} t={t}")
+    with pytest.raises(FsibeError):
+        beam_placement(Params(nx=64, ny=64, iBeam=2, iMotion=2), 1)

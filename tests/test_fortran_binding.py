@@ -19,13 +19,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "fortran", "build")
 
 
-def _run(name, tmp_path):
+def _run(name, tmp_path, *args):
     exe = os.path.join(BIN, name)
     if not os.path.exists(exe):
         pytest.skip(f"{name} not built (needs the reference's modules)")
     out = str(tmp_path / (name + ".bin"))
-    print(f"running {name}", flush=True)
-    r = subprocess.run([exe, out], capture_output=True, text=True, timeout=120)
+    print(f"running {name} {' '.join(args)}", flush=True)
+    r = subprocess.run([exe, out, *args], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and os.path.exists(out), r.stdout + r.stderr
     return np.fromfile(out, dtype=np.uint8)
 
@@ -103,3 +103,42 @@ def test_time_loop_through_fortran_module(tmp_path, oracle_mod):
     assert close(force, b["Force"]) and close(force_press, b["Force_press"])
     assert close(p_beam, b["pressure"]) and close(tau_beam, b["tau"])
     assert close(force_unst, b["Force_unst"])
+
+
+def test_two_beam_time_loop_through_fortran_module(tmp_path, oracle_mod):
+    """the same loop with two beams in tandem (iMotion 303): the Fortran
+    module's loops over beams(1:2) (create_mask with each beam's mouvement,
+    cal_drag and GetForces per beam) against the oracle's"""
+    rd = _Reader(_run("drive_step_hip", tmp_path, "2"))
+    nx, ny, ns = rd.ints(3)
+    f = rd.reals(nx * ny, (ny, nx))
+    time, dt1 = rd.reals(2)
+    vortk = rd.reals(nx * ny, (ny, nx))
+    beams = []
+    for _ in range(2):
+        force, force_unst, force_press = rd.reals(2), rd.reals(2), rd.reals(2)
+        beams.append((force, force_unst, force_press, *(rd.reals(ns) for _ in range(4))))
+    o = oracle_mod.Oracle(nx=nx, ny=ny, ns=ns, xl=4.0, yl=4.0, nu=5e-3, eps=1e-3, eps_sponge=1e-3, cfl=1.0,
+                          dt_fixed=2e-4, Time_end=1e6, N_smooth=2.0, t_beam=0.05, length=1.0,
+                          SpongeSize=0.4, x0=0.8, y0=2.0, AngleBeam=5.0, iMeanVelocity=3, iSponge=3,
+                          iBeam=2, iMotion=303, iViscous=1, iSaveBeam=1, idealis=1, U_mean_true=1.0)
+    o.set_vortk(o.coftxy(f))
+    o.run(10)
+    d = o.diag()
+    assert abs(time - d.time) < 1e-15 and dt1 == d.dt1
+    err = rel_l2(vortk, o.field("vortk"))
+    print(f"fortran two-beam loop: vortk rel L2 {err:.2e}", flush=True)
+    assert err < 1e-10
+
+    def close(a, e):
+        e = np.asarray(e)
+        floor = 1e-6 * max(np.abs(e).max(), 1e-300)
+        return np.all(np.abs(np.asarray(a) - e) <= 1e-9 * np.maximum(np.abs(e), floor))
+
+    for ib, (force, force_unst, force_press, p_beam, tau_beam, bx, by) in enumerate(beams):
+        b = o.beam_arrays(ib)
+        assert np.array_equal(bx, b["x"]) and np.array_equal(by, b["y"]), ib
+        assert close(force, b["Force"]) and close(force_press, b["Force_press"]), ib
+        assert close(p_beam, b["pressure"]) and close(tau_beam, b["tau"]), ib
+        assert close(force_unst, b["Force_unst"]), ib
+    assert not np.array_equal(beams[0][0], beams[1][0])
