@@ -176,3 +176,80 @@ def test_tandem_coupled(oracle_mod):
     assert out["Force"] == g.step_out(beam=0)["Force"]
     # the beams bend differently (opposite heave phase, the second in the first's wake)
     assert np.abs(g.beam(0)["theta"] - g.beam(1)["theta"]).max() > 0.0
+
+
+@pytest.mark.parametrize("coupled", [False, True])
+def test_tandem_restart(tmp_path, coupled, oracle_mod):
+    """six backup records per beam (save_fields.f90:132-139): the file holds
+    each beam at its own placement, and a restored run continues the
+    uninterrupted one (12 + 9 against 21 steps)"""
+    from fsibe_2d_amd import restart_files as rf
+    from fsibe_2d_amd.solver import FluidSim
+
+    p = _params(nx=96, ny=64, iMotion=301, iFLUSI=1 if coupled else 0, mue=1.0, eta=0.1)
+    vk = _noise(oracle_mod, p, seed=4)
+    a = FluidSim(p)
+    a.set_vortk(vk)
+    a.run(12)
+    path = tmp_path / "runtime_backup0.in"
+    a.backup_write(path)
+    d = rf.read_backup(path, p.nx, p.ny, p.ns, nbeams=2)
+    for ib in range(2):
+        bd, gb = d["beams"][ib], a.beam(ib)
+        assert bd["x0"] == p.x0 + 1.25 * ib and bd["y0"] == p.y0
+        assert np.array_equal(bd["y"], gb["y"]) and np.array_equal(bd["pressure_new"], gb["pressure"])
+        assert tuple(bd["Force"]) == a.step_out(beam=ib)["Force"]
+    b = FluidSim(p)
+    b.backup_read(path)
+    if coupled:  # the reference's restart begins each BDF2 solver with a Crank-Nicolson step
+        for ib in range(2):
+            s = a.solid_of(ib)
+            i = s.info()
+            s.set_info(i["dt_old"], i["grav"], True)
+    a.run(9)
+    b.run(9)
+    errs = {k: rel_l2(a.field(k), b.field(k)) for k in ("vortk", "u1", "u2", "mask")}
+    print(f"[tandem restart coupled={coupled}] " + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+    assert all(v <= 1e-13 for v in errs.values()), errs
+    for ib in range(2):
+        oa, ob = a.step_out(beam=ib), b.step_out(beam=ib)
+        for k in ("Force", "Force_press", "Force_unst"):
+            x, y = np.asarray(oa[k]), np.asarray(ob[k])
+            assert np.abs(x - y).max() <= 1e-11 * max(np.abs(y).max(), 1e-300), (ib, k)
+        ba, bb = a.beam(ib), b.beam(ib)
+        for k in ba:
+            # the loads are differences of pressures interpolated on either
+            # side of the beam, larger than the loads early in the soft start:
+            # the restored state's rounding shows there relative to the load
+            tol = 1e-9 if k in ("pressure", "tau") else 1e-11
+            assert np.abs(ba[k] - bb[k]).max() <= tol * max(np.abs(bb[k]).max(), 1e-300), (ib, k)
+    a.close()
+    b.close()
+
+
+def test_tandem_batch_member(oracle_mod):
+    """a two-beam member of a batch beside one-beam members, prescribed and
+    solved: bit-identical to its own run"""
+    from fsibe_2d_amd.solver import Batch, FluidSim
+
+    ps = [_params(iMotion=302), _params(iBeam=1, iMotion=2), _params(iMotion=306, iFLUSI=1, mue=1.0, eta=0.1),
+          _params(nx=96, ny=96, iMotion=300)]
+    vks = [_noise(oracle_mod, p, 30 + i) for i, p in enumerate(ps)]
+    b = Batch(ps, vks)
+    steps = 8
+    b.run(steps)
+    for i, p in enumerate(ps):
+        g = FluidSim(p)
+        g.set_vortk(vks[i])
+        g.run(steps)
+        m = b.members[i]
+        assert m.time_state() == g.time_state(), i
+        assert np.array_equal(m.field("vortk"), g.field("vortk")), i
+        for ib in range(p.iBeam):
+            for k in ("x", "y", "theta", "pressure", "tau"):
+                assert np.array_equal(m.beam(ib)[k], g.beam(ib)[k]), (i, ib, k)
+            so, go = m.step_out(beam=ib), g.step_out(beam=ib)
+            for k in ("Force", "Force_unst", "Force_press", "bsi", "ibes_iter"):
+                assert so[k] == go[k], (i, ib, k)
+        g.close()
+    b.close()
