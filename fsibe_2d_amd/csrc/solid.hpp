@@ -179,13 +179,19 @@ struct Band {
   std::vector<int> last, piv;
   std::vector<double> inv;  // 1 / U(j, j) after factor()
   void reset(int nn) {
-    n = nn;
-    a.assign((size_t)n * kBW, 0.0);
-    last.resize(n);
-    piv.resize(n);
-    inv.resize(n);
+    if (n != nn || a.size() != (size_t)nn * kBW) {
+      n = nn;
+      a.assign((size_t)n * kBW, 0.0);
+      last.resize(n);
+      piv.resize(n);
+      inv.resize(n);
+    } else {
+      std::fill(a.begin(), a.end(), 0.0);
+    }
     for (int r = 0; r < n; ++r) last[r] = r;
   }
+  // flat index of entry (r, c) in a
+  static size_t index(int r, int c) { return (size_t)r * kBW + kKL - r + c; }
   double *row(int r) { return a.data() + (size_t)r * kBW + kKL - r; }
   const double *row(int r) const { return a.data() + (size_t)r * kBW + kKL - r; }
   double &at(int r, int c) {
@@ -266,6 +272,10 @@ struct Solver {
   int method = kBDF2;     // TimeMethodSolid (restored after a startup step)
   Diff dn, de;            // stencils on ns points and on the ns + 3 extended points
   Band band, lu;          // the Jacobian being assembled; the factorisation in use
+  // the Jacobian's sparsity is the same at every iteration: the band index
+  // of each emitted entry, in emission order, and each row's last column
+  std::vector<size_t> jofs;
+  std::vector<int> jlast;
   int refactored = 0;     // Newton iterations after the first that factored afresh
   std::vector<int> rowpos, colpos;  // reference equation k / unknown l (1-based) -> band row / column
   // per-step context shared by F_nonlinear and Jacobi
@@ -291,6 +301,7 @@ struct Solver {
     p = pp;
     ns = p.ns;
     x0 = p.x0; y0 = p.y0; phase = 0.0;
+    jofs.clear();
     if (ns < 8) { err = "the solid solver needs ns >= 8 (one-sided fourth-derivative stencils)"; return false; }
     if (!(p.TimeMethodSolid == kEulerImplicit || p.TimeMethodSolid == kCrankNicholson || p.TimeMethodSolid == kBDF2)) {
       err = "TimeMethodSolid must be 1 (implicit Euler), 2 (Crank-Nicolson) or 5 (BDF2)";
@@ -767,14 +778,22 @@ struct Solver {
     refactored = 0;
     for (;;) {
       band.reset(nx);
-      bool outside = false;
       residual(x.data(), [&](int k, double v) { rhs[rowpos[k]] = -1.0 * v; });
-      jacobian(x.data(), [&](int k, int l, double v) {
-        const int r = rowpos[k], c = colpos[l];
-        if (c < r - kKL || c > r + kKU) outside = true;
-        else band.at(r, c) = v;
-      });
-      if (outside) { err = "IBES: Jacobian entry outside the band"; return false; }
+      if (jofs.empty()) {
+        bool outside = false;
+        jacobian(x.data(), [&](int k, int l, double v) {
+          const int r = rowpos[k], c = colpos[l];
+          if (c < r - kKL || c > r + kKU) outside = true;
+          else { band.at(r, c) = v; jofs.push_back(Band::index(r, c)); }
+        });
+        if (outside) { jofs.clear(); err = "IBES: Jacobian entry outside the band"; return false; }
+        jlast = band.last;
+      } else {
+        double *a = band.a.data();
+        size_t e = 0;
+        jacobian(x.data(), [&](int, int, double v) { a[jofs[e++]] = v; });
+        band.last = jlast;
+      }
       bool fresh = iter == 0;
       if (!fresh) {
         dx = rhs;
