@@ -949,7 +949,7 @@ struct fsibe_sim {
   Box mbox, fbox;
   Window win[2];  // cal_drag's window per beam
   int nprim = 0, ntg = 0;
-  size_t off_prim = 0, off_tg = 0, off_nd = 0, off_cd = 0, off_val = 0;
+  size_t off_prim = 0, off_beam = 0, off_tg = 0, off_nd = 0, off_cd = 0, off_val = 0;
   int nprim_cap = 0;  // room for mask primitives in the staging buffer (and the mask kernel's LDS)
   ForceGeom fg;
   bool forces_ready = false;
@@ -1011,19 +1011,39 @@ static void prof_host(fsibe_sim *s, int k, double t0) {
   }
 }
 
-// stage bytes [off, off + bytes) of the host record into device memory, on
-// the step's stream; then ev_upload marks the host record free again
-static int upload_stage(fsibe_sim *s, size_t off, size_t bytes) {
-  const int n = (int)(bytes / sizeof(double2));
+// the host-written part of the geometry record (primitives, beam block)
+// into device memory, on the step's stream; then ev_upload marks the host
+// record free again
+static int upload_geometry(fsibe_sim *s) {
+  const int n = (int)(s->off_tg / sizeof(double2));
   if (n > 0) {
     const int blocks = std::min(64, (n + 255) / 256);
-    k_stage_copy<<<blocks, 256, 0, s->stream>>>(reinterpret_cast<const double2 *>(s->m_stage + off),
-                                              reinterpret_cast<double2 *>(s->d_stage + off), n);
+    k_stage_copy<<<blocks, 256, 0, s->stream>>>(reinterpret_cast<const double2 *>(s->m_stage),
+                                              reinterpret_cast<double2 *>(s->d_stage), n);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(s->ev_upload, s->stream));
   s->upload_pending = true;
   return FSIBE_OK;
+}
+
+// the force kernel's tables, built by the mask kernel beside its tiles
+static BeamGeo beam_geo(const fsibe_sim *s, int nthreads) {
+  BeamGeo g;
+  g.beam = reinterpret_cast<const double *>(s->d_stage + s->off_beam);
+  g.ns = s->ns; g.nb = s->nb;
+  g.pi = s->pi;
+  const bool bsi = s->visc && s->p.iCylinder == 0;
+  g.nc = bsi ? kNC : 0;
+  g.rr = s->p.t_beam + s->p.N_smooth * std::max(s->dy, s->dx);
+  g.r1 = s->p.t_beam - 0.0 * s->p.N_smooth * std::max(s->dy, s->dx);
+  g.dphi = s->pi / (double)(kNC - 1);
+  g.tg = reinterpret_cast<double *>(s->d_stage + s->off_tg);
+  g.nd = reinterpret_cast<double *>(s->d_stage + s->off_nd);
+  g.cd = reinterpret_cast<double *>(s->d_stage + s->off_cd);
+  const int items = s->nb * (s->ns + 2 * g.nc);
+  g.fwg = (s->forces_on && s->nb > 0) ? (items + nthreads - 1) / nthreads : 0;
+  return g;
 }
 
 static void prof_begin(fsibe_sim *s, int k) {
@@ -1360,7 +1380,11 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     const int nprim_max = 2 * nsb + 8;
     const int ntg_max = 6 * nsb + 6 * kNC * nb1;
     s->nprim_cap = nprim_max;
+    // host-written part: the primitives and the beam block X, Y, TH, alpha;
+    // device-only: the force kernel's tables (built by the mask kernel) and
+    // the interpolated values
     s->off_prim = al(sizeof(Prim) * nprim_max);
+    s->off_beam = al(sizeof(double) * (3 * (size_t)nsb + 2));
     s->off_tg = al(sizeof(double) * 3 * ntg_max);
     s->off_nd = al(sizeof(double) * 4 * nsb);
     s->off_cd = al(sizeof(double) * 4 * 2 * kNC * nb1);
@@ -1492,7 +1516,8 @@ static int build_mask_geometry(fsibe_sim *s) {
       P.ix1 = std::min(fnint((std::max(p1x, p2x) + t_star) / dx), nx - 1);
       P.iy0 = std::max(fnint((std::min(p1y, p2y) - t_star) / dy), 0);
       P.iy1 = std::min(fnint((std::max(p1y, p2y) + t_star) / dy), ny - 1);
-      const double ca = cos(al), sa = sin(al);
+      double ca, sa;
+      sincos(al, &sa, &ca);  // glibc's sincos returns sin and cos bit for bit
       seg_c[i] = ca;
       seg_s[i] = -sa;
       P.a[0] = ca * p1x - sa * p1y; P.a[1] = sa * p1x + ca * p1y;
@@ -1507,13 +1532,14 @@ static int build_mask_geometry(fsibe_sim *s) {
       P.type = PRIM_SHARP;
       const double al = TH[ns - 1] + s->alpha[ib];
       const double pex = X[ns - 1], pey = Y[ns - 1];
-      const double p1x = pex - 2.0 * mdx * cos(al), p1y = pey - 2.0 * mdx * sin(al);
-      const double p2x = pex + 4.0 * mdx * cos(al), p2y = pey + 4.0 * mdx * sin(al);
+      double ca, sa;
+      sincos(al, &sa, &ca);
+      const double p1x = pex - 2.0 * mdx * ca, p1y = pey - 2.0 * mdx * sa;
+      const double p2x = pex + 4.0 * mdx * ca, p2y = pey + 4.0 * mdx * sa;
       P.ix0 = std::max(fnint((std::min(p1x, p2x) - t_star) / dx), 0);
       P.ix1 = std::min(fnint((std::max(p1x, p2x) + t_star) / dx), nx - 1);
       P.iy0 = std::max(fnint((std::min(p1y, p2y) - t_star) / dy), 0);
       P.iy1 = std::min(fnint((std::max(p1y, p2y) + t_star) / dy), ny - 1);
-      const double ca = cos(al), sa = sin(al);
       P.a[0] = ca * p1x + sa * p1y; P.a[1] = -sa * p1x + ca * p1y;
       P.a[2] = ca * p2x + sa * p2y; P.a[3] = -sa * p2x + ca * p2y;
       P.a[4] = ca; P.a[5] = sa; P.a[6] = t_star; P.a[7] = VX[ns - 1]; P.a[8] = VY[ns - 1]; P.a[9] = mdx;
@@ -1541,6 +1567,13 @@ static int build_mask_geometry(fsibe_sim *s) {
     if (Y[0] < 0.0) return set_err(FSIBE_EDOMAIN, "Mask:: Beam Hinge. Index out of region (create_mask.f90:685-688)");
     dot(X[0], Y[0], VX[0], VY[0], t);
     if (s->p.iSharpTrailing == 0) dot(X[ns - 1], Y[ns - 1], VX[ns - 1], VY[ns - 1], t);
+  }
+  // the beam block the device reads: X, Y, TH of every beam, then alpha
+  {
+    double *bb = reinterpret_cast<double *>(s->h_stage + s->off_beam);
+    const int nsb = s->nb * ns;
+    for (int i = 0; i < nsb; ++i) { bb[i] = s->bx[i]; bb[nsb + i] = s->by[i]; bb[2 * nsb + i] = s->btheta[i]; }
+    for (int ib = 0; ib < s->nb; ++ib) bb[3 * nsb + ib] = s->alpha[ib];
   }
   if (np > s->nprim_cap) return set_err(FSIBE_ESIZE, "mask: %d primitives, room for %d", np, s->nprim_cap);
   s->nprim = np;
@@ -1596,6 +1629,49 @@ static int build_mask_geometry(fsibe_sim *s) {
 
 // interpolation targets (BeamForces.f90:150-165, :351-362, :389-400), node
 // and circle data of GetForces, and the pressure box they need
+// the interpolation targets of every beam as the force kernel forms them
+// (BeamForces.f90:150-165, :351-362, :389-400), here only to decide whether
+// one leaves the field and to bound their stencils; the host needs them only
+// where the bound from the nodes alone does not settle it (near the edges)
+static void exact_target_box(const fsibe_sim *s, double rr, double r1, bool bsi, double dxi, double dyi, Box &fb,
+                             bool &bad) {
+  const int nx = s->nx, ny = s->ny, ns = s->ns;
+  const double x2 = s->p.xl - s->dx, y2 = s->p.yl - s->dy;
+  auto target = [&](double xt, double yt) {
+    if (xt > x2 || xt < 0.0 || yt > y2 || yt < 0.0) bad = true;
+    const int i = (int)((xt - 0.0) / dxi), j = (int)((yt - 0.0) / dyi);
+    if (i - 4 < 0 || i + 4 > nx - 1 || j - 4 < 0 || j + 4 > ny - 1) bad = true;
+    fb.add(i - 4, i + 4, j - 4, j + 4);
+  };
+  for (int ib = 0; ib < s->nb; ++ib) {
+    const double *X = s->bx.data() + ib * ns, *Y = s->by.data() + ib * ns, *TH = s->btheta.data() + ib * ns;
+    const double alpha = s->alpha[ib];
+    for (int n = 0; n < ns; ++n) {
+      const double th = TH[n] + alpha;
+      const double st = sin(th), ct = cos(th);
+      target(X[n] - r1 * st, Y[n] + r1 * ct);
+      target(X[n] + r1 * st, Y[n] - r1 * ct);
+      target(X[n] - rr * st, Y[n] + rr * ct);
+      target(X[n] + rr * st, Y[n] - rr * ct);
+    }
+    if (bsi) {
+      const double dphi = s->pi / (double)(kNC - 1);
+      for (int i = 0; i < kNC; ++i) {
+        const double phi = -(0.5 * s->pi - (TH[ns - 1] + alpha)) + (double)i * dphi;
+        target(X[ns - 1] + rr * cos(phi), Y[ns - 1] + rr * sin(phi));
+        const double phl = (0.5 * s->pi + (TH[1] + alpha)) + (double)i * dphi;
+        target(X[0] + rr * cos(phl), Y[0] + rr * sin(phl));
+      }
+    }
+  }
+}
+
+// The force kernel builds the interpolation targets itself from the beam
+// block.  Every target lies within rr of a node (rr >= r1), so the nodes'
+// box grown by rr bounds them, and their 9 x 9 stencils (plus one cell for
+// the rounding of the device's target coordinates) bound the pressure box.
+// Where that bound reaches the field's edge the exact targets decide the
+// reference's error (Interpolation.f90:19-24).
 static int build_force_geometry(fsibe_sim *s, double time) {
   const int nx = s->nx, ny = s->ny, ns = s->ns;
   const double dx = s->dx, dy = s->dy;
@@ -1603,73 +1679,46 @@ static int build_force_geometry(fsibe_sim *s, double time) {
   s->ntg = 0;
   Box fb;
   if (s->forces_on) {
-    double *tg = reinterpret_cast<double *>(s->h_stage + s->off_tg);
-    double *nd = reinterpret_cast<double *>(s->h_stage + s->off_nd);
-    double *cd = reinterpret_cast<double *>(s->h_stage + s->off_cd);
     const double x2 = s->p.xl - dx, y2 = s->p.yl - dy;
     const double dxi = (x2 - 0.0) / (double)(nx - 1), dyi = (y2 - 0.0) / (double)(ny - 1);
-    int nt = 0;
-    bool bad = false;
-    auto target = [&](double xt, double yt, int fid) {
-      if (xt > x2 || xt < 0.0 || yt > y2 || yt < 0.0) bad = true;
-      const int i = (int)((xt - 0.0) / dxi), j = (int)((yt - 0.0) / dyi);
-      if (i - 4 < 0 || i + 4 > nx - 1 || j - 4 < 0 || j + 4 > ny - 1) bad = true;
-      fb.add(i - 4, i + 4, j - 4, j + 4);
-      tg[3 * nt] = xt; tg[3 * nt + 1] = yt; tg[3 * nt + 2] = (double)fid;
-      ++nt;
-    };
     const double rr = s->p.t_beam + s->p.N_smooth * std::max(dy, dx);
     const double r1 = s->p.t_beam - 0.0 * s->p.N_smooth * std::max(dy, dx);
     const bool bsi = s->visc && s->p.iCylinder == 0;
-    // GetForces loops over the beams (BeamForces.f90:24-47): beam after beam
-    for (int ib = 0; ib < s->nb; ++ib, nd += 4 * ns, cd += 8 * kNC) {
-      const double *X = s->bx.data() + ib * ns, *Y = s->by.data() + ib * ns, *TH = s->btheta.data() + ib * ns;
-      const double alpha = s->alpha[ib];
-      for (int n = 0; n < ns; ++n) {
-        const double th = TH[n] + alpha;
-        const double st = sin(th), ct = cos(th);
-        nd[4 * n] = st; nd[4 * n + 1] = ct; nd[4 * n + 2] = sin(2.0 * th); nd[4 * n + 3] = cos(2.0 * th);
-        const double xu = X[n] - rr * st, yu = Y[n] + rr * ct, xb = X[n] + rr * st, yb = Y[n] - rr * ct;
-        const double xu1 = X[n] - r1 * st, yu1 = Y[n] + r1 * ct, xb1 = X[n] + r1 * st, yb1 = Y[n] - r1 * ct;
-        target(xu1, yu1, 0);
-        target(xb1, yb1, 0);
-        target(xu, yu, 1);
-        target(xu, yu, 2);
-        target(xb, yb, 1);
-        target(xb, yb, 2);
-      }
-      if (bsi) {
-        const double dphi = s->pi / (double)(kNC - 1);
-        double beta = TH[ns - 1] + alpha;
-        for (int i = 0; i < kNC; ++i) {
-          const double phi = -(0.5 * s->pi - beta) + (double)i * dphi;
-          const double xc = X[ns - 1] + rr * cos(phi), yc = Y[ns - 1] + rr * sin(phi);
-          cd[4 * i] = cos(phi); cd[4 * i + 1] = sin(phi); cd[4 * i + 2] = cos(2.0 * phi); cd[4 * i + 3] = sin(2.0 * phi);
-          target(xc, yc, 1);
-          target(xc, yc, 2);
-          target(xc, yc, 0);
-        }
-        beta = TH[1] + alpha;  // BeamForces.f90:390 takes node 1
-        for (int i = 0; i < kNC; ++i) {
-          const double phi = (0.5 * s->pi + beta) + (double)i * dphi;
-          const double xc = X[0] + rr * cos(phi), yc = Y[0] + rr * sin(phi);
-          const int q = kNC + i;
-          cd[4 * q] = cos(phi); cd[4 * q + 1] = sin(phi); cd[4 * q + 2] = cos(2.0 * phi); cd[4 * q + 3] = sin(2.0 * phi);
-          target(xc, yc, 1);
-          target(xc, yc, 2);
-          target(xc, yc, 0);
-        }
-      }
+    const int nsb = s->nb * ns;
+    double xmin = s->bx[0], xmax = s->bx[0], ymin = s->by[0], ymax = s->by[0];
+    for (int i = 1; i < nsb; ++i) {
+      xmin = std::min(xmin, s->bx[i]); xmax = std::max(xmax, s->bx[i]);
+      ymin = std::min(ymin, s->by[i]); ymax = std::max(ymax, s->by[i]);
     }
-    if (bad) return set_err(FSIBE_EDOMAIN, "DeltaInterpolation: target coordinates not in the field "
-                                           "(Interpolation.f90:19-24) or stencil leaves the grid");
-    s->ntg = nt;
+    // a margin of a hundredth of a cell covers the rounding of the targets
+    const double mx = 0.01 * dxi, my = 0.01 * dyi;
+    const double xa = xmin - rr - mx, xb = xmax + rr + mx, ya = ymin - rr - my, yb = ymax + rr + my;
+    const bool inside = xa >= 0.0 && ya >= 0.0 && xb <= x2 && yb <= y2;
+    int i0 = 0, i1 = -1, j0 = 0, j1 = -1;
+    if (inside) {
+      i0 = (int)(xa / dxi); i1 = (int)(xb / dxi); j0 = (int)(ya / dyi); j1 = (int)(yb / dyi);
+    }
+    if (inside && i0 - 4 >= 0 && i1 + 4 <= nx - 1 && j0 - 4 >= 0 && j1 + 4 <= ny - 1) {
+      fb.add(i0 - 4, i1 + 4, j0 - 4, j1 + 4);
+    } else {
+      bool bad = false;
+      exact_target_box(s, rr, r1, bsi, dxi, dyi, fb, bad);
+      if (bad) return set_err(FSIBE_EDOMAIN, "DeltaInterpolation: target coordinates not in the field "
+                                             "(Interpolation.f90:19-24) or stencil leaves the grid");
+    }
+    // one more cell each side for the device's rounding, inside the grid
+    fb.x0 = std::max(fb.x0 - 1, 0); fb.x1 = std::min(fb.x1 + 1, nx - 1);
+    fb.y0 = std::max(fb.y0 - 1, 0); fb.y1 = std::min(fb.y1 + 1, ny - 1);
+    const int nc = bsi ? kNC : 0;
+    s->ntg = s->nb * (6 * ns + 6 * nc);
     ForceGeom &g = s->fg;
-    g.ns = ns; g.nc = bsi ? kNC : 0; g.nt = nt;
+    g.ns = ns; g.nc = nc; g.nt = s->ntg;
     g.bx0 = fb.x0; g.by0 = fb.y0; g.bw = fb.w(); g.bh = fb.h();
     g.dxi = dxi; g.dyi = dyi; g.pi = s->pi; g.ds = s->ds;
     g.dL = (s->pi / (double)(kNC - 1)) * rr;
     g.visc = s->visc; g.bsi_on = bsi;
+    g.nb = s->nb; g.nx = nx; g.ny = ny; g.pad = 0;
+    g.rr = rr; g.r1 = r1; g.dphi = s->pi / (double)(kNC - 1);
     // soft startup (BeamForces.f90:246-252)
     const double Tr = s->p.T_release, tau = tau_eff(s->p);
     double soft;
@@ -1748,8 +1797,7 @@ extern "C" int fsibe_create_mask_beams(fsibe_sim *s, double time, const double *
 
 static int force_geometry(fsibe_sim *s, double time) {
   const double th0 = now_ms();
-  CHK(build_force_geometry(s, time));
-  CHK(upload_stage(s, s->off_tg, s->off_val - s->off_tg));
+  CHK(build_force_geometry(s, time));  // host scalars and box only: the kernel builds its targets
   prof_host(s, P_REDUCE, th0);
   return FSIBE_OK;
 }
@@ -1781,7 +1829,7 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
     CHK(build_mask_geometry(s));
     s->forces_ready = false;
   }
-  CHK(upload_stage(s, 0, with_forces ? s->off_val : s->off_tg));
+  CHK(upload_geometry(s));
   prof_host(s, P_REDUCE, th0);
   if (!s->mbox.empty()) {
     MaskConsts mc;
@@ -1810,12 +1858,16 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
       mc.nx_frame = s->nx;
       mc.frame_M = std::max(1.0e-3, s->p.eps);
     }
+    const BeamGeo bg = beam_geo(s, kMaskThreads);
+    const int tiles_x = (s->mbox.w() + kMaskTile - 1) / kMaskTile, tiles_y = (s->mbox.h() + kMaskTile - 1) / kMaskTile;
+    // one more row of workgroups builds the force tables beside the tiles
     dim3 blk(kMaskThreads);
-    dim3 grd((s->mbox.w() + kMaskTile - 1) / kMaskTile, (s->mbox.h() + kMaskTile - 1) / kMaskTile);
+    dim3 grd(std::max(tiles_x, bg.fwg), tiles_y + (bg.fwg > 0 ? 1 : 0));
     prof_begin(s, P_MASK);
-    k_mask<<<grd, blk, sizeof(int4) * std::max(s->nprim, 1), s->stream>>>(reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim,
-                                       s->wallrow ? s->wallrow + 0 : nullptr, mc, s->mbox.x0, s->mbox.y0,
-                                       s->mbox.w(), s->mbox.h(), s->mask_m, s->mask_vx, s->mask_vy, s->mask_win);
+    k_mask<<<grd, blk, sizeof(int4) * std::max(s->nprim, 1), s->stream>>>(
+        reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim, s->wallrow ? s->wallrow + 0 : nullptr, mc,
+        bg, tiles_y, s->mbox.x0, s->mbox.y0, s->mbox.w(), s->mbox.h(), s->mask_m, s->mask_vx, s->mask_vy,
+        s->mask_win);
     HIPCHK(hipGetLastError());
     prof_end(s, P_MASK);
   }
@@ -2333,9 +2385,10 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     CHK(create_mask_impl(s, s->time, s->bx.data(), s->by.data(), s->bvx.data(), s->bvy.data(),
                          s->btheta.data(), alpha, LE, !split));
   } else if (p.iBeam > 0) {
-    CHK(build_geometry(s, s->time));  // soft-startup factor and targets follow time
+    // soft-startup factor follows time
     if (s->upload_pending) { HIPCHK(hipEventSynchronize(s->ev_upload)); s->upload_pending = false; }
-    CHK(upload_stage(s, 0, s->off_val));
+    CHK(build_geometry(s, s->time));
+    CHK(upload_geometry(s));
   }
   if (s->startup_done) s->dt0 = s->dt1;
   if (split) {
@@ -3054,7 +3107,9 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
     a.bx0 = s->mbox.x0; a.by0 = s->mbox.y0; a.bw = s->mbox.w(); a.bh = s->mbox.h();
     a.tiles_x = (a.bw + kMaskTile - 1) / kMaskTile;
     a.tiles = s->mbox.empty() ? 0 : a.tiles_x * ((a.bh + kMaskTile - 1) / kMaskTile);
-    maxblk[B_MASK] = std::max(maxblk[B_MASK], a.tiles);
+    a.bg = beam_geo(s, kMaskThreadsBatch);
+    if (s->mbox.empty()) a.bg.fwg = 0;
+    maxblk[B_MASK] = std::max(maxblk[B_MASK], a.tiles + a.bg.fwg);
   }
   {  // K1: A = Fy^-1[W0], B = Fy^-1[i ky W0]
     ColsInvArgs &a = reinterpret_cast<ColsInvArgs *>(A + g.off[B_K1])[j];
@@ -3115,9 +3170,9 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
     ForcesArgs &a = reinterpret_cast<ForcesArgs *>(A + g.off[B_K10])[j];
     const bool ok = fz && s->ntg > 0 && s->ntg == s->nb * (6 * s->fg.ns + 6 * s->fg.nc);
     a.tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
-    a.press = s->fb_p; a.sa = s->fb_sa; a.sb = s->fb_sb;
     a.nd = reinterpret_cast<const double *>(s->d_stage + s->off_nd);
     a.cd = reinterpret_cast<const double *>(s->d_stage + s->off_cd);
+    a.press = s->fb_p; a.sa = s->fb_sa; a.sb = s->fb_sb;
     a.val = reinterpret_cast<double *>(s->d_stage + s->off_val);
     a.pbeam = pbeam_dev(s, s->slot); a.tbeam = pbeam_dev(s, s->slot) + nodes_all(s);
     a.part = parts_dev(s, s->slot); a.g = s->fg;
@@ -3295,7 +3350,7 @@ static int group_prepare(fsibe_batch *b, int gi, std::array<int, B_NST> &need) {
   });
   if (bad.load() >= 0)
     return set_err(FSIBE_EDOMAIN, "batch member %d: %s", g.mem[bad.load()], errs[bad.load()].c_str());
-  int mx[B_NST] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int mx[B_NST] = {};
   for (size_t j = 0; j < g.mem.size(); ++j) fill_batch_args(b, g, (int)j, b->m[g.mem[j]], mx);
   for (int k = 0; k < B_NST; ++k) need[k] = mx[k];
   return FSIBE_OK;
@@ -3312,7 +3367,7 @@ static int group_gather(fsibe_batch *b, int gi) {
   for (int i : g.mem) {
     const fsibe_sim *s = b->m[i];
     segs.push_back(StageSeg{reinterpret_cast<const double2 *>(s->m_stage), reinterpret_cast<double2 *>(s->d_stage),
-                            (long long)(s->off_val / sizeof(double2))});
+                            (long long)(s->off_tg / sizeof(double2))});
   }
   for (size_t k0 = 0; k0 < segs.size(); k0 += kMaxStageSegs) {
     StageSegs ss{};

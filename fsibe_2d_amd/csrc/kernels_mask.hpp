@@ -137,6 +137,69 @@ __device__ __forceinline__ double prim_value(const Prim &P, const MaskConsts &c,
   return -2.0;
 }
 
+// ---- the force kernel's tables (on the device) ------------------------------
+// From the beam block X[nb ns], Y[nb ns], TH[nb ns], alpha[nb] (beam b's
+// nodes from b ns) that the host writes with the primitives, workgroups past
+// the mask's tiles build the force kernel's tables: per node
+// (GetForcesInterp BeamForces.f90:150-165) th = theta + alpha, nd = sin th,
+// cos th, sin 2th, cos 2th and the targets P(u1) P(b1) A(u) B(u) A(b) B(b)
+// at X -+ r sin th, Y +- r cos th (r1 for the pressure pair, rr for the
+// stresses); per circle point (BeamSurfaceIntegral :351-362, :389-400) phi,
+// cd = cos phi, sin phi, cos 2phi, sin 2phi and the targets A, B, P at the
+// end node + rr (cos phi, sin phi).  Table layout: beam b's targets from
+// b (6 ns + 6 nc) (node n: 6n .. 6n+5, circle point i: 6 ns + 3i + {0 A,
+// 1 B, 2 P}), nd[4 (b ns + n)], cd[4 (2 b nc + i)].
+struct BeamGeo {
+  const double *beam;  // the beam block in device memory
+  int ns, nb, nc;      // nodes per beam, beams, circle points per beam (0: none)
+  int fwg;             // workgroups past the tiles that build the force tables (0: no forces)
+  double pi, rr, r1, dphi;
+  double *tg, *nd, *cd;
+};
+
+// the force tables, one thread per node or circle point (item w)
+__device__ __forceinline__ void force_tables(const BeamGeo &g, int w) {
+#pragma clang fp contract(off)
+  const int ns = g.ns, nc = g.nc, per = ns + 2 * nc;
+  if (w >= g.nb * per) return;
+  const int b = w / per, k = w - b * per, nsb = g.nb * ns;
+  const double *X = g.beam + b * ns, *Y = g.beam + nsb + b * ns, *TH = g.beam + 2 * nsb + b * ns;
+  const double alpha = g.beam[3 * nsb + b];
+  double *tg = g.tg + (size_t)3 * b * (6 * ns + 6 * nc);
+  if (k < ns) {
+    const double th = TH[k] + alpha;
+    const double st = sin(th), ct = cos(th);
+    double *nd = g.nd + 4 * ((size_t)b * ns + k);
+    nd[0] = st; nd[1] = ct; nd[2] = sin(2.0 * th); nd[3] = cos(2.0 * th);
+    const double xu = X[k] - g.rr * st, yu = Y[k] + g.rr * ct, xb = X[k] + g.rr * st, yb = Y[k] - g.rr * ct;
+    const double xu1 = X[k] - g.r1 * st, yu1 = Y[k] + g.r1 * ct, xb1 = X[k] + g.r1 * st, yb1 = Y[k] - g.r1 * ct;
+    const double t6[18] = {xu1, yu1, 0.0, xb1, yb1, 0.0, xu, yu, 1.0, xu, yu, 2.0, xb, yb, 1.0, xb, yb, 2.0};
+    double *o = tg + 18 * k;
+#pragma unroll
+    for (int q = 0; q < 18; ++q) o[q] = t6[q];
+  } else {
+    const int i = k - ns;  // circle point: i < nc trailing end, else leading end
+    double phi, px, py;
+    if (i < nc) {
+      const double beta = TH[ns - 1] + alpha;
+      phi = -(0.5 * g.pi - beta) + (double)i * g.dphi;
+      px = X[ns - 1]; py = Y[ns - 1];
+    } else {
+      const double beta = TH[1] + alpha;  // BeamForces.f90:390 takes node 1
+      phi = (0.5 * g.pi + beta) + (double)(i - nc) * g.dphi;
+      px = X[0]; py = Y[0];
+    }
+    const double cp = cos(phi), sp = sin(phi);
+    double *cd = g.cd + 4 * ((size_t)2 * b * nc + i);
+    cd[0] = cp; cd[1] = sp; cd[2] = cos(2.0 * phi); cd[3] = sin(2.0 * phi);
+    const double xc = px + g.rr * cp, yc = py + g.rr * sp;
+    double *o = tg + 18 * ns + 9 * i;
+    o[0] = xc; o[1] = yc; o[2] = 1.0;
+    o[3] = xc; o[4] = yc; o[5] = 2.0;
+    o[6] = xc; o[7] = yc; o[8] = 0.0;
+  }
+}
+
 // owin (may be null): index of the primitive that set each pixel's value
 // (-1: none, the wall or zero), read by the adjoint of the mask
 template <int WAVES>
@@ -280,10 +343,18 @@ __device__ __forceinline__ void mask_body(const Prim *__restrict__ prims, int np
   if (owin) owin[q] = wi;
 }
 
+// grid: the tiles (tiles_y rows of gridDim.x), then one more row whose first
+// bg.fwg workgroups build the force tables
 __global__ void __launch_bounds__(kMaskThreads) k_mask(const Prim *__restrict__ prims, int np,
-                                              const double *__restrict__ wallrow, MaskConsts c, int bx0, int by0,
-                                              int bw, int bh, double *__restrict__ om, double *__restrict__ ovx,
+                                              const double *__restrict__ wallrow, MaskConsts c, BeamGeo bg,
+                                              int tiles_y, int bx0, int by0, int bw, int bh,
+                                              double *__restrict__ om, double *__restrict__ ovx,
                                               double *__restrict__ ovy, int *__restrict__ owin) {
+  if ((int)blockIdx.y >= tiles_y) {
+    if ((int)blockIdx.x < bg.fwg) force_tables(bg, blockIdx.x * kMaskThreads + threadIdx.x);
+    return;
+  }
+  if ((int)blockIdx.x * kMaskTile >= bw) return;  // the grid is as wide as the force row needs
   mask_body<kMaskWaves>(prims, np, wallrow, c, bx0, by0, bw, bh, om, ovx, ovy, owin, blockIdx.x, blockIdx.y);
 }
 
@@ -293,14 +364,20 @@ struct MaskArgs {
   const double *wallrow;
   double *om, *ovx, *ovy;
   MaskConsts c;
+  BeamGeo bg;
   int np, bx0, by0, bw, bh, tiles_x, tiles, pad;
 };
+// blocks past a member's tiles: its first bg.fwg build its force tables
 __global__ void __launch_bounds__(kMaskThreadsBatch) kb_mask(const MaskArgs *__restrict__ arr) {
   const MaskArgs &a = arr[blockIdx.y];
-  if ((int)blockIdx.x >= a.tiles) return;
+  if ((int)blockIdx.x >= a.tiles) {
+    const int w = (int)blockIdx.x - a.tiles;
+    if (w < a.bg.fwg) force_tables(a.bg, w * kMaskThreadsBatch + threadIdx.x);
+    return;
+  }
   const int tby = blockIdx.x / a.tiles_x, tbx = blockIdx.x - tby * a.tiles_x;
-  mask_body<kMaskWavesBatch>(a.prims, a.np, a.wallrow, a.c, a.bx0, a.by0, a.bw, a.bh, a.om, a.ovx, a.ovy, nullptr, tbx,
-                             tby);
+  mask_body<kMaskWavesBatch>(a.prims, a.np, a.wallrow, a.c, a.bx0, a.by0, a.bw, a.bh, a.om, a.ovx, a.ovy, nullptr,
+                             tbx, tby);
 }
 
 }  // namespace fsibe

@@ -714,6 +714,8 @@ struct ForceGeom {
   double dxi, dyi, pi;      // DeltaInterpolation spacing (Interpolation.f90:15-16)
   double ds, dL, soft;
   int visc, bsi_on;
+  int nb, nx, ny, pad;      // beams; grid (the stencil is kept inside it)
+  double rr, r1, dphi;      // target offsets t+N max(dx,dy) and t (BeamForces.f90:150-151), pi/(nc-1)
 };
 
 // phi_4star, Interpolation.f90:82-91
@@ -765,6 +767,12 @@ __device__ __forceinline__ void interp6(const double *__restrict__ tg, int t0, i
     f = fid == 0 ? press : fid == 1 ? sa : sb;
     i = (int)((xt - 0.0) / g.dxi);
     j = (int)((yt - 0.0) / g.dyi);
+    // the host's box holds every stencil with a cell to spare; the clamp
+    // keeps a stencil that the device's rounding moved by a cell inside it
+    // (phi4star vanishes beyond 2.5 cells, so a moved stencil adds the same
+    // nonzero terms in the same order)
+    i = min(max(i, g.bx0 + 4), g.bx0 + g.bw - 5);
+    j = min(max(j, g.by0 + 4), g.by0 + g.bh - 5);
     wxs[lt][a] = phi4star(fabs(((double)(i - 4 + a) * g.dxi + 0.0) - xt), g.dxi, g.pi);
     wys[lt][a] = phi4star(fabs(((double)(j - 4 + a) * g.dyi + 0.0) - yt), g.dyi, g.pi);
   }
