@@ -2062,9 +2062,11 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     const size_t lds = sizeof(double2) * 3 * padlen(s->H);
     prof_begin(s, P_ROWS_PRESS);
     CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
-      KLAUNCH(k_rows_press<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->T[2], s->u1, s->u2,
-              s->vor_mean, mb, s->win[0], s->win[1], rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal,
-              0.0, 0.0, 1,
+      // the step's instances: mode 1, with or without the pressure rhs
+      using FT = decltype(F);
+      auto *kern = s->pressure_on ? &k_rows_press<FT, 1, 1> : &k_rows_press<FT, 1, 0>;
+      KLAUNCH(kern, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->T[2], s->u1, s->u2, s->vor_mean, mb, s->win[0],
+              s->win[1], rows_dev(s, s->slot), s->umax_part, s->T[4], s->T[5], s->dscal, 0.0, 0.0, 1,
               (int)s->pressure_on, F, s->twx, s->sc);
       return FSIBE_OK;
     }));

@@ -469,7 +469,9 @@ constexpr int kNSUM = 15;
 constexpr int kNSUM1 = 11;  // the sums of a run with at most one beam
 struct Window { int x0, x1, y0, y1; };
 
-template <class FX>
+// CMODE, CPRES: mode and pres fixed at compile time (-1: the run-time
+// arguments decide), so that the step's instance carries no dead branches
+template <class FX, int CMODE = -1, int CPRES = -1>
 __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
                                                     const double2 *__restrict__ Cpsiy, double *__restrict__ u1,
                                                     double *__restrict__ u2, double *__restrict__ vor_mean,
@@ -480,6 +482,8 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
                                                     const double2 *__restrict__ twx, StepConsts c, int bid, int nblk) {
   extern __shared__ double2 lds[];
   __shared__ double red[kNSUM * 16];
+  if constexpr (CMODE >= 0) mode = CMODE;
+  if constexpr (CPRES >= 0) pres = CPRES;
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
   const int y = bid;
   double2 *f0 = lds, *f1 = lds + PL, *f2 = lds + 2 * PL;
@@ -936,7 +940,7 @@ __global__ void __launch_bounds__(FY::LB) k_cols_ab2(const double2 *__restrict__
                                                      const double2 *__restrict__ twy, StepConsts c) {
   cols_ab2_body(Rc, Rpx, W0, NL1, NL0, W1, s, explicit_step, F, twy, c, blockIdx.x, gridDim.x);
 }
-template <class FX>
+template <class FX, int CMODE = -1, int CPRES = -1>
 __global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
                                                        const double2 *__restrict__ Cpsiy, double *__restrict__ u1,
                                                        double *__restrict__ u2, double *__restrict__ vor_mean,
@@ -946,7 +950,8 @@ __global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict
                                                        double2 *__restrict__ Rqy, const DevScalars *__restrict__ s,
                                                        double UMx0, double UMy0, int mode, int pres, FX F,
                                                        const double2 *__restrict__ twx, StepConsts c) {
-  rows_press_body(Cw, Cpsi, Cpsiy, u1, u2, vor_mean, mb, win, win2, sums, umax_part, Rq, Rqy, s, UMx0, UMy0, mode, pres,
+  rows_press_body<FX, CMODE, CPRES>(Cw, Cpsi, Cpsiy, u1, u2, vor_mean, mb, win, win2, sums, umax_part, Rq, Rqy, s, UMx0,
+                                    UMy0, mode, pres,
                   F, twx, c, blockIdx.x, gridDim.x);
 }
 template <class FY>

@@ -250,12 +250,21 @@ struct Band {
         if (l != 0.0) b[i] = b[i] - l * b[j];
       }
     }
-    // back substitution; the terms of b[j+2..] are added first so that
-    // only the last one waits for b[j+1]
+    // back substitution; the terms of b[j+2..] go into four partial sums
+    // (the chain of one sum would wait on every subtraction), and only the
+    // last term waits for b[j+1]
     for (int j = n - 1; j >= 0; --j) {
       const double *rj = row(j);
-      double s = b[j];
-      for (int c = last[j]; c >= j + 2; --c) s = s - rj[c] * b[c];
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int c = j + 2;
+      for (; c + 3 <= last[j]; c += 4) {
+        s0 = s0 + rj[c] * b[c];
+        s1 = s1 + rj[c + 1] * b[c + 1];
+        s2 = s2 + rj[c + 2] * b[c + 2];
+        s3 = s3 + rj[c + 3] * b[c + 3];
+      }
+      for (; c <= last[j]; ++c) s0 = s0 + rj[c] * b[c];
+      double s = b[j] - ((s0 + s1) + (s2 + s3));
       if (last[j] >= j + 1) s = s - rj[j + 1] * b[j + 1];
       b[j] = s * inv[j];
     }
@@ -547,9 +556,12 @@ struct Solver {
   }
 
   // Jacobi (SolidSolver.f90:671-863): emit(k, l, dF_k/dx_l), the entry the
-  // reference stores as J(l, k)
+  // reference stores as J(l, k).  The Jacobian only steers Newton's
+  // iterates (the root is F's), so its divisions by the grid constants may
+  // become products with their reciprocals (a rounding-level change).
   template <class Emit>
   void jacobian(const double *x, Emit &&emit) {
+#pragma clang fp reciprocal(on)
     const double *th = x + 1;
     const double *T = x + ns + 3 + 1;
     const double eta = p.eta, mue = p.mue, sigma = p.sigma;
@@ -764,63 +776,77 @@ struct Solver {
     const std::vector<double> &th_old = th_old_, &thd_old = thd_old_;
     const double *thg = x.data() + 1;
     // Newton (SolidSolver.f90:229-316).  The first iteration factors its
-    // Jacobian.  Later iterations solve J_k dx = -F_k with that factorisation
-    // and one step of iterative refinement against the assembled J_k.  With
-    // rho = ||J_0^-1 (J_k - J_0)||, the correction c is about rho |dx| and
-    // the refined step differs from the exact solve by about rho^2 |dx|.
-    // Within one time step rho is 1e-8 .. 1e-5 (larger when the fluid loads
-    // move the iterate more), so a refined step is the exact Newton step to
-    // 1e-10 of its size, and the next iteration's Newton correction removes
-    // even that.  If |c| > 1e-4 |dx| (and c is not below the rounding of x
-    // itself, 1e-14 max|x|, as in a converged last iteration) the iteration
-    // factors J_k instead.
+    // Jacobian J_0.  A later iteration first takes the chord step
+    // dx = -J_0^-1 F_k with that factorisation.  It differs from the Newton
+    // step -J_k^-1 F_k by about rho |dx|, rho = ||J_0^-1 (J_k - J_0)||, and
+    // since F_k is what the previous step left, |dx| is about rho times the
+    // previous |dx|: a chord step at most 1e-4 of the previous one (or below
+    // the rounding of x itself, 1e-14 max|x|, as in a converged last
+    // iteration) bounds rho by about 1e-4 and is taken, within 1e-8 of the
+    // previous step of the Newton iterate, which the next iteration's
+    // correction removes.  Otherwise the iteration assembles J_k: it solves
+    // with J_0's factorisation and one step of iterative refinement against
+    // J_k (a correction c of about rho |dx|, leaving rho^2 |dx|), or, if
+    // |c| > 1e-4 |dx|, factors J_k instead.
     iter = 0;
     refactored = 0;
+    double dm_prev = 0.0;
     for (;;) {
-      band.reset(nx);
       residual(x.data(), [&](int k, double v) { rhs[rowpos[k]] = -1.0 * v; });
-      if (jofs.empty()) {
-        bool outside = false;
-        jacobian(x.data(), [&](int k, int l, double v) {
-          const int r = rowpos[k], c = colpos[l];
-          if (c < r - kKL || c > r + kKU) outside = true;
-          else { band.at(r, c) = v; jofs.push_back(Band::index(r, c)); }
-        });
-        if (outside) { jofs.clear(); err = "IBES: Jacobian entry outside the band"; return false; }
-        jlast = band.last;
-      } else {
-        double *a = band.a.data();
-        size_t e = 0;
-        jacobian(x.data(), [&](int, int, double v) { a[jofs[e++]] = v; });
-        band.last = jlast;
-      }
-      bool fresh = iter == 0;
-      if (!fresh) {
-        dx = rhs;
-        lu.solve_factored(dx.data());
-        band.matvec(dx.data(), tmp.data());
-        double cm = 0.0, dm = 0.0;
-        for (int r = 0; r < nx; ++r) tmp[r] = rhs[r] - tmp[r];
-        lu.solve_factored(tmp.data());
-        for (int r = 0; r < nx; ++r) {
-          cm = std::max(cm, std::fabs(tmp[r]));
-          dm = std::max(dm, std::fabs(dx[r]));
-        }
-        double xm = 0.0;
+      bool have = false;
+      double xm = 0.0;
+      if (iter > 0) {
         for (int l = 0; l < nx; ++l) xm = std::max(xm, std::fabs(x[l]));
-        if (cm <= 1e-4 * dm || cm <= 1e-14 * xm) {
-          for (int r = 0; r < nx; ++r) dx[r] = dx[r] + tmp[r];
-        } else {
-          fresh = true;
-          ++refactored;
-        }
-      }
-      if (fresh) {
-        std::swap(lu, band);
-        if (!lu.factor()) { err = "IBES: singular Newton matrix"; return false; }
         dx = rhs;
         lu.solve_factored(dx.data());
+        double dm = 0.0;
+        for (int r = 0; r < nx; ++r) dm = std::max(dm, std::fabs(dx[r]));
+        have = dm <= 1e-4 * dm_prev || dm <= 1e-14 * xm;
       }
+      if (!have) {
+        band.reset(nx);
+        if (jofs.empty()) {
+          bool outside = false;
+          jacobian(x.data(), [&](int k, int l, double v) {
+            const int r = rowpos[k], c = colpos[l];
+            if (c < r - kKL || c > r + kKU) outside = true;
+            else { band.at(r, c) = v; jofs.push_back(Band::index(r, c)); }
+          });
+          if (outside) { jofs.clear(); err = "IBES: Jacobian entry outside the band"; return false; }
+          jlast = band.last;
+        } else {
+          double *a = band.a.data();
+          size_t e = 0;
+          jacobian(x.data(), [&](int, int, double v) { a[jofs[e++]] = v; });
+          band.last = jlast;
+        }
+        bool fresh = iter == 0;
+        if (!fresh) {
+          // dx holds the chord step J_0^-1 (-F_k)
+          band.matvec(dx.data(), tmp.data());
+          double cm = 0.0, dm = 0.0;
+          for (int r = 0; r < nx; ++r) tmp[r] = rhs[r] - tmp[r];
+          lu.solve_factored(tmp.data());
+          for (int r = 0; r < nx; ++r) {
+            cm = std::max(cm, std::fabs(tmp[r]));
+            dm = std::max(dm, std::fabs(dx[r]));
+          }
+          if (cm <= 1e-4 * dm || cm <= 1e-14 * xm) {
+            for (int r = 0; r < nx; ++r) dx[r] = dx[r] + tmp[r];
+          } else {
+            fresh = true;
+            ++refactored;
+          }
+        }
+        if (fresh) {
+          std::swap(lu, band);
+          if (!lu.factor()) { err = "IBES: singular Newton matrix"; return false; }
+          dx = rhs;
+          lu.solve_factored(dx.data());
+        }
+      }
+      dm_prev = 0.0;
+      for (int r = 0; r < nx; ++r) dm_prev = std::max(dm_prev, std::fabs(dx[r]));
       ++iter;
       // dx is in band order; the update and the norms run in the reference's order
       double s2 = 0.0, x2 = 0.0;
