@@ -923,13 +923,12 @@ struct fsibe_sim {
   // the next step's first column pass (it reads only vortk) already runs:
   // launched for time level k1_level before the host solves the beam
   bool k1_ready = false;
-  // K2's rows outside k2_b0 .. k2_b1 launched ahead with K1 (iFLUSI=1 loop)
+  // K2's convective part of every row launched ahead with K1 (iFLUSI=1 loop)
   bool k2_ready = false;
   // launch choices of the column stages K1, K4, K5, K7 (the autotuner's,
   // once fsibe_run has measured them for this grid; else the static ones)
   ColSel csel[4];
   bool csel_ready = false;
-  int k2_b0 = 0, k2_b1 = -1;
   int k1_level = -1;
   // what each slot holds: the fluid step's row sums, the force terms
   bool slot_fluid[2] = {false, false}, slot_forces[2] = {false, false};
@@ -1937,19 +1936,15 @@ static int launch_k1(fsibe_sim *s, int level) {
   return FSIBE_OK;
 }
 
-// K2 of the next step on the rows that no mask can reach: outside the rows
-// of this step's mask box widened by kK2Margin rows on each side (the beam
-// moves far less than a cell per step; a mask box that leaves the band only
-// makes evolve_fluid recompute more rows).  These rows read K1's output, the
-// velocity and the sponge, none of which the beam solve changes, and skip
-// the penalisation term (empty mask box); the K3 scalars wait for the rest.
-constexpr int kK2Margin = 4;
+// K2 of the next step without the penalisation term, on every row: it reads
+// K1's output, the velocity and the sponge, none of which the beam solve
+// changes (empty mask box: Rc = Fx[r], Rpx = 0, row sums 0).  evolve_fluid
+// then adds the penalisation on the rows of the new mask box (k_rows_pen,
+// which also computes the K3 scalars).
 static int launch_k2_ahead(fsibe_sim *s) {
   s->k2_ready = false;
-  if (s->mbox.empty()) return FSIBE_OK;
-  const int b0 = std::max(0, s->mbox.y0 - kK2Margin), b1 = std::min(s->ny - 1, s->mbox.y1 + kK2Margin);
-  const int na = b0, nb = s->ny - 1 - b1;
-  if (na + nb <= 0) return FSIBE_OK;
+  if (s->mbox.empty() || !s->penal_on) return FSIBE_OK;
+  const int na = s->ny, nb = 0, b1 = s->ny - 1;
   MaskBox none;
   none.x0 = 0; none.x1 = -1; none.y0 = 0; none.y1 = -1; none.w = 0;
   none.m = none.vx = none.vy = nullptr;
@@ -1962,8 +1957,6 @@ static int launch_k2_ahead(fsibe_sim *s) {
             s->penal_on ? s->rowpen : nullptr);
     return FSIBE_OK;
   }));
-  s->k2_b0 = b0;
-  s->k2_b1 = b1;
   s->k2_ready = true;
   return FSIBE_OK;
 }
@@ -1992,26 +1985,29 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
   }
   MaskBox mb = mask_box(s);
   // K2 (block 0 also computes the K3 scalars from the previous u_max partials).
-  // When the rows outside a band around the previous mask box went ahead,
-  // the band and the rows of this mask box remain.
+  // When the convective part of every row went ahead, the penalisation of
+  // this mask box's rows remains.
   {
-    int ra = 0, na = s->ny;
-    if (k2_ahead) {
-      int lo = s->k2_b0, hi = s->k2_b1;
-      if (mb.x1 >= mb.x0) { lo = std::min(lo, mb.y0); hi = std::max(hi, mb.y1); }
-      ra = lo;
-      na = hi - lo + 1;
-    }
     const size_t lds = sizeof(double2) * 3 * padlen(s->H);
     prof_begin(s, P_ROWS_NL);
-    CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
-      KLAUNCH(k_rows_nl<decltype(F)>, na, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
-              s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
-              F, s->twx, s->sc,
-              s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), ra, na,
-              0, 1, s->penal_on ? s->rowpen : nullptr);
-      return FSIBE_OK;
-    }));
+    if (k2_ahead) {
+      const int y0 = mb.x1 >= mb.x0 ? mb.y0 : 0, nrow = mb.x1 >= mb.x0 ? mb.y1 - mb.y0 + 1 : 0;
+      CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+        KLAUNCH(k_rows_pen<decltype(F)>, std::max(nrow, 1), nt, lds, s->stream, s->u1, s->u2, mb, s->T[2], s->T[3],
+                F, s->twx, s->sc, y0, nrow, s->rowpen, s->umax_part, s->ny, s->dscal, s->fp, time, dt0,
+                (int)explicit_step, res_dev(s, s->slot));
+        return FSIBE_OK;
+      }));
+    } else {
+      CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+        KLAUNCH(k_rows_nl<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
+                s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
+                F, s->twx, s->sc,
+                s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), 0, s->ny,
+                0, 1, s->penal_on ? s->rowpen : nullptr);
+        return FSIBE_OK;
+      }));
+    }
     prof_end(s, P_ROWS_NL);
     if (s->pipe) HIPCHK(hipEventRecord(s->ev_dt, s->stream));
   }

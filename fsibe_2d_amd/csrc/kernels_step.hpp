@@ -393,6 +393,69 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
   }
 }
 
+// K2, penalisation part of a row whose convective part went ahead: the
+// ahead pass (rows_nl_body with an empty mask box) left Rc = Fx[r] (with
+// the zero spectra of a row without mask, whose arithmetic leaves it
+// unchanged) and Rpx = 0 on every row; on the mask rows this adds px, py =
+// -chi (u - u_s) (cal_nlk NavierStokes.f90:88-96), their transforms and the
+// row sums of the penalisation term, with the arithmetic of rows_nl_body's
+// mask rows: Rc = Fx[r] - i kx Fx[py] (as (re - kx im, im + kx re)),
+// Rpx = Fx[px].  Launched with rows_nl_body's thread count, so the row
+// sums add in the same order.
+template <class FX>
+__device__ __forceinline__ void rows_pen_body(const double *__restrict__ u1, const double *__restrict__ u2, MaskBox mb,
+                                              double2 *__restrict__ Rc, double2 *__restrict__ Rpx, FX F,
+                                              const double2 *__restrict__ twx, StepConsts c, int y,
+                                              double *__restrict__ rowpen) {
+  extern __shared__ double2 lds[];
+  const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
+  double2 *f0 = lds, *f1 = lds + PL;
+  double ps1 = 0.0, ps2 = 0.0;
+  for (int n = tid; n < H; n += nt) {
+    const int q = pidx(n);
+    const double2 a1 = reinterpret_cast<const double2 *>(u1 + (size_t)y * nx)[n];
+    const double2 a2 = reinterpret_cast<const double2 *>(u2 + (size_t)y * nx)[n];
+    double2 px = make_double2(0.0, 0.0), py = make_double2(0.0, 0.0);
+    const int x = 2 * n;
+    if (mb.inside(x, y)) {
+      const size_t qq = (size_t)(y - mb.y0) * mb.w + (x - mb.x0);
+      px.x = -(mb.m[qq] * (a1.x - mb.vx[qq]));
+      py.x = -(mb.m[qq] * (a2.x - mb.vy[qq]));
+    }
+    if (mb.inside(x + 1, y)) {
+      const size_t qq = (size_t)(y - mb.y0) * mb.w + (x + 1 - mb.x0);
+      px.y = -(mb.m[qq] * (a1.y - mb.vx[qq]));
+      py.y = -(mb.m[qq] * (a2.y - mb.vy[qq]));
+    }
+    ps1 += px.x + px.y;
+    ps2 += py.x + py.y;
+    f0[q] = px;
+    f1[q] = py;
+  }
+  if (rowpen) {  // f_mean of cal_nlk, NavierStokes.f90:97-100: per-row sums
+    __shared__ double rp[32];
+    ps1 = wave_sum(ps1);
+    ps2 = wave_sum(ps2);
+    if ((tid & 63) == 0) { rp[2 * (tid >> 6)] = ps1; rp[2 * (tid >> 6) + 1] = ps2; }
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0.0, b = 0.0;
+      for (int w = 0; w < nt / 64; ++w) { a += rp[2 * w]; b += rp[2 * w + 1]; }
+      rowpen[2 * y] = a;
+      rowpen[2 * y + 1] = b;
+    }
+  }
+  __syncthreads();
+  F.template fft<false, 2>(lds, twx, tid, nt);
+  rows_r2c_store<2>(lds, H, PL, twx, tid, nt, [&](int k, const double2 *x) {
+    const double2 xpx = x[0], xpy = x[1];
+    const double kf = (double)k * c.sx;
+    const double2 xa = Rc[(size_t)y * H + k];
+    Rc[(size_t)y * H + k] = make_double2(xa.x - kf * xpy.y, xa.y + kf * xpy.x);
+    Rpx[(size_t)y * H + k] = xpx;
+  });
+}
+
 // ============================================================================
 // K4: nlk(n0) and the integrating-factor AB2 / Euler update of vortk
 // (EvolveFluidAB2 NavierStokes.f90:478-498, EvolveFluidExplicit :444-449)
@@ -893,6 +956,24 @@ __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ 
   const int b = blockIdx.x;
   rows_nl_body(A, B, u1, u2, ms, mb, penal_on, Rc, Rpx, F, twx, c, umax_part, nparts, sc, fl, time, dt0,
                explicit_step, res, b < na ? ra + b : rb + (b - na), scal && b == 0, rowpen);
+}
+
+// K2's penalisation rows y0 .. y0 + nrow - 1 after the ahead pass; block 0
+// also computes the K3 scalars (blocks past the rows do only that)
+template <class FX>
+__global__ void __launch_bounds__(FX::LB) k_rows_pen(const double *__restrict__ u1, const double *__restrict__ u2,
+                                                     MaskBox mb, double2 *__restrict__ Rc, double2 *__restrict__ Rpx,
+                                                     FX F, const double2 *__restrict__ twx, StepConsts c, int y0,
+                                                     int nrow, double *__restrict__ rowpen,
+                                                     const double *__restrict__ umax_part, int nparts,
+                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
+                                                     double dt0, int explicit_step, StepOutDev *__restrict__ res) {
+  if (blockIdx.x == 0) {
+    __shared__ double red[16];
+    step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
+  }
+  if ((int)blockIdx.x >= nrow) return;
+  rows_pen_body(u1, u2, mb, Rc, Rpx, F, twx, c, y0 + blockIdx.x, rowpen);
 }
 
 // iMeanVelocity = 11 (zero mean pressure): f_mean = sum(penal) / (nx ny)
