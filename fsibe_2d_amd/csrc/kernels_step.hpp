@@ -143,6 +143,32 @@ __device__ void block_sum(double (&v)[NV], double *red) {
 // derived fields.  Grid: ntiles * nfield blocks, one (tile, field) each.
 // field 0: F, 1: psi = F/|k|^2, 2: i ky psi, 3: -ky^2 psi, 4: i ky F
 // ============================================================================
+// the derived field fsel of spectral value w at (row idx, column kx)
+__device__ __forceinline__ double2 inv_field_value(int fsel, double2 w, int idx, int kx, int ny, const StepConsts &c) {
+  double2 v;
+  if (fsel == 0) {
+    v = w;
+  } else if (fsel == 4) {
+    const double kyf = (double)ky_signed(idx, ny) * c.sy;
+    v = make_double2(-kyf * w.y, kyf * w.x);
+  } else {
+    const int ky = ky_signed(idx, ny);
+    const int aky = ky < 0 ? -ky : ky;
+    double2 psi;
+    if ((kx == 0 && ky == 0) || idx == (ny >> 1)) {
+      psi = make_double2(0.0, 0.0);
+    } else {
+      const double quot = ((double)(kx * kx) * c.scalex + (double)(aky * aky) * c.scaley);
+      psi = make_double2(w.x / quot, w.y / quot);
+    }
+    const double kyf = (double)ky * c.sy;
+    if (fsel == 1) v = psi;
+    else if (fsel == 2) v = make_double2(-kyf * psi.y, kyf * psi.x);
+    else v = make_double2(-(kyf * kyf) * psi.x, -(kyf * kyf) * psi.y);
+  }
+  return v;
+}
+
 template <class FY>
 __device__ __forceinline__ void cols_inv_fields_body(const double2 *__restrict__ W, double2 *__restrict__ o0,
                                                             double2 *__restrict__ o1, double2 *__restrict__ o2,
@@ -156,32 +182,9 @@ __device__ __forceinline__ void cols_inv_fields_body(const double2 *__restrict__
   const int kx0 = tile * TX;
   const int fsel = slot == 0 ? fsel0 : slot == 1 ? fsel1 : slot == 2 ? fsel2 : fsel3;
   double2 *out = slot == 0 ? o0 : slot == 1 ? o1 : slot == 2 ? o2 : o3;
-  const int hy = ny >> 1;
   for (int e = tid; e < ny * TX; e += nt) {
     const int idx = e / TX, cc = e - idx * TX, kx = kx0 + cc;
-    const double2 w = W[(size_t)idx * H + kx];
-    double2 v;
-    if (fsel == 0) {
-      v = w;
-    } else if (fsel == 4) {
-      const double kyf = (double)ky_signed(idx, ny) * c.sy;
-      v = make_double2(-kyf * w.y, kyf * w.x);
-    } else {
-      const int ky = ky_signed(idx, ny);
-      const int aky = ky < 0 ? -ky : ky;
-      double2 psi;
-      if ((kx == 0 && ky == 0) || idx == hy) {
-        psi = make_double2(0.0, 0.0);
-      } else {
-        const double quot = ((double)(kx * kx) * c.scalex + (double)(aky * aky) * c.scaley);
-        psi = make_double2(w.x / quot, w.y / quot);
-      }
-      const double kyf = (double)ky * c.sy;
-      if (fsel == 1) v = psi;
-      else if (fsel == 2) v = make_double2(-kyf * psi.y, kyf * psi.x);
-      else v = make_double2(-(kyf * kyf) * psi.x, -(kyf * kyf) * psi.y);
-    }
-    lds[pidx(idx) * TX + cc] = v;
+    lds[pidx(idx) * TX + cc] = inv_field_value(fsel, W[(size_t)idx * H + kx], idx, kx, ny, c);
   }
   __syncthreads();
   F.template fft<true, 1, 1>(lds, twy, tid, nt);
