@@ -422,13 +422,15 @@ def batch_bench(nsims: int, steps: int, warmup: int = 3):
                                      for k, v in g["column_stages"].items()} for g in desc["groups"]}}
 
 
-def adjoint_bench(steps: int = 1000, chk: int = 50, smooth: bool = True):
+def adjoint_bench(steps: int = 1000, chk: int = 50, smooth: bool = True, bchk: int = 10, bslots: int = 10):
     """the differentiation workload (adjoint_512: 512^2, Re 500, dt 1e-4,
     motion 601 with coefficients 0.05 N(0,1) of seed 3): wall time of J
     alone and of J + dJ/dc (forward with checkpoints every chk steps,
     recompute of each segment, reverse sweep), each gradient component
     against the central difference of J with h = 1e-6 and 1e-7, and the
-    gradient along a random unit direction against the same (h = 1e-6)"""
+    gradient along a random unit direction against the same (h = 1e-6).
+    With the continuous mask also the bounded-memory form: units of bchk
+    steps, bslots snapshot slots, the binomial schedule."""
     import numpy as np
 
     from fsibe_2d_amd.adjoint import Adjoint
@@ -468,6 +470,30 @@ def adjoint_bench(steps: int = 1000, chk: int = 50, smooth: bool = True):
         comp[f"{hh:g}"] = float(np.max(np.abs(fdk - g) / np.abs(fdk)))
     a.close()
     f, gr = min(t_f), min(t_g)
+    bino = None
+    if smooth and bchk > 0:
+        # the same gradient with bounded memory: units of bchk steps and
+        # bslots snapshot slots, reversed by the binomial (revolve) schedule
+        from fsibe_2d_amd.adjoint import plan_cost
+
+        b = Adjoint(p, vk, steps, bchk, snapshots=bslots)
+        b.gradient(c)
+        t_b = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            Jb, gb = b.gradient(c)
+            t_b.append(time.perf_counter() - t0)
+        st = b.stats()
+        b.close()
+        nu, nu_d = -(-steps // bchk), -(-steps // chk)
+        mib = lambda n_f: round(n_f * p.nx * p.ny * 8 / 2**20, 1)
+        bino = {"chk": bchk, "snapshot_slots": st["slots"], "units": nu,
+                "units_advanced": plan_cost(nu, bslots), "steps_advanced": st["steps_advanced"],
+                "steps_recorded": st["steps_recorded"], "gradient_ms": round(1e3 * min(t_b), 3),
+                "cost_ratio": round(min(t_b) / f, 3),
+                "identical_to_default": bool(Jb == Jg and np.array_equal(gb, g)),
+                "memory_mib": {"snapshots": mib(7 * st["slots"]), "tape": mib(7.5 * bchk)},
+                "default_memory_mib": {"snapshots": mib(7 * (nu_d - 1)), "tape": mib(7.5 * chk)}}
     return {"workload": f"{p.nx}^2, Re=500, dt=1e-4, {steps} steps, motion 601 (8 coefficients, "
                         f"0.05 N(0,1) seed 3), J = (1/T) sum(-Force(1) dt), checkpoints every {chk} steps, "
                         + ("mask cut-offs and cal_drag window moved out (iSmoothCutoff=1)" if smooth
@@ -477,7 +503,8 @@ def adjoint_bench(steps: int = 1000, chk: int = 50, smooth: bool = True):
             "cost_ratio": round(gr / f, 3), "reverse_ms": round(tm["reverse_ms"], 3),
             "fd_max_rel_err_per_component": comp,
             "directional_fd": {"h": h, "adjoint": float(g @ d), "fd": fd,
-                               "rel_err": abs(float(g @ d) - fd) / abs(fd)}}
+                               "rel_err": abs(float(g @ d) - fd) / abs(fd)},
+            "binomial": bino}
 
 
 def main(argv=None) -> int:

@@ -92,7 +92,8 @@ EXPORTS = (
     "fsibe_adjoint_timings", "fsibe_backup_write", "fsibe_backup_read", "fsibe_inicond_write",
     "fsibe_inicond_read", "fsibe_pressure_snapshot", "fsibe_save_field", "fsibe_save_gif",
     "fsibe_create_mask_beams", "fsibe_get_step_out_beam", "fsibe_beam_placement", "fsibe_integrate_position_beam",
-    "fsibe_solid_create_beam", "fsibe_sim_solid_beam",
+    "fsibe_solid_create_beam", "fsibe_sim_solid_beam", "fsibe_adjoint_create_binomial", "fsibe_adjoint_stats",
+    "fsibe_adjoint_plan", "fsibe_adjoint_plan_cost",
 )
 
 _lib = None
@@ -177,6 +178,12 @@ def lib():
     L.fsibe_adjoint_objective.argtypes = [vp, dp, C.POINTER(C.c_double)]
     L.fsibe_adjoint_gradient.argtypes = [vp, dp, C.POINTER(C.c_double), dp]
     L.fsibe_adjoint_timings.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.fsibe_adjoint_create_binomial.argtypes = [C.POINTER(FsibeParams), vp, C.c_int, C.c_int, C.c_int, pvp]
+    L.fsibe_adjoint_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.fsibe_adjoint_plan.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_longlong)]
+    L.fsibe_adjoint_plan_cost.argtypes = [C.c_int, C.c_int]
+    L.fsibe_adjoint_plan_cost.restype = C.c_longlong
     L.fsibe_backup_write.argtypes = [vp, C.c_char_p, C.c_int, C.c_double, C.c_int]
     L.fsibe_backup_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                     C.POINTER(C.c_int)]

@@ -6,11 +6,16 @@
 // Force(1) the cal_drag force
 // (NavierStokes.f90:567-575); parameters: the 8 coefficients of motion 601
 // (two harmonics of heave and of pitch).  Gradient by a reverse sweep over the
-// adjoint kernels of kernels_adjoint.hpp with two-level checkpointing: the
-// forward run stores the simulation's state every `chk` steps; the reverse
-// sweep restores each checkpoint, recomputes its `chk` steps recording what
-// the adjoint needs (the tape), and runs the adjoint steps of that segment
-// backwards.  Memory: nsteps/chk checkpoints plus one tape of chk steps.
+// adjoint kernels of kernels_adjoint.hpp.  The steps are cut into units
+// (segments) of `chk` steps.  A unit is reversed by recomputing its steps
+// from the state at its start while recording what the adjoint needs (the
+// tape), then running its adjoint steps backwards.  The states at unit starts
+// come from snapshot slots placed by a binomial checkpointing schedule
+// (rev_plan): with a slot for every unit start, that is one forward sweep
+// storing each unit's start; with fewer slots, the sweep advances again from
+// the nearest stored state, and the number of recomputed units is the least
+// any schedule with that many slots can take.  Memory: the slots plus one
+// tape of chk steps.
 
 static void adj_copy_d(hipStream_t st, const double *src, double *dst, size_t n) {
   if (n == 0) return;
@@ -141,9 +146,124 @@ static int snap_restore(fsibe_sim *s, SimSnap &q) {
   return FSIBE_OK;
 }
 
+// ---- binomial checkpointing ------------------------------------------------
+// Reversing l units with the state of unit 0 always at hand and s snapshot
+// slots: the least number of units advanced (run forward without a tape,
+// the first sweep included) is
+//     t(l, s) = r l - B(s + 2, r - 1),   B(c, r) = (c + r)! / (c! r!),
+// with r the least integer such that B(s + 1, r) >= l (Griewank, "Achieving
+// logarithmic growth of temporal and spatial complexity in reverse
+// automatic differentiation", Optim. Methods Softw. 1 (1992); Griewank and
+// Walther, ACM TOMS 26 (2000), Algorithm 799 "revolve": s + 1 checkpoints
+// there, counting the one of the initial state).  It satisfies the
+// recursion t(l, 0) = l (l - 1) / 2, t(1, s) = 0,
+//     t(l, s) = min over 1 <= k < l of  k + t(l - k, s - 1) + t(k, s)
+// (advance k units, store the state there, reverse the l - k units above it
+// with one slot fewer, then the k units below with the slot free again),
+// and it is convex in l, so the objective of that minimum is convex in k
+// and its least minimiser is found by bisection on its forward difference.
+
+// B(c, r) = C(c + r, r), saturated at 2^62
+static long long rev_binom(int c, long long r) {
+  const long long cap = 1ll << 62;
+  __int128 b = 1;
+  for (long long i = 1; i <= r; ++i) {
+    b = b * (c + i) / i;
+    if (b > cap) return cap;
+  }
+  return (long long)b;
+}
+
+static long long rev_cost(long long l, int s) {
+  if (l <= 1) return 0;
+  if (s <= 0) return l * (l - 1) / 2;
+  long long r = 0;
+  while (rev_binom(s + 1, r) < l) ++r;
+  return r * l - rev_binom(s + 2, r - 1);
+}
+
+// the least k in [1, l - 1] minimising k + t(l - k, s - 1) + t(k, s)
+static long long rev_split(long long l, int s) {
+  auto f = [&](long long k) { return k + rev_cost(l - k, s - 1) + rev_cost(k, s); };
+  long long lo = 1, hi = l - 1;  // the answer is the first k with f(k+1) - f(k) >= 0, or l - 1
+  while (lo < hi) {
+    const long long mid = lo + (hi - lo) / 2;
+    if (f(mid + 1) - f(mid) >= 0) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// one action of the schedule: restore the state of unit `unit` from slot
+// `arg` (-1: the initial state), advance `arg` units from unit `unit`, store
+// the state of unit `unit` in slot `arg`, or record and reverse unit `unit`
+enum RevOp { REV_RESTORE = 0, REV_ADVANCE = 1, REV_SAVE = 2, REV_REVERSE = 3 };
+struct RevAct {
+  int op, unit, arg;
+};
+
+// reverse units [a, b), the state of unit a being in slot sa, with s free
+// slots (ids taken from `free_ids`)
+static void rev_plan_rec(long long a, long long b, int s, int sa, std::vector<int> &free_ids,
+                         std::vector<RevAct> &out) {
+  while (b - a > 1 && s > 0) {
+    const long long k = rev_split(b - a, s);
+    const int q = free_ids.back();
+    free_ids.pop_back();
+    out.push_back({REV_RESTORE, (int)a, sa});
+    out.push_back({REV_ADVANCE, (int)a, (int)k});
+    out.push_back({REV_SAVE, (int)(a + k), q});
+    rev_plan_rec(a + k, b, s - 1, q, free_ids, out);
+    free_ids.push_back(q);
+    b = a + k;
+  }
+  for (long long i = b - 1; i >= a; --i) {  // no slot left: advance from a for each unit
+    out.push_back({REV_RESTORE, (int)a, sa});
+    if (i > a) out.push_back({REV_ADVANCE, (int)a, (int)(i - a)});
+    out.push_back({REV_REVERSE, (int)i, 0});
+  }
+}
+
+static std::vector<RevAct> rev_plan(int nunits, int nslots) {
+  std::vector<RevAct> out;
+  std::vector<int> free_ids;
+  for (int q = nslots - 1; q >= 0; --q) free_ids.push_back(q);
+  if (nunits > 0) rev_plan_rec(0, nunits, nslots, -1, free_ids, out);
+  return out;
+}
+
+// The schedule as integer triples (op, unit, arg) for a host to inspect:
+// ops[3*i .. 3*i+2] of action i, *n_acts actions (FSIBE_ESIZE with the count
+// when cap is short), *advances the number of units it advances.
+extern "C" int fsibe_adjoint_plan(int nunits, int nslots, int *ops, int cap, int *n_acts, long long *advances) {
+  if (nunits < 1 || nslots < 0 || nunits > (1 << 24)) return set_err(FSIBE_EARG, "adjoint plan: bad sizes");
+  nslots = std::min(nslots, nunits - 1);
+  const std::vector<RevAct> pl = rev_plan(nunits, nslots);
+  long long adv = 0;
+  for (const RevAct &x : pl)
+    if (x.op == REV_ADVANCE) adv += x.arg;
+  if (n_acts) *n_acts = (int)pl.size();
+  if (advances) *advances = adv;
+  if (!ops) return FSIBE_OK;
+  if ((size_t)cap < pl.size()) return set_err(FSIBE_ESIZE, "adjoint plan: %zu actions", pl.size());
+  for (size_t i = 0; i < pl.size(); ++i) {
+    ops[3 * i] = pl[i].op;
+    ops[3 * i + 1] = pl[i].unit;
+    ops[3 * i + 2] = pl[i].arg;
+  }
+  return FSIBE_OK;
+}
+
+// the least number of units any schedule advances (t(l, s) above)
+extern "C" long long fsibe_adjoint_plan_cost(int nunits, int nslots) {
+  if (nunits < 1 || nslots < 0) return -1;
+  return rev_cost(nunits, std::min(nslots, nunits - 1));
+}
+
 struct fsibe_adjoint {
   fsibe_sim *s = nullptr;
-  int nsteps = 0, chk = 0, nseg = 0;
+  int nsteps = 0, chk = 0, nseg = 0, nslots = 0;
+  long long steps_advanced = 0, steps_recorded = 0;
   SimSnap init;
   std::vector<SimSnap> snaps;
   AdjTape tape;
@@ -173,8 +293,8 @@ extern "C" int fsibe_adjoint_destroy(fsibe_adjoint *a) {
   return FSIBE_OK;
 }
 
-extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, int nsteps, int chk,
-                                    fsibe_adjoint **out) {
+extern "C" int fsibe_adjoint_create_binomial(const fsibe_params *p, const double *vortk, int nsteps, int chk,
+                                             int nsnap, fsibe_adjoint **out) {
   if (!p || !out || nsteps < 1 || chk < 1) return set_err(FSIBE_EARG, "adjoint: bad argument");
   *out = nullptr;
   if (p->iFLUSI != 0 || p->iBeam != 1 || p->iMotion != 601)
@@ -196,6 +316,7 @@ extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, 
   a->nsteps = nsteps;
   a->chk = std::min(chk, nsteps);
   a->nseg = (nsteps + a->chk - 1) / a->chk;
+  a->nslots = nsnap < 0 ? a->nseg - 1 : std::min(nsnap, a->nseg - 1);
   const size_t NH = (size_t)s->H * s->ny, N = (size_t)s->nx * s->ny;
   const int cap = a->chk;
   AdjTape &t = a->tape;
@@ -225,7 +346,7 @@ extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, 
   a->part = reinterpret_cast<double *>(D(sizeof(double) * 4 * a->part_blocks * cap));
   a->d_nodes = reinterpret_cast<double *>(D(sizeof(double) * std::max(s->ns, 1)));
   a->init.dev = D(snap_bytes(s));
-  a->snaps.resize(a->nseg);
+  a->snaps.resize(a->nslots);
   for (auto &q : a->snaps) q.dev = D(snap_bytes(s));
   if (!ok) return fail(set_err(FSIBE_EHIP, "adjoint: device allocation failed"));
   if (hipHostMalloc((void **)&a->h_part, sizeof(double) * 4 * a->part_blocks * cap, hipHostMallocDefault) != hipSuccess)
@@ -242,6 +363,11 @@ extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, 
   return FSIBE_OK;
 }
 
+extern "C" int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, int nsteps, int chk,
+                                    fsibe_adjoint **out) {
+  return fsibe_adjoint_create_binomial(p, vortk, nsteps, chk, -1, out);
+}
+
 // the initial state with the motion coefficients c
 static int adj_begin(fsibe_adjoint *a, const double coef[8]) {
   fsibe_sim *s = a->s;
@@ -254,15 +380,23 @@ static int adj_begin(fsibe_adjoint *a, const double coef[8]) {
   return FSIBE_OK;
 }
 
-static int adj_forward(fsibe_adjoint *a, bool save, double *J) {
+// first step of unit k
+static int unit_start(const fsibe_adjoint *a, int k) { return std::min(k * a->chk, a->nsteps); }
+
+// run the steps of units [k, k + n) forward from the state of unit k
+static int adj_advance(fsibe_adjoint *a, int k, int n) {
   fsibe_sim *s = a->s;
-  for (int k = 0; k < a->nseg; ++k) {
-    if (save) CHK(snap_save(s, a->snaps[k]));
-    const int len = std::min(a->chk, a->nsteps - k * a->chk);
-    fsibe_step_out o;
-    CHK(fsibe_run(s, len, &o));
-    if (s->it + 1 < k * a->chk + len) return set_err(FSIBE_ESTATE, "adjoint: the run reached Time_end");
-  }
+  const int first = unit_start(a, k), end = unit_start(a, k + n);
+  fsibe_step_out o;
+  CHK(fsibe_run(s, end - first, &o));
+  if (s->it + 1 < end) return set_err(FSIBE_ESTATE, "adjoint: the run reached Time_end");
+  a->steps_advanced += end - first;
+  return FSIBE_OK;
+}
+
+static int adj_forward(fsibe_adjoint *a, double *J) {
+  fsibe_sim *s = a->s;
+  CHK(adj_advance(a, 0, a->nseg));
   if (!(s->jsum_t > 0.0)) return set_err(FSIBE_ESTATE, "adjoint: no step taken");
   *J = s->jsum_f / s->jsum_t;
   return FSIBE_OK;
@@ -271,8 +405,10 @@ static int adj_forward(fsibe_adjoint *a, bool save, double *J) {
 extern "C" int fsibe_adjoint_objective(fsibe_adjoint *a, const double coef[8], double *J) {
   if (!a || !coef || !J) return set_err(FSIBE_EARG, "null argument");
   const double t0 = now_ms();
+  a->steps_advanced = 0;
+  a->steps_recorded = 0;
   CHK(adj_begin(a, coef));
-  CHK(adj_forward(a, false, J));
+  CHK(adj_forward(a, J));
   a->ms_forward = now_ms() - t0;
   return FSIBE_OK;
 }
@@ -351,71 +487,124 @@ static int adj_reverse_step(fsibe_adjoint *a, int j, const double *u1n, double c
   return FSIBE_OK;
 }
 
+// record unit k (its steps recomputed from the state of its start, which the
+// simulation holds) and run its adjoint steps backwards; the motion
+// parameters' partial sums go into g.  The first unit recorded is the last
+// one: its run ends the forward sweep and fixes J and T.
+static int adj_record_reverse(fsibe_adjoint *a, int k, double *J, double *T, double g[8]) {
+  fsibe_sim *s = a->s;
+  const int first = unit_start(a, k), len = unit_start(a, k + 1) - first;
+  AdjTape &t = a->tape;
+  t.n = 0;
+  s->tape = &t;
+  int rc = FSIBE_OK;
+  // pipelined as fsibe_run's prescribed-motion loop: the host prepares
+  // step j+1 while step j runs and reads each step's results one step late
+  for (int j = 0; j < len && rc == FSIBE_OK; ++j) {
+    rc = run_one(s, true, false);
+    if (rc == FSIBE_OK) {
+      t.rec[j].dt1 = s->dt1;
+      if (t.n != j + 1) rc = set_err(FSIBE_ESTATE, "adjoint: step %d was not recorded", first + j);
+      else if (s->dt1 != s->p.dt_fixed)
+        rc = set_err(FSIBE_EDOMAIN, "adjoint: step %d took dt1 = %g from the CFL bound or Time_end, not dt_fixed",
+                     first + j, s->dt1);
+    }
+  }
+  s->tape = nullptr;
+  s->pipe = false;
+  if (s->res_pending) {  // the last step's results (the reverse sweep needs none of them)
+    if (hipStreamSynchronize(s->stream) != hipSuccess && rc == FSIBE_OK) rc = set_err(FSIBE_EHIP, "adjoint: sync failed");
+    parse_results(s, s->slot ^ 1);
+    s->res_pending = false;
+  }
+  CHK(rc);
+  a->steps_recorded += len;
+  if (!(*T > 0.0)) {
+    if (k != a->nseg - 1) return set_err(FSIBE_ESTATE, "adjoint: the schedule reversed unit %d first", k);
+    if (s->it + 1 < a->nsteps) return set_err(FSIBE_ESTATE, "adjoint: the run reached Time_end");
+    if (!(s->jsum_t > 0.0)) return set_err(FSIBE_ESTATE, "adjoint: no step taken");
+    *J = s->jsum_f / s->jsum_t;
+    *T = s->jsum_t;
+  }
+  const double dxdy = s->dx * s->dy;
+  for (int j = len - 1; j >= 0; --j) {
+    const double *u1n = j == len - 1 ? s->u1 : t.u1 + (size_t)(j + 1) * t.N;
+    CHK(adj_reverse_step(a, j, u1n, -(t.rec[j].dt1 * dxdy) / *T));
+  }
+  HIPCHK(hipMemcpyAsync(a->h_part, a->part, sizeof(double) * 4 * a->part_blocks * len, hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int j = len - 1; j >= 0; --j) {
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    const double *pp = a->h_part + (size_t)j * 4 * a->part_blocks;
+    for (int b = 0; b < a->part_blocks; ++b)
+      for (int i = 0; i < 4; ++i) q[i] += pp[4 * b + i];
+    double jac[4][8];
+    motion601_jac(s->p, t.rec[j].tau, jac);
+    for (int c = 0; c < 8; ++c)
+      for (int i = 0; i < 4; ++i) g[c] += q[i] * jac[i][c];
+  }
+  return FSIBE_OK;
+}
+
 extern "C" int fsibe_adjoint_gradient(fsibe_adjoint *a, const double coef[8], double *J, double grad[8]) {
   if (!a || !coef || !J || !grad) return set_err(FSIBE_EARG, "null argument");
   fsibe_sim *s = a->s;
   const double t0 = now_ms();
+  double t1 = -1.0;
+  a->steps_advanced = 0;
+  a->steps_recorded = 0;
   CHK(adj_begin(a, coef));
-  CHK(adj_forward(a, true, J));
-  const double t1 = now_ms();
-  const double T = s->jsum_t;
+  int cur = 0;  // the unit whose start the simulation's state is
   const size_t NH = (size_t)s->H * s->ny, N = (size_t)s->nx * s->ny;
   adj_zero(s->stream, reinterpret_cast<double *>(a->lW), 2 * NH);
   adj_zero(s->stream, reinterpret_cast<double *>(a->lNL), 2 * NH);
   adj_zero(s->stream, a->lu1, N);
   adj_zero(s->stream, a->lu2, N);
-  double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const double dxdy = s->dx * s->dy;
-  for (int k = a->nseg - 1; k >= 0; --k) {
-    const int len = std::min(a->chk, a->nsteps - k * a->chk);
-    // recompute the segment, recording it
-    CHK(snap_restore(s, a->snaps[k]));
-    AdjTape &t = a->tape;
-    t.n = 0;
-    s->tape = &t;
-    int rc = FSIBE_OK;
-    // pipelined as fsibe_run's prescribed-motion loop: the host prepares
-    // step j+1 while step j runs and reads each step's results one step late
-    for (int j = 0; j < len && rc == FSIBE_OK; ++j) {
-      rc = run_one(s, true, false);
-      if (rc == FSIBE_OK) {
-        t.rec[j].dt1 = s->dt1;
-        if (t.n != j + 1) rc = set_err(FSIBE_ESTATE, "adjoint: step %d was not recorded", k * a->chk + j);
-        else if (s->dt1 != s->p.dt_fixed)
-          rc = set_err(FSIBE_EDOMAIN, "adjoint: step %d took dt1 = %g from the CFL bound or Time_end, not dt_fixed",
-                       k * a->chk + j, s->dt1);
-      }
-    }
-    s->tape = nullptr;
-    s->pipe = false;
-    if (s->res_pending) {  // the last step's results (the reverse sweep needs none of them)
-      if (hipStreamSynchronize(s->stream) != hipSuccess && rc == FSIBE_OK) rc = set_err(FSIBE_EHIP, "adjoint: sync failed");
-      parse_results(s, s->slot ^ 1);
-      s->res_pending = false;
-    }
-    CHK(rc);
-    for (int j = len - 1; j >= 0; --j) {
-      const double *u1n = j == len - 1 ? s->u1 : t.u1 + (size_t)(j + 1) * t.N;
-      CHK(adj_reverse_step(a, j, u1n, -(t.rec[j].dt1 * dxdy) / T));
-    }
-    HIPCHK(hipMemcpyAsync(a->h_part, a->part, sizeof(double) * 4 * a->part_blocks * len, hipMemcpyDeviceToHost,
-                          s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    for (int j = len - 1; j >= 0; --j) {
-      double q[4] = {0.0, 0.0, 0.0, 0.0};
-      const double *pp = a->h_part + (size_t)j * 4 * a->part_blocks;
-      for (int b = 0; b < a->part_blocks; ++b)
-        for (int i = 0; i < 4; ++i) q[i] += pp[4 * b + i];
-      double jac[4][8];
-      motion601_jac(s->p, t.rec[j].tau, jac);
-      for (int c = 0; c < 8; ++c)
-        for (int i = 0; i < 4; ++i) g[c] += q[i] * jac[i][c];
+  double g[8] = {0, 0, 0, 0, 0, 0, 0, 0}, T = 0.0;
+  for (const RevAct &x : rev_plan(a->nseg, a->nslots)) {
+    switch (x.op) {
+      case REV_RESTORE:
+        if (cur != x.unit) {
+          if (x.arg < 0) CHK(adj_begin(a, coef));
+          else CHK(snap_restore(s, a->snaps[x.arg]));
+          cur = x.unit;
+        }
+        break;
+      case REV_ADVANCE:
+        if (cur != x.unit) return set_err(FSIBE_ESTATE, "adjoint: schedule out of order");
+        CHK(adj_advance(a, x.unit, x.arg));
+        cur = x.unit + x.arg;
+        break;
+      case REV_SAVE:
+        if (cur != x.unit) return set_err(FSIBE_ESTATE, "adjoint: schedule out of order");
+        CHK(snap_save(s, a->snaps[x.arg]));
+        break;
+      case REV_REVERSE:
+        if (cur != x.unit) return set_err(FSIBE_ESTATE, "adjoint: schedule out of order");
+        if (t1 < 0.0) t1 = now_ms();
+        CHK(adj_record_reverse(a, x.unit, J, &T, g));
+        cur = x.unit + 1;
+        break;
+      default: break;
     }
   }
   for (int c = 0; c < 8; ++c) grad[c] = g[c];
   const double t2 = now_ms();
   a->ms_gradient = t2 - t0;
   a->ms_reverse = t2 - t1;
+  return FSIBE_OK;
+}
+
+// the checkpoint slots and, for the last objective or gradient call, the
+// steps advanced without a tape (a gradient's forward sweep included) and
+// the steps recorded
+extern "C" int fsibe_adjoint_stats(fsibe_adjoint *a, int *slots, long long *steps_advanced,
+                                   long long *steps_recorded) {
+  if (!a) return set_err(FSIBE_EARG, "null adjoint");
+  if (slots) *slots = a->nslots;
+  if (steps_advanced) *steps_advanced = a->steps_advanced;
+  if (steps_recorded) *steps_recorded = a->steps_recorded;
   return FSIBE_OK;
 }
 

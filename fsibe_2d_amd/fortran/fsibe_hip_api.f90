@@ -321,6 +321,17 @@ module fsibe_hip_api
       type(c_ptr), intent(out) :: adj
       integer(c_int) :: rc
     end function
+    ! the same with at most nsnap checkpoint slots (binomial schedule;
+    ! nsnap < 0: one per unit of chk steps); identical results
+    function fsibe_adjoint_create_binomial(p, vortk, nsteps, chk, nsnap, adj) &
+        bind(C, name="fsibe_adjoint_create_binomial") result(rc)
+      import :: c_int, c_double, c_ptr, fsibe_params_t
+      type(fsibe_params_t), intent(in) :: p
+      real(c_double), intent(in) :: vortk(*)
+      integer(c_int), value :: nsteps, chk, nsnap
+      type(c_ptr), intent(out) :: adj
+      integer(c_int) :: rc
+    end function
     function fsibe_adjoint_destroy(adj) bind(C, name="fsibe_adjoint_destroy") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: adj

@@ -393,17 +393,42 @@ int fsibe_batch_describe(fsibe_batch *batch, char *buf, size_t len);
  * over nsteps steps from the initial vorticity vortk (packed layout), where
  * Force(1) is cal_drag's penalisation force (NavierStokes.f90:567-575).
  * fsibe_adjoint_gradient returns J and dJ/dc by the discrete adjoint of the
- * GPU step (reverse sweep over recomputed segments of chk steps, one
- * checkpoint per segment); fsibe_adjoint_objective runs the forward only.
- * The time step is taken as fixed (dt_fixed, CFL not binding). */
+ * GPU step (reverse sweep over recomputed units of chk steps, each recorded
+ * on a tape of chk steps just before its adjoint steps run);
+ * fsibe_adjoint_objective runs the forward only.  The time step is taken as
+ * fixed (dt_fixed, CFL not binding).
+ * fsibe_adjoint_create keeps the state of every unit's start (one forward
+ * sweep, ceil(nsteps/chk) - 1 snapshot slots of 7 fields each).
+ * fsibe_adjoint_create_binomial bounds the slots to nsnap (nsnap < 0: one
+ * per unit start, as fsibe_adjoint_create): the reverse sweep then runs the
+ * binomial checkpointing schedule of Griewank's revolve, which advances the
+ * fewest units any schedule with nsnap slots can (fsibe_adjoint_plan_cost);
+ * results are bit-identical for every nsnap. */
 typedef struct fsibe_adjoint fsibe_adjoint;
 int fsibe_adjoint_create(const fsibe_params *p, const double *vortk, int nsteps, int chk, fsibe_adjoint **out);
+int fsibe_adjoint_create_binomial(const fsibe_params *p, const double *vortk, int nsteps, int chk, int nsnap,
+                                  fsibe_adjoint **out);
 int fsibe_adjoint_destroy(fsibe_adjoint *adj);
 int fsibe_adjoint_objective(fsibe_adjoint *adj, const double coef[8], double *J);
 int fsibe_adjoint_gradient(fsibe_adjoint *adj, const double coef[8], double *J, double grad[8]);
 /* wall times of the last objective / gradient call (the gradient's time
  * includes its forward run; reverse_ms is the part after it) */
 int fsibe_adjoint_timings(fsibe_adjoint *adj, double *forward_ms, double *gradient_ms, double *reverse_ms);
+/* the snapshot slots, and the steps the last objective / gradient call ran
+ * forward without a tape (a gradient's forward sweep included) and with one */
+int fsibe_adjoint_stats(fsibe_adjoint *adj, int *slots, long long *steps_advanced, long long *steps_recorded);
+/* The reverse schedule for nunits units and nslots slots (host only, no
+ * GPU): action i is ops[3i..3i+2] = (op, unit, arg) with op 0 = restore the
+ * state of unit `unit` from slot arg (-1: the initial state), 1 = advance
+ * arg units from unit `unit`, 2 = store the state of unit `unit` in slot
+ * arg, 3 = record unit `unit` and run its adjoint steps.  *n_acts is the
+ * action count (FSIBE_ESIZE when it exceeds cap; ops may be NULL to ask for
+ * it), *advances the units it advances. */
+int fsibe_adjoint_plan(int nunits, int nslots, int *ops, int cap, int *n_acts, long long *advances);
+/* the least number of units advanced by any schedule reversing nunits units
+ * with nslots slots and the initial state at hand: r l - C(nslots+1+r, r-1)
+ * with r the least integer such that C(nslots+1+r, r) >= l (-1 for bad sizes) */
+long long fsibe_adjoint_plan_cost(int nunits, int nslots);
 
 /* ---- restart files --------------------------------------------------------
  * MakeRuntimeBackup (save_fields.f90:109-152): the runtime_backup<n>.in file

@@ -79,6 +79,37 @@ def test_gradient_independent_of_checkpoint_spacing():
         assert np.array_equal(g, res[0][1])
 
 
+def test_binomial_snapshot_bounds_give_identical_gradients():
+    """fewer snapshot slots than unit starts (the revolve schedule) change
+    only how often units are recomputed: J and dJ/dc are bit-identical for 0,
+    1, 2 and 3 slots and for a slot per unit start (22 steps in 8 units of 3,
+    the last unit 1 step), and the steps run forward are the schedule's
+    least count of advanced units in steps, plus each step recorded once"""
+    from fsibe_2d_amd.adjoint import Adjoint, plan
+
+    p, vk = _case(64, 24, seed=6)
+    c = _coef(9)
+    nsteps, chk = 22, 3
+    units = [min(chk, nsteps - k * chk) for k in range(8)]
+    res = []
+    for snaps in (None, 3, 2, 1, 0):
+        a = Adjoint(p, vk, nsteps, chk, snapshots=snaps)
+        res.append(a.gradient(c))
+        st = a.stats()
+        nslots = 7 if snaps is None else snaps
+        acts, _ = plan(8, nslots)
+        adv = sum(sum(units[u:u + n]) for op, u, n in acts if op == 1)
+        print(f"[adjoint binomial] slots {st['slots']} advanced {st['steps_advanced']} "
+              f"recorded {st['steps_recorded']}", flush=True)
+        assert st["slots"] == nslots
+        assert st["steps_recorded"] == nsteps
+        assert st["steps_advanced"] == adv
+        a.close()
+    for J, g in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(g, res[0][1])
+
+
 def test_objective_is_the_force_record_of_a_plain_run():
     """J = (1/T) sum (-Force(1) dt1) from the step_out records of FluidSim
     stepping the same run one step at a time"""
