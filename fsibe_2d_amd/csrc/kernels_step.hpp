@@ -107,13 +107,46 @@ __device__ __forceinline__ double dealias(const StepConsts &c, int kx, int ky) {
 }
 
 // ---- deterministic block reductions (fixed shape for fixed blockDim) --------
+// Wave reductions on the VALU with data-parallel primitives (DPP moves of the
+// two 32-bit halves; 64-bit operations take no DPP operand on this ISA):
+// pairs and halves of each quad, rotations by 4 and 8 within each row of 16
+// lanes, then the row broadcasts 15 and 31, which leave the total in lane
+// 63.  Lanes that a row mask leaves out take `old`, a neutral term (0 for a
+// sum, the lane's own value for a maximum).  The order is fixed, so results
+// are deterministic; lane 63's value is broadcast to the wave.  Every lane of
+// the wave must be active.
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ double dpp_move(double v, double old) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)(o & 0xffffffffll), (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+constexpr int kDppQuad1032 = 0xB1, kDppQuad2301 = 0x4E;  // quad_perm [1,0,3,2], [2,3,0,1]
+constexpr int kDppRor4 = 0x124, kDppRor8 = 0x128, kDppBcast15 = 0x142, kDppBcast31 = 0x143;
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = v + dpp_move<kDppQuad1032>(v, 0.0);
+  v = v + dpp_move<kDppQuad2301>(v, 0.0);
+  v = v + dpp_move<kDppRor4>(v, 0.0);
+  v = v + dpp_move<kDppRor8>(v, 0.0);
+  v = v + dpp_move<kDppBcast15, 0xa>(v, 0.0);
+  v = v + dpp_move<kDppBcast31, 0xc>(v, 0.0);
+  return lane63(v);
 }
 __device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  v = fmax(v, dpp_move<kDppQuad1032>(v, v));
+  v = fmax(v, dpp_move<kDppQuad2301>(v, v));
+  v = fmax(v, dpp_move<kDppRor4>(v, v));
+  v = fmax(v, dpp_move<kDppRor8>(v, v));
+  v = fmax(v, dpp_move<kDppBcast15, 0xa>(v, v));
+  v = fmax(v, dpp_move<kDppBcast31, 0xc>(v, v));
+  return lane63(v);
 }
 
 // reduces NV values per thread to thread 0; red must hold NV*nwaves doubles
