@@ -149,6 +149,16 @@ __device__ __forceinline__ double wave_max(double v) {
   return lane63(v);
 }
 
+// the wave sums of v[0..NV) into dst[i*nw + wid] (lane 0 writes)
+template <int NV>
+__device__ __forceinline__ void wave_sums(const double *v, double *dst, int lane, int wid, int nw) {
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double q = wave_sum(v[i]);
+    if (lane == 0) dst[i * nw + wid] = q;
+  }
+}
+
 // reduces NV values per thread to thread 0; red must hold NV*nwaves doubles
 template <int NV>
 __device__ void block_sum(double (&v)[NV], double *red) {
@@ -580,7 +590,7 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
                                                     double UMx0, double UMy0, int mode, int pres, FX F,
                                                     const double2 *__restrict__ twx, StepConsts c, int bid, int nblk) {
   extern __shared__ double2 lds[];
-  __shared__ double red[kNSUM * 16];
+  __shared__ double red[(kNSUM + 1) * 16];
   if constexpr (CMODE >= 0) mode = CMODE;
   if constexpr (CPRES >= 0) pres = CPRES;
   const int H = F.H(), nx = 2 * H, tid = threadIdx.x, nt = blockDim.x, PL = padlen(H);
@@ -672,40 +682,35 @@ __device__ __forceinline__ void rows_press_body(const double2 *__restrict__ Cw, 
       f1[qn] = make_double2(q2.x + w.x * a1.x, q2.y + w.y * a1.y);
     }
   }
-  // u_max partial (next step's CFL)
+  // the u_max partial (next step's CFL) and the row sums in one pass: the
+  // waves' values go to LDS, one barrier (which also orders the physics
+  // loop's LDS writes before the forward transforms), then thread 0 adds
+  // them wave by wave, as block_sum does.  Rows outside the mask box reduce
+  // only the two energy sums (the mask sums 2..10 are exactly 0), rows of
+  // one window the first kNSUM1 (11..14 are exactly 0).
   {
+    const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
     um = wave_max(um);
-    __syncthreads();
-    if ((tid & 63) == 0) red[tid >> 6] = um;
+    if (lane == 0) red[wid] = um;
+    const int nv = !mode ? 0 : !mrow ? 2 : two ? kNSUM : kNSUM1;
+    if (nv == kNSUM) wave_sums<kNSUM>(acc, red + nw, lane, wid, nw);
+    else if (nv == kNSUM1) wave_sums<kNSUM1>(acc, red + nw, lane, wid, nw);
+    else if (nv == 2) wave_sums<2>(acc, red + nw, lane, wid, nw);
     __syncthreads();
     if (tid == 0) {
       double m = 0.0;
-      for (int w = 0; w < nt / 64; ++w) m = fmax(m, red[w]);
+      for (int w = 0; w < nw; ++w) m = fmax(m, red[w]);
       umax_part[y] = m;
+      if (mode)
+        for (int i = 0; i < kNSUM; ++i) {
+          double q = 0.0;
+          if (i < nv)
+            for (int w = 0; w < nw; ++w) q += red[nw + i * nw + w];
+          sums[(size_t)y * kNSUM + i] = q;
+        }
     }
-    __syncthreads();
   }
   if (!mode) return;
-  if (mrow && two) {
-    block_sum<kNSUM>(acc, red);
-  } else if (mrow) {
-    // one window: the sums 11..14 are exactly 0
-    double a1[kNSUM1];
-#pragma unroll
-    for (int i = 0; i < kNSUM1; ++i) a1[i] = acc[i];
-    block_sum<kNSUM1>(a1, red);
-#pragma unroll
-    for (int i = 0; i < kNSUM1; ++i) acc[i] = a1[i];
-  } else {
-    // a row outside the mask box: the mask sums 2..10 are exactly 0, and
-    // the wave reductions (two LDS permutes per step and value) are skipped
-    double a2[2] = {acc[0], acc[1]};
-    block_sum<2>(a2, red);
-    acc[0] = a2[0];
-    acc[1] = a2[1];
-  }
-  if (tid == 0)
-    for (int i = 0; i < kNSUM; ++i) sums[(size_t)y * kNSUM + i] = acc[i];
   if (!pres) return;
   F.template fft<false, 2>(lds, twx, tid, nt);
   rows_r2c_store<2>(lds, H, PL, twx, tid, nt, [&](int k, const double2 *x) {
