@@ -517,7 +517,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--batch-sims", type=int, default=64, help="members of the batched workload (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-steps", type=int, default=1000, help="steps per simulation of the batched workload")
     ap.add_argument("--adjoint-steps", type=int, default=1000, help="steps of the adjoint leg (0: skip)")
     ap.add_argument("--turb-n", type=int, default=4096, help="grid of the solver-only turbulence leg")
     ap.add_argument("--turb-steps", type=int, default=200, help="steps of the turbulence leg (0: skip)")

@@ -163,23 +163,23 @@ static int snap_restore(fsibe_sim *s, SimSnap &q) {
 // and it is convex in l, so the objective of that minimum is convex in k
 // and its least minimiser is found by bisection on its forward difference.
 
-// B(c, r) = C(c + r, r), saturated at 2^62
-static long long rev_binom(int c, long long r) {
-  const long long cap = 1ll << 62;
-  __int128 b = 1;
-  for (long long i = 1; i <= r; ++i) {
-    b = b * (c + i) / i;
-    if (b > cap) return cap;
-  }
-  return (long long)b;
-}
-
+// t(l, s); the binomials B(s+1, r) grow term by term (B(c, r) = B(c, r-1)
+// (c + r) / r, exact in 128 bits and saturated at 2^62)
 static long long rev_cost(long long l, int s) {
   if (l <= 1) return 0;
   if (s <= 0) return l * (l - 1) / 2;
+  const __int128 cap = (__int128)1 << 62;
+  const int c = s + 1;
   long long r = 0;
-  while (rev_binom(s + 1, r) < l) ++r;
-  return r * l - rev_binom(s + 2, r - 1);
+  __int128 b = 1, b2 = 1;  // B(c, r) and B(c + 1, r - 1) (B(c + 1, -1) unused: l > 1 makes r >= 1)
+  while (b < l) {
+    ++r;
+    b = b * (c + r) / r;
+    if (r >= 2) b2 = b2 * (c + 1 + (r - 1)) / (r - 1);
+    if (b > cap) b = cap;
+    if (b2 > cap) b2 = cap;
+  }
+  return r * l - (long long)b2;
 }
 
 // the least k in [1, l - 1] minimising k + t(l - k, s - 1) + t(k, s)
@@ -206,6 +206,25 @@ struct RevAct {
 // slots (ids taken from `free_ids`)
 static void rev_plan_rec(long long a, long long b, int s, int sa, std::vector<int> &free_ids,
                          std::vector<RevAct> &out) {
+  if (b - a > 1 && s >= b - a - 1) {
+    // a slot for every unit start above a: one sweep storing each (the
+    // schedule's splits would all be k = 1), then every unit in turn
+    std::vector<int> ids((size_t)(b - a));
+    ids[0] = sa;
+    out.push_back({REV_RESTORE, (int)a, sa});
+    for (long long u = a + 1; u < b; ++u) {
+      ids[(size_t)(u - a)] = free_ids.back();
+      free_ids.pop_back();
+      out.push_back({REV_ADVANCE, (int)(u - 1), 1});
+      out.push_back({REV_SAVE, (int)u, ids[(size_t)(u - a)]});
+    }
+    for (long long u = b - 1; u >= a; --u) {
+      out.push_back({REV_RESTORE, (int)u, ids[(size_t)(u - a)]});
+      out.push_back({REV_REVERSE, (int)u, 0});
+      if (u > a) free_ids.push_back(ids[(size_t)(u - a)]);
+    }
+    return;
+  }
   while (b - a > 1 && s > 0) {
     const long long k = rev_split(b - a, s);
     const int q = free_ids.back();

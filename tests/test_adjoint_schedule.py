@@ -104,6 +104,16 @@ def test_long_horizon_schedule():
     assert adv < 4 * 10000
 
 
+def test_extreme_slot_counts():
+    """a slot per unit start for 100 000 units (one sweep storing each unit's
+    start, built without deep recursion) and a single slot for 3000 units
+    (t = r l - C(r + 2, r - 1) with r ~ sqrt(2 l))"""
+    assert play(100000, 99999) == 99999
+    adv = play(3000, 1)
+    r = next(r for r in range(1, 200) if comb(2 + r, r) >= 3000)
+    assert adv == plan_cost(3000, 1) == r * 3000 - comb(3 + r - 1, r - 1)
+
+
 def test_bad_sizes_are_refused():
     from fsibe_2d_amd._lib import lib
 
