@@ -8,6 +8,7 @@
 // geometry, interpolation targets, bounding boxes), uploads it in one copy per
 // step, and reads back one small result record per step.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <array>
@@ -280,11 +281,24 @@ static int with_cols_sel(const FftPlan &py, int H, ColSel sel, Fn &&fn) {
   }
 }
 
-#define KLAUNCH(KT, grid, nt, lds, st, ...)                         \
-  do {                                                              \
-    CHK(prep_lds(KT, lds));                                         \
-    KT<<<grid, nt, lds, st>>>(__VA_ARGS__);                         \
-    HIPCHK(hipGetLastError());                                      \
+// Kernel timing (fsibe_profile_enable): a profiled region arms its event pair
+// here, and its first KLAUNCH launches with hipExtLaunchKernelGGL, which
+// records the pair at the kernel's own start and end.  An event pair
+// recorded around a launch would also time the launch's dispatch.  A region
+// with one launch is timed by the kernel alone; with several, from the
+// first kernel's start to an event after the last launch.
+static thread_local hipEvent_t t_prof_start = nullptr, t_prof_stop = nullptr;
+static thread_local int t_prof_launches = 0;
+
+#define KLAUNCH(KT, grid, nt, lds, st, ...)                                                             \
+  do {                                                                                                  \
+    CHK(prep_lds(KT, lds));                                                                             \
+    if (t_prof_start && t_prof_launches++ == 0)                                                         \
+      hipExtLaunchKernelGGL(KT, dim3(grid), dim3(nt), (std::uint32_t)(lds), st, t_prof_start, t_prof_stop, \
+                            0u, __VA_ARGS__);                                                           \
+    else                                                                                                \
+      KT<<<grid, nt, lds, st>>>(__VA_ARGS__);                                                           \
+    HIPCHK(hipGetLastError());                                                                          \
   } while (0)
 
 static int env_int(const char *name, int dflt) {
@@ -1045,15 +1059,38 @@ static BeamGeo beam_geo(const fsibe_sim *s, int nthreads) {
   return g;
 }
 
+// The start event is also recorded here, for regions whose kernels are not
+// launched through KLAUNCH (their interval then includes the dispatch).
 static void prof_begin(fsibe_sim *s, int k) {
-  if (s->prof.on && (s->prof.mask >> k & 1u)) (void)hipEventRecord(s->prof.ev[k][0], s->stream);
+  if (s->prof.on && (s->prof.mask >> k & 1u)) {
+    (void)hipEventRecord(s->prof.ev[k][0], s->stream);
+    t_prof_start = s->prof.ev[k][0];
+    t_prof_stop = s->prof.ev[k][1];
+    t_prof_launches = 0;
+  }
 }
 static void prof_end(fsibe_sim *s, int k) {
   if (s->prof.on && (s->prof.mask >> k & 1u)) {
-    (void)hipEventRecord(s->prof.ev[k][1], s->stream);
+    if (t_prof_launches != 1) (void)hipEventRecord(s->prof.ev[k][1], s->stream);
     s->prof.rec[k] = true;
+    t_prof_start = t_prof_stop = nullptr;
+    t_prof_launches = 0;
   }
 }
+// disarms the kernel timing at the end of the enclosing scope, so that a
+// region left by an error return (before its prof_end) leaves no armed
+// events for a later launch
+struct ProfDisarm {
+  ~ProfDisarm() {
+    t_prof_start = t_prof_stop = nullptr;
+    t_prof_launches = 0;
+  }
+};
+#define PROF_CAT2(a, b) a##b
+#define PROF_CAT(a, b) PROF_CAT2(a, b)
+#define PROF_BEGIN(s, k) \
+  prof_begin(s, k);      \
+  ProfDisarm PROF_CAT(prof_disarm_, __LINE__)
 // after a stream synchronisation: accumulate the recorded intervals
 static void prof_collect(fsibe_sim *s) {
   if (!s->prof.on) return;
@@ -1862,7 +1899,7 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
     // one more row of workgroups builds the force tables beside the tiles
     dim3 blk(kMaskThreads);
     dim3 grd(std::max(tiles_x, bg.fwg), tiles_y + (bg.fwg > 0 ? 1 : 0));
-    prof_begin(s, P_MASK);
+    PROF_BEGIN(s, P_MASK);
     k_mask<<<grd, blk, sizeof(int4) * std::max(s->nprim, 1), s->stream>>>(
         reinterpret_cast<const Prim *>(s->d_stage + s->off_prim), s->nprim, s->wallrow ? s->wallrow + 0 : nullptr, mc,
         bg, tiles_y, s->mbox.x0, s->mbox.y0, s->mbox.w(), s->mbox.h(), s->mask_m, s->mask_vx, s->mask_vy,
@@ -1930,7 +1967,7 @@ static int launch_k1(fsibe_sim *s, int level) {
   const bool sp10 = s->p.iSponge == 10 && s->penal_on && s->ms;
   double2 *outs[3] = {s->T[0], s->T[1], s->T[4]};
   const int fsel[3] = {0, 4, 2};
-  prof_begin(s, P_COLS_W0);
+  PROF_BEGIN(s, P_COLS_W0);
   CHK(cols_inv_fields(s, s->W[level], outs, fsel, sp10 ? 3 : 2));
   prof_end(s, P_COLS_W0);
   return FSIBE_OK;
@@ -1970,7 +2007,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     CHK(fsibe_create_mask(s, time, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr));
   const int n0 = s->n0, n1 = s->n1;
   if (explicit_step) {
-    prof_begin(s, P_VEL);
+    PROF_BEGIN(s, P_VEL);
     CHK(velocity_only(s, time));
     prof_end(s, P_VEL);
   }
@@ -1989,7 +2026,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
   // this mask box's rows remains.
   {
     const size_t lds = sizeof(double2) * 3 * padlen(s->H);
-    prof_begin(s, P_ROWS_NL);
+    PROF_BEGIN(s, P_ROWS_NL);
     if (k2_ahead) {
       const int y0 = mb.x1 >= mb.x0 ? mb.y0 : 0, nrow = mb.x1 >= mb.x0 ? mb.y1 - mb.y0 + 1 : 0;
       CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
@@ -2033,7 +2070,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
   }
   // K4
   {
-    prof_begin(s, P_COLS_AB2);
+    PROF_BEGIN(s, P_COLS_AB2);
     CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 1, 1), [&](auto F, int nt, int TX) -> int {
       const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
       // without penalisation K2 leaves Rpx unwritten (it is 0) and K4 skips it
@@ -2049,14 +2086,14 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     const bool st = s->forces_on && s->visc;
     double2 *outs[4] = {s->T[0], s->T[1], s->T[2], s->T[3]};
     const int fsel[4] = {0, 1, 2, 3};
-    prof_begin(s, P_COLS_W1);
+    PROF_BEGIN(s, P_COLS_W1);
     CHK(cols_inv_fields(s, s->W[n1], outs, fsel, st ? 4 : 3, 2));
     prof_end(s, P_COLS_W1);
   }
   // K6
   {
     const size_t lds = sizeof(double2) * 3 * padlen(s->H);
-    prof_begin(s, P_ROWS_PRESS);
+    PROF_BEGIN(s, P_ROWS_PRESS);
     CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
       // the step's instances: mode 1, with or without the pressure rhs
       using FT = decltype(F);
@@ -2079,7 +2116,7 @@ static int evolve_pressure_box(fsibe_sim *s) {
   if (s->forces_on && s->forces_ready && !s->fbox.empty()) {
     const Box &fb = s->fbox;
     {
-      prof_begin(s, P_COLS_PRESS);
+      PROF_BEGIN(s, P_COLS_PRESS);
       CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 3, 1), [&](auto F, int nt, int TX) -> int {
         const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
         KLAUNCH(k_cols_press<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[4], s->T[5], s->T[6], fb.y0, fb.y1,
@@ -2090,7 +2127,7 @@ static int evolve_pressure_box(fsibe_sim *s) {
     }
     {
       const size_t lds = sizeof(double2) * 3 * padlen(s->H);
-      prof_begin(s, P_ROWS_BOX);
+      PROF_BEGIN(s, P_ROWS_BOX);
       CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
         KLAUNCH(k_rows_box<decltype(F)>, fb.h(), nt, lds, s->stream, s->T[6], s->T[1], s->T[2], s->T[3], s->u1,
                 s->u2, s->fb_p, s->fb_sa, s->fb_sb, fb.y0, fb.x0, fb.x1, fb.w(), (int)s->visc, F, s->twx, s->sc);
@@ -2117,7 +2154,7 @@ static int launch_forces(fsibe_sim *s) {
   const int units = s->nb * (s->fg.ns + s->fg.nc);
   if (s->ntg != s->nb * (6 * s->fg.ns + 3 * 2 * s->fg.nc) || s->fg.nc > kNC)
     return set_err(FSIBE_ESTATE, "interpolation targets do not match the beam");
-  prof_begin(s, P_INTERP);
+  PROF_BEGIN(s, P_INTERP);
   k_forces<<<units, kForceThreads, 0, s->stream>>>(
       tg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
       reinterpret_cast<const double *>(s->d_stage + s->off_cd), pbeam_dev(s, s->slot),
