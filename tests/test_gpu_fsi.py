@@ -116,12 +116,12 @@ def test_coupled_loop_in_two_calls(oracle_mod):
     assert np.array_equal(a.beam()["y"], b.beam()["y"])
 
 
-@pytest.mark.parametrize("n,steps", [(1024, 3)])
+@pytest.mark.parametrize("n,steps", [(1024, 3), (2048, 2)])
 def test_full_size_coupled(n, steps, oracle_mod):
     """the benchmark workload (1024^2, Re 1000, 128 nodes, mass ratio 0.5,
-    stiffness 0.05, dt 5e-5) coupled; Force_unst keeps the 1e-5 bar of its
-    conditioning at this size (a one-step difference of a sum over ~4000
-    pixels divided by dt)"""
+    stiffness 0.05, dt 5e-5) coupled, and the same at 2048^2; Force_unst
+    keeps the 1e-5 bar of its conditioning at these sizes (a one-step
+    difference of a sum over ~4000 pixels divided by dt)"""
     from fsibe_2d_amd import fsi_large
     from fsibe_2d_amd.solver import FluidSim
 
