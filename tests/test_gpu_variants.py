@@ -96,6 +96,58 @@ def test_variant(name, oracle_mod):
     _check(o, g, name, beam=kw.get("iBeam", 1) > 0)
 
 
+# option groups that the listed variants test one at a time; the
+# combinations below draw one option per group (seeded), so that options
+# meet that no single variant puts together
+COMBO_GROUPS = (
+    ("geometry", ({}, dict(iWalls=1, h_channel=0.4, U_mean_true=0.8), dict(iSharpTrailing=1), dict(N_smooth=0.0),
+                  dict(iSmoothCutoff=1), dict(iBeam=0, iCylinder=1, R_cylinder=0.3, x0=2.0, y0=2.0, iSaveBeam=0))),
+    ("sponge", ({}, dict(iSponge=0), dict(iSponge=1, SpongeSize=0.4), dict(iSponge=2, SpongeSize=0.4),
+                dict(iSponge=10, SpongeSize=0.4), dict(iSponge=22, SpongeSize=0.4))),
+    ("mean", ({}, dict(iMeanVelocity=1, T_fullspeed=0.01), dict(iMeanVelocity=2, T_fullspeed=0.01, theta_inf=0.3),
+              dict(iMeanVelocity=11))),
+    ("motion", ({}, dict(iMotion=3, AngleBeam=10.0),
+                dict(iMotion=601, heave_freq=2.0, motion_coef=(0.05, -0.02, 0.01, 0.03, 0.1, -0.04, 0.02, 0.05)))),
+    ("loads", ({}, dict(iViscous=0))),
+    ("grid", (dict(nx=96, ny=64), dict(nx=64, ny=96), dict(nx=72, ny=72))),
+)
+
+
+def _combo_index(seed):
+    rng = np.random.default_rng(seed)
+    return [int(rng.integers(len(opts))) for _, opts in COMBO_GROUPS]
+
+
+def _combo(seed):
+    kw = dict(ns=32, dt_fixed=2e-4)
+    names = []
+    for (gname, opts), k in zip(COMBO_GROUPS, _combo_index(seed)):
+        kw.update(opts[k])
+        names.append(f"{gname}{k}")
+    return kw, "-".join(names)
+
+
+# seeds whose combinations together take every option of every group
+COMBO_SEEDS = (0, 4, 90, 21, 11, 24, 1, 2, 3, 5)
+
+
+def test_combo_seeds_cover_every_option():
+    taken = {(g, c) for s in COMBO_SEEDS for g, c in zip(range(len(COMBO_GROUPS)), _combo_index(s))}
+    assert taken == {(g, c) for g, (_, opts) in enumerate(COMBO_GROUPS) for c in range(len(opts))}
+
+
+@pytest.mark.parametrize("seed", COMBO_SEEDS)
+def test_variant_combinations(seed, oracle_mod):
+    """seeded combinations of the variants above (one option per group:
+    mask geometry, sponge, mean flow, kinematics, loads, grid), 12 steps
+    each against the oracle, with the bars of test_variant"""
+    kw, tag = _combo(seed)
+    o, g = _pair(oracle_mod, **kw)
+    o.run(12)
+    g.run(12)
+    _check(o, g, f"combo {seed} {tag}", beam=kw.get("iBeam", 1) > 0)
+
+
 def _bent_beam(ns, t, ds, x0, y0, alpha):
     """node positions of a beam bent by theta(s) = 0.35 sin(pi s) + 0.2 t s,
     built like integrate_position (integrate_position.f90:27-45)"""
