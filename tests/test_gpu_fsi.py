@@ -97,6 +97,28 @@ def test_coupled_loop(name, oracle_mod):
     g.close()
 
 
+def test_coupled_loop_long_horizon_cfl(oracle_mod):
+    """800 coupled steps at 128^2 with the time step chosen by the CFL bound
+    each step (dt_fixed = 0: dt1 from the device's u_max), so that the
+    rounding of every stage and of the step sizes accumulates over a long
+    horizon: the time, vortk, theta and the tip stay within the bars of
+    the short runs"""
+    from fsibe_2d_amd import Params
+    from fsibe_2d_amd.solver import FluidSim
+
+    p = Params(nx=128, ny=128, ns=48, iFLUSI=1, mue=1.0, eta=0.1, dt_fixed=0.0, cfl=0.4)
+    o0 = oracle_mod.Oracle(nx=p.nx, ny=p.ny, iBeam=0, iSponge=0)
+    vk = o0.init_noise(seed=5, amp=1e-3, kmax2=64)
+    nsteps = 800
+    o, so, t_o = oracle_fsi(oracle_mod, p, vk, nsteps)
+    g = FluidSim(p)
+    g.set_vortk(vk)
+    g.run(nsteps)
+    _compare(g, o, so, t_o, "long horizon, CFL dt")
+    assert t_o > 0.1 and np.abs(so.theta).max() > 1e-4  # a long run, and the beam bends
+    g.close()
+
+
 def test_coupled_loop_in_two_calls(oracle_mod):
     """fsibe_run(10) then fsibe_run(15) is the same run as fsibe_run(25)"""
     from fsibe_2d_amd import Params
