@@ -198,6 +198,21 @@ def test_host_moved_bent_beam(oracle_mod):
     assert err <= 1e-10
 
 
+@pytest.mark.parametrize("nx,ny", [(768, 512), (512, 1536), (1536, 384)])
+def test_non_square_mid_size(nx, ny, oracle_mod):
+    """non-square grids whose row and column lengths take the runtime FFT
+    plans (1536 = 2^9 3, and 768 = 2^8 3 rows of 384 complex values) or the
+    compile-time ones on the other axis, domain 4 x 4 (dx != dy), the beam
+    workload's physics, 4 steps against the oracle.  Force_unst has the bar
+    of the full-size runs (test_full_size_fsi): with dt = 5e-5 it is the
+    difference of two window sums that agree to ~1e-8 of their size, so the
+    summation order's rounding alone moves it by ~1e-8 relative here."""
+    o, g = _pair(oracle_mod, nx=nx, ny=ny, ns=64, nu=1e-3, dt_fixed=5e-5)
+    o.run(4)
+    g.run(4)
+    _check(o, g, f"{nx}x{ny}", unst_rel=1e-5)
+
+
 @pytest.mark.parametrize("n,steps", [(1024, 3), (2048, 2)])
 def test_full_size_fsi(n, steps, oracle_mod):
     """the benchmark workload at its real sizes (nu = 1e-3, 128 nodes, dt 5e-5)
