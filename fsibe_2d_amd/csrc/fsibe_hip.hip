@@ -238,7 +238,7 @@ struct ColSel {
 // share a launch
 constexpr bool ct_extra(int ny, int tx) {
   return ((ny == 256 || ny == 384 || ny == 512 || ny == 768) && (tx == 2 || tx == 4)) ||
-         (ny == 1024 && (tx == 1 || tx == 4)) || (ny == 2048 && tx == 1);
+         (ny == 1024 && (tx == 1 || tx == 4)) || (ny == 2048 && tx == 1) || (ny == 4096 && tx == 2);
 }
 
 static bool col_sel_valid(int ny, int H, int bm, ColSel c) {
@@ -266,7 +266,7 @@ static int with_cols_sel(const FftPlan &py, int H, ColSel sel, Fn &&fn) {
 #define COLSEL(NN, TT) \
       case NN * 8 + TT: return fn(ColCT<NN, TT, ct_nt(NN * TT * BM / 8, 1024)>{py, TT}, ct_nt(NN * TT * BM / 8, 1024), TT);
       COLSEL(256, 2) COLSEL(256, 4) COLSEL(384, 2) COLSEL(384, 4) COLSEL(512, 2) COLSEL(512, 4) COLSEL(768, 2)
-      COLSEL(768, 4) COLSEL(1024, 1) COLSEL(1024, 4) COLSEL(2048, 1)
+      COLSEL(768, 4) COLSEL(1024, 1) COLSEL(1024, 4) COLSEL(2048, 1) COLSEL(4096, 2)
 #undef COLSEL
       default: return set_err(FSIBE_ESIZE, "no compile-time column plan for %d x %d", py.L, sel.tx);
     }
