@@ -256,6 +256,18 @@ static bool col_sel_valid(int ny, int H, int bm, ColSel c) {
   return shape_of((long long)bm * c.tx * ny, has_radix16(pl)).emax > 0;
 }
 
+// K4 (launched through with_cols_sel<2>) holds two fields per column when
+// some member has penalisation (px), else one: its tiles' LDS is sized by
+// that, and a runtime-plan choice must also have a launch shape for two
+static bool k4_sel_valid(int ny, int H, bool px, ColSel c) {
+  if (!col_sel_valid(ny, H, px ? 2 : 1, c)) return false;
+  if (c.ct) return true;
+  FftPlan pl;
+  make_plan(ny, pl);
+  return shape_of(2LL * c.tx * ny, has_radix16(pl)).emax > 0;
+}
+static size_t k4_lds(int ny, bool px, int TX) { return sizeof(double2) * (px ? 2 : 1) * TX * padlen(ny); }
+
 // fn(policy, nthreads, TX) for the column stage launch choice `sel`
 // (col_sel_valid must hold)
 template <int BM, class Fn>
@@ -2072,7 +2084,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
   {
     PROF_BEGIN(s, P_COLS_AB2);
     CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 1, 1), [&](auto F, int nt, int TX) -> int {
-      const size_t lds = sizeof(double2) * 2 * TX * padlen(s->ny);
+      const size_t lds = k4_lds(s->ny, s->penal_on, TX);
       // without penalisation K2 leaves Rpx unwritten (it is 0) and K4 skips it
       KLAUNCH(k_cols_ab2<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[2], s->penal_on ? s->T[3] : nullptr,
               s->W[n0], s->NL[n1],
@@ -2756,6 +2768,9 @@ static const size_t kBatchArgSize[B_NST] = {sizeof(MaskArgs),      sizeof(ColsIn
 struct BatchGroup {
   int nx, ny, H, nf5;
   bool pres;
+  // some member has penalisation: K4's tiles hold two fields (Rc, Rpx);
+  // without, one (members without penalisation pass no Rpx)
+  bool px = false;
   std::vector<int> mem;
   hipStream_t stream = nullptr;  // each group runs its stages on its own stream
   hipEvent_t ev_dt = nullptr;    // after the group's nonlinear-term stage
@@ -2860,7 +2875,7 @@ static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int
     } else if (k == B_K4) {
       ColsAb2Args &a = reinterpret_cast<ColsAb2Args *>(h)[j];
       a = ColsAb2Args{};
-      a.Rc = s->T[2]; a.Rpx = s->T[3]; a.W0 = s->W[s->n0]; a.NL1 = s->NL[s->n1];
+      a.Rc = s->T[2]; a.Rpx = s->penal_on ? s->T[3] : nullptr; a.W0 = s->W[s->n0]; a.NL1 = s->NL[s->n1];
       a.NL0 = o; a.W1 = o + NH;
       a.s = s->dscal; a.explicit_step = 0; a.nblk = nb = s->H / tx; a.c = s->sc;
     } else {
@@ -2891,7 +2906,7 @@ static int tune_launch(const fsibe_batch *b, const BatchGroup &g, int k, ColSel 
     });
   if (k == B_K4)
     return with_cols_sel<2>(s0->py, s0->H, c, [&](auto F, int nt, int TX) -> int {
-      const size_t lds = sizeof(double2) * 2 * TX * padlen(s0->ny);
+      const size_t lds = k4_lds(s0->ny, g.px, TX);
       CHK(prep_lds(kb_cols_ab2<decltype(F)>, lds));
       for (int r = 0; r < reps; ++r)
         kb_cols_ab2<decltype(F)><<<dim3(gx, nm), nt, lds, g.stream>>>(reinterpret_cast<const ColsAb2Args *>(d), F, s0->twy);
@@ -2939,8 +2954,10 @@ static int tune_batch(fsibe_batch *b) {
       const int bm = (k == B_K4 || k == B_K7) ? 2 : 1;
       std::vector<ColSel> cand;
       for (int ct = 1; ct >= 0; --ct)
-        for (int tx = 1; tx <= 16; tx *= 2)
-          if (col_sel_valid(s0->ny, s0->H, bm, ColSel{ct, tx})) cand.push_back(ColSel{ct, tx});
+        for (int tx = 1; tx <= 16; tx *= 2) {
+          const ColSel c{ct, tx};
+          if (k == B_K4 ? k4_sel_valid(s0->ny, s0->H, g.px, c) : col_sel_valid(s0->ny, s0->H, bm, c)) cand.push_back(c);
+        }
       std::vector<float> best(cand.size(), 1e30f);
       for (int round = 0; rc == FSIBE_OK && round < 2; ++round)
         for (size_t ci = 0; rc == FSIBE_OK && ci < cand.size(); ++ci) {
@@ -2992,20 +3009,20 @@ static int tune_batch(fsibe_batch *b) {
 // that one simulation at its first fsibe_run, and kept for every
 // simulation of that grid.
 static std::mutex g_csel_mu;
-static std::map<std::array<int, 3>, std::array<ColSel, 4>> g_csel;
+static std::map<std::array<int, 4>, std::array<ColSel, 4>> g_csel;
 
 static int sim_tune_cols(fsibe_sim *s) {
   if (s->csel_ready) return FSIBE_OK;
   if (env_int("FSIBE_AUTOTUNE", 1) == 0) return FSIBE_OK;  // the static choices (reproducible profiles)
   const int nf5 = (s->forces_on && s->visc) ? 4 : 3;
-  const std::array<int, 3> key{s->nx, s->ny, nf5};
+  const std::array<int, 4> key{s->nx, s->ny, nf5, s->penal_on ? 1 : 0};
   std::lock_guard<std::mutex> lock(g_csel_mu);
   auto it = g_csel.find(key);
   if (it == g_csel.end()) {
     fsibe_batch t;  // borrows the simulation; releases nothing of it
     t.m.push_back(s);
     BatchGroup g;
-    g.nx = s->nx; g.ny = s->ny; g.H = s->H; g.nf5 = nf5; g.pres = s->pressure_on;
+    g.nx = s->nx; g.ny = s->ny; g.H = s->H; g.nf5 = nf5; g.pres = s->pressure_on; g.px = s->penal_on;
     g.mem.push_back(0);
     g.stream = s->stream;
     g.sel[B_K1] = static_sel(s, 1, 2);
@@ -3066,6 +3083,7 @@ extern "C" int fsibe_batch_create(const fsibe_params *params, int n, const doubl
       g = &b->groups.back();
       g->nx = s->nx; g->ny = s->ny; g->H = s->H; g->nf5 = nf5; g->pres = s->pressure_on;
     }
+    g->px = g->px || s->penal_on;
     g->mem.push_back(i);
     g->nprim_cap = std::max(g->nprim_cap, s->nprim_cap);
   }
@@ -3164,7 +3182,8 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
   }
   {  // K4
     ColsAb2Args &a = reinterpret_cast<ColsAb2Args *>(A + g.off[B_K4])[j];
-    a.Rc = s->T[2]; a.Rpx = s->T[3]; a.W0 = s->W[n0]; a.NL1 = s->NL[n1]; a.NL0 = s->NL[n0]; a.W1 = s->W[n1];
+    a.Rc = s->T[2]; a.Rpx = s->penal_on ? s->T[3] : nullptr; a.W0 = s->W[n0]; a.NL1 = s->NL[n1]; a.NL0 = s->NL[n0];
+    a.W1 = s->W[n1];
     a.s = s->dscal; a.explicit_step = 0; a.nblk = s->H / g.tx[B_K4]; a.c = s->sc;
     maxblk[B_K4] = std::max(maxblk[B_K4], a.nblk);
   }
@@ -3268,7 +3287,7 @@ static int batch_stages(fsibe_batch *b, const std::vector<std::array<int, B_NST>
           break;
         case B_K4:
           CHK(with_cols_sel<2>(s0->py, s0->H, g.sel[B_K4], [&](auto F, int nt, int TX) -> int {
-            SLAUNCH(kb_cols_ab2<decltype(F)>, gx, nm, nt, sizeof(double2) * 2 * TX * padlen(s0->ny), g.stream, mode,
+            SLAUNCH(kb_cols_ab2<decltype(F)>, gx, nm, nt, k4_lds(s0->ny, g.px, TX), g.stream, mode,
                     reinterpret_cast<const ColsAb2Args *>(arr(g, B_K4)), F, s0->twy);
             return FSIBE_OK;
           }));
@@ -3579,7 +3598,7 @@ extern "C" int fsibe_batch_set_tiles(fsibe_batch *b, int group, int stage, int c
   BatchGroup &g = b->groups[group];
   const int bm = (k == B_K4 || k == B_K7) ? 2 : 1;
   const ColSel c{ct ? 1 : 0, tx};
-  if (!col_sel_valid(g.ny, g.H, bm, c))
+  if (!(k == B_K4 ? k4_sel_valid(g.ny, g.H, g.px, c) : col_sel_valid(g.ny, g.H, bm, c)))
     return set_err(FSIBE_EARG, "%s plan with %d columns per tile is not a launch choice for %d x %d",
                    c.ct ? "compile-time" : "runtime", tx, g.nx, g.ny);
   for (auto &q : b->groups) HIPCHK(hipStreamSynchronize(q.stream));

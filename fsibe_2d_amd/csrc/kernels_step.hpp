@@ -515,9 +515,11 @@ __device__ __forceinline__ void cols_ab2_body(const double2 *__restrict__ Rc, co
   extern __shared__ double2 lds[];
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
   const int kx0 = xcd_remap(bid, nblk) * TX;
-  const int TT = 2 * TX;
-  // Rpx null: no penalisation anywhere in the simulation, its spectrum is 0
+  // Rpx null: no penalisation anywhere in the simulation, its spectrum is 0,
+  // and the tile holds Rc alone (half the LDS, so that 4096-point columns
+  // fit two columns per tile)
   const bool px_on = Rpx != nullptr;
+  const int TT = px_on ? 2 * TX : TX;
   for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
     lds[pidx(y) * TT + cc] = Rc[(size_t)y * H + kx0 + cc];
@@ -525,7 +527,7 @@ __device__ __forceinline__ void cols_ab2_body(const double2 *__restrict__ Rc, co
   }
   __syncthreads();
   if (px_on) F.template fft<false, 2, 2>(lds, twy, tid, nt);
-  else F.template fft<false, 1, 2>(lds, twy, tid, nt);
+  else F.template fft<false, 1, 1>(lds, twy, tid, nt);
   const double b10 = s->b10, b11 = s->b11, cx1 = s->cx1, cy1 = s->cy1, cx0 = s->cx0, cy0 = s->cy0;
   const int hy = ny >> 1;
   for (int e = tid; e < ny * TX; e += nt) {
