@@ -943,6 +943,9 @@ struct fsibe_sim {
   size_t res_stride = 0;
   int slot = 0;
   bool res_pending = false;  // pipelined run: results of the previous step not yet read
+  // coupled run: the slot whose row sums (energies, cal_drag) are read after
+  // the next step's launches, off the beam solve's path (-1: none)
+  int rows_pending = -1;
   bool pipe = false;         // record ev_dt after the nonlinear-term kernel
   hipEvent_t ev_dt;
   hipEvent_t ev_step;        // after a step's last kernel (iFLUSI=1 loop)
@@ -2190,13 +2193,9 @@ static void sync_last(fsibe_sim *s) {
   r.ibes_iter = b.ibes_iter;
 }
 
-// Read result slot k (its kernels have finished) into s->last: dt1, u_max;
-// the fluid step's row sums (energies cal_energy NavierStokes.f90:236-256,
-// cal_drag Force / Force_unst :567-589, BeamSurfaceIntegral drag/lift
-// BeamForces.f90:483-486) added over rows in order; the force terms of
-// k_forces (Force_press :261-267, surface integrals :485-496) added over
-// units in order.  Each part is read once per step.
-static void parse_results(fsibe_sim *s, int k) {
+// Result slot k's head: dt1, u_max and the force kernel's terms, the part
+// the beam solve needs (GetForces' p_beam, tau_beam)
+static void parse_head(fsibe_sim *s, int k) {
   const int nsb = nodes_all(s), nb = s->nb, nr = std::max(nb, 1);
   const char *base = s->h_res + (size_t)k * s->res_stride;
   const StepOutDev *o = reinterpret_cast<const StepOutDev *>(base);
@@ -2205,6 +2204,45 @@ static void parse_results(fsibe_sim *s, int k) {
   r.u_max = o->u_max;
   r.status = 0;
   r.ibes_iter = 0;
+  if (s->slot_forces[k]) {
+    s->slot_forces[k] = false;
+    const double *part = reinterpret_cast<const double *>(base + parts_off(s));
+    const int per = s->slot_units[k] / nr;
+    for (int ib = 0; ib < nr; ++ib) {
+      fsibe_step_out &rb = s->last_b[ib];
+      double tot[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int u = ib * per; u < (ib + 1) * per; ++u)
+        for (int i = 0; i < 8; ++i) tot[i] += part[8 * u + i];
+      rb.Force_press[0] = tot[0];
+      rb.Force_press[1] = tot[1];
+      rb.fx_int = tot[0];
+      rb.fy_int = tot[1];
+      if (s->fg.bsi_on) {
+        rb.bsi[2] = tot[2] + tot[4] + tot[6];
+        rb.bsi[3] = tot[3] + tot[5] + tot[7];
+        rb.bsi[4] = tot[2]; rb.bsi[5] = tot[4]; rb.bsi[6] = tot[6];
+        rb.bsi[7] = tot[3]; rb.bsi[8] = tot[5]; rb.bsi[9] = tot[7];
+      }
+    }
+    const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
+    for (int i = 0; i < s->nb * s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
+  }
+  for (int ib = 0; ib < nr; ++ib) {
+    fsibe_step_out &rb = s->last_b[ib];
+    rb.dt1 = r.dt1; rb.u_max = r.u_max; rb.status = 0; rb.ibes_iter = 0;
+  }
+}
+
+// Result slot k's row sums of the fluid step (energies cal_energy
+// NavierStokes.f90:236-256, cal_drag Force / Force_unst :567-589,
+// BeamSurfaceIntegral drag/lift BeamForces.f90:483-486) added over rows in
+// order; completes s->last (after parse_head of the same slot and the beam
+// solve, whose iteration count it keeps)
+static void parse_rows(fsibe_sim *s, int k) {
+  const int nb = s->nb, nr = std::max(nb, 1);
+  const char *base = s->h_res + (size_t)k * s->res_stride;
+  const StepOutDev *o = reinterpret_cast<const StepOutDev *>(base);
+  fsibe_step_out &r = s->last;
   const double dxdy = s->dx * s->dy;
   if (s->slot_fluid[k]) {
     s->slot_fluid[k] = false;
@@ -2246,38 +2284,30 @@ static void parse_results(fsibe_sim *s, int k) {
       s->jsum_t += o->dt1;
     }
   }
-  if (s->slot_forces[k]) {
-    s->slot_forces[k] = false;
-    const double *part = reinterpret_cast<const double *>(base + parts_off(s));
-    const int per = s->slot_units[k] / nr;
-    for (int ib = 0; ib < nr; ++ib) {
-      fsibe_step_out &rb = s->last_b[ib];
-      double tot[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int u = ib * per; u < (ib + 1) * per; ++u)
-        for (int i = 0; i < 8; ++i) tot[i] += part[8 * u + i];
-      rb.Force_press[0] = tot[0];
-      rb.Force_press[1] = tot[1];
-      rb.fx_int = tot[0];
-      rb.fy_int = tot[1];
-      if (s->fg.bsi_on) {
-        rb.bsi[2] = tot[2] + tot[4] + tot[6];
-        rb.bsi[3] = tot[3] + tot[5] + tot[7];
-        rb.bsi[4] = tot[2]; rb.bsi[5] = tot[4]; rb.bsi[6] = tot[6];
-        rb.bsi[7] = tot[3]; rb.bsi[8] = tot[5]; rb.bsi[9] = tot[7];
-      }
-    }
-    const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
-    for (int i = 0; i < s->nb * s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
-  }
   // each beam's record carries the step's common fields; s->last the first
   // beam's per-beam fields
   for (int ib = 0; ib < nr; ++ib) {
     fsibe_step_out &rb = s->last_b[ib];
-    rb.dt1 = r.dt1; rb.u_max = r.u_max; rb.status = 0; rb.ibes_iter = 0;
     for (int i = 0; i < 6; ++i) rb.energies[i] = r.energies[i];
     rb.bsi[0] = r.bsi[0]; rb.bsi[1] = r.bsi[1];
   }
   sync_last(s);
+}
+
+// Read result slot k (its kernels have finished) into s->last: the head
+// (dt1, u_max, the force terms of k_forces, Force_press :261-267 and the
+// surface integrals :485-496, added over units in order, and p_beam /
+// tau_beam), then the row sums.  Each part is read once per step.
+static void parse_results(fsibe_sim *s, int k) {
+  parse_head(s, k);
+  parse_rows(s, k);
+}
+
+// the coupled loop's deferred row sums of the previous step, if any
+static void flush_rows(fsibe_sim *s) {
+  if (s->rows_pending < 0) return;
+  parse_rows(s, s->rows_pending);
+  s->rows_pending = -1;
 }
 
 static int fetch_results(fsibe_sim *s) {
@@ -2458,11 +2488,18 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     if (p.iFLUSI == 1 && p.iBeam > 0 && s->time > p.T_release)
       for (int ib = 0; ib < s->nb; ++ib)
         if (s->solid[ib]) (void)s->solid[ib]->sv.predict(s->time, s->solid[ib]->b);
+    // the previous step's row sums while this step's kernels run
+    flush_rows(s);
     CHK(wait_event(s->ev_step, "the step's results (pressure and force kernels)"));
     s->upload_pending = false;
-    parse_results(s, s->slot);
+    // the beam solve needs the head; the row sums are read during the next
+    // step, which writes the other slot
+    parse_head(s, s->slot);
     s->dt1 = s->last.dt1;
+    s->rows_pending = s->slot;
+    s->slot ^= 1;
   } else if (!pipe) {
+    flush_rows(s);
     const double th0 = now_ms();
     CHK(fetch_results(s));
     prof_host(s, P_FORCES, th0);
@@ -2533,6 +2570,7 @@ extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
   s->k1_ready = false;
   s->k2_ready = false;
   s->pipe = false;
+  flush_rows(s);
   if (s->res_pending) {
     HIPCHK(hipStreamSynchronize(s->stream));
     s->upload_pending = false;
