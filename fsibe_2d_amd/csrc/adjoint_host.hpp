@@ -142,7 +142,7 @@ static int snap_restore(fsibe_sim *s, SimSnap &q) {
   s->slot_forces[0] = s->slot_forces[1] = false;
   s->k1_ready = false;
   s->k2_ready = false;
-  s->upload_pending = false;
+  s->upload_pending = 0;
   return FSIBE_OK;
 }
 

@@ -313,6 +313,23 @@ static thread_local int t_prof_launches = 0;
     HIPCHK(hipGetLastError());                                                                          \
   } while (0)
 
+// KLAUNCH whose kernel also records `ev` at its end, carried by its own
+// dispatch: an event recorded between two kernels is a marker that stalls
+// the queue until the first has drained (~5.6 us per marker measured in the
+// step's kernel traces).  Under kernel timing the launch keeps KLAUNCH's
+// events and `ev` is recorded after it.
+#define KLAUNCH_EV(KT, grid, nt, lds, st, ev, ...)                                                      \
+  do {                                                                                                  \
+    if (t_prof_start) {                                                                                 \
+      KLAUNCH(KT, grid, nt, lds, st, __VA_ARGS__);                                                      \
+      HIPCHK(hipEventRecord(ev, st));                                                                   \
+    } else {                                                                                            \
+      CHK(prep_lds(KT, lds));                                                                           \
+      hipExtLaunchKernelGGL(KT, dim3(grid), dim3(nt), (std::uint32_t)(lds), st, nullptr, ev, 0u, __VA_ARGS__); \
+      HIPCHK(hipGetLastError());                                                                        \
+    }                                                                                                   \
+  } while (0)
+
 static int env_int(const char *name, int dflt) {
   const char *v = getenv(name);
   return v ? atoi(v) : dflt;
@@ -918,8 +935,9 @@ struct fsibe_sim {
   FlowParams fp;
   FftPlan px, py;
   hipStream_t stream;
-  hipEvent_t ev_upload;
-  bool upload_pending = false;
+  // bit b: a copy from host staging buffer b may still be running (cleared
+  // by every wait that follows the copies)
+  int upload_pending = 0;
   double2 *twx, *twy;
   double2 *W[2], *NL[2], *T[7];
   double *u1, *u2, *vor_mean, *ms, *wallrow;
@@ -932,9 +950,14 @@ struct fsibe_sim {
   double *mask_m, *mask_vx, *mask_vy;
   int *mask_win = nullptr;  // winning primitive per mask pixel (adjoint runs only)
   double *fb_p, *fb_sa, *fb_sb;
-  // per-step upload area: prims | targets | node data | circle data | vals
-  char *h_stage, *d_stage;
+  // per-step upload area: prims | targets | node data | circle data | vals.
+  // Two host buffers, written in turn: the host builds step n+1's geometry
+  // in one while the copy of step n may still read the other, so no event
+  // marks a copy's end (the loops' own waits follow every copy).
+  char *h_stage = nullptr, *d_stage;
   char *m_stage = nullptr;  // device address of h_stage (mapped)
+  char *h_stage_b[2] = {nullptr, nullptr}, *m_stage_b[2] = {nullptr, nullptr};
+  int stage_cur = 0;        // the buffer h_stage / m_stage point to
   size_t stage_bytes;
   // two result slots (StepOutDev, p_beam[nsb], tau_beam[nsb]) in pinned,
   // host-coherent memory that the kernels write directly; d_res is the
@@ -1040,8 +1063,11 @@ static void prof_host(fsibe_sim *s, int k, double t0) {
 }
 
 // the host-written part of the geometry record (primitives, beam block)
-// into device memory, on the step's stream; then ev_upload marks the host
-// record free again
+// into device memory, on the step's stream; the builders then write the
+// other host buffer.  No event marks the copy's end: an event between two
+// kernels stalls the queue until the first has drained (~5 us here, before
+// the mask kernel could start), and the loops wait for a later kernel of
+// the same step anyway.
 static int upload_geometry(fsibe_sim *s) {
   const int n = (int)(s->off_tg / sizeof(double2));
   if (n > 0) {
@@ -1050,8 +1076,20 @@ static int upload_geometry(fsibe_sim *s) {
                                               reinterpret_cast<double2 *>(s->d_stage), n);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipEventRecord(s->ev_upload, s->stream));
-  s->upload_pending = true;
+  s->upload_pending |= 1 << s->stage_cur;
+  s->stage_cur ^= 1;
+  s->h_stage = s->h_stage_b[s->stage_cur];
+  s->m_stage = s->m_stage_b[s->stage_cur];
+  return FSIBE_OK;
+}
+
+// before the builders write h_stage: a copy still reading it (two uploads
+// without a wait in between, e.g. repeated fsibe_create_mask calls) ends first
+static int stage_writable(fsibe_sim *s) {
+  if (s->upload_pending >> s->stage_cur & 1) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->upload_pending = 0;
+  }
   return FSIBE_OK;
 }
 
@@ -1293,9 +1331,9 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   if (s->spec) fsibe_spectral_destroy(s->spec);
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
-  if (s->h_stage) (void)hipHostFree(s->h_stage);
+  for (int b = 0; b < 2; ++b)
+    if (s->h_stage_b[b]) (void)hipHostFree(s->h_stage_b[b]);
   if (s->h_res) (void)hipHostFree(s->h_res);
-  (void)hipEventDestroy(s->ev_upload);
   (void)hipEventDestroy(s->ev_dt);
   (void)hipEventDestroy(s->ev_step);
   for (int k = 0; k < P_NK; ++k)
@@ -1379,8 +1417,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     }                                                                                   \
   } while (0)
   SIMCHK(hipStreamCreate(&s->stream));
-  SIMCHK(hipEventCreateWithFlags(&s->ev_upload, hipEventDisableTiming));
-  SIMCHK(hipEventCreateWithFlags(&s->ev_dt, hipEventDisableTiming));
+  SIMCHK(hipEventCreate(&s->ev_dt));  // bound to K2's dispatch in the pipelined loop
   SIMCHK(hipEventCreateWithFlags(&s->ev_step, hipEventDisableTiming));
   const size_t N = (size_t)s->nx * s->ny, NH = (size_t)s->H * s->ny;
   auto tx = twiddles(s->nx), ty = twiddles(s->ny);
@@ -1442,8 +1479,12 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
     s->off_val = al(sizeof(double) * ntg_max);
     s->stage_bytes = off;
   }
-  SIMCHK(hipHostMalloc((void **)&s->h_stage, s->stage_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-  SIMCHK(hipHostGetDevicePointer((void **)&s->m_stage, s->h_stage, 0));
+  for (int b = 0; b < 2; ++b) {
+    SIMCHK(hipHostMalloc((void **)&s->h_stage_b[b], s->stage_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    SIMCHK(hipHostGetDevicePointer((void **)&s->m_stage_b[b], s->h_stage_b[b], 0));
+  }
+  s->h_stage = s->h_stage_b[0];
+  s->m_stage = s->m_stage_b[0];
   SIMCHK(hipMalloc(&s->d_stage, s->stage_bytes));
   SIMCHK(hipMemset(s->d_stage, 0, s->stage_bytes));
   s->ms = nullptr;
@@ -1859,10 +1900,7 @@ static int create_mask_impl(fsibe_sim *s, double time, const double *x, const do
   if (!s) return set_err(FSIBE_EARG, "null sim");
   if (s->ns > 0 && (!x || !y || !vx || !vy || !theta)) return set_err(FSIBE_EARG, "null beam array");
   std::vector<double>().swap(s->restored);
-  if (s->upload_pending) {
-    HIPCHK(hipEventSynchronize(s->ev_upload));
-    s->upload_pending = false;
-  }
+  CHK(stage_writable(s));
   for (int i = 0; i < s->nb * s->ns; ++i) {
     s->bx[i] = x[i]; s->by[i] = y[i]; s->bvx[i] = vx[i]; s->bvy[i] = vy[i]; s->btheta[i] = theta[i];
   }
@@ -2050,6 +2088,18 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
                 (int)explicit_step, res_dev(s, s->slot));
         return FSIBE_OK;
       }));
+      if (s->pipe) HIPCHK(hipEventRecord(s->ev_dt, s->stream));
+    } else if (s->pipe) {
+      // ev_dt at K2's end: the host reads dt1 and then the previous step's
+      // results (kernels before this one in the stream)
+      CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
+        KLAUNCH_EV(k_rows_nl<decltype(F)>, s->ny, nt, lds, s->stream, s->ev_dt, s->T[0], s->T[1], s->u1, s->u2,
+                   s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
+                   F, s->twx, s->sc,
+                   s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), 0,
+                   s->ny, 0, 1, s->penal_on ? s->rowpen : nullptr);
+        return FSIBE_OK;
+      }));
     } else {
       CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
         KLAUNCH(k_rows_nl<decltype(F)>, s->ny, nt, lds, s->stream, s->T[0], s->T[1], s->u1, s->u2,
@@ -2061,7 +2111,6 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
       }));
     }
     prof_end(s, P_ROWS_NL);
-    if (s->pipe) HIPCHK(hipEventRecord(s->ev_dt, s->stream));
   }
   if (s->tape) CHK(tape_fluid(s, explicit_step));
   if (s->p.iMeanVelocity == 11) {
@@ -2312,7 +2361,7 @@ static void flush_rows(fsibe_sim *s) {
 
 static int fetch_results(fsibe_sim *s) {
   HIPCHK(hipStreamSynchronize(s->stream));
-  s->upload_pending = false;
+  s->upload_pending = 0;
   prof_collect(s);
   parse_results(s, s->slot);
   return FSIBE_OK;
@@ -2463,7 +2512,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
                          s->btheta.data(), alpha, LE, !split));
   } else if (p.iBeam > 0) {
     // soft-startup factor follows time
-    if (s->upload_pending) { HIPCHK(hipEventSynchronize(s->ev_upload)); s->upload_pending = false; }
+    CHK(stage_writable(s));
     CHK(build_geometry(s, s->time));
     CHK(upload_geometry(s));
   }
@@ -2491,7 +2540,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     // the previous step's row sums while this step's kernels run
     flush_rows(s);
     CHK(wait_event(s->ev_step, "the step's results (pressure and force kernels)"));
-    s->upload_pending = false;
+    s->upload_pending = 0;
     // the beam solve needs the head; the row sums are read during the next
     // step, which writes the other slot
     parse_head(s, s->slot);
@@ -2506,7 +2555,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     s->dt1 = s->last.dt1;
   } else {
     CHK(wait_event(s->ev_dt, "the step's time step dt1 (nonlinear-term kernel)"));
-    s->upload_pending = false;  // the geometry upload precedes the nonlinear-term kernel
+    s->upload_pending = 0;  // the geometry upload precedes the nonlinear-term kernel
     s->dt1 = *reinterpret_cast<volatile const double *>(s->h_res + (size_t)s->slot * s->res_stride);
     if (s->res_pending) parse_results(s, s->slot ^ 1);  // finished before this step started
     s->res_pending = true;
@@ -2573,7 +2622,7 @@ extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
   flush_rows(s);
   if (s->res_pending) {
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->upload_pending = false;
+    s->upload_pending = 0;
     parse_results(s, s->slot ^ 1);
     s->res_pending = false;
   }
@@ -3495,7 +3544,7 @@ static int group_finish(fsibe_batch *b, int gi) {
   std::vector<std::string> errs(g.mem.size());
   b->pool->run((int)g.mem.size(), [&](int j) {
     fsibe_sim *s = b->m[g.mem[j]];
-    s->upload_pending = false;
+    s->upload_pending = 0;
     parse_results(s, s->slot);
     s->dt1 = s->last.dt1;
     s->slot ^= 1;
