@@ -1418,7 +1418,7 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   } while (0)
   SIMCHK(hipStreamCreate(&s->stream));
   SIMCHK(hipEventCreate(&s->ev_dt));  // bound to K2's dispatch in the pipelined loop
-  SIMCHK(hipEventCreateWithFlags(&s->ev_step, hipEventDisableTiming));
+  SIMCHK(hipEventCreate(&s->ev_step));  // bound to the force kernel's dispatch in the coupled loop
   const size_t N = (size_t)s->nx * s->ny, NH = (size_t)s->H * s->ny;
   auto tx = twiddles(s->nx), ty = twiddles(s->ny);
   SIMCHK(hipMalloc(&s->twx, sizeof(double2) * s->nx));
@@ -2209,7 +2209,10 @@ static int evolve(fsibe_sim *s, double time, double dt0, bool explicit_step) {
   return evolve_pressure_box(s);
 }
 
-static int launch_forces(fsibe_sim *s) {
+// end_ev (optional): recorded at the force kernel's end by its own dispatch;
+// *recorded tells whether it was (no launch, or kernel timing: it was not)
+static int launch_forces(fsibe_sim *s, hipEvent_t end_ev = nullptr, bool *recorded = nullptr) {
+  if (recorded) *recorded = false;
   if (!s->forces_on || !s->forces_ready || !s->press_box_valid) return FSIBE_OK;
   const double *tg = reinterpret_cast<const double *>(s->d_stage + s->off_tg);
   double *val = reinterpret_cast<double *>(s->d_stage + s->off_val);
@@ -2219,10 +2222,18 @@ static int launch_forces(fsibe_sim *s) {
   if (s->ntg != s->nb * (6 * s->fg.ns + 3 * 2 * s->fg.nc) || s->fg.nc > kNC)
     return set_err(FSIBE_ESTATE, "interpolation targets do not match the beam");
   PROF_BEGIN(s, P_INTERP);
-  k_forces<<<units, kForceThreads, 0, s->stream>>>(
-      tg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, reinterpret_cast<const double *>(s->d_stage + s->off_nd),
-      reinterpret_cast<const double *>(s->d_stage + s->off_cd), pbeam_dev(s, s->slot),
-      pbeam_dev(s, s->slot) + nodes_all(s), parts_dev(s, s->slot));
+  const double *nd = reinterpret_cast<const double *>(s->d_stage + s->off_nd);
+  const double *cd = reinterpret_cast<const double *>(s->d_stage + s->off_cd);
+  if (end_ev && !t_prof_start) {
+    hipExtLaunchKernelGGL(k_forces, dim3(units), dim3(kForceThreads), 0u, s->stream, nullptr, end_ev, 0u, tg, s->fb_p,
+                          s->fb_sa, s->fb_sb, s->fg, val, nd, cd, pbeam_dev(s, s->slot),
+                          pbeam_dev(s, s->slot) + nodes_all(s), parts_dev(s, s->slot));
+    *recorded = true;
+  } else {
+    k_forces<<<units, kForceThreads, 0, s->stream>>>(tg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, nd, cd,
+                                                     pbeam_dev(s, s->slot), pbeam_dev(s, s->slot) + nodes_all(s),
+                                                     parts_dev(s, s->slot));
+  }
   HIPCHK(hipGetLastError());
   prof_end(s, P_INTERP);
   s->slot_forces[s->slot] = true;
@@ -2524,10 +2535,13 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
   } else {
     CHK(evolve(s, s->time, s->startup_done ? s->dt0 : 0.0, !s->startup_done));
   }
-  if (s->forces_on) CHK(launch_forces(s));
+  // iFLUSI=1 with the next step ahead: ev_step at the force kernel's end,
+  // carried by its dispatch (a marker after it would idle the queue first)
+  bool step_recorded = false;
+  if (s->forces_on) CHK(launch_forces(s, (!pipe && k1_ahead) ? s->ev_step : nullptr, &step_recorded));
   if (!pipe && k1_ahead) {
     // iFLUSI=1: the next step's K1 runs while the host solves the beam
-    HIPCHK(hipEventRecord(s->ev_step, s->stream));
+    if (!step_recorded) HIPCHK(hipEventRecord(s->ev_step, s->stream));
     CHK(launch_k1(s, s->n1));
     s->k1_ready = true;
     s->k1_level = s->n1;
