@@ -958,6 +958,12 @@ struct fsibe_sim {
   char *m_stage = nullptr;  // device address of h_stage (mapped)
   char *h_stage_b[2] = {nullptr, nullptr}, *m_stage_b[2] = {nullptr, nullptr};
   int stage_cur = 0;        // the buffer h_stage / m_stage point to
+  // coupled loop: the next step's record copy, queued before the beam solve
+  // behind a gate in host memory that the upload opens (k_stage_copy_gated)
+  long long *h_gate = nullptr, *m_gate = nullptr;
+  long long gate_seq = 0, gate_ticks = 0;
+  long long gate_opened = 0;  // the last gate an upload opened (checked after the step's wait)
+  bool gate_armed = false;
   size_t stage_bytes;
   // two result slots (StepOutDev, p_beam[nsb], tau_beam[nsb]) in pinned,
   // host-coherent memory that the kernels write directly; d_res is the
@@ -1070,7 +1076,12 @@ static void prof_host(fsibe_sim *s, int k, double t0) {
 // the same step anyway.
 static int upload_geometry(fsibe_sim *s) {
   const int n = (int)(s->off_tg / sizeof(double2));
-  if (n > 0) {
+  if (s->gate_armed) {
+    // the copy from this buffer is queued: the record is written, open it
+    __atomic_store_n(s->h_gate, s->gate_seq, __ATOMIC_RELEASE);
+    s->gate_armed = false;
+    s->gate_opened = s->gate_seq;
+  } else if (n > 0) {
     const int blocks = std::min(64, (n + 255) / 256);
     k_stage_copy<<<blocks, 256, 0, s->stream>>>(reinterpret_cast<const double2 *>(s->m_stage),
                                               reinterpret_cast<double2 *>(s->d_stage), n);
@@ -1081,6 +1092,42 @@ static int upload_geometry(fsibe_sim *s) {
   s->h_stage = s->h_stage_b[s->stage_cur];
   s->m_stage = s->m_stage_b[s->stage_cur];
   return FSIBE_OK;
+}
+
+// Coupled loop, before the beam solve: queue the next step's record copy
+// (from the buffer the builders write next) behind the gate, so that after
+// the solve the host only opens the gate instead of launching the copy into
+// an idle queue.  The stream's later work waits behind it, so every path
+// that leaves the loop opens it (open_gate).
+static int arm_upload(fsibe_sim *s) {
+  const int n = (int)(s->off_tg / sizeof(double2));
+  if (n <= 0 || s->gate_armed || !s->h_gate || (s->upload_pending >> s->stage_cur & 1)) return FSIBE_OK;
+  const long long seq = ++s->gate_seq;
+  const int blocks = std::min(64, (n + 255) / 256);
+  k_stage_copy_gated<<<blocks, 256, 0, s->stream>>>(reinterpret_cast<const double2 *>(s->m_stage),
+                                                    reinterpret_cast<double2 *>(s->d_stage), n, s->m_gate, seq,
+                                                    s->gate_ticks);
+  HIPCHK(hipGetLastError());
+  s->gate_armed = true;
+  return FSIBE_OK;
+}
+
+// after a wait that follows the copy: a gate that timed out copied a stale
+// record, so the step's mask is wrong
+static int gate_check(fsibe_sim *s) {
+  if (s->gate_opened && __atomic_load_n(s->h_gate + 1, __ATOMIC_ACQUIRE) == s->gate_opened)
+    return set_err(FSIBE_EHIP, "the geometry copy stopped waiting for its record (2 s) and copied a stale one");
+  return FSIBE_OK;
+}
+
+// a queued copy that no upload will open: open it (it copies a stale
+// record, overwritten by the next upload before any mask kernel reads it);
+// the builders wait for it before writing that buffer again
+static void open_gate(fsibe_sim *s) {
+  if (!s->gate_armed) return;
+  __atomic_store_n(s->h_gate, s->gate_seq, __ATOMIC_RELEASE);
+  s->gate_armed = false;
+  s->upload_pending |= 1 << s->stage_cur;
 }
 
 // before the builders write h_stage: a copy still reading it (two uploads
@@ -1323,6 +1370,7 @@ extern "C" void *fsibe_sim_stream(fsibe_sim *s) { return s ? (void *)s->stream :
 
 extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
   if (!s) return FSIBE_OK;
+  open_gate(s);
   (void)hipStreamSynchronize(s->stream);
   void *ptrs[] = {s->twx, s->twy, s->W[0], s->W[1], s->NL[0], s->NL[1], s->T[0], s->T[1], s->T[2], s->T[3],
                   s->T[4], s->T[5], s->T[6], s->u1, s->u2, s->vor_mean, s->ms, s->wallrow,
@@ -1333,6 +1381,7 @@ extern "C" int fsibe_sim_destroy(fsibe_sim *s) {
     if (q) (void)hipFree(q);
   for (int b = 0; b < 2; ++b)
     if (s->h_stage_b[b]) (void)hipHostFree(s->h_stage_b[b]);
+  if (s->h_gate) (void)hipHostFree(s->h_gate);
   if (s->h_res) (void)hipHostFree(s->h_res);
   (void)hipEventDestroy(s->ev_dt);
   (void)hipEventDestroy(s->ev_step);
@@ -1485,6 +1534,18 @@ extern "C" int fsibe_sim_create(const fsibe_params *pp, fsibe_sim **out) {
   }
   s->h_stage = s->h_stage_b[0];
   s->m_stage = s->m_stage_b[0];
+  SIMCHK(hipHostMalloc((void **)&s->h_gate, 256, hipHostMallocMapped | hipHostMallocCoherent));
+  SIMCHK(hipHostGetDevicePointer((void **)&s->m_gate, s->h_gate, 0));
+  s->h_gate[0] = 0;
+  s->h_gate[1] = 0;
+  {
+    int khz = 0, dev = 0;
+    SIMCHK(hipGetDevice(&dev));
+    SIMCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    s->gate_ticks = 2000LL * (long long)std::max(khz, 1000);  // 2 s of the wall clock
+    // a shorter bound (wall-clock ticks) lets a test take the timeout path
+    if (const char *e = getenv("FSIBE_GATE_TICKS")) s->gate_ticks = std::max(0LL, atoll(e));
+  }
   SIMCHK(hipMalloc(&s->d_stage, s->stage_bytes));
   SIMCHK(hipMemset(s->d_stage, 0, s->stage_bytes));
   s->ms = nullptr;
@@ -2546,6 +2607,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     s->k1_ready = true;
     s->k1_level = s->n1;
     CHK(launch_k2_ahead(s));
+    CHK(arm_upload(s));
     // the beam solver's predictor needs only the state at t_n: it runs while
     // the step's fluid kernels finish
     if (p.iFLUSI == 1 && p.iBeam > 0 && s->time > p.T_release)
@@ -2555,6 +2617,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     flush_rows(s);
     CHK(wait_event(s->ev_step, "the step's results (pressure and force kernels)"));
     s->upload_pending = 0;
+    CHK(gate_check(s));
     // the beam solve needs the head; the row sums are read during the next
     // step, which writes the other slot
     parse_head(s, s->slot);
@@ -2565,6 +2628,7 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
     flush_rows(s);
     const double th0 = now_ms();
     CHK(fetch_results(s));
+    CHK(gate_check(s));
     prof_host(s, P_FORCES, th0);
     s->dt1 = s->last.dt1;
   } else {
@@ -2630,6 +2694,7 @@ extern "C" int fsibe_run(fsibe_sim *s, int nsteps, fsibe_step_out *out) {
     if (rc != FSIBE_OK) break;
     if (s->last.status == 1) break;  // continue_timestep = .false.
   }
+  open_gate(s);
   s->k1_ready = false;
   s->k2_ready = false;
   s->pipe = false;

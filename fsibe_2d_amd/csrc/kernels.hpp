@@ -377,6 +377,27 @@ __global__ void k_stage_copy(const double2 *__restrict__ src, double2 *__restric
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
+// The same copy, queued before the host has written src: each workgroup
+// waits until the host stores a value >= seq into gate[0] (host memory),
+// then copies.  The wait is bounded: after max_ticks of the constant-rate
+// wall clock it stores seq into gate[1], which the host checks after the
+// step, and copies anyway.
+__global__ void k_stage_copy_gated(const double2 *__restrict__ src, double2 *__restrict__ dst, int n,
+                                   long long *gate, long long seq, long long max_ticks) {
+  if (threadIdx.x == 0) {
+    const long long t0 = wall_clock64();
+    while (__hip_atomic_load(gate, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) < seq) {
+      if (wall_clock64() - t0 > max_ticks) {
+        __hip_atomic_store(gate + 1, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 // Several staging copies in one launch: segment blockIdx.y copies n 16-byte
 // words from src (mapped host memory) to dst (device memory).
 struct StageSeg {

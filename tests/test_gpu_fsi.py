@@ -119,6 +119,26 @@ def test_coupled_loop_long_horizon_cfl(oracle_mod):
     g.close()
 
 
+def test_geometry_gate_timeout_is_an_error(monkeypatch):
+    """the coupled loop queues the next step's geometry copy before the
+    beam solve, behind a bounded wait on a gate in host memory that the
+    host opens once the record is written.  With the bound set to 0
+    wall-clock ticks (FSIBE_GATE_TICKS, read when a simulation is created)
+    the copy stops waiting before the host opens the gate (the 128-node
+    beam solve takes longer than the 64^2 kernels ahead of the copy), and
+    the loop reports the stale record as an error instead of stepping on
+    with it; every kernel still ends normally"""
+    from fsibe_2d_amd import FsibeError, Params
+    from fsibe_2d_amd.solver import FluidSim
+
+    monkeypatch.setenv("FSIBE_GATE_TICKS", "0")
+    p = Params(nx=64, ny=64, ns=128, iFLUSI=1, dt_fixed=2e-4)
+    g = FluidSim(p)
+    with pytest.raises(FsibeError, match="stale"):
+        g.run(10)
+    g.close()
+
+
 def test_coupled_loop_in_two_calls(oracle_mod):
     """fsibe_run(10) then fsibe_run(15) is the same run as fsibe_run(25)"""
     from fsibe_2d_amd import Params
