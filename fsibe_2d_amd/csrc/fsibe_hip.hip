@@ -1894,6 +1894,22 @@ static int build_geometry(fsibe_sim *s, double time) {
   return build_force_geometry(s, time);
 }
 
+// rows of Rpx (K2's penalisation spectrum, the input of K4) that may be
+// nonzero: the mask box's rows (K2 writes 0 on the others), or every row
+// when the iSponge = 10 term adds to it
+static void rpx_rows(const fsibe_sim *s, const MaskBox &mb, int &py0, int &py1) {
+  if (s->p.iSponge == 10) {
+    py0 = 0;
+    py1 = s->ny - 1;
+  } else if (mb.x1 >= mb.x0) {
+    py0 = mb.y0;
+    py1 = mb.y1;
+  } else {
+    py0 = 1;
+    py1 = 0;
+  }
+}
+
 static MaskBox mask_box(const fsibe_sim *s) {
   MaskBox m;
   m.x0 = s->mbox.x0; m.x1 = s->mbox.x1; m.y0 = s->mbox.y0; m.y1 = s->mbox.y1; m.w = s->mbox.w();
@@ -2198,10 +2214,12 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     PROF_BEGIN(s, P_COLS_AB2);
     CHK(with_cols_sel<2>(s->py, s->H, sim_sel(s, 1, 1), [&](auto F, int nt, int TX) -> int {
       const size_t lds = k4_lds(s->ny, s->penal_on, TX);
+      int py0, py1;
+      rpx_rows(s, mb, py0, py1);
       // without penalisation K2 leaves Rpx unwritten (it is 0) and K4 skips it
       KLAUNCH(k_cols_ab2<decltype(F)>, s->H / TX, nt, lds, s->stream, s->T[2], s->penal_on ? s->T[3] : nullptr,
               s->W[n0], s->NL[n1],
-              s->NL[n0], s->W[n1], s->dscal, (int)explicit_step, F, s->twy, s->sc);
+              s->NL[n0], s->W[n1], s->dscal, (int)explicit_step, F, s->twy, s->sc, py0, py1);
       return FSIBE_OK;
     }));
     prof_end(s, P_COLS_AB2);
@@ -3044,6 +3062,7 @@ static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int
       a.Rc = s->T[2]; a.Rpx = s->penal_on ? s->T[3] : nullptr; a.W0 = s->W[s->n0]; a.NL1 = s->NL[s->n1];
       a.NL0 = o; a.W1 = o + NH;
       a.s = s->dscal; a.explicit_step = 0; a.nblk = nb = s->H / tx; a.c = s->sc;
+      a.py0 = 0; a.py1 = s->ny - 1;
     } else {
       ColsPressArgs &a = reinterpret_cast<ColsPressArgs *>(h)[j];
       a = ColsPressArgs{};
@@ -3351,6 +3370,7 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
     a.Rc = s->T[2]; a.Rpx = s->penal_on ? s->T[3] : nullptr; a.W0 = s->W[n0]; a.NL1 = s->NL[n1]; a.NL0 = s->NL[n0];
     a.W1 = s->W[n1];
     a.s = s->dscal; a.explicit_step = 0; a.nblk = s->H / g.tx[B_K4]; a.c = s->sc;
+    rpx_rows(s, mb, a.py0, a.py1);
     maxblk[B_K4] = std::max(maxblk[B_K4], a.nblk);
   }
   {  // K5

@@ -511,19 +511,23 @@ __device__ __forceinline__ void cols_ab2_body(const double2 *__restrict__ Rc, co
                                                      const double2 *__restrict__ W0, const double2 *__restrict__ NL1,
                                                      double2 *__restrict__ NL0, double2 *__restrict__ W1,
                                                      const DevScalars *__restrict__ s, int explicit_step, FY F,
-                                                     const double2 *__restrict__ twy, StepConsts c, int bid, int nblk) {
+                                                     const double2 *__restrict__ twy, StepConsts c, int py0, int py1,
+                                                     int bid, int nblk) {
   extern __shared__ double2 lds[];
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
   const int kx0 = xcd_remap(bid, nblk) * TX;
   // Rpx null: no penalisation anywhere in the simulation, its spectrum is 0,
   // and the tile holds Rc alone (half the LDS, so that 4096-point columns
-  // fit two columns per tile)
+  // fit two columns per tile).  Rows outside [py0, py1] of Rpx are 0 (the
+  // mask box's rows hold the penalisation; K2 writes 0 elsewhere): they are
+  // not read
   const bool px_on = Rpx != nullptr;
   const int TT = px_on ? 2 * TX : TX;
   for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
     lds[pidx(y) * TT + cc] = Rc[(size_t)y * H + kx0 + cc];
-    if (px_on) lds[pidx(y) * TT + TX + cc] = Rpx[(size_t)y * H + kx0 + cc];
+    if (px_on)
+      lds[pidx(y) * TT + TX + cc] = (y >= py0 && y <= py1) ? Rpx[(size_t)y * H + kx0 + cc] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   if (px_on) F.template fft<false, 2, 2>(lds, twy, tid, nt);
@@ -1061,8 +1065,8 @@ __global__ void __launch_bounds__(FY::LB) k_cols_ab2(const double2 *__restrict__
                                                      const double2 *__restrict__ W0, const double2 *__restrict__ NL1,
                                                      double2 *__restrict__ NL0, double2 *__restrict__ W1,
                                                      const DevScalars *__restrict__ s, int explicit_step, FY F,
-                                                     const double2 *__restrict__ twy, StepConsts c) {
-  cols_ab2_body(Rc, Rpx, W0, NL1, NL0, W1, s, explicit_step, F, twy, c, blockIdx.x, gridDim.x);
+                                                     const double2 *__restrict__ twy, StepConsts c, int py0, int py1) {
+  cols_ab2_body(Rc, Rpx, W0, NL1, NL0, W1, s, explicit_step, F, twy, c, py0, py1, blockIdx.x, gridDim.x);
 }
 template <class FX, int CMODE = -1, int CPRES = -1>
 __global__ void __launch_bounds__(FX::LB) k_rows_press(const double2 *__restrict__ Cw, const double2 *__restrict__ Cpsi,
@@ -1129,6 +1133,7 @@ struct ColsAb2Args {
   double2 *NL0, *W1;
   const DevScalars *s;
   int explicit_step, nblk;
+  int py0, py1;  // rows of Rpx that may be nonzero
   StepConsts c;
 };
 struct RowsPressArgs {
@@ -1183,7 +1188,8 @@ __global__ void __launch_bounds__(FY::LB) kb_cols_ab2(const ColsAb2Args *__restr
                                                       const double2 *__restrict__ twy) {
   const ColsAb2Args &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
-  cols_ab2_body(a.Rc, a.Rpx, a.W0, a.NL1, a.NL0, a.W1, a.s, a.explicit_step, F, twy, a.c, blockIdx.x, a.nblk);
+  cols_ab2_body(a.Rc, a.Rpx, a.W0, a.NL1, a.NL0, a.W1, a.s, a.explicit_step, F, twy, a.c, a.py0, a.py1, blockIdx.x,
+                a.nblk);
 }
 template <class FX>
 __global__ void __launch_bounds__(FX::LB) kb_rows_press(const RowsPressArgs *__restrict__ arr, FX F,
