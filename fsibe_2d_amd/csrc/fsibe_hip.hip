@@ -2121,7 +2121,7 @@ static int launch_k2_ahead(fsibe_sim *s) {
             s->sponge_on ? s->ms : nullptr, none, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
             F, s->twx, s->sc,
             s->umax_part, s->ny, s->dscal, s->fp, 0.0, 0.0, 0, res_dev(s, s->slot), 0, na, b1 + 1, 0,
-            s->penal_on ? s->rowpen : nullptr);
+            s->penal_on ? s->rowpen : nullptr, (int)(s->p.iSponge == 10));
     return FSIBE_OK;
   }));
   s->k2_ready = true;
@@ -2174,7 +2174,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
                    s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
                    F, s->twx, s->sc,
                    s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), 0,
-                   s->ny, 0, 1, s->penal_on ? s->rowpen : nullptr);
+                   s->ny, 0, 1, s->penal_on ? s->rowpen : nullptr, (int)(s->p.iSponge == 10));
         return FSIBE_OK;
       }));
     } else {
@@ -2183,7 +2183,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
                 s->sponge_on ? s->ms : nullptr, mb, (int)s->penal_on, s->T[2], s->penal_on ? s->T[3] : nullptr,
                 F, s->twx, s->sc,
                 s->umax_part, s->ny, s->dscal, s->fp, time, dt0, (int)explicit_step, res_dev(s, s->slot), 0, s->ny,
-                0, 1, s->penal_on ? s->rowpen : nullptr);
+                0, 1, s->penal_on ? s->rowpen : nullptr, (int)(s->p.iSponge == 10));
         return FSIBE_OK;
       }));
     }
@@ -3363,6 +3363,7 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
     a.mb = mb; a.penal_on = s->penal_on; a.nparts = s->ny; a.explicit_step = 0; a.nblk = s->ny;
     a.Rc = s->T[2]; a.Rpx = s->T[3]; a.c = s->sc; a.umax_part = s->umax_part; a.sc = s->dscal; a.fl = s->fp;
     a.time = s->time; a.dt0 = s->dt0; a.res = res_dev(s, s->slot);
+    a.rpx_zero = s->p.iSponge == 10 ? 1 : 0;
     maxblk[B_K2] = std::max(maxblk[B_K2], a.nblk);
   }
   {  // K4

@@ -356,7 +356,7 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
                                                     double dt0, int explicit_step, StepOutDev *__restrict__ res, int y,
-                                                    bool scal, double *__restrict__ rowpen) {
+                                                    bool scal, double *__restrict__ rowpen, int rpx_zero) {
   extern __shared__ double2 lds[];
   __shared__ double red[16];
   if (scal) step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
@@ -434,7 +434,10 @@ __device__ __forceinline__ void rows_nl_body(const double2 *__restrict__ A, cons
       const double2 xa = x[0];
       const double kf = (double)k * c.sx;
       Rc[(size_t)y * H + k] = make_double2(xa.x - kf * 0.0, xa.y + kf * 0.0);
-      if (Rpx) Rpx[(size_t)y * H + k] = make_double2(0.0, 0.0);  // null: no penalisation anywhere
+      // Rpx off the mask rows is not read (K4 reads the mask rows only), so
+      // its zeros are written only for the iSponge = 10 term, which adds
+      // to every row (rpx_zero); null: no penalisation anywhere
+      if (Rpx && rpx_zero) Rpx[(size_t)y * H + k] = make_double2(0.0, 0.0);
     });
   }
 }
@@ -998,11 +1001,12 @@ __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ 
                                                     const double *__restrict__ umax_part, int nparts,
                                                     DevScalars *__restrict__ sc, FlowParams fl, double time,
                                                     double dt0, int explicit_step, StepOutDev *__restrict__ res,
-                                                    int ra, int na, int rb, int scal, double *__restrict__ rowpen) {
+                                                    int ra, int na, int rb, int scal, double *__restrict__ rowpen,
+                                                    int rpx_zero) {
   // rows ra .. ra+na-1, then rb onwards (two row ranges; one when rb is unused)
   const int b = blockIdx.x;
   rows_nl_body(A, B, u1, u2, ms, mb, penal_on, Rc, Rpx, F, twx, c, umax_part, nparts, sc, fl, time, dt0,
-               explicit_step, res, b < na ? ra + b : rb + (b - na), scal && b == 0, rowpen);
+               explicit_step, res, b < na ? ra + b : rb + (b - na), scal && b == 0, rowpen, rpx_zero);
 }
 
 // K2's penalisation rows y0 .. y0 + nrow - 1 after the ahead pass; block 0
@@ -1127,6 +1131,7 @@ struct RowsNlArgs {
   FlowParams fl;
   double time, dt0;
   StepOutDev *res;
+  int rpx_zero;  // write Rpx's zero rows (iSponge = 10 adds to every row)
 };
 struct ColsAb2Args {
   const double2 *Rc, *Rpx, *W0, *NL1;
@@ -1181,7 +1186,7 @@ __global__ void __launch_bounds__(FX::LB) kb_rows_nl(const RowsNlArgs *__restric
   const RowsNlArgs &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
   rows_nl_body(a.A, a.B, a.u1, a.u2, a.ms, a.mb, a.penal_on, a.Rc, a.Rpx, F, twx, a.c, a.umax_part, a.nparts, a.sc,
-               a.fl, a.time, a.dt0, a.explicit_step, a.res, (int)blockIdx.x, blockIdx.x == 0, nullptr);
+               a.fl, a.time, a.dt0, a.explicit_step, a.res, (int)blockIdx.x, blockIdx.x == 0, nullptr, a.rpx_zero);
 }
 template <class FY>
 __global__ void __launch_bounds__(FY::LB) kb_cols_ab2(const ColsAb2Args *__restrict__ arr, FY F,
