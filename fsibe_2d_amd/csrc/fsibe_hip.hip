@@ -1910,6 +1910,18 @@ static void rpx_rows(const fsibe_sim *s, const MaskBox &mb, int &py0, int &py1) 
   }
 }
 
+// rows of -ky^2 psi (K5's field 3) that K8 reads: the force box's, once the
+// force geometry of the step is built (else every row)
+static void stress_rows(const fsibe_sim *s, int &r0, int &r1) {
+  if (s->forces_ready && !s->fbox.empty()) {
+    r0 = s->fbox.y0;
+    r1 = s->fbox.y1;
+  } else {
+    r0 = 0;
+    r1 = s->ny - 1;
+  }
+}
+
 static MaskBox mask_box(const fsibe_sim *s) {
   MaskBox m;
   m.x0 = s->mbox.x0; m.x1 = s->mbox.x1; m.y0 = s->mbox.y0; m.y1 = s->mbox.y1; m.w = s->mbox.w();
@@ -2053,15 +2065,16 @@ static ColSel sim_sel(const fsibe_sim *s, int which, int split) {
   return static_sel(s, (which == 1 || which == 3) ? 2 : 1, split);
 }
 
+// (fields -ky^2 psi are stored on rows r0_3 .. r1_3 only)
 static int cols_inv_fields(fsibe_sim *s, const double2 *W, double2 *const *outs, const int *fsel, int nf,
-                           int which = 0) {
+                           int which = 0, int r0_3 = 0, int r1_3 = 1 << 30) {
   double2 *o[4] = {outs[0], nf > 1 ? outs[1] : nullptr, nf > 2 ? outs[2] : nullptr, nf > 3 ? outs[3] : nullptr};
   int f[4] = {fsel[0], nf > 1 ? fsel[1] : 0, nf > 2 ? fsel[2] : 0, nf > 3 ? fsel[3] : 0};
   return with_cols_sel<1>(s->py, s->H, sim_sel(s, which, nf), [&](auto F, int nt, int TX) -> int {
     const size_t lds = sizeof(double2) * TX * padlen(s->ny);
     const int grid = (s->H / TX) * nf;
     KLAUNCH(k_cols_inv_fields<decltype(F)>, grid, nt, lds, s->stream, W, o[0], o[1], o[2], o[3], f[0], f[1], f[2],
-            f[3], nf, F, s->twy, s->sc);
+            f[3], nf, F, s->twy, s->sc, r0_3, r1_3);
     return FSIBE_OK;
   });
 }
@@ -2229,8 +2242,10 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     const bool st = s->forces_on && s->visc;
     double2 *outs[4] = {s->T[0], s->T[1], s->T[2], s->T[3]};
     const int fsel[4] = {0, 1, 2, 3};
+    int r0_3, r1_3;
+    stress_rows(s, r0_3, r1_3);
     PROF_BEGIN(s, P_COLS_W1);
-    CHK(cols_inv_fields(s, s->W[n1], outs, fsel, st ? 4 : 3, 2));
+    CHK(cols_inv_fields(s, s->W[n1], outs, fsel, st ? 4 : 3, 2, r0_3, r1_3));
     prof_end(s, P_COLS_W1);
   }
   // K6
@@ -2592,8 +2607,9 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
   s->pipe = pipe;
   const fsibe_params &p = s->p;
   const bool moving = (p.iSponge >= 4 && s->time < p.T_fullspeed) || p.iFLUSI == 1 || p.iMotion > 0;
-  // iFLUSI=1 waits for every step: the host builds the force geometry
-  // (interpolation targets) while the GPU already runs the fluid kernels
+  // iFLUSI=1 waits for every step: the mask's record goes up (and its copy
+  // starts) before the host bounds the force box (the device builds the
+  // interpolation targets), which K5 and the pressure box need
   const bool split = !pipe && p.iFLUSI == 1 && p.iBeam > 0 && s->forces_on;
   if (p.iBeam > 0 && (moving || !s->have_beam)) {
     double alpha[2], LE[12];
@@ -2608,8 +2624,9 @@ static int run_one(fsibe_sim *s, bool pipe, bool k1_ahead) {
   }
   if (s->startup_done) s->dt0 = s->dt1;
   if (split) {
-    CHK(evolve_fluid(s, s->time, s->startup_done ? s->dt0 : 0.0, !s->startup_done));
+    // the force box first: K5 stores -ky^2 psi on its rows only
     CHK(force_geometry(s, s->time));
+    CHK(evolve_fluid(s, s->time, s->startup_done ? s->dt0 : 0.0, !s->startup_done));
     CHK(evolve_pressure_box(s));
   } else {
     CHK(evolve(s, s->time, s->startup_done ? s->dt0 : 0.0, !s->startup_done));
@@ -3054,6 +3071,7 @@ static size_t tune_records(const fsibe_batch *b, const BatchGroup &g, int k, int
         a.fsel[q] = k == B_K1 ? (q == 1 ? 4 : 0) : (q < nf ? q : 0);
       }
       a.nfield = nf;
+      a.r0_3 = 0; a.r1_3 = s->ny - 1;
       a.nblk = nb = (s->H / tx) * nf;
       a.c = s->sc;
     } else if (k == B_K4) {
@@ -3353,6 +3371,7 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
     ColsInvArgs &a = reinterpret_cast<ColsInvArgs *>(A + g.off[B_K1])[j];
     a.W = s->W[n0]; a.o[0] = s->T[0]; a.o[1] = s->T[1]; a.o[2] = nullptr; a.o[3] = nullptr;
     a.fsel[0] = 0; a.fsel[1] = 4; a.fsel[2] = 0; a.fsel[3] = 0; a.nfield = 2;
+    a.r0_3 = 0; a.r1_3 = s->ny - 1;
     a.nblk = (s->H / g.tx[B_K1]) * 2;
     a.c = s->sc;
     maxblk[B_K1] = std::max(maxblk[B_K1], a.nblk);
@@ -3379,6 +3398,7 @@ static void fill_batch_args(fsibe_batch *b, BatchGroup &g, int j, fsibe_sim *s, 
     a.W = s->W[n1];
     for (int k = 0; k < 4; ++k) { a.o[k] = k < g.nf5 ? s->T[k] : nullptr; a.fsel[k] = k < g.nf5 ? k : 0; }
     a.nfield = g.nf5; a.nblk = (s->H / g.tx[B_K5]) * g.nf5; a.c = s->sc;
+    stress_rows(s, a.r0_3, a.r1_3);
     maxblk[B_K5] = std::max(maxblk[B_K5], a.nblk);
   }
   {  // K6

@@ -217,7 +217,8 @@ __device__ __forceinline__ void cols_inv_fields_body(const double2 *__restrict__
                                                             double2 *__restrict__ o1, double2 *__restrict__ o2,
                                                             double2 *__restrict__ o3, int fsel0, int fsel1,
                                                             int fsel2, int fsel3, int nfield, FY F,
-                                                            const double2 *__restrict__ twy, StepConsts c, int bid, int nblk) {
+                                                            const double2 *__restrict__ twy, StepConsts c, int r0_3,
+                                                            int r1_3, int bid, int nblk) {
   extern __shared__ double2 lds[];
   const int ny = F.NY(), TX = F.tx(), H = c.H, tid = threadIdx.x, nt = blockDim.x;
   const int lb = xcd_remap(bid, nblk);
@@ -231,9 +232,12 @@ __device__ __forceinline__ void cols_inv_fields_body(const double2 *__restrict__
   }
   __syncthreads();
   F.template fft<true, 1, 1>(lds, twy, tid, nt);
+  // -ky^2 psi (field 3) feeds only the stresses on the force box's rows
+  // [r0_3, r1_3]; its other rows are not stored
+  const int ya = fsel == 3 ? r0_3 : 0, yb = fsel == 3 ? r1_3 : ny - 1;
   for (int e = tid; e < ny * TX; e += nt) {
     const int y = e / TX, cc = e - y * TX;
-    out[(size_t)y * H + kx0 + cc] = lds[pidx(y) * TX + cc];
+    if (y >= ya && y <= yb) out[(size_t)y * H + kx0 + cc] = lds[pidx(y) * TX + cc];
   }
 }
 
@@ -989,8 +993,10 @@ __global__ void __launch_bounds__(FY::LB) k_cols_inv_fields(const double2 *__res
                                                             double2 *__restrict__ o1, double2 *__restrict__ o2,
                                                             double2 *__restrict__ o3, int fsel0, int fsel1,
                                                             int fsel2, int fsel3, int nfield, FY F,
-                                                            const double2 *__restrict__ twy, StepConsts c) {
-  cols_inv_fields_body(W, o0, o1, o2, o3, fsel0, fsel1, fsel2, fsel3, nfield, F, twy, c, blockIdx.x, gridDim.x);
+                                                            const double2 *__restrict__ twy, StepConsts c, int r0_3,
+                                                            int r1_3) {
+  cols_inv_fields_body(W, o0, o1, o2, o3, fsel0, fsel1, fsel2, fsel3, nfield, F, twy, c, r0_3, r1_3, blockIdx.x,
+                       gridDim.x);
 }
 template <class FX>
 __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ A, const double2 *__restrict__ B,
@@ -1117,6 +1123,7 @@ struct ColsInvArgs {
   const double2 *W;
   double2 *o[4];
   int fsel[4], nfield, nblk;
+  int r0_3, r1_3;  // rows of field 3 (-ky^2 psi) that are stored
   StepConsts c;
 };
 struct RowsNlArgs {
@@ -1178,7 +1185,7 @@ __global__ void __launch_bounds__(FY::LB) kb_cols_inv_fields(const ColsInvArgs *
   const ColsInvArgs &a = arr[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
   cols_inv_fields_body(a.W, a.o[0], a.o[1], a.o[2], a.o[3], a.fsel[0], a.fsel[1], a.fsel[2], a.fsel[3], a.nfield, F,
-                       twy, a.c, blockIdx.x, a.nblk);
+                       twy, a.c, a.r0_3, a.r1_3, blockIdx.x, a.nblk);
 }
 template <class FX>
 __global__ void __launch_bounds__(FX::LB) kb_rows_nl(const RowsNlArgs *__restrict__ arr, FX F,
