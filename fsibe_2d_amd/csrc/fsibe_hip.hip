@@ -2173,7 +2173,7 @@ static int evolve_fluid(fsibe_sim *s, double time, double dt0, bool explicit_ste
     if (k2_ahead) {
       const int y0 = mb.x1 >= mb.x0 ? mb.y0 : 0, nrow = mb.x1 >= mb.x0 ? mb.y1 - mb.y0 + 1 : 0;
       CHK(with_rows<3>(s->px, [&](auto F, int nt) -> int {
-        KLAUNCH(k_rows_pen<decltype(F)>, std::max(nrow, 1), nt, lds, s->stream, s->u1, s->u2, mb, s->T[2], s->T[3],
+        KLAUNCH(k_rows_pen<decltype(F)>, nrow + 1, nt, lds, s->stream, s->u1, s->u2, mb, s->T[2], s->T[3],
                 F, s->twx, s->sc, y0, nrow, s->rowpen, s->umax_part, s->ny, s->dscal, s->fp, time, dt0,
                 (int)explicit_step, res_dev(s, s->slot));
         return FSIBE_OK;

@@ -1015,8 +1015,8 @@ __global__ void __launch_bounds__(FX::LB) k_rows_nl(const double2 *__restrict__ 
                explicit_step, res, b < na ? ra + b : rb + (b - na), scal && b == 0, rowpen, rpx_zero);
 }
 
-// K2's penalisation rows y0 .. y0 + nrow - 1 after the ahead pass; block 0
-// also computes the K3 scalars (blocks past the rows do only that)
+// K2's penalisation rows y0 .. y0 + nrow - 1 after the ahead pass; block
+// nrow (the last) computes the K3 scalars
 template <class FX>
 __global__ void __launch_bounds__(FX::LB) k_rows_pen(const double *__restrict__ u1, const double *__restrict__ u2,
                                                      MaskBox mb, double2 *__restrict__ Rc, double2 *__restrict__ Rpx,
@@ -1025,11 +1025,11 @@ __global__ void __launch_bounds__(FX::LB) k_rows_pen(const double *__restrict__ 
                                                      const double *__restrict__ umax_part, int nparts,
                                                      DevScalars *__restrict__ sc, FlowParams fl, double time,
                                                      double dt0, int explicit_step, StepOutDev *__restrict__ res) {
-  if (blockIdx.x == 0) {
+  if ((int)blockIdx.x >= nrow) {
     __shared__ double red[16];
     step_scalars(umax_part, nparts, sc, fl, time, dt0, explicit_step, red, res);
+    return;
   }
-  if ((int)blockIdx.x >= nrow) return;
   rows_pen_body(u1, u2, mb, Rc, Rpx, F, twx, c, y0 + blockIdx.x, rowpen);
 }
 
