@@ -2322,7 +2322,7 @@ static int launch_forces(fsibe_sim *s, hipEvent_t end_ev = nullptr, bool *record
     hipExtLaunchKernelGGL(k_forces, dim3(units), dim3(kForceThreads), 0u, s->stream, nullptr, end_ev, 0u, tg, s->fb_p,
                           s->fb_sa, s->fb_sb, s->fg, val, nd, cd, pbeam_dev(s, s->slot),
                           pbeam_dev(s, s->slot) + nodes_all(s), parts_dev(s, s->slot));
-    *recorded = true;
+    if (recorded) *recorded = true;
   } else {
     k_forces<<<units, kForceThreads, 0, s->stream>>>(tg, s->fb_p, s->fb_sa, s->fb_sb, s->fg, val, nd, cd,
                                                      pbeam_dev(s, s->slot), pbeam_dev(s, s->slot) + nodes_all(s),
