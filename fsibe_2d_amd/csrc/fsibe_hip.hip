@@ -2347,8 +2347,8 @@ static void sync_last(fsibe_sim *s) {
   r.ibes_iter = b.ibes_iter;
 }
 
-// Result slot k's head: dt1, u_max and the force kernel's terms, the part
-// the beam solve needs (GetForces' p_beam, tau_beam)
+// Result slot k's head: dt1, u_max and the node loads, the part the beam
+// solve needs (GetForces' p_beam, tau_beam)
 static void parse_head(fsibe_sim *s, int k) {
   const int nsb = nodes_all(s), nb = s->nb, nr = std::max(nb, 1);
   const char *base = s->h_res + (size_t)k * s->res_stride;
@@ -2358,6 +2358,26 @@ static void parse_head(fsibe_sim *s, int k) {
   r.u_max = o->u_max;
   r.status = 0;
   r.ibes_iter = 0;
+  if (s->slot_forces[k]) {
+    const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
+    for (int i = 0; i < s->nb * s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
+  }
+  for (int ib = 0; ib < nr; ++ib) {
+    fsibe_step_out &rb = s->last_b[ib];
+    rb.dt1 = r.dt1; rb.u_max = r.u_max; rb.status = 0; rb.ibes_iter = 0;
+  }
+}
+
+// Result slot k's row sums of the fluid step (energies cal_energy
+// NavierStokes.f90:236-256, cal_drag Force / Force_unst :567-589,
+// BeamSurfaceIntegral drag/lift BeamForces.f90:483-486) added over rows in
+// order; completes s->last (after parse_head of the same slot and the beam
+// solve, whose iteration count it keeps)
+static void parse_rows(fsibe_sim *s, int k) {
+  const int nb = s->nb, nr = std::max(nb, 1);
+  const char *base = s->h_res + (size_t)k * s->res_stride;
+  // the force kernel's per-unit terms: Force_press and the surface
+  // integrals (not needed by the beam solve, which takes p_beam, tau_beam)
   if (s->slot_forces[k]) {
     s->slot_forces[k] = false;
     const double *part = reinterpret_cast<const double *>(base + parts_off(s));
@@ -2378,23 +2398,7 @@ static void parse_head(fsibe_sim *s, int k) {
         rb.bsi[7] = tot[3]; rb.bsi[8] = tot[5]; rb.bsi[9] = tot[7];
       }
     }
-    const double *pb = reinterpret_cast<const double *>(base + sizeof(StepOutDev));
-    for (int i = 0; i < s->nb * s->ns; ++i) { s->pbeam[i] = pb[i]; s->tbeam[i] = pb[nsb + i]; }
   }
-  for (int ib = 0; ib < nr; ++ib) {
-    fsibe_step_out &rb = s->last_b[ib];
-    rb.dt1 = r.dt1; rb.u_max = r.u_max; rb.status = 0; rb.ibes_iter = 0;
-  }
-}
-
-// Result slot k's row sums of the fluid step (energies cal_energy
-// NavierStokes.f90:236-256, cal_drag Force / Force_unst :567-589,
-// BeamSurfaceIntegral drag/lift BeamForces.f90:483-486) added over rows in
-// order; completes s->last (after parse_head of the same slot and the beam
-// solve, whose iteration count it keeps)
-static void parse_rows(fsibe_sim *s, int k) {
-  const int nb = s->nb, nr = std::max(nb, 1);
-  const char *base = s->h_res + (size_t)k * s->res_stride;
   const StepOutDev *o = reinterpret_cast<const StepOutDev *>(base);
   fsibe_step_out &r = s->last;
   const double dxdy = s->dx * s->dy;
